@@ -1,0 +1,342 @@
+// gnx_api.hip -- the C ABI (include/gnx.h): contexts, argument checks,
+// scratch ownership and the host-side sequencing of the hot-path kernels.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "gnx_internal.h"
+
+using namespace gnx;
+
+struct gnx_ctx {
+  int device = 0;
+  int L = 0, LT = 0;
+  long V = 0;
+  Geom g{};
+  hipStream_t stream = nullptr;
+  double* tmp = nullptr;        // 2V doubles
+  double* red_part = nullptr;   // reduction partials
+  int red_nparts = 0;
+  double* red_out = nullptr;    // 16 device scalars
+  double* host_out = nullptr;   // 16 pinned host scalars
+  CGWork cg;
+  int cg_nrhs_cap = 0;
+};
+
+static thread_local char g_err[512] = "";
+
+static void set_err(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      set_err("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return GNX_ERR_HIP;                                                               \
+    }                                                                                   \
+  } while (0)
+
+#define CHECK_ARG(cond, msg)     \
+  do {                           \
+    if (!(cond)) {               \
+      set_err("%s", msg);        \
+      return GNX_ERR_ARG;        \
+    }                            \
+  } while (0)
+
+namespace gnx {
+Geom make_geom(int L, int LT) {
+  Geom g;
+  g.L = L;
+  g.LT = LT;
+  g.V = L * L * LT;
+  g.half = g.V / 2;
+  g.LLT = L * LT;
+  g.magic_LT = (unsigned)((((unsigned long long)1 << 32) + LT - 1) / LT);
+  g.magic_L = (unsigned)((((unsigned long long)1 << 32) + L - 1) / L);
+  return g;
+}
+}  // namespace gnx
+
+// geometry table export (f-order, reference layout)
+__global__ void k_geometry_tables(Geom g, int* fpp, int* fmm, double* eta_pp, double* eta_mm, int* f2lex) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= g.V) return;
+  Site s = decode(g, f);
+  NbrF n = neighbours_f(g, s);
+  HalfEta e = half_eta(g, s);
+  for (int d = 0; d < 3; d++) {
+    if (fpp) fpp[3 * f + d] = n.pp[d];
+    if (fmm) fmm[3 * f + d] = n.mm[d];
+    if (eta_pp) eta_pp[3 * f + d] = 2.0 * e.pp[d];
+    if (eta_mm) eta_mm[3 * f + d] = 2.0 * e.mm[d];
+  }
+  if (f2lex) f2lex[f] = s.lex;
+}
+
+static gnx_status ensure_cg(gnx_ctx* c, int nrhs) {
+  if (nrhs <= c->cg_nrhs_cap) return GNX_OK;
+  CGWork& w = c->cg;
+  (void)hipFree(w.r); (void)hipFree(w.p0); (void)hipFree(w.p1); (void)hipFree(w.t);
+  (void)hipFree(w.part); (void)hipFree(w.res);
+  w = CGWork();
+  c->cg_nrhs_cap = 0;
+  size_t vb = sizeof(double2) * (size_t)c->V * nrhs;
+  HIP_TRY(hipMalloc(&w.r, vb));
+  HIP_TRY(hipMalloc(&w.p0, vb));
+  HIP_TRY(hipMalloc(&w.p1, vb));
+  HIP_TRY(hipMalloc(&w.t, vb));
+  w.max_blocks = 4096;
+  HIP_TRY(hipMalloc(&w.part, sizeof(double) * 4 * (size_t)nrhs * w.max_blocks));
+  HIP_TRY(hipMalloc(&w.res, sizeof(CGDevResult) * nrhs));
+  w.vec_elems = (long)c->V * nrhs;
+  c->cg_nrhs_cap = nrhs;
+  return GNX_OK;
+}
+
+extern "C" {
+
+int gnx_abi_version(void) { return GNX_ABI_VERSION; }
+const char* gnx_last_error(void) { return g_err; }
+
+gnx_status gnx_create(int lspace, int ltime, int device, gnx_ctx** out) {
+  CHECK_ARG(out != nullptr, "gnx_create: out is NULL");
+  *out = nullptr;
+  CHECK_ARG(lspace >= 4 && ltime >= 4 && lspace % 2 == 0 && ltime % 2 == 0,
+            "gnx_create: L and LT must be even and >= 4");
+  long V = (long)lspace * lspace * ltime;
+  if (V > (1L << 26)) {
+    set_err("gnx_create: volume %ld exceeds 2^26 sites", V);
+    return GNX_ERR_SIZE;
+  }
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  CHECK_ARG(device >= 0 && device < ndev, "gnx_create: no such device");
+  HIP_TRY(hipSetDevice(device));
+  gnx_ctx* c = new gnx_ctx();
+  c->device = device;
+  c->L = lspace;
+  c->LT = ltime;
+  c->V = V;
+  c->g = make_geom(lspace, ltime);
+  c->red_nparts = 1024;
+  hipError_t e;
+  if ((e = hipMalloc(&c->tmp, sizeof(double) * 2 * V)) != hipSuccess ||
+      (e = hipMalloc(&c->red_part, sizeof(double) * c->red_nparts)) != hipSuccess ||
+      (e = hipMalloc(&c->red_out, sizeof(double) * 16)) != hipSuccess ||
+      (e = hipHostMalloc(&c->host_out, sizeof(double) * 16, hipHostMallocDefault)) != hipSuccess) {
+    set_err("gnx_create: allocation failed: %s", hipGetErrorString(e));
+    gnx_destroy(c);
+    return GNX_ERR_HIP;
+  }
+  gnx_status s = ensure_cg(c, 1);
+  if (s != GNX_OK) {
+    gnx_destroy(c);
+    return s;
+  }
+  *out = c;
+  return GNX_OK;
+}
+
+gnx_status gnx_destroy(gnx_ctx* c) {
+  if (!c) return GNX_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipFree(c->tmp);
+  (void)hipFree(c->red_part);
+  (void)hipFree(c->red_out);
+  if (c->host_out) (void)hipHostFree(c->host_out);
+  (void)hipFree(c->cg.r); (void)hipFree(c->cg.p0); (void)hipFree(c->cg.p1); (void)hipFree(c->cg.t);
+  (void)hipFree(c->cg.part); (void)hipFree(c->cg.res);
+  delete c;
+  return GNX_OK;
+}
+
+gnx_status gnx_set_stream(gnx_ctx* c, void* s) {
+  CHECK_ARG(c, "null context");
+  c->stream = (hipStream_t)s;
+  return GNX_OK;
+}
+
+gnx_status gnx_synchronize(gnx_ctx* c) {
+  CHECK_ARG(c, "null context");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GNX_OK;
+}
+
+long gnx_volume(const gnx_ctx* c) { return c ? c->V : -1; }
+
+gnx_status gnx_geometry_tables(gnx_ctx* c, int* fpp, int* fmm, double* eta_pp, double* eta_mm, int* f2lex) {
+  CHECK_ARG(c, "null context");
+  HIP_TRY(hipSetDevice(c->device));
+  hipLaunchKernelGGL(k_geometry_tables, dim3((c->V + 255) / 256), dim3(256), 0, c->stream, c->g, fpp, fmm,
+                     eta_pp, eta_mm, f2lex);
+  HIP_TRY(hipGetLastError());
+  return GNX_OK;
+}
+
+gnx_status gnx_sigma_mass(gnx_ctx* c, const double* sigma, double* sm) {
+  CHECK_ARG(c && sigma && sm, "gnx_sigma_mass: null argument");
+  CHECK_ARG((const void*)sigma != (const void*)sm, "gnx_sigma_mass: sigma and sm alias");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(launch_sigma_mass(c->g, sigma, sm, c->stream));
+  return GNX_OK;
+}
+
+gnx_status gnx_dirac(gnx_ctx* c, gnx_op op, const double* sm, const double* in, double* out) {
+  CHECK_ARG(c && sm && in && out, "gnx_dirac: null argument");
+  CHECK_ARG((const void*)in != (const void*)out, "gnx_dirac: in and out alias");
+  CHECK_ARG(op >= GNX_OP_M && op <= GNX_OP_SUB_MDAG, "gnx_dirac: bad op");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(launch_dirac(c->g, (int)op, sm, in, out, c->tmp, c->stream));
+  return GNX_OK;
+}
+
+static void fill_result(gnx_cg_result* r, const CGDevResult& d) {
+  r->converged = d.converged;
+  r->early_exit = d.early_exit;
+  r->iterations = d.iterations;
+  r->reliable_updates = d.reliable_updates;
+  r->residual = d.residual;
+}
+
+gnx_status gnx_cg_batch(gnx_ctx* c, const double* sm, const double* b, double* x, int nrhs, double tol,
+                        long max_iter, gnx_cg_start start, gnx_prec prec, gnx_cg_result* res) {
+  CHECK_ARG(c && sm && b && x && res, "gnx_cg: null argument");
+  CHECK_ARG(nrhs >= 1 && nrhs <= 64, "gnx_cg: nrhs must be in [1, 64]");
+  CHECK_ARG(start == GNX_START_NULL || start == GNX_START_IMPOSED, "gnx_cg: bad start mode");
+  CHECK_ARG(tol >= 0.0, "gnx_cg: negative tolerance");
+  CHECK_ARG((const void*)b != (const void*)x, "gnx_cg: b and x alias");
+  if (max_iter <= 0) max_iter = c->V + 1;
+  HIP_TRY(hipSetDevice(c->device));
+  if (prec != GNX_PREC_F64) {
+    set_err("gnx_cg: mixed precision not available in this build");
+    return GNX_ERR_ARG;
+  }
+  gnx_status s = ensure_cg(c, nrhs);
+  if (s != GNX_OK) return s;
+  CGDevResult hr[64];
+  HIP_TRY(cg_f64_run(c->g, sm, (const double2*)b, (double2*)x, tol, max_iter, (int)start, c->cg, nrhs, c->stream,
+                     c->device, hr));
+  int all = 1;
+  for (int k = 0; k < nrhs; k++) {
+    fill_result(&res[k], hr[k]);
+    all &= hr[k].converged;
+  }
+  if (!all) {
+    set_err("gnx_cg: not converged within %ld iterations", max_iter);
+    return GNX_ERR_NOCONV;
+  }
+  return GNX_OK;
+}
+
+gnx_status gnx_cg(gnx_ctx* c, const double* sm, const double* b, double* x, double tol, long max_iter,
+                  gnx_cg_start start, gnx_prec prec, gnx_cg_result* res) {
+  return gnx_cg_batch(c, sm, b, x, 1, tol, max_iter, start, prec, res);
+}
+
+gnx_status gnx_sigma_force_from_solutions(gnx_ctx* c, const double* sigma, double beta_s, int nflav,
+                                          const double* X0, const double* Y0, const double* X1, const double* Y1,
+                                          double* force) {
+  CHECK_ARG(c && sigma && X0 && Y0 && X1 && Y1 && force, "gnx_sigma_force: null argument");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(launch_sigma_force(c->g, sigma, beta_s, nflav, X0, Y0, X1, Y1, force, c->stream));
+  return GNX_OK;
+}
+
+gnx_status gnx_sigma_force(gnx_ctx* c, const double* sigma, const double* sm, double beta_s, int nflav,
+                           double tol, long max_iter, const double* const phi[2], double* const X[2],
+                           double* const Y[2], double* force, gnx_cg_result res[2]) {
+  CHECK_ARG(c && sigma && sm && phi && X && Y && force && res, "gnx_sigma_force: null argument");
+  CHECK_ARG(phi[0] && phi[1] && X[0] && X[1] && Y[0] && Y[1], "gnx_sigma_force: null vector");
+  long V2 = 2 * c->V;
+  gnx_status s;
+  if (phi[1] == phi[0] + V2 && X[1] == X[0] + V2) {
+    s = gnx_cg_batch(c, sm, phi[0], X[0], 2, tol, max_iter, GNX_START_NULL, GNX_PREC_F64, res);
+  } else {
+    s = gnx_cg_batch(c, sm, phi[0], X[0], 1, tol, max_iter, GNX_START_NULL, GNX_PREC_F64, &res[0]);
+    if (s != GNX_OK) return s;
+    s = gnx_cg_batch(c, sm, phi[1], X[1], 1, tol, max_iter, GNX_START_NULL, GNX_PREC_F64, &res[1]);
+  }
+  if (s != GNX_OK) return s;
+  for (int f = 0; f < 2; f++) HIP_TRY(launch_dirac(c->g, GNX_OP_MDAG, sm, X[f], Y[f], c->tmp, c->stream));
+  HIP_TRY(launch_sigma_force(c->g, sigma, beta_s, nflav, X[0], Y[0], X[1], Y[1], force, c->stream));
+  return GNX_OK;
+}
+
+gnx_status gnx_evolve_momenta(gnx_ctx* c, double* mom, const double* force, double eps) {
+  CHECK_ARG(c && mom && force, "gnx_evolve_momenta: null argument");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(launch_evolve_momenta(c->g, mom, force, eps, c->stream));
+  return GNX_OK;
+}
+
+gnx_status gnx_evolve_sigma(gnx_ctx* c, double* sigma, double* sm, const double* mom, double eps) {
+  CHECK_ARG(c && sigma && sm && mom, "gnx_evolve_sigma: null argument");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(launch_evolve_sigma(c->g, sigma, sm, mom, eps, c->stream));
+  return GNX_OK;
+}
+
+// reductions: results land in c->red_out[slot], copied to host at the end
+static gnx_status reduce_to(gnx_ctx* c, int n, int kind, const void* a, const void* b, double scale, int slot) {
+  HIP_TRY(launch_reduce(n, kind, a, b, scale, c->red_part, c->red_nparts, c->red_out + slot, c->stream));
+  return GNX_OK;
+}
+
+static gnx_status fetch(gnx_ctx* c, int n) {
+  HIP_TRY(hipMemcpyAsync(c->host_out, c->red_out, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GNX_OK;
+}
+
+gnx_status gnx_energy(gnx_ctx* c, const double* mom, const double* sigma, double beta_s, int nflav,
+                      const double* const phi[2], const double* const X[2], double* H) {
+  CHECK_ARG(c && mom && sigma && phi && X && H, "gnx_energy: null argument");
+  HIP_TRY(hipSetDevice(c->device));
+  gnx_status s;
+  if ((s = reduce_to(c, (int)c->V, 3, mom, nullptr, 1.0, 0)) != GNX_OK) return s;
+  if ((s = reduce_to(c, (int)c->V, 1, phi[0], X[0], 1.0, 1)) != GNX_OK) return s;
+  if ((s = reduce_to(c, (int)c->V, 1, phi[1], X[1], 1.0, 2)) != GNX_OK) return s;
+  if ((s = reduce_to(c, (int)c->V, 3, sigma, nullptr, 1.0, 3)) != GNX_OK) return s;
+  if ((s = fetch(c, 4)) != GNX_OK) return s;
+  const double* o = c->host_out;
+  *H = o[0] / 2.0 + 0.5 * o[1] + 0.5 * o[2] + (nflav * beta_s / 4.0) * o[3];
+  return GNX_OK;
+}
+
+gnx_status gnx_dot(gnx_ctx* c, const double* a, const double* b, double* re, double* im) {
+  CHECK_ARG(c && a && b, "gnx_dot: null argument");
+  HIP_TRY(hipSetDevice(c->device));
+  gnx_status s;
+  if ((s = reduce_to(c, (int)c->V, 1, a, b, 1.0, 0)) != GNX_OK) return s;
+  if ((s = reduce_to(c, (int)c->V, 2, a, b, 1.0, 1)) != GNX_OK) return s;
+  if ((s = fetch(c, 2)) != GNX_OK) return s;
+  if (re) *re = c->host_out[0];
+  if (im) *im = c->host_out[1];
+  return GNX_OK;
+}
+
+gnx_status gnx_norm2(gnx_ctx* c, const double* a, double* out) {
+  CHECK_ARG(c && a && out, "gnx_norm2: null argument");
+  HIP_TRY(hipSetDevice(c->device));
+  gnx_status s;
+  if ((s = reduce_to(c, (int)c->V, 0, a, nullptr, 1.0, 0)) != GNX_OK) return s;
+  if ((s = fetch(c, 1)) != GNX_OK) return s;
+  *out = c->host_out[0];
+  return GNX_OK;
+}
+
+gnx_status gnx_scale_by_field(gnx_ctx* c, const double* field, const double* x, double* y) {
+  CHECK_ARG(c && field && x && y, "gnx_scale_by_field: null argument");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(launch_scale_by_field((int)c->V, field, x, y, c->stream));
+  return GNX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,203 @@
+// gnx_kernels.hip -- single-pass kernels of the fermion hot path:
+// sigma mass, Dirac operator applications, sigma force, HMC field updates,
+// deterministic FP64 reductions for energies and condensate dot products.
+// Every kernel is a memory-bound stencil or sweep: one site per thread,
+// coalesced double2 loads in f-order, no tables read from memory.
+#include "gnx_internal.h"
+
+namespace gnx {
+
+// ------------------------------------------------------------------ sigma mass
+// sigma.c:75-94.  The eight dual-lattice corners below site x, summed in the
+// reference's order, then / 8.0.
+__global__ void k_sigma_mass(Geom g, const double* __restrict__ sigma, double* __restrict__ sm) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= g.V) return;
+  Site s = decode(g, f);
+  sm[f] = sigma_mass_at(g, s, sigma);
+}
+
+// sigma += eps*mom (moleculardynamics.c:828-829); the sigma mass is recomputed
+// by a second launch (it reads neighbours of the updated field).
+__global__ void k_evolve_sigma(int V, double* __restrict__ sigma, const double* __restrict__ mom, double eps) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= V) return;
+  sigma[f] += (eps * mom[f]);
+}
+
+// mom += force*eps (moleculardynamics.c:843-848)
+__global__ void k_evolve_momenta(int V, double* __restrict__ mom, const double* __restrict__ force, double eps) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= V) return;
+  mom[f] += (force[f] * eps);
+}
+
+// ------------------------------------------------------------------ Dirac
+__global__ void k_dirac(Geom g, int op, const double* __restrict__ sm, const double2* __restrict__ in,
+                        double2* __restrict__ out) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= g.V) return;
+  Site s = decode(g, f);
+  auto ld = [&](int j) { return in[j]; };
+  double2 y;
+  switch (op) {
+    case GNX_OP_M: y = apply_M(g, s, sm[f], ld); break;
+    case GNX_OP_MDAG: y = apply_Mdag(g, s, sm[f], ld); break;
+    case GNX_OP_SUB_M: y = apply_sub_M(g, s, sm[f], out[f], ld); break;
+    default: y = apply_sub_Mdag(g, s, sm[f], out[f], ld); break;
+  }
+  out[f] = y;
+}
+
+// ------------------------------------------------------------------ sigma force
+// moleculardynamics.c:606-620 (fermion part, flavour-outer, corner order
+// {x, x+0+1, x+0+2, x+1+2, x+0, x+1, x+2, x+0+1+2}) then :576-581.
+__global__ void k_sigma_force(Geom g, const double* __restrict__ sigma, double beta_s, int nflav,
+                              const double2* __restrict__ X0, const double2* __restrict__ Y0,
+                              const double2* __restrict__ X1, const double2* __restrict__ Y1,
+                              double* __restrict__ force) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= g.V) return;
+  Site s = decode(g, f);
+  double F = 0.0;
+  F = force_flavour(g, s, X0, Y0, F);
+  F = force_flavour(g, s, X1, Y1, F);
+  F -= (nflav * beta_s / 2.0) * sigma[f];
+  force[f] = F;
+}
+
+// ------------------------------------------------------------------ reductions
+// Deterministic two-pass reduction: per-block partials in a fixed tree, then a
+// single block sums the partials in index order.  out[k] for k < nval.
+// kind selects the summand:
+//   RED_NORM2    |a|^2                      (complex a)
+//   RED_DOT_RE   Re(conj a * b)
+//   RED_DOT_IM   Im(conj a * b)
+//   RED_RSQ      a_r^2 (real field)
+//   RED_SUM      a_r (real field)
+enum RedKind { RED_NORM2 = 0, RED_DOT_RE = 1, RED_DOT_IM = 2, RED_RSQ = 3, RED_SUM = 4 };
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_reduce_partial(int n, int kind, const double2* __restrict__ a,
+                                                        const double2* __restrict__ b,
+                                                        const double* __restrict__ ar,
+                                                        double* __restrict__ part) {
+  __shared__ double lds[BS / 64];
+  double acc = 0.0;
+  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+    switch (kind) {
+      case RED_NORM2: acc += cnorm2(a[i]); break;
+      case RED_DOT_RE: acc += cdot_re(a[i], b[i]); break;
+      case RED_DOT_IM: acc += cdot_im(a[i], b[i]); break;
+      case RED_RSQ: acc += ar[i] * ar[i]; break;
+      default: acc += ar[i]; break;
+    }
+  }
+  double s = block_sum(acc, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_reduce_final(int nparts, const double* __restrict__ part,
+                                                      double scale, double* __restrict__ out) {
+  __shared__ double lds[BS / 64];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += BS) acc += part[i];
+  double s = block_sum(acc, lds);
+  if (threadIdx.x == 0) *out = s * scale;
+}
+
+// ------------------------------------------------------------------ elementwise
+__global__ void k_axpy_real(int V, double alpha, const double2* __restrict__ xv, double2* __restrict__ y) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= V) return;
+  y[f] = cadd(y[f], rmul(alpha, xv[f]));
+}
+
+// y = x * sm (complex times real field, condensate.c:221,225)
+__global__ void k_scale_by_field(int V, const double* __restrict__ sm, const double2* __restrict__ xv,
+                                 double2* __restrict__ y) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= V) return;
+  double2 v = xv[f];
+  double m = sm[f];
+  y[f] = make_double2(v.x * m, v.y * m);
+}
+
+// ------------------------------------------------------------------ host launchers
+static inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+hipError_t launch_sigma_mass(const Geom& g, const double* sigma, double* sm, hipStream_t st) {
+  hipLaunchKernelGGL(k_sigma_mass, dim3(nblk(g.V, 256)), dim3(256), 0, st, g, sigma, sm);
+  return hipGetLastError();
+}
+
+hipError_t launch_evolve_sigma(const Geom& g, double* sigma, double* sm, const double* mom, double eps,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(k_evolve_sigma, dim3(nblk(g.V, 256)), dim3(256), 0, st, g.V, sigma, mom, eps);
+  hipLaunchKernelGGL(k_sigma_mass, dim3(nblk(g.V, 256)), dim3(256), 0, st, g, (const double*)sigma, sm);
+  return hipGetLastError();
+}
+
+hipError_t launch_evolve_momenta(const Geom& g, double* mom, const double* force, double eps, hipStream_t st) {
+  hipLaunchKernelGGL(k_evolve_momenta, dim3(nblk(g.V, 256)), dim3(256), 0, st, g.V, mom, force, eps);
+  return hipGetLastError();
+}
+
+hipError_t launch_dirac(const Geom& g, int op, const double* sm, const double* in, double* out, double* tmp,
+                        hipStream_t st) {
+  const double2* vin = (const double2*)in;
+  double2* vout = (double2*)out;
+  dim3 grid(nblk(g.V, 256)), blk(256);
+  switch (op) {
+    case GNX_OP_M:
+    case GNX_OP_MDAG:
+    case GNX_OP_SUB_M:
+    case GNX_OP_SUB_MDAG:
+      hipLaunchKernelGGL(k_dirac, grid, blk, 0, st, g, op, sm, vin, vout);
+      break;
+    case GNX_OP_MMDAG:
+      hipLaunchKernelGGL(k_dirac, grid, blk, 0, st, g, (int)GNX_OP_MDAG, sm, vin, (double2*)tmp);
+      hipLaunchKernelGGL(k_dirac, grid, blk, 0, st, g, (int)GNX_OP_M, sm, (const double2*)tmp, vout);
+      break;
+    case GNX_OP_MDAGM:
+      hipLaunchKernelGGL(k_dirac, grid, blk, 0, st, g, (int)GNX_OP_M, sm, vin, (double2*)tmp);
+      hipLaunchKernelGGL(k_dirac, grid, blk, 0, st, g, (int)GNX_OP_MDAG, sm, (const double2*)tmp, vout);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_sigma_force(const Geom& g, const double* sigma, double beta_s, int nflav, const double* X0,
+                              const double* Y0, const double* X1, const double* Y1, double* force,
+                              hipStream_t st) {
+  hipLaunchKernelGGL(k_sigma_force, dim3(nblk(g.V, 256)), dim3(256), 0, st, g, sigma, beta_s, nflav,
+                     (const double2*)X0, (const double2*)Y0, (const double2*)X1, (const double2*)Y1, force);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce(int n, int kind, const void* a, const void* b, double scale, double* part, int nparts,
+                         double* out, hipStream_t st) {
+  int g = nblk(n, 256);
+  if (g > nparts) g = nparts;
+  if (g < 1) g = 1;
+  hipLaunchKernelGGL(k_reduce_partial<256>, dim3(g), dim3(256), 0, st, n, kind, (const double2*)a,
+                     (const double2*)b, (const double*)a, part);
+  hipLaunchKernelGGL(k_reduce_final<256>, dim3(1), dim3(256), 0, st, g, (const double*)part, scale, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpy_real(int V, double alpha, const double* xv, double* y, hipStream_t st) {
+  hipLaunchKernelGGL(k_axpy_real, dim3(nblk(V, 256)), dim3(256), 0, st, V, alpha, (const double2*)xv, (double2*)y);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale_by_field(int V, const double* sm, const double* xv, double* y, hipStream_t st) {
+  hipLaunchKernelGGL(k_scale_by_field, dim3(nblk(V, 256)), dim3(256), 0, st, V, sm, (const double2*)xv,
+                     (double2*)y);
+  return hipGetLastError();
+}
+
+}  // namespace gnx

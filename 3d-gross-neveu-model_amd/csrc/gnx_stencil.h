@@ -1,0 +1,172 @@
+// gnx_stencil.h -- the staggered Dirac operator at one site, in the exact
+// accumulation order of the reference (bit-exact in FP64):
+//
+//   M   (dirac.c:46-82):  y = sm*v; for d=0..2: y += 0.5*eta_pp*v[pp]; y -= 0.5*eta_mm*v[mm]
+//   M^+ (dirac.c:88-123): scatter form in the reference; its per-site order is
+//       the ascending f-index of the six source sites.  Within the
+//       opposite-parity half f-order is lex order, so the sources sort as
+//       [below along x0][below x1][below x2][above x2][above x1][above x0],
+//       where "below/above" compares the neighbour's coordinate with the
+//       site's; at a wrap both neighbours along that axis fall on one side.
+//   r -= M v (dirac.c:213-247): same order as M with flipped signs.
+//
+// Each hop term 0.5*eta*v is exact (a power of two times +-v), so only the
+// order of the additions matters; the mass product sm*v is rounded first
+// (the library is compiled with -ffp-contract=off).  Absent slots add -0.0,
+// the exact additive identity.
+#pragma once
+
+#include "gnx_common.h"
+
+namespace gnx {
+
+template <typename LoadF>
+GNX_DEV double2 apply_M(const Geom& g, const Site& s, double sm, LoadF load) {
+  NbrF n = neighbours_f(g, s);
+  HalfEta e = half_eta(g, s);
+  int f = f_of_lex(g, s.lex, s.P);
+  double2 acc = rmul(sm, load(f));
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    acc = cadd(acc, rmul(e.pp[d], load(n.pp[d])));
+    acc = cadd(acc, rmul(-e.mm[d], load(n.mm[d])));
+  }
+  return acc;
+}
+
+// r - M v, reference order (r -= sm*v, then -= pp, += mm per direction)
+template <typename LoadF>
+GNX_DEV double2 apply_sub_M(const Geom& g, const Site& s, double sm, double2 r, LoadF load) {
+  NbrF n = neighbours_f(g, s);
+  HalfEta e = half_eta(g, s);
+  int f = f_of_lex(g, s.lex, s.P);
+  double2 acc = csub(r, rmul(sm, load(f)));
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    acc = cadd(acc, rmul(-e.pp[d], load(n.pp[d])));
+    acc = cadd(acc, rmul(e.mm[d], load(n.mm[d])));
+  }
+  return acc;
+}
+
+// Ordered accumulation of the six M^+ (or -M^+) hop terms onto acc.
+// tpp[d]: term from source pp(z,d); tmm[d]: term from source mm(z,d).
+GNX_DEV double2 accumulate_adj_order(const Geom& g, const Site& s, double2 acc,
+                                     const double2 tpp[3], const double2 tmm[3]) {
+  const double2 z = make_double2(-0.0, -0.0);
+  const int c[3] = {s.x0, s.x1, s.x2};
+  const int E[3] = {g.L, g.L, g.LT};
+  // below-slots: d = 0, 1, 2
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    bool top = (c[d] == E[d] - 1), bot = (c[d] == 0);
+    double2 s1 = top ? tpp[d] : (bot ? z : tmm[d]);
+    double2 s2 = top ? tmm[d] : z;
+    acc = cadd(acc, s1);
+    acc = cadd(acc, s2);
+  }
+  // above-slots: d = 2, 1, 0
+#pragma unroll
+  for (int dd = 0; dd < 3; dd++) {
+    int d = 2 - dd;
+    bool top = (c[d] == E[d] - 1), bot = (c[d] == 0);
+    double2 s3 = top ? z : tpp[d];
+    double2 s4 = bot ? tmm[d] : z;
+    acc = cadd(acc, s3);
+    acc = cadd(acc, s4);
+  }
+  return acc;
+}
+
+// M^+ v at site s.  Source mm(z,d) contributes +0.5*eta_pp[mm(z,d),d]*v, source
+// pp(z,d) contributes -0.5*eta_mm[pp(z,d),d]*v.  eta_d depends only on
+// coordinates below d, so eta at the source equals eta at z except for the
+// antiperiodic time flips: eta_pp[mm(z,2),2] flips when z2 == 0 and
+// eta_mm[pp(z,2),2] flips when z2 == LT-1.
+GNX_DEV void adj_terms(const Geom& g, const Site& s, double hp[3], double hm[3]) {
+  double e1 = (s.x0 & 1) ? -0.5 : 0.5;
+  double e2 = ((s.x0 + s.x1) & 1) ? -0.5 : 0.5;
+  hp[0] = 0.5;  hm[0] = -0.5;
+  hp[1] = e1;   hm[1] = -e1;
+  hp[2] = (s.x2 == 0) ? -e2 : e2;           // coefficient of v[mm(z,2)]
+  hm[2] = (s.x2 == g.LT - 1) ? e2 : -e2;    // coefficient of v[pp(z,2)]
+}
+
+template <typename LoadF>
+GNX_DEV double2 apply_Mdag(const Geom& g, const Site& s, double sm, LoadF load) {
+  NbrF n = neighbours_f(g, s);
+  double cm[3], cp[3];
+  adj_terms(g, s, cm, cp);
+  int f = f_of_lex(g, s.lex, s.P);
+  double2 tpp[3], tmm[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    tmm[d] = rmul(cm[d], load(n.mm[d]));
+    tpp[d] = rmul(cp[d], load(n.pp[d]));
+  }
+  return accumulate_adj_order(g, s, rmul(sm, load(f)), tpp, tmm);
+}
+
+// r - M^+ v (dirac.c:253-287): r -= sm*v, then the scatter terms negated, in
+// the same ascending-source order.
+template <typename LoadF>
+GNX_DEV double2 apply_sub_Mdag(const Geom& g, const Site& s, double sm, double2 r, LoadF load) {
+  NbrF n = neighbours_f(g, s);
+  double cm[3], cp[3];
+  adj_terms(g, s, cm, cp);
+  int f = f_of_lex(g, s.lex, s.P);
+  double2 tpp[3], tmm[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    tmm[d] = rmul(-cm[d], load(n.mm[d]));
+    tpp[d] = rmul(-cp[d], load(n.pp[d]));
+  }
+  return accumulate_adj_order(g, s, csub(r, rmul(sm, load(f))), tpp, tmm);
+}
+
+// ------------------------------------------------------------------ sigma sector
+// f-index of the site with coordinates (c0,c1,c2)
+GNX_DEV int f_of_coords(const Geom& g, int c0, int c1, int c2) {
+  int lex = (c0 * g.L + c1) * g.LT + c2;
+  return (lex >> 1) + ((c0 + c1 + c2) & 1) * g.half;
+}
+
+GNX_DEV double sigma_mass_at(const Geom& g, const Site& s, const double* __restrict__ sigma) {
+  int a0 = s.x0, a1 = s.x1, a2 = s.x2;
+  int b0 = a0 == 0 ? g.L - 1 : a0 - 1;
+  int b1 = a1 == 0 ? g.L - 1 : a1 - 1;
+  int b2 = a2 == 0 ? g.LT - 1 : a2 - 1;
+  double m = sigma[f_of_coords(g, a0, a1, a2)];
+  m += sigma[f_of_coords(g, b0, a1, a2)];
+  m += sigma[f_of_coords(g, a0, b1, a2)];
+  m += sigma[f_of_coords(g, a0, a1, b2)];
+  m += sigma[f_of_coords(g, b0, b1, a2)];
+  m += sigma[f_of_coords(g, b0, a1, b2)];
+  m += sigma[f_of_coords(g, a0, b1, b2)];
+  m += sigma[f_of_coords(g, b0, b1, b2)];
+  return m / 8.0;
+}
+
+// sigma force of one flavour at dual site s (moleculardynamics.c:606-620):
+// corners {x, x+0+1, x+0+2, x+1+2, x+0, x+1, x+2, x+0+1+2}, in that order.
+GNX_DEV double force_flavour(const Geom& g, const Site& s, const double2* __restrict__ X,
+                             const double2* __restrict__ Y, double F) {
+  int a0 = s.x0, a1 = s.x1, a2 = s.x2;
+  int u0 = a0 == g.L - 1 ? 0 : a0 + 1;
+  int u1 = a1 == g.L - 1 ? 0 : a1 + 1;
+  int u2 = a2 == g.LT - 1 ? 0 : a2 + 1;
+  int c[8];
+  c[0] = f_of_coords(g, a0, a1, a2);
+  c[1] = f_of_coords(g, u0, u1, a2);
+  c[2] = f_of_coords(g, u0, a1, u2);
+  c[3] = f_of_coords(g, a0, u1, u2);
+  c[4] = f_of_coords(g, u0, a1, a2);
+  c[5] = f_of_coords(g, a0, u1, a2);
+  c[6] = f_of_coords(g, a0, a1, u2);
+  c[7] = f_of_coords(g, u0, u1, u2);
+#pragma unroll
+  for (int i = 0; i < 8; i++) F += (0.125) * cdot_re(X[c[i]], Y[c[i]]);
+  return F;
+}
+
+}  // namespace gnx

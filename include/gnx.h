@@ -1,0 +1,145 @@
+/*
+ * gnx.h -- C ABI of the MI355X-native fermion hot path of the 3D Z2
+ * Gross-Neveu model (gfx950 HIP kernels).
+ *
+ * Conventions
+ *   - Every vector/field argument is a DEVICE pointer (hipMalloc'd or any
+ *     device allocation of the caller), never a host pointer.
+ *   - Complex vectors are double[2*V] with (re, im) interleaved per site, in
+ *     the reference's checkerboard "fermion lexicographic" order
+ *     f = n/2 + parity*V/2 (coordinate.c:301-313), i.e. exactly the layout of
+ *     the reference's `Fermion` (fermion.h:49-52).  Real fields (sigma, sigma
+ *     mass, momenta, forces) are double[V] in the same f-order
+ *     (lattice.h:16-20).
+ *   - V = L*L*LT.  L and LT must be even and >= 4 (the checkerboard index
+ *     is a bijection only for even LT, coordinate.c:301-313).
+ *   - Work is enqueued on the context's stream (gnx_set_stream); calls that
+ *     return host scalars (CG results, energies, dots) synchronise it.
+ *   - The caller owns every buffer it passes; the context owns its scratch.
+ *   - No function calls exit(); errors come back as gnx_status, with a
+ *     message in gnx_last_error().
+ */
+#ifndef GNX_H
+#define GNX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GNX_ABI_VERSION 1
+
+typedef struct gnx_ctx gnx_ctx;
+
+typedef enum gnx_status {
+    GNX_OK = 0,
+    GNX_ERR_ARG = 1,      /* bad argument (null pointer, bad size, bad enum) */
+    GNX_ERR_HIP = 2,      /* a HIP runtime call failed                        */
+    GNX_ERR_NOCONV = 3,   /* CG exhausted max_iter (reference returns false)  */
+    GNX_ERR_SIZE = 4,     /* request exceeds a compiled or allocated limit    */
+    GNX_ERR_STATE = 5     /* call sequence error                              */
+} gnx_status;
+
+/* operator selector for gnx_dirac */
+typedef enum gnx_op {
+    GNX_OP_M = 0,         /* out = M in        DiracFermion         dirac.c:46-82   */
+    GNX_OP_MDAG = 1,      /* out = M^dag in    AdjDiracFermion      dirac.c:88-123  */
+    GNX_OP_MMDAG = 2,     /* out = M M^dag in  (the CG normal operator, conjugategradient.c:68,83) */
+    GNX_OP_MDAGM = 3,     /* out = M^dag M in  (offered beside it; not used by the CG) */
+    GNX_OP_SUB_M = 4,     /* out -= M in       SubtractDiracFermion dirac.c:213-247 */
+    GNX_OP_SUB_MDAG = 5   /* out -= M^dag in   SubtractAdjDiracFermion dirac.c:253-287 */
+} gnx_op;
+
+/* CG start modes */
+typedef enum gnx_cg_start {
+    GNX_START_NULL = 0,     /* START_X0_NULL prologue: r=b-MM^dag x_in, early exit
+                               keeps x_in; else restart from x=0
+                               (conjugategradient.c:13-107) */
+    GNX_START_IMPOSED = 1   /* x=0, r=b (conjugategradient.c:115-182) */
+} gnx_cg_start;
+
+/* CG arithmetic */
+typedef enum gnx_prec {
+    GNX_PREC_F64 = 0,       /* plain FP64 drop-in                                */
+    GNX_PREC_F32 = 1,       /* FP32 inner iterations, FP64 reliable updates      */
+    GNX_PREC_BF16 = 2       /* BF16 storage / FP32 arithmetic inner iterations,
+                               FP64 reliable updates                             */
+} gnx_prec;
+
+typedef struct gnx_cg_result {
+    int converged;        /* 1: ||r|| < tol (absolute, conjugategradient.c:87-89)  */
+    int early_exit;       /* 1: returned before the loop (prologue or ||b|| < tol) */
+    int iterations;       /* loop iterations (k+1 at exit); 0 on early exit        */
+    int reliable_updates; /* mixed precision: FP64 residual recomputations         */
+    double residual;      /* ||r|| of the recursively updated residual at exit     */
+} gnx_cg_result;
+
+int         gnx_abi_version(void);
+const char *gnx_last_error(void);
+
+/* Context: one lattice of L x L x LT sites on one HIP device. */
+gnx_status gnx_create(int lspace, int ltime, int device, gnx_ctx **out);
+gnx_status gnx_destroy(gnx_ctx *ctx);
+gnx_status gnx_set_stream(gnx_ctx *ctx, void *hip_stream);   /* NULL: default stream */
+gnx_status gnx_synchronize(gnx_ctx *ctx);
+long       gnx_volume(const gnx_ctx *ctx);
+
+/* A1. Geometry tables in the reference's f-order layout (fermion.h:12-33),
+ * written to device buffers of V*3 ints/doubles and V ints.  The kernels do
+ * not read tables; this export exists so the tables can be checked bit-exactly. */
+gnx_status gnx_geometry_tables(gnx_ctx *ctx, int *fermionpp, int *fermionmm, double *eta_pp,
+                               double *eta_mm, int *f2lex);
+
+/* A2. ComputeFullSigmaMass (sigma.c:75-94): sm[x] = (1/8) sum of sigma over
+ * the 8 dual sites below x, reference summation order. */
+gnx_status gnx_sigma_mass(gnx_ctx *ctx, const double *sigma, double *sm);
+
+/* A3/A4/A5 and the normal operators.  `in` and `out` must not alias. */
+gnx_status gnx_dirac(gnx_ctx *ctx, gnx_op op, const double *sm, const double *in, double *out);
+
+/* A6/A7. Solve (M M^dag) x = b to ||r||_2 < tol (absolute).
+ * max_iter <= 0 selects the reference cap V+1 (macro.h:134).
+ * Returns GNX_ERR_NOCONV when the cap is reached (x holds the last iterate). */
+gnx_status gnx_cg(gnx_ctx *ctx, const double *sm, const double *b, double *x, double tol,
+                  long max_iter, gnx_cg_start start, gnx_prec prec, gnx_cg_result *res);
+
+/* Batched CG: nrhs right-hand sides b[k] = b + k*2V, solutions x + k*2V,
+ * one fused persistent launch; res[nrhs].  Each system keeps its own alpha,
+ * beta and exit iteration. */
+gnx_status gnx_cg_batch(gnx_ctx *ctx, const double *sm, const double *b, double *x, int nrhs,
+                        double tol, long max_iter, gnx_cg_start start, gnx_prec prec,
+                        gnx_cg_result *res);
+
+/* A9. Sigma force (moleculardynamics.c:590-647):
+ *   X_f = CG(phi_f) with START_X0_NULL from the incoming X_f, Y_f = M^dag X_f,
+ *   F = sum_f sum_corners 0.125 Re(conj X_f Y_f) - (nflav*beta_s/2) sigma.
+ * phi, X, Y are arrays of two device vectors.  X is in/out. */
+gnx_status gnx_sigma_force(gnx_ctx *ctx, const double *sigma, const double *sm, double beta_s,
+                           int nflav, double tol, long max_iter, const double *const phi[2],
+                           double *const X[2], double *const Y[2], double *force,
+                           gnx_cg_result res[2]);
+
+/* Force from given solutions only (the kernel after the solves). */
+gnx_status gnx_sigma_force_from_solutions(gnx_ctx *ctx, const double *sigma, double beta_s, int nflav,
+                                          const double *X0, const double *Y0, const double *X1,
+                                          const double *Y1, double *force);
+
+/* A10. HMC pieces. */
+gnx_status gnx_evolve_momenta(gnx_ctx *ctx, double *mom, const double *force, double eps);
+gnx_status gnx_evolve_sigma(gnx_ctx *ctx, double *sigma, double *sm, const double *mom, double eps);
+/* H = sum p^2/2 + sum_f sum_x 0.5 Re(conj phi_f X_f) + (nflav beta_s/4) sum sigma^2
+ * (moleculardynamics.c:299-319), deterministic FP64 tree reductions. */
+gnx_status gnx_energy(gnx_ctx *ctx, const double *mom, const double *sigma, double beta_s, int nflav,
+                      const double *const phi[2], const double *const X[2], double *H);
+
+/* A12 helpers: deterministic reductions (host result). */
+gnx_status gnx_dot(gnx_ctx *ctx, const double *a, const double *b, double *re, double *im);
+gnx_status gnx_norm2(gnx_ctx *ctx, const double *a, double *out);
+/* y = x * field (complex vector times real field, condensate.c:221,225) */
+gnx_status gnx_scale_by_field(gnx_ctx *ctx, const double *field, const double *x, double *y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GNX_H */

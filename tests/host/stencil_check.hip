@@ -1,0 +1,108 @@
+// CPU check of the device-side geometry and stencil code (the same inline
+// functions the kernels use, instantiated for the host) against the oracle.
+// Bit-exact: tables, sigma mass, M, M^dag, r-=Mv, r-=M^dag v, sigma force.
+// Build and run by tests/test_host_stencil.py; exit status 0 on success.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../3d-gross-neveu-model_amd/csrc/gnx_stencil.h"
+#include "../../oracle/gnx_oracle.h"
+
+using namespace gnx;
+
+namespace gnx {
+Geom make_geom(int L, int LT) {
+  Geom g;
+  g.L = L; g.LT = LT; g.V = L * L * LT; g.half = g.V / 2; g.LLT = L * LT;
+  g.magic_LT = (unsigned)((((unsigned long long)1 << 32) + LT - 1) / LT);
+  g.magic_L = (unsigned)((((unsigned long long)1 << 32) + L - 1) / L);
+  return g;
+}
+}  // namespace gnx
+
+static int fails = 0;
+#define EXPECT(c, ...) do { if (!(c)) { if (fails < 20) { printf(__VA_ARGS__); printf("\n"); } fails++; } } while (0)
+
+static bool same(double a, double b) { return a == b; }
+
+static void check(int L, int LT) {
+  orc_geom og;
+  if (orc_geom_init(&og, L, LT) != 0) { printf("bad size\n"); exit(2); }
+  Geom g = make_geom(L, LT);
+  int V = g.V;
+  orc_rng rng;
+  orc_rng_init(&rng, 1000 + L * 7 + LT);
+  std::vector<double> sigma(V), sm(V), smo(V), b(2 * V), c(2 * V), out(2 * V);
+  orc_draw_sigma(&rng, V, 2.0 / 3.4, sigma.data());
+  orc_draw_z2_complex(&rng, V, b.data());
+  for (int i = 0; i < 2 * V; i++) c[i] = orc_gauss1(&rng);
+  orc_sigma_mass(&og, sigma.data(), smo.data());
+  const double2* bv = (const double2*)b.data();
+  auto ld = [&](int j) { return bv[j]; };
+  for (int f = 0; f < V; f++) {
+    Site s = decode(g, f);
+    EXPECT(s.lex == og.f2lex[f], "L=%d LT=%d f=%d lex %d vs %d", L, LT, f, s.lex, og.f2lex[f]);
+    EXPECT(f_of_lex(g, s.lex, s.P) == f, "f roundtrip %d", f);
+    NbrF n = neighbours_f(g, s);
+    HalfEta e = half_eta(g, s);
+    for (int d = 0; d < 3; d++) {
+      EXPECT(n.pp[d] == og.fpp[3 * f + d], "pp f=%d d=%d", f, d);
+      EXPECT(n.mm[d] == og.fmm[3 * f + d], "mm f=%d d=%d", f, d);
+      EXPECT(2.0 * e.pp[d] == og.eta_pp[3 * f + d], "eta_pp f=%d d=%d", f, d);
+      EXPECT(2.0 * e.mm[d] == og.eta_mm[3 * f + d], "eta_mm f=%d d=%d", f, d);
+    }
+    sm[f] = sigma_mass_at(g, s, sigma.data());
+    EXPECT(same(sm[f], smo[f]), "sigma mass f=%d", f);
+  }
+  std::vector<double> ref(2 * V);
+  // M
+  orc_dirac(&og, smo.data(), b.data(), ref.data());
+  for (int f = 0; f < V; f++) {
+    double2 y = apply_M(g, decode(g, f), smo[f], ld);
+    EXPECT(same(y.x, ref[2 * f]) && same(y.y, ref[2 * f + 1]), "M L=%d f=%d", L, f);
+  }
+  orc_adj_dirac(&og, smo.data(), b.data(), ref.data());
+  for (int f = 0; f < V; f++) {
+    double2 y = apply_Mdag(g, decode(g, f), smo[f], ld);
+    EXPECT(same(y.x, ref[2 * f]) && same(y.y, ref[2 * f + 1]), "Mdag L=%d LT=%d f=%d", L, LT, f);
+  }
+  ref = c;
+  orc_sub_dirac(&og, smo.data(), b.data(), ref.data());
+  for (int f = 0; f < V; f++) {
+    double2 r = make_double2(c[2 * f], c[2 * f + 1]);
+    double2 y = apply_sub_M(g, decode(g, f), smo[f], r, ld);
+    EXPECT(same(y.x, ref[2 * f]) && same(y.y, ref[2 * f + 1]), "subM f=%d", f);
+  }
+  ref = c;
+  orc_sub_adj_dirac(&og, smo.data(), b.data(), ref.data());
+  for (int f = 0; f < V; f++) {
+    double2 r = make_double2(c[2 * f], c[2 * f + 1]);
+    double2 y = apply_sub_Mdag(g, decode(g, f), smo[f], r, ld);
+    EXPECT(same(y.x, ref[2 * f]) && same(y.y, ref[2 * f + 1]), "subMdag f=%d", f);
+  }
+  // force from arbitrary X, Y
+  std::vector<double> X0(2 * V), Y0(2 * V), X1(2 * V), Y1(2 * V), F(V), Fo(V);
+  for (int i = 0; i < 2 * V; i++) { X0[i] = orc_gauss1(&rng); Y0[i] = orc_gauss1(&rng); X1[i] = orc_gauss1(&rng); Y1[i] = orc_gauss1(&rng); }
+  double* Xs[2] = {X0.data(), X1.data()};
+  double* Ys[2] = {Y0.data(), Y1.data()};
+  orc_sigma_force_from_solutions(&og, sigma.data(), 0.85, Xs, Ys, Fo.data());
+  for (int f = 0; f < V; f++) {
+    Site s = decode(g, f);
+    double Fv = 0.0;
+    Fv = force_flavour(g, s, (const double2*)X0.data(), (const double2*)Y0.data(), Fv);
+    Fv = force_flavour(g, s, (const double2*)X1.data(), (const double2*)Y1.data(), Fv);
+    Fv -= (4 * 0.85 / 2.0) * sigma[f];
+    EXPECT(same(Fv, Fo[f]), "force f=%d", f);
+  }
+  orc_geom_free(&og);
+}
+
+int main() {
+  int sizes[][2] = {{4, 4}, {4, 6}, {6, 4}, {6, 10}, {8, 8}, {10, 4}, {12, 12}, {16, 8}, {16, 16}, {24, 24}, {32, 32}};
+  for (auto& s : sizes) check(s[0], s[1]);
+  if (fails) { printf("%d failures\n", fails); return 1; }
+  printf("stencil_check: all sizes bit-exact\n");
+  return 0;
+}
