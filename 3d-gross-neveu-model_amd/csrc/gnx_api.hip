@@ -2,52 +2,23 @@
 // scratch ownership and the host-side sequencing of the hot-path kernels.
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "gnx_internal.h"
 
 using namespace gnx;
 
-struct gnx_ctx {
-  int device = 0;
-  int L = 0, LT = 0;
-  long V = 0;
-  Geom g{};
-  hipStream_t stream = nullptr;
-  double* tmp = nullptr;        // 2V doubles
-  double* red_part = nullptr;   // reduction partials
-  int red_nparts = 0;
-  double* red_out = nullptr;    // 16 device scalars
-  double* host_out = nullptr;   // 16 pinned host scalars
-  CGWork cg;
-  int cg_nrhs_cap = 0;
-};
-
+namespace gnx {
 static thread_local char g_err[512] = "";
 
-static void set_err(const char* fmt, ...) {
+void set_err(const char* fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
   vsnprintf(g_err, sizeof(g_err), fmt, ap);
   va_end(ap);
 }
-
-#define HIP_TRY(expr)                                                                   \
-  do {                                                                                  \
-    hipError_t e_ = (expr);                                                             \
-    if (e_ != hipSuccess) {                                                             \
-      set_err("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
-      return GNX_ERR_HIP;                                                               \
-    }                                                                                   \
-  } while (0)
-
-#define CHECK_ARG(cond, msg)     \
-  do {                           \
-    if (!(cond)) {               \
-      set_err("%s", msg);        \
-      return GNX_ERR_ARG;        \
-    }                            \
-  } while (0)
+}  // namespace gnx
 
 namespace gnx {
 Geom make_geom(int L, int LT) {
@@ -79,7 +50,7 @@ __global__ void k_geometry_tables(Geom g, int* fpp, int* fmm, double* eta_pp, do
   if (f2lex) f2lex[f] = s.lex;
 }
 
-static gnx_status ensure_cg(gnx_ctx* c, int nrhs) {
+gnx_status gnx::ensure_cg(gnx_ctx* c, int nrhs) {
   if (nrhs <= c->cg_nrhs_cap) return GNX_OK;
   CGWork& w = c->cg;
   (void)hipFree(w.r); (void)hipFree(w.p0); (void)hipFree(w.p1); (void)hipFree(w.t);
@@ -99,10 +70,17 @@ static gnx_status ensure_cg(gnx_ctx* c, int nrhs) {
   return GNX_OK;
 }
 
+hipError_t gnx::cg_run(gnx_ctx* c, const double* sm, const double2* b, double2* x, double tol, long max_iter,
+                      int start, int nrhs, CGDevResult* host_res) {
+  if (c->cg_engine == 1)
+    return cg_f64_run(c->g, sm, b, x, tol, max_iter, start, c->cg, nrhs, c->stream, c->device, host_res);
+  return cgm_f64_run(&c->cgm, c->g, sm, b, x, tol, max_iter, start, c->cg, nrhs, c->stream, host_res);
+}
+
 extern "C" {
 
 int gnx_abi_version(void) { return GNX_ABI_VERSION; }
-const char* gnx_last_error(void) { return g_err; }
+const char* gnx_last_error(void) { return gnx::g_err; }
 
 gnx_status gnx_create(int lspace, int ltime, int device, gnx_ctx** out) {
   CHECK_ARG(out != nullptr, "gnx_create: out is NULL");
@@ -125,6 +103,10 @@ gnx_status gnx_create(int lspace, int ltime, int device, gnx_ctx** out) {
   c->V = V;
   c->g = make_geom(lspace, ltime);
   c->red_nparts = 1024;
+  {
+    const char* eng = getenv("GNX_CG_ENGINE");
+    c->cg_engine = (eng && strcmp(eng, "persistent") == 0) ? 1 : 0;
+  }
   hipError_t e;
   if ((e = hipMalloc(&c->tmp, sizeof(double) * 2 * V)) != hipSuccess ||
       (e = hipMalloc(&c->red_part, sizeof(double) * c->red_nparts)) != hipSuccess ||
@@ -152,6 +134,7 @@ gnx_status gnx_destroy(gnx_ctx* c) {
   if (c->host_out) (void)hipHostFree(c->host_out);
   (void)hipFree(c->cg.r); (void)hipFree(c->cg.p0); (void)hipFree(c->cg.p1); (void)hipFree(c->cg.t);
   (void)hipFree(c->cg.part); (void)hipFree(c->cg.res);
+  if (c->cgm) cgm_release(c->cgm);
   delete c;
   return GNX_OK;
 }
@@ -221,8 +204,7 @@ gnx_status gnx_cg_batch(gnx_ctx* c, const double* sm, const double* b, double* x
   gnx_status s = ensure_cg(c, nrhs);
   if (s != GNX_OK) return s;
   CGDevResult hr[64];
-  HIP_TRY(cg_f64_run(c->g, sm, (const double2*)b, (double2*)x, tol, max_iter, (int)start, c->cg, nrhs, c->stream,
-                     c->device, hr));
+  HIP_TRY(cg_run(c, sm, (const double2*)b, (double2*)x, tol, max_iter, (int)start, nrhs, hr));
   int all = 1;
   for (int k = 0; k < nrhs; k++) {
     fill_result(&res[k], hr[k]);

@@ -1,0 +1,436 @@
+// gnx_cgm.hip -- FP64 conjugate gradient on A = M M^dag (conjugategradient.c:13-182)
+// as a device-side state machine of two kernels per iteration, replayed from
+// a captured hipGraph.
+//
+// Why kernels and not a persistent kernel: on gfx950 a grid-wide barrier
+// inside a kernel costs 10-80 us (cooperative grid sync or an atomic counter,
+// growing with the block count), a kernel node inside a hipGraph about
+// 1.7 us.  Standard CG needs two global reductions per iteration, so each
+// iteration is two kernels whose boundaries are the barriers:
+//
+//   A(k): rho_k from the partials of the previous B (or of the start kernel);
+//         exit test sqrt(rho_k) < tol (absolute, conjugategradient.c:87-89);
+//         beta = rho_k / rho_{k-1} (0 at k = 0);
+//         p_new = r + beta p_old at the site and, recomputed, at its six
+//         neighbours; t = M^dag p_new; partial |t|^2.
+//   B(k): gamma from A's partials; alpha = rho_k / gamma; x += alpha p_new;
+//         r += (-alpha) M t; partial |r|^2.
+//
+// Every block reduces the previous kernel's per-block partials in the same
+// fixed order, so all blocks hold bit-identical scalars.  Per-system state
+// (rho, active, iteration count) is double-buffered: kernel j reads
+// state[j & 1] and its block 0 writes state[(j+1) & 1]; no block reads what
+// another block of the same kernel writes.  Systems of a batch (blockIdx.y)
+// are independent: each keeps its alpha, beta and exit iteration, and a
+// finished system's blocks return at once.  The host replays a fixed graph of
+// CHUNK iterations until every system is inactive.
+#include <string.h>
+
+#include "gnx_internal.h"
+
+namespace gnx {
+
+constexpr int CGM_BS = 256;
+
+struct CGMState {
+  double rho;
+  int active;
+  int iters;
+  int converged;
+  int early;
+};
+
+struct CGMArgs {
+  Geom g;
+  const double* sm;
+  const double2* b;
+  double2* x;
+  double2* r;
+  double2* p0;
+  double2* p1;
+  double2* t;
+  double* part;        // [2 slots][nrhs][nb]
+  CGMState* state;     // [2][nrhs]
+  CGDevResult* res;    // [nrhs]
+  double tol;
+  long max_iter;
+  int nrhs;
+  int nb;
+  int start;           // GNX_START_NULL / GNX_START_IMPOSED
+};
+
+GNX_DEVONLY double* slot(const CGMArgs& a, int s, int rhs) {
+  return a.part + ((size_t)s * a.nrhs + rhs) * a.nb;
+}
+
+GNX_DEVONLY double reduce_slot(const CGMArgs& a, int s, int rhs, double* lds) {
+  const double* src = slot(a, s, rhs);
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < a.nb; i += CGM_BS) acc += src[i];
+  return block_sum(acc, lds);
+}
+
+// ---- start kernels ---------------------------------------------------------
+// imposed start (conjugategradient.c:121-128): x = 0, r = p = b, partial |b|^2
+__global__ __launch_bounds__(CGM_BS) void k_cgm_start_imposed(const CGMArgs* __restrict__ ap) {
+  const CGMArgs a = *ap;
+  __shared__ double lds[CGM_BS / 64];
+  const int rhs = blockIdx.y;
+  const size_t off = (size_t)rhs * a.g.V;
+  const int f = blockIdx.x * CGM_BS + threadIdx.x;
+  double acc = 0.0;
+  if (f < a.g.V) {
+    double2 bb = a.b[off + f];
+    a.x[off + f] = make_double2(0.0, 0.0);
+    a.r[off + f] = bb;
+    a.p0[off + f] = bb;
+    acc = cnorm2(bb);
+  }
+  acc = block_sum(acc, lds);
+  if (threadIdx.x == 0) slot(a, 1, rhs)[blockIdx.x] = acc;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    CGMState s = {0.0, 1, 0, 0, 0};
+    a.state[rhs] = s;
+  }
+}
+
+// null start, pass 1 (conjugategradient.c:20): t = M^dag x_in
+__global__ __launch_bounds__(CGM_BS) void k_cgm_null_p1(const CGMArgs* __restrict__ ap) {
+  const CGMArgs a = *ap;
+  const size_t off = (size_t)blockIdx.y * a.g.V;
+  const int f = blockIdx.x * CGM_BS + threadIdx.x;
+  if (f >= a.g.V) return;
+  const double2* xin = a.x + off;
+  a.t[off + f] = apply_Mdag(a.g, decode(a.g, f), a.sm[f], [&](int q) { return xin[q]; });
+}
+
+// null start, pass 2 (conjugategradient.c:19-21): r = b - M t; partials |r|^2
+// (slot 0) and |b|^2 (slot 1)
+__global__ __launch_bounds__(CGM_BS) void k_cgm_null_p2(const CGMArgs* __restrict__ ap) {
+  const CGMArgs a = *ap;
+  __shared__ double lds[CGM_BS / 64];
+  const int rhs = blockIdx.y;
+  const size_t off = (size_t)rhs * a.g.V;
+  const int f = blockIdx.x * CGM_BS + threadIdx.x;
+  double ar = 0.0, ab = 0.0;
+  if (f < a.g.V) {
+    const double2* t = a.t + off;
+    double2 bb = a.b[off + f];
+    double2 rr = apply_sub_M(a.g, decode(a.g, f), a.sm[f], bb, [&](int q) { return t[q]; });
+    a.r[off + f] = rr;
+    ar = cnorm2(rr);
+    ab = cnorm2(bb);
+  }
+  ar = block_sum(ar, lds);
+  ab = block_sum(ab, lds);
+  if (threadIdx.x == 0) {
+    slot(a, 0, rhs)[blockIdx.x] = ar;
+    slot(a, 1, rhs)[blockIdx.x] = ab;
+  }
+}
+
+// null start, pass 3 (conjugategradient.c:22-33): early exit keeps x_in;
+// otherwise x = 0, r = p = b.  The |b|^2 partials stay in slot 1 for A(0).
+__global__ __launch_bounds__(CGM_BS) void k_cgm_null_p3(const CGMArgs* __restrict__ ap) {
+  const CGMArgs a = *ap;
+  __shared__ double lds[CGM_BS / 64];
+  const int rhs = blockIdx.y;
+  const size_t off = (size_t)rhs * a.g.V;
+  const double nr = reduce_slot(a, 0, rhs, lds);
+  const bool early = sqrt(nr) < a.tol;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    CGMState s = {0.0, early ? 0 : 1, 0, early ? 1 : 0, early ? 1 : 0};
+    a.state[rhs] = s;
+    if (early) {
+      CGDevResult r = {1, 1, 0, 0, sqrt(nr), {0, 0, 0}};
+      a.res[rhs] = r;
+    }
+  }
+  if (early) return;
+  const int f = blockIdx.x * CGM_BS + threadIdx.x;
+  if (f < a.g.V) {
+    double2 bb = a.b[off + f];
+    a.x[off + f] = make_double2(0.0, 0.0);
+    a.r[off + f] = bb;
+    a.p0[off + f] = bb;
+  }
+}
+
+// ---- iteration kernels -----------------------------------------------------
+__global__ __launch_bounds__(CGM_BS) void k_cgm_A(const CGMArgs* __restrict__ ap) {
+  const CGMArgs& a = *ap;
+  __shared__ double lds[CGM_BS / 64];
+  const int rhs = blockIdx.y;
+  const CGMState s = a.state[rhs];          // state[0]: written by the previous B / start
+  CGMState* out = a.state + a.nrhs + rhs;   // state[1]
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (!s.active) {
+    if (lead) *out = s;
+    return;
+  }
+  const Geom g = a.g;
+  const double rho = reduce_slot(a, 1, rhs, lds);
+  double beta = 0.0;
+  if (s.iters == 0) {
+    // ||b|| < tol returns before the loop only in the imposed start
+    // (conjugategradient.c:123-128); the null start already tested its
+    // prologue residual instead (:22)
+    if (a.start == GNX_START_IMPOSED && sqrt(rho) < a.tol) {
+      if (lead) {
+        CGMState n = {rho, 0, 0, 1, 1};
+        *out = n;
+        CGDevResult r = {1, 1, 0, 0, sqrt(rho), {0, 0, 0}};
+        a.res[rhs] = r;
+      }
+      return;
+    }
+  } else {
+    const bool conv = sqrt(rho) < a.tol;
+    if (conv || s.iters >= a.max_iter) {
+      if (lead) {
+        CGMState n = {rho, 0, s.iters, conv ? 1 : 0, 0};
+        *out = n;
+        CGDevResult r = {conv ? 1 : 0, 0, s.iters, 0, sqrt(rho), {0, 0, 0}};
+        a.res[rhs] = r;
+      }
+      return;
+    }
+    beta = rho / s.rho;
+  }
+  const size_t off = (size_t)rhs * g.V;
+  const double2* r = a.r + off;
+  const double2* pold = ((s.iters & 1) ? a.p1 : a.p0) + off;
+  double2* pnew = ((s.iters & 1) ? a.p0 : a.p1) + off;
+  const int f = blockIdx.x * CGM_BS + threadIdx.x;
+  double acc = 0.0;
+  if (f < g.V) {
+    auto ldp = [&](int q) { return cadd(r[q], rmul(beta, pold[q])); };
+    pnew[f] = ldp(f);
+    double2 tt = apply_Mdag(g, decode(g, f), a.sm[f], ldp);
+    a.t[off + f] = tt;
+    acc = cnorm2(tt);
+  }
+  acc = block_sum(acc, lds);
+  if (threadIdx.x == 0) slot(a, 0, rhs)[blockIdx.x] = acc;
+  if (lead) {
+    CGMState n = {rho, 1, s.iters, 0, 0};
+    *out = n;
+  }
+}
+
+__global__ __launch_bounds__(CGM_BS) void k_cgm_B(const CGMArgs* __restrict__ ap) {
+  const CGMArgs& a = *ap;
+  __shared__ double lds[CGM_BS / 64];
+  const int rhs = blockIdx.y;
+  const CGMState s = a.state[a.nrhs + rhs];  // state[1]
+  CGMState* out = a.state + rhs;             // state[0]
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (!s.active) {
+    if (lead) *out = s;
+    return;
+  }
+  const Geom g = a.g;
+  const double gamma = reduce_slot(a, 0, rhs, lds);
+  const double al = s.rho / gamma;
+  const size_t off = (size_t)rhs * g.V;
+  const double2* pnew = ((s.iters & 1) ? a.p0 : a.p1) + off;
+  const double2* t = a.t + off;
+  double2* r = a.r + off;
+  double2* x = a.x + off;
+  const int f = blockIdx.x * CGM_BS + threadIdx.x;
+  double acc = 0.0;
+  if (f < g.V) {
+    x[f] = cadd(x[f], rmul(al, pnew[f]));
+    double2 w = apply_M(g, decode(g, f), a.sm[f], [&](int q) { return t[q]; });
+    double2 rr = cadd(r[f], rmul(-al, w));
+    r[f] = rr;
+    acc = cnorm2(rr);
+  }
+  acc = block_sum(acc, lds);
+  if (threadIdx.x == 0) slot(a, 1, rhs)[blockIdx.x] = acc;
+  if (lead) {
+    CGMState n = {s.rho, 1, s.iters + 1, 0, 0};
+    *out = n;
+  }
+}
+
+// ---- host side ---------------------------------------------------------------
+constexpr int CGM_CHUNK = 16;  // iterations per graph replay
+
+struct CGMGraph {
+  int nrhs = 0;
+  int nb = 0;
+  hipGraphExec_t exec = nullptr;
+};
+
+struct CGMachine {
+  CGMArgs* dargs = nullptr;     // device copy of the arguments
+  CGMArgs* hargs = nullptr;     // pinned host staging
+  CGMState* state = nullptr;    // device [2][cap]
+  CGMState* hstate = nullptr;   // pinned [cap]
+  CGDevResult* hres = nullptr;  // pinned [cap]
+  double* part = nullptr;
+  int cap = 0;
+  int nb_cap = 0;
+  hipStream_t capture = nullptr;
+  CGMGraph graphs[8];
+  int last_iters = 0;
+};
+
+static void cgm_free(CGMachine* m) {
+  if (!m) return;
+  for (auto& g : m->graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  if (m->dargs) (void)hipFree(m->dargs);
+  if (m->hargs) (void)hipHostFree(m->hargs);
+  if (m->state) (void)hipFree(m->state);
+  if (m->hstate) (void)hipHostFree(m->hstate);
+  if (m->hres) (void)hipHostFree(m->hres);
+  if (m->part) (void)hipFree(m->part);
+  if (m->capture) (void)hipStreamDestroy(m->capture);
+  delete m;
+}
+
+void cgm_release(void* m) { cgm_free((CGMachine*)m); }
+
+static hipError_t cgm_get(void** slotp, int nrhs, int nb, CGMachine** out) {
+  CGMachine* m = (CGMachine*)*slotp;
+  if (m && (m->cap < nrhs || m->nb_cap < nb)) {
+    cgm_free(m);
+    m = nullptr;
+    *slotp = nullptr;
+  }
+  if (!m) {
+    m = new CGMachine();
+    int cap = nrhs < 2 ? 2 : nrhs;
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = hipMalloc(&m->dargs, sizeof(CGMArgs));
+    if (e == hipSuccess) e = hipHostMalloc((void**)&m->hargs, sizeof(CGMArgs), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&m->state, sizeof(CGMState) * 2 * cap);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&m->hstate, sizeof(CGMState) * cap, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&m->hres, sizeof(CGDevResult) * cap, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&m->part, sizeof(double) * 2 * (size_t)cap * nb);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->capture, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      cgm_free(m);
+      return e;
+    }
+    m->cap = cap;
+    m->nb_cap = nb;
+    *slotp = m;
+  }
+  *out = m;
+  return hipSuccess;
+}
+
+static hipError_t cgm_graph(CGMachine* m, int nrhs, int nb, hipGraphExec_t* exec) {
+  for (auto& g : m->graphs)
+    if (g.exec && g.nrhs == nrhs && g.nb == nb) {
+      *exec = g.exec;
+      return hipSuccess;
+    }
+  CGMGraph* slotg = nullptr;
+  for (auto& g : m->graphs)
+    if (!g.exec) {
+      slotg = &g;
+      break;
+    }
+  if (!slotg) {
+    slotg = &m->graphs[0];
+    (void)hipGraphExecDestroy(slotg->exec);
+    slotg->exec = nullptr;
+  }
+  hipGraph_t graph;
+  hipError_t e = hipStreamBeginCapture(m->capture, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  dim3 grid(nb, nrhs), blk(CGM_BS);
+  for (int i = 0; i < CGM_CHUNK; i++) {
+    hipLaunchKernelGGL(k_cgm_A, grid, blk, 0, m->capture, (const CGMArgs*)m->dargs);
+    hipLaunchKernelGGL(k_cgm_B, grid, blk, 0, m->capture, (const CGMArgs*)m->dargs);
+  }
+  e = hipStreamEndCapture(m->capture, &graph);
+  if (e != hipSuccess) return e;
+  e = hipGraphInstantiate(&slotg->exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return e;
+  slotg->nrhs = nrhs;
+  slotg->nb = nb;
+  *exec = slotg->exec;
+  return hipSuccess;
+}
+
+hipError_t cgm_f64_run(void** machine_slot, const Geom& g, const double* sm, const double2* b, double2* x,
+                       double tol, long max_iter, int start, CGWork& w, int nrhs, hipStream_t st,
+                       CGDevResult* host_res) {
+  const int nb = (g.V + CGM_BS - 1) / CGM_BS;
+  CGMachine* m = nullptr;
+  hipError_t e = cgm_get(machine_slot, nrhs, nb, &m);
+  if (e != hipSuccess) return e;
+  hipGraphExec_t exec;
+  e = cgm_graph(m, nrhs, nb, &exec);
+  if (e != hipSuccess) return e;
+  // the previous solve's staging may still be read by an in-flight copy
+  e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return e;
+  CGMArgs& a = *m->hargs;
+  a.g = g;
+  a.sm = sm;
+  a.b = b;
+  a.x = x;
+  a.r = w.r;
+  a.p0 = w.p0;
+  a.p1 = w.p1;
+  a.t = w.t;
+  a.part = m->part;
+  a.state = m->state;
+  a.res = w.res;
+  a.tol = tol;
+  a.max_iter = max_iter;
+  a.nrhs = nrhs;
+  a.nb = nb;
+  a.start = start;
+  e = hipMemcpyAsync(m->dargs, m->hargs, sizeof(CGMArgs), hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return e;
+  dim3 grid(nb, nrhs), blk(CGM_BS);
+  if (start == GNX_START_NULL) {
+    hipLaunchKernelGGL(k_cgm_null_p1, grid, blk, 0, st, (const CGMArgs*)m->dargs);
+    hipLaunchKernelGGL(k_cgm_null_p2, grid, blk, 0, st, (const CGMArgs*)m->dargs);
+    hipLaunchKernelGGL(k_cgm_null_p3, grid, blk, 0, st, (const CGMArgs*)m->dargs);
+  } else {
+    hipLaunchKernelGGL(k_cgm_start_imposed, grid, blk, 0, st, (const CGMArgs*)m->dargs);
+  }
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  // replay: first the expected number of chunks, then one at a time
+  long done_iters = 0;
+  int burst = (m->last_iters + CGM_CHUNK - 1) / CGM_CHUNK;
+  if (burst < 1) burst = 1;
+  const long cap_iters = max_iter + CGM_CHUNK;  // A(k) of the last chunk observes k == max_iter
+  for (;;) {
+    for (int i = 0; i < burst; i++) {
+      e = hipGraphLaunch(exec, st);
+      if (e != hipSuccess) return e;
+    }
+    done_iters += (long)burst * CGM_CHUNK;
+    e = hipMemcpyAsync(m->hstate, m->state, sizeof(CGMState) * nrhs, hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    bool any = false;
+    for (int k = 0; k < nrhs; k++) any |= m->hstate[k].active != 0;
+    if (!any) break;
+    if (done_iters > cap_iters) return hipErrorUnknown;  // state machine failed to stop
+    burst = 1;
+  }
+  int maxit = 0;
+  for (int k = 0; k < nrhs; k++) maxit = m->hstate[k].iters > maxit ? m->hstate[k].iters : maxit;
+  m->last_iters = maxit;
+  if (host_res) {
+    e = hipMemcpyAsync(host_res, w.res, sizeof(CGDevResult) * nrhs, hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(st);
+  }
+  return e;
+}
+
+}  // namespace gnx

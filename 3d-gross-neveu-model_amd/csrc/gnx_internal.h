@@ -2,6 +2,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
 
 #include "../../include/gnx.h"
 #include "gnx_common.h"
@@ -50,4 +52,54 @@ struct CGWork {
 hipError_t cg_f64_run(const Geom& g, const double* sm, const double2* b, double2* x, double tol, long max_iter,
                       int start, CGWork& w, int nrhs, hipStream_t st, int device, CGDevResult* host_res);
 
+hipError_t cgm_f64_run(void** machine_slot, const Geom& g, const double* sm, const double2* b, double2* x,
+                       double tol, long max_iter, int start, CGWork& w, int nrhs, hipStream_t st,
+                       CGDevResult* host_res);
+void cgm_release(void* machine);
+
+void set_err(const char* fmt, ...);
+
 }  // namespace gnx
+
+struct gnx_ctx {
+  int device = 0;
+  int L = 0, LT = 0;
+  long V = 0;
+  gnx::Geom g{};
+  hipStream_t stream = nullptr;
+  double* tmp = nullptr;        // 2V doubles
+  double* red_part = nullptr;   // reduction partials
+  int red_nparts = 0;
+  double* red_out = nullptr;    // 16 device scalars
+  double* host_out = nullptr;   // 16 pinned host scalars
+  gnx::CGWork cg;
+  int cg_nrhs_cap = 0;
+  void* cgm = nullptr;          // graph-replayed CG state machine (gnx_cgm.hip)
+  int cg_engine = 0;            // 0: graph state machine, 1: persistent cooperative kernel
+};
+
+
+namespace gnx {
+gnx_status ensure_cg(gnx_ctx* c, int nrhs);
+// FP64 CG on the context's engine; host_res[nrhs] filled on return
+hipError_t cg_run(gnx_ctx* c, const double* sm, const double2* b, double2* x, double tol, long max_iter, int start,
+                  int nrhs, CGDevResult* host_res);
+}
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      gnx::set_err("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return GNX_ERR_HIP;                                                               \
+    }                                                                                   \
+  } while (0)
+
+#define CHECK_ARG(cond, msg)     \
+  do {                           \
+    if (!(cond)) {               \
+      gnx::set_err("%s", msg);        \
+      return GNX_ERR_ARG;        \
+    }                            \
+  } while (0)
+

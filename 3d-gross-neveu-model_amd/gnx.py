@@ -35,6 +35,11 @@ class CGResult(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class TrajResult(C.Structure):
+    _fields_ = [("dH", C.c_double), ("H0", C.c_double), ("H1", C.c_double), ("ok", C.c_int),
+                ("cg_solves", C.c_int), ("cg_iterations", C.c_long)]
+
+
 _lib = None
 
 
@@ -71,6 +76,18 @@ def lib():
         L.gnx_dot.argtypes = [vp, vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.gnx_norm2.argtypes = [vp, vp, C.POINTER(C.c_double)]
         L.gnx_scale_by_field.argtypes = [vp, vp, vp, vp]
+        dp = C.POINTER(C.c_double)
+        L.gnx_hmc_create.argtypes = [vp, C.c_double, C.c_int, C.POINTER(vp)]
+        L.gnx_hmc_destroy.argtypes = [vp]
+        L.gnx_hmc_set_sigma.argtypes = [vp, dp]
+        L.gnx_hmc_get_sigma.argtypes = [vp, dp, dp]
+        L.gnx_hmc_device_fields.argtypes = [vp] + [C.POINTER(vp)] * 7
+        L.gnx_hmc_trajectory.argtypes = [vp, dp, dp, dp, C.c_double, C.c_int, C.c_double, C.c_double,
+                                         C.c_long, C.POINTER(TrajResult)]
+        L.gnx_hmc_accept.argtypes = [vp, C.c_int]
+        L.gnx_hmc_condensate.argtypes = [vp, dp, C.c_double, C.c_long, dp]
+        L.gnx_hmc_condensate_sigma.argtypes = [vp, dp]
+        L.gnx_hmc_cg_stats.argtypes = [vp, dp, C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -213,3 +230,75 @@ class Context:
 
     def scale_by_field(self, field, x, y):
         _check(lib().gnx_scale_by_field(self.h, _ptr(field), _ptr(as_real(x)), _ptr(as_real(y))))
+
+
+def _dptr(a):
+    import numpy as np
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class HMC:
+    """Device-resident sigma-HMC state (gnx_hmc_*).  Draws come from the caller."""
+
+    def __init__(self, ctx, beta_s=0.85, nflav=4):
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(lib().gnx_hmc_create(ctx.h, beta_s, nflav, C.byref(h)))
+        self.h = h
+        self.beta_s = beta_s
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gnx_hmc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_sigma(self, sigma):
+        a, p = _dptr(sigma)
+        _check(lib().gnx_hmc_set_sigma(self.h, p))
+
+    def get_sigma(self):
+        import numpy as np
+        s = np.zeros(self.ctx.V)
+        m = np.zeros(self.ctx.V)
+        _check(lib().gnx_hmc_get_sigma(self.h, s.ctypes.data_as(C.POINTER(C.c_double)),
+                                       m.ctypes.data_as(C.POINTER(C.c_double))))
+        return s, m
+
+    def trajectory(self, mom, chi0, chi1, eps, steps, tol_high, tol_medium, max_iter=0):
+        r = TrajResult()
+        a, pa = _dptr(mom)
+        b, pb = _dptr(chi0)
+        c, pc = _dptr(chi1)
+        st = lib().gnx_hmc_trajectory(self.h, pa, pb, pc, eps, steps, tol_high, tol_medium, max_iter, C.byref(r))
+        if st not in (0, 3):
+            _check(st)
+        return st, r
+
+    def accept(self, accepted):
+        _check(lib().gnx_hmc_accept(self.h, int(accepted)))
+
+    def condensate(self, z2, tol, max_iter=0):
+        import numpy as np
+        out = np.zeros(11)
+        a, p = _dptr(z2)
+        _check(lib().gnx_hmc_condensate(self.h, p, tol, max_iter, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def condensate_sigma(self):
+        import numpy as np
+        out = np.zeros(8)
+        _check(lib().gnx_hmc_condensate_sigma(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def cg_stats(self):
+        it = C.c_double()
+        calls = C.c_int()
+        _check(lib().gnx_hmc_cg_stats(self.h, C.byref(it), C.byref(calls)))
+        return it.value, calls.value

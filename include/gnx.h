@@ -139,6 +139,52 @@ gnx_status gnx_norm2(gnx_ctx *ctx, const double *a, double *out);
 /* y = x * field (complex vector times real field, condensate.c:221,225) */
 gnx_status gnx_scale_by_field(gnx_ctx *ctx, const double *field, const double *x, double *y);
 
+
+/* ------------------------------------------------------------------------
+ * A10/A12/A14. Device-resident sigma-HMC state of one lattice.
+ * The host keeps the random-number generator (random.c) and passes each
+ * trajectory's draws in the reference's order (momenta V x Gauss1, then chi_0,
+ * then chi_1, each V x Gauss1; moleculardynamics.c:1241-1250); the
+ * Metropolis uniform is drawn by the caller after the trajectory and the
+ * decision handed back with gnx_hmc_accept (moleculardynamics.c:1385-1409).
+ * ------------------------------------------------------------------------ */
+typedef struct gnx_hmc gnx_hmc;
+
+typedef struct gnx_traj_result {
+    double dH, H0, H1;      /* final - starting energy, moleculardynamics.c:1379-1385 */
+    int ok;                 /* 0: a CG exhausted max_iter; sigma restored (:1262-1283) */
+    int cg_solves;          /* solves that converged inside the loop */
+    long cg_iterations;     /* sum of their iteration counts (kit->iterations) */
+} gnx_traj_result;
+
+gnx_status gnx_hmc_create(gnx_ctx *ctx, double beta_s, int nflav, gnx_hmc **out);
+gnx_status gnx_hmc_destroy(gnx_hmc *h);
+/* host <-> device copies of sigma (V doubles, f-order); set recomputes sm */
+gnx_status gnx_hmc_set_sigma(gnx_hmc *h, const double *sigma_host);
+gnx_status gnx_hmc_get_sigma(gnx_hmc *h, double *sigma_host, double *sm_host);
+/* device pointers of the state: sigma, sm, momenta, force [V]; X, Y, phi [2][2V] */
+gnx_status gnx_hmc_device_fields(gnx_hmc *h, double **sigma, double **sm, double **mom, double **force,
+                                 double **X, double **Y, double **phi);
+/* One LeapFrog2MinimumNormSigmaVelocityFirst trajectory (moleculardynamics.c:1234-1379).
+ * mom_host, chi0_host, chi1_host: V real draws each.  On return sigma holds
+ * the proposal; res->dH = H1 - H0. */
+gnx_status gnx_hmc_trajectory(gnx_hmc *h, const double *mom_host, const double *chi0_host,
+                              const double *chi1_host, double eps, int steps, double tol_high,
+                              double tol_medium, long max_iter, gnx_traj_result *res);
+/* accepted = 0 restores sigma and its mass (moleculardynamics.c:1407-1408) */
+gnx_status gnx_hmc_accept(gnx_hmc *h, int accepted);
+/* ComputeCondensateDirac (condensate.c:114-244) with the caller's real Z2
+ * draws z2_host[6*V] (4 trace vectors, then the two susceptibility vectors);
+ * the first solve starts from Y_0 of the last force, as the driver passes
+ * inverted_phi_obs[0] (observables.c:247).  out[11] = tr1, tr1_abs, tr2, tr4,
+ * mu2, mu4, chi, chi_abs, chip, 12_nosigma, 12_withsigma. */
+gnx_status gnx_hmc_condensate(gnx_hmc *h, const double *z2_host, double tol, long max_iter, double *out);
+/* CondensateSigmaPi (observables.c:323-369): out[8] = sigma, |sigma|, mu2,
+ * mu4, chi, chip, chip_pbc1, chip_pbc2 */
+gnx_status gnx_hmc_condensate_sigma(gnx_hmc *h, double *out);
+/* accumulated CG statistics (kit->iterations, kit->CG_call) */
+gnx_status gnx_hmc_cg_stats(gnx_hmc *h, double *iterations, int *calls);
+
 #ifdef __cplusplus
 }
 #endif
