@@ -222,6 +222,15 @@ gnx_status gnx_cg(gnx_ctx* c, const double* sm, const double* b, double* x, doub
   return gnx_cg_batch(c, sm, b, x, 1, tol, max_iter, start, prec, res);
 }
 
+gnx_status gnx_cg_shape(gnx_ctx* c, int nrhs, int* spt, int* xcd) {
+  CHECK_ARG(c, "gnx_cg_shape: null context");
+  if (cgm_shape_of(c->cgm, nrhs, spt, xcd) < 0) {
+    set_err("gnx_cg_shape: no tuned shape for nrhs=%d yet", nrhs);
+    return GNX_ERR_STATE;
+  }
+  return GNX_OK;
+}
+
 gnx_status gnx_sigma_force_from_solutions(gnx_ctx* c, const double* sigma, double beta_s, int nflav,
                                           const double* X0, const double* Y0, const double* X1, const double* Y1,
                                           double* force) {

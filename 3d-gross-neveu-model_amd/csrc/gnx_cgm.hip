@@ -24,6 +24,7 @@
 // are independent: each keeps its alpha, beta and exit iteration, and a
 // finished system's blocks return at once.  The host replays a fixed graph of
 // CHUNK iterations until every system is inactive.
+#include <stdlib.h>
 #include <string.h>
 
 #include "gnx_internal.h"
@@ -55,18 +56,43 @@ struct CGMArgs {
   double tol;
   long max_iter;
   int nrhs;
-  int nb;
+  int nb;              // blocks of the iteration kernels (per system)
+  int nb0;             // blocks of the start kernels (one site per thread)
+  int nbs;             // stride of a partial slot (>= nb, nb0)
   int start;           // GNX_START_NULL / GNX_START_IMPOSED
 };
 
-GNX_DEVONLY double* slot(const CGMArgs& a, int s, int rhs) {
-  return a.part + ((size_t)s * a.nrhs + rhs) * a.nb;
+// Launch shapes: SPT sites per thread; XCD = remap blocks so that each XCD
+// (workgroups are dispatched round-robin over the 8 XCDs) sweeps one
+// contiguous lex slab of both parities, keeping a slab's halo in its own L2.
+constexpr int NUM_XCD = 8;
+
+GNX_DEVONLY int xcd_block(int b, int nb) {
+  int q = nb / NUM_XCD, rem = nb % NUM_XCD, x = b % NUM_XCD;
+  return x * q + (x < rem ? x : rem) + b / NUM_XCD;
 }
 
-GNX_DEVONLY double reduce_slot(const CGMArgs& a, int s, int rhs, double* lds) {
+template <int SPT, bool XCD>
+struct SiteMap {
+  int base;
+  GNX_DEVONLY SiteMap(int nb) { base = (XCD ? xcd_block(blockIdx.x, nb) : (int)blockIdx.x) * SPT * CGM_BS; }
+  // j-th logical site of this thread, or -1
+  GNX_DEVONLY int f(const Geom& g, int i) const {
+    int j = base + i * CGM_BS + (int)threadIdx.x;
+    if (j >= g.V) return -1;
+    return XCD ? (j >> 1) + (j & 1) * g.half : j;
+  }
+};
+
+GNX_DEVONLY double* slot(const CGMArgs& a, int s, int rhs) {
+  return a.part + ((size_t)s * a.nrhs + rhs) * a.nbs;
+}
+
+// sum of the first n partials of a slot, fixed order, identical in every block
+GNX_DEVONLY double reduce_slot(const CGMArgs& a, int s, int rhs, int n, double* lds) {
   const double* src = slot(a, s, rhs);
   double acc = 0.0;
-  for (int i = threadIdx.x; i < a.nb; i += CGM_BS) acc += src[i];
+  for (int i = threadIdx.x; i < n; i += CGM_BS) acc += src[i];
   return block_sum(acc, lds);
 }
 
@@ -136,7 +162,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_null_p3(const CGMArgs* __restric
   __shared__ double lds[CGM_BS / 64];
   const int rhs = blockIdx.y;
   const size_t off = (size_t)rhs * a.g.V;
-  const double nr = reduce_slot(a, 0, rhs, lds);
+  const double nr = reduce_slot(a, 0, rhs, a.nb0, lds);
   const bool early = sqrt(nr) < a.tol;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     CGMState s = {0.0, early ? 0 : 1, 0, early ? 1 : 0, early ? 1 : 0};
@@ -157,6 +183,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_null_p3(const CGMArgs* __restric
 }
 
 // ---- iteration kernels -----------------------------------------------------
+template <int SPT, bool XCD>
 __global__ __launch_bounds__(CGM_BS) void k_cgm_A(const CGMArgs* __restrict__ ap) {
   const CGMArgs& a = *ap;
   __shared__ double lds[CGM_BS / 64];
@@ -169,7 +196,8 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_A(const CGMArgs* __restrict__ ap
     return;
   }
   const Geom g = a.g;
-  const double rho = reduce_slot(a, 1, rhs, lds);
+  // slot 1 holds the start kernel's partials before the first iteration
+  const double rho = reduce_slot(a, 1, rhs, s.iters == 0 ? a.nb0 : a.nb, lds);
   double beta = 0.0;
   if (s.iters == 0) {
     // ||b|| < tol returns before the loop only in the imposed start
@@ -201,14 +229,18 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_A(const CGMArgs* __restrict__ ap
   const double2* r = a.r + off;
   const double2* pold = ((s.iters & 1) ? a.p1 : a.p0) + off;
   double2* pnew = ((s.iters & 1) ? a.p0 : a.p1) + off;
-  const int f = blockIdx.x * CGM_BS + threadIdx.x;
+  double2* t = a.t + off;
+  const SiteMap<SPT, XCD> map(a.nb);
   double acc = 0.0;
-  if (f < g.V) {
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    const int f = map.f(g, i);
+    if (f < 0) break;
     auto ldp = [&](int q) { return cadd(r[q], rmul(beta, pold[q])); };
     pnew[f] = ldp(f);
     double2 tt = apply_Mdag(g, decode(g, f), a.sm[f], ldp);
-    a.t[off + f] = tt;
-    acc = cnorm2(tt);
+    t[f] = tt;
+    acc += cnorm2(tt);
   }
   acc = block_sum(acc, lds);
   if (threadIdx.x == 0) slot(a, 0, rhs)[blockIdx.x] = acc;
@@ -218,6 +250,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_A(const CGMArgs* __restrict__ ap
   }
 }
 
+template <int SPT, bool XCD>
 __global__ __launch_bounds__(CGM_BS) void k_cgm_B(const CGMArgs* __restrict__ ap) {
   const CGMArgs& a = *ap;
   __shared__ double lds[CGM_BS / 64];
@@ -230,21 +263,30 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_B(const CGMArgs* __restrict__ ap
     return;
   }
   const Geom g = a.g;
-  const double gamma = reduce_slot(a, 0, rhs, lds);
-  const double al = s.rho / gamma;
   const size_t off = (size_t)rhs * g.V;
   const double2* pnew = ((s.iters & 1) ? a.p0 : a.p1) + off;
   const double2* t = a.t + off;
   double2* r = a.r + off;
   double2* x = a.x + off;
-  const int f = blockIdx.x * CGM_BS + threadIdx.x;
+  const SiteMap<SPT, XCD> map(a.nb);
+  // the stencil does not depend on alpha: issue it before the reduction
+  double2 w[SPT];
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    const int f = map.f(g, i);
+    if (f >= 0) w[i] = apply_M(g, decode(g, f), a.sm[f], [&](int q) { return t[q]; });
+  }
+  const double gamma = reduce_slot(a, 0, rhs, a.nb, lds);
+  const double al = s.rho / gamma;
   double acc = 0.0;
-  if (f < g.V) {
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    const int f = map.f(g, i);
+    if (f < 0) break;
     x[f] = cadd(x[f], rmul(al, pnew[f]));
-    double2 w = apply_M(g, decode(g, f), a.sm[f], [&](int q) { return t[q]; });
-    double2 rr = cadd(r[f], rmul(-al, w));
+    double2 rr = cadd(r[f], rmul(-al, w[i]));
     r[f] = rr;
-    acc = cnorm2(rr);
+    acc += cnorm2(rr);
   }
   acc = block_sum(acc, lds);
   if (threadIdx.x == 0) slot(a, 1, rhs)[blockIdx.x] = acc;
@@ -254,13 +296,36 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_B(const CGMArgs* __restrict__ ap
   }
 }
 
+// launch shapes the tuner may pick from
+struct CGMShape {
+  int spt;
+  bool xcd;
+  const void* A;
+  const void* B;
+};
+
+static const CGMShape kShapes[] = {
+    {1, false, (const void*)k_cgm_A<1, false>, (const void*)k_cgm_B<1, false>},
+    {1, true, (const void*)k_cgm_A<1, true>, (const void*)k_cgm_B<1, true>},
+    {2, false, (const void*)k_cgm_A<2, false>, (const void*)k_cgm_B<2, false>},
+    {2, true, (const void*)k_cgm_A<2, true>, (const void*)k_cgm_B<2, true>},
+    {4, false, (const void*)k_cgm_A<4, false>, (const void*)k_cgm_B<4, false>},
+    {4, true, (const void*)k_cgm_A<4, true>, (const void*)k_cgm_B<4, true>},
+};
+constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+
 // ---- host side ---------------------------------------------------------------
 constexpr int CGM_CHUNK = 16;  // iterations per graph replay
 
 struct CGMGraph {
   int nrhs = 0;
-  int nb = 0;
+  int shape = -1;
   hipGraphExec_t exec = nullptr;
+};
+
+struct CGMTune {
+  int nrhs = 0;
+  int shape = -1;
 };
 
 struct CGMachine {
@@ -268,13 +333,14 @@ struct CGMachine {
   CGMArgs* hargs = nullptr;     // pinned host staging
   CGMState* state = nullptr;    // device [2][cap]
   CGMState* hstate = nullptr;   // pinned [cap]
-  CGDevResult* hres = nullptr;  // pinned [cap]
   double* part = nullptr;
   int cap = 0;
-  int nb_cap = 0;
+  int nbs = 0;
   hipStream_t capture = nullptr;
   CGMGraph graphs[8];
+  CGMTune tuned[8];
   int last_iters = 0;
+  int forced_shape = -1;        // GNX_CG_SHAPE
 };
 
 static void cgm_free(CGMachine* m) {
@@ -285,7 +351,6 @@ static void cgm_free(CGMachine* m) {
   if (m->hargs) (void)hipHostFree(m->hargs);
   if (m->state) (void)hipFree(m->state);
   if (m->hstate) (void)hipHostFree(m->hstate);
-  if (m->hres) (void)hipHostFree(m->hres);
   if (m->part) (void)hipFree(m->part);
   if (m->capture) (void)hipStreamDestroy(m->capture);
   delete m;
@@ -293,9 +358,9 @@ static void cgm_free(CGMachine* m) {
 
 void cgm_release(void* m) { cgm_free((CGMachine*)m); }
 
-static hipError_t cgm_get(void** slotp, int nrhs, int nb, CGMachine** out) {
+static hipError_t cgm_get(void** slotp, int nrhs, int nbs, CGMachine** out) {
   CGMachine* m = (CGMachine*)*slotp;
-  if (m && (m->cap < nrhs || m->nb_cap < nb)) {
+  if (m && (m->cap < nrhs || m->nbs < nbs)) {
     cgm_free(m);
     m = nullptr;
     *slotp = nullptr;
@@ -308,31 +373,40 @@ static hipError_t cgm_get(void** slotp, int nrhs, int nb, CGMachine** out) {
     if (e == hipSuccess) e = hipHostMalloc((void**)&m->hargs, sizeof(CGMArgs), hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc(&m->state, sizeof(CGMState) * 2 * cap);
     if (e == hipSuccess) e = hipHostMalloc((void**)&m->hstate, sizeof(CGMState) * cap, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&m->hres, sizeof(CGDevResult) * cap, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipMalloc(&m->part, sizeof(double) * 2 * (size_t)cap * nb);
+    if (e == hipSuccess) e = hipMalloc(&m->part, sizeof(double) * 2 * (size_t)cap * nbs);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->capture, hipStreamNonBlocking);
     if (e != hipSuccess) {
       cgm_free(m);
       return e;
     }
     m->cap = cap;
-    m->nb_cap = nb;
+    m->nbs = nbs;
+    const char* env = getenv("GNX_CG_SHAPE");
+    if (env) {
+      int v = atoi(env);
+      if (v >= 0 && v < kNumShapes) m->forced_shape = v;
+    }
     *slotp = m;
   }
   *out = m;
   return hipSuccess;
 }
 
-static hipError_t cgm_graph(CGMachine* m, int nrhs, int nb, hipGraphExec_t* exec) {
-  for (auto& g : m->graphs)
-    if (g.exec && g.nrhs == nrhs && g.nb == nb) {
-      *exec = g.exec;
+static int shape_blocks(const Geom& g, int shape) {
+  int per = CGM_BS * kShapes[shape].spt;
+  return (g.V + per - 1) / per;
+}
+
+static hipError_t cgm_graph(CGMachine* m, const Geom& g, int nrhs, int shape, hipGraphExec_t* exec) {
+  for (auto& gr : m->graphs)
+    if (gr.exec && gr.nrhs == nrhs && gr.shape == shape) {
+      *exec = gr.exec;
       return hipSuccess;
     }
   CGMGraph* slotg = nullptr;
-  for (auto& g : m->graphs)
-    if (!g.exec) {
-      slotg = &g;
+  for (auto& gr : m->graphs)
+    if (!gr.exec) {
+      slotg = &gr;
       break;
     }
   if (!slotg) {
@@ -343,10 +417,12 @@ static hipError_t cgm_graph(CGMachine* m, int nrhs, int nb, hipGraphExec_t* exec
   hipGraph_t graph;
   hipError_t e = hipStreamBeginCapture(m->capture, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return e;
-  dim3 grid(nb, nrhs), blk(CGM_BS);
+  dim3 grid(shape_blocks(g, shape), nrhs), blk(CGM_BS);
+  const CGMArgs* dp = m->dargs;
+  void* args[] = {(void*)&dp};
   for (int i = 0; i < CGM_CHUNK; i++) {
-    hipLaunchKernelGGL(k_cgm_A, grid, blk, 0, m->capture, (const CGMArgs*)m->dargs);
-    hipLaunchKernelGGL(k_cgm_B, grid, blk, 0, m->capture, (const CGMArgs*)m->dargs);
+    (void)hipLaunchKernel(kShapes[shape].A, grid, blk, args, 0, m->capture);
+    (void)hipLaunchKernel(kShapes[shape].B, grid, blk, args, 0, m->capture);
   }
   e = hipStreamEndCapture(m->capture, &graph);
   if (e != hipSuccess) return e;
@@ -354,24 +430,18 @@ static hipError_t cgm_graph(CGMachine* m, int nrhs, int nb, hipGraphExec_t* exec
   (void)hipGraphDestroy(graph);
   if (e != hipSuccess) return e;
   slotg->nrhs = nrhs;
-  slotg->nb = nb;
+  slotg->shape = shape;
   *exec = slotg->exec;
   return hipSuccess;
 }
 
-hipError_t cgm_f64_run(void** machine_slot, const Geom& g, const double* sm, const double2* b, double2* x,
-                       double tol, long max_iter, int start, CGWork& w, int nrhs, hipStream_t st,
-                       CGDevResult* host_res) {
-  const int nb = (g.V + CGM_BS - 1) / CGM_BS;
-  CGMachine* m = nullptr;
-  hipError_t e = cgm_get(machine_slot, nrhs, nb, &m);
-  if (e != hipSuccess) return e;
-  hipGraphExec_t exec;
-  e = cgm_graph(m, nrhs, nb, &exec);
-  if (e != hipSuccess) return e;
+// stage the arguments and run the start kernels
+static hipError_t cgm_begin(CGMachine* m, const Geom& g, const double* sm, const double2* b, double2* x, double tol,
+                            long max_iter, int start, CGWork& w, int nrhs, int shape, hipStream_t st) {
   // the previous solve's staging may still be read by an in-flight copy
-  e = hipStreamSynchronize(st);
+  hipError_t e = hipStreamSynchronize(st);
   if (e != hipSuccess) return e;
+  const int nb0 = (g.V + CGM_BS - 1) / CGM_BS;
   CGMArgs& a = *m->hargs;
   a.g = g;
   a.sm = sm;
@@ -387,11 +457,13 @@ hipError_t cgm_f64_run(void** machine_slot, const Geom& g, const double* sm, con
   a.tol = tol;
   a.max_iter = max_iter;
   a.nrhs = nrhs;
-  a.nb = nb;
+  a.nb = shape_blocks(g, shape);
+  a.nb0 = nb0;
+  a.nbs = m->nbs;
   a.start = start;
   e = hipMemcpyAsync(m->dargs, m->hargs, sizeof(CGMArgs), hipMemcpyHostToDevice, st);
   if (e != hipSuccess) return e;
-  dim3 grid(nb, nrhs), blk(CGM_BS);
+  dim3 grid(nb0, nrhs), blk(CGM_BS);
   if (start == GNX_START_NULL) {
     hipLaunchKernelGGL(k_cgm_null_p1, grid, blk, 0, st, (const CGMArgs*)m->dargs);
     hipLaunchKernelGGL(k_cgm_null_p2, grid, blk, 0, st, (const CGMArgs*)m->dargs);
@@ -399,7 +471,87 @@ hipError_t cgm_f64_run(void** machine_slot, const Geom& g, const double* sm, con
   } else {
     hipLaunchKernelGGL(k_cgm_start_imposed, grid, blk, 0, st, (const CGMArgs*)m->dargs);
   }
-  e = hipGetLastError();
+  return hipGetLastError();
+}
+
+// Autotuner: time every launch shape on 2*CHUNK iterations of the caller's
+// own system (imposed start into scratch copies, so the caller's x is not
+// touched) and keep the fastest per batch size.
+static hipError_t cgm_tune(CGMachine* m, const Geom& g, const double* sm, const double2* b, CGWork& w, int nrhs,
+                           hipStream_t st, int* best) {
+  for (auto& t : m->tuned)
+    if (t.shape >= 0 && t.nrhs == nrhs) {
+      *best = t.shape;
+      return hipSuccess;
+    }
+  if (m->forced_shape >= 0) {
+    *best = m->forced_shape;
+    for (auto& t : m->tuned)
+      if (t.shape < 0) {
+        t.nrhs = nrhs;
+        t.shape = m->forced_shape;
+        break;
+      }
+    return hipSuccess;
+  }
+  double2* xs = nullptr;
+  hipError_t e = hipMalloc(&xs, sizeof(double2) * (size_t)g.V * nrhs);
+  if (e != hipSuccess) return e;
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  float best_ms = 1e30f;
+  int best_shape = 0;
+  for (int sh = 0; sh < kNumShapes && e == hipSuccess; sh++) {
+    hipGraphExec_t exec;
+    e = cgm_graph(m, g, nrhs, sh, &exec);
+    if (e != hipSuccess) break;
+    float ms_min = 1e30f;
+    for (int rep = 0; rep < 3 && e == hipSuccess; rep++) {
+      e = cgm_begin(m, g, sm, b, xs, 0.0, 2 * CGM_CHUNK, GNX_START_IMPOSED, w, nrhs, sh, st);
+      if (e != hipSuccess) break;
+      (void)hipEventRecord(e0, st);
+      for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipGraphLaunch(exec, st);
+      (void)hipEventRecord(e1, st);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (rep > 0 && ms < ms_min) ms_min = ms;
+    }
+    if (ms_min < best_ms) {
+      best_ms = ms_min;
+      best_shape = sh;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(xs);
+  if (e != hipSuccess) return e;
+  for (auto& t : m->tuned)
+    if (t.shape < 0) {
+      t.nrhs = nrhs;
+      t.shape = best_shape;
+      break;
+    }
+  *best = best_shape;
+  return hipSuccess;
+}
+
+hipError_t cgm_f64_run(void** machine_slot, const Geom& g, const double* sm, const double2* b, double2* x,
+                       double tol, long max_iter, int start, CGWork& w, int nrhs, hipStream_t st,
+                       CGDevResult* host_res) {
+  const int nbs = (g.V + CGM_BS - 1) / CGM_BS;
+  CGMachine* m = nullptr;
+  hipError_t e = cgm_get(machine_slot, nrhs, nbs, &m);
+  if (e != hipSuccess) return e;
+  int shape = 0;
+  e = cgm_tune(m, g, sm, b, w, nrhs, st, &shape);
+  if (e != hipSuccess) return e;
+  hipGraphExec_t exec;
+  e = cgm_graph(m, g, nrhs, shape, &exec);
+  if (e != hipSuccess) return e;
+  e = cgm_begin(m, g, sm, b, x, tol, max_iter, start, w, nrhs, shape, st);
   if (e != hipSuccess) return e;
   // replay: first the expected number of chunks, then one at a time
   long done_iters = 0;
@@ -431,6 +583,18 @@ hipError_t cgm_f64_run(void** machine_slot, const Geom& g, const double* sm, con
     e = hipStreamSynchronize(st);
   }
   return e;
+}
+
+int cgm_shape_of(void* machine, int nrhs, int* spt, int* xcd) {
+  CGMachine* m = (CGMachine*)machine;
+  if (!m) return -1;
+  for (auto& t : m->tuned)
+    if (t.shape >= 0 && t.nrhs == nrhs) {
+      if (spt) *spt = kShapes[t.shape].spt;
+      if (xcd) *xcd = kShapes[t.shape].xcd ? 1 : 0;
+      return t.shape;
+    }
+  return -1;
 }
 
 }  // namespace gnx

@@ -56,6 +56,7 @@ hipError_t cgm_f64_run(void** machine_slot, const Geom& g, const double* sm, con
                        double tol, long max_iter, int start, CGWork& w, int nrhs, hipStream_t st,
                        CGDevResult* host_res);
 void cgm_release(void* machine);
+int cgm_shape_of(void* machine, int nrhs, int* spt, int* xcd);
 
 void set_err(const char* fmt, ...);
 

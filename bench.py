@@ -72,6 +72,7 @@ def main():
     ap.add_argument("--L", type=int, default=64)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch", default="4,16", help="batch sizes of the batched FP64 CG leg ('' to skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
@@ -152,7 +153,7 @@ def main():
         "data": "synthetic: sigma iid N(0, 2/(4*0.85)), complex Z2 source, numpy seed 1231",
         "config": {"workload": f"single-RHS FP64 CG on M M^dag to |r|<{args.tol:g} at {L}^3",
                    "L": L, "V": V, "tol": args.tol, "start": "imposed", "cg_iterations": it,
-                   "true_residual": true_res},
+                   "true_residual": true_res, "cg_shape": ctx.cg_shape(1)},
         "dirac": {"apps_per_s": 1.0 / t_app, "GB_s": dirac_gbs, "frac_hbm": dirac_gbs / HBM_PEAK_GBS,
                   "bytes_per_site": BYTES_DIRAC_F64},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -160,6 +161,27 @@ def main():
                      "kernel": "k_cg_f64", "kernel_ms": avg_k * 1e3,
                      "bytes_per_site_iter": BYTES_CG_ITER_F64},
     }
+    # ---------------- batched FP64 CG ----------------
+    batched = {}
+    for nb in [int(v) for v in args.batch.split(",") if v]:
+        rng = np.random.default_rng(7 + nb)
+        B = torch.from_numpy(np.where(rng.random(2 * V * nb) <= 0.5, 1.0, -1.0)).to(dev)
+        X = ctx.vec(nb)
+        ctx.cg_batch(sm, B, X, nb, args.tol, start=gnx.START_IMPOSED)
+        torch.cuda.synchronize()
+        reps = max(1, 8 // nb)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            st, res = ctx.cg_batch(sm, B, X, nb, args.tol, start=gnx.START_IMPOSED)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        its = [r.iterations for r in res]
+        batched[str(nb)] = {"rhs_solves_per_s": nb * reps / dt, "ms_per_batch": dt / reps * 1e3,
+                            "iterations_max": max(its), "iterations_min": min(its),
+                            "all_converged": all(r.converged for r in res), "shape": ctx.cg_shape(nb),
+                            "algorithmic_GB_s": BYTES_CG_ITER_F64 * V * sum(its) / (dt / reps) / 1e9}
+        del B, X
+    result["batched_f64"] = batched
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(L, sigma_h, b_h, it)
     print(json.dumps(result), flush=True)

@@ -111,6 +111,12 @@ gnx_status gnx_cg_batch(gnx_ctx *ctx, const double *sm, const double *b, double 
                         double tol, long max_iter, gnx_cg_start start, gnx_prec prec,
                         gnx_cg_result *res);
 
+/* Launch shape the CG autotuner picked for batches of nrhs systems on this
+ * context (sites per thread, XCD-slab block mapping); GNX_ERR_STATE if no
+ * solve of that batch size has run yet.  GNX_CG_SHAPE=<n> in the environment
+ * pins the shape instead of tuning. */
+gnx_status gnx_cg_shape(gnx_ctx *ctx, int nrhs, int *sites_per_thread, int *xcd_mapped);
+
 /* A9. Sigma force (moleculardynamics.c:590-647):
  *   X_f = CG(phi_f) with START_X0_NULL from the incoming X_f, Y_f = M^dag X_f,
  *   F = sum_f sum_corners 0.125 Re(conj X_f Y_f) - (nflav*beta_s/2) sigma.
