@@ -135,6 +135,7 @@ gnx_status gnx_destroy(gnx_ctx* c) {
   (void)hipFree(c->cg.r); (void)hipFree(c->cg.p0); (void)hipFree(c->cg.p1); (void)hipFree(c->cg.t);
   (void)hipFree(c->cg.part); (void)hipFree(c->cg.res);
   if (c->cgm) cgm_release(c->cgm);
+  if (c->cgmix) cgmix_release(c->cgmix);
   delete c;
   return GNX_OK;
 }
@@ -197,10 +198,22 @@ gnx_status gnx_cg_batch(gnx_ctx* c, const double* sm, const double* b, double* x
   CHECK_ARG((const void*)b != (const void*)x, "gnx_cg: b and x alias");
   if (max_iter <= 0) max_iter = c->V + 1;
   HIP_TRY(hipSetDevice(c->device));
-  if (prec != GNX_PREC_F64) {
-    set_err("gnx_cg: mixed precision not available in this build");
-    return GNX_ERR_ARG;
+  if (prec == GNX_PREC_F32 || prec == GNX_PREC_BF16) {
+    CGDevResult hr[64];
+    HIP_TRY(cgmix_run(&c->cgmix, c->g, sm, (const double2*)b, (double2*)x, tol, c->reliable_delta, max_iter,
+                      (int)start, (int)prec, nrhs, c->stream, hr));
+    int all = 1;
+    for (int k = 0; k < nrhs; k++) {
+      fill_result(&res[k], hr[k]);
+      all &= hr[k].converged;
+    }
+    if (!all) {
+      set_err("gnx_cg: not converged within %ld iterations", max_iter);
+      return GNX_ERR_NOCONV;
+    }
+    return GNX_OK;
   }
+  CHECK_ARG(prec == GNX_PREC_F64, "gnx_cg: bad precision");
   gnx_status s = ensure_cg(c, nrhs);
   if (s != GNX_OK) return s;
   CGDevResult hr[64];
@@ -220,6 +233,13 @@ gnx_status gnx_cg_batch(gnx_ctx* c, const double* sm, const double* b, double* x
 gnx_status gnx_cg(gnx_ctx* c, const double* sm, const double* b, double* x, double tol, long max_iter,
                   gnx_cg_start start, gnx_prec prec, gnx_cg_result* res) {
   return gnx_cg_batch(c, sm, b, x, 1, tol, max_iter, start, prec, res);
+}
+
+gnx_status gnx_set_reliable_delta(gnx_ctx* c, double delta) {
+  CHECK_ARG(c, "gnx_set_reliable_delta: null context");
+  CHECK_ARG(delta > 0.0 && delta < 1.0, "gnx_set_reliable_delta: delta must be in (0, 1)");
+  c->reliable_delta = delta;
+  return GNX_OK;
 }
 
 gnx_status gnx_cg_shape(gnx_ctx* c, int nrhs, int* spt, int* xcd) {

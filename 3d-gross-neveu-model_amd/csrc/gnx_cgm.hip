@@ -27,18 +27,20 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "gnx_engine.h"
 #include "gnx_internal.h"
 
 namespace gnx {
 
-constexpr int CGM_BS = 256;
+constexpr int CGM_BS = ENG_BS;
 
+// per-system state; `active` and `iters` lead, as the replay engine polls them
 struct CGMState {
-  double rho;
   int active;
   int iters;
   int converged;
   int early;
+  double rho;
 };
 
 struct CGMArgs {
@@ -115,7 +117,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_start_imposed(const CGMArgs* __r
   acc = block_sum(acc, lds);
   if (threadIdx.x == 0) slot(a, 1, rhs)[blockIdx.x] = acc;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    CGMState s = {0.0, 1, 0, 0, 0};
+    CGMState s = {1, 0, 0, 0, 0.0};
     a.state[rhs] = s;
   }
 }
@@ -165,7 +167,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_null_p3(const CGMArgs* __restric
   const double nr = reduce_slot(a, 0, rhs, a.nb0, lds);
   const bool early = sqrt(nr) < a.tol;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    CGMState s = {0.0, early ? 0 : 1, 0, early ? 1 : 0, early ? 1 : 0};
+    CGMState s = {early ? 0 : 1, 0, early ? 1 : 0, early ? 1 : 0, 0.0};
     a.state[rhs] = s;
     if (early) {
       CGDevResult r = {1, 1, 0, 0, sqrt(nr), {0, 0, 0}};
@@ -205,7 +207,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_A(const CGMArgs* __restrict__ ap
     // prologue residual instead (:22)
     if (a.start == GNX_START_IMPOSED && sqrt(rho) < a.tol) {
       if (lead) {
-        CGMState n = {rho, 0, 0, 1, 1};
+        CGMState n = {0, 0, 1, 1, rho};
         *out = n;
         CGDevResult r = {1, 1, 0, 0, sqrt(rho), {0, 0, 0}};
         a.res[rhs] = r;
@@ -216,7 +218,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_A(const CGMArgs* __restrict__ ap
     const bool conv = sqrt(rho) < a.tol;
     if (conv || s.iters >= a.max_iter) {
       if (lead) {
-        CGMState n = {rho, 0, s.iters, conv ? 1 : 0, 0};
+        CGMState n = {0, s.iters, conv ? 1 : 0, 0, rho};
         *out = n;
         CGDevResult r = {conv ? 1 : 0, 0, s.iters, 0, sqrt(rho), {0, 0, 0}};
         a.res[rhs] = r;
@@ -245,7 +247,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_A(const CGMArgs* __restrict__ ap
   acc = block_sum(acc, lds);
   if (threadIdx.x == 0) slot(a, 0, rhs)[blockIdx.x] = acc;
   if (lead) {
-    CGMState n = {rho, 1, s.iters, 0, 0};
+    CGMState n = {1, s.iters, 0, 0, rho};
     *out = n;
   }
 }
@@ -291,7 +293,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_B(const CGMArgs* __restrict__ ap
   acc = block_sum(acc, lds);
   if (threadIdx.x == 0) slot(a, 1, rhs)[blockIdx.x] = acc;
   if (lead) {
-    CGMState n = {s.rho, 1, s.iters + 1, 0, 0};
+    CGMState n = {1, s.iters + 1, 0, 0, s.rho};
     *out = n;
   }
 }
@@ -315,72 +317,30 @@ static const CGMShape kShapes[] = {
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
 // ---- host side ---------------------------------------------------------------
-constexpr int CGM_CHUNK = 16;  // iterations per graph replay
-
-struct CGMGraph {
-  int nrhs = 0;
-  int shape = -1;
-  hipGraphExec_t exec = nullptr;
-};
-
-struct CGMTune {
-  int nrhs = 0;
-  int shape = -1;
-};
-
-struct CGMachine {
-  CGMArgs* dargs = nullptr;     // device copy of the arguments
-  CGMArgs* hargs = nullptr;     // pinned host staging
-  CGMState* state = nullptr;    // device [2][cap]
-  CGMState* hstate = nullptr;   // pinned [cap]
-  double* part = nullptr;
-  int cap = 0;
-  int nbs = 0;
-  hipStream_t capture = nullptr;
-  CGMGraph graphs[8];
-  CGMTune tuned[8];
-  int last_iters = 0;
-  int forced_shape = -1;        // GNX_CG_SHAPE
-};
-
-static void cgm_free(CGMachine* m) {
-  if (!m) return;
-  for (auto& g : m->graphs)
-    if (g.exec) (void)hipGraphExecDestroy(g.exec);
-  if (m->dargs) (void)hipFree(m->dargs);
-  if (m->hargs) (void)hipHostFree(m->hargs);
-  if (m->state) (void)hipFree(m->state);
-  if (m->hstate) (void)hipHostFree(m->hstate);
-  if (m->part) (void)hipFree(m->part);
-  if (m->capture) (void)hipStreamDestroy(m->capture);
-  delete m;
+static int shape_blocks(const Geom& g, int shape) {
+  int per = CGM_BS * kShapes[shape].spt;
+  return (g.V + per - 1) / per;
 }
 
-void cgm_release(void* m) { cgm_free((CGMachine*)m); }
+static EngineKernels shape_kernels(const Geom& g, int shape) {
+  EngineKernels k;
+  k.X = kShapes[shape].A;
+  k.Y = kShapes[shape].B;
+  k.blocks = shape_blocks(g, shape);
+  k.tag = shape;
+  return k;
+}
 
-static hipError_t cgm_get(void** slotp, int nrhs, int nbs, CGMachine** out) {
-  CGMachine* m = (CGMachine*)*slotp;
-  if (m && (m->cap < nrhs || m->nbs < nbs)) {
-    cgm_free(m);
-    m = nullptr;
-    *slotp = nullptr;
-  }
-  if (!m) {
-    m = new CGMachine();
-    int cap = nrhs < 2 ? 2 : nrhs;
-    hipError_t e = hipSuccess;
-    if (e == hipSuccess) e = hipMalloc(&m->dargs, sizeof(CGMArgs));
-    if (e == hipSuccess) e = hipHostMalloc((void**)&m->hargs, sizeof(CGMArgs), hipHostMallocDefault);
-    if (e == hipSuccess) e = hipMalloc(&m->state, sizeof(CGMState) * 2 * cap);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&m->hstate, sizeof(CGMState) * cap, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipMalloc(&m->part, sizeof(double) * 2 * (size_t)cap * nbs);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->capture, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-      cgm_free(m);
-      return e;
-    }
-    m->cap = cap;
-    m->nbs = nbs;
+struct CGMachine {
+  GraphEngine eng;
+  int forced_shape = -1;  // GNX_CG_SHAPE
+};
+
+void cgm_release(void* m) { delete (CGMachine*)m; }
+
+static CGMachine* cgm_get(void** slotp) {
+  if (!*slotp) {
+    CGMachine* m = new CGMachine();
     const char* env = getenv("GNX_CG_SHAPE");
     if (env) {
       int v = atoi(env);
@@ -388,61 +348,16 @@ static hipError_t cgm_get(void** slotp, int nrhs, int nbs, CGMachine** out) {
     }
     *slotp = m;
   }
-  *out = m;
-  return hipSuccess;
-}
-
-static int shape_blocks(const Geom& g, int shape) {
-  int per = CGM_BS * kShapes[shape].spt;
-  return (g.V + per - 1) / per;
-}
-
-static hipError_t cgm_graph(CGMachine* m, const Geom& g, int nrhs, int shape, hipGraphExec_t* exec) {
-  for (auto& gr : m->graphs)
-    if (gr.exec && gr.nrhs == nrhs && gr.shape == shape) {
-      *exec = gr.exec;
-      return hipSuccess;
-    }
-  CGMGraph* slotg = nullptr;
-  for (auto& gr : m->graphs)
-    if (!gr.exec) {
-      slotg = &gr;
-      break;
-    }
-  if (!slotg) {
-    slotg = &m->graphs[0];
-    (void)hipGraphExecDestroy(slotg->exec);
-    slotg->exec = nullptr;
-  }
-  hipGraph_t graph;
-  hipError_t e = hipStreamBeginCapture(m->capture, hipStreamCaptureModeThreadLocal);
-  if (e != hipSuccess) return e;
-  dim3 grid(shape_blocks(g, shape), nrhs), blk(CGM_BS);
-  const CGMArgs* dp = m->dargs;
-  void* args[] = {(void*)&dp};
-  for (int i = 0; i < CGM_CHUNK; i++) {
-    (void)hipLaunchKernel(kShapes[shape].A, grid, blk, args, 0, m->capture);
-    (void)hipLaunchKernel(kShapes[shape].B, grid, blk, args, 0, m->capture);
-  }
-  e = hipStreamEndCapture(m->capture, &graph);
-  if (e != hipSuccess) return e;
-  e = hipGraphInstantiate(&slotg->exec, graph, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(graph);
-  if (e != hipSuccess) return e;
-  slotg->nrhs = nrhs;
-  slotg->shape = shape;
-  *exec = slotg->exec;
-  return hipSuccess;
+  return (CGMachine*)*slotp;
 }
 
 // stage the arguments and run the start kernels
 static hipError_t cgm_begin(CGMachine* m, const Geom& g, const double* sm, const double2* b, double2* x, double tol,
                             long max_iter, int start, CGWork& w, int nrhs, int shape, hipStream_t st) {
-  // the previous solve's staging may still be read by an in-flight copy
-  hipError_t e = hipStreamSynchronize(st);
-  if (e != hipSuccess) return e;
   const int nb0 = (g.V + CGM_BS - 1) / CGM_BS;
-  CGMArgs& a = *m->hargs;
+  CGMArgs& a = *(CGMArgs*)m->eng.host_args();
+  hipError_t e = hipStreamSynchronize(st);  // the staging buffer may still be in flight
+  if (e != hipSuccess) return e;
   a.g = g;
   a.sm = sm;
   a.b = b;
@@ -451,87 +366,73 @@ static hipError_t cgm_begin(CGMachine* m, const Geom& g, const double* sm, const
   a.p0 = w.p0;
   a.p1 = w.p1;
   a.t = w.t;
-  a.part = m->part;
-  a.state = m->state;
+  a.part = m->eng.dev_part();
+  a.state = (CGMState*)m->eng.dev_state();
   a.res = w.res;
   a.tol = tol;
   a.max_iter = max_iter;
   a.nrhs = nrhs;
   a.nb = shape_blocks(g, shape);
   a.nb0 = nb0;
-  a.nbs = m->nbs;
+  a.nbs = nb0;
   a.start = start;
-  e = hipMemcpyAsync(m->dargs, m->hargs, sizeof(CGMArgs), hipMemcpyHostToDevice, st);
+  e = m->eng.stage_args(st);
   if (e != hipSuccess) return e;
+  const CGMArgs* dp = (const CGMArgs*)m->eng.dev_args();
   dim3 grid(nb0, nrhs), blk(CGM_BS);
   if (start == GNX_START_NULL) {
-    hipLaunchKernelGGL(k_cgm_null_p1, grid, blk, 0, st, (const CGMArgs*)m->dargs);
-    hipLaunchKernelGGL(k_cgm_null_p2, grid, blk, 0, st, (const CGMArgs*)m->dargs);
-    hipLaunchKernelGGL(k_cgm_null_p3, grid, blk, 0, st, (const CGMArgs*)m->dargs);
+    hipLaunchKernelGGL(k_cgm_null_p1, grid, blk, 0, st, dp);
+    hipLaunchKernelGGL(k_cgm_null_p2, grid, blk, 0, st, dp);
+    hipLaunchKernelGGL(k_cgm_null_p3, grid, blk, 0, st, dp);
   } else {
-    hipLaunchKernelGGL(k_cgm_start_imposed, grid, blk, 0, st, (const CGMArgs*)m->dargs);
+    hipLaunchKernelGGL(k_cgm_start_imposed, grid, blk, 0, st, dp);
   }
   return hipGetLastError();
 }
 
 // Autotuner: time every launch shape on 2*CHUNK iterations of the caller's
-// own system (imposed start into scratch copies, so the caller's x is not
+// own system (imposed start into a scratch solution, so the caller's x is not
 // touched) and keep the fastest per batch size.
 static hipError_t cgm_tune(CGMachine* m, const Geom& g, const double* sm, const double2* b, CGWork& w, int nrhs,
                            hipStream_t st, int* best) {
-  for (auto& t : m->tuned)
-    if (t.shape >= 0 && t.nrhs == nrhs) {
-      *best = t.shape;
+  GraphEngine& E = m->eng;
+  for (int i = 0; i < 8; i++)
+    if (E.tuned_tag[i] >= 0 && E.tuned_nrhs[i] == nrhs) {
+      *best = E.tuned_tag[i];
       return hipSuccess;
     }
-  if (m->forced_shape >= 0) {
-    *best = m->forced_shape;
-    for (auto& t : m->tuned)
-      if (t.shape < 0) {
-        t.nrhs = nrhs;
-        t.shape = m->forced_shape;
-        break;
-      }
-    return hipSuccess;
-  }
-  double2* xs = nullptr;
-  hipError_t e = hipMalloc(&xs, sizeof(double2) * (size_t)g.V * nrhs);
-  if (e != hipSuccess) return e;
-  hipEvent_t e0, e1;
-  (void)hipEventCreate(&e0);
-  (void)hipEventCreate(&e1);
-  float best_ms = 1e30f;
   int best_shape = 0;
-  for (int sh = 0; sh < kNumShapes && e == hipSuccess; sh++) {
-    hipGraphExec_t exec;
-    e = cgm_graph(m, g, nrhs, sh, &exec);
-    if (e != hipSuccess) break;
-    float ms_min = 1e30f;
-    for (int rep = 0; rep < 3 && e == hipSuccess; rep++) {
-      e = cgm_begin(m, g, sm, b, xs, 0.0, 2 * CGM_CHUNK, GNX_START_IMPOSED, w, nrhs, sh, st);
-      if (e != hipSuccess) break;
-      (void)hipEventRecord(e0, st);
-      for (int k = 0; k < 2 && e == hipSuccess; k++) e = hipGraphLaunch(exec, st);
-      (void)hipEventRecord(e1, st);
-      if (e == hipSuccess) e = hipEventSynchronize(e1);
-      float ms = 0.f;
-      (void)hipEventElapsedTime(&ms, e0, e1);
-      if (rep > 0 && ms < ms_min) ms_min = ms;
+  hipError_t e = hipSuccess;
+  if (m->forced_shape >= 0) {
+    best_shape = m->forced_shape;
+  } else {
+    double2* xs = nullptr;
+    e = hipMalloc(&xs, sizeof(double2) * (size_t)g.V * nrhs);
+    if (e != hipSuccess) return e;
+    float best_ms = 1e30f;
+    for (int sh = 0; sh < kNumShapes && e == hipSuccess; sh++) {
+      hipGraphExec_t exec;
+      e = E.graph(shape_kernels(g, sh), nrhs, &exec);
+      float ms_min = 1e30f;
+      for (int rep = 0; rep < 3 && e == hipSuccess; rep++) {
+        e = cgm_begin(m, g, sm, b, xs, 0.0, 2 * ENG_CHUNK, GNX_START_IMPOSED, w, nrhs, sh, st);
+        float ms = 0.f;
+        if (e == hipSuccess) e = E.time_replays(exec, 2, st, &ms);
+        if (rep > 0 && ms < ms_min) ms_min = ms;
+      }
+      if (e == hipSuccess && ms_min < best_ms) {
+        best_ms = ms_min;
+        best_shape = sh;
+      }
     }
-    if (ms_min < best_ms) {
-      best_ms = ms_min;
-      best_shape = sh;
-    }
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(xs);
+    if (e != hipSuccess) return e;
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipStreamSynchronize(st);
-  (void)hipFree(xs);
-  if (e != hipSuccess) return e;
-  for (auto& t : m->tuned)
-    if (t.shape < 0) {
-      t.nrhs = nrhs;
-      t.shape = best_shape;
+  for (int i = 0; i < 8; i++)
+    if (E.tuned_tag[i] < 0) {
+      E.tuned_tag[i] = best_shape;
+      E.tuned_nrhs[i] = nrhs;
       break;
     }
   *best = best_shape;
@@ -542,41 +443,19 @@ hipError_t cgm_f64_run(void** machine_slot, const Geom& g, const double* sm, con
                        double tol, long max_iter, int start, CGWork& w, int nrhs, hipStream_t st,
                        CGDevResult* host_res) {
   const int nbs = (g.V + CGM_BS - 1) / CGM_BS;
-  CGMachine* m = nullptr;
-  hipError_t e = cgm_get(machine_slot, nrhs, nbs, &m);
+  CGMachine* m = cgm_get(machine_slot);
+  hipError_t e = m->eng.ensure(nrhs, sizeof(CGMArgs), sizeof(CGMState), 2 * (size_t)(nrhs < 2 ? 2 : nrhs) * nbs);
   if (e != hipSuccess) return e;
   int shape = 0;
   e = cgm_tune(m, g, sm, b, w, nrhs, st, &shape);
   if (e != hipSuccess) return e;
   hipGraphExec_t exec;
-  e = cgm_graph(m, g, nrhs, shape, &exec);
+  e = m->eng.graph(shape_kernels(g, shape), nrhs, &exec);
   if (e != hipSuccess) return e;
   e = cgm_begin(m, g, sm, b, x, tol, max_iter, start, w, nrhs, shape, st);
   if (e != hipSuccess) return e;
-  // replay: first the expected number of chunks, then one at a time
-  long done_iters = 0;
-  int burst = (m->last_iters + CGM_CHUNK - 1) / CGM_CHUNK;
-  if (burst < 1) burst = 1;
-  const long cap_iters = max_iter + CGM_CHUNK;  // A(k) of the last chunk observes k == max_iter
-  for (;;) {
-    for (int i = 0; i < burst; i++) {
-      e = hipGraphLaunch(exec, st);
-      if (e != hipSuccess) return e;
-    }
-    done_iters += (long)burst * CGM_CHUNK;
-    e = hipMemcpyAsync(m->hstate, m->state, sizeof(CGMState) * nrhs, hipMemcpyDeviceToHost, st);
-    if (e != hipSuccess) return e;
-    e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return e;
-    bool any = false;
-    for (int k = 0; k < nrhs; k++) any |= m->hstate[k].active != 0;
-    if (!any) break;
-    if (done_iters > cap_iters) return hipErrorUnknown;  // state machine failed to stop
-    burst = 1;
-  }
-  int maxit = 0;
-  for (int k = 0; k < nrhs; k++) maxit = m->hstate[k].iters > maxit ? m->hstate[k].iters : maxit;
-  m->last_iters = maxit;
+  e = m->eng.run(exec, nrhs, max_iter, st, nullptr);
+  if (e != hipSuccess) return e;
   if (host_res) {
     e = hipMemcpyAsync(host_res, w.res, sizeof(CGDevResult) * nrhs, hipMemcpyDeviceToHost, st);
     if (e != hipSuccess) return e;
@@ -588,11 +467,12 @@ hipError_t cgm_f64_run(void** machine_slot, const Geom& g, const double* sm, con
 int cgm_shape_of(void* machine, int nrhs, int* spt, int* xcd) {
   CGMachine* m = (CGMachine*)machine;
   if (!m) return -1;
-  for (auto& t : m->tuned)
-    if (t.shape >= 0 && t.nrhs == nrhs) {
-      if (spt) *spt = kShapes[t.shape].spt;
-      if (xcd) *xcd = kShapes[t.shape].xcd ? 1 : 0;
-      return t.shape;
+  for (int i = 0; i < 8; i++)
+    if (m->eng.tuned_tag[i] >= 0 && m->eng.tuned_nrhs[i] == nrhs) {
+      int sh = m->eng.tuned_tag[i];
+      if (spt) *spt = kShapes[sh].spt;
+      if (xcd) *xcd = kShapes[sh].xcd ? 1 : 0;
+      return sh;
     }
   return -1;
 }

@@ -111,8 +111,6 @@ GNX_DEV double cnorm2(double2 a) { return a.x * a.x + a.y * a.y; }
 GNX_DEV double cdot_re(double2 a, double2 b) { return a.x * b.x + a.y * b.y; }
 GNX_DEV double cdot_im(double2 a, double2 b) { return a.x * b.y - a.y * b.x; }
 
-GNX_DEV float2 caddf(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-GNX_DEV float2 rmulf(float s, float2 a) { return make_float2(s * a.x, s * a.y); }
 
 // ---------------------------------------------------------------- reductions
 // Deterministic wavefront sum (fixed butterfly order, 64 lanes).

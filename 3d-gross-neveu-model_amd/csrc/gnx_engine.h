@@ -1,0 +1,67 @@
+// gnx_engine.h -- host machinery shared by the graph-replayed solvers.
+//
+// A solver is a pair of kernels (X, Y) that together perform one iteration
+// of every active system of a batch, plus start kernels launched before the
+// replay.  The kernels read their arguments from a device copy of a plain
+// struct and keep per-system state in a device array state[2][nrhs]; the
+// first two ints of each state entry are `active` and `iters`.  The engine
+// captures CHUNK (X, Y) pairs into one hipGraph per (kernel pair, batch
+// size), replays it until no system is active, and polls only the `active`
+// flags between replays.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace gnx {
+
+constexpr int ENG_BS = 256;     // threads per block of every solver kernel
+constexpr int ENG_CHUNK = 16;   // iterations per graph replay
+
+struct EngineKernels {
+  const void* X;
+  const void* Y;
+  int blocks;   // blocks per system (grid.x); grid.y = nrhs
+  int tag;      // identifies the pair in the graph cache
+};
+
+class GraphEngine {
+ public:
+  ~GraphEngine();
+  // (re)allocate for nrhs systems; args/state sizes in bytes, partials in doubles
+  hipError_t ensure(int nrhs, size_t args_bytes, size_t state_bytes, size_t part_doubles);
+  void* host_args() { return hargs_; }
+  const void* dev_args() const { return dargs_; }
+  void* dev_state() { return state_; }
+  double* dev_part() { return part_; }
+  // wait for the stream, then copy the staged host arguments to the device
+  hipError_t stage_args(hipStream_t st);
+  hipError_t graph(const EngineKernels& k, int nrhs, hipGraphExec_t* exec);
+  // replay until every system is inactive; returns the largest `iters`
+  hipError_t run(hipGraphExec_t exec, int nrhs, long max_slots, hipStream_t st, int* max_iters);
+  // time `chunks` replays (after the caller staged a start), in ms
+  hipError_t time_replays(hipGraphExec_t exec, int chunks, hipStream_t st, float* ms);
+
+  int last_iters = 0;
+  int tuned_tag[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+  int tuned_nrhs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+ private:
+  void release();
+  struct Entry {
+    int tag = -1;
+    int nrhs = 0;
+    hipGraphExec_t exec = nullptr;
+  };
+  void* dargs_ = nullptr;
+  void* hargs_ = nullptr;
+  void* state_ = nullptr;
+  char* hstate_ = nullptr;
+  double* part_ = nullptr;
+  size_t args_bytes_ = 0, state_bytes_ = 0, part_doubles_ = 0;
+  int cap_ = 0;
+  hipStream_t capture_ = nullptr;
+  Entry graphs_[12];
+  int next_evict_ = 0;
+};
+
+}  // namespace gnx

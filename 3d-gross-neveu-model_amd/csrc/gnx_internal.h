@@ -56,6 +56,10 @@ hipError_t cgm_f64_run(void** machine_slot, const Geom& g, const double* sm, con
                        double tol, long max_iter, int start, CGWork& w, int nrhs, hipStream_t st,
                        CGDevResult* host_res);
 void cgm_release(void* machine);
+hipError_t cgmix_run(void** machine_slot, const Geom& g, const double* sm, const double2* b, double2* x, double tol,
+                     double delta, long max_iter, int start, int prec, int nrhs, hipStream_t st,
+                     CGDevResult* host_res);
+void cgmix_release(void* machine);
 int cgm_shape_of(void* machine, int nrhs, int* spt, int* xcd);
 
 void set_err(const char* fmt, ...);
@@ -76,6 +80,8 @@ struct gnx_ctx {
   gnx::CGWork cg;
   int cg_nrhs_cap = 0;
   void* cgm = nullptr;          // graph-replayed CG state machine (gnx_cgm.hip)
+  void* cgmix = nullptr;        // mixed-precision reliable-update CG (gnx_cgmix.hip)
+  double reliable_delta = 0.1;  // residual drop that triggers an FP64 reliable update
   int cg_engine = 0;            // 0: graph state machine, 1: persistent cooperative kernel
 };
 

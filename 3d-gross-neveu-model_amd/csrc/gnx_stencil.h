@@ -16,16 +16,38 @@
 // the exact additive identity.
 #pragma once
 
+#include <utility>
+
 #include "gnx_common.h"
 
 namespace gnx {
 
-template <typename LoadF>
-GNX_DEV double2 apply_M(const Geom& g, const Site& s, double sm, LoadF load) {
+// Complex-type traits: the operator code is shared by the FP64 drop-in
+// (double2, bit-exact) and the FP32-arithmetic inner solvers (float2).
+template <typename T2>
+struct CT;
+template <>
+struct CT<double2> {
+  using R = double;
+  GNX_DEV static double2 nzero() { return make_double2(-0.0, -0.0); }
+};
+template <>
+struct CT<float2> {
+  using R = float;
+  GNX_DEV static float2 nzero() { return make_float2(-0.0f, -0.0f); }
+};
+GNX_DEV float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+GNX_DEV float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+GNX_DEV float2 rmul(float s, float2 a) { return make_float2(s * a.x, s * a.y); }
+GNX_DEV float2 rmul(double s, float2 a) { return rmul((float)s, a); }
+GNX_DEV float cnorm2(float2 a) { return a.x * a.x + a.y * a.y; }
+
+template <typename LoadF, typename T2 = decltype(std::declval<LoadF>()(0))>
+GNX_DEV T2 apply_M(const Geom& g, const Site& s, typename CT<T2>::R sm, LoadF load) {
   NbrF n = neighbours_f(g, s);
   HalfEta e = half_eta(g, s);
   int f = f_of_lex(g, s.lex, s.P);
-  double2 acc = rmul(sm, load(f));
+  T2 acc = rmul(sm, load(f));
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     acc = cadd(acc, rmul(e.pp[d], load(n.pp[d])));
@@ -35,12 +57,12 @@ GNX_DEV double2 apply_M(const Geom& g, const Site& s, double sm, LoadF load) {
 }
 
 // r - M v, reference order (r -= sm*v, then -= pp, += mm per direction)
-template <typename LoadF>
-GNX_DEV double2 apply_sub_M(const Geom& g, const Site& s, double sm, double2 r, LoadF load) {
+template <typename LoadF, typename T2 = decltype(std::declval<LoadF>()(0))>
+GNX_DEV T2 apply_sub_M(const Geom& g, const Site& s, typename CT<T2>::R sm, T2 r, LoadF load) {
   NbrF n = neighbours_f(g, s);
   HalfEta e = half_eta(g, s);
   int f = f_of_lex(g, s.lex, s.P);
-  double2 acc = csub(r, rmul(sm, load(f)));
+  T2 acc = csub(r, rmul(sm, load(f)));
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     acc = cadd(acc, rmul(-e.pp[d], load(n.pp[d])));
@@ -51,17 +73,17 @@ GNX_DEV double2 apply_sub_M(const Geom& g, const Site& s, double sm, double2 r, 
 
 // Ordered accumulation of the six M^+ (or -M^+) hop terms onto acc.
 // tpp[d]: term from source pp(z,d); tmm[d]: term from source mm(z,d).
-GNX_DEV double2 accumulate_adj_order(const Geom& g, const Site& s, double2 acc,
-                                     const double2 tpp[3], const double2 tmm[3]) {
-  const double2 z = make_double2(-0.0, -0.0);
+template <typename T2>
+GNX_DEV T2 accumulate_adj_order(const Geom& g, const Site& s, T2 acc, const T2 tpp[3], const T2 tmm[3]) {
+  const T2 z = CT<T2>::nzero();
   const int c[3] = {s.x0, s.x1, s.x2};
   const int E[3] = {g.L, g.L, g.LT};
   // below-slots: d = 0, 1, 2
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     bool top = (c[d] == E[d] - 1), bot = (c[d] == 0);
-    double2 s1 = top ? tpp[d] : (bot ? z : tmm[d]);
-    double2 s2 = top ? tmm[d] : z;
+    T2 s1 = top ? tpp[d] : (bot ? z : tmm[d]);
+    T2 s2 = top ? tmm[d] : z;
     acc = cadd(acc, s1);
     acc = cadd(acc, s2);
   }
@@ -70,8 +92,8 @@ GNX_DEV double2 accumulate_adj_order(const Geom& g, const Site& s, double2 acc,
   for (int dd = 0; dd < 3; dd++) {
     int d = 2 - dd;
     bool top = (c[d] == E[d] - 1), bot = (c[d] == 0);
-    double2 s3 = top ? z : tpp[d];
-    double2 s4 = bot ? tmm[d] : z;
+    T2 s3 = top ? z : tpp[d];
+    T2 s4 = bot ? tmm[d] : z;
     acc = cadd(acc, s3);
     acc = cadd(acc, s4);
   }
@@ -92,13 +114,13 @@ GNX_DEV void adj_terms(const Geom& g, const Site& s, double hp[3], double hm[3])
   hm[2] = (s.x2 == g.LT - 1) ? e2 : -e2;    // coefficient of v[pp(z,2)]
 }
 
-template <typename LoadF>
-GNX_DEV double2 apply_Mdag(const Geom& g, const Site& s, double sm, LoadF load) {
+template <typename LoadF, typename T2 = decltype(std::declval<LoadF>()(0))>
+GNX_DEV T2 apply_Mdag(const Geom& g, const Site& s, typename CT<T2>::R sm, LoadF load) {
   NbrF n = neighbours_f(g, s);
   double cm[3], cp[3];
   adj_terms(g, s, cm, cp);
   int f = f_of_lex(g, s.lex, s.P);
-  double2 tpp[3], tmm[3];
+  T2 tpp[3], tmm[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     tmm[d] = rmul(cm[d], load(n.mm[d]));
@@ -109,13 +131,13 @@ GNX_DEV double2 apply_Mdag(const Geom& g, const Site& s, double sm, LoadF load) 
 
 // r - M^+ v (dirac.c:253-287): r -= sm*v, then the scatter terms negated, in
 // the same ascending-source order.
-template <typename LoadF>
-GNX_DEV double2 apply_sub_Mdag(const Geom& g, const Site& s, double sm, double2 r, LoadF load) {
+template <typename LoadF, typename T2 = decltype(std::declval<LoadF>()(0))>
+GNX_DEV T2 apply_sub_Mdag(const Geom& g, const Site& s, typename CT<T2>::R sm, T2 r, LoadF load) {
   NbrF n = neighbours_f(g, s);
   double cm[3], cp[3];
   adj_terms(g, s, cm, cp);
   int f = f_of_lex(g, s.lex, s.P);
-  double2 tpp[3], tmm[3];
+  T2 tpp[3], tmm[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     tmm[d] = rmul(-cm[d], load(n.mm[d]));

@@ -67,6 +67,7 @@ def lib():
         L.gnx_cg.argtypes = [vp, vp, vp, vp, C.c_double, C.c_long, C.c_int, C.c_int, C.POINTER(CGResult)]
         L.gnx_cg_batch.argtypes = [vp, vp, vp, vp, C.c_int, C.c_double, C.c_long, C.c_int, C.c_int,
                                    C.POINTER(CGResult)]
+        L.gnx_set_reliable_delta.argtypes = [vp, C.c_double]
         L.gnx_cg_shape.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.gnx_sigma_force.argtypes = [vp, vp, vp, C.c_double, C.c_int, C.c_double, C.c_long,
                                       vp * 2, vp * 2, vp * 2, vp, CGResult * 2]
@@ -187,6 +188,9 @@ class Context:
         if st not in (0, 3):
             _check(st)
         return st, list(res)
+
+    def set_reliable_delta(self, delta):
+        _check(lib().gnx_set_reliable_delta(self.h, delta))
 
     def cg_shape(self, nrhs=1):
         spt, xcd = C.c_int(), C.c_int()

@@ -182,6 +182,41 @@ def main():
                             "algorithmic_GB_s": BYTES_CG_ITER_F64 * V * sum(its) / (dt / reps) / 1e9}
         del B, X
     result["batched_f64"] = batched
+
+    # ---------------- mixed precision (FP64 reliable updates) ----------------
+    mixed = {}
+    for pname, pcode in (("f32", gnx.PREC_F32), ("bf16", gnx.PREC_BF16)):
+        for nb in [1] + [int(v) for v in args.batch.split(",") if v]:
+            rng = np.random.default_rng(11 + nb)
+            B = b if nb == 1 else torch.from_numpy(np.where(rng.random(2 * V * nb) <= 0.5, 1.0, -1.0)).to(dev)
+            X = ctx.vec(nb)
+            ctx.cg_batch(sm, B, X, nb, args.tol, start=gnx.START_IMPOSED, prec=pcode)
+            torch.cuda.synchronize()
+            reps = max(2, 8 // nb)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                st, res = ctx.cg_batch(sm, B, X, nb, args.tol, start=gnx.START_IMPOSED, prec=pcode)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            # error against the FP64 GPU solution of the same system (first RHS)
+            xr = ctx.vec()
+            ctx.cg(sm, B[:2 * V], xr, args.tol, start=gnx.START_IMPOSED)
+            tr = ctx.vec()
+            ctx.dirac(gnx.OP_MMDAG, sm, X[:2 * V], tr)
+            torch.cuda.synchronize()
+            mixed[f"{pname}_x{nb}"] = {
+                "rhs_solves_per_s": nb * reps / dt, "ms_per_batch": dt / reps * 1e3,
+                "iterations_max": max(r.iterations for r in res),
+                "reliable_updates_max": max(r.reliable_updates for r in res),
+                "all_converged": all(r.converged for r in res),
+                "true_residual_rhs0": float(torch.linalg.norm(B[:2 * V] - tr).item()),
+                "rel_err_vs_f64_rhs0": float((torch.linalg.norm(X[:2 * V] - xr) / torch.linalg.norm(xr)).item()),
+                "speedup_vs_f64_single": nb * reps / dt / solves_per_s}
+            del X
+    result["mixed"] = mixed
+    best = max(mixed.values(), key=lambda d: d["rhs_solves_per_s"])
+    result["best_mixed_batched_rhs_solves_per_s"] = best["rhs_solves_per_s"]
+    result["best_mixed_batched_speedup_vs_f64"] = best["speedup_vs_f64_single"]
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(L, sigma_h, b_h, it)
     print(json.dumps(result), flush=True)

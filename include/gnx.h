@@ -111,6 +111,10 @@ gnx_status gnx_cg_batch(gnx_ctx *ctx, const double *sm, const double *b, double 
                         double tol, long max_iter, gnx_cg_start start, gnx_prec prec,
                         gnx_cg_result *res);
 
+/* Mixed precision: residual reduction factor since the last FP64 reliable
+ * update that triggers the next one (default 0.1). */
+gnx_status gnx_set_reliable_delta(gnx_ctx *ctx, double delta);
+
 /* Launch shape the CG autotuner picked for batches of nrhs systems on this
  * context (sites per thread, XCD-slab block mapping); GNX_ERR_STATE if no
  * solve of that batch size has run yet.  GNX_CG_SHAPE=<n> in the environment

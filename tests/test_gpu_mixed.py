@@ -1,0 +1,77 @@
+"""Mixed-precision CG (FP32 and BF16-storage inner iterations, FP64 reliable
+updates) against the FP64 oracle.
+
+Bars (written here): the solver only stops on the FP64 true residual, so
+  - the true residual ||b - M M^dag x|| recomputed by the oracle is < tol;
+  - ||x - x_ref|| <= 1e-6 ||x_ref|| against the oracle's FP64 solution at the
+    same absolute tolerance 1e-10 (both within tol/lambda_min of the exact
+    solution);
+  - batched solves equal single solves bit-for-bit.
+"""
+import numpy as np
+import pytest
+
+from gpu_util import banner, dev, host, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def true_res(lat, sm, b, x):
+    return np.linalg.norm(b - lat.M(sm, lat.Mdag(sm, x)))
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16"])
+@pytest.mark.parametrize("L", [8, 16, 24])
+def test_mixed_vs_oracle(P, gnx_mod, prec, L):
+    banner(f"mixed {prec} {L}^3")
+    import torch
+    lat = P.Lattice(L)
+    r = P.Random(300 + L)
+    sigma = r.sigma(lat.V, P.sigma_variance(0.85))
+    sm = lat.sigma_mass(sigma)
+    b = r.z2_complex(lat.V)
+    xr, ok, st = lat.cg(sm, b, None, tol=1e-10, start="imposed")
+    ctx = gnx_mod.Context(L)
+    x = ctx.vec()
+    p = gnx_mod.PREC_F32 if prec == "f32" else gnx_mod.PREC_BF16
+    s, res = ctx.cg(dev(sm), dev(b), x, 1e-10, start=gnx_mod.START_IMPOSED, prec=p)
+    torch.cuda.synchronize()
+    assert s == 0 and res.converged and res.reliable_updates >= 1
+    xg = host(x)
+    assert true_res(lat, sm, b, xg) < 1e-10
+    assert rel(xg, xr) < 1e-6
+    banner(f"  iterations fp64={st.last_iters} {prec}={res.iterations} updates={res.reliable_updates}")
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16"])
+def test_mixed_batch_and_null_start(P, gnx_mod, prec):
+    banner(f"mixed {prec} batch of 3 and null-start early exit")
+    import torch
+    L = 12
+    lat = P.Lattice(L)
+    r = P.Random(41)
+    sigma = r.sigma(lat.V, P.sigma_variance(0.85))
+    sm = dev(lat.sigma_mass(sigma))
+    bs = [r.z2_complex(lat.V) for _ in range(3)]
+    ctx = gnx_mod.Context(L)
+    p = gnx_mod.PREC_F32 if prec == "f32" else gnx_mod.PREC_BF16
+    X = ctx.vec(3)
+    st, res = ctx.cg_batch(sm, dev(np.concatenate(bs)), X, 3, 1e-10, start=gnx_mod.START_IMPOSED, prec=p)
+    assert st == 0
+    for k in range(3):
+        x = ctx.vec()
+        s1, r1 = ctx.cg(sm, dev(bs[k]), x, 1e-10, start=gnx_mod.START_IMPOSED, prec=p)
+        assert r1.iterations == res[k].iterations
+        assert np.array_equal(host(x), host(X[k * 2 * lat.V:(k + 1) * 2 * lat.V]))
+    # null start from the converged solution: the prologue exits, x untouched
+    x = X[:2 * lat.V].clone()
+    before = host(x).copy()
+    s2, r2 = ctx.cg(sm, dev(bs[0]), x, 1e-10, start=gnx_mod.START_NULL, prec=p)
+    torch.cuda.synchronize()
+    assert s2 == 0 and r2.early_exit == 1 and np.array_equal(host(x), before)
+    # null start from a non-converged x: restarts from zero and converges
+    x = ctx.vec()
+    x[:] = 1.0
+    s3, r3 = ctx.cg(sm, dev(bs[1]), x, 1e-10, start=gnx_mod.START_NULL, prec=p)
+    assert s3 == 0 and r3.converged and r3.early_exit == 0
+    assert np.array_equal(host(x), host(X[2 * lat.V:4 * lat.V]))
