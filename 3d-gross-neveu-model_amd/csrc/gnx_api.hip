@@ -50,31 +50,9 @@ __global__ void k_geometry_tables(Geom g, int* fpp, int* fmm, double* eta_pp, do
   if (f2lex) f2lex[f] = s.lex;
 }
 
-gnx_status gnx::ensure_cg(gnx_ctx* c, int nrhs) {
-  if (nrhs <= c->cg_nrhs_cap) return GNX_OK;
-  CGWork& w = c->cg;
-  (void)hipFree(w.r); (void)hipFree(w.p0); (void)hipFree(w.p1); (void)hipFree(w.t);
-  (void)hipFree(w.part); (void)hipFree(w.res);
-  w = CGWork();
-  c->cg_nrhs_cap = 0;
-  size_t vb = sizeof(double2) * (size_t)c->V * nrhs;
-  HIP_TRY(hipMalloc(&w.r, vb));
-  HIP_TRY(hipMalloc(&w.p0, vb));
-  HIP_TRY(hipMalloc(&w.p1, vb));
-  HIP_TRY(hipMalloc(&w.t, vb));
-  w.max_blocks = 4096;
-  HIP_TRY(hipMalloc(&w.part, sizeof(double) * 4 * (size_t)nrhs * w.max_blocks));
-  HIP_TRY(hipMalloc(&w.res, sizeof(CGDevResult) * nrhs));
-  w.vec_elems = (long)c->V * nrhs;
-  c->cg_nrhs_cap = nrhs;
-  return GNX_OK;
-}
-
 hipError_t gnx::cg_run(gnx_ctx* c, const double* sm, const double2* b, double2* x, double tol, long max_iter,
                       int start, int nrhs, CGDevResult* host_res) {
-  if (c->cg_engine == 1)
-    return cg_f64_run(c->g, sm, b, x, tol, max_iter, start, c->cg, nrhs, c->stream, c->device, host_res);
-  return cgm_f64_run(&c->cgm, c->g, sm, b, x, tol, max_iter, start, c->cg, nrhs, c->stream, host_res);
+  return cgm_f64_run(&c->cgm, c->g, sm, b, x, tol, max_iter, start, nrhs, c->stream, host_res);
 }
 
 extern "C" {
@@ -103,10 +81,7 @@ gnx_status gnx_create(int lspace, int ltime, int device, gnx_ctx** out) {
   c->V = V;
   c->g = make_geom(lspace, ltime);
   c->red_nparts = 1024;
-  {
-    const char* eng = getenv("GNX_CG_ENGINE");
-    c->cg_engine = (eng && strcmp(eng, "persistent") == 0) ? 1 : 0;
-  }
+
   hipError_t e;
   if ((e = hipMalloc(&c->tmp, sizeof(double) * 2 * V)) != hipSuccess ||
       (e = hipMalloc(&c->red_part, sizeof(double) * c->red_nparts)) != hipSuccess ||
@@ -115,11 +90,6 @@ gnx_status gnx_create(int lspace, int ltime, int device, gnx_ctx** out) {
     set_err("gnx_create: allocation failed: %s", hipGetErrorString(e));
     gnx_destroy(c);
     return GNX_ERR_HIP;
-  }
-  gnx_status s = ensure_cg(c, 1);
-  if (s != GNX_OK) {
-    gnx_destroy(c);
-    return s;
   }
   *out = c;
   return GNX_OK;
@@ -132,8 +102,6 @@ gnx_status gnx_destroy(gnx_ctx* c) {
   (void)hipFree(c->red_part);
   (void)hipFree(c->red_out);
   if (c->host_out) (void)hipHostFree(c->host_out);
-  (void)hipFree(c->cg.r); (void)hipFree(c->cg.p0); (void)hipFree(c->cg.p1); (void)hipFree(c->cg.t);
-  (void)hipFree(c->cg.part); (void)hipFree(c->cg.res);
   if (c->cgm) cgm_release(c->cgm);
   if (c->cgmix) cgmix_release(c->cgmix);
   delete c;
@@ -214,8 +182,6 @@ gnx_status gnx_cg_batch(gnx_ctx* c, const double* sm, const double* b, double* x
     return GNX_OK;
   }
   CHECK_ARG(prec == GNX_PREC_F64, "gnx_cg: bad precision");
-  gnx_status s = ensure_cg(c, nrhs);
-  if (s != GNX_OK) return s;
   CGDevResult hr[64];
   HIP_TRY(cg_run(c, sm, (const double2*)b, (double2*)x, tol, max_iter, (int)start, nrhs, hr));
   int all = 1;

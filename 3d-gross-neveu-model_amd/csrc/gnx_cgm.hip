@@ -5,7 +5,7 @@
 // Why kernels and not a persistent kernel: on gfx950 a grid-wide barrier
 // inside a kernel costs 10-80 us (cooperative grid sync or an atomic counter,
 // growing with the block count), a kernel node inside a hipGraph about
-// 1.7 us.  Standard CG needs two global reductions per iteration, so each
+// 1.6 us.  Standard CG needs two global reductions per iteration, so each
 // iteration is two kernels whose boundaries are the barriers:
 //
 //   A(k): rho_k from the partials of the previous B (or of the start kernel);
@@ -16,19 +16,27 @@
 //   B(k): gamma from A's partials; alpha = rho_k / gamma; x += alpha p_new;
 //         r += (-alpha) M t; partial |r|^2.
 //
-// Every block reduces the previous kernel's per-block partials in the same
-// fixed order, so all blocks hold bit-identical scalars.  Per-system state
-// (rho, active, iteration count) is double-buffered: kernel j reads
-// state[j & 1] and its block 0 writes state[(j+1) & 1]; no block reads what
-// another block of the same kernel writes.  Systems of a batch (blockIdx.y)
-// are independent: each keeps its alpha, beta and exit iteration, and a
-// finished system's blocks return at once.  The host replays a fixed graph of
-// CHUNK iterations until every system is inactive.
+// The per-site arithmetic is the reference's (x += alpha p, r -= alpha M t,
+// p = r + beta p, M and M^dag in reference order); only the global sums are
+// taken in a fixed tree order.  Every block reduces the previous kernel's
+// per-block partials in the same order, so all blocks hold bit-identical
+// scalars.  Per-system state is double-buffered: A reads state[0] and its
+// block 0 writes state[1]; B the reverse; no block reads what another block
+// of the same kernel writes.  Systems of a batch (blockIdx.y) are
+// independent: each keeps its alpha, beta and exit iteration.
+//
+// Latency: each kernel issues every load it needs (state, partials, the
+// stencil operands) before its first reduction, so one memory round trip is
+// on the critical path.  Kernel arguments are passed by value and point at
+// buffers this solver owns (the caller's sigma mass and x are copied in and
+// the solution copied out), and the parity of p_old/p_new is baked into each
+// graph node (one slot pair = one iteration, a chunk holds an even count).
 #include <stdlib.h>
 #include <string.h>
 
 #include "gnx_engine.h"
 #include "gnx_internal.h"
+#include "gnx_solver.h"
 
 namespace gnx {
 
@@ -41,113 +49,113 @@ struct CGMState {
   int converged;
   int early;
   double rho;
+  double tol;
+  long max_iter;
+  int start;
+  int pad;
 };
 
-struct CGMArgs {
+// field-wise state write (an aggregate copy makes the compiler stage the
+// struct through LDS)
+GNX_DEVONLY void put_state(CGMState* o, int active, int iters, int conv, int early, double rho,
+                           const CGMState& cfg) {
+  o->active = active;
+  o->iters = iters;
+  o->converged = conv;
+  o->early = early;
+  o->rho = rho;
+  o->tol = cfg.tol;
+  o->max_iter = cfg.max_iter;
+  o->start = cfg.start;
+  o->pad = 0;
+}
+
+// kernel arguments: fixed per solver instance, passed by value
+struct CGMFix {
   Geom g;
-  const double* sm;
-  const double2* b;
-  double2* x;
+  double* sm;        // copy of the caller's sigma mass
+  double2* x;        // solutions [nrhs][V]
   double2* r;
   double2* p0;
   double2* p1;
   double2* t;
-  double* part;        // [2 slots][nrhs][nb]
-  CGMState* state;     // [2][nrhs]
-  CGDevResult* res;    // [nrhs]
-  double tol;
-  long max_iter;
+  double* part;      // [2 slots][nrhs][nb]
+  CGMState* state;   // [2][nrhs]
+  CGDevResult* res;  // [nrhs]
   int nrhs;
-  int nb;              // blocks of the iteration kernels (per system)
-  int nb0;             // blocks of the start kernels (one site per thread)
-  int nbs;             // stride of a partial slot (>= nb, nb0)
-  int start;           // GNX_START_NULL / GNX_START_IMPOSED
+  int nb;            // blocks per system of every kernel of this shape
 };
 
-// Launch shapes: SPT sites per thread; XCD = remap blocks so that each XCD
-// (workgroups are dispatched round-robin over the 8 XCDs) sweeps one
-// contiguous lex slab of both parities, keeping a slab's halo in its own L2.
-constexpr int NUM_XCD = 8;
-
-GNX_DEVONLY int xcd_block(int b, int nb) {
-  int q = nb / NUM_XCD, rem = nb % NUM_XCD, x = b % NUM_XCD;
-  return x * q + (x < rem ? x : rem) + b / NUM_XCD;
+GNX_DEV double* slot(const CGMFix& a, int s, int rhs) {
+  return a.part + ((size_t)s * a.nrhs + rhs) * a.nb;
 }
 
+// ---- start kernels (launched per solve, outside the graph) ------------------
+// imposed start (conjugategradient.c:121-128): x = 0, r = p0 = b, partial |b|^2
 template <int SPT, bool XCD>
-struct SiteMap {
-  int base;
-  GNX_DEVONLY SiteMap(int nb) { base = (XCD ? xcd_block(blockIdx.x, nb) : (int)blockIdx.x) * SPT * CGM_BS; }
-  // j-th logical site of this thread, or -1
-  GNX_DEVONLY int f(const Geom& g, int i) const {
-    int j = base + i * CGM_BS + (int)threadIdx.x;
-    if (j >= g.V) return -1;
-    return XCD ? (j >> 1) + (j & 1) * g.half : j;
-  }
-};
-
-GNX_DEVONLY double* slot(const CGMArgs& a, int s, int rhs) {
-  return a.part + ((size_t)s * a.nrhs + rhs) * a.nbs;
-}
-
-// sum of the first n partials of a slot, fixed order, identical in every block
-GNX_DEVONLY double reduce_slot(const CGMArgs& a, int s, int rhs, int n, double* lds) {
-  const double* src = slot(a, s, rhs);
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < n; i += CGM_BS) acc += src[i];
-  return block_sum(acc, lds);
-}
-
-// ---- start kernels ---------------------------------------------------------
-// imposed start (conjugategradient.c:121-128): x = 0, r = p = b, partial |b|^2
-__global__ __launch_bounds__(CGM_BS) void k_cgm_start_imposed(const CGMArgs* __restrict__ ap) {
-  const CGMArgs a = *ap;
+__global__ __launch_bounds__(CGM_BS) void k_cgm_start_imposed(CGMFix a, const double* __restrict__ sm_in,
+                                                               const double2* __restrict__ b, double tol,
+                                                               long max_iter) {
   __shared__ double lds[CGM_BS / 64];
   const int rhs = blockIdx.y;
   const size_t off = (size_t)rhs * a.g.V;
-  const int f = blockIdx.x * CGM_BS + threadIdx.x;
+  const SiteMap<SPT, XCD> map(a.nb);
   double acc = 0.0;
-  if (f < a.g.V) {
-    double2 bb = a.b[off + f];
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    const int f = map.f(a.g, i);
+    if (f < 0) break;
+    double2 bb = b[off + f];
     a.x[off + f] = make_double2(0.0, 0.0);
     a.r[off + f] = bb;
     a.p0[off + f] = bb;
-    acc = cnorm2(bb);
+    if (rhs == 0) a.sm[f] = sm_in[f];
+    acc += cnorm2(bb);
   }
   acc = block_sum(acc, lds);
   if (threadIdx.x == 0) slot(a, 1, rhs)[blockIdx.x] = acc;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    CGMState s = {1, 0, 0, 0, 0.0};
+    CGMState s = {1, 0, 0, 0, 0.0, tol, max_iter, GNX_START_IMPOSED, 0};
     a.state[rhs] = s;
   }
 }
 
-// null start, pass 1 (conjugategradient.c:20): t = M^dag x_in
-__global__ __launch_bounds__(CGM_BS) void k_cgm_null_p1(const CGMArgs* __restrict__ ap) {
-  const CGMArgs a = *ap;
+// null start, pass 1 (conjugategradient.c:20): t = M^dag x_in; x = x_in; sm copied
+template <int SPT, bool XCD>
+__global__ __launch_bounds__(CGM_BS) void k_cgm_null_p1(CGMFix a, const double* __restrict__ sm_in,
+                                                         const double2* __restrict__ xin) {
   const size_t off = (size_t)blockIdx.y * a.g.V;
-  const int f = blockIdx.x * CGM_BS + threadIdx.x;
-  if (f >= a.g.V) return;
-  const double2* xin = a.x + off;
-  a.t[off + f] = apply_Mdag(a.g, decode(a.g, f), a.sm[f], [&](int q) { return xin[q]; });
+  const SiteMap<SPT, XCD> map(a.nb);
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    const int f = map.f(a.g, i);
+    if (f < 0) break;
+    const double2* xv = xin + off;
+    a.t[off + f] = apply_Mdag(a.g, decode(a.g, f), sm_in[f], [&](int q) { return xv[q]; });
+    a.x[off + f] = xv[f];
+    if (blockIdx.y == 0) a.sm[f] = sm_in[f];
+  }
 }
 
 // null start, pass 2 (conjugategradient.c:19-21): r = b - M t; partials |r|^2
 // (slot 0) and |b|^2 (slot 1)
-__global__ __launch_bounds__(CGM_BS) void k_cgm_null_p2(const CGMArgs* __restrict__ ap) {
-  const CGMArgs a = *ap;
+template <int SPT, bool XCD>
+__global__ __launch_bounds__(CGM_BS) void k_cgm_null_p2(CGMFix a, const double2* __restrict__ b) {
   __shared__ double lds[CGM_BS / 64];
   const int rhs = blockIdx.y;
   const size_t off = (size_t)rhs * a.g.V;
-  const int f = blockIdx.x * CGM_BS + threadIdx.x;
+  const SiteMap<SPT, XCD> map(a.nb);
   double ar = 0.0, ab = 0.0;
-  if (f < a.g.V) {
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    const int f = map.f(a.g, i);
+    if (f < 0) break;
     const double2* t = a.t + off;
-    double2 bb = a.b[off + f];
+    double2 bb = b[off + f];
     double2 rr = apply_sub_M(a.g, decode(a.g, f), a.sm[f], bb, [&](int q) { return t[q]; });
     a.r[off + f] = rr;
-    ar = cnorm2(rr);
-    ab = cnorm2(bb);
+    ar += cnorm2(rr);
+    ab += cnorm2(bb);
   }
   ar = block_sum(ar, lds);
   ab = block_sum(ab, lds);
@@ -157,17 +165,18 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_null_p2(const CGMArgs* __restric
   }
 }
 
-// null start, pass 3 (conjugategradient.c:22-33): early exit keeps x_in;
-// otherwise x = 0, r = p = b.  The |b|^2 partials stay in slot 1 for A(0).
-__global__ __launch_bounds__(CGM_BS) void k_cgm_null_p3(const CGMArgs* __restrict__ ap) {
-  const CGMArgs a = *ap;
+// null start, pass 3 (conjugategradient.c:22-33): an early exit keeps x_in;
+// otherwise x = 0, r = p0 = b.  The |b|^2 partials stay in slot 1 for A(0).
+template <int SPT, bool XCD>
+__global__ __launch_bounds__(CGM_BS) void k_cgm_null_p3(CGMFix a, const double2* __restrict__ b, double tol,
+                                                         long max_iter) {
   __shared__ double lds[CGM_BS / 64];
   const int rhs = blockIdx.y;
   const size_t off = (size_t)rhs * a.g.V;
-  const double nr = reduce_slot(a, 0, rhs, a.nb0, lds);
-  const bool early = sqrt(nr) < a.tol;
+  const double nr = block_sum(partial_share(slot(a, 0, rhs), a.nb), lds);
+  const bool early = sqrt(nr) < tol;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    CGMState s = {early ? 0 : 1, 0, early ? 1 : 0, early ? 1 : 0, 0.0};
+    CGMState s = {early ? 0 : 1, 0, early ? 1 : 0, early ? 1 : 0, 0.0, tol, max_iter, GNX_START_NULL, 0};
     a.state[rhs] = s;
     if (early) {
       CGDevResult r = {1, 1, 0, 0, sqrt(nr), {0, 0, 0}};
@@ -175,165 +184,204 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_null_p3(const CGMArgs* __restric
     }
   }
   if (early) return;
-  const int f = blockIdx.x * CGM_BS + threadIdx.x;
-  if (f < a.g.V) {
-    double2 bb = a.b[off + f];
+  const SiteMap<SPT, XCD> map(a.nb);
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    const int f = map.f(a.g, i);
+    if (f < 0) break;
+    double2 bb = b[off + f];
     a.x[off + f] = make_double2(0.0, 0.0);
     a.r[off + f] = bb;
     a.p0[off + f] = bb;
   }
 }
 
-// ---- iteration kernels -----------------------------------------------------
+// ---- iteration kernels -------------------------------------------------------
+// odd = iteration parity: p_old = odd ? p1 : p0, p_new the other one
+// sin/sout: this kernel's input and output state halves; pin/pout: the slot
+// it reduces and the slot it writes (separate restrict pointers, so the
+// compiler may keep the reads ahead of the writes)
 template <int SPT, bool XCD>
-__global__ __launch_bounds__(CGM_BS) void k_cgm_A(const CGMArgs* __restrict__ ap) {
-  const CGMArgs& a = *ap;
+__global__ __launch_bounds__(CGM_BS) void k_cgm_A(CGMFix a, int odd, const CGMState* __restrict__ sin,
+                                                   CGMState* __restrict__ sout, const double* __restrict__ pin,
+                                                   double* __restrict__ pout) {
   __shared__ double lds[CGM_BS / 64];
+  const Geom g = a.g;
   const int rhs = blockIdx.y;
-  const CGMState s = a.state[rhs];          // state[0]: written by the previous B / start
-  CGMState* out = a.state + a.nrhs + rhs;   // state[1]
+  const size_t off = (size_t)rhs * g.V;
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const double2* r = a.r + off;
+  const double2* pold = (odd ? a.p1 : a.p0) + off;
+  double2* pnew = (odd ? a.p0 : a.p1) + off;
+  const SiteMap<SPT, XCD> map(a.nb);
+  // issue every load before the reduction
+  const CGMState s = sin[rhs];
+  const double share = partial_share(pin + (size_t)rhs * a.nb, a.nb);
+  int fs[SPT];
+  double smv[SPT];
+  double2 rv[SPT][7], pv[SPT][7];
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    fs[i] = map.f(g, i);
+    if (fs[i] < 0) continue;
+    NbrF n = neighbours_f(g, decode(g, fs[i]));
+    int q[7] = {fs[i], n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
+    smv[i] = a.sm[fs[i]];
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+      rv[i][k] = r[q[k]];
+      pv[i][k] = pold[q[k]];
+    }
+  }
+  const double rho = block_sum(share, lds);
+  CGMState* out = sout + rhs;
   if (!s.active) {
-    if (lead) *out = s;
+    if (lead) put_state(out, 0, s.iters, s.converged, s.early, s.rho, s);
     return;
   }
-  const Geom g = a.g;
-  // slot 1 holds the start kernel's partials before the first iteration
-  const double rho = reduce_slot(a, 1, rhs, s.iters == 0 ? a.nb0 : a.nb, lds);
   double beta = 0.0;
   if (s.iters == 0) {
     // ||b|| < tol returns before the loop only in the imposed start
-    // (conjugategradient.c:123-128); the null start already tested its
-    // prologue residual instead (:22)
-    if (a.start == GNX_START_IMPOSED && sqrt(rho) < a.tol) {
+    // (conjugategradient.c:123-128); the null start tested its prologue
+    // residual instead (:22)
+    if (s.start == GNX_START_IMPOSED && sqrt(rho) < s.tol) {
       if (lead) {
-        CGMState n = {0, 0, 1, 1, rho};
-        *out = n;
-        CGDevResult r = {1, 1, 0, 0, sqrt(rho), {0, 0, 0}};
-        a.res[rhs] = r;
+        put_state(out, 0, 0, 1, 1, rho, s);
+        CGDevResult res = {1, 1, 0, 0, sqrt(rho), {0, 0, 0}};
+        a.res[rhs] = res;
       }
       return;
     }
   } else {
-    const bool conv = sqrt(rho) < a.tol;
-    if (conv || s.iters >= a.max_iter) {
+    const bool conv = sqrt(rho) < s.tol;
+    if (conv || s.iters >= s.max_iter) {
       if (lead) {
-        CGMState n = {0, s.iters, conv ? 1 : 0, 0, rho};
-        *out = n;
-        CGDevResult r = {conv ? 1 : 0, 0, s.iters, 0, sqrt(rho), {0, 0, 0}};
-        a.res[rhs] = r;
+        put_state(out, 0, s.iters, conv ? 1 : 0, 0, rho, s);
+        CGDevResult res = {conv ? 1 : 0, 0, s.iters, 0, sqrt(rho), {0, 0, 0}};
+        a.res[rhs] = res;
       }
       return;
     }
     beta = rho / s.rho;
   }
-  const size_t off = (size_t)rhs * g.V;
-  const double2* r = a.r + off;
-  const double2* pold = ((s.iters & 1) ? a.p1 : a.p0) + off;
-  double2* pnew = ((s.iters & 1) ? a.p0 : a.p1) + off;
-  double2* t = a.t + off;
-  const SiteMap<SPT, XCD> map(a.nb);
   double acc = 0.0;
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
-    const int f = map.f(g, i);
-    if (f < 0) break;
-    auto ldp = [&](int q) { return cadd(r[q], rmul(beta, pold[q])); };
-    pnew[f] = ldp(f);
-    double2 tt = apply_Mdag(g, decode(g, f), a.sm[f], ldp);
-    t[f] = tt;
+    if (fs[i] < 0) continue;
+    const Site st = decode(g, fs[i]);
+    // p_new at the site and its neighbours, from the preloaded r and p_old
+    const double2 p0v = cadd(rv[i][0], rmul(beta, pv[i][0]));
+    double2 ppv[3], pmv[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      ppv[d] = cadd(rv[i][1 + d], rmul(beta, pv[i][1 + d]));
+      pmv[d] = cadd(rv[i][4 + d], rmul(beta, pv[i][4 + d]));
+    }
+    pnew[fs[i]] = p0v;
+    double2 tt = apply_Mdag_vals(g, st, smv[i], p0v, ppv, pmv);
+    a.t[off + fs[i]] = tt;
     acc += cnorm2(tt);
   }
   acc = block_sum(acc, lds);
-  if (threadIdx.x == 0) slot(a, 0, rhs)[blockIdx.x] = acc;
-  if (lead) {
-    CGMState n = {1, s.iters, 0, 0, rho};
-    *out = n;
-  }
+  if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + blockIdx.x] = acc;
+  if (lead) put_state(out, 1, s.iters, 0, 0, rho, s);
 }
 
 template <int SPT, bool XCD>
-__global__ __launch_bounds__(CGM_BS) void k_cgm_B(const CGMArgs* __restrict__ ap) {
-  const CGMArgs& a = *ap;
+__global__ __launch_bounds__(CGM_BS) void k_cgm_B(CGMFix a, int odd, const CGMState* __restrict__ sin,
+                                                   CGMState* __restrict__ sout, const double* __restrict__ pin,
+                                                   double* __restrict__ pout) {
   __shared__ double lds[CGM_BS / 64];
-  const int rhs = blockIdx.y;
-  const CGMState s = a.state[a.nrhs + rhs];  // state[1]
-  CGMState* out = a.state + rhs;             // state[0]
-  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-  if (!s.active) {
-    if (lead) *out = s;
-    return;
-  }
   const Geom g = a.g;
+  const int rhs = blockIdx.y;
   const size_t off = (size_t)rhs * g.V;
-  const double2* pnew = ((s.iters & 1) ? a.p0 : a.p1) + off;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const double2* pnew = (odd ? a.p0 : a.p1) + off;
   const double2* t = a.t + off;
   double2* r = a.r + off;
   double2* x = a.x + off;
   const SiteMap<SPT, XCD> map(a.nb);
-  // the stencil does not depend on alpha: issue it before the reduction
-  double2 w[SPT];
+  const CGMState s = sin[rhs];
+  const double share = partial_share(pin + (size_t)rhs * a.nb, a.nb);
+  int fs[SPT];
+  double2 w[SPT], pv[SPT], xv[SPT], rv[SPT];
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
-    const int f = map.f(g, i);
-    if (f >= 0) w[i] = apply_M(g, decode(g, f), a.sm[f], [&](int q) { return t[q]; });
+    fs[i] = map.f(g, i);
+    if (fs[i] < 0) continue;
+    w[i] = apply_M(g, decode(g, fs[i]), a.sm[fs[i]], [&](int q) { return t[q]; });
+    pv[i] = pnew[fs[i]];
+    xv[i] = x[fs[i]];
+    rv[i] = r[fs[i]];
   }
-  const double gamma = reduce_slot(a, 0, rhs, a.nb, lds);
+  const double gamma = block_sum(share, lds);
+  CGMState* out = sout + rhs;
+  if (!s.active) {
+    if (lead) put_state(out, 0, s.iters, s.converged, s.early, s.rho, s);
+    return;
+  }
   const double al = s.rho / gamma;
   double acc = 0.0;
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
-    const int f = map.f(g, i);
-    if (f < 0) break;
-    x[f] = cadd(x[f], rmul(al, pnew[f]));
-    double2 rr = cadd(r[f], rmul(-al, w[i]));
-    r[f] = rr;
+    if (fs[i] < 0) continue;
+    x[fs[i]] = cadd(xv[i], rmul(al, pv[i]));
+    double2 rr = cadd(rv[i], rmul(-al, w[i]));
+    r[fs[i]] = rr;
     acc += cnorm2(rr);
   }
   acc = block_sum(acc, lds);
-  if (threadIdx.x == 0) slot(a, 1, rhs)[blockIdx.x] = acc;
-  if (lead) {
-    CGMState n = {1, s.iters + 1, 0, 0, s.rho};
-    *out = n;
-  }
+  if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + blockIdx.x] = acc;
+  if (lead) put_state(out, 1, s.iters + 1, 0, 0, s.rho, s);
 }
 
-// launch shapes the tuner may pick from
+// ---- shapes the tuner chooses from -----------------------------------------------
 struct CGMShape {
   int spt;
   bool xcd;
   const void* A;
   const void* B;
+  const void* start_imposed;
+  const void* null_p1;
+  const void* null_p2;
+  const void* null_p3;
 };
 
-static const CGMShape kShapes[] = {
-    {1, false, (const void*)k_cgm_A<1, false>, (const void*)k_cgm_B<1, false>},
-    {1, true, (const void*)k_cgm_A<1, true>, (const void*)k_cgm_B<1, true>},
-    {2, false, (const void*)k_cgm_A<2, false>, (const void*)k_cgm_B<2, false>},
-    {2, true, (const void*)k_cgm_A<2, true>, (const void*)k_cgm_B<2, true>},
-    {4, false, (const void*)k_cgm_A<4, false>, (const void*)k_cgm_B<4, false>},
-    {4, true, (const void*)k_cgm_A<4, true>, (const void*)k_cgm_B<4, true>},
-};
+#define GNX_SHAPE(S, X)                                                                               \
+  {S, X, (const void*)k_cgm_A<S, X>, (const void*)k_cgm_B<S, X>, (const void*)k_cgm_start_imposed<S, X>, \
+   (const void*)k_cgm_null_p1<S, X>, (const void*)k_cgm_null_p2<S, X>, (const void*)k_cgm_null_p3<S, X>}
+static const CGMShape kShapes[] = {GNX_SHAPE(1, false), GNX_SHAPE(1, true), GNX_SHAPE(2, false),
+                                   GNX_SHAPE(2, true)};
+#undef GNX_SHAPE
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
-// ---- host side ---------------------------------------------------------------
 static int shape_blocks(const Geom& g, int shape) {
   int per = CGM_BS * kShapes[shape].spt;
   return (g.V + per - 1) / per;
 }
 
-static EngineKernels shape_kernels(const Geom& g, int shape) {
-  EngineKernels k;
-  k.X = kShapes[shape].A;
-  k.Y = kShapes[shape].B;
-  k.blocks = shape_blocks(g, shape);
-  k.tag = shape;
-  return k;
-}
-
+// ---- host side ---------------------------------------------------------------
 struct CGMachine {
   GraphEngine eng;
-  int forced_shape = -1;  // GNX_CG_SHAPE
+  double* sm = nullptr;
+  double2* x = nullptr;
+  double2* r = nullptr;
+  double2* p0 = nullptr;
+  double2* p1 = nullptr;
+  double2* t = nullptr;
+  CGDevResult* res = nullptr;
+  long cap = 0;          // complex elements per vector
+  int forced_shape = -1; // GNX_CG_SHAPE
+  void free_vectors() {
+    void* bufs[] = {sm, x, r, p0, p1, t, res};
+    for (void* p : bufs)
+      if (p) (void)hipFree(p);
+    sm = nullptr;
+    x = r = p0 = p1 = t = nullptr;
+    res = nullptr;
+    cap = 0;
+  }
+  ~CGMachine() { free_vectors(); }
 };
 
 void cgm_release(void* m) { delete (CGMachine*)m; }
@@ -351,49 +399,95 @@ static CGMachine* cgm_get(void** slotp) {
   return (CGMachine*)*slotp;
 }
 
-// stage the arguments and run the start kernels
-static hipError_t cgm_begin(CGMachine* m, const Geom& g, const double* sm, const double2* b, double2* x, double tol,
-                            long max_iter, int start, CGWork& w, int nrhs, int shape, hipStream_t st) {
-  const int nb0 = (g.V + CGM_BS - 1) / CGM_BS;
-  CGMArgs& a = *(CGMArgs*)m->eng.host_args();
-  hipError_t e = hipStreamSynchronize(st);  // the staging buffer may still be in flight
+static hipError_t cgm_ensure(CGMachine* m, const Geom& g, int nrhs) {
+  const int nb_max = (g.V + CGM_BS - 1) / CGM_BS;
+  const int cap_rhs = nrhs < 2 ? 2 : nrhs;
+  hipError_t e = m->eng.ensure(cap_rhs, 1, sizeof(CGMState), 2 * (size_t)cap_rhs * nb_max);
   if (e != hipSuccess) return e;
+  const long need = (long)g.V * cap_rhs;
+  if (m->cap >= need && m->res) return hipSuccess;
+  m->free_vectors();
+  m->eng.invalidate_graphs();  // captured arguments point at the old vectors
+  e = hipMalloc(&m->sm, sizeof(double) * g.V);
+  if (e == hipSuccess) e = hipMalloc(&m->x, sizeof(double2) * need);
+  if (e == hipSuccess) e = hipMalloc(&m->r, sizeof(double2) * need);
+  if (e == hipSuccess) e = hipMalloc(&m->p0, sizeof(double2) * need);
+  if (e == hipSuccess) e = hipMalloc(&m->p1, sizeof(double2) * need);
+  if (e == hipSuccess) e = hipMalloc(&m->t, sizeof(double2) * need);
+  if (e == hipSuccess) e = hipMalloc(&m->res, sizeof(CGDevResult) * cap_rhs);
+  if (e != hipSuccess) {
+    m->free_vectors();
+    return e;
+  }
+  m->cap = need;
+  return hipSuccess;
+}
+
+static CGMFix cgm_fix(CGMachine* m, const Geom& g, int nrhs, int shape) {
+  CGMFix a;
   a.g = g;
-  a.sm = sm;
-  a.b = b;
-  a.x = x;
-  a.r = w.r;
-  a.p0 = w.p0;
-  a.p1 = w.p1;
-  a.t = w.t;
+  a.sm = m->sm;
+  a.x = m->x;
+  a.r = m->r;
+  a.p0 = m->p0;
+  a.p1 = m->p1;
+  a.t = m->t;
   a.part = m->eng.dev_part();
   a.state = (CGMState*)m->eng.dev_state();
-  a.res = w.res;
-  a.tol = tol;
-  a.max_iter = max_iter;
+  a.res = m->res;
   a.nrhs = nrhs;
   a.nb = shape_blocks(g, shape);
-  a.nb0 = nb0;
-  a.nbs = nb0;
-  a.start = start;
-  e = m->eng.stage_args(st);
-  if (e != hipSuccess) return e;
-  const CGMArgs* dp = (const CGMArgs*)m->eng.dev_args();
-  dim3 grid(nb0, nrhs), blk(CGM_BS);
+  return a;
+}
+
+static hipError_t cgm_graph(CGMachine* m, const Geom& g, int nrhs, int shape, hipGraphExec_t* exec) {
+  CGMFix a = cgm_fix(m, g, nrhs, shape);
+  dim3 grid(a.nb, nrhs), blk(CGM_BS);
+  const CGMState* s0 = a.state;
+  const CGMState* s1 = a.state + nrhs;
+  CGMState* w0 = a.state;
+  CGMState* w1 = a.state + nrhs;
+  const double* q0 = slot(a, 0, 0);
+  const double* q1 = slot(a, 1, 0);
+  double* o0 = a.part;
+  double* o1 = a.part + (size_t)nrhs * a.nb;
+  auto body = [&](hipStream_t cs) {
+    for (int i = 0; i < ENG_CHUNK; i++) {
+      int odd = i & 1;
+      // A: state[0] -> state[1], reduces slot 1, writes slot 0; B the reverse
+      void* argsA[] = {(void*)&a, (void*)&odd, (void*)&s0, (void*)&w1, (void*)&q1, (void*)&o0};
+      void* argsB[] = {(void*)&a, (void*)&odd, (void*)&s1, (void*)&w0, (void*)&q0, (void*)&o1};
+      (void)hipLaunchKernel(kShapes[shape].A, grid, blk, argsA, 0, cs);
+      (void)hipLaunchKernel(kShapes[shape].B, grid, blk, argsB, 0, cs);
+    }
+  };
+  return m->eng.graph(shape, nrhs, body, exec);
+}
+
+// run the start kernels of one solve
+static hipError_t cgm_begin(CGMachine* m, const Geom& g, const double* sm, const double2* b, const double2* xin,
+                            double tol, long max_iter, int start, int nrhs, int shape, hipStream_t st) {
+  CGMFix a = cgm_fix(m, g, nrhs, shape);
+  dim3 grid(a.nb, nrhs), blk(CGM_BS);
+  hipError_t e;
   if (start == GNX_START_NULL) {
-    hipLaunchKernelGGL(k_cgm_null_p1, grid, blk, 0, st, dp);
-    hipLaunchKernelGGL(k_cgm_null_p2, grid, blk, 0, st, dp);
-    hipLaunchKernelGGL(k_cgm_null_p3, grid, blk, 0, st, dp);
+    void* a1[] = {(void*)&a, (void*)&sm, (void*)&xin};
+    e = hipLaunchKernel(kShapes[shape].null_p1, grid, blk, a1, 0, st);
+    void* a2[] = {(void*)&a, (void*)&b};
+    if (e == hipSuccess) e = hipLaunchKernel(kShapes[shape].null_p2, grid, blk, a2, 0, st);
+    void* a3[] = {(void*)&a, (void*)&b, (void*)&tol, (void*)&max_iter};
+    if (e == hipSuccess) e = hipLaunchKernel(kShapes[shape].null_p3, grid, blk, a3, 0, st);
   } else {
-    hipLaunchKernelGGL(k_cgm_start_imposed, grid, blk, 0, st, dp);
+    void* a0[] = {(void*)&a, (void*)&sm, (void*)&b, (void*)&tol, (void*)&max_iter};
+    e = hipLaunchKernel(kShapes[shape].start_imposed, grid, blk, a0, 0, st);
   }
-  return hipGetLastError();
+  return e;
 }
 
 // Autotuner: time every launch shape on 2*CHUNK iterations of the caller's
-// own system (imposed start into a scratch solution, so the caller's x is not
+// own system (imposed start into the solver's buffers; the caller's x is not
 // touched) and keep the fastest per batch size.
-static hipError_t cgm_tune(CGMachine* m, const Geom& g, const double* sm, const double2* b, CGWork& w, int nrhs,
+static hipError_t cgm_tune(CGMachine* m, const Geom& g, const double* sm, const double2* b, int nrhs,
                            hipStream_t st, int* best) {
   GraphEngine& E = m->eng;
   for (int i = 0; i < 8; i++)
@@ -406,16 +500,14 @@ static hipError_t cgm_tune(CGMachine* m, const Geom& g, const double* sm, const 
   if (m->forced_shape >= 0) {
     best_shape = m->forced_shape;
   } else {
-    double2* xs = nullptr;
-    e = hipMalloc(&xs, sizeof(double2) * (size_t)g.V * nrhs);
-    if (e != hipSuccess) return e;
     float best_ms = 1e30f;
     for (int sh = 0; sh < kNumShapes && e == hipSuccess; sh++) {
       hipGraphExec_t exec;
-      e = E.graph(shape_kernels(g, sh), nrhs, &exec);
+      e = cgm_graph(m, g, nrhs, sh, &exec);
       float ms_min = 1e30f;
       for (int rep = 0; rep < 3 && e == hipSuccess; rep++) {
-        e = cgm_begin(m, g, sm, b, xs, 0.0, 2 * ENG_CHUNK, GNX_START_IMPOSED, w, nrhs, sh, st);
+        e = cgm_begin(m, g, sm, b, nullptr, 0.0, 2 * ENG_CHUNK, GNX_START_IMPOSED, nrhs, sh, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
         float ms = 0.f;
         if (e == hipSuccess) e = E.time_replays(exec, 2, st, &ms);
         if (rep > 0 && ms < ms_min) ms_min = ms;
@@ -425,8 +517,6 @@ static hipError_t cgm_tune(CGMachine* m, const Geom& g, const double* sm, const 
         best_shape = sh;
       }
     }
-    (void)hipStreamSynchronize(st);
-    (void)hipFree(xs);
     if (e != hipSuccess) return e;
   }
   for (int i = 0; i < 8; i++)
@@ -440,28 +530,27 @@ static hipError_t cgm_tune(CGMachine* m, const Geom& g, const double* sm, const 
 }
 
 hipError_t cgm_f64_run(void** machine_slot, const Geom& g, const double* sm, const double2* b, double2* x,
-                       double tol, long max_iter, int start, CGWork& w, int nrhs, hipStream_t st,
-                       CGDevResult* host_res) {
-  const int nbs = (g.V + CGM_BS - 1) / CGM_BS;
+                       double tol, long max_iter, int start, int nrhs, hipStream_t st, CGDevResult* host_res) {
   CGMachine* m = cgm_get(machine_slot);
-  hipError_t e = m->eng.ensure(nrhs, sizeof(CGMArgs), sizeof(CGMState), 2 * (size_t)(nrhs < 2 ? 2 : nrhs) * nbs);
+  hipError_t e = cgm_ensure(m, g, nrhs);
   if (e != hipSuccess) return e;
   int shape = 0;
-  e = cgm_tune(m, g, sm, b, w, nrhs, st, &shape);
+  e = cgm_tune(m, g, sm, b, nrhs, st, &shape);
   if (e != hipSuccess) return e;
   hipGraphExec_t exec;
-  e = m->eng.graph(shape_kernels(g, shape), nrhs, &exec);
+  e = cgm_graph(m, g, nrhs, shape, &exec);
   if (e != hipSuccess) return e;
-  e = cgm_begin(m, g, sm, b, x, tol, max_iter, start, w, nrhs, shape, st);
+  e = cgm_begin(m, g, sm, b, x, tol, max_iter, start, nrhs, shape, st);
   if (e != hipSuccess) return e;
   e = m->eng.run(exec, nrhs, max_iter, st, nullptr);
   if (e != hipSuccess) return e;
+  e = hipMemcpyAsync(x, m->x, sizeof(double2) * (size_t)g.V * nrhs, hipMemcpyDeviceToDevice, st);
+  if (e != hipSuccess) return e;
   if (host_res) {
-    e = hipMemcpyAsync(host_res, w.res, sizeof(CGDevResult) * nrhs, hipMemcpyDeviceToHost, st);
+    e = hipMemcpyAsync(host_res, m->res, sizeof(CGDevResult) * nrhs, hipMemcpyDeviceToHost, st);
     if (e != hipSuccess) return e;
-    e = hipStreamSynchronize(st);
   }
-  return e;
+  return hipStreamSynchronize(st);
 }
 
 int cgm_shape_of(void* machine, int nrhs, int* spt, int* xcd) {

@@ -5,33 +5,35 @@
 // Inner iterations keep r, p, t in reduced-precision storage (FP32, or BF16
 // with FP32 arithmetic) and the solution correction xs in FP32; every global
 // sum is accumulated in FP64.  When the iterated residual has dropped by a
-// factor `delta` since the last update (or claims convergence), a reliable
-// update folds xs into the FP64 solution and recomputes the true residual
-// r = b - M M^dag x in FP64; convergence is declared only on that true
-// residual (||r|| < tol, absolute, as conjugategradient.c:87-89).  The search
-// direction is kept across updates (no restart), so the Krylov space is not
-// lost.  FP64 is used only for x, the true residual, the sums and the
-// scalars.
+// factor `delta` since the last update, claims convergence or reaches the
+// iteration cap, a reliable update folds xs into the FP64 solution and
+// recomputes the true residual r = b - M M^dag x in FP64; convergence is
+// declared only on that true residual (||r|| < tol, absolute, as
+// conjugategradient.c:87-89).  The search direction is kept across updates
+// (no restart), so the Krylov space is not lost.  FP64 is used only for x,
+// the true residual, the sums and the scalars.
 //
 // Kernel slots, per system (same graph engine as the FP64 solver):
-//   X: reduce rho; if a reliable update just finished, test the true
-//      residual; otherwise test ||r_it|| < delta * rmax -> start an update
-//      (X does u = M^dag (x + xs) in FP64); else p = r + beta p, t = M^dag p.
-//   Y: finish an update (x += xs, xs = 0, r64 = b - M u, r = (S)r64,
-//      partial |r64|^2) or do the normal step (xs += alpha p, r -= alpha M t,
-//      partial |r|^2).
+//   X: reduce rho; after an update, test the true residual; otherwise test
+//      ||r_it|| < delta * rmax -> start an update (u = M^dag (x + xs) in
+//      FP64, p copied forward so the slot parity keeps its meaning); else
+//      p = r + beta p, t = M^dag p.
+//   Y: finish an update (x += xs, xs = 0, r = (S)(b - M u), partial |r|^2 in
+//      FP64) or the normal step (xs += alpha p, r -= alpha M t, partial |r|^2).
+// Loads are issued before each kernel's reduction and kernel arguments are
+// passed by value, as in the FP64 solver.
 #include <stdlib.h>
 #include <string.h>
 
 #include "gnx_engine.h"
 #include "gnx_internal.h"
+#include "gnx_solver.h"
 
 namespace gnx {
 
 // ---- storage formats --------------------------------------------------------
 struct StoreF32 {
   using T = float2;
-  static constexpr int bytes = 8;
   GNX_DEVONLY static float2 ld(const T* p, int i) { return p[i]; }
   GNX_DEVONLY static void st(T* p, int i, float2 v) { p[i] = v; }
 };
@@ -39,7 +41,6 @@ struct StoreF32 {
 // bfloat16 pair (re in the low half), round-to-nearest-even on store
 struct StoreBF16 {
   using T = unsigned int;
-  static constexpr int bytes = 4;
   GNX_DEVONLY static float up(unsigned int h) { return __uint_as_float(h << 16); }
   GNX_DEVONLY static unsigned int down(float f) {
     unsigned int u = __float_as_uint(f);
@@ -59,23 +60,45 @@ enum MixPhase { PH_NORMAL = 0, PH_UPDATE = 1, PH_UPDATED = 2 };
 
 struct MixState {
   int active;
-  int iters;      // CG iterations (normal X/Y pairs)
+  int iters;      // CG iterations (normal slot pairs)
   int phase;
   int nrel;       // reliable updates done
   int converged;
   int early;
+  int start;
+  int pad;
   double rho;     // rho of the current direction (alpha = rho / gamma)
   double rmax;    // ||r|| at the last reliable update
+  double tol;
+  double delta;
+  long max_iter;
 };
 
-struct MixArgs {
+GNX_DEVONLY void put_mstate(MixState* o, int active, int iters, int phase, int nrel, int conv, int early,
+                            double rho, double rmax, const MixState& cfg) {
+  o->active = active;
+  o->iters = iters;
+  o->phase = phase;
+  o->nrel = nrel;
+  o->converged = conv;
+  o->early = early;
+  o->start = cfg.start;
+  o->pad = 0;
+  o->rho = rho;
+  o->rmax = rmax;
+  o->tol = cfg.tol;
+  o->delta = cfg.delta;
+  o->max_iter = cfg.max_iter;
+}
+
+// kernel arguments, fixed per solver instance (buffers the solver owns)
+struct MixFix {
   Geom g;
-  const double* sm;
+  double* sm;       // FP64 copy of the caller's sigma mass
   float* sm32;
-  const double2* b;
-  double2* x;       // FP64 solution (caller)
-  double2* r64;     // FP64 true residual
-  double2* u64;     // FP64 M^dag (x + xs)
+  double2* b;       // copy of the right-hand sides
+  double2* x;       // FP64 solutions
+  double2* u;       // FP64 M^dag (x + xs) of a reliable update
   float2* xs;       // FP32 solution correction
   void* r;          // storage-format residual
   void* p0;
@@ -84,74 +107,78 @@ struct MixArgs {
   double* part;     // [2 slots][nrhs][nb]
   MixState* state;  // [2][nrhs]
   CGDevResult* res;
-  double tol;
-  double delta;
-  long max_iter;
   int nrhs;
   int nb;
-  int start;
 };
 
-GNX_DEVONLY double* mslot(const MixArgs& a, int s, int rhs) { return a.part + ((size_t)s * a.nrhs + rhs) * a.nb; }
-
-GNX_DEVONLY double mreduce(const MixArgs& a, int s, int rhs, double* lds) {
-  const double* src = mslot(a, s, rhs);
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < a.nb; i += MIX_BS) acc += src[i];
-  return block_sum(acc, lds);
-}
+GNX_DEV double* mslot(const MixFix& a, int s, int rhs) { return a.part + ((size_t)s * a.nrhs + rhs) * a.nb; }
 
 GNX_DEVONLY double dnorm2(float2 v) { return (double)v.x * v.x + (double)v.y * v.y; }
 
 // ---- start kernels ------------------------------------------------------------
-// imposed start: x = 0, r64 = b, r = p0 = (S)b, xs = 0, sm32; partial |b|^2
+// imposed start: x = 0, r = p0 = (S)b, xs = 0, sm and b copied; partial |b|^2
 template <typename S>
-__global__ __launch_bounds__(MIX_BS) void k_mix_start_imposed(const MixArgs* __restrict__ ap) {
-  const MixArgs& a = *ap;
+__global__ __launch_bounds__(MIX_BS) void k_mix_start_imposed(MixFix a, const double* __restrict__ sm_in,
+                                                              const double2* __restrict__ b_in, double tol,
+                                                              double delta, long max_iter) {
   __shared__ double lds[MIX_BS / 64];
   const int rhs = blockIdx.y;
   const size_t off = (size_t)rhs * a.g.V;
   const int f = blockIdx.x * MIX_BS + threadIdx.x;
   double acc = 0.0;
   if (f < a.g.V) {
-    double2 bb = a.b[off + f];
+    double2 bb = b_in[off + f];
+    a.b[off + f] = bb;
     a.x[off + f] = make_double2(0.0, 0.0);
-    a.r64[off + f] = bb;
     float2 bs = make_float2((float)bb.x, (float)bb.y);
     S::st((typename S::T*)a.r + off, f, bs);
     S::st((typename S::T*)a.p0 + off, f, bs);
     a.xs[off + f] = make_float2(0.f, 0.f);
-    if (rhs == 0) a.sm32[f] = (float)a.sm[f];
+    if (rhs == 0) {
+      a.sm[f] = sm_in[f];
+      a.sm32[f] = (float)sm_in[f];
+    }
     acc = cnorm2(bb);
   }
   acc = block_sum(acc, lds);
   if (threadIdx.x == 0) mslot(a, 1, rhs)[blockIdx.x] = acc;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    MixState s = {1, 0, PH_NORMAL, 0, 0, 0, 0.0, 0.0};
-    a.state[rhs] = s;
+    MixState cfg;
+    cfg.start = GNX_START_IMPOSED;
+    cfg.tol = tol;
+    cfg.delta = delta;
+    cfg.max_iter = max_iter;
+    put_mstate(a.state + rhs, 1, 0, PH_NORMAL, 0, 0, 0, 0.0, 0.0, cfg);
   }
 }
 
-// null start (conjugategradient.c:18-33) pass 1: u64 = M^dag x_in
-__global__ __launch_bounds__(MIX_BS) void k_mix_null_p1(const MixArgs* __restrict__ ap) {
-  const MixArgs& a = *ap;
+// null start (conjugategradient.c:18-33) pass 1: u = M^dag x_in, x = x_in,
+// b and sm copied
+__global__ __launch_bounds__(MIX_BS) void k_mix_null_p1(MixFix a, const double* __restrict__ sm_in,
+                                                         const double2* __restrict__ b_in,
+                                                         const double2* __restrict__ xin) {
   const size_t off = (size_t)blockIdx.y * a.g.V;
   const int f = blockIdx.x * MIX_BS + threadIdx.x;
   if (f >= a.g.V) return;
-  const double2* xin = a.x + off;
-  a.u64[off + f] = apply_Mdag(a.g, decode(a.g, f), a.sm[f], [&](int q) { return xin[q]; });
+  const double2* xv = xin + off;
+  a.u[off + f] = apply_Mdag(a.g, decode(a.g, f), sm_in[f], [&](int q) { return xv[q]; });
+  a.x[off + f] = xv[f];
+  a.b[off + f] = b_in[off + f];
+  if (blockIdx.y == 0) {
+    a.sm[f] = sm_in[f];
+    a.sm32[f] = (float)sm_in[f];
+  }
 }
 
-// pass 2: r64 = b - M u64, partials |r64|^2 (slot 0), |b|^2 (slot 1)
-__global__ __launch_bounds__(MIX_BS) void k_mix_null_p2(const MixArgs* __restrict__ ap) {
-  const MixArgs& a = *ap;
+// pass 2: partials |b - M u|^2 (slot 0) and |b|^2 (slot 1)
+__global__ __launch_bounds__(MIX_BS) void k_mix_null_p2(MixFix a) {
   __shared__ double lds[MIX_BS / 64];
   const int rhs = blockIdx.y;
   const size_t off = (size_t)rhs * a.g.V;
   const int f = blockIdx.x * MIX_BS + threadIdx.x;
   double ar = 0.0, ab = 0.0;
   if (f < a.g.V) {
-    const double2* u = a.u64 + off;
+    const double2* u = a.u + off;
     double2 bb = a.b[off + f];
     double2 rr = apply_sub_M(a.g, decode(a.g, f), a.sm[f], bb, [&](int q) { return u[q]; });
     ar = cnorm2(rr);
@@ -165,29 +192,30 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_null_p2(const MixArgs* __restric
   }
 }
 
-// pass 3: early exit keeps x_in, otherwise as the imposed start (slot 1 kept)
+// pass 3: an early exit keeps x_in, otherwise as the imposed start (slot 1 kept)
 template <typename S>
-__global__ __launch_bounds__(MIX_BS) void k_mix_null_p3(const MixArgs* __restrict__ ap) {
-  const MixArgs& a = *ap;
+__global__ __launch_bounds__(MIX_BS) void k_mix_null_p3(MixFix a, double tol, double delta, long max_iter) {
   __shared__ double lds[MIX_BS / 64];
   const int rhs = blockIdx.y;
   const size_t off = (size_t)rhs * a.g.V;
-  const double nr = mreduce(a, 0, rhs, lds);
-  const bool early = sqrt(nr) < a.tol;
+  const double nr = block_sum(partial_share(mslot(a, 0, rhs), a.nb), lds);
+  const bool early = sqrt(nr) < tol;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    MixState s = {early ? 0 : 1, 0, PH_NORMAL, 0, early ? 1 : 0, early ? 1 : 0, 0.0, 0.0};
-    a.state[rhs] = s;
+    MixState cfg;
+    cfg.start = GNX_START_NULL;
+    cfg.tol = tol;
+    cfg.delta = delta;
+    cfg.max_iter = max_iter;
+    put_mstate(a.state + rhs, early ? 0 : 1, 0, PH_NORMAL, 0, early ? 1 : 0, early ? 1 : 0, 0.0, 0.0, cfg);
     if (early) {
       CGDevResult r = {1, 1, 0, 0, sqrt(nr), {0, 0, 0}};
       a.res[rhs] = r;
     }
   }
   const int f = blockIdx.x * MIX_BS + threadIdx.x;
-  if (f < a.g.V && rhs == 0) a.sm32[f] = (float)a.sm[f];
   if (early || f >= a.g.V) return;
   double2 bb = a.b[off + f];
   a.x[off + f] = make_double2(0.0, 0.0);
-  a.r64[off + f] = bb;
   float2 bs = make_float2((float)bb.x, (float)bb.y);
   S::st((typename S::T*)a.r + off, f, bs);
   S::st((typename S::T*)a.p0 + off, f, bs);
@@ -195,277 +223,306 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_null_p3(const MixArgs* __restric
 }
 
 // ---- iteration kernels -----------------------------------------------------------
+// odd: slot parity (p_old = odd ? p1 : p0); sin/sout state halves, pin/pout slots
 template <typename S>
-__global__ __launch_bounds__(MIX_BS) void k_mix_X(const MixArgs* __restrict__ ap) {
-  const MixArgs& a = *ap;
+__global__ __launch_bounds__(MIX_BS) void k_mix_X(MixFix a, int odd, const MixState* __restrict__ sin,
+                                                  MixState* __restrict__ sout, const double* __restrict__ pin,
+                                                  double* __restrict__ pout) {
   __shared__ double lds[MIX_BS / 64];
+  using T = typename S::T;
+  const Geom g = a.g;
   const int rhs = blockIdx.y;
-  const MixState s = a.state[rhs];
-  MixState* out = a.state + a.nrhs + rhs;
+  const size_t off = (size_t)rhs * g.V;
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const T* r = (const T*)a.r + off;
+  const T* pold = (const T*)(odd ? a.p1 : a.p0) + off;
+  T* pnew = (T*)(odd ? a.p0 : a.p1) + off;
+  const int f = blockIdx.x * MIX_BS + threadIdx.x;
+  const bool in = f < g.V;
+  // loads of the normal step, issued before the reduction
+  const MixState s = sin[rhs];
+  const double share = partial_share(pin + (size_t)rhs * a.nb, a.nb);
+  float2 rv[7], pv[7];
+  float smv = 0.f;
+  Site st = decode(g, in ? f : 0);
+  if (in) {
+    NbrF n = neighbours_f(g, st);
+    int q[7] = {f, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
+    smv = a.sm32[f];
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+      rv[k] = S::ld(r, q[k]);
+      pv[k] = S::ld(pold, q[k]);
+    }
+  }
+  const double rho = block_sum(share, lds);
+  const double nr = sqrt(rho);
+  MixState* out = sout + rhs;
   if (!s.active) {
-    if (lead) *out = s;
+    if (lead) put_mstate(out, 0, s.iters, s.phase, s.nrel, s.converged, s.early, s.rho, s.rmax, s);
     return;
   }
-  const Geom g = a.g;
-  const size_t off = (size_t)rhs * g.V;
-  const double rho = mreduce(a, 1, rhs, lds);
-  const double nr = sqrt(rho);
-  MixState n = s;
   double beta = 0.0;
+  double rmax = s.rmax;
+  int nrel = s.nrel;
   bool update = false;
   if (s.phase == PH_UPDATED) {
     // rho is the true FP64 residual norm^2 after a reliable update
-    n.nrel = s.nrel + 1;
-    if (nr < a.tol) {
+    nrel = s.nrel + 1;
+    if (nr < s.tol || s.iters >= s.max_iter) {
+      const int conv = nr < s.tol ? 1 : 0;
       if (lead) {
-        n.active = 0;
-        n.converged = 1;
-        n.phase = PH_NORMAL;
-        *out = n;
-        CGDevResult r = {1, 0, s.iters, n.nrel, nr, {0, 0, 0}};
-        a.res[rhs] = r;
+        put_mstate(out, 0, s.iters, PH_NORMAL, nrel, conv, 0, rho, nr, s);
+        CGDevResult res = {conv, 0, s.iters, nrel, nr, {0, 0, 0}};
+        a.res[rhs] = res;
       }
       return;
     }
-    if (s.iters >= a.max_iter) {  // cap reached: x already holds every correction
-      if (lead) {
-        n.active = 0;
-        n.phase = PH_NORMAL;
-        *out = n;
-        CGDevResult r = {0, 0, s.iters, n.nrel, nr, {0, 0, 0}};
-        a.res[rhs] = r;
-      }
-      return;
-    }
-    n.rmax = nr;
+    rmax = nr;
     beta = rho / s.rho;
   } else if (s.iters == 0) {
-    if (a.start == GNX_START_IMPOSED && nr < a.tol) {  // conjugategradient.c:123-128
+    if (s.start == GNX_START_IMPOSED && nr < s.tol) {  // conjugategradient.c:123-128
       if (lead) {
-        n.active = 0;
-        n.converged = 1;
-        n.early = 1;
-        *out = n;
-        CGDevResult r = {1, 1, 0, 0, nr, {0, 0, 0}};
-        a.res[rhs] = r;
+        put_mstate(out, 0, 0, PH_NORMAL, 0, 1, 1, rho, nr, s);
+        CGDevResult res = {1, 1, 0, 0, nr, {0, 0, 0}};
+        a.res[rhs] = res;
       }
       return;
     }
-    n.rmax = nr;
+    rmax = nr;
   } else {
     // fold the correction and test the true residual when the iterated one
     // has dropped by delta, claims convergence, or the iteration cap is hit
-    update = (nr < a.delta * s.rmax) || (nr < a.tol) || (s.iters >= a.max_iter);
+    update = (nr < s.delta * s.rmax) || (nr < s.tol) || (s.iters >= s.max_iter);
     beta = rho / s.rho;
   }
-  const int f = blockIdx.x * MIX_BS + threadIdx.x;
   if (update) {
     // reliable update, part 1: u = M^dag (x + xs) in FP64 (x and xs are not
-    // written in this kernel, so neighbours read consistent values)
-    if (f < g.V) {
+    // written in this kernel, so neighbours read consistent values); the
+    // direction is carried into the other parity buffer unchanged
+    if (in) {
       const double2* x = a.x + off;
       const float2* xs = a.xs + off;
       auto ldx = [&](int q) {
         float2 c = xs[q];
         return cadd(x[q], make_double2((double)c.x, (double)c.y));
       };
-      a.u64[off + f] = apply_Mdag(g, decode(g, f), a.sm[f], ldx);
+      a.u[off + f] = apply_Mdag(g, st, a.sm[f], ldx);
+      S::st(pnew, f, pv[0]);
     }
-    if (lead) {
-      n.phase = PH_UPDATE;  // keep n.rho: beta after the update is rho_true / rho_k
-      *out = n;
-    }
+    if (lead) put_mstate(out, 1, s.iters, PH_UPDATE, nrel, 0, 0, s.rho, rmax, s);
     return;
   }
   // normal step: p = r + beta p, t = M^dag p
-  using T = typename S::T;
-  const T* r = (const T*)a.r + off;
-  const T* pold = (const T*)((s.iters & 1) ? a.p1 : a.p0) + off;
-  T* pnew = (T*)((s.iters & 1) ? a.p0 : a.p1) + off;
   const float fb = (float)beta;
   double acc = 0.0;
-  if (f < g.V) {
-    auto ldp = [&](int q) { return cadd(S::ld(r, q), rmul(fb, S::ld(pold, q))); };
-    S::st(pnew, f, ldp(f));
-    float2 tt = apply_Mdag(g, decode(g, f), a.sm32[f], ldp);
+  if (in) {
+    float2 p0v = cadd(rv[0], rmul(fb, pv[0]));
+    float2 ppv[3], pmv[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      ppv[d] = cadd(rv[1 + d], rmul(fb, pv[1 + d]));
+      pmv[d] = cadd(rv[4 + d], rmul(fb, pv[4 + d]));
+    }
+    S::st(pnew, f, p0v);
+    float2 tt = apply_Mdag_vals(g, st, smv, p0v, ppv, pmv);
     S::st((T*)a.t + off, f, tt);
     acc = dnorm2(tt);
   }
   acc = block_sum(acc, lds);
-  if (threadIdx.x == 0) mslot(a, 0, rhs)[blockIdx.x] = acc;
-  if (lead) {
-    n.phase = PH_NORMAL;
-    n.rho = rho;
-    *out = n;
-  }
+  if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + blockIdx.x] = acc;
+  if (lead) put_mstate(out, 1, s.iters, PH_NORMAL, nrel, 0, 0, rho, rmax, s);
 }
 
 template <typename S>
-__global__ __launch_bounds__(MIX_BS) void k_mix_Y(const MixArgs* __restrict__ ap) {
-  const MixArgs& a = *ap;
+__global__ __launch_bounds__(MIX_BS) void k_mix_Y(MixFix a, int odd, const MixState* __restrict__ sin,
+                                                  MixState* __restrict__ sout, const double* __restrict__ pin,
+                                                  double* __restrict__ pout) {
   __shared__ double lds[MIX_BS / 64];
+  using T = typename S::T;
+  const Geom g = a.g;
   const int rhs = blockIdx.y;
-  const MixState s = a.state[a.nrhs + rhs];
-  MixState* out = a.state + rhs;
+  const size_t off = (size_t)rhs * g.V;
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const int f = blockIdx.x * MIX_BS + threadIdx.x;
+  const bool in = f < g.V;
+  const MixState s = sin[rhs];
+  const double share = partial_share(pin + (size_t)rhs * a.nb, a.nb);
+  MixState* out = sout + rhs;
   if (!s.active) {
-    if (lead) *out = s;
+    if (lead) put_mstate(out, 0, s.iters, s.phase, s.nrel, s.converged, s.early, s.rho, s.rmax, s);
     return;
   }
-  const Geom g = a.g;
-  const size_t off = (size_t)rhs * g.V;
-  const int f = blockIdx.x * MIX_BS + threadIdx.x;
-  using T = typename S::T;
-  MixState n = s;
   double acc = 0.0;
   if (s.phase == PH_UPDATE) {
-    // reliable update, part 2: x += xs, xs = 0, r64 = b - M u, r = (S) r64
-    if (f < g.V) {
+    // reliable update, part 2: x += xs, xs = 0, r = (S)(b - M u)
+    if (in) {
       float2 c = a.xs[off + f];
       a.x[off + f] = cadd(a.x[off + f], make_double2((double)c.x, (double)c.y));
       a.xs[off + f] = make_float2(0.f, 0.f);
-      const double2* u = a.u64 + off;
+      const double2* u = a.u + off;
       double2 rr = apply_sub_M(g, decode(g, f), a.sm[f], a.b[off + f], [&](int q) { return u[q]; });
-      a.r64[off + f] = rr;
       S::st((T*)a.r + off, f, make_float2((float)rr.x, (float)rr.y));
       acc = cnorm2(rr);
     }
-    n.phase = PH_UPDATED;
-  } else {
-    const double gamma = mreduce(a, 0, rhs, lds);
-    const float al = (float)(s.rho / gamma);
-    if (f < g.V) {
-      const T* pnew = (const T*)((s.iters & 1) ? a.p0 : a.p1) + off;
-      const T* t = (const T*)a.t + off;
-      T* r = (T*)a.r + off;
-      float2 w = apply_M(g, decode(g, f), a.sm32[f], [&](int q) { return S::ld(t, q); });
-      float2 pv = S::ld(pnew, f);
-      float2 xv = a.xs[off + f];
-      a.xs[off + f] = make_float2(xv.x + al * pv.x, xv.y + al * pv.y);
-      float2 rv = S::ld(r, f);
-      float2 rn = make_float2(rv.x - al * w.x, rv.y - al * w.y);
-      S::st(r, f, rn);
-      acc = dnorm2(rn);
-    }
-    n.iters = s.iters + 1;
+    acc = block_sum(acc, lds);
+    if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + blockIdx.x] = acc;
+    if (lead) put_mstate(out, 1, s.iters, PH_UPDATED, s.nrel, 0, 0, s.rho, s.rmax, s);
+    return;
+  }
+  // normal step: loads first, then gamma
+  const T* pnew = (const T*)(odd ? a.p0 : a.p1) + off;
+  const T* t = (const T*)a.t + off;
+  T* r = (T*)a.r + off;
+  float2 w = make_float2(0.f, 0.f), pv = w, xv = w, rv = w;
+  if (in) {
+    w = apply_M(g, decode(g, f), a.sm32[f], [&](int q) { return S::ld(t, q); });
+    pv = S::ld(pnew, f);
+    xv = a.xs[off + f];
+    rv = S::ld(r, f);
+  }
+  const double gamma = block_sum(share, lds);
+  const float al = (float)(s.rho / gamma);
+  if (in) {
+    a.xs[off + f] = make_float2(xv.x + al * pv.x, xv.y + al * pv.y);
+    float2 rn = make_float2(rv.x - al * w.x, rv.y - al * w.y);
+    S::st(r, f, rn);
+    acc = dnorm2(rn);
   }
   acc = block_sum(acc, lds);
-  if (threadIdx.x == 0) mslot(a, 1, rhs)[blockIdx.x] = acc;
-  if (lead) *out = n;
+  if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + blockIdx.x] = acc;
+  if (lead) put_mstate(out, 1, s.iters + 1, PH_NORMAL, s.nrel, 0, 0, s.rho, s.rmax, s);
 }
 
 // ---- host side ---------------------------------------------------------------
-struct MixWork {
-  double2* r64 = nullptr;
-  double2* u64 = nullptr;
-  float2* xs = nullptr;
+struct MixMachine {
+  GraphEngine eng;
+  double* sm = nullptr;
   float* sm32 = nullptr;
+  double2* b = nullptr;
+  double2* x = nullptr;
+  double2* u = nullptr;
+  float2* xs = nullptr;
   void* r = nullptr;
   void* p0 = nullptr;
   void* p1 = nullptr;
   void* t = nullptr;
   CGDevResult* res = nullptr;
   long cap = 0;  // complex elements per vector
-  int store_bytes = 0;
-};
-
-struct MixMachine {
-  GraphEngine eng;
-  MixWork w;
-  ~MixMachine() {
-    void* bufs[] = {w.r64, w.u64, w.xs, w.sm32, w.r, w.p0, w.p1, w.t, w.res};
+  void free_vectors() {
+    void* bufs[] = {sm, sm32, b, x, u, xs, r, p0, p1, t, res};
     for (void* p : bufs)
       if (p) (void)hipFree(p);
+    sm = nullptr;
+    sm32 = nullptr;
+    b = x = u = nullptr;
+    xs = nullptr;
+    r = p0 = p1 = t = nullptr;
+    res = nullptr;
+    cap = 0;
   }
+  ~MixMachine() { free_vectors(); }
 };
 
 void cgmix_release(void* m) { delete (MixMachine*)m; }
 
-static hipError_t mix_work(MixMachine* m, long V, int nrhs, int sbytes) {
-  MixWork& w = m->w;
-  long need = V * nrhs;
-  if (w.cap >= need && w.store_bytes >= sbytes) return hipSuccess;
-  void* bufs[] = {w.r64, w.u64, w.xs, w.sm32, w.r, w.p0, w.p1, w.t, w.res};
-  for (void* p : bufs)
-    if (p) (void)hipFree(p);
-  w = MixWork();
-  hipError_t e = hipMalloc(&w.r64, sizeof(double2) * need);
-  if (e == hipSuccess) e = hipMalloc(&w.u64, sizeof(double2) * need);
-  if (e == hipSuccess) e = hipMalloc(&w.xs, sizeof(float2) * need);
-  if (e == hipSuccess) e = hipMalloc(&w.sm32, sizeof(float) * V);
-  if (e == hipSuccess) e = hipMalloc(&w.r, (size_t)8 * need);
-  if (e == hipSuccess) e = hipMalloc(&w.p0, (size_t)8 * need);
-  if (e == hipSuccess) e = hipMalloc(&w.p1, (size_t)8 * need);
-  if (e == hipSuccess) e = hipMalloc(&w.t, (size_t)8 * need);
-  if (e == hipSuccess) e = hipMalloc(&w.res, sizeof(CGDevResult) * nrhs);
+static hipError_t mix_ensure(MixMachine* m, const Geom& g, int nrhs) {
+  const int nb = (g.V + MIX_BS - 1) / MIX_BS;
+  const int cap_rhs = nrhs < 2 ? 2 : nrhs;
+  hipError_t e = m->eng.ensure(cap_rhs, 1, sizeof(MixState), 2 * (size_t)cap_rhs * nb);
   if (e != hipSuccess) return e;
-  w.cap = need;
-  w.store_bytes = 8;
+  const long need = (long)g.V * cap_rhs;
+  if (m->cap >= need && m->res) return hipSuccess;
+  m->free_vectors();
+  m->eng.invalidate_graphs();
+  e = hipMalloc(&m->sm, sizeof(double) * g.V);
+  if (e == hipSuccess) e = hipMalloc(&m->sm32, sizeof(float) * g.V);
+  if (e == hipSuccess) e = hipMalloc(&m->b, sizeof(double2) * need);
+  if (e == hipSuccess) e = hipMalloc(&m->x, sizeof(double2) * need);
+  if (e == hipSuccess) e = hipMalloc(&m->u, sizeof(double2) * need);
+  if (e == hipSuccess) e = hipMalloc(&m->xs, sizeof(float2) * need);
+  if (e == hipSuccess) e = hipMalloc(&m->r, sizeof(float2) * need);
+  if (e == hipSuccess) e = hipMalloc(&m->p0, sizeof(float2) * need);
+  if (e == hipSuccess) e = hipMalloc(&m->p1, sizeof(float2) * need);
+  if (e == hipSuccess) e = hipMalloc(&m->t, sizeof(float2) * need);
+  if (e == hipSuccess) e = hipMalloc(&m->res, sizeof(CGDevResult) * cap_rhs);
+  if (e != hipSuccess) {
+    m->free_vectors();
+    return e;
+  }
+  m->cap = need;
   return hipSuccess;
+}
+
+static MixFix mix_fix(MixMachine* m, const Geom& g, int nrhs) {
+  MixFix a;
+  a.g = g;
+  a.sm = m->sm;
+  a.sm32 = m->sm32;
+  a.b = m->b;
+  a.x = m->x;
+  a.u = m->u;
+  a.xs = m->xs;
+  a.r = m->r;
+  a.p0 = m->p0;
+  a.p1 = m->p1;
+  a.t = m->t;
+  a.part = m->eng.dev_part();
+  a.state = (MixState*)m->eng.dev_state();
+  a.res = m->res;
+  a.nrhs = nrhs;
+  a.nb = (g.V + MIX_BS - 1) / MIX_BS;
+  return a;
 }
 
 template <typename S>
 static hipError_t mix_run_t(MixMachine* m, const Geom& g, const double* sm, const double2* b, double2* x,
                             double tol, double delta, long max_iter, int start, int nrhs, hipStream_t st,
                             CGDevResult* host_res, int tag) {
-  const int nb = (g.V + MIX_BS - 1) / MIX_BS;
-  hipError_t e = m->eng.ensure(nrhs, sizeof(MixArgs), sizeof(MixState), 2 * (size_t)(nrhs < 2 ? 2 : nrhs) * nb);
+  hipError_t e = mix_ensure(m, g, nrhs);
   if (e != hipSuccess) return e;
-  e = mix_work(m, g.V, nrhs, S::bytes);
-  if (e != hipSuccess) return e;
-  EngineKernels k = {(const void*)k_mix_X<S>, (const void*)k_mix_Y<S>, nb, tag};
+  MixFix a = mix_fix(m, g, nrhs);
+  dim3 grid(a.nb, nrhs), blk(MIX_BS);
+  const MixState* s0 = a.state;
+  const MixState* s1 = a.state + nrhs;
+  MixState* w0 = a.state;
+  MixState* w1 = a.state + nrhs;
+  const double* q0 = mslot(a, 0, 0);
+  const double* q1 = mslot(a, 1, 0);
+  double* o0 = mslot(a, 0, 0);
+  double* o1 = mslot(a, 1, 0);
+  auto body = [&](hipStream_t cs) {
+    for (int i = 0; i < ENG_CHUNK; i++) {
+      int odd = i & 1;
+      // X: state[0] -> state[1], reduces slot 1, writes slot 0; Y the reverse
+      void* argsX[] = {(void*)&a, (void*)&odd, (void*)&s0, (void*)&w1, (void*)&q1, (void*)&o0};
+      void* argsY[] = {(void*)&a, (void*)&odd, (void*)&s1, (void*)&w0, (void*)&q0, (void*)&o1};
+      (void)hipLaunchKernel((const void*)k_mix_X<S>, grid, blk, argsX, 0, cs);
+      (void)hipLaunchKernel((const void*)k_mix_Y<S>, grid, blk, argsY, 0, cs);
+    }
+  };
   hipGraphExec_t exec;
-  e = m->eng.graph(k, nrhs, &exec);
+  e = m->eng.graph(tag, nrhs, body, &exec);
   if (e != hipSuccess) return e;
-  e = hipStreamSynchronize(st);
-  if (e != hipSuccess) return e;
-  MixArgs& a = *(MixArgs*)m->eng.host_args();
-  MixWork& w = m->w;
-  a.g = g;
-  a.sm = sm;
-  a.sm32 = w.sm32;
-  a.b = b;
-  a.x = x;
-  a.r64 = w.r64;
-  a.u64 = w.u64;
-  a.xs = w.xs;
-  a.r = w.r;
-  a.p0 = w.p0;
-  a.p1 = w.p1;
-  a.t = w.t;
-  a.part = m->eng.dev_part();
-  a.state = (MixState*)m->eng.dev_state();
-  a.res = w.res;
-  a.tol = tol;
-  a.delta = delta;
-  a.max_iter = max_iter;
-  a.nrhs = nrhs;
-  a.nb = nb;
-  a.start = start;
-  e = m->eng.stage_args(st);
-  if (e != hipSuccess) return e;
-  const MixArgs* dp = (const MixArgs*)m->eng.dev_args();
-  dim3 grid(nb, nrhs), blk(MIX_BS);
   if (start == GNX_START_NULL) {
-    hipLaunchKernelGGL(k_mix_null_p1, grid, blk, 0, st, dp);
-    hipLaunchKernelGGL(k_mix_null_p2, grid, blk, 0, st, dp);
-    hipLaunchKernelGGL(k_mix_null_p3<S>, grid, blk, 0, st, dp);
+    hipLaunchKernelGGL(k_mix_null_p1, grid, blk, 0, st, a, sm, b, (const double2*)x);
+    hipLaunchKernelGGL(k_mix_null_p2, grid, blk, 0, st, a);
+    hipLaunchKernelGGL(k_mix_null_p3<S>, grid, blk, 0, st, a, tol, delta, max_iter);
   } else {
-    hipLaunchKernelGGL(k_mix_start_imposed<S>, grid, blk, 0, st, dp);
+    hipLaunchKernelGGL(k_mix_start_imposed<S>, grid, blk, 0, st, a, sm, b, tol, delta, max_iter);
   }
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  // a system needs at most max_iter normal slots plus one per reliable update;
-  // updates happen at most once per normal step
+  // at most max_iter normal slot pairs plus one update pair after each
   e = m->eng.run(exec, nrhs, 2 * max_iter + 4, st, nullptr);
   if (e != hipSuccess) return e;
+  e = hipMemcpyAsync(x, m->x, sizeof(double2) * (size_t)g.V * nrhs, hipMemcpyDeviceToDevice, st);
+  if (e != hipSuccess) return e;
   if (host_res) {
-    e = hipMemcpyAsync(host_res, w.res, sizeof(CGDevResult) * nrhs, hipMemcpyDeviceToHost, st);
+    e = hipMemcpyAsync(host_res, m->res, sizeof(CGDevResult) * nrhs, hipMemcpyDeviceToHost, st);
     if (e != hipSuccess) return e;
-    e = hipStreamSynchronize(st);
   }
-  return e;
+  return hipStreamSynchronize(st);
 }
 
 hipError_t cgmix_run(void** machine_slot, const Geom& g, const double* sm, const double2* b, double2* x, double tol,

@@ -12,17 +12,15 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
+
 namespace gnx {
 
 constexpr int ENG_BS = 256;     // threads per block of every solver kernel
 constexpr int ENG_CHUNK = 16;   // iterations per graph replay
 
-struct EngineKernels {
-  const void* X;
-  const void* Y;
-  int blocks;   // blocks per system (grid.x); grid.y = nrhs
-  int tag;      // identifies the pair in the graph cache
-};
+// launches ENG_CHUNK iterations (every X and Y kernel) into the capture stream
+using ChunkBody = std::function<void(hipStream_t)>;
 
 class GraphEngine {
  public:
@@ -35,7 +33,10 @@ class GraphEngine {
   double* dev_part() { return part_; }
   // wait for the stream, then copy the staged host arguments to the device
   hipError_t stage_args(hipStream_t st);
-  hipError_t graph(const EngineKernels& k, int nrhs, hipGraphExec_t* exec);
+  // graph of one chunk for (tag, nrhs), captured from `body` on first use
+  hipError_t graph(int tag, int nrhs, const ChunkBody& body, hipGraphExec_t* exec);
+  // drop every captured graph (their kernel arguments point at old buffers)
+  void invalidate_graphs();
   // replay until every system is inactive; returns the largest `iters`
   hipError_t run(hipGraphExec_t exec, int nrhs, long max_slots, hipStream_t st, int* max_iters);
   // time `chunks` replays (after the caller staged a start), in ms

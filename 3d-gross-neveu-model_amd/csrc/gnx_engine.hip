@@ -7,12 +7,17 @@ namespace gnx {
 
 GraphEngine::~GraphEngine() { release(); }
 
-void GraphEngine::release() {
+void GraphEngine::invalidate_graphs() {
   for (auto& g : graphs_)
     if (g.exec) {
       (void)hipGraphExecDestroy(g.exec);
       g = Entry();
     }
+  for (int i = 0; i < 8; i++) tuned_tag[i] = -1;
+}
+
+void GraphEngine::release() {
+  invalidate_graphs();
   if (dargs_) (void)hipFree(dargs_);
   if (hargs_) (void)hipHostFree(hargs_);
   if (state_) (void)hipFree(state_);
@@ -58,9 +63,9 @@ hipError_t GraphEngine::stage_args(hipStream_t st) {
   return hipMemcpyAsync(dargs_, hargs_, args_bytes_, hipMemcpyHostToDevice, st);
 }
 
-hipError_t GraphEngine::graph(const EngineKernels& k, int nrhs, hipGraphExec_t* exec) {
+hipError_t GraphEngine::graph(int tag, int nrhs, const ChunkBody& body, hipGraphExec_t* exec) {
   for (auto& g : graphs_)
-    if (g.exec && g.tag == k.tag && g.nrhs == nrhs) {
+    if (g.exec && g.tag == tag && g.nrhs == nrhs) {
       *exec = g.exec;
       return hipSuccess;
     }
@@ -79,13 +84,7 @@ hipError_t GraphEngine::graph(const EngineKernels& k, int nrhs, hipGraphExec_t* 
   hipGraph_t graph;
   hipError_t e = hipStreamBeginCapture(capture_, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return e;
-  dim3 grid(k.blocks, nrhs), blk(ENG_BS);
-  const void* dp = dargs_;
-  void* args[] = {(void*)&dp};
-  for (int i = 0; i < ENG_CHUNK; i++) {
-    (void)hipLaunchKernel(k.X, grid, blk, args, 0, capture_);
-    (void)hipLaunchKernel(k.Y, grid, blk, args, 0, capture_);
-  }
+  body(capture_);
   e = hipStreamEndCapture(capture_, &graph);
   if (e != hipSuccess) return e;
   e = hipGraphInstantiate(&slot->exec, graph, nullptr, nullptr, 0);
@@ -94,7 +93,7 @@ hipError_t GraphEngine::graph(const EngineKernels& k, int nrhs, hipGraphExec_t* 
     slot->exec = nullptr;
     return e;
   }
-  slot->tag = k.tag;
+  slot->tag = tag;
   slot->nrhs = nrhs;
   *exec = slot->exec;
   return hipSuccess;

@@ -97,8 +97,6 @@ inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
 gnx_status hmc_cg(gnx_hmc* h, const double2* b, double2* x, double tol, long max_iter, int start, int nrhs,
                   bool* ok) {
   gnx_ctx* c = h->c;
-  gnx_status s = ensure_cg(c, nrhs);
-  if (s != GNX_OK) return s;
   CGDevResult r[2];
   HIP_TRY(cg_run(c, h->sm, b, x, tol, max_iter, start, nrhs, r));
   *ok = true;

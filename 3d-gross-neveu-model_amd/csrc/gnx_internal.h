@@ -37,24 +37,8 @@ struct CGDevResult {
   double pad[3];
 };
 
-// Scratch owned by a context for the persistent CG.
-struct CGWork {
-  double2* r = nullptr;
-  double2* p0 = nullptr;
-  double2* p1 = nullptr;
-  double2* t = nullptr;
-  double* part = nullptr;     // 2 slots x 2 values x max_blocks
-  CGDevResult* res = nullptr;
-  int max_blocks = 0;
-  long vec_elems = 0;         // complex elements per work vector (V * nrhs capacity)
-};
-
-hipError_t cg_f64_run(const Geom& g, const double* sm, const double2* b, double2* x, double tol, long max_iter,
-                      int start, CGWork& w, int nrhs, hipStream_t st, int device, CGDevResult* host_res);
-
 hipError_t cgm_f64_run(void** machine_slot, const Geom& g, const double* sm, const double2* b, double2* x,
-                       double tol, long max_iter, int start, CGWork& w, int nrhs, hipStream_t st,
-                       CGDevResult* host_res);
+                       double tol, long max_iter, int start, int nrhs, hipStream_t st, CGDevResult* host_res);
 void cgm_release(void* machine);
 hipError_t cgmix_run(void** machine_slot, const Geom& g, const double* sm, const double2* b, double2* x, double tol,
                      double delta, long max_iter, int start, int prec, int nrhs, hipStream_t st,
@@ -77,17 +61,13 @@ struct gnx_ctx {
   int red_nparts = 0;
   double* red_out = nullptr;    // 16 device scalars
   double* host_out = nullptr;   // 16 pinned host scalars
-  gnx::CGWork cg;
-  int cg_nrhs_cap = 0;
   void* cgm = nullptr;          // graph-replayed CG state machine (gnx_cgm.hip)
   void* cgmix = nullptr;        // mixed-precision reliable-update CG (gnx_cgmix.hip)
   double reliable_delta = 0.1;  // residual drop that triggers an FP64 reliable update
-  int cg_engine = 0;            // 0: graph state machine, 1: persistent cooperative kernel
 };
 
 
 namespace gnx {
-gnx_status ensure_cg(gnx_ctx* c, int nrhs);
 // FP64 CG on the context's engine; host_res[nrhs] filled on return
 hipError_t cg_run(gnx_ctx* c, const double* sm, const double2* b, double2* x, double tol, long max_iter, int start,
                   int nrhs, CGDevResult* host_res);

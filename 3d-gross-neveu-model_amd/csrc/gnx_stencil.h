@@ -114,19 +114,32 @@ GNX_DEV void adj_terms(const Geom& g, const Site& s, double hp[3], double hm[3])
   hm[2] = (s.x2 == g.LT - 1) ? e2 : -e2;    // coefficient of v[pp(z,2)]
 }
 
-template <typename LoadF, typename T2 = decltype(std::declval<LoadF>()(0))>
-GNX_DEV T2 apply_Mdag(const Geom& g, const Site& s, typename CT<T2>::R sm, LoadF load) {
-  NbrF n = neighbours_f(g, s);
+// M^+ from operand values already in registers:
+// v0 = v[site], vpp[d] = v[pp(z,d)], vmm[d] = v[mm(z,d)]
+template <typename T2>
+GNX_DEV T2 apply_Mdag_vals(const Geom& g, const Site& s, typename CT<T2>::R sm, T2 v0, const T2 vpp[3],
+                           const T2 vmm[3]) {
   double cm[3], cp[3];
   adj_terms(g, s, cm, cp);
-  int f = f_of_lex(g, s.lex, s.P);
   T2 tpp[3], tmm[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    tmm[d] = rmul(cm[d], load(n.mm[d]));
-    tpp[d] = rmul(cp[d], load(n.pp[d]));
+    tmm[d] = rmul(cm[d], vmm[d]);
+    tpp[d] = rmul(cp[d], vpp[d]);
   }
-  return accumulate_adj_order(g, s, rmul(sm, load(f)), tpp, tmm);
+  return accumulate_adj_order(g, s, rmul(sm, v0), tpp, tmm);
+}
+
+template <typename LoadF, typename T2 = decltype(std::declval<LoadF>()(0))>
+GNX_DEV T2 apply_Mdag(const Geom& g, const Site& s, typename CT<T2>::R sm, LoadF load) {
+  NbrF n = neighbours_f(g, s);
+  T2 vpp[3], vmm[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    vmm[d] = load(n.mm[d]);
+    vpp[d] = load(n.pp[d]);
+  }
+  return apply_Mdag_vals(g, s, sm, load(f_of_lex(g, s.lex, s.P)), vpp, vmm);
 }
 
 // r - M^+ v (dirac.c:253-287): r -= sm*v, then the scatter terms negated, in
