@@ -226,6 +226,16 @@ gnx_status gnx_sigma_force_from_solutions(gnx_ctx* c, const double* sigma, doubl
   return GNX_OK;
 }
 
+gnx_status gnx_sigma_force_terms(gnx_ctx* c, const double* sigma, double beta_s, int nflav, const double* X0,
+                                 const double* Y0, const double* X1, const double* Y1, double* force,
+                                 int accumulate, int sigma_term) {
+  CHECK_ARG(c && X0 && Y0 && X1 && Y1 && force, "gnx_sigma_force_terms: null argument");
+  CHECK_ARG(sigma || !sigma_term, "gnx_sigma_force_terms: the sigma term needs sigma");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(launch_sigma_force(c->g, sigma, beta_s, nflav, X0, Y0, X1, Y1, force, c->stream, accumulate, sigma_term));
+  return GNX_OK;
+}
+
 gnx_status gnx_sigma_force(gnx_ctx* c, const double* sigma, const double* sm, double beta_s, int nflav,
                            double tol, long max_iter, const double* const phi[2], double* const X[2],
                            double* const Y[2], double* force, gnx_cg_result res[2]) {

@@ -21,7 +21,7 @@ hipError_t launch_dirac(const Geom& g, int op, const double* sm, const double* i
                         hipStream_t st);
 hipError_t launch_sigma_force(const Geom& g, const double* sigma, double beta_s, int nflav, const double* X0,
                               const double* Y0, const double* X1, const double* Y1, double* force,
-                              hipStream_t st);
+                              hipStream_t st, int accumulate = 0, int sigma_term = 1);
 hipError_t launch_reduce(int n, int kind, const void* a, const void* b, double scale, double* part, int nparts,
                          double* out, hipStream_t st);
 hipError_t launch_axpy_real(int V, double alpha, const double* xv, double* y, hipStream_t st);

@@ -52,17 +52,20 @@ __global__ void k_dirac(Geom g, int op, const double* __restrict__ sm, const dou
 // ------------------------------------------------------------------ sigma force
 // moleculardynamics.c:606-620 (fermion part, flavour-outer, corner order
 // {x, x+0+1, x+0+2, x+1+2, x+0, x+1, x+2, x+0+1+2}) then :576-581.
+// accumulate: add onto the incoming force instead of starting from 0 (the
+// reference's AddForceFromFermionSectorRespectToSigma adds to a caller-zeroed
+// force); sigma_term: finish with AddForceFromSigmaSector
 __global__ void k_sigma_force(Geom g, const double* __restrict__ sigma, double beta_s, int nflav,
                               const double2* __restrict__ X0, const double2* __restrict__ Y0,
                               const double2* __restrict__ X1, const double2* __restrict__ Y1,
-                              double* __restrict__ force) {
+                              double* __restrict__ force, int accumulate, int sigma_term) {
   int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= g.V) return;
   Site s = decode(g, f);
-  double F = 0.0;
+  double F = accumulate ? force[f] : 0.0;
   F = force_flavour(g, s, X0, Y0, F);
   F = force_flavour(g, s, X1, Y1, F);
-  F -= (nflav * beta_s / 2.0) * sigma[f];
+  if (sigma_term) F -= (nflav * beta_s / 2.0) * sigma[f];
   force[f] = F;
 }
 
@@ -172,9 +175,10 @@ hipError_t launch_dirac(const Geom& g, int op, const double* sm, const double* i
 
 hipError_t launch_sigma_force(const Geom& g, const double* sigma, double beta_s, int nflav, const double* X0,
                               const double* Y0, const double* X1, const double* Y1, double* force,
-                              hipStream_t st) {
+                              hipStream_t st, int accumulate, int sigma_term) {
   hipLaunchKernelGGL(k_sigma_force, dim3(nblk(g.V, 256)), dim3(256), 0, st, g, sigma, beta_s, nflav,
-                     (const double2*)X0, (const double2*)Y0, (const double2*)X1, (const double2*)Y1, force);
+                     (const double2*)X0, (const double2*)Y0, (const double2*)X1, (const double2*)Y1, force,
+                     accumulate, sigma_term);
   return hipGetLastError();
 }
 

@@ -135,6 +135,14 @@ gnx_status gnx_sigma_force_from_solutions(gnx_ctx *ctx, const double *sigma, dou
                                           const double *X0, const double *Y0, const double *X1,
                                           const double *Y1, double *force);
 
+/* The two force terms separately: accumulate != 0 adds the fermion part onto
+ * the incoming force (AddForceFromFermionSectorRespectToSigma,
+ * moleculardynamics.c:606-620); sigma_term != 0 then subtracts
+ * (nflav*beta_s/2) sigma (AddForceFromSigmaSector, :576-581). */
+gnx_status gnx_sigma_force_terms(gnx_ctx *ctx, const double *sigma, double beta_s, int nflav,
+                                 const double *X0, const double *Y0, const double *X1, const double *Y1,
+                                 double *force, int accumulate, int sigma_term);
+
 /* A10. HMC pieces. */
 gnx_status gnx_evolve_momenta(gnx_ctx *ctx, double *mom, const double *force, double eps);
 gnx_status gnx_evolve_sigma(gnx_ctx *ctx, double *sigma, double *sm, const double *mom, double eps);

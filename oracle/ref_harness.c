@@ -16,8 +16,11 @@
 #include "simulation.h"
 #include "observables.h"
 
+#include <stddef.h>
 #include <stdint.h>
 #include <string.h>
+
+#include "../tests/refshim/layout_fields.h"
 
 static FILE *g_out;
 
@@ -140,6 +143,16 @@ int main(int argc, char **argv)
 
     int meta[4] = {LSPACE, LTIME, VOLUME, NFLAV};
     rec("meta", 'i', meta, 4);
+
+    /* struct layouts the drop-in library mirrors */
+    int layout[64], nl = 0;
+#define OFF(T, F) layout[nl++] = (int)offsetof(struct T, F);
+#define SZ(T) layout[nl++] = (int)sizeof(struct T);
+    GNX_LAYOUT_FIELDS(OFF)
+    GNX_LAYOUT_TYPES(SZ)
+#undef OFF
+#undef SZ
+    rec("struct_layout", 'i', layout, nl);
 
     /* ---------------- RNG stream ---------------- */
     Initrand(1231);

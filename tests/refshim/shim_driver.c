@@ -1,0 +1,213 @@
+/*
+ * shim_driver.c -- drives the drop-in library (include/gnx_refapi.h) the way
+ * the reference's own driver drives the functions it replaces, through the
+ * reference's struct layouts, and dumps the same named records as
+ * oracle/ref_harness.c so tests/test_refshim.py can compare them with
+ * the golden vectors of the reference itself.  The host RNG is the oracle's
+ * dSFMT restatement (pinned to the reference's stream).
+ *
+ * TEST INFRASTRUCTURE ONLY.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/gnx_refapi.h"
+#include "../../oracle/gnx_oracle.h"
+
+#define L 8
+#define V (L * L * L)
+#define HALF (V / 2)
+
+static FILE *g_out;
+static orc_rng R;
+static double drv_gauss1(void) { return orc_gauss1(&R); }
+static double drv_casuale(void) { return orc_casuale(&R); }
+
+static void rec(const char *name, char kind, const void *data, int64_t count)
+{
+    char hdr[40];
+    memset(hdr, 0, sizeof(hdr));
+    strncpy(hdr, name, 31);
+    hdr[32] = kind;
+    fwrite(hdr, 1, 40, g_out);
+    fwrite(&count, sizeof(int64_t), 1, g_out);
+    fwrite(data, kind == 'd' ? sizeof(double) : sizeof(int), (size_t)count, g_out);
+}
+
+static void rec_f(const char *name, const struct Fermion *f) { rec(name, 'd', f->fermion, 2 * V); }
+
+static gnx_ref_complex *cvec(void) { return (gnx_ref_complex *)calloc(V, sizeof(gnx_ref_complex)); }
+
+static void many(struct ManyFermions *m)
+{
+    m->flav = (struct Fermion *)calloc(2, sizeof(struct Fermion));
+    for (int f = 0; f < 2; f++) m->flav[f].fermion = cvec();
+}
+
+static void z2_complex(struct Fermion *f)
+{
+    for (int x = 0; x < V; x++) {
+        double re = (drv_casuale() <= 0.5) ? 1.0 : -1.0;
+        double im = (drv_casuale() <= 0.5) ? 1.0 : -1.0;
+        f->fermion[x] = re + I * im;
+    }
+}
+
+static void draw_sigma(struct Lattice *lat)
+{
+    double var = 2.0 / (4 * 0.85);
+    for (int x = 0; x < HALF; x++) orc_gauss2mv(&R, &lat->sigma[x], &lat->sigma[x + HALF], 0.0, var);
+    ComputeFullSigmaMass(lat, NULL, lat->sigmamass);
+}
+
+static double last_dh(const char *path)
+{
+    FILE *f = fopen(path, "r");
+    char line[4096], last[4096];
+    last[0] = 0;
+    while (fgets(line, sizeof(line), f)) strcpy(last, line);
+    fclose(f);
+    double v[13];
+    int n = 0;
+    char *tok = strtok(last, "\t\n");
+    while (tok && n < 13) { v[n++] = atof(tok); tok = strtok(NULL, "\t\n"); }
+    return n >= 12 ? v[11] : NAN;
+}
+
+int main(int argc, char **argv)
+{
+    if (argc < 3) { fprintf(stderr, "usage: %s out.bin accfile\n", argv[0]); return 1; }
+    g_out = fopen(argv[1], "wb");
+    const char *accfile = argv[2];
+    if (GnxRefInit(L, L, 0, drv_gauss1, drv_casuale) != 0) return 3;
+    GnxRefSetTolerances(1e-9, 1e-3);
+
+    struct Lattice lat;
+    memset(&lat, 0, sizeof(lat));
+    lat.sigma = (double *)calloc(V, sizeof(double));
+    lat.sigmamass = (double *)calloc(V, sizeof(double));
+    struct Info info;
+    memset(&info, 0, sizeof(info));
+    info.beta_scalar = 0.85;
+    struct Fermionkit kit;
+    memset(&kit, 0, sizeof(kit));
+    many(&kit.r_k); many(&kit.p_k); many(&kit.chi); many(&kit.phi);
+    many(&kit.inverted_phi); many(&kit.inverted_phi_obs);
+    kit.Ddag_inverted_phi_obs.fermion = cvec();
+    struct Forcesigma force = {(double *)calloc(V, sizeof(double))};
+    struct Momentasigma mom = {(double *)calloc(V, sizeof(double))};
+    struct Fermion rvf = {cvec()};
+    struct RandomVec rv = {&rvf};
+
+    /* ---------------- operators ---------------- */
+    orc_rng_init(&R, 1231);
+    draw_sigma(&lat);
+    rec("op_sigmamass", 'd', lat.sigmamass, V);
+    struct Fermion b = {cvec()}, c = {cvec()}, y = {cvec()}, x = {cvec()};
+    z2_complex(&b);
+    z2_complex(&c);
+    DiracFermion(&lat, &b, &y, &info, NULL);         rec_f("op_Mb", &y);
+    AdjDiracFermion(&lat, &b, &y, &info, NULL);      rec_f("op_Mdagb", &y);
+    memcpy(y.fermion, c.fermion, sizeof(gnx_ref_complex) * V);
+    SubtractDiracFermion(&lat, &b, &y, &info, NULL); rec_f("op_c_minus_Mb", &y);
+    memcpy(y.fermion, c.fermion, sizeof(gnx_ref_complex) * V);
+    SubtractAdjDiracFermion(&lat, &b, &y, &info, NULL); rec_f("op_c_minus_Mdagb", &y);
+
+    /* ---------------- CG ---------------- */
+    double tols[3] = {1e-10, 1e-6, 1e-3};
+    const char *tn[3] = {"1e-10", "1e-6", "1e-3"};
+    for (int i = 0; i < 3; i++) {
+        char name[64];
+        memcpy(x.fermion, c.fermion, sizeof(gnx_ref_complex) * V);
+        kit.iterations = 0.0; kit.CG_call = 0;
+        bool ok = ConjugateGradientFastDiracAdjFastDiracToTheMinusOne(&lat, &b, &x, &info, NULL, tols[i], &kit, 0);
+        snprintf(name, sizeof(name), "cg_null_x_%s", tn[i]); rec_f(name, &x);
+        double st[3] = {(double)ok, kit.iterations, (double)kit.CG_call};
+        snprintf(name, sizeof(name), "cg_null_stat_%s", tn[i]); rec(name, 'd', st, 3);
+        kit.iterations = 0.0; kit.CG_call = 0;
+        ok = ConjugateGradientFastDiracAdjFastDiracToTheMinusOne(&lat, &b, &x, &info, NULL, tols[i], &kit, 0);
+        double st2[3] = {(double)ok, kit.iterations, (double)kit.CG_call};
+        snprintf(name, sizeof(name), "cg_null_again_stat_%s", tn[i]); rec(name, 'd', st2, 3);
+        snprintf(name, sizeof(name), "cg_null_again_x_%s", tn[i]); rec_f(name, &x);
+        memcpy(x.fermion, c.fermion, sizeof(gnx_ref_complex) * V);
+        kit.iterations = 0.0; kit.CG_call = 0;
+        ok = StartX0ImposedConjugateGradientFastDiracAdjFastDiracToTheMinusOne(&lat, &b, &x, &info, NULL, tols[i], &kit, 0);
+        snprintf(name, sizeof(name), "cg_imp_x_%s", tn[i]); rec_f(name, &x);
+        double st3[3] = {(double)ok, kit.iterations, (double)kit.CG_call};
+        snprintf(name, sizeof(name), "cg_imp_stat_%s", tn[i]); rec(name, 'd', st3, 3);
+    }
+
+    /* ---------------- sigma force ---------------- */
+    orc_rng_init(&R, 4242);
+    for (int f = 0; f < 2; f++) {
+        for (int r = 0; r < V; r++) kit.chi.flav[f].fermion[r] = drv_gauss1();
+        DiracFermion(&lat, &kit.chi.flav[f], &kit.phi.flav[f], &info, NULL);
+        memset(kit.inverted_phi.flav[f].fermion, 0, sizeof(gnx_ref_complex) * V);
+    }
+    kit.iterations = 0.0; kit.CG_call = 0;
+    ComputeSigmaForce(&lat, &info, NULL, &force, &kit, 1e-10);
+    rec("force_F", 'd', force.force, V);
+    rec_f("force_X0", &kit.inverted_phi.flav[0]);
+    rec_f("force_X1", &kit.inverted_phi.flav[1]);
+    rec_f("force_Y0", &kit.inverted_phi_obs.flav[0]);
+    rec_f("force_Y1", &kit.inverted_phi_obs.flav[1]);
+    double fst[2] = {kit.iterations, (double)kit.CG_call};
+    rec("force_stat", 'd', fst, 2);
+    /* the accumulate-only entry point on a zeroed force gives the fermion part */
+    double *fsave = (double *)malloc(sizeof(double) * V);
+    memcpy(fsave, force.force, sizeof(double) * V);
+    memset(force.force, 0, sizeof(double) * V);
+    AddForceFromFermionSectorRespectToSigma(&lat, &info, NULL, &force, &kit, 1e-10);
+    rec("force_fermion_only", 'd', force.force, V);
+
+    /* ---------------- HMC + measurements ---------------- */
+    const char *tags[3] = {"hmc_e0.1", "hmc_e0.2", "hmc_e0.05s4"};
+    double eps[3] = {0.1, 0.2, 0.05};
+    int steps[3] = {2, 2, 4}, ntraj[3] = {20, 20, 5};
+    for (int t = 0; t < 3; t++) {
+        memset(&info, 0, sizeof(info));
+        info.beta_scalar = 0.85;
+        info.epsilon_sigma_HMC = eps[t];
+        info.steps_sigma_HMC = steps[t];
+        orc_rng_init(&R, 1231);
+        draw_sigma(&lat);
+        for (int f = 0; f < 2; f++) {
+            memset(kit.inverted_phi.flav[f].fermion, 0, sizeof(gnx_ref_complex) * V);
+            memset(kit.inverted_phi_obs.flav[f].fermion, 0, sizeof(gnx_ref_complex) * V);
+        }
+        kit.iterations = 0.0; kit.CG_call = 0;
+        remove(accfile);
+        double dh[64], acc[64], cond[64 * 11], iters[64], calls[64];
+        char name[64];
+        for (int k = 0; k < ntraj[t]; k++) {
+            int before = info.hybrid_accepted_sigma;
+            if (!LeapFrog2MinimumNormSigmaVelocityFirst(&lat, &info, &force, &mom, NULL, &kit, eps[t], steps[t], accfile)) {
+                fprintf(stderr, "inversion failure\n");
+                return 2;
+            }
+            acc[k] = (double)(info.hybrid_accepted_sigma - before);
+            dh[k] = last_dh(accfile);
+            double *o = &cond[11 * k];
+            ComputeCondensateDirac(&lat, &o[0], &o[1], &o[2], &o[3], &o[4], &o[5], &o[6], &o[7], &o[8], &o[9], &o[10],
+                                   &rv, &kit.inverted_phi_obs.flav[0], &info, NULL, 1e-9, &kit, 0);
+            iters[k] = kit.iterations;
+            calls[k] = (double)kit.CG_call;
+            if (k == 0) {
+                snprintf(name, sizeof(name), "%s_sigma1", tags[t]);
+                rec(name, 'd', lat.sigma, V);
+            }
+        }
+        snprintf(name, sizeof(name), "%s_dh", tags[t]);     rec(name, 'd', dh, ntraj[t]);
+        snprintf(name, sizeof(name), "%s_acc", tags[t]);    rec(name, 'd', acc, ntraj[t]);
+        snprintf(name, sizeof(name), "%s_cond", tags[t]);   rec(name, 'd', cond, 11 * ntraj[t]);
+        snprintf(name, sizeof(name), "%s_iters", tags[t]);  rec(name, 'd', iters, ntraj[t]);
+        snprintf(name, sizeof(name), "%s_calls", tags[t]);  rec(name, 'd', calls, ntraj[t]);
+        snprintf(name, sizeof(name), "%s_sigmaN", tags[t]); rec(name, 'd', lat.sigma, V);
+    }
+    GnxRefFinalize();
+    fclose(g_out);
+    return 0;
+}
