@@ -277,9 +277,9 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_A(CGMFix a, int odd, const CGMSt
       ppv[d] = cadd(rv[i][1 + d], rmul(beta, pv[i][1 + d]));
       pmv[d] = cadd(rv[i][4 + d], rmul(beta, pv[i][4 + d]));
     }
-    pnew[fs[i]] = p0v;
+    st_nt(&pnew[fs[i]], p0v);
     double2 tt = apply_Mdag_vals(g, st, smv[i], p0v, ppv, pmv);
-    a.t[off + fs[i]] = tt;
+    st_nt(&a.t[off + fs[i]], tt);
     acc += cnorm2(tt);
   }
   acc = block_sum(acc, lds);
@@ -325,9 +325,9 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_B(CGMFix a, int odd, const CGMSt
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
     if (fs[i] < 0) continue;
-    x[fs[i]] = cadd(xv[i], rmul(al, pv[i]));
+    st_nt(&x[fs[i]], cadd(xv[i], rmul(al, pv[i])));
     double2 rr = cadd(rv[i], rmul(-al, w[i]));
-    r[fs[i]] = rr;
+    st_nt(&r[fs[i]], rr);
     acc += cnorm2(rr);
   }
   acc = block_sum(acc, lds);
@@ -351,7 +351,7 @@ struct CGMShape {
   {S, X, (const void*)k_cgm_A<S, X>, (const void*)k_cgm_B<S, X>, (const void*)k_cgm_start_imposed<S, X>, \
    (const void*)k_cgm_null_p1<S, X>, (const void*)k_cgm_null_p2<S, X>, (const void*)k_cgm_null_p3<S, X>}
 static const CGMShape kShapes[] = {GNX_SHAPE(1, false), GNX_SHAPE(1, true), GNX_SHAPE(2, false),
-                                   GNX_SHAPE(2, true)};
+                                   GNX_SHAPE(2, true),  GNX_SHAPE(4, false), GNX_SHAPE(4, true)};
 #undef GNX_SHAPE
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 

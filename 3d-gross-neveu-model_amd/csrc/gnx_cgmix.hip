@@ -35,7 +35,7 @@ namespace gnx {
 struct StoreF32 {
   using T = float2;
   GNX_DEVONLY static float2 ld(const T* p, int i) { return p[i]; }
-  GNX_DEVONLY static void st(T* p, int i, float2 v) { p[i] = v; }
+  GNX_DEVONLY static void st(T* p, int i, float2 v) { st_nt(&p[i], v); }
 };
 
 // bfloat16 pair (re in the low half), round-to-nearest-even on store
@@ -51,7 +51,7 @@ struct StoreBF16 {
     unsigned int v = p[i];
     return make_float2(up(v & 0xFFFFu), up(v >> 16));
   }
-  GNX_DEVONLY static void st(T* p, int i, float2 v) { p[i] = down(v.x) | (down(v.y) << 16); }
+  GNX_DEVONLY static void st(T* p, int i, float2 v) { st_nt(&p[i], down(v.x) | (down(v.y) << 16)); }
 };
 
 constexpr int MIX_BS = ENG_BS;
@@ -384,7 +384,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Y(MixFix a, int odd, const MixSt
   const double gamma = block_sum(share, lds);
   const float al = (float)(s.rho / gamma);
   if (in) {
-    a.xs[off + f] = make_float2(xv.x + al * pv.x, xv.y + al * pv.y);
+    st_nt(&a.xs[off + f], make_float2(xv.x + al * pv.x, xv.y + al * pv.y));
     float2 rn = make_float2(rv.x - al * w.x, rv.y - al * w.y);
     S::st(r, f, rn);
     acc = dnorm2(rn);

@@ -29,6 +29,23 @@ struct SiteMap {
   }
 };
 
+// ---- streaming stores ----------------------------------------------------------
+// Vector outputs of a solver kernel are not re-read by the kernel that writes
+// them, and the next kernel (on any XCD) reads them from memory anyway, so
+// they are written non-temporally instead of occupying the L2 until the
+// end-of-kernel write-back.
+typedef double gnx_d2v __attribute__((ext_vector_type(2)));
+typedef float gnx_f2v __attribute__((ext_vector_type(2)));
+GNX_DEVONLY void st_nt(double2* p, double2 v) {
+  gnx_d2v w = {v.x, v.y};
+  __builtin_nontemporal_store(w, (gnx_d2v*)p);
+}
+GNX_DEVONLY void st_nt(float2* p, float2 v) {
+  gnx_f2v w = {v.x, v.y};
+  __builtin_nontemporal_store(w, (gnx_f2v*)p);
+}
+GNX_DEVONLY void st_nt(unsigned int* p, unsigned int v) { __builtin_nontemporal_store(v, p); }
+
 // per-thread share of n partials, fixed order; the loads of a round are
 // issued together (one memory latency per 8*BS partials, not one per load)
 GNX_DEVONLY double partial_share(const double* __restrict__ src, int n) {

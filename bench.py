@@ -73,6 +73,10 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", default="4,16", help="batch sizes of the batched FP64 CG leg ('' to skip)")
+    ap.add_argument("--mixed", default="f32,bf16", help="precisions of the mixed-precision legs ('' to skip)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also time a plain device copy of MIB MiB (the reference kernel for "
+                         "rocprofv3 FETCH_SIZE/WRITE_SIZE scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
@@ -185,7 +189,9 @@ def main():
 
     # ---------------- mixed precision (FP64 reliable updates) ----------------
     mixed = {}
-    for pname, pcode in (("f32", gnx.PREC_F32), ("bf16", gnx.PREC_BF16)):
+    precs = {"f32": gnx.PREC_F32, "bf16": gnx.PREC_BF16}
+    for pname in [v for v in args.mixed.split(",") if v]:
+        pcode = precs[pname]
         for nb in [1] + [int(v) for v in args.batch.split(",") if v]:
             rng = np.random.default_rng(11 + nb)
             B = b if nb == 1 else torch.from_numpy(np.where(rng.random(2 * V * nb) <= 0.5, 1.0, -1.0)).to(dev)
@@ -214,9 +220,25 @@ def main():
                 "speedup_vs_f64_single": nb * reps / dt / solves_per_s}
             del X
     result["mixed"] = mixed
-    best = max(mixed.values(), key=lambda d: d["rhs_solves_per_s"])
-    result["best_mixed_batched_rhs_solves_per_s"] = best["rhs_solves_per_s"]
-    result["best_mixed_batched_speedup_vs_f64"] = best["speedup_vs_f64_single"]
+    if mixed:
+        best = max(mixed.values(), key=lambda d: d["rhs_solves_per_s"])
+        result["best_mixed_batched_rhs_solves_per_s"] = best["rhs_solves_per_s"]
+        result["best_mixed_batched_speedup_vs_f64"] = best["speedup_vs_f64_single"]
+    if args.copy_calibration > 0:
+        n = args.copy_calibration * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        c0.record()
+        for _ in range(5):
+            dst.copy_(src)
+        c1.record()
+        c1.synchronize()
+        t = c0.elapsed_time(c1) * 1e-3 / 5
+        result["copy_calibration"] = {"bytes_read": 8 * n, "bytes_written": 8 * n, "copies": 5,
+                                      "GB_s": 16 * n / t / 1e9}
+        del src, dst
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(L, sigma_h, b_h, it)
     print(json.dumps(result), flush=True)

@@ -94,22 +94,31 @@ def test_noconv_and_determinism(torch_dev, P, gnx_mod):
     assert np.array_equal(host(x1), host(x2))
 
 
-def test_batch_matches_single(torch_dev, P, gnx_mod):
-    banner("cg batch of 3 == 3 single solves")
-    import torch
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5])
+def test_batch_matches_single(torch_dev, P, gnx_mod, shape, monkeypatch):
+    """Every launch shape (sites per thread 1/2/4, with and without the XCD
+    slab mapping) against the oracle; with the shape fixed, a batch of 3 is
+    bit-identical to 3 single solves (same partial sums, same order)."""
+    banner(f"cg batch of 3 == 3 single solves, shape {shape}")
+    monkeypatch.setenv("GNX_CG_SHAPE", str(shape))
     L = 12
     lat = P.Lattice(L)
     r = P.Random(9)
     sigma = r.sigma(lat.V, P.sigma_variance(0.85))
-    sm = dev(lat.sigma_mass(sigma))
+    smh = lat.sigma_mass(sigma)
+    sm = dev(smh)
     bs = [r.z2_complex(lat.V) for _ in range(3)]
     ctx = gnx_mod.Context(L)
     B = dev(np.concatenate(bs))
     X = ctx.vec(3)
     st, res = ctx.cg_batch(sm, B, X, 3, 1e-10, start=gnx_mod.START_IMPOSED)
     assert st == 0
+    assert ctx.cg_shape(3) == {"sites_per_thread": [1, 1, 2, 2, 4, 4][shape], "xcd_slab": shape % 2 == 1}
     for k in range(3):
         x = ctx.vec()
         s1, r1 = ctx.cg(sm, dev(bs[k]), x, 1e-10, start=gnx_mod.START_IMPOSED)
         assert r1.iterations == res[k].iterations
         assert np.array_equal(host(x), host(X[k * 2 * lat.V:(k + 1) * 2 * lat.V]))
+        xr, ok, stc = lat.cg(smh, bs[k], None, tol=1e-10, start="imposed")
+        assert abs(res[k].iterations - stc.last_iters) <= 1
+        assert rel(host(x), xr) < 1e-7
