@@ -165,6 +165,23 @@ def main():
                      "kernel": "k_cg_f64", "kernel_ms": avg_k * 1e3,
                      "bytes_per_site_iter": BYTES_CG_ITER_F64},
     }
+    # ---------------- PCIe-inclusive drop-in rate ----------------
+    # the reference-named drop-in receives host vectors: sigma mass, b and
+    # x_in go up and x comes back on every call (pageable host memory)
+    xh = np.zeros(2 * V)
+    smh = sm.cpu().numpy()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        smd = torch.from_numpy(smh).to(dev)
+        bd = torch.from_numpy(b_h).to(dev)
+        xd = torch.from_numpy(xh).to(dev)
+        ctx.cg(smd, bd, xd, args.tol, start=gnx.START_IMPOSED)
+        xh = xd.cpu().numpy()
+    pcie_rate = reps / (time.perf_counter() - t0)
+    result["pcie_inclusive"] = {"rhs_solves_per_s": pcie_rate,
+                                "note": "host sigma mass, b, x_in uploaded and x downloaded per solve"}
+
     # ---------------- batched FP64 CG ----------------
     batched = {}
     for nb in [int(v) for v in args.batch.split(",") if v]:

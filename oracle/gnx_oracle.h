@@ -14,19 +14,23 @@
  * the reference compiled with gcc -O3.  Lattice sizes are runtime values
  * (the reference fixes them at compile time, macro.h:21-24).
  *
- * Pinning (how each function is checked, see tests/):
- *   geometry  : golden tables dumped from the reference at 8^3, plus closed
- *               forms and bijection/inverse invariants at other L.
- *   sigma mass, M, M^dag, r-=Mv : golden vectors (bit-exact) at 8^3, dense
- *               adjoint identity <u,Mv> = <M^dag u,v>, free-field spectrum.
- *   CG (both starts): golden solutions/iteration counts at 8^3, plane-wave
- *               closed-form solution at constant sigma-mass.
- *   sigma force: golden force at 8^3, finite difference of the action.
- *   HMC trajectory: golden Delta-H / accept sequence at 8^3 (20 trajectories).
- *   condensate: golden 11 outputs at 8^3, free-field trace closed form.
+ * Pinning (tests/test_oracle_golden.py against vectors the reference's own
+ * sources produced at 8^3; tests/test_oracle_invariants.py at other shapes):
+ *   geometry  : golden tables at 8^3; closed forms of coordinates, f-index,
+ *               neighbours, eta and the time boundary on 4^2x6, 6^2x10, ...;
+ *               pp(mm(x)) = x, parity flip.
+ *   sigma mass, M, M^dag, r-=Mv : golden vectors (bit-exact) at 8^3; dense
+ *               M^dag == transpose(M) bit for bit, M + M^dag = 2 sm.
+ *   CG (both starts): golden solutions, iteration counts, early exits and
+ *               counters at 8^3 (three tolerances); free-field plane-wave
+ *               eigenvalues c^2 + sum sin^2 p and CG solution b / lambda.
+ *   sigma force: golden force, X and Y at 8^3; central finite difference of
+ *               the action (dense linear algebra) at 4^3.
+ *   HMC trajectory, energy: golden Delta-H, accept, sigma after one and after
+ *               all trajectories, CG counters at 8^3 (45 trajectories,
+ *               three step sizes).
+ *   condensate, CondensateSigmaPi: golden outputs after every trajectory.
  *   RNG       : golden dSFMT-132049 stream after seeding, Gauss1/Gauss2 values.
- *   link sector: parity unpinned (no live reference path); checked by gauge
- *               covariance and U==1 reduction only.
  */
 #ifndef GNX_ORACLE_H
 #define GNX_ORACLE_H
