@@ -8,8 +8,9 @@ solve with inputs resident in HBM.
 
 Reported: RHS-solves/s of the FP64 drop-in (the headline `value`), the Dirac
 application rate and GB/s, mixed-precision and batched solve rates, the
-roofline of the dominant kernel (the persistent CG) from HIP events on its
-stream, and the serial CPU baseline (the oracle, 1 core) on a bounded sample
+roofline of the dominant kernels (the CG iteration pair) from HIP events on
+their stream, with the HBM traffic per iteration recorded from rocprofv3
+counter runs of the same kernels, and the serial CPU baseline (the oracle, 1 core) on a bounded sample
 of the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--L 64]
@@ -28,6 +29,95 @@ sys.path.insert(0, os.path.join(ROOT, "3d-gross-neveu-model_amd"))
 HBM_PEAK_GBS = 8000.0
 BYTES_DIRAC_F64 = 40      # 2*16 + 8 per site per M or M^dag application
 BYTES_CG_ITER_F64 = 144   # 8*16 + 2*8 per site per CG iteration
+PROFILES = os.path.join(ROOT, "profiles")
+TRAFFIC_JSON = os.path.join(PROFILES, "cg_f64_L64_traffic.json")
+# sources whose change invalidates a recorded traffic measurement
+KERNEL_SOURCES = ["gnx_cgm.hip", "gnx_solver.h", "gnx_stencil.h", "gnx_common.h", "gnx_engine.h", "gnx_engine.hip"]
+
+
+def kernel_sources_sha():
+    import hashlib
+    h = hashlib.sha256()
+    for n in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, "3d-gross-neveu-model_amd", "csrc", n), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
+def shape_tag(shape):
+    return f"<{shape['sites_per_thread']}, {'true' if shape['xcd_slab'] else 'false'}>"
+
+
+def recorded_traffic(L, shape):
+    """HBM bytes per CG iteration measured by rocprofv3 for these kernel
+    sources and launch shape (see summarize_rocprof), else None."""
+    try:
+        with open(TRAFFIC_JSON) as f:
+            t = json.load(f)
+    except OSError:
+        return None
+    if t.get("sources_sha256") != kernel_sources_sha() or t.get("L") != L:
+        return None
+    return t.get("bytes_per_iteration", {}).get(shape_tag(shape))
+
+
+def summarize_rocprof(kt_dir, fetch_dir, write_dir, L):
+    """Summaries of rocprofv3 runs of this bench (no GPU needed): the
+    kernel-trace statistics of the CG kernels, and HBM bytes per CG
+    iteration from FETCH_SIZE / WRITE_SIZE passes, each scaled by what the
+    counter reads for the plain device copy of --copy-calibration."""
+    import csv
+    import statistics
+
+    def short(n):
+        return n.split("(")[0].replace("void ", "").replace("gnx::", "")
+
+    out = []
+    stats = list(csv.DictReader(open(os.path.join(kt_dir, "run_kernel_stats.csv"))))
+    out.append("| Kernel | Calls | Average (ns) | Min (ns) | Max (ns) | Percent |")
+    out.append("|---|---:|---:|---:|---:|---:|")
+    for r in stats[:16]:
+        out.append(f"| `{short(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']):.0f} | {r['MinNs']} | "
+                   f"{r['MaxNs']} | {float(r['Percentage']):.2f} |")
+    counters = {}
+    for ctr, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        per = {}
+        copy = []
+        for r in csv.DictReader(open(os.path.join(d, "run_counter_collection.csv"))):
+            n = short(r["Kernel_Name"])
+            v = float(r["Counter_Value"])
+            if n == "__amd_rocclr_copyBuffer":
+                copy.append((int(r["Grid_Size"]), v))
+            elif n.startswith(("k_cgm_A", "k_cgm_B")) and int(r["Grid_Size"]) * int(n.split("<")[1][0]) == L ** 3:
+                per.setdefault(n, []).append(v)
+        big = max(g for g, _ in copy)
+        copy_kib = statistics.median(v for g, v in copy if g == big)
+        counters[ctr] = (per, copy_kib)
+    calib_kib = 256 * 1024  # --copy-calibration 256: 256 MiB read and 256 MiB written
+    scale = {c: calib_kib / counters[c][1] for c in counters}
+    out.append("")
+    out.append(f"Copy calibration (256 MiB read + 256 MiB written): FETCH_SIZE reads "
+               f"{counters['FETCH_SIZE'][1]:.0f} KiB (scale {scale['FETCH_SIZE']:.3f}), WRITE_SIZE "
+               f"{counters['WRITE_SIZE'][1]:.0f} KiB (scale {scale['WRITE_SIZE']:.3f}).")
+    out.append("")
+    out.append("| Kernel (single RHS) | Dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM MB (scaled) |")
+    out.append("|---|---:|---:|---:|---:|")
+    bpi = {}
+    for n in sorted(set(counters["FETCH_SIZE"][0]) & set(counters["WRITE_SIZE"][0])):
+        fk = statistics.median(counters["FETCH_SIZE"][0][n])
+        wk = statistics.median(counters["WRITE_SIZE"][0][n])
+        mb = (fk * scale["FETCH_SIZE"] + wk * scale["WRITE_SIZE"]) * 1024 / 1e6
+        out.append(f"| `{n}` | {len(counters['FETCH_SIZE'][0][n])} | {fk:.0f} | {wk:.0f} | {mb:.2f} |")
+        tag = n[n.index("<"):]
+        bpi[tag] = bpi.get(tag, 0.0) + mb * 1e6
+    alg = BYTES_CG_ITER_F64 * L ** 3
+    out.append("")
+    for tag, b in sorted(bpi.items()):
+        out.append(f"Shape {tag}: {b / 1e6:.2f} MB per CG iteration (A + B) = {b / alg:.2f} x the "
+                   f"{alg / 1e6:.2f} MB algorithmic ({BYTES_CG_ITER_F64} B/site).")
+    rec = {"L": L, "sources_sha256": kernel_sources_sha(), "bytes_per_iteration": bpi,
+           "fetch_scale": scale["FETCH_SIZE"], "write_scale": scale["WRITE_SIZE"]}
+    return "\n".join(out), rec
 
 
 def make_inputs(L, seed=1231, beta_s=0.85, nflav=4):
@@ -77,7 +167,16 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also time a plain device copy of MIB MiB (the reference kernel for "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE scaling)")
+    ap.add_argument("--summarize-rocprof", nargs=3, metavar=("KT_DIR", "FETCH_DIR", "WRITE_DIR"),
+                    help="print the summary of rocprofv3 csv runs of this bench and record the "
+                         "measured traffic for later runs (no GPU needed)")
     args = ap.parse_args()
+    if args.summarize_rocprof:
+        text, rec = summarize_rocprof(*args.summarize_rocprof, L=args.L)
+        print(text)
+        with open(TRAFFIC_JSON, "w") as f:
+            json.dump(rec, f, indent=1)
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU sharding is a design, not a measured path", file=sys.stderr)
@@ -161,8 +260,10 @@ def main():
         "dirac": {"apps_per_s": 1.0 / t_app, "GB_s": dirac_gbs, "frac_hbm": dirac_gbs / HBM_PEAK_GBS,
                   "bytes_per_site": BYTES_DIRAC_F64},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_cg_f64", "kernel_ms": avg_k * 1e3,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": recorded_traffic(L, ctx.cg_shape(1)),
+                     "kernel": "k_cgm_A + k_cgm_B (one CG iteration)",
+                     "us_per_iteration": avg_k / it * 1e6, "solve_ms": avg_k * 1e3,
                      "bytes_per_site_iter": BYTES_CG_ITER_F64},
     }
     # ---------------- PCIe-inclusive drop-in rate ----------------
