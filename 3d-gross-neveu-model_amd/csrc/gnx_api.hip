@@ -102,6 +102,8 @@ gnx_status gnx_destroy(gnx_ctx* c) {
   (void)hipFree(c->red_part);
   (void)hipFree(c->red_out);
   if (c->host_out) (void)hipHostFree(c->host_out);
+  if (c->bat_part) (void)hipFree(c->bat_part);
+  if (c->bat_out) (void)hipFree(c->bat_out);
   if (c->cgm) cgm_release(c->cgm);
   if (c->cgmix) cgmix_release(c->cgmix);
   delete c;
@@ -199,6 +201,32 @@ gnx_status gnx_cg_batch(gnx_ctx* c, const double* sm, const double* b, double* x
 gnx_status gnx_cg(gnx_ctx* c, const double* sm, const double* b, double* x, double tol, long max_iter,
                   gnx_cg_start start, gnx_prec prec, gnx_cg_result* res) {
   return gnx_cg_batch(c, sm, b, x, 1, tol, max_iter, start, prec, res);
+}
+
+gnx_status gnx_condensate_batch(gnx_ctx* c, const double* sm, const double* noise, int nsrc, double tol,
+                                long max_iter, gnx_prec prec, double* X, double* per_source, gnx_cg_result* res) {
+  CHECK_ARG(c && sm && noise && X && per_source && res, "gnx_condensate_batch: null argument");
+  CHECK_ARG(nsrc >= 1 && nsrc <= 64, "gnx_condensate_batch: nsrc must be in [1, 64]");
+  CHECK_ARG((const void*)noise != (const void*)X, "gnx_condensate_batch: noise and X alias");
+  HIP_TRY(hipSetDevice(c->device));
+  // one batched solve of M M^dag X_k = v_k (the reference inverts each vector
+  // in turn from the previous solution, whose prologue exit never applies to a
+  // fresh noise vector, so the imposed start gives the same solutions)
+  gnx_status st = gnx_cg_batch(c, sm, noise, X, nsrc, tol, max_iter, GNX_START_IMPOSED, prec, res);
+  if (st != GNX_OK) return st;
+  const size_t need = (size_t)nsrc * (size_t)((c->V + 255) / 256);
+  if (c->bat_cap < need) {
+    if (c->bat_part) (void)hipFree(c->bat_part);
+    c->bat_part = nullptr;
+    c->bat_cap = 0;
+    HIP_TRY(hipMalloc(&c->bat_part, sizeof(double) * need));
+    c->bat_cap = need;
+  }
+  if (!c->bat_out) HIP_TRY(hipMalloc(&c->bat_out, sizeof(double) * 64));
+  HIP_TRY(launch_mdag_dot_batch(c->g, sm, X, noise, nsrc, c->bat_part, c->bat_out, c->stream));
+  HIP_TRY(hipMemcpyAsync(per_source, c->bat_out, sizeof(double) * nsrc, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return GNX_OK;
 }
 
 gnx_status gnx_set_reliable_delta(gnx_ctx* c, double delta) {

@@ -24,6 +24,8 @@ hipError_t launch_sigma_force(const Geom& g, const double* sigma, double beta_s,
                               hipStream_t st, int accumulate = 0, int sigma_term = 1);
 hipError_t launch_reduce(int n, int kind, const void* a, const void* b, double scale, double* part, int nparts,
                          double* out, hipStream_t st);
+hipError_t launch_mdag_dot_batch(const Geom& g, const double* sm, const double* X, const double* v, int n,
+                                 double* part, double* out, hipStream_t st);
 hipError_t launch_axpy_real(int V, double alpha, const double* xv, double* y, hipStream_t st);
 hipError_t launch_scale_by_field(int V, const double* sm, const double* xv, double* y, hipStream_t st);
 
@@ -64,6 +66,9 @@ struct gnx_ctx {
   void* cgm = nullptr;          // graph-replayed CG state machine (gnx_cgm.hip)
   void* cgmix = nullptr;        // mixed-precision reliable-update CG (gnx_cgmix.hip)
   double reliable_delta = 0.1;  // residual drop that triggers an FP64 reliable update
+  double* bat_part = nullptr;   // per-vector partials of the batched trace terms
+  size_t bat_cap = 0;           // doubles in bat_part
+  double* bat_out = nullptr;    // 64 device scalars
 };
 
 

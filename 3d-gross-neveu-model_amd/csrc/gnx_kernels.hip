@@ -110,6 +110,40 @@ __global__ __launch_bounds__(BS) void k_reduce_final(int nparts, const double* _
   if (threadIdx.x == 0) *out = s * scale;
 }
 
+// ------------------------------------------------------------------ batched trace terms
+// Trace term of the condensate for a batch of noise vectors
+// (condensate.c:140-145 for each vector): y = M^dag X_k at the site (gather
+// form, reference add order) and the block's partial sum of Re(conj(v_k) y);
+// blockIdx.y = k.  part is [n][gridDim.x].
+template <int BS>
+__global__ __launch_bounds__(BS) void k_mdag_dot(Geom g, const double* __restrict__ sm,
+                                                  const double2* __restrict__ X, const double2* __restrict__ v,
+                                                  double* __restrict__ part) {
+  __shared__ double lds[BS / 64];
+  const size_t off = (size_t)blockIdx.y * g.V;
+  const int f = blockIdx.x * BS + threadIdx.x;
+  double acc = 0.0;
+  if (f < g.V) {
+    const double2* x = X + off;
+    const double2 y = apply_Mdag(g, decode(g, f), sm[f], [&](int q) { return x[q]; });
+    acc = cdot_re(v[off + f], y);
+  }
+  acc = block_sum(acc, lds);
+  if (threadIdx.x == 0) part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = acc;
+}
+
+// out[k] = scale * (sum of row k of part), one block per row, fixed order
+template <int BS>
+__global__ __launch_bounds__(BS) void k_reduce_rows(int nb, const double* __restrict__ part, double scale,
+                                                     double* __restrict__ out) {
+  __shared__ double lds[BS / 64];
+  const double* p = part + (size_t)blockIdx.x * nb;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nb; i += BS) acc += p[i];
+  const double s = block_sum(acc, lds);
+  if (threadIdx.x == 0) out[blockIdx.x] = s * scale;
+}
+
 // ------------------------------------------------------------------ elementwise
 __global__ void k_axpy_real(int V, double alpha, const double2* __restrict__ xv, double2* __restrict__ y) {
   int f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -201,6 +235,16 @@ hipError_t launch_axpy_real(int V, double alpha, const double* xv, double* y, hi
 hipError_t launch_scale_by_field(int V, const double* sm, const double* xv, double* y, hipStream_t st) {
   hipLaunchKernelGGL(k_scale_by_field, dim3(nblk(V, 256)), dim3(256), 0, st, V, sm, (const double2*)xv,
                      (double2*)y);
+  return hipGetLastError();
+}
+
+// part must hold n * ceil(V/256) doubles, out n doubles
+hipError_t launch_mdag_dot_batch(const Geom& g, const double* sm, const double* X, const double* v, int n,
+                                 double* part, double* out, hipStream_t st) {
+  constexpr int BS = 256;
+  const int nb = (g.V + BS - 1) / BS;
+  k_mdag_dot<BS><<<dim3(nb, n), BS, 0, st>>>(g, sm, (const double2*)X, (const double2*)v, part);
+  k_reduce_rows<BS><<<n, BS, 0, st>>>(nb, part, 1.0 / g.V, out);
   return hipGetLastError();
 }
 

@@ -68,6 +68,8 @@ def lib():
         L.gnx_cg_batch.argtypes = [vp, vp, vp, vp, C.c_int, C.c_double, C.c_long, C.c_int, C.c_int,
                                    C.POINTER(CGResult)]
         L.gnx_set_reliable_delta.argtypes = [vp, C.c_double]
+        L.gnx_condensate_batch.argtypes = [vp, vp, vp, C.c_int, C.c_double, C.c_long, C.c_int, vp,
+                                           C.POINTER(C.c_double), C.POINTER(CGResult)]
         L.gnx_cg_shape.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.gnx_sigma_force.argtypes = [vp, vp, vp, C.c_double, C.c_int, C.c_double, C.c_long,
                                       vp * 2, vp * 2, vp * 2, vp, CGResult * 2]
@@ -188,6 +190,16 @@ class Context:
         if st not in (0, 3):
             _check(st)
         return st, list(res)
+
+    def condensate_batch(self, sm, noise, nsrc, tol, X, max_iter=0, prec=PREC_F64):
+        """Re(v_k^dag M^-1 v_k)/V for nsrc noise vectors in one batched solve."""
+        res = (CGResult * nsrc)()
+        out = (C.c_double * nsrc)()
+        st = lib().gnx_condensate_batch(self.h, _ptr(sm), _ptr(as_real(noise)), nsrc, tol, max_iter, prec,
+                                        _ptr(as_real(X)), out, res)
+        if st not in (0, 3):
+            _check(st)
+        return st, list(out), list(res)
 
     def set_reliable_delta(self, delta):
         _check(lib().gnx_set_reliable_delta(self.h, delta))

@@ -154,6 +154,61 @@ def cpu_baseline(L, sigma, b, iters_full, budget_s=15.0):
     }
 
 
+def condensate_leg(ctx_factory, torch, gnx, spec, tol):
+    """Chiral condensate Tr M^-1 / V from N complex Z2 sources on one sigma
+    field (condensate.c:127-145 per source): one batched solve in FP64 and in
+    mixed FP32, against N single FP64 solves of the same sources.  Complex Z2
+    has E|v|^2 = 2, so each source's estimate is Re(v^dag M^-1 v) / (2V)."""
+    Lc, n = (int(v) for v in spec.split(":"))
+    V = Lc ** 3
+    sigma_h, _ = make_inputs(Lc, seed=4848)
+    rng = np.random.default_rng(4849)
+    noise_h = np.where(rng.random(2 * V * n) <= 0.5, 1.0, -1.0)
+    ctx = ctx_factory(Lc)
+    dev = ctx.device
+    sm = ctx.field()
+    ctx.sigma_mass(torch.from_numpy(sigma_h).to(dev), sm)
+    noise = torch.from_numpy(noise_h).to(dev)
+    X = ctx.vec(n)
+    out = {"workload": f"{n} complex Z2 sources at {Lc}^3, tol {tol:g}"}
+    for name, prec in (("f64", gnx.PREC_F64), ("f32", gnx.PREC_F32)):
+        ctx.condensate_batch(sm, noise, n, tol, X, prec=prec)  # tune + warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st, per, res = ctx.condensate_batch(sm, noise, n, tol, X, prec=prec)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        est = np.array(per) / 2.0
+        out[f"batched_{name}"] = {"estimate": float(est.mean()),
+                                  "noise_variance_of_mean": float(est.var(ddof=1) / n),
+                                  "seconds": dt, "rhs_solves_per_s": n / dt,
+                                  "iterations_max": max(r.iterations for r in res),
+                                  "all_converged": all(r.converged for r in res)}
+    # per-source FP64 reference: the same sources one at a time
+    x1 = ctx.vec()
+    y1 = ctx.vec()
+    per1 = []
+    t0 = time.perf_counter()
+    for k in range(n):
+        v = noise[k * 2 * V:(k + 1) * 2 * V]
+        ctx.cg(sm, v, x1, tol, start=gnx.START_IMPOSED)
+        ctx.dirac(gnx.OP_MDAG, sm, x1, y1)
+        per1.append(ctx.dot(v, y1).real / V / 2.0)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    est1 = np.array(per1)
+    out["per_source_f64"] = {"estimate": float(est1.mean()),
+                             "noise_variance_of_mean": float(est1.var(ddof=1) / n),
+                             "seconds": dt, "rhs_solves_per_s": n / dt}
+    sem = float(np.sqrt(est1.var(ddof=1) / n))
+    for name in ("f64", "f32"):
+        b = out[f"batched_{name}"]
+        b["diff_vs_per_source"] = abs(b["estimate"] - est1.mean())
+        b["diff_in_statistical_errors"] = b["diff_vs_per_source"] / sem
+        b["speedup_vs_per_source_f64"] = b["rhs_solves_per_s"] / out["per_source_f64"]["rhs_solves_per_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +219,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", default="4,16", help="batch sizes of the batched FP64 CG leg ('' to skip)")
     ap.add_argument("--mixed", default="f32,bf16", help="precisions of the mixed-precision legs ('' to skip)")
+    ap.add_argument("--condensate", default="48:64", metavar="L:N",
+                    help="batched stochastic condensate leg: lattice L^3, N complex Z2 sources ('' to skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also time a plain device copy of MIB MiB (the reference kernel for "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE scaling)")
@@ -342,6 +399,9 @@ def main():
         best = max(mixed.values(), key=lambda d: d["rhs_solves_per_s"])
         result["best_mixed_batched_rhs_solves_per_s"] = best["rhs_solves_per_s"]
         result["best_mixed_batched_speedup_vs_f64"] = best["speedup_vs_f64_single"]
+    if args.condensate:
+        result["condensate_batched"] = condensate_leg(ctx_factory=gnx.Context, torch=torch, gnx=gnx,
+                                                      spec=args.condensate, tol=args.tol)
     if args.copy_calibration > 0:
         n = args.copy_calibration * (1 << 20) // 8
         src = torch.ones(n, dtype=torch.float64, device=dev)

@@ -111,6 +111,16 @@ gnx_status gnx_cg_batch(gnx_ctx *ctx, const double *sm, const double *b, double 
                         double tol, long max_iter, gnx_cg_start start, gnx_prec prec,
                         gnx_cg_result *res);
 
+/* A12, batched.  Stochastic trace term of the condensate for nsrc noise
+ * vectors at once (condensate.c:127-145 is this for one vector at a time):
+ * noise[k] = noise + k*2V, one batched solve M M^dag X_k = v_k (imposed
+ * start, precision prec, absolute tol; X receives the solutions), then
+ *   per_source[k] = sum_x Re(conj(v_k[x]) (M^dag X_k)[x]) / V = Re(v_k^dag M^-1 v_k) / V
+ * on the device.  per_source is HOST memory [nsrc]; res[nsrc].  nsrc <= 64. */
+gnx_status gnx_condensate_batch(gnx_ctx *ctx, const double *sm, const double *noise, int nsrc, double tol,
+                                long max_iter, gnx_prec prec, double *X, double *per_source,
+                                gnx_cg_result *res);
+
 /* Mixed precision: residual reduction factor since the last FP64 reliable
  * update that triggers the next one (default 0.1). */
 gnx_status gnx_set_reliable_delta(gnx_ctx *ctx, double delta);
