@@ -52,7 +52,12 @@ __global__ void k_geometry_tables(Geom g, int* fpp, int* fmm, double* eta_pp, do
 
 hipError_t gnx::cg_run(gnx_ctx* c, const double* sm, const double2* b, double2* x, double tol, long max_iter,
                       int start, int nrhs, CGDevResult* host_res) {
-  return cgm_f64_run(&c->cgm, c->g, sm, b, x, tol, max_iter, start, nrhs, c->stream, host_res);
+  return cgm_f64_run(&c->cgm, c->g, sm, nullptr, b, x, tol, max_iter, start, nrhs, c->stream, host_res);
+}
+
+hipError_t gnx::cg_run_link(gnx_ctx* c, const double* sm, const double2* U, const double2* b, double2* x, double tol,
+                           long max_iter, int start, int nrhs, CGDevResult* host_res) {
+  return cgm_f64_run(&c->cgm, c->g, sm, U, b, x, tol, max_iter, start, nrhs, c->stream, host_res);
 }
 
 extern "C" {

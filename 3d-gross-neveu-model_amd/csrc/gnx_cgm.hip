@@ -74,6 +74,7 @@ GNX_DEVONLY void put_state(CGMState* o, int active, int iters, int conv, int ear
 struct CGMFix {
   Geom g;
   double* sm;        // copy of the caller's sigma mass
+  double2* U;        // copy of the caller's links (link-coupled operator only)
   double2* x;        // solutions [nrhs][V]
   double2* r;
   double2* p0;
@@ -121,7 +122,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_start_imposed(CGMFix a, const do
 }
 
 // null start, pass 1 (conjugategradient.c:20): t = M^dag x_in; x = x_in; sm copied
-template <int SPT, bool XCD>
+template <int SPT, bool XCD, bool LINK>
 __global__ __launch_bounds__(CGM_BS) void k_cgm_null_p1(CGMFix a, const double* __restrict__ sm_in,
                                                          const double2* __restrict__ xin) {
   const size_t off = (size_t)blockIdx.y * a.g.V;
@@ -131,7 +132,11 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_null_p1(CGMFix a, const double* 
     const int f = map.f(a.g, i);
     if (f < 0) break;
     const double2* xv = xin + off;
-    a.t[off + f] = apply_Mdag(a.g, decode(a.g, f), sm_in[f], [&](int q) { return xv[q]; });
+    if (LINK)
+      a.t[off + f] = apply_link(a.g, decode(a.g, f), 1, sm_in[f], make_double2(0.0, 0.0), a.U,
+                                [&](int q) { return xv[q]; });
+    else
+      a.t[off + f] = apply_Mdag(a.g, decode(a.g, f), sm_in[f], [&](int q) { return xv[q]; });
     a.x[off + f] = xv[f];
     if (blockIdx.y == 0) a.sm[f] = sm_in[f];
   }
@@ -139,7 +144,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_null_p1(CGMFix a, const double* 
 
 // null start, pass 2 (conjugategradient.c:19-21): r = b - M t; partials |r|^2
 // (slot 0) and |b|^2 (slot 1)
-template <int SPT, bool XCD>
+template <int SPT, bool XCD, bool LINK>
 __global__ __launch_bounds__(CGM_BS) void k_cgm_null_p2(CGMFix a, const double2* __restrict__ b) {
   __shared__ double lds[CGM_BS / 64];
   const int rhs = blockIdx.y;
@@ -152,7 +157,8 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_null_p2(CGMFix a, const double2*
     if (f < 0) break;
     const double2* t = a.t + off;
     double2 bb = b[off + f];
-    double2 rr = apply_sub_M(a.g, decode(a.g, f), a.sm[f], bb, [&](int q) { return t[q]; });
+    double2 rr = LINK ? apply_link(a.g, decode(a.g, f), 2, a.sm[f], bb, a.U, [&](int q) { return t[q]; })
+                      : apply_sub_M(a.g, decode(a.g, f), a.sm[f], bb, [&](int q) { return t[q]; });
     a.r[off + f] = rr;
     ar += cnorm2(rr);
     ab += cnorm2(bb);
@@ -201,7 +207,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_null_p3(CGMFix a, const double2*
 // sin/sout: this kernel's input and output state halves; pin/pout: the slot
 // it reduces and the slot it writes (separate restrict pointers, so the
 // compiler may keep the reads ahead of the writes)
-template <int SPT, bool XCD>
+template <int SPT, bool XCD, bool LINK>
 __global__ __launch_bounds__(CGM_BS) void k_cgm_A(CGMFix a, int odd, const CGMState* __restrict__ sin,
                                                    CGMState* __restrict__ sout, const double* __restrict__ pin,
                                                    double* __restrict__ pout) {
@@ -220,6 +226,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_A(CGMFix a, int odd, const CGMSt
   int fs[SPT];
   double smv[SPT];
   double2 rv[SPT][7], pv[SPT][7];
+  double2 Uo[LINK ? SPT : 1][3], Um[LINK ? SPT : 1][3];
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
     fs[i] = map.f(g, i);
@@ -227,6 +234,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_A(CGMFix a, int odd, const CGMSt
     NbrF n = neighbours_f(g, decode(g, fs[i]));
     int q[7] = {fs[i], n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
     smv[i] = a.sm[fs[i]];
+    if (LINK) load_links(a.U, fs[i], n, Uo[LINK ? i : 0], Um[LINK ? i : 0]);
 #pragma unroll
     for (int k = 0; k < 7; k++) {
       rv[i][k] = r[q[k]];
@@ -278,7 +286,8 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_A(CGMFix a, int odd, const CGMSt
       pmv[d] = cadd(rv[i][4 + d], rmul(beta, pv[i][4 + d]));
     }
     st_nt(&pnew[fs[i]], p0v);
-    double2 tt = apply_Mdag_vals(g, st, smv[i], p0v, ppv, pmv);
+    double2 tt = LINK ? link_hops<-1>(g, st, rmul(smv[i], p0v), ppv, pmv, Uo[LINK ? i : 0], Um[LINK ? i : 0])
+                      : apply_Mdag_vals(g, st, smv[i], p0v, ppv, pmv);
     st_nt(&a.t[off + fs[i]], tt);
     acc += cnorm2(tt);
   }
@@ -287,7 +296,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_A(CGMFix a, int odd, const CGMSt
   if (lead) put_state(out, 1, s.iters, 0, 0, rho, s);
 }
 
-template <int SPT, bool XCD>
+template <int SPT, bool XCD, bool LINK>
 __global__ __launch_bounds__(CGM_BS) void k_cgm_B(CGMFix a, int odd, const CGMState* __restrict__ sin,
                                                    CGMState* __restrict__ sout, const double* __restrict__ pin,
                                                    double* __restrict__ pout) {
@@ -309,7 +318,9 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_B(CGMFix a, int odd, const CGMSt
   for (int i = 0; i < SPT; i++) {
     fs[i] = map.f(g, i);
     if (fs[i] < 0) continue;
-    w[i] = apply_M(g, decode(g, fs[i]), a.sm[fs[i]], [&](int q) { return t[q]; });
+    w[i] = LINK ? apply_link(g, decode(g, fs[i]), 0, a.sm[fs[i]], make_double2(0.0, 0.0), a.U,
+                             [&](int q) { return t[q]; })
+                : apply_M(g, decode(g, fs[i]), a.sm[fs[i]], [&](int q) { return t[q]; });
     pv[i] = pnew[fs[i]];
     xv[i] = x[fs[i]];
     rv[i] = r[fs[i]];
@@ -339,6 +350,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_B(CGMFix a, int odd, const CGMSt
 struct CGMShape {
   int spt;
   bool xcd;
+  bool link;
   const void* A;
   const void* B;
   const void* start_imposed;
@@ -347,13 +359,18 @@ struct CGMShape {
   const void* null_p3;
 };
 
-#define GNX_SHAPE(S, X)                                                                               \
-  {S, X, (const void*)k_cgm_A<S, X>, (const void*)k_cgm_B<S, X>, (const void*)k_cgm_start_imposed<S, X>, \
-   (const void*)k_cgm_null_p1<S, X>, (const void*)k_cgm_null_p2<S, X>, (const void*)k_cgm_null_p3<S, X>}
-static const CGMShape kShapes[] = {GNX_SHAPE(1, false), GNX_SHAPE(1, true), GNX_SHAPE(2, false),
-                                   GNX_SHAPE(2, true),  GNX_SHAPE(4, false), GNX_SHAPE(4, true)};
+#define GNX_SHAPE(S, X, K)                                                                                  \
+  {S, X, K, (const void*)k_cgm_A<S, X, K>, (const void*)k_cgm_B<S, X, K>, (const void*)k_cgm_start_imposed<S, X>, \
+   (const void*)k_cgm_null_p1<S, X, K>, (const void*)k_cgm_null_p2<S, X, K>, (const void*)k_cgm_null_p3<S, X>}
+// shapes 0..5: the live operator; 6..11: the link-coupled operator
+static const CGMShape kShapes[] = {
+    GNX_SHAPE(1, false, false), GNX_SHAPE(1, true, false), GNX_SHAPE(2, false, false),
+    GNX_SHAPE(2, true, false),  GNX_SHAPE(4, false, false), GNX_SHAPE(4, true, false),
+    GNX_SHAPE(1, false, true),  GNX_SHAPE(1, true, true),   GNX_SHAPE(2, false, true),
+    GNX_SHAPE(2, true, true),   GNX_SHAPE(4, false, true),  GNX_SHAPE(4, true, true)};
 #undef GNX_SHAPE
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
+constexpr int kShapesPerOp = kNumShapes / 2;
 
 static int shape_blocks(const Geom& g, int shape) {
   int per = CGM_BS * kShapes[shape].spt;
@@ -364,6 +381,7 @@ static int shape_blocks(const Geom& g, int shape) {
 struct CGMachine {
   GraphEngine eng;
   double* sm = nullptr;
+  double2* U = nullptr;  // [3V] link copy (link-coupled solves)
   double2* x = nullptr;
   double2* r = nullptr;
   double2* p0 = nullptr;
@@ -371,12 +389,13 @@ struct CGMachine {
   double2* t = nullptr;
   CGDevResult* res = nullptr;
   long cap = 0;          // complex elements per vector
-  int forced_shape = -1; // GNX_CG_SHAPE
+  int forced_shape = -1; // GNX_CG_SHAPE (index within an operator's shapes)
   void free_vectors() {
-    void* bufs[] = {sm, x, r, p0, p1, t, res};
+    void* bufs[] = {sm, U, x, r, p0, p1, t, res};
     for (void* p : bufs)
       if (p) (void)hipFree(p);
     sm = nullptr;
+    U = nullptr;
     x = r = p0 = p1 = t = nullptr;
     res = nullptr;
     cap = 0;
@@ -392,7 +411,7 @@ static CGMachine* cgm_get(void** slotp) {
     const char* env = getenv("GNX_CG_SHAPE");
     if (env) {
       int v = atoi(env);
-      if (v >= 0 && v < kNumShapes) m->forced_shape = v;
+      if (v >= 0 && v < kShapesPerOp) m->forced_shape = v;
     }
     *slotp = m;
   }
@@ -409,6 +428,7 @@ static hipError_t cgm_ensure(CGMachine* m, const Geom& g, int nrhs) {
   m->free_vectors();
   m->eng.invalidate_graphs();  // captured arguments point at the old vectors
   e = hipMalloc(&m->sm, sizeof(double) * g.V);
+  if (e == hipSuccess) e = hipMalloc(&m->U, sizeof(double2) * 3 * (size_t)g.V);
   if (e == hipSuccess) e = hipMalloc(&m->x, sizeof(double2) * need);
   if (e == hipSuccess) e = hipMalloc(&m->r, sizeof(double2) * need);
   if (e == hipSuccess) e = hipMalloc(&m->p0, sizeof(double2) * need);
@@ -427,6 +447,7 @@ static CGMFix cgm_fix(CGMachine* m, const Geom& g, int nrhs, int shape) {
   CGMFix a;
   a.g = g;
   a.sm = m->sm;
+  a.U = m->U;
   a.x = m->x;
   a.r = m->r;
   a.p0 = m->p0;
@@ -465,11 +486,16 @@ static hipError_t cgm_graph(CGMachine* m, const Geom& g, int nrhs, int shape, hi
 }
 
 // run the start kernels of one solve
-static hipError_t cgm_begin(CGMachine* m, const Geom& g, const double* sm, const double2* b, const double2* xin,
-                            double tol, long max_iter, int start, int nrhs, int shape, hipStream_t st) {
+static hipError_t cgm_begin(CGMachine* m, const Geom& g, const double* sm, const double2* U, const double2* b,
+                            const double2* xin, double tol, long max_iter, int start, int nrhs, int shape,
+                            hipStream_t st) {
   CGMFix a = cgm_fix(m, g, nrhs, shape);
   dim3 grid(a.nb, nrhs), blk(CGM_BS);
   hipError_t e;
+  if (kShapes[shape].link) {
+    e = hipMemcpyAsync(m->U, U, sizeof(double2) * 3 * (size_t)g.V, hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return e;
+  }
   if (start == GNX_START_NULL) {
     void* a1[] = {(void*)&a, (void*)&sm, (void*)&xin};
     e = hipLaunchKernel(kShapes[shape].null_p1, grid, blk, a1, 0, st);
@@ -487,26 +513,29 @@ static hipError_t cgm_begin(CGMachine* m, const Geom& g, const double* sm, const
 // Autotuner: time every launch shape on 2*CHUNK iterations of the caller's
 // own system (imposed start into the solver's buffers; the caller's x is not
 // touched) and keep the fastest per batch size.
-static hipError_t cgm_tune(CGMachine* m, const Geom& g, const double* sm, const double2* b, int nrhs,
-                           hipStream_t st, int* best) {
+static hipError_t cgm_tune(CGMachine* m, const Geom& g, const double* sm, const double2* U, const double2* b,
+                           int nrhs, hipStream_t st, int* best) {
   GraphEngine& E = m->eng;
+  const bool link = U != nullptr;
+  const int key = nrhs + (link ? 1000 : 0);
+  const int first = link ? kShapesPerOp : 0;
   for (int i = 0; i < 8; i++)
-    if (E.tuned_tag[i] >= 0 && E.tuned_nrhs[i] == nrhs) {
+    if (E.tuned_tag[i] >= 0 && E.tuned_nrhs[i] == key) {
       *best = E.tuned_tag[i];
       return hipSuccess;
     }
-  int best_shape = 0;
+  int best_shape = first;
   hipError_t e = hipSuccess;
   if (m->forced_shape >= 0) {
-    best_shape = m->forced_shape;
+    best_shape = first + m->forced_shape;
   } else {
     float best_ms = 1e30f;
-    for (int sh = 0; sh < kNumShapes && e == hipSuccess; sh++) {
+    for (int sh = first; sh < first + kShapesPerOp && e == hipSuccess; sh++) {
       hipGraphExec_t exec;
       e = cgm_graph(m, g, nrhs, sh, &exec);
       float ms_min = 1e30f;
       for (int rep = 0; rep < 3 && e == hipSuccess; rep++) {
-        e = cgm_begin(m, g, sm, b, nullptr, 0.0, 2 * ENG_CHUNK, GNX_START_IMPOSED, nrhs, sh, st);
+        e = cgm_begin(m, g, sm, U, b, nullptr, 0.0, 2 * ENG_CHUNK, GNX_START_IMPOSED, nrhs, sh, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         float ms = 0.f;
         if (e == hipSuccess) e = E.time_replays(exec, 2, st, &ms);
@@ -522,25 +551,26 @@ static hipError_t cgm_tune(CGMachine* m, const Geom& g, const double* sm, const 
   for (int i = 0; i < 8; i++)
     if (E.tuned_tag[i] < 0) {
       E.tuned_tag[i] = best_shape;
-      E.tuned_nrhs[i] = nrhs;
+      E.tuned_nrhs[i] = key;
       break;
     }
   *best = best_shape;
   return hipSuccess;
 }
 
-hipError_t cgm_f64_run(void** machine_slot, const Geom& g, const double* sm, const double2* b, double2* x,
-                       double tol, long max_iter, int start, int nrhs, hipStream_t st, CGDevResult* host_res) {
+hipError_t cgm_f64_run(void** machine_slot, const Geom& g, const double* sm, const double2* U, const double2* b,
+                       double2* x, double tol, long max_iter, int start, int nrhs, hipStream_t st,
+                       CGDevResult* host_res) {
   CGMachine* m = cgm_get(machine_slot);
   hipError_t e = cgm_ensure(m, g, nrhs);
   if (e != hipSuccess) return e;
   int shape = 0;
-  e = cgm_tune(m, g, sm, b, nrhs, st, &shape);
+  e = cgm_tune(m, g, sm, U, b, nrhs, st, &shape);
   if (e != hipSuccess) return e;
   hipGraphExec_t exec;
   e = cgm_graph(m, g, nrhs, shape, &exec);
   if (e != hipSuccess) return e;
-  e = cgm_begin(m, g, sm, b, x, tol, max_iter, start, nrhs, shape, st);
+  e = cgm_begin(m, g, sm, U, b, x, tol, max_iter, start, nrhs, shape, st);
   if (e != hipSuccess) return e;
   e = m->eng.run(exec, nrhs, max_iter, st, nullptr);
   if (e != hipSuccess) return e;

@@ -39,8 +39,10 @@ struct CGDevResult {
   double pad[3];
 };
 
-hipError_t cgm_f64_run(void** machine_slot, const Geom& g, const double* sm, const double2* b, double2* x,
-                       double tol, long max_iter, int start, int nrhs, hipStream_t st, CGDevResult* host_res);
+// U: links in f-order [3V] for the link-coupled operator, nullptr for the live one
+hipError_t cgm_f64_run(void** machine_slot, const Geom& g, const double* sm, const double2* U, const double2* b,
+                       double2* x, double tol, long max_iter, int start, int nrhs, hipStream_t st,
+                       CGDevResult* host_res);
 void cgm_release(void* machine);
 hipError_t cgmix_run(void** machine_slot, const Geom& g, const double* sm, const double2* b, double2* x, double tol,
                      double delta, long max_iter, int start, int prec, int nrhs, hipStream_t st,
@@ -76,6 +78,9 @@ namespace gnx {
 // FP64 CG on the context's engine; host_res[nrhs] filled on return
 hipError_t cg_run(gnx_ctx* c, const double* sm, const double2* b, double2* x, double tol, long max_iter, int start,
                   int nrhs, CGDevResult* host_res);
+// the same on the link-coupled operator (links f-order [3V])
+hipError_t cg_run_link(gnx_ctx* c, const double* sm, const double2* U, const double2* b, double2* x, double tol,
+                       long max_iter, int start, int nrhs, CGDevResult* host_res);
 }
 
 #define HIP_TRY(expr)                                                                   \

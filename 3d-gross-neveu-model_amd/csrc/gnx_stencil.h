@@ -159,6 +159,66 @@ GNX_DEV T2 apply_sub_Mdag(const Geom& g, const Site& s, typename CT<T2>::R sm, T
   return accumulate_adj_order(g, s, csub(r, rmul(sm, load(f))), tpp, tmm);
 }
 
+// ------------------------------------------------------------------ link-coupled operator
+// The link-coupled staggered operator of the reference's commented-out
+// copy_dirac.c:46-123 (the live dirac.c ignores the links):
+//   M_U v(x)   = sm v(x) + sum_d [ (0.5 eta_pp U_d(x)) v(x+d) - (0.5 eta_mm conj U_d(x-d)) v(x-d) ]
+//   M_U^+ v(x) = sm v(x) - sum_d [ same two terms ]   (gather form, d = 0,1,2)
+// Links are U(1) phases stored in f-order, three per site: U[3 f + d] =
+// U_d(site f).  Complex products use the term order of C99 complex
+// multiplication, (ac - bd, ad + bc), with no fused multiply-add.
+GNX_DEV double2 cmul(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+GNX_DEV double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+
+// hop terms of site s onto acc: SIGN = +1 for M_U, -1 for M_U^+ and r -= M_U v
+// vpp[d] = v(x+d), vmm[d] = v(x-d), Uo[d] = U_d(x), Um[d] = U_d(x-d)
+template <int SIGN>
+GNX_DEV double2 link_hops(const Geom& g, const Site& s, double2 acc, const double2 vpp[3], const double2 vmm[3],
+                          const double2 Uo[3], const double2 Um[3]) {
+  HalfEta e = half_eta(g, s);
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double2 tp = cmul(rmul(e.pp[d], Uo[d]), vpp[d]);
+    const double2 tm = cmul(rmul(e.mm[d], cconj(Um[d])), vmm[d]);
+    if (SIGN > 0) {
+      acc = cadd(acc, tp);
+      acc = csub(acc, tm);
+    } else {
+      acc = csub(acc, tp);
+      acc = cadd(acc, tm);
+    }
+  }
+  return acc;
+}
+
+// links of site s: own three and the three of its mm neighbours
+GNX_DEV void load_links(const double2* __restrict__ U, int f, const NbrF& n, double2 Uo[3], double2 Um[3]) {
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    Uo[d] = U[3 * f + d];
+    Um[d] = U[3 * n.mm[d] + d];
+  }
+}
+
+// op: 0 = M_U v, 1 = M_U^+ v, 2 = r - M_U v
+template <typename LoadF>
+GNX_DEV double2 apply_link(const Geom& g, const Site& s, int op, double sm, double2 r, const double2* __restrict__ U,
+                           LoadF load) {
+  NbrF n = neighbours_f(g, s);
+  const int f = f_of_lex(g, s.lex, s.P);
+  double2 vpp[3], vmm[3], Uo[3], Um[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    vpp[d] = load(n.pp[d]);
+    vmm[d] = load(n.mm[d]);
+  }
+  load_links(U, f, n, Uo, Um);
+  const double2 m0 = rmul(sm, load(f));
+  if (op == 0) return link_hops<1>(g, s, m0, vpp, vmm, Uo, Um);
+  if (op == 1) return link_hops<-1>(g, s, m0, vpp, vmm, Uo, Um);
+  return link_hops<-1>(g, s, csub(r, m0), vpp, vmm, Uo, Um);
+}
+
 // ------------------------------------------------------------------ sigma sector
 // f-index of the site with coordinates (c0,c1,c2)
 GNX_DEV int f_of_coords(const Geom& g, int c0, int c1, int c2) {

@@ -92,6 +92,26 @@ def lib():
         L.gnx_hmc_condensate.argtypes = [vp, dp, C.c_double, C.c_long, dp]
         L.gnx_hmc_condensate_sigma.argtypes = [vp, dp]
         L.gnx_hmc_cg_stats.argtypes = [vp, dp, C.POINTER(C.c_int)]
+        L.gnx_links_from_lex.argtypes = [vp, vp, vp]
+        L.gnx_links_to_lex.argtypes = [vp, vp, vp]
+        L.gnx_link_dirac.argtypes = [vp, C.c_int, vp, vp, vp, vp]
+        L.gnx_link_cg_batch.argtypes = [vp, vp, vp, vp, vp, C.c_int, C.c_double, C.c_long, C.c_int,
+                                        C.POINTER(CGResult)]
+        L.gnx_gauge_force.argtypes = [vp, vp, C.c_double, vp]
+        L.gnx_link_force_from_solutions.argtypes = [vp, vp, vp, vp, vp, vp, vp, C.c_int]
+        L.gnx_plaquette_sum.argtypes = [vp, vp, dp]
+        L.gnx_link_hmc_create.argtypes = [vp, C.c_double, C.c_int, C.c_int, C.POINTER(vp)]
+        L.gnx_link_hmc_destroy.argtypes = [vp]
+        L.gnx_link_hmc_set_fields.argtypes = [vp, dp, dp]
+        L.gnx_link_hmc_set_solutions.argtypes = [vp, dp, dp]
+        L.gnx_link_hmc_get_links.argtypes = [vp, dp]
+        L.gnx_link_hmc_device_fields.argtypes = [vp] + [C.POINTER(vp)] * 7
+        L.gnx_link_hmc_compute_force.argtypes = [vp, C.c_double, C.c_long, C.POINTER(C.c_int)]
+        L.gnx_link_hmc_trajectory.argtypes = [vp, dp, dp, dp, C.c_double, C.c_int, C.c_double, C.c_long,
+                                              C.POINTER(TrajResult)]
+        L.gnx_link_hmc_accept.argtypes = [vp, C.c_int]
+        L.gnx_link_hmc_energy.argtypes = [vp, dp]
+        L.gnx_link_hmc_cg_stats.argtypes = [vp, dp, C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -200,6 +220,37 @@ class Context:
         if st not in (0, 3):
             _check(st)
         return st, list(out), list(res)
+
+    # -- A11: link sector (links f-order [V][3] complex on the device) --
+    def links_from_lex(self, ulex, uf):
+        _check(lib().gnx_links_from_lex(self.h, _ptr(as_real(ulex)), _ptr(as_real(uf))))
+
+    def links_to_lex(self, uf, ulex):
+        _check(lib().gnx_links_to_lex(self.h, _ptr(as_real(uf)), _ptr(as_real(ulex))))
+
+    def link_dirac(self, op, sm, uf, vin, vout):
+        _check(lib().gnx_link_dirac(self.h, op, _ptr(sm), _ptr(as_real(uf)), _ptr(as_real(vin)), _ptr(as_real(vout))))
+
+    def link_cg_batch(self, sm, uf, b, x, nrhs, tol, max_iter=0, start=START_NULL):
+        res = (CGResult * nrhs)()
+        st = lib().gnx_link_cg_batch(self.h, _ptr(sm), _ptr(as_real(uf)), _ptr(as_real(b)), _ptr(as_real(x)), nrhs,
+                                     tol, max_iter, start, res)
+        if st not in (0, 3):
+            _check(st)
+        return st, list(res)
+
+    def gauge_force(self, uf, beta, force):
+        _check(lib().gnx_gauge_force(self.h, _ptr(as_real(uf)), beta, _ptr(force)))
+
+    def link_force_from_solutions(self, uf, X0, Y0, X1, Y1, force, accumulate=False):
+        _check(lib().gnx_link_force_from_solutions(self.h, _ptr(as_real(uf)), _ptr(as_real(X0)), _ptr(as_real(Y0)),
+                                                   _ptr(as_real(X1)), _ptr(as_real(Y1)), _ptr(force),
+                                                   int(accumulate)))
+
+    def plaquette_sum(self, uf):
+        o = C.c_double()
+        _check(lib().gnx_plaquette_sum(self.h, _ptr(as_real(uf)), C.byref(o)))
+        return o.value
 
     def set_reliable_delta(self, delta):
         _check(lib().gnx_set_reliable_delta(self.h, delta))
@@ -323,4 +374,81 @@ class HMC:
         it = C.c_double()
         calls = C.c_int()
         _check(lib().gnx_hmc_cg_stats(self.h, C.byref(it), C.byref(calls)))
+        return it.value, calls.value
+
+
+class LinkHMC:
+    """Device-resident link HMC (gnx_link_hmc_*) at fixed sigma mass."""
+
+    def __init__(self, ctx, beta=1.0, pure_gauge=True, linked=True):
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(lib().gnx_link_hmc_create(ctx.h, beta, int(pure_gauge), int(linked), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gnx_link_hmc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_fields(self, ulex=None, sm=None):
+        """ulex: complex128[3V] (lex order), sm: float64[V], host arrays."""
+        import numpy as np
+        a = b = None
+        pa = pb = None
+        if ulex is not None:
+            a, pa = _dptr(np.ascontiguousarray(ulex, np.complex128).view(np.float64))
+        if sm is not None:
+            b, pb = _dptr(sm)
+        _check(lib().gnx_link_hmc_set_fields(self.h, pa, pb))
+
+    def set_solutions(self, X0, X1):
+        a, pa = _dptr(X0)
+        b, pb = _dptr(X1)
+        _check(lib().gnx_link_hmc_set_solutions(self.h, pa, pb))
+
+    def get_links(self):
+        import numpy as np
+        out = np.zeros(6 * self.ctx.V)
+        _check(lib().gnx_link_hmc_get_links(self.h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out.view(np.complex128)
+
+    def device_fields(self):
+        ps = [C.c_void_p() for _ in range(7)]
+        _check(lib().gnx_link_hmc_device_fields(self.h, *[C.byref(p) for p in ps]))
+        return [p.value for p in ps]
+
+    def compute_force(self, tol, max_iter=0):
+        ok = C.c_int()
+        _check(lib().gnx_link_hmc_compute_force(self.h, tol, max_iter, C.byref(ok)))
+        return bool(ok.value)
+
+    def trajectory(self, mom, chi0, chi1, eps, steps, tol, max_iter=0):
+        r = TrajResult()
+        a, pa = _dptr(mom)
+        b, pb = _dptr(chi0)
+        c, pc = _dptr(chi1)
+        st = lib().gnx_link_hmc_trajectory(self.h, pa, pb, pc, eps, steps, tol, max_iter, C.byref(r))
+        if st not in (0, 3):
+            _check(st)
+        return st, r
+
+    def accept(self, accepted):
+        _check(lib().gnx_link_hmc_accept(self.h, int(accepted)))
+
+    def energy(self):
+        o = C.c_double()
+        _check(lib().gnx_link_hmc_energy(self.h, C.byref(o)))
+        return o.value
+
+    def cg_stats(self):
+        it = C.c_double()
+        calls = C.c_int()
+        _check(lib().gnx_link_hmc_cg_stats(self.h, C.byref(it), C.byref(calls)))
         return it.value, calls.value

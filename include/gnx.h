@@ -213,6 +213,64 @@ gnx_status gnx_hmc_condensate_sigma(gnx_hmc *h, double *out);
 /* accumulated CG statistics (kit->iterations, kit->CG_call) */
 gnx_status gnx_hmc_cg_stats(gnx_hmc *h, double *iterations, int *calls);
 
+/* ------------------------------------------------------------------------
+ * A11: the U(1) link sector.  The reference's live operator (dirac.c)
+ * ignores the links; the link-coupled operator is the commented-out
+ * copy_dirac.c:46-123:
+ *   M_U v(x) = sm v(x) + sum_d [0.5 eta_pp(x,d) U_d(x) v(x+d) - 0.5 eta_mm(x,d) conj(U_d(x-d)) v(x-d)]
+ * and M_U^dag the same with the hop terms negated.  On the device, links are
+ * f-order with three per site, uf[2*(3 f + d)] = (Re, Im) U_d(f); the
+ * reference's dlink is lex order [V][3] (lattice.h:15), converted by
+ * gnx_links_from_lex / gnx_links_to_lex.  Momenta and forces are f-order
+ * [V][3] (moleculardynamics.c:389).  No reference path reaches this code,
+ * so parity is pinned by invariants (gauge covariance, U = 1 reduction,
+ * finite differences), not by reference outputs.
+ * ------------------------------------------------------------------------ */
+gnx_status gnx_links_from_lex(gnx_ctx *ctx, const double *ulex, double *uf);
+gnx_status gnx_links_to_lex(gnx_ctx *ctx, const double *uf, double *ulex);
+/* op: GNX_OP_M (copy_dirac.c:46-82), GNX_OP_MDAG (:88-123), GNX_OP_SUB_M (out -= M_U in) */
+gnx_status gnx_link_dirac(gnx_ctx *ctx, gnx_op op, const double *sm, const double *uf, const double *in,
+                          double *out);
+/* FP64 CG on M_U M_U^dag, conjugategradient.c semantics (null or imposed start) */
+gnx_status gnx_link_cg_batch(gnx_ctx *ctx, const double *sm, const double *uf, const double *b, double *x,
+                             int nrhs, double tol, long max_iter, gnx_cg_start start, gnx_cg_result *res);
+/* AddForceFromPureGaugeSector (moleculardynamics.c:127-138, linklattice.c:146-172): overwrites force */
+gnx_status gnx_gauge_force(gnx_ctx *ctx, const double *uf, double beta, double *force);
+/* AddForceFromFermionSectorRespectToLink after its solves (moleculardynamics.c:181-190):
+ * force (+)= sum_f Re(...) with X_f, Y_f = M^dag X_f; accumulate = 0 starts from zero */
+gnx_status gnx_link_force_from_solutions(gnx_ctx *ctx, const double *uf, const double *X0, const double *Y0,
+                                         const double *X1, const double *Y1, double *force, int accumulate);
+/* sum_x sum_{mu > nu} Re Pi_{mu nu}(x) (TotalRealTracePlaquette, linklattice.c:62-80) */
+gnx_status gnx_plaquette_sum(gnx_ctx *ctx, const double *uf, double *out);
+
+/* Device-resident LeapFrog2Link (moleculardynamics.c:419-542) at fixed sigma
+ * mass.  pure_gauge adds -beta sum Re Pi to the action (PURE_GAUGE,
+ * macro.h:47); linked = 0 solves with the live operator as the reference
+ * does, linked = 1 with M_U.  The caller draws (3V/2 Gauss2MeanVariance
+ * momentum pairs, then chi_0, chi_1: V x Gauss1 each) and decides the
+ * Metropolis test with gnx_link_hmc_accept. */
+typedef struct gnx_link_hmc gnx_link_hmc;
+gnx_status gnx_link_hmc_create(gnx_ctx *ctx, double beta, int pure_gauge, int linked, gnx_link_hmc **out);
+gnx_status gnx_link_hmc_destroy(gnx_link_hmc *h);
+/* host links in lex order [V][3] complex (NULL: keep) and sigma mass [V] (NULL: keep) */
+gnx_status gnx_link_hmc_set_fields(gnx_link_hmc *h, const double *ulex_host, const double *sm_host);
+/* starting guesses X_0, X_1 (kit->inverted_phi) for the next null-start solves */
+gnx_status gnx_link_hmc_set_solutions(gnx_link_hmc *h, const double *X0_host, const double *X1_host);
+gnx_status gnx_link_hmc_get_links(gnx_link_hmc *h, double *ulex_host);
+/* device pointers: links (f-order), sm, momenta [3V], force [3V], X, Y, phi [2][2V] */
+gnx_status gnx_link_hmc_device_fields(gnx_link_hmc *h, double **uf, double **sm, double **mom, double **force,
+                                      double **X, double **Y, double **phi);
+/* ComputeForce (moleculardynamics.c:203-222) on the current fields; *ok = 0 if a solve failed */
+gnx_status gnx_link_hmc_compute_force(gnx_link_hmc *h, double tol, long max_iter, int *ok);
+gnx_status gnx_link_hmc_trajectory(gnx_link_hmc *h, const double *mom_host, const double *chi0_host,
+                                   const double *chi1_host, double eps, int steps, double tol, long max_iter,
+                                   gnx_traj_result *res);
+/* accepted = 0 restores the links of the trajectory's start (:518-520) */
+gnx_status gnx_link_hmc_accept(gnx_link_hmc *h, int accepted);
+/* ComputeFullEnergyWithPhiWithoutSigma (moleculardynamics.c:268-294) on the current fields */
+gnx_status gnx_link_hmc_energy(gnx_link_hmc *h, double *H);
+gnx_status gnx_link_hmc_cg_stats(gnx_link_hmc *h, double *iterations, int *calls);
+
 #ifdef __cplusplus
 }
 #endif

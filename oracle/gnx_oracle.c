@@ -290,6 +290,63 @@ void orc_sub_adj_dirac(const orc_geom *g, const double *sm, const double *in, do
     }
 }
 
+/* ----------------------------------------------------------------------- */
+/* link-coupled operator: the commented-out copy_dirac.c:46-123 (the live    */
+/* operator, dirac.c, ignores the links).  U is the reference's dlink layout: */
+/* lex order [V][3] complex (lattice.h:15); the mm link of f-site x is the    */
+/* link of its mm neighbour, lat->mm[COOR(site[x], d)].                       */
+/* ----------------------------------------------------------------------- */
+static inline int lex_of_f(const orc_geom *g, int f) { return g->f2lex[f]; }
+
+/* copy_dirac.c:46-82 */
+void orc_link_dirac(const orc_geom *g, const double *sm, const double *Ud, const double *in, double *out)
+{
+    const cplx *v = (const cplx *)in;
+    const cplx *U = (const cplx *)Ud;
+    cplx *t = (cplx *)out;
+    int V = g->V;
+    for (int x = 0; x < V; x++) t[x] = (sm[x]) * v[x];
+    for (int x = 0; x < V; x++)
+    for (int d = 0; d < 3; d++) {
+        int fp = g->fpp[3 * x + d], fm = g->fmm[3 * x + d];
+        t[x] += (0.5) * g->eta_pp[3 * x + d] * U[3 * lex_of_f(g, x) + d] * v[fp];
+        t[x] -= (0.5) * g->eta_mm[3 * x + d] * conj(U[3 * lex_of_f(g, fm) + d]) * v[fm];
+    }
+}
+
+/* copy_dirac.c:88-123 (gather form, D^dag = -D) */
+void orc_link_adj_dirac(const orc_geom *g, const double *sm, const double *Ud, const double *in, double *out)
+{
+    const cplx *v = (const cplx *)in;
+    const cplx *U = (const cplx *)Ud;
+    cplx *t = (cplx *)out;
+    int V = g->V;
+    for (int x = 0; x < V; x++) t[x] = (sm[x]) * v[x];
+    for (int x = 0; x < V; x++)
+    for (int d = 0; d < 3; d++) {
+        int fp = g->fpp[3 * x + d], fm = g->fmm[3 * x + d];
+        t[x] -= (0.5) * g->eta_pp[3 * x + d] * U[3 * lex_of_f(g, x) + d] * v[fp];
+        t[x] += (0.5) * g->eta_mm[3 * x + d] * conj(U[3 * lex_of_f(g, fm) + d]) * v[fm];
+    }
+}
+
+/* out -= M_U in: the terms of copy_dirac.c:46-82 with flipped signs (the CG
+ * prologue, conjugategradient.c:21, on the link-coupled operator) */
+void orc_link_sub_dirac(const orc_geom *g, const double *sm, const double *Ud, const double *in, double *out)
+{
+    const cplx *v = (const cplx *)in;
+    const cplx *U = (const cplx *)Ud;
+    cplx *t = (cplx *)out;
+    int V = g->V;
+    for (int x = 0; x < V; x++) t[x] -= (sm[x]) * v[x];
+    for (int x = 0; x < V; x++)
+    for (int d = 0; d < 3; d++) {
+        int fp = g->fpp[3 * x + d], fm = g->fmm[3 * x + d];
+        t[x] -= (0.5) * g->eta_pp[3 * x + d] * U[3 * lex_of_f(g, x) + d] * v[fp];
+        t[x] += (0.5) * g->eta_mm[3 * x + d] * conj(U[3 * lex_of_f(g, fm) + d]) * v[fm];
+    }
+}
+
 /* fermion.h:67 */
 double orc_norm(int V, const double *in)
 {
@@ -308,18 +365,42 @@ static inline void axpy_c(int V, const cplx *v, cplx coeff, cplx *target)
 /* ======================================================================= */
 /* CG on A = M M^dag (conjugategradient.c)                                  */
 /* ======================================================================= */
-static bool cg_loop(const orc_geom *g, const double *sm, cplx *X, cplx *r, cplx *p,
+/* the operator a solve runs on: the live sigma-only M (U == NULL) or the
+ * link-coupled M_U of copy_dirac.c */
+typedef struct {
+    const orc_geom *g;
+    const double *sm;
+    const double *U;
+} orc_op;
+
+static void op_M(const orc_op *o, const double *in, double *out)
+{
+    if (o->U) orc_link_dirac(o->g, o->sm, o->U, in, out);
+    else orc_dirac(o->g, o->sm, in, out);
+}
+static void op_Mdag(const orc_op *o, const double *in, double *out)
+{
+    if (o->U) orc_link_adj_dirac(o->g, o->sm, o->U, in, out);
+    else orc_adj_dirac(o->g, o->sm, in, out);
+}
+static void op_subM(const orc_op *o, const double *in, double *out)
+{
+    if (o->U) orc_link_sub_dirac(o->g, o->sm, o->U, in, out);
+    else orc_sub_dirac(o->g, o->sm, in, out);
+}
+
+static bool cg_loop(const orc_op *o, cplx *X, cplx *r, cplx *p,
                     cplx *t1, cplx *t2, double tol, long max_iter, orc_cgstat *st)
 {
-    int V = g->V;
+    int V = o->g->V;
     for (long iteration = 0; iteration < max_iter; iteration++) {
         double alpha_k = 0.0, norm_r_k = 0.0, norm_p_k = 0.0;
         for (int x = 0; x < V; x++) norm_r_k += pow(cabs(r[x]), 2.0);
-        orc_adj_dirac(g, sm, (const double *)p, (double *)t1);
+        op_Mdag(o, (const double *)p, (double *)t1);
         for (int x = 0; x < V; x++) norm_p_k += pow(cabs(t1[x]), 2.0);
         alpha_k = norm_r_k / norm_p_k;
         axpy_c(V, p, alpha_k, X);
-        orc_dirac(g, sm, (const double *)t1, (double *)t2);
+        op_M(o, (const double *)t1, (double *)t2);
         axpy_c(V, t2, -alpha_k, r);
         double intermediate_norm = orc_norm(V, (const double *)r);
         if (intermediate_norm < tol) {
@@ -335,37 +416,66 @@ static bool cg_loop(const orc_geom *g, const double *sm, cplx *X, cplx *r, cplx 
     return false;
 }
 
-/* conjugategradient.c:13-107 (START_X0_NULL) */
-bool orc_cg_null(const orc_geom *g, const double *sm, const double *b, double *x,
-                 double tol, long max_iter, double *work, orc_cgstat *st)
+static bool cg_null_op(const orc_op *o, const double *b, double *x, double tol, long max_iter, double *work,
+                       orc_cgstat *st)
 {
-    int V = g->V;
+    int V = o->g->V;
     const cplx *B = (const cplx *)b;
     cplx *X = (cplx *)x;
     cplx *r = (cplx *)work, *p = r + V, *t1 = p + V, *t2 = t1 + V;
     memcpy(r, B, sizeof(cplx) * V);
-    orc_adj_dirac(g, sm, x, (double *)t1);
-    orc_sub_dirac(g, sm, (const double *)t1, (double *)r);
+    op_Mdag(o, x, (double *)t1);
+    op_subM(o, (const double *)t1, (double *)r);
     if (orc_norm(V, (const double *)r) < tol) { if (st) st->last_iters = -1; return true; }
     for (int i = 0; i < V; i++) X[i] = 0.0;
     memcpy(r, B, sizeof(cplx) * V);
     memcpy(p, B, sizeof(cplx) * V);
-    return cg_loop(g, sm, X, r, p, t1, t2, tol, max_iter, st);
+    return cg_loop(o, X, r, p, t1, t2, tol, max_iter, st);
+}
+
+static bool cg_imposed_op(const orc_op *o, const double *b, double *x, double tol, long max_iter, double *work,
+                          orc_cgstat *st)
+{
+    int V = o->g->V;
+    const cplx *B = (const cplx *)b;
+    cplx *X = (cplx *)x;
+    cplx *r = (cplx *)work, *p = r + V, *t1 = p + V, *t2 = t1 + V;
+    for (int i = 0; i < V; i++) X[i] = 0.0;
+    memcpy(r, B, sizeof(cplx) * V);
+    if (orc_norm(V, (const double *)r) < tol) { if (st) st->last_iters = -1; return true; }
+    memcpy(p, B, sizeof(cplx) * V);
+    return cg_loop(o, X, r, p, t1, t2, tol, max_iter, st);
+}
+
+/* conjugategradient.c:13-107 (START_X0_NULL) */
+bool orc_cg_null(const orc_geom *g, const double *sm, const double *b, double *x,
+                 double tol, long max_iter, double *work, orc_cgstat *st)
+{
+    orc_op o = {g, sm, NULL};
+    return cg_null_op(&o, b, x, tol, max_iter, work, st);
 }
 
 /* conjugategradient.c:115-182 (StartX0Imposed) */
 bool orc_cg_imposed(const orc_geom *g, const double *sm, const double *b, double *x,
                     double tol, long max_iter, double *work, orc_cgstat *st)
 {
-    int V = g->V;
-    const cplx *B = (const cplx *)b;
-    cplx *X = (cplx *)x;
-    cplx *r = (cplx *)work, *p = r + V, *t1 = p + V, *t2 = t1 + V;
-    for (int i = 0; i < V; i++) X[i] = 0.0;
-    memcpy(r, B, sizeof(cplx) * V);
-    if (orc_norm(V, (const double *)r) < tol) { if (st) st->last_iters = -1; return true; }
-    memcpy(p, B, sizeof(cplx) * V);
-    return cg_loop(g, sm, X, r, p, t1, t2, tol, max_iter, st);
+    orc_op o = {g, sm, NULL};
+    return cg_imposed_op(&o, b, x, tol, max_iter, work, st);
+}
+
+/* the same solvers on the link-coupled operator */
+bool orc_link_cg_null(const orc_geom *g, const double *sm, const double *U, const double *b, double *x,
+                      double tol, long max_iter, double *work, orc_cgstat *st)
+{
+    orc_op o = {g, sm, U};
+    return cg_null_op(&o, b, x, tol, max_iter, work, st);
+}
+
+bool orc_link_cg_imposed(const orc_geom *g, const double *sm, const double *U, const double *b, double *x,
+                         double tol, long max_iter, double *work, orc_cgstat *st)
+{
+    orc_op o = {g, sm, U};
+    return cg_imposed_op(&o, b, x, tol, max_iter, work, st);
 }
 
 /* ======================================================================= */
@@ -621,4 +731,172 @@ void orc_condensate_sigma(const orc_geom *g, const double *sigma, double *out)
     out[5] = (V * pow(cabs(box[2]), 2));
     out[6] = (V * pow(cabs(box[0]), 2));
     out[7] = (V * pow(cabs(box[1]), 2));
+}
+
+/* ======================================================================= */
+/* link sector (moleculardynamics.c:127-222, 268-294, 381-542;             */
+/* linklattice.c:16-172).  U is lex order [V][3] complex (lattice.h:15);    */
+/* momenta and force are f-order [V][3] (moleculardynamics.c:389).          */
+/* No live reference path reaches this code (observables.c:127,183 are      */
+/* commented out), so it is pinned by invariants only.                      */
+/* ======================================================================= */
+static inline int lex_pp(const orc_geom *g, int lex, int d) { return g->f2lex[g->fpp[3 * g->lex2f[lex] + d]]; }
+static inline int lex_mm(const orc_geom *g, int lex, int d) { return g->f2lex[g->fmm[3 * g->lex2f[lex] + d]]; }
+
+/* linklattice.c:16-33 (LinkLink, LinkAdjLink of link.h:20-21) */
+static double plaq_re(const orc_geom *g, const cplx *U, int site, int mu, int nu)
+{
+    cplx t1 = U[3 * site + mu] * U[3 * lex_pp(g, site, mu) + nu];
+    cplx t2 = t1 * conj(U[3 * lex_pp(g, site, nu) + mu]);
+    t1 = t2 * conj(U[3 * site + nu]);
+    return creal(t1);
+}
+
+/* linklattice.c:62-80 */
+double orc_plaquette_sum(const orc_geom *g, const double *Ud)
+{
+    const cplx *U = (const cplx *)Ud;
+    double temp = 0.0;
+    for (int x = 0; x < g->V; x++)
+        for (int mu = 0; mu < 3; mu++)
+            for (int nu = 0; nu < mu; nu++) temp += plaq_re(g, U, x, mu, nu);
+    return temp;
+}
+
+/* linklattice.c:146-172 */
+static double imag_all_plaquettes(const orc_geom *g, const cplx *U, int site, int mu)
+{
+    double total = 0.0;
+    for (int nu = 0; nu < 3; nu++)
+        if (nu != mu) {
+            cplx t1 = U[3 * site + mu] * U[3 * lex_pp(g, site, mu) + nu];
+            cplx t2 = t1 * conj(U[3 * lex_pp(g, site, nu) + mu]);
+            t1 = t2 * conj(U[3 * site + nu]);
+            total -= cimag(t1);
+            int smn = lex_mm(g, site, nu);
+            t1 = U[3 * smn + mu] * U[3 * lex_pp(g, smn, mu) + nu];
+            t2 = t1 * conj(U[3 * site + mu]);
+            t1 = t2 * conj(U[3 * smn + nu]);
+            total += cimag(t1);
+        }
+    return total;
+}
+
+/* moleculardynamics.c:127-138 (overwrites its elements) */
+void orc_gauge_force(const orc_geom *g, const double *Ud, double beta, double *force)
+{
+    const cplx *U = (const cplx *)Ud;
+    for (int x = 0; x < g->V; x++)
+        for (int d = 0; d < 3; d++) force[3 * g->lex2f[x] + d] = beta * imag_all_plaquettes(g, U, x, d);
+}
+
+/* moleculardynamics.c:181-190, after the solves (accumulates) */
+void orc_link_force_from_solutions(const orc_geom *g, const double *Ud, double *const Xd[2], double *const Yd[2],
+                                   double *force)
+{
+    const cplx *U = (const cplx *)Ud;
+    for (int f = 0; f < ORC_HALFNFLAV; f++) {
+        const cplx *X = (const cplx *)Xd[f];
+        const cplx *Y = (const cplx *)Yd[f];
+        for (int x = 0; x < g->V; x++)
+            for (int mu = 0; mu < 3; mu++) {
+                int site = g->f2lex[x];
+                int fp = g->fpp[3 * x + mu];
+                force[3 * x + mu] += creal(conj(X[x]) * I * g->eta_pp[3 * x + mu] * U[3 * site + mu] * Y[fp]);
+                force[3 * x + mu] += creal(conj(X[fp]) * I * g->eta_mm[3 * fp + mu] * conj(U[3 * site + mu]) * Y[x]);
+            }
+    }
+}
+
+/* ComputeForce (moleculardynamics.c:203-222) with
+ * AddForceFromFermionSectorRespectToLink (:164-193).  linked: the solves and
+ * M^dag run on M_U (copy_dirac.c); otherwise on the live M, as the reference. */
+bool orc_link_force(const orc_geom *g, const double *sm, const double *U, double beta, int pure_gauge, int linked,
+                    double tol, long max_iter, orc_kit *k, double *force)
+{
+    orc_op o = {g, sm, linked ? U : NULL};
+    for (int r = 0; r < 3 * g->V; r++) force[r] = 0.0;
+    if (pure_gauge) orc_gauge_force(g, U, beta, force);
+    for (int f = 0; f < ORC_HALFNFLAV; f++)
+        if (!cg_null_op(&o, k->phi[f], k->inv_phi[f], tol, max_iter, k->work, &k->st)) return false;
+    for (int f = 0; f < ORC_HALFNFLAV; f++) op_Mdag(&o, k->inv_phi[f], k->inv_phi_obs[f]);
+    orc_link_force_from_solutions(g, U, k->inv_phi, k->inv_phi_obs, force);
+    return true;
+}
+
+/* ComputeFullEnergyWithPhiWithoutSigma (moleculardynamics.c:268-294) */
+double orc_link_energy(const orc_geom *g, const double *U, const double *mom, double beta, int pure_gauge,
+                       double *const phid[2], double *const Xd[2])
+{
+    double total_energy = 0.0;
+    for (int r = 0; r < 3 * g->V; r++) total_energy += pow(mom[r], 2.0) / 2.0;
+    if (pure_gauge) total_energy -= (beta * orc_plaquette_sum(g, U));
+    for (int f = 0; f < ORC_HALFNFLAV; f++) {
+        const cplx *phi = (const cplx *)phid[f];
+        const cplx *X = (const cplx *)Xd[f];
+        for (int x = 0; x < g->V; x++) total_energy += creal(conj(phi[x]) * X[x]);
+    }
+    return total_energy;
+}
+
+/* moleculardynamics.c:381-391, Q_CHARGE = 1 (macro.h:26) */
+static void link_evolve_coords(const orc_geom *g, double *Ud, const double *mom, double eps)
+{
+    cplx *U = (cplx *)Ud;
+    for (int x = 0; x < g->V; x++)
+        for (int d = 0; d < 3; d++)
+            U[3 * x + d] = cexp(I * (carg(U[3 * x + d]) + 1 * eps * mom[3 * g->lex2f[x] + d]));
+}
+
+/* moleculardynamics.c:400-407 */
+static void link_evolve_momenta(int n, double *mom, const double *force, double eps)
+{
+    for (int r = 0; r < n; r++) mom[r] += 1 * (force[r] * eps);
+}
+
+/* LeapFrog2Link (moleculardynamics.c:419-542), RNG draws in the reference
+ * order: 3V/2 Gauss2MeanVariance momentum pairs (:38-46), chi_0, chi_1
+ * (fermion.c:337-345), the Metropolis uniform. */
+void orc_leapfrog2link(const orc_geom *g, orc_rng *rng, const double *sm, double *U, double *mom, double *force,
+                       double beta, double epsilon, int step, double tol_high, long max_iter, int pure_gauge,
+                       int linked, orc_kit *k, orc_traj *out)
+{
+    int V = g->V;
+    orc_op o = {g, sm, linked ? U : NULL};
+    memset(out, 0, sizeof(*out));
+    for (int r = 0; r < 3 * V / 2; r++) orc_gauss2mv(rng, &mom[2 * r], &mom[2 * r + 1], 0.0, 1.0);
+    for (int f = 0; f < ORC_HALFNFLAV; f++) {
+        cplx *chi = (cplx *)k->chi[f];
+        for (int r = 0; r < V; r++) chi[r] = orc_gauss1(rng);
+        op_M(&o, k->chi[f], k->phi[f]);
+    }
+    double *copy = (double *)malloc(sizeof(double) * 6 * V);
+    memcpy(copy, U, sizeof(double) * 6 * V);
+#define ORC_LFAIL() do { memcpy(U, copy, sizeof(double) * 6 * V); free(copy); out->ok = 0; return; } while (0)
+    for (int f = 0; f < ORC_HALFNFLAV; f++)
+        if (!cg_null_op(&o, k->phi[f], k->inv_phi[f], tol_high, max_iter, k->work, &k->st)) ORC_LFAIL();
+    double starting_energy = orc_link_energy(g, U, mom, beta, pure_gauge, k->phi, k->inv_phi);
+    if (!orc_link_force(g, sm, U, beta, pure_gauge, linked, tol_high, max_iter, k, force)) ORC_LFAIL();
+    link_evolve_momenta(3 * V, mom, force, epsilon / 2.0);
+    for (int s = 0; s < step - 1; s++) {
+        link_evolve_coords(g, U, mom, epsilon);
+        if (!orc_link_force(g, sm, U, beta, pure_gauge, linked, tol_high, max_iter, k, force)) ORC_LFAIL();
+        link_evolve_momenta(3 * V, mom, force, epsilon);
+    }
+    link_evolve_coords(g, U, mom, epsilon);
+    if (!orc_link_force(g, sm, U, beta, pure_gauge, linked, tol_high, max_iter, k, force)) ORC_LFAIL();
+    link_evolve_momenta(3 * V, mom, force, epsilon / 2.0);
+    double final_energy = orc_link_energy(g, U, mom, beta, pure_gauge, k->phi, k->inv_phi);
+    out->H0 = starting_energy;
+    out->H1 = final_energy;
+    out->dH = final_energy - starting_energy;
+    out->ok = 1;
+    if (orc_casuale(rng) <= exp(-(final_energy - starting_energy))) {
+        out->accepted = 1;
+    } else {
+        memcpy(U, copy, sizeof(double) * 6 * V);
+        out->accepted = 0;
+    }
+    free(copy);
+#undef ORC_LFAIL
 }

@@ -145,6 +145,31 @@ void orc_condensate(const orc_geom *g, orc_rng *rng, const double *sm, double to
  * chi, chip, chip_pbc1, chip_pbc2 */
 void orc_condensate_sigma(const orc_geom *g, const double *sigma, double *out);
 
+/* ---------------- link sector (config with links on) --------------------- */
+/* The link-coupled operator of the commented-out copy_dirac.c:46-123; the
+ * live operator (dirac.c) ignores the links.  U: lex order [V][3] complex
+ * (lattice.h:15).  mom/force: f-order [V][3] (moleculardynamics.c:389).
+ * Parity against the reference is unpinned (no live path); see
+ * tests/test_oracle_links.py for the invariants that pin it. */
+void orc_link_dirac(const orc_geom *g, const double *sm, const double *U, const double *in, double *out);
+void orc_link_adj_dirac(const orc_geom *g, const double *sm, const double *U, const double *in, double *out);
+void orc_link_sub_dirac(const orc_geom *g, const double *sm, const double *U, const double *in, double *out);
+bool orc_link_cg_null(const orc_geom *g, const double *sm, const double *U, const double *b, double *x,
+                      double tol, long max_iter, double *work, orc_cgstat *st);
+bool orc_link_cg_imposed(const orc_geom *g, const double *sm, const double *U, const double *b, double *x,
+                         double tol, long max_iter, double *work, orc_cgstat *st);
+double orc_plaquette_sum(const orc_geom *g, const double *U);                         /* linklattice.c:62-80 */
+void orc_gauge_force(const orc_geom *g, const double *U, double beta, double *force);  /* md.c:127-138 */
+void orc_link_force_from_solutions(const orc_geom *g, const double *U, double *const X[2], double *const Y[2],
+                                   double *force);                                     /* md.c:181-190 */
+bool orc_link_force(const orc_geom *g, const double *sm, const double *U, double beta, int pure_gauge, int linked,
+                    double tol, long max_iter, orc_kit *k, double *force);             /* md.c:164-222 */
+double orc_link_energy(const orc_geom *g, const double *U, const double *mom, double beta, int pure_gauge,
+                       double *const phi[2], double *const X[2]);                      /* md.c:268-294 */
+void orc_leapfrog2link(const orc_geom *g, orc_rng *rng, const double *sm, double *U, double *mom, double *force,
+                       double beta, double eps, int steps, double tol_high, long max_iter, int pure_gauge,
+                       int linked, orc_kit *k, orc_traj *out);                         /* md.c:419-542 */
+
 #ifdef __cplusplus
 }
 #endif

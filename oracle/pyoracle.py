@@ -98,6 +98,23 @@ def lib():
                                   C.c_double, C.c_double, C.c_long, K, C.POINTER(Traj)]
         L.orc_condensate.argtypes = [G, R, _dp, C.c_double, C.c_long, K, C.POINTER(C.c_double), _dp]
         L.orc_condensate_sigma.argtypes = [G, _dp, _dp]
+        for n in ("orc_link_dirac", "orc_link_adj_dirac", "orc_link_sub_dirac"):
+            getattr(L, n).argtypes = [G, _dp, _dp, _dp, _dp]
+        for n in ("orc_link_cg_null", "orc_link_cg_imposed"):
+            getattr(L, n).argtypes = [G, _dp, _dp, _dp, _dp, C.c_double, C.c_long, _dp, C.POINTER(CGStat)]
+            getattr(L, n).restype = C.c_bool
+        L.orc_plaquette_sum.argtypes = [G, _dp]
+        L.orc_plaquette_sum.restype = C.c_double
+        L.orc_gauge_force.argtypes = [G, _dp, C.c_double, _dp]
+        L.orc_link_force_from_solutions.argtypes = [G, _dp, C.POINTER(C.c_double) * 2, C.POINTER(C.c_double) * 2,
+                                                    _dp]
+        L.orc_link_force.argtypes = [G, _dp, _dp, C.c_double, C.c_int, C.c_int, C.c_double, C.c_long, K, _dp]
+        L.orc_link_force.restype = C.c_bool
+        L.orc_link_energy.argtypes = [G, _dp, _dp, C.c_double, C.c_int, C.POINTER(C.c_double) * 2,
+                                      C.POINTER(C.c_double) * 2]
+        L.orc_link_energy.restype = C.c_double
+        L.orc_leapfrog2link.argtypes = [G, R, _dp, _dp, _dp, _dp, C.c_double, C.c_double, C.c_int, C.c_double,
+                                        C.c_long, C.c_int, C.c_int, K, C.POINTER(Traj)]
         _lib = L
     return _lib
 
@@ -159,6 +176,57 @@ class Lattice:
         return x, bool(ok), st
 
 
+    # -- link sector (link-coupled operator of copy_dirac.c; U lex [V][3]) --
+    def _lop(self, name, sm, U, v, out=None):
+        v = as_pairs(v)
+        out = np.zeros(2 * self.V) if out is None else as_pairs(out).copy()
+        getattr(lib(), name)(C.byref(self.g), np.ascontiguousarray(sm, np.float64), as_pairs(U), v, out)
+        return out
+
+    def link_M(self, sm, U, v):
+        return self._lop("orc_link_dirac", sm, U, v)
+
+    def link_Mdag(self, sm, U, v):
+        return self._lop("orc_link_adj_dirac", sm, U, v)
+
+    def link_sub_M(self, sm, U, v, target):
+        return self._lop("orc_link_sub_dirac", sm, U, v, target)
+
+    def link_cg(self, sm, U, b, x0=None, tol=1e-10, max_iter=None, start="null"):
+        b = as_pairs(b)
+        x = np.zeros(2 * self.V) if x0 is None else as_pairs(x0).copy()
+        work = np.zeros(8 * self.V)
+        st = CGStat()
+        fn = lib().orc_link_cg_null if start == "null" else lib().orc_link_cg_imposed
+        ok = fn(C.byref(self.g), np.ascontiguousarray(sm, np.float64), as_pairs(U), b, x, tol,
+                max_iter if max_iter is not None else self.V + 1, work, C.byref(st))
+        return x, bool(ok), st
+
+    def plaquette_sum(self, U):
+        return lib().orc_plaquette_sum(C.byref(self.g), as_pairs(U))
+
+    def gauge_force(self, U, beta):
+        F = np.zeros(3 * self.V)
+        lib().orc_gauge_force(C.byref(self.g), as_pairs(U), beta, F)
+        return F
+
+    def link_force_from_solutions(self, U, X, Y, force=None):
+        F = np.zeros(3 * self.V) if force is None else force
+        dp = C.POINTER(C.c_double)
+        Xs = [as_pairs(a) for a in X]
+        Ys = [as_pairs(a) for a in Y]
+        lib().orc_link_force_from_solutions(C.byref(self.g), as_pairs(U),
+                                            (dp * 2)(*[a.ctypes.data_as(dp) for a in Xs]),
+                                            (dp * 2)(*[a.ctypes.data_as(dp) for a in Ys]), F)
+        return F
+
+
+def random_links(V, seed):
+    """U(1) links exp(i theta), theta uniform in [0, 2 pi), lex order [V][3]."""
+    th = np.random.default_rng(seed).uniform(0.0, 2 * np.pi, 3 * V)
+    return np.exp(1j * th)
+
+
 def as_pairs(v):
     """complex128[V] or float64[2V] -> contiguous float64[2V] (re, im interleaved)."""
     a = np.asarray(v)
@@ -207,6 +275,39 @@ class Random:
         v = np.empty(2 * V)
         lib().orc_draw_z2_real(C.byref(self.r), V, v)
         return v
+
+
+class LinkHMC:
+    """Serial link HMC (LeapFrog2Link) at fixed sigma, driving the oracle."""
+
+    def __init__(self, lat, seed, sm, U, beta=1.0, pure_gauge=True, linked=True):
+        self.lat = lat
+        self.rng = Random(seed)
+        self.sm = np.ascontiguousarray(sm, np.float64)
+        self.U = as_pairs(np.asarray(U, np.complex128)).copy()
+        self.beta = beta
+        self.pure_gauge = int(pure_gauge)
+        self.linked = int(linked)
+        self.mom = np.zeros(3 * lat.V)
+        self.force = np.zeros(3 * lat.V)
+        self.kit = Kit()
+        lib().orc_kit_alloc(C.byref(self.kit), lat.V)
+
+    def __del__(self):
+        try:
+            lib().orc_kit_free(C.byref(self.kit))
+        except Exception:
+            pass
+
+    def trajectory(self, eps, steps, tol, max_iter=None):
+        t = Traj()
+        lib().orc_leapfrog2link(C.byref(self.lat.g), C.byref(self.rng.r), self.sm, self.U, self.mom, self.force,
+                                self.beta, eps, steps, tol, max_iter if max_iter is not None else self.lat.V + 1,
+                                self.pure_gauge, self.linked, C.byref(self.kit), C.byref(t))
+        return t
+
+    def links(self):
+        return as_complex(self.U).copy()
 
 
 def sigma_variance(beta_s, nflav=4):

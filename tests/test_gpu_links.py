@@ -1,0 +1,127 @@
+"""GPU link sector (A11) against the oracle's link sector.
+
+The oracle's link sector is itself pinned by invariants only
+(tests/test_oracle_links.py): the reference has no live link-fermion path.
+Bars here (written):
+  - links lex <-> f-order round trip exact;
+  - M_U, M_U^dag, r -= M_U v bit-exact against the oracle (same products,
+    same add order, no FMA);
+  - gauge force and link fermion force (from the same X, Y) bit-exact;
+    plaquette sum within 1e-13 relative (tree vs serial sum);
+  - link CG: iterations within +-1 and solution within 1e-8 relative;
+  - LeapFrog2Link driven with the oracle's draws: Delta-H within 1e-8
+    (absolute, |Delta-H| < 1) and identical accept decisions over three
+    trajectories, both for the reference's live-operator semantics
+    (linked = 0) and the link-coupled theory (linked = 1); links after the
+    trajectories within 1e-9.
+"""
+import numpy as np
+import pytest
+
+from gpu_util import banner, dev, host, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(P, L, LT, seed, shift=0.5):
+    lat = P.Lattice(L, LT)
+    r = P.Random(seed)
+    sm = lat.sigma_mass(r.sigma(lat.V, P.sigma_variance(0.85))) + shift
+    U = P.random_links(lat.V, seed)
+    return lat, r, sm, U
+
+
+@pytest.mark.parametrize("L,LT", [(8, 8), (6, 10)])
+def test_link_ops_vs_oracle(P, gnx_mod, L, LT):
+    banner(f"link ops {L}^2x{LT}")
+    import torch
+    lat, r, sm, U = setup(P, L, LT, 31, shift=0.0)
+    V = lat.V
+    ctx = gnx_mod.Context(L, LT)
+    ulex = dev(U.view(np.float64))
+    uf = torch.zeros_like(ulex)
+    ctx.links_from_lex(ulex, uf)
+    back = torch.zeros_like(ulex)
+    ctx.links_to_lex(uf, back)
+    assert np.array_equal(host(back), U.view(np.float64))
+    v, t = r.z2_complex(V), r.z2_complex(V)
+    smd = dev(sm)
+    out = ctx.vec()
+    ctx.link_dirac(gnx_mod.OP_M, smd, uf, dev(v), out)
+    assert np.array_equal(host(out), lat.link_M(sm, U, v))
+    ctx.link_dirac(gnx_mod.OP_MDAG, smd, uf, dev(v), out)
+    assert np.array_equal(host(out), lat.link_Mdag(sm, U, v))
+    out = dev(t)
+    ctx.link_dirac(gnx_mod.OP_SUB_M, smd, uf, dev(v), out)
+    assert np.array_equal(host(out), lat.link_sub_M(sm, U, v, t))
+
+
+def test_link_forces_vs_oracle(P, gnx_mod):
+    banner("gauge and link fermion forces 8^3")
+    import torch
+    lat, r, sm, U = setup(P, 8, 8, 32)
+    V = lat.V
+    ctx = gnx_mod.Context(8)
+    uf = torch.zeros(6 * V, dtype=torch.float64, device="cuda")
+    ctx.links_from_lex(dev(U.view(np.float64)), uf)
+    F = torch.zeros(3 * V, dtype=torch.float64, device="cuda")
+    ctx.gauge_force(uf, 1.0, F)
+    Fo = lat.gauge_force(U, 1.0)
+    assert np.array_equal(host(F), Fo)
+    X = [r.z2_complex(V) for _ in range(2)]
+    Y = [lat.link_Mdag(sm, U, x) for x in X]
+    ctx.link_force_from_solutions(uf, dev(X[0]), dev(Y[0]), dev(X[1]), dev(Y[1]), F, accumulate=True)
+    Fo2 = lat.link_force_from_solutions(U, X, Y, force=Fo.copy())
+    assert np.array_equal(host(F), Fo2)
+    p = ctx.plaquette_sum(uf)
+    po = lat.plaquette_sum(U)
+    assert abs(p - po) <= 1e-13 * max(1.0, abs(po))
+
+
+@pytest.mark.parametrize("start", ["null", "imposed"])
+def test_link_cg_vs_oracle(P, gnx_mod, start):
+    banner(f"link cg 8^3 {start}")
+    import torch
+    lat, r, sm, U = setup(P, 8, 8, 33)
+    V = lat.V
+    b = r.z2_complex(V)
+    x0 = r.z2_complex(V)
+    xr, ok, st = lat.link_cg(sm, U, b, x0=x0, tol=1e-10, start=start, max_iter=20000)
+    assert ok
+    ctx = gnx_mod.Context(8)
+    uf = torch.zeros(6 * V, dtype=torch.float64, device="cuda")
+    ctx.links_from_lex(dev(U.view(np.float64)), uf)
+    x = dev(x0)
+    s, res = ctx.link_cg_batch(dev(sm), uf, dev(b), x, 1, 1e-10, max_iter=20000,
+                               start=gnx_mod.START_NULL if start == "null" else gnx_mod.START_IMPOSED)
+    assert s == 0 and res[0].converged
+    assert abs(res[0].iterations - st.last_iters) <= 1
+    assert rel(host(x), xr) < 1e-8
+
+
+@pytest.mark.parametrize("linked", [0, 1])
+def test_link_trajectories_vs_oracle(P, gnx_mod, linked):
+    banner(f"LeapFrog2Link 8^3 linked={linked}")
+    lat, r, sm, U = setup(P, 8, 8, 34)
+    V = lat.V
+    eps, steps, tol, beta = 0.02, 10, 1e-10, 1.0
+    ref = P.LinkHMC(lat, 777, sm, U, beta=beta, pure_gauge=True, linked=linked)
+    draws = P.Random(777)  # the same stream, drawn here for the GPU
+    ctx = gnx_mod.Context(8)
+    h = gnx_mod.LinkHMC(ctx, beta=beta, pure_gauge=True, linked=linked)
+    h.set_fields(U, sm)
+    for k in range(3):
+        t_ref = ref.trajectory(eps, steps, tol, max_iter=20000)
+        assert t_ref.ok
+        mom = np.empty(3 * V)
+        for i in range(3 * V // 2):
+            mom[2 * i], mom[2 * i + 1] = draws.gauss2mv(0.0, 1.0)
+        chi0 = np.array([draws.gauss1() for _ in range(V)])
+        chi1 = np.array([draws.gauss1() for _ in range(V)])
+        st, t = h.trajectory(mom, chi0, chi1, eps, steps, tol, max_iter=20000)
+        assert st == 0 and t.ok
+        acc = draws.casuale() <= np.exp(-t.dH)
+        h.accept(acc)
+        assert abs(t.dH - t_ref.dH) <= 1e-8 * max(1.0, abs(t_ref.dH)), (k, t.dH, t_ref.dH)
+        assert acc == bool(t_ref.accepted)
+    np.testing.assert_allclose(h.get_links(), ref.links(), rtol=0, atol=1e-9)
