@@ -24,6 +24,14 @@ struct Shim {
   int L = 0, LT = 0;
   double (*gauss1)(void) = nullptr;
   double (*casuale)(void) = nullptr;
+  void (*gauss2mv)(double*, double*, double, double) = nullptr;
+  int pure_gauge = 0;        // PURE_GAUGE, macro.h:47 (off as shipped)
+  gnx_link_hmc* lhmc = nullptr;
+  double lhmc_beta = 0.0;
+  int lhmc_pure = -1;
+  double* uf = nullptr;      // device [3V] complex links, f-order
+  double* ulex = nullptr;    // device [3V] complex links, lex order
+  double* lforce = nullptr;  // device [3V]
   double tol_high = 1e-9;    // HIGH_PRECISION_INVERSION, macro.h:136
   double tol_medium = 1e-3;  // MEDIUM_PRECISION_INVERSION, macro.h:135
   double* sm = nullptr;      // device V
@@ -172,6 +180,9 @@ int GnxRefInit(int lspace, int ltime, int device, double (*gauss1)(void), double
   if (e == hipSuccess) e = hipMalloc(&g_shim.phi, 2 * vb);
   if (e == hipSuccess) e = hipMalloc(&g_shim.X, 2 * vb);
   if (e == hipSuccess) e = hipMalloc(&g_shim.Y, 2 * vb);
+  if (e == hipSuccess) e = hipMalloc(&g_shim.uf, 3 * vb);
+  if (e == hipSuccess) e = hipMalloc(&g_shim.ulex, 3 * vb);
+  if (e == hipSuccess) e = hipMalloc(&g_shim.lforce, 3 * fb);
   if (!ok_hip(e, "GnxRefInit allocation")) {
     GnxRefFinalize();
     return 2;
@@ -181,16 +192,26 @@ int GnxRefInit(int lspace, int ltime, int device, double (*gauss1)(void), double
 
 void GnxRefFinalize(void) {
   if (g_shim.hmc) gnx_hmc_destroy(g_shim.hmc);
-  double* bufs[] = {g_shim.sm, g_shim.sigma, g_shim.force, g_shim.in, g_shim.out, g_shim.phi, g_shim.X, g_shim.Y};
+  if (g_shim.lhmc) gnx_link_hmc_destroy(g_shim.lhmc);
+  double* bufs[] = {g_shim.sm,  g_shim.sigma, g_shim.force, g_shim.in,   g_shim.out,   g_shim.phi,
+                    g_shim.X,   g_shim.Y,     g_shim.uf,    g_shim.ulex, g_shim.lforce};
   for (double* p : bufs)
     if (p) (void)hipFree(p);
   if (g_shim.ctx) gnx_destroy(g_shim.ctx);
   double (*g1)(void) = g_shim.gauss1;
   double (*cs)(void) = g_shim.casuale;
+  void (*g2)(double*, double*, double, double) = g_shim.gauss2mv;
+  const int pg = g_shim.pure_gauge;
   g_shim = Shim();
   g_shim.gauss1 = g1;
   g_shim.casuale = cs;
+  g_shim.gauss2mv = g2;
+  g_shim.pure_gauge = pg;
 }
+
+void GnxRefBindGauss2(void (*gauss2mv)(double*, double*, double, double)) { g_shim.gauss2mv = gauss2mv; }
+
+void GnxRefSetPureGauge(int on) { g_shim.pure_gauge = on ? 1 : 0; }
 
 void GnxRefSetTolerances(double high, double medium) {
   g_shim.tol_high = high;
@@ -246,6 +267,150 @@ bool ComputeSigmaForce(const Lattice* lat, const Info* info, const Fermioncoor*,
                        Fermionkit* kit, double inversion_precision) {
   // SetForceSigmaToZero + fermion part + sigma part (moleculardynamics.c:631-647)
   return force_terms(lat, info, force, kit, inversion_precision, true);
+}
+
+// AddForceFromFermionSectorRespectToLink (moleculardynamics.c:164-193): the
+// solves and M^dag use the live operator, the force the links.  The force
+// accumulates onto g_shim.lforce, uploaded from the caller unless on_device.
+static bool link_fermion_force(const Lattice* lat, Force* force, Fermionkit* kit, bool on_device) {
+  const long V = g_shim.V;
+  const size_t vb = vbytes();
+  if (!up(g_shim.sm, lat->sigmamass, fbytes()) || !up(g_shim.ulex, lat->dlink, 3 * vb)) return false;
+  if (!on_device && !up(g_shim.lforce, force->force, 3 * fbytes())) return false;
+  for (int f = 0; f < 2; f++)
+    if (!up(g_shim.phi + f * 2 * V, kit->phi.flav[f].fermion, vb) ||
+        !up(g_shim.X + f * 2 * V, kit->inverted_phi.flav[f].fermion, vb))
+      return false;
+  gnx_cg_result r[2];
+  gnx_status s = gnx_cg_batch(g_shim.ctx, g_shim.sm, g_shim.phi, g_shim.X, 2, g_shim.tol_high, 0, GNX_START_NULL,
+                              GNX_PREC_F64, r);
+  for (int f = 0; f < 2; f++) {
+    down(kit->inverted_phi.flav[f].fermion, g_shim.X + f * 2 * V, vb);
+    if (r[f].converged && !r[f].early_exit) {
+      kit->iterations += r[f].iterations;
+      kit->CG_call++;
+    }
+  }
+  if (s != GNX_OK) {
+    if (s != GNX_ERR_NOCONV) report("gnx_cg_batch", s);
+    return false;
+  }
+  for (int f = 0; f < 2; f++)
+    report("gnx_dirac", gnx_dirac(g_shim.ctx, GNX_OP_MDAG, g_shim.sm, g_shim.X + f * 2 * V, g_shim.Y + f * 2 * V));
+  report("gnx_links_from_lex", gnx_links_from_lex(g_shim.ctx, g_shim.ulex, g_shim.uf));
+  s = gnx_link_force_from_solutions(g_shim.ctx, g_shim.uf, g_shim.X, g_shim.Y, g_shim.X + 2 * V, g_shim.Y + 2 * V,
+                                    g_shim.lforce, 1);
+  if (s != GNX_OK) {
+    report("gnx_link_force_from_solutions", s);
+    return false;
+  }
+  report("gnx_synchronize", gnx_synchronize(g_shim.ctx));
+  for (int f = 0; f < 2; f++) down(kit->inverted_phi_obs.flav[f].fermion, g_shim.Y + f * 2 * V, vb);
+  down(force->force, g_shim.lforce, 3 * fbytes());
+  return true;
+}
+
+bool AddForceFromFermionSectorRespectToLink(const Lattice* lat, const Info*, const Fermioncoor*, Force* force,
+                                            Fermionkit* kit) {
+  if (!ready("AddForceFromFermionSectorRespectToLink")) return false;
+  return link_fermion_force(lat, force, kit, false);
+}
+
+bool ComputeForce(const Lattice* lattice, const Info* info, Force* force, const Fermioncoor*, Fermionkit* kit) {
+  if (!ready("ComputeForce")) return false;
+  // SetForceToZero, the PURE_GAUGE part, then the fermion part (moleculardynamics.c:203-222)
+  if (g_shim.pure_gauge) {
+    if (!up(g_shim.ulex, lattice->dlink, 3 * vbytes())) return false;
+    report("gnx_links_from_lex", gnx_links_from_lex(g_shim.ctx, g_shim.ulex, g_shim.uf));
+    report("gnx_gauge_force", gnx_gauge_force(g_shim.ctx, g_shim.uf, info->beta, g_shim.lforce));
+  } else if (!ok_hip(hipMemset(g_shim.lforce, 0, 3 * fbytes()), "ComputeForce")) {
+    return false;
+  }
+  return link_fermion_force(lattice, force, kit, true);
+}
+
+static gnx_link_hmc* lhmc_for(double beta) {
+  if (g_shim.lhmc && g_shim.lhmc_beta == beta && g_shim.lhmc_pure == g_shim.pure_gauge) return g_shim.lhmc;
+  if (g_shim.lhmc) gnx_link_hmc_destroy(g_shim.lhmc);
+  g_shim.lhmc = nullptr;
+  gnx_status s = gnx_link_hmc_create(g_shim.ctx, beta, g_shim.pure_gauge, 0, &g_shim.lhmc);
+  if (s != GNX_OK) {
+    report("gnx_link_hmc_create", s);
+    return nullptr;
+  }
+  g_shim.lhmc_beta = beta;
+  g_shim.lhmc_pure = g_shim.pure_gauge;
+  return g_shim.lhmc;
+}
+
+bool LeapFrog2Link(Lattice* lattice, Info* info, Force* force, Momenta* momenta, const Fermioncoor*, Fermionkit* kit,
+                   double epsilon, const int step, const char* acceptance_file) {
+  if (!ready("LeapFrog2Link")) return false;
+  if (!g_shim.gauss1 || !g_shim.casuale || !g_shim.gauss2mv) {
+    fprintf(stderr, "gnx_ref: LeapFrog2Link needs Gauss1, Casuale (GnxRefInit) and Gauss2MeanVariance "
+                    "(GnxRefBindGauss2)\n");
+    return false;
+  }
+  const long V = g_shim.V;
+  FILE* opf = fopen(acceptance_file, "a");
+  // draws in the reference's order (moleculardynamics.c:38-46, 428-438)
+  for (long r = 0; r < 3 * V / 2; r++) g_shim.gauss2mv(&momenta->momenta[2 * r], &momenta->momenta[2 * r + 1], 0.0, 1.0);
+  std::vector<double> chi[2] = {std::vector<double>(V), std::vector<double>(V)};
+  for (int f = 0; f < 2; f++)
+    for (long r = 0; r < V; r++) {
+      chi[f][r] = g_shim.gauss1();
+      kit->chi.flav[f].fermion[r] = chi[f][r];
+    }
+  gnx_link_hmc* h = lhmc_for(info->beta);
+  if (!h) {
+    if (opf) fclose(opf);
+    return false;
+  }
+  report("gnx_link_hmc_set_fields", gnx_link_hmc_set_fields(h, (const double*)lattice->dlink, lattice->sigmamass));
+  report("gnx_link_hmc_set_solutions",
+         gnx_link_hmc_set_solutions(h, (const double*)kit->inverted_phi.flav[0].fermion,
+                                    (const double*)kit->inverted_phi.flav[1].fermion));
+  gnx_traj_result res;
+  gnx_status s = gnx_link_hmc_trajectory(h, momenta->momenta, chi[0].data(), chi[1].data(), epsilon, step,
+                                         g_shim.tol_high, 0, &res);
+  double *duf, *dsm, *dmom, *dforce, *dX, *dY, *dphi;
+  gnx_link_hmc_device_fields(h, &duf, &dsm, &dmom, &dforce, &dX, &dY, &dphi);
+  kit->iterations += (double)res.cg_iterations;
+  kit->CG_call += res.cg_solves;
+  const size_t vb = vbytes();
+  if (s != GNX_OK || !res.ok) {
+    // a failed solve restores the links (moleculardynamics.c:444-449)
+    if (s != GNX_OK) report("gnx_link_hmc_trajectory", s);
+    gnx_link_hmc_get_links(h, (double*)lattice->dlink);
+    if (opf) fclose(opf);
+    return false;
+  }
+  if (opf) fprintf(opf, "# Slink\t%.16f\n", res.H0);
+  info->delta_H_link = res.dH;
+  if (g_shim.casuale() <= exp(-(res.H1 - res.H0))) {
+    info->hybrid_accepted_link++;
+    info->hybrid_done_link++;
+    info->hybrid_done_acceptance_link = (double)(info->hybrid_accepted_link) / (double)(info->hybrid_done_link);
+    if (info->story_link >= 0) info->story_link++;
+    else info->story_link = -1;
+  } else {
+    report("gnx_link_hmc_accept", gnx_link_hmc_accept(h, 0));
+    info->hybrid_done_link++;
+    info->hybrid_done_acceptance_link = (double)(info->hybrid_accepted_link) / (double)(info->hybrid_done_link);
+    if (info->story_link >= 0) info->story_link = -1;
+    else info->story_link--;
+  }
+  gnx_link_hmc_get_links(h, (double*)lattice->dlink);
+  down(momenta->momenta, dmom, 3 * fbytes());
+  down(force->force, dforce, 3 * fbytes());
+  for (int f = 0; f < 2; f++) {
+    down(kit->phi.flav[f].fermion, dphi + f * 2 * V, vb);
+    down(kit->inverted_phi.flav[f].fermion, dX + f * 2 * V, vb);
+    down(kit->inverted_phi_obs.flav[f].fermion, dY + f * 2 * V, vb);
+  }
+  // with SIGMA_INTERACTION on (the shipped build) no acceptance line follows (:523-526)
+  if (opf) fclose(opf);
+  return true;
 }
 
 bool LeapFrog2MinimumNormSigmaVelocityFirst(Lattice* lattice, Info* info, Forcesigma* force, Momentasigma* momenta,

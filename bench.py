@@ -209,6 +209,74 @@ def condensate_leg(ctx_factory, torch, gnx, spec, tol):
     return out
 
 
+def ensemble_leg(gnx, sizes, ntraj, betas=(0.6, 0.75, 0.85, 1.0), eps=0.02, steps=10):
+    """Independent chains on lattices of uneven size, links on: per chain and
+    trajectory one sigma LFMN2VF trajectory and one LeapFrog2Link
+    trajectory (10 steps each, beta = 1, pure gauge on), each followed by its
+    Metropolis test.  Links start uniform in [0, 2 pi), sigma iid
+    N(0, 2/(4 beta_s)).  The link trajectory follows the reference as
+    written: solves with the live operator, force from the links
+    (moleculardynamics.c:419-542).  Every chain has its own context and
+    stream and is driven by its own host thread, so kernels of different
+    chains run concurrently on the GPU."""
+    import concurrent.futures as cf
+    chains = []
+    for L in sizes:
+        for bs in betas:
+            V = L ** 3
+            seed = 5000 + 10 * L + int(bs * 100)
+            rng = np.random.default_rng(seed)
+            ctx = gnx.Context(L)
+            h = gnx.HMC(ctx, beta_s=bs)
+            h.set_sigma(rng.normal(0.0, np.sqrt(2.0 / (4 * bs)), V))
+            lh = gnx.LinkHMC(ctx, beta=1.0, pure_gauge=True, linked=False)
+            _, sm = h.get_sigma()
+            lh.set_fields(np.exp(1j * rng.uniform(0.0, 2 * np.pi, 3 * V)), sm)
+            chains.append({"L": L, "beta_s": bs, "V": V, "ctx": ctx, "h": h, "lh": lh, "rng": rng,
+                           "dH_sigma": [], "dH_link": [], "acc_sigma": 0, "acc_link": 0, "cg_iterations": 0})
+
+    def run(c, n):
+        V, rng = c["V"], c["rng"]
+        for _ in range(n):
+            st, t = c["h"].trajectory(rng.standard_normal(V), rng.standard_normal(V), rng.standard_normal(V),
+                                      eps, steps, 1e-10, 1e-3)
+            a = bool(t.ok) and rng.random() <= np.exp(-t.dH)
+            c["h"].accept(a)
+            _, sm = c["h"].get_sigma()
+            c["lh"].set_fields(None, sm)
+            st2, tl = c["lh"].trajectory(rng.standard_normal(3 * V), rng.standard_normal(V),
+                                         rng.standard_normal(V), eps, steps, 1e-10)
+            al = bool(tl.ok) and rng.random() <= np.exp(-tl.dH)
+            c["lh"].accept(al)
+            c["dH_sigma"].append(t.dH)
+            c["dH_link"].append(tl.dH)
+            c["acc_sigma"] += int(a)
+            c["acc_link"] += int(al)
+            c["cg_iterations"] += t.cg_iterations + tl.cg_iterations
+        return n
+
+    # one warm-up trajectory per chain (each context tunes its solver shapes)
+    with cf.ThreadPoolExecutor(max_workers=len(chains)) as ex:
+        list(ex.map(lambda c: run(c, 1), chains))
+    for c in chains:
+        c["dH_sigma"].clear(), c["dH_link"].clear()
+        c["acc_sigma"] = c["acc_link"] = c["cg_iterations"] = 0
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=len(chains)) as ex:
+        list(ex.map(lambda c: run(c, ntraj), chains))
+    dt = time.perf_counter() - t0
+    out = {"workload": f"{len(chains)} chains on {sizes}^3 x beta_s {list(betas)}, links on (beta 1), "
+                       f"{ntraj} sigma + link trajectories each, {steps} steps of {eps}",
+           "trajectories_per_s": len(chains) * ntraj / dt, "seconds": dt,
+           "chains": [{"L": c["L"], "beta_s": c["beta_s"], "dH_sigma": c["dH_sigma"], "dH_link": c["dH_link"],
+                       "accepted_sigma": c["acc_sigma"], "accepted_link": c["acc_link"],
+                       "cg_iterations": c["cg_iterations"]} for c in chains]}
+    for c in chains:
+        c["lh"].close()
+        c["h"].close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,6 +289,9 @@ def main():
     ap.add_argument("--mixed", default="f32,bf16", help="precisions of the mixed-precision legs ('' to skip)")
     ap.add_argument("--condensate", default="48:64", metavar="L:N",
                     help="batched stochastic condensate leg: lattice L^3, N complex Z2 sources ('' to skip)")
+    ap.add_argument("--chains", default="16,24,32,48", metavar="L,L,...",
+                    help="uneven-ensemble leg: 4 chains per lattice size, sigma and link HMC ('' to skip)")
+    ap.add_argument("--chain-trajectories", type=int, default=2)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also time a plain device copy of MIB MiB (the reference kernel for "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE scaling)")
@@ -402,6 +473,9 @@ def main():
     if args.condensate:
         result["condensate_batched"] = condensate_leg(ctx_factory=gnx.Context, torch=torch, gnx=gnx,
                                                       spec=args.condensate, tol=args.tol)
+    if args.chains:
+        result["ensembles_with_links"] = ensemble_leg(gnx=gnx, sizes=[int(v) for v in args.chains.split(",")],
+                                                      ntraj=args.chain_trajectories)
     if args.copy_calibration > 0:
         n = args.copy_calibration * (1 << 20) // 8
         src = torch.ones(n, dtype=torch.float64, device=dev)

@@ -133,7 +133,9 @@ struct Info {
     int story_pi;
 };
 
-/* moleculardynamics.h:62-88, condensate.h:11-15 */
+/* moleculardynamics.h:17-30, 62-88, condensate.h:11-15 */
+struct Momenta { double *momenta; };
+struct Force { double *force; };
 struct Momentasigma { double *momenta; };
 struct Forcesigma { double *force; };
 struct RandomVec { struct Fermion *vec; };
@@ -147,6 +149,12 @@ void GnxRefFinalize(void);
  * the trajectory uses them as the reference's integrator does
  * (defaults 1e-9 / 1e-3, the shipped values). */
 void GnxRefSetTolerances(double high, double medium);
+/* The driver's Gauss2MeanVariance (random.c:129-143), used by
+ * LeapFrog2Link's momentum draw (moleculardynamics.c:38-46). */
+void GnxRefBindGauss2(void (*gauss2mv)(double *ris1, double *ris2, double mean, double variance));
+/* PURE_GAUGE of the driver's macro.h:47 (off in the shipped build): adds the
+ * pure-gauge force and action to ComputeForce and LeapFrog2Link. */
+void GnxRefSetPureGauge(int on);
 
 /* ---- replaced functions, reference signatures ------------------------- */
 /* dirac.h:10-15 (dirac.c:46-82, 88-123, 213-247, 253-287) */
@@ -178,6 +186,18 @@ bool AddForceFromFermionSectorRespectToSigma(const struct Lattice *lat, const st
                                              struct Fermionkit *kit, double inversion_precision);
 bool ComputeSigmaForce(const struct Lattice *lat, const struct Info *info, const struct Fermioncoor *fcor,
                        struct Forcesigma *force, struct Fermionkit *kit, double inversion_precision);
+
+/* moleculardynamics.h:48-49, 79 (moleculardynamics.c:164-222, 419-542).  As
+ * in the reference, the solves use the live operator (dirac.c ignores the
+ * links) and the force differentiates the link-coupled one. */
+bool AddForceFromFermionSectorRespectToLink(const struct Lattice *lat, const struct Info *info,
+                                            const struct Fermioncoor *fcor, struct Force *force,
+                                            struct Fermionkit *kit);
+bool ComputeForce(const struct Lattice *lattice, const struct Info *info, struct Force *force,
+                  const struct Fermioncoor *fermioncoor, struct Fermionkit *kit);
+bool LeapFrog2Link(struct Lattice *lattice, struct Info *info, struct Force *force, struct Momenta *momenta,
+                   const struct Fermioncoor *fermioncoor, struct Fermionkit *kit, double epsilon, const int step,
+                   const char *acceptance_file);
 
 /* moleculardynamics.h:130 (moleculardynamics.c:1234-1434) */
 bool LeapFrog2MinimumNormSigmaVelocityFirst(struct Lattice *lattice, struct Info *info, struct Forcesigma *force,

@@ -31,6 +31,14 @@
  *               three step sizes).
  *   condensate, CondensateSigmaPi: golden outputs after every trajectory.
  *   RNG       : golden dSFMT-132049 stream after seeding, Gauss1/Gauss2 values.
+ *   link sector, live operator (the reference as written): golden
+ *               AddForceFromFermionSectorRespectToLink and three LeapFrog2Link
+ *               trajectories of the reference, called directly (no driver path
+ *               reaches them), reproduced bit for bit.
+ *   link sector, link-coupled operator (copy_dirac.c, commented out in the
+ *               reference): tests/test_oracle_links.py invariants (U = 1
+ *               reduction, M^dag = M^H, gauge covariance, dense CG, forces by
+ *               finite differences, second-order integrator).
  */
 #ifndef GNX_ORACLE_H
 #define GNX_ORACLE_H
@@ -149,8 +157,8 @@ void orc_condensate_sigma(const orc_geom *g, const double *sigma, double *out);
 /* The link-coupled operator of the commented-out copy_dirac.c:46-123; the
  * live operator (dirac.c) ignores the links.  U: lex order [V][3] complex
  * (lattice.h:15).  mom/force: f-order [V][3] (moleculardynamics.c:389).
- * Parity against the reference is unpinned (no live path); see
- * tests/test_oracle_links.py for the invariants that pin it. */
+ * With linked = 0 these follow the reference as written (golden-pinned);
+ * the link-coupled operator itself is pinned by invariants. */
 void orc_link_dirac(const orc_geom *g, const double *sm, const double *U, const double *in, double *out);
 void orc_link_adj_dirac(const orc_geom *g, const double *sm, const double *U, const double *in, double *out);
 void orc_link_sub_dirac(const orc_geom *g, const double *sm, const double *U, const double *in, double *out);

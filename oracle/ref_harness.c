@@ -268,6 +268,67 @@ int main(int argc, char **argv)
     run_hmc(lat, fcor, kit, sim.momentasigma, sim.forcesigma, &rv, 0.2, 2, 20, "hmc_e0.2", accfile);
     run_hmc(lat, fcor, kit, sim.momentasigma, sim.forcesigma, &rv, 0.05, 4, 5, "hmc_e0.05s4", accfile);
 
+    /* ---------------- link sector: the reference's own link functions ---------------- */
+    /* (reachable from no driver path, observables.c:127,183; called directly here) */
+    Initrand(555);
+    for (int i = 0; i < DIM * VOLUME; i++) lat->dlink[i].link = cexp(I_UNIT * 2.0 * PI * Casuale());
+    rec("link_U", 'd', lat->dlink, 2 * DIM * VOLUME);
+    Initrand(4343);
+    draw_sigma(lat, fcor, 0.85);
+    rec("link_sigmamass", 'd', lat->sigmamass, VOLUME);
+    for (int f = 0; f < HALFNFLAV; f++) {
+        DrawFullFermionFromGaussianDistribution(&(kit->chi.flav[f]));
+        DiracFermion(lat, &(kit->chi.flav[f]), &(kit->phi.flav[f]), &info, fcor);
+        SetFermionToZero(&(kit->inverted_phi.flav[f]));
+    }
+    rec_f("link_phi0", &(kit->phi.flav[0]));
+    rec_f("link_phi1", &(kit->phi.flav[1]));
+    struct Force lforce;
+    AllocateForce(&lforce);
+    SetForceToZero(&lforce);
+    kit->iterations = 0.0; kit->CG_call = 0;
+    double lfst[3];
+    lfst[0] = (double)AddForceFromFermionSectorRespectToLink(lat, &info, fcor, &lforce, kit);
+    lfst[1] = kit->iterations; lfst[2] = (double)kit->CG_call;
+    rec("link_force_F", 'd', lforce.force, DIM * VOLUME);
+    rec_f("link_force_X0", &(kit->inverted_phi.flav[0]));
+    rec_f("link_force_X1", &(kit->inverted_phi.flav[1]));
+    rec_f("link_force_Y0", &(kit->inverted_phi_obs.flav[0]));
+    rec_f("link_force_Y1", &(kit->inverted_phi_obs.flav[1]));
+    rec("link_force_stat", 'd', lfst, 3);
+
+    /* LeapFrog2Link as shipped (PURE_GAUGE off): 3 trajectories, eps 0.02, 10 steps */
+    struct Momenta lmom;
+    AllocateMomenta(&lmom);
+    struct Info linfo;
+    setup_info(&linfo, 0.85, 0.1, 2);
+    linfo.beta = 1.0;
+    linfo.epsilon_link_HMC = 0.02;
+    linfo.steps_link_HMC = 10;
+    Initrand(4444);
+    for (int f = 0; f < HALFNFLAV; f++) SetFermionToZero(&(kit->inverted_phi.flav[f]));
+    kit->iterations = 0.0; kit->CG_call = 0;
+    remove(accfile);
+    double ldh[3], lacc[3];
+    for (int t = 0; t < 3; t++) {
+        int before = linfo.hybrid_accepted_link;
+        if (!LeapFrog2Link(lat, &linfo, &lforce, &lmom, fcor, kit, 0.02, 10, accfile)) {
+            fprintf(stderr, "link inversion failure\n");
+            exit(2);
+        }
+        ldh[t] = linfo.delta_H_link;
+        lacc[t] = (double)(linfo.hybrid_accepted_link - before);
+        if (t == 0) {
+            rec("link_traj_mom1", 'd', lmom.momenta, DIM * VOLUME);
+            rec("link_traj_force1", 'd', lforce.force, DIM * VOLUME);
+        }
+    }
+    double lst[2] = {kit->iterations, (double)kit->CG_call};
+    rec("link_traj_dh", 'd', ldh, 3);
+    rec("link_traj_acc", 'd', lacc, 3);
+    rec("link_traj_U", 'd', lat->dlink, 2 * DIM * VOLUME);
+    rec("link_traj_stat", 'd', lst, 2);
+
     fclose(g_out);
     return 0;
 }

@@ -25,6 +25,7 @@ static FILE *g_out;
 static orc_rng R;
 static double drv_gauss1(void) { return orc_gauss1(&R); }
 static double drv_casuale(void) { return orc_casuale(&R); }
+static void drv_gauss2mv(double *a, double *b, double mean, double var) { orc_gauss2mv(&R, a, b, mean, var); }
 
 static void rec(const char *name, char kind, const void *data, int64_t count)
 {
@@ -207,6 +208,61 @@ int main(int argc, char **argv)
         snprintf(name, sizeof(name), "%s_calls", tags[t]);  rec(name, 'd', calls, ntraj[t]);
         snprintf(name, sizeof(name), "%s_sigmaN", tags[t]); rec(name, 'd', lat.sigma, V);
     }
+    /* ---------------- link sector (reference link functions through the drop-in) ---------------- */
+    GnxRefBindGauss2(drv_gauss2mv);
+    lat.dlink = (struct Link *)calloc(3 * V, sizeof(struct Link));
+    orc_rng_init(&R, 555);
+    for (int i = 0; i < 3 * V; i++) lat.dlink[i].link = cexp(I * 2.0 * M_PI * drv_casuale());
+    orc_rng_init(&R, 4343);
+    draw_sigma(&lat);
+    memset(&info, 0, sizeof(info));
+    info.beta_scalar = 0.85;
+    info.beta = 1.0;
+    for (int f = 0; f < 2; f++) {
+        for (int r = 0; r < V; r++) kit.chi.flav[f].fermion[r] = drv_gauss1();
+        DiracFermion(&lat, &kit.chi.flav[f], &kit.phi.flav[f], &info, NULL);
+        memset(kit.inverted_phi.flav[f].fermion, 0, sizeof(gnx_ref_complex) * V);
+    }
+    struct Force lforce = {(double *)calloc(3 * V, sizeof(double))};
+    kit.iterations = 0.0; kit.CG_call = 0;
+    double lfst[3];
+    lfst[0] = (double)AddForceFromFermionSectorRespectToLink(&lat, &info, NULL, &lforce, &kit);
+    lfst[1] = kit.iterations; lfst[2] = (double)kit.CG_call;
+    rec("link_force_F", 'd', lforce.force, 3 * V);
+    rec_f("link_force_X0", &kit.inverted_phi.flav[0]);
+    rec_f("link_force_X1", &kit.inverted_phi.flav[1]);
+    rec_f("link_force_Y0", &kit.inverted_phi_obs.flav[0]);
+    rec_f("link_force_Y1", &kit.inverted_phi_obs.flav[1]);
+    rec("link_force_stat", 'd', lfst, 3);
+    struct Momenta lmom = {(double *)calloc(3 * V, sizeof(double))};
+    info.epsilon_link_HMC = 0.02;
+    info.steps_link_HMC = 10;
+    orc_rng_init(&R, 4444);
+    for (int f = 0; f < 2; f++) memset(kit.inverted_phi.flav[f].fermion, 0, sizeof(gnx_ref_complex) * V);
+    kit.iterations = 0.0; kit.CG_call = 0;
+    char laccfile[1024];
+    snprintf(laccfile, sizeof(laccfile), "%s.link", accfile);
+    remove(laccfile);
+    double ldh[3], lacc[3];
+    for (int t = 0; t < 3; t++) {
+        int before = info.hybrid_accepted_link;
+        if (!LeapFrog2Link(&lat, &info, &lforce, &lmom, NULL, &kit, 0.02, 10, laccfile)) {
+            fprintf(stderr, "link inversion failure\n");
+            return 2;
+        }
+        ldh[t] = info.delta_H_link;
+        lacc[t] = (double)(info.hybrid_accepted_link - before);
+        if (t == 0) {
+            rec("link_traj_mom1", 'd', lmom.momenta, 3 * V);
+            rec("link_traj_force1", 'd', lforce.force, 3 * V);
+        }
+    }
+    double lst[2] = {kit.iterations, (double)kit.CG_call};
+    rec("link_traj_dh", 'd', ldh, 3);
+    rec("link_traj_acc", 'd', lacc, 3);
+    rec("link_traj_U", 'd', lat.dlink, 6 * V);
+    rec("link_traj_stat", 'd', lst, 2);
+
     GnxRefFinalize();
     fclose(g_out);
     return 0;

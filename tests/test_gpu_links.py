@@ -1,7 +1,11 @@
-"""GPU link sector (A11) against the oracle's link sector.
+"""GPU link sector (A11) against the oracle's link sector and the reference.
 
-The oracle's link sector is itself pinned by invariants only
-(tests/test_oracle_links.py): the reference has no live link-fermion path.
+No driver path of the reference reaches its link functions, but they are
+compiled and callable: the golden file holds the reference's own
+AddForceFromFermionSectorRespectToLink and three LeapFrog2Link trajectories
+(live-operator solves, as shipped), which the oracle reproduces bit for bit.
+The link-coupled operator (copy_dirac.c, commented out in the reference) is
+pinned by the invariants of tests/test_oracle_links.py.
 Bars here (written):
   - links lex <-> f-order round trip exact;
   - M_U, M_U^dag, r -= M_U v bit-exact against the oracle (same products,
@@ -13,7 +17,9 @@ Bars here (written):
     (absolute, |Delta-H| < 1) and identical accept decisions over three
     trajectories, both for the reference's live-operator semantics
     (linked = 0) and the link-coupled theory (linked = 1); links after the
-    trajectories within 1e-9.
+    trajectories within 1e-9;
+  - against the reference's golden LeapFrog2Link: Delta-H within 1e-8
+    relative, identical accept decisions, links within 1e-10.
 """
 import numpy as np
 import pytest
@@ -125,3 +131,36 @@ def test_link_trajectories_vs_oracle(P, gnx_mod, linked):
         assert abs(t.dH - t_ref.dH) <= 1e-8 * max(1.0, abs(t_ref.dH)), (k, t.dH, t_ref.dH)
         assert acc == bool(t_ref.accepted)
     np.testing.assert_allclose(h.get_links(), ref.links(), rtol=0, atol=1e-9)
+
+
+def test_link_trajectories_vs_reference_golden(golden, P, gnx_mod):
+    """The reference's own LeapFrog2Link (as shipped: live-operator solves,
+    PURE_GAUGE off) recorded at 8^3: Delta-H within 1e-8, identical accept
+    decisions, links after three trajectories within 1e-10, CG counters
+    within one iteration per solve."""
+    banner("LeapFrog2Link vs reference golden 8^3")
+    lat = P.Lattice(8)
+    V = lat.V
+    U = golden["link_U"].view(np.complex128)
+    sm = golden["link_sigmamass"]
+    draws = P.Random(4444)
+    ctx = gnx_mod.Context(8)
+    h = gnx_mod.LinkHMC(ctx, beta=1.0, pure_gauge=False, linked=False)
+    h.set_fields(U, sm)
+    for k in range(3):
+        mom = np.empty(3 * V)
+        for i in range(3 * V // 2):
+            mom[2 * i], mom[2 * i + 1] = draws.gauss2mv(0.0, 1.0)
+        chi0 = np.array([draws.gauss1() for _ in range(V)])
+        chi1 = np.array([draws.gauss1() for _ in range(V)])
+        st, t = h.trajectory(mom, chi0, chi1, 0.02, 10, 1e-9)
+        assert st == 0 and t.ok
+        acc = draws.casuale() <= np.exp(-t.dH)
+        h.accept(acc)
+        g = golden["link_traj_dh"][k]
+        assert abs(t.dH - g) <= 1e-8 * max(1.0, abs(g)), (k, t.dH, g)
+        assert acc == bool(golden["link_traj_acc"][k])
+    np.testing.assert_allclose(h.get_links().view(np.float64), golden["link_traj_U"], rtol=0, atol=1e-10)
+    it, calls = h.cg_stats()
+    assert calls == golden["link_traj_stat"][1]
+    assert abs(it - golden["link_traj_stat"][0]) <= calls

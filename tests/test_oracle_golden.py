@@ -122,3 +122,47 @@ def test_hmc_and_measurements(golden, P, tag, eps, steps, ntraj):
     assert np.array_equal(np.array(cs).ravel(), golden[tag + "_csig"])
     assert np.array_equal(h.sigma, golden[tag + "_sigmaN"])
     assert h.kit.st.iterations == golden[tag + "_iters"][-1]
+
+
+def test_link_force_golden(golden, P):
+    """AddForceFromFermionSectorRespectToLink as the reference computes it
+    (moleculardynamics.c:164-193: live-operator solves, force from the links)."""
+    lat = P.Lattice(8)
+    U = golden["link_U"].view(np.complex128)
+    sm = golden["link_sigmamass"]
+    h = P.LinkHMC(lat, 1, sm, U, beta=1.0, pure_gauge=False, linked=False)
+    for f in range(2):
+        np.ctypeslib.as_array(h.kit.phi[f], shape=(2 * lat.V,))[:] = golden[f"link_phi{f}"]
+    F = np.zeros(3 * lat.V)
+    ok = P.lib().orc_link_force(P.C.byref(lat.g), sm, h.U, 1.0, 0, 0, 1e-9, lat.V + 1, P.C.byref(h.kit), F)
+    assert ok
+    assert np.array_equal(F, golden["link_force_F"])
+    for f in range(2):
+        assert np.array_equal(np.ctypeslib.as_array(h.kit.inv_phi[f], shape=(2 * lat.V,)), golden[f"link_force_X{f}"])
+        assert np.array_equal(np.ctypeslib.as_array(h.kit.inv_phi_obs[f], shape=(2 * lat.V,)),
+                              golden[f"link_force_Y{f}"])
+    assert h.kit.st.iterations == golden["link_force_stat"][1]
+    assert h.kit.st.cg_call == golden["link_force_stat"][2]
+
+
+def test_leapfrog2link_golden(golden, P):
+    """LeapFrog2Link (moleculardynamics.c:419-542) as shipped (PURE_GAUGE off),
+    3 trajectories of 10 steps, eps 0.02, seed 4444."""
+    lat = P.Lattice(8)
+    U = golden["link_U"].view(np.complex128)
+    sm = golden["link_sigmamass"]
+    h = P.LinkHMC(lat, 4444, sm, U, beta=1.0, pure_gauge=False, linked=False)
+    dh, acc = [], []
+    for t in range(3):
+        tr = h.trajectory(0.02, 10, 1e-9)
+        assert tr.ok
+        dh.append(tr.dH)
+        acc.append(tr.accepted)
+        if t == 0:
+            assert np.array_equal(h.mom, golden["link_traj_mom1"])
+            assert np.array_equal(h.force, golden["link_traj_force1"])
+    assert ["%.16f" % v for v in dh] == ["%.16f" % v for v in golden["link_traj_dh"]]
+    assert np.array_equal(acc, golden["link_traj_acc"])
+    assert np.array_equal(h.U, golden["link_traj_U"])
+    assert h.kit.st.iterations == golden["link_traj_stat"][0]
+    assert h.kit.st.cg_call == golden["link_traj_stat"][1]

@@ -132,3 +132,25 @@ def test_shim_hmc(shim, golden, tag):
     # the acceptance file has the reference's 13 tab-separated columns
     last = open(acc).read().strip().splitlines()[-1].split("\t")
     assert len([c for c in last if c.strip()]) >= 12
+
+
+@pytest.mark.gpu
+def test_shim_link_functions(shim, golden):
+    """AddForceFromFermionSectorRespectToLink and LeapFrog2Link through the
+    drop-in, against the reference's own records."""
+    recs, acc = shim
+    for k in ("link_force_F", "link_force_X0", "link_force_X1", "link_force_Y0", "link_force_Y1"):
+        g = golden[k]
+        assert np.max(np.abs(recs[k] - g)) <= 1e-7 * max(1.0, np.max(np.abs(g))), k
+    assert recs["link_force_stat"][0] == 1.0 and recs["link_force_stat"][2] == golden["link_force_stat"][2]
+    np.testing.assert_allclose(recs["link_traj_dh"], golden["link_traj_dh"], rtol=1e-8, atol=1e-8)
+    np.testing.assert_array_equal(recs["link_traj_acc"], golden["link_traj_acc"])
+    # momenta after the first trajectory carry the GPU forces (solver-tolerance differences)
+    np.testing.assert_allclose(recs["link_traj_mom1"], golden["link_traj_mom1"], rtol=0, atol=1e-8)
+    np.testing.assert_allclose(recs["link_traj_U"], golden["link_traj_U"], rtol=0, atol=1e-10)
+    np.testing.assert_allclose(recs["link_traj_force1"], golden["link_traj_force1"], rtol=0,
+                               atol=1e-7 * max(1.0, np.max(np.abs(golden["link_traj_force1"]))))
+    assert recs["link_traj_stat"][1] == golden["link_traj_stat"][1]
+    # with SIGMA_INTERACTION on only the starting energy is written (moleculardynamics.c:454, 523-526)
+    lines = open(str(acc) + ".link").read().strip().splitlines()
+    assert len(lines) == 3 and all(l.startswith("# Slink\t") for l in lines)
