@@ -250,6 +250,17 @@ gnx_status gnx_cg_shape(gnx_ctx* c, int nrhs, int* spt, int* xcd) {
   return GNX_OK;
 }
 
+gnx_status gnx_cg_mixed_shape(gnx_ctx* c, int nrhs, gnx_prec prec, int* spt) {
+  CHECK_ARG(c && spt, "gnx_cg_mixed_shape: null argument");
+  const int v = cgmix_spt_of(c->cgmix, nrhs, (int)prec);
+  if (v < 0) {
+    set_err("gnx_cg_mixed_shape: no tuned shape for nrhs=%d yet", nrhs);
+    return GNX_ERR_STATE;
+  }
+  *spt = v;
+  return GNX_OK;
+}
+
 gnx_status gnx_sigma_force_from_solutions(gnx_ctx* c, const double* sigma, double beta_s, int nflav,
                                           const double* X0, const double* Y0, const double* X1, const double* Y1,
                                           double* force) {

@@ -116,17 +116,28 @@ GNX_DEV double* mslot(const MixFix& a, int s, int rhs) { return a.part + ((size_
 GNX_DEVONLY double dnorm2(float2 v) { return (double)v.x * v.x + (double)v.y * v.y; }
 
 // ---- start kernels ------------------------------------------------------------
+// Every kernel of a solve uses the same SPT (sites per thread), so the
+// per-block partial slots have one layout: nb = ceil(V / (SPT * MIX_BS)).
+// Thread sites: f = blockIdx.x * SPT * MIX_BS + i * MIX_BS + threadIdx.x.
+template <int SPT>
+GNX_DEVONLY int mix_site(const Geom& g, int i) {
+  const int f = blockIdx.x * SPT * MIX_BS + i * MIX_BS + (int)threadIdx.x;
+  return f < g.V ? f : -1;
+}
+
 // imposed start: x = 0, r = p0 = (S)b, xs = 0, sm and b copied; partial |b|^2
-template <typename S>
+template <typename S, int SPT>
 __global__ __launch_bounds__(MIX_BS) void k_mix_start_imposed(MixFix a, const double* __restrict__ sm_in,
                                                               const double2* __restrict__ b_in, double tol,
                                                               double delta, long max_iter) {
   __shared__ double lds[MIX_BS / 64];
   const int rhs = blockIdx.y;
   const size_t off = (size_t)rhs * a.g.V;
-  const int f = blockIdx.x * MIX_BS + threadIdx.x;
   double acc = 0.0;
-  if (f < a.g.V) {
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    const int f = mix_site<SPT>(a.g, i);
+    if (f < 0) break;
     double2 bb = b_in[off + f];
     a.b[off + f] = bb;
     a.x[off + f] = make_double2(0.0, 0.0);
@@ -138,7 +149,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_start_imposed(MixFix a, const do
       a.sm[f] = sm_in[f];
       a.sm32[f] = (float)sm_in[f];
     }
-    acc = cnorm2(bb);
+    acc += cnorm2(bb);
   }
   acc = block_sum(acc, lds);
   if (threadIdx.x == 0) mslot(a, 1, rhs)[blockIdx.x] = acc;
@@ -154,35 +165,42 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_start_imposed(MixFix a, const do
 
 // null start (conjugategradient.c:18-33) pass 1: u = M^dag x_in, x = x_in,
 // b and sm copied
+template <int SPT>
 __global__ __launch_bounds__(MIX_BS) void k_mix_null_p1(MixFix a, const double* __restrict__ sm_in,
                                                          const double2* __restrict__ b_in,
                                                          const double2* __restrict__ xin) {
   const size_t off = (size_t)blockIdx.y * a.g.V;
-  const int f = blockIdx.x * MIX_BS + threadIdx.x;
-  if (f >= a.g.V) return;
   const double2* xv = xin + off;
-  a.u[off + f] = apply_Mdag(a.g, decode(a.g, f), sm_in[f], [&](int q) { return xv[q]; });
-  a.x[off + f] = xv[f];
-  a.b[off + f] = b_in[off + f];
-  if (blockIdx.y == 0) {
-    a.sm[f] = sm_in[f];
-    a.sm32[f] = (float)sm_in[f];
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    const int f = mix_site<SPT>(a.g, i);
+    if (f < 0) break;
+    a.u[off + f] = apply_Mdag(a.g, decode(a.g, f), sm_in[f], [&](int q) { return xv[q]; });
+    a.x[off + f] = xv[f];
+    a.b[off + f] = b_in[off + f];
+    if (blockIdx.y == 0) {
+      a.sm[f] = sm_in[f];
+      a.sm32[f] = (float)sm_in[f];
+    }
   }
 }
 
 // pass 2: partials |b - M u|^2 (slot 0) and |b|^2 (slot 1)
+template <int SPT>
 __global__ __launch_bounds__(MIX_BS) void k_mix_null_p2(MixFix a) {
   __shared__ double lds[MIX_BS / 64];
   const int rhs = blockIdx.y;
   const size_t off = (size_t)rhs * a.g.V;
-  const int f = blockIdx.x * MIX_BS + threadIdx.x;
+  const double2* u = a.u + off;
   double ar = 0.0, ab = 0.0;
-  if (f < a.g.V) {
-    const double2* u = a.u + off;
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    const int f = mix_site<SPT>(a.g, i);
+    if (f < 0) break;
     double2 bb = a.b[off + f];
     double2 rr = apply_sub_M(a.g, decode(a.g, f), a.sm[f], bb, [&](int q) { return u[q]; });
-    ar = cnorm2(rr);
-    ab = cnorm2(bb);
+    ar += cnorm2(rr);
+    ab += cnorm2(bb);
   }
   ar = block_sum(ar, lds);
   ab = block_sum(ab, lds);
@@ -193,7 +211,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_null_p2(MixFix a) {
 }
 
 // pass 3: an early exit keeps x_in, otherwise as the imposed start (slot 1 kept)
-template <typename S>
+template <typename S, int SPT>
 __global__ __launch_bounds__(MIX_BS) void k_mix_null_p3(MixFix a, double tol, double delta, long max_iter) {
   __shared__ double lds[MIX_BS / 64];
   const int rhs = blockIdx.y;
@@ -212,19 +230,23 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_null_p3(MixFix a, double tol, do
       a.res[rhs] = r;
     }
   }
-  const int f = blockIdx.x * MIX_BS + threadIdx.x;
-  if (early || f >= a.g.V) return;
-  double2 bb = a.b[off + f];
-  a.x[off + f] = make_double2(0.0, 0.0);
-  float2 bs = make_float2((float)bb.x, (float)bb.y);
-  S::st((typename S::T*)a.r + off, f, bs);
-  S::st((typename S::T*)a.p0 + off, f, bs);
-  a.xs[off + f] = make_float2(0.f, 0.f);
+  if (early) return;
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    const int f = mix_site<SPT>(a.g, i);
+    if (f < 0) break;
+    double2 bb = a.b[off + f];
+    a.x[off + f] = make_double2(0.0, 0.0);
+    float2 bs = make_float2((float)bb.x, (float)bb.y);
+    S::st((typename S::T*)a.r + off, f, bs);
+    S::st((typename S::T*)a.p0 + off, f, bs);
+    a.xs[off + f] = make_float2(0.f, 0.f);
+  }
 }
 
 // ---- iteration kernels -----------------------------------------------------------
 // odd: slot parity (p_old = odd ? p1 : p0); sin/sout state halves, pin/pout slots
-template <typename S>
+template <typename S, int SPT>
 __global__ __launch_bounds__(MIX_BS) void k_mix_X(MixFix a, int odd, const MixState* __restrict__ sin,
                                                   MixState* __restrict__ sout, const double* __restrict__ pin,
                                                   double* __restrict__ pout) {
@@ -237,22 +259,23 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_X(MixFix a, int odd, const MixSt
   const T* r = (const T*)a.r + off;
   const T* pold = (const T*)(odd ? a.p1 : a.p0) + off;
   T* pnew = (T*)(odd ? a.p0 : a.p1) + off;
-  const int f = blockIdx.x * MIX_BS + threadIdx.x;
-  const bool in = f < g.V;
   // loads of the normal step, issued before the reduction
   const MixState s = sin[rhs];
   const double share = partial_share(pin + (size_t)rhs * a.nb, a.nb);
-  float2 rv[7], pv[7];
-  float smv = 0.f;
-  Site st = decode(g, in ? f : 0);
-  if (in) {
-    NbrF n = neighbours_f(g, st);
-    int q[7] = {f, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
-    smv = a.sm32[f];
+  int fs[SPT];
+  float2 rv[SPT][7], pv[SPT][7];
+  float smv[SPT];
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    fs[i] = mix_site<SPT>(g, i);
+    if (fs[i] < 0) continue;
+    NbrF n = neighbours_f(g, decode(g, fs[i]));
+    int q[7] = {fs[i], n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
+    smv[i] = a.sm32[fs[i]];
 #pragma unroll
     for (int k = 0; k < 7; k++) {
-      rv[k] = S::ld(r, q[k]);
-      pv[k] = S::ld(pold, q[k]);
+      rv[i][k] = S::ld(r, q[k]);
+      pv[i][k] = S::ld(pold, q[k]);
     }
   }
   const double rho = block_sum(share, lds);
@@ -300,15 +323,17 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_X(MixFix a, int odd, const MixSt
     // reliable update, part 1: u = M^dag (x + xs) in FP64 (x and xs are not
     // written in this kernel, so neighbours read consistent values); the
     // direction is carried into the other parity buffer unchanged
-    if (in) {
-      const double2* x = a.x + off;
-      const float2* xs = a.xs + off;
-      auto ldx = [&](int q) {
-        float2 c = xs[q];
-        return cadd(x[q], make_double2((double)c.x, (double)c.y));
-      };
-      a.u[off + f] = apply_Mdag(g, st, a.sm[f], ldx);
-      S::st(pnew, f, pv[0]);
+    const double2* x = a.x + off;
+    const float2* xs = a.xs + off;
+    auto ldx = [&](int q) {
+      float2 c = xs[q];
+      return cadd(x[q], make_double2((double)c.x, (double)c.y));
+    };
+#pragma unroll
+    for (int i = 0; i < SPT; i++) {
+      if (fs[i] < 0) continue;
+      a.u[off + fs[i]] = apply_Mdag(g, decode(g, fs[i]), a.sm[fs[i]], ldx);
+      S::st(pnew, fs[i], pv[i][0]);
     }
     if (lead) put_mstate(out, 1, s.iters, PH_UPDATE, nrel, 0, 0, s.rho, rmax, s);
     return;
@@ -316,25 +341,28 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_X(MixFix a, int odd, const MixSt
   // normal step: p = r + beta p, t = M^dag p
   const float fb = (float)beta;
   double acc = 0.0;
-  if (in) {
-    float2 p0v = cadd(rv[0], rmul(fb, pv[0]));
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    if (fs[i] < 0) continue;
+    const Site st = decode(g, fs[i]);
+    float2 p0v = cadd(rv[i][0], rmul(fb, pv[i][0]));
     float2 ppv[3], pmv[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      ppv[d] = cadd(rv[1 + d], rmul(fb, pv[1 + d]));
-      pmv[d] = cadd(rv[4 + d], rmul(fb, pv[4 + d]));
+      ppv[d] = cadd(rv[i][1 + d], rmul(fb, pv[i][1 + d]));
+      pmv[d] = cadd(rv[i][4 + d], rmul(fb, pv[i][4 + d]));
     }
-    S::st(pnew, f, p0v);
-    float2 tt = apply_Mdag_vals(g, st, smv, p0v, ppv, pmv);
-    S::st((T*)a.t + off, f, tt);
-    acc = dnorm2(tt);
+    S::st(pnew, fs[i], p0v);
+    float2 tt = apply_Mdag_vals(g, st, smv[i], p0v, ppv, pmv);
+    S::st((T*)a.t + off, fs[i], tt);
+    acc += dnorm2(tt);
   }
   acc = block_sum(acc, lds);
   if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + blockIdx.x] = acc;
   if (lead) put_mstate(out, 1, s.iters, PH_NORMAL, nrel, 0, 0, rho, rmax, s);
 }
 
-template <typename S>
+template <typename S, int SPT>
 __global__ __launch_bounds__(MIX_BS) void k_mix_Y(MixFix a, int odd, const MixState* __restrict__ sin,
                                                   MixState* __restrict__ sout, const double* __restrict__ pin,
                                                   double* __restrict__ pout) {
@@ -344,8 +372,6 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Y(MixFix a, int odd, const MixSt
   const int rhs = blockIdx.y;
   const size_t off = (size_t)rhs * g.V;
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-  const int f = blockIdx.x * MIX_BS + threadIdx.x;
-  const bool in = f < g.V;
   const MixState s = sin[rhs];
   const double share = partial_share(pin + (size_t)rhs * a.nb, a.nb);
   MixState* out = sout + rhs;
@@ -356,14 +382,17 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Y(MixFix a, int odd, const MixSt
   double acc = 0.0;
   if (s.phase == PH_UPDATE) {
     // reliable update, part 2: x += xs, xs = 0, r = (S)(b - M u)
-    if (in) {
+    const double2* u = a.u + off;
+#pragma unroll
+    for (int i = 0; i < SPT; i++) {
+      const int f = mix_site<SPT>(g, i);
+      if (f < 0) break;
       float2 c = a.xs[off + f];
       a.x[off + f] = cadd(a.x[off + f], make_double2((double)c.x, (double)c.y));
       a.xs[off + f] = make_float2(0.f, 0.f);
-      const double2* u = a.u + off;
       double2 rr = apply_sub_M(g, decode(g, f), a.sm[f], a.b[off + f], [&](int q) { return u[q]; });
       S::st((T*)a.r + off, f, make_float2((float)rr.x, (float)rr.y));
-      acc = cnorm2(rr);
+      acc += cnorm2(rr);
     }
     acc = block_sum(acc, lds);
     if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + blockIdx.x] = acc;
@@ -374,20 +403,26 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Y(MixFix a, int odd, const MixSt
   const T* pnew = (const T*)(odd ? a.p0 : a.p1) + off;
   const T* t = (const T*)a.t + off;
   T* r = (T*)a.r + off;
-  float2 w = make_float2(0.f, 0.f), pv = w, xv = w, rv = w;
-  if (in) {
-    w = apply_M(g, decode(g, f), a.sm32[f], [&](int q) { return S::ld(t, q); });
-    pv = S::ld(pnew, f);
-    xv = a.xs[off + f];
-    rv = S::ld(r, f);
+  int fs[SPT];
+  float2 w[SPT], pv[SPT], xv[SPT], rv[SPT];
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    fs[i] = mix_site<SPT>(g, i);
+    if (fs[i] < 0) continue;
+    w[i] = apply_M(g, decode(g, fs[i]), a.sm32[fs[i]], [&](int q) { return S::ld(t, q); });
+    pv[i] = S::ld(pnew, fs[i]);
+    xv[i] = a.xs[off + fs[i]];
+    rv[i] = S::ld(r, fs[i]);
   }
   const double gamma = block_sum(share, lds);
   const float al = (float)(s.rho / gamma);
-  if (in) {
-    st_nt(&a.xs[off + f], make_float2(xv.x + al * pv.x, xv.y + al * pv.y));
-    float2 rn = make_float2(rv.x - al * w.x, rv.y - al * w.y);
-    S::st(r, f, rn);
-    acc = dnorm2(rn);
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    if (fs[i] < 0) continue;
+    st_nt(&a.xs[off + fs[i]], make_float2(xv[i].x + al * pv[i].x, xv[i].y + al * pv[i].y));
+    float2 rn = make_float2(rv[i].x - al * w[i].x, rv[i].y - al * w[i].y);
+    S::st(r, fs[i], rn);
+    acc += dnorm2(rn);
   }
   acc = block_sum(acc, lds);
   if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + blockIdx.x] = acc;
@@ -454,7 +489,29 @@ static hipError_t mix_ensure(MixMachine* m, const Geom& g, int nrhs) {
   return hipSuccess;
 }
 
-static MixFix mix_fix(MixMachine* m, const Geom& g, int nrhs) {
+// launch shapes: sites per thread (the kernels of one solve share it)
+struct MixShape {
+  int spt;
+  const void* X;
+  const void* Y;
+  const void* start_imposed;
+  const void* null_p1;
+  const void* null_p2;
+  const void* null_p3;
+};
+
+#define GNX_MIX_SHAPE(S, N)                                                                              \
+  {N, (const void*)k_mix_X<S, N>, (const void*)k_mix_Y<S, N>, (const void*)k_mix_start_imposed<S, N>, \
+   (const void*)k_mix_null_p1<N>, (const void*)k_mix_null_p2<N>, (const void*)k_mix_null_p3<S, N>}
+template <typename S>
+static const MixShape* mix_shapes() {
+  static const MixShape t[] = {GNX_MIX_SHAPE(S, 1), GNX_MIX_SHAPE(S, 2), GNX_MIX_SHAPE(S, 4)};
+  return t;
+}
+#undef GNX_MIX_SHAPE
+constexpr int kMixShapes = 3;
+
+static MixFix mix_fix(MixMachine* m, const Geom& g, int nrhs, int spt) {
   MixFix a;
   a.g = g;
   a.sm = m->sm;
@@ -471,17 +528,14 @@ static MixFix mix_fix(MixMachine* m, const Geom& g, int nrhs) {
   a.state = (MixState*)m->eng.dev_state();
   a.res = m->res;
   a.nrhs = nrhs;
-  a.nb = (g.V + MIX_BS - 1) / MIX_BS;
+  a.nb = (g.V + spt * MIX_BS - 1) / (spt * MIX_BS);
   return a;
 }
 
 template <typename S>
-static hipError_t mix_run_t(MixMachine* m, const Geom& g, const double* sm, const double2* b, double2* x,
-                            double tol, double delta, long max_iter, int start, int nrhs, hipStream_t st,
-                            CGDevResult* host_res, int tag) {
-  hipError_t e = mix_ensure(m, g, nrhs);
-  if (e != hipSuccess) return e;
-  MixFix a = mix_fix(m, g, nrhs);
+static hipError_t mix_graph(MixMachine* m, const Geom& g, int nrhs, int shape, int tag, hipGraphExec_t* exec) {
+  const MixShape& sh = mix_shapes<S>()[shape];
+  MixFix a = mix_fix(m, g, nrhs, sh.spt);
   dim3 grid(a.nb, nrhs), blk(MIX_BS);
   const MixState* s0 = a.state;
   const MixState* s1 = a.state + nrhs;
@@ -497,21 +551,95 @@ static hipError_t mix_run_t(MixMachine* m, const Geom& g, const double* sm, cons
       // X: state[0] -> state[1], reduces slot 1, writes slot 0; Y the reverse
       void* argsX[] = {(void*)&a, (void*)&odd, (void*)&s0, (void*)&w1, (void*)&q1, (void*)&o0};
       void* argsY[] = {(void*)&a, (void*)&odd, (void*)&s1, (void*)&w0, (void*)&q0, (void*)&o1};
-      (void)hipLaunchKernel((const void*)k_mix_X<S>, grid, blk, argsX, 0, cs);
-      (void)hipLaunchKernel((const void*)k_mix_Y<S>, grid, blk, argsY, 0, cs);
+      (void)hipLaunchKernel(sh.X, grid, blk, argsX, 0, cs);
+      (void)hipLaunchKernel(sh.Y, grid, blk, argsY, 0, cs);
     }
   };
-  hipGraphExec_t exec;
-  e = m->eng.graph(tag, nrhs, body, &exec);
-  if (e != hipSuccess) return e;
+  return m->eng.graph(tag + 10 * shape, nrhs, body, exec);
+}
+
+template <typename S>
+static hipError_t mix_begin(MixMachine* m, const Geom& g, const double* sm, const double2* b, const double2* xin,
+                            double tol, double delta, long max_iter, int start, int nrhs, int shape,
+                            hipStream_t st) {
+  const MixShape& sh = mix_shapes<S>()[shape];
+  MixFix a = mix_fix(m, g, nrhs, sh.spt);
+  dim3 grid(a.nb, nrhs), blk(MIX_BS);
+  hipError_t e;
   if (start == GNX_START_NULL) {
-    hipLaunchKernelGGL(k_mix_null_p1, grid, blk, 0, st, a, sm, b, (const double2*)x);
-    hipLaunchKernelGGL(k_mix_null_p2, grid, blk, 0, st, a);
-    hipLaunchKernelGGL(k_mix_null_p3<S>, grid, blk, 0, st, a, tol, delta, max_iter);
+    void* a1[] = {(void*)&a, (void*)&sm, (void*)&b, (void*)&xin};
+    e = hipLaunchKernel(sh.null_p1, grid, blk, a1, 0, st);
+    void* a2[] = {(void*)&a};
+    if (e == hipSuccess) e = hipLaunchKernel(sh.null_p2, grid, blk, a2, 0, st);
+    void* a3[] = {(void*)&a, (void*)&tol, (void*)&delta, (void*)&max_iter};
+    if (e == hipSuccess) e = hipLaunchKernel(sh.null_p3, grid, blk, a3, 0, st);
   } else {
-    hipLaunchKernelGGL(k_mix_start_imposed<S>, grid, blk, 0, st, a, sm, b, tol, delta, max_iter);
+    void* a0[] = {(void*)&a, (void*)&sm, (void*)&b, (void*)&tol, (void*)&delta, (void*)&max_iter};
+    e = hipLaunchKernel(sh.start_imposed, grid, blk, a0, 0, st);
   }
-  e = hipGetLastError();
+  return e;
+}
+
+// Autotuner: two chunks of every shape on the caller's own system, fastest
+// kept per (batch size, storage format); GNX_MIX_SHAPE=<0..2> pins one.
+template <typename S>
+static hipError_t mix_tune(MixMachine* m, const Geom& g, const double* sm, const double2* b, double delta,
+                           int nrhs, int tag, hipStream_t st, int* best) {
+  GraphEngine& E = m->eng;
+  const int key = nrhs + 1000 * tag;
+  for (int i = 0; i < 8; i++)
+    if (E.tuned_tag[i] >= 0 && E.tuned_nrhs[i] == key) {
+      *best = E.tuned_tag[i];
+      return hipSuccess;
+    }
+  int best_shape = 0;
+  hipError_t e = hipSuccess;
+  const char* env = getenv("GNX_MIX_SHAPE");
+  if (env && atoi(env) >= 0 && atoi(env) < kMixShapes) {
+    best_shape = atoi(env);
+  } else {
+    float best_ms = 1e30f;
+    for (int sh = 0; sh < kMixShapes && e == hipSuccess; sh++) {
+      hipGraphExec_t exec;
+      e = mix_graph<S>(m, g, nrhs, sh, tag, &exec);
+      float ms_min = 1e30f;
+      for (int rep = 0; rep < 3 && e == hipSuccess; rep++) {
+        e = mix_begin<S>(m, g, sm, b, nullptr, 0.0, delta, 2 * ENG_CHUNK, GNX_START_IMPOSED, nrhs, sh, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        float ms = 0.f;
+        if (e == hipSuccess) e = E.time_replays(exec, 2, st, &ms);
+        if (rep > 0 && ms < ms_min) ms_min = ms;
+      }
+      if (e == hipSuccess && ms_min < best_ms) {
+        best_ms = ms_min;
+        best_shape = sh;
+      }
+    }
+    if (e != hipSuccess) return e;
+  }
+  for (int i = 0; i < 8; i++)
+    if (E.tuned_tag[i] < 0) {
+      E.tuned_tag[i] = best_shape;
+      E.tuned_nrhs[i] = key;
+      break;
+    }
+  *best = best_shape;
+  return hipSuccess;
+}
+
+template <typename S>
+static hipError_t mix_run_t(MixMachine* m, const Geom& g, const double* sm, const double2* b, double2* x,
+                            double tol, double delta, long max_iter, int start, int nrhs, hipStream_t st,
+                            CGDevResult* host_res, int tag) {
+  hipError_t e = mix_ensure(m, g, nrhs);
+  if (e != hipSuccess) return e;
+  int shape = 0;
+  e = mix_tune<S>(m, g, sm, b, delta, nrhs, tag, st, &shape);
+  if (e != hipSuccess) return e;
+  hipGraphExec_t exec;
+  e = mix_graph<S>(m, g, nrhs, shape, tag, &exec);
+  if (e != hipSuccess) return e;
+  e = mix_begin<S>(m, g, sm, b, x, tol, delta, max_iter, start, nrhs, shape, st);
   if (e != hipSuccess) return e;
   // at most max_iter normal slot pairs plus one update pair after each
   e = m->eng.run(exec, nrhs, 2 * max_iter + 4, st, nullptr);
@@ -533,6 +661,15 @@ hipError_t cgmix_run(void** machine_slot, const Geom& g, const double* sm, const
   if (prec == GNX_PREC_F32)
     return mix_run_t<StoreF32>(m, g, sm, b, x, tol, delta, max_iter, start, nrhs, st, host_res, 100);
   return mix_run_t<StoreBF16>(m, g, sm, b, x, tol, delta, max_iter, start, nrhs, st, host_res, 101);
+}
+
+int cgmix_spt_of(void* machine, int nrhs, int prec) {
+  MixMachine* m = (MixMachine*)machine;
+  if (!m) return -1;
+  const int key = nrhs + 1000 * (prec == GNX_PREC_F32 ? 100 : 101);
+  for (int i = 0; i < 8; i++)
+    if (m->eng.tuned_tag[i] >= 0 && m->eng.tuned_nrhs[i] == key) return mix_shapes<StoreF32>()[m->eng.tuned_tag[i]].spt;
+  return -1;
 }
 
 }  // namespace gnx

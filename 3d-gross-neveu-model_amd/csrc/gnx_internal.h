@@ -48,6 +48,7 @@ hipError_t cgmix_run(void** machine_slot, const Geom& g, const double* sm, const
                      double delta, long max_iter, int start, int prec, int nrhs, hipStream_t st,
                      CGDevResult* host_res);
 void cgmix_release(void* machine);
+int cgmix_spt_of(void* machine, int nrhs, int prec);  // tuned sites per thread, -1 if none
 int cgm_shape_of(void* machine, int nrhs, int* spt, int* xcd);
 
 void set_err(const char* fmt, ...);

@@ -463,7 +463,8 @@ def main():
                 "all_converged": all(r.converged for r in res),
                 "true_residual_rhs0": float(torch.linalg.norm(B[:2 * V] - tr).item()),
                 "rel_err_vs_f64_rhs0": float((torch.linalg.norm(X[:2 * V] - xr) / torch.linalg.norm(xr)).item()),
-                "speedup_vs_f64_single": nb * reps / dt / solves_per_s}
+                "speedup_vs_f64_single": nb * reps / dt / solves_per_s,
+                "shape": ctx.cg_mixed_shape(nb, pcode)}
             del X
     result["mixed"] = mixed
     if mixed:

@@ -130,6 +130,9 @@ gnx_status gnx_set_reliable_delta(gnx_ctx *ctx, double delta);
  * solve of that batch size has run yet.  GNX_CG_SHAPE=<n> in the environment
  * pins the shape instead of tuning. */
 gnx_status gnx_cg_shape(gnx_ctx *ctx, int nrhs, int *sites_per_thread, int *xcd_mapped);
+/* The same for the mixed-precision solver (prec GNX_PREC_F32 or GNX_PREC_BF16);
+ * GNX_MIX_SHAPE=<0..2> pins 1, 2 or 4 sites per thread. */
+gnx_status gnx_cg_mixed_shape(gnx_ctx *ctx, int nrhs, gnx_prec prec, int *sites_per_thread);
 
 /* A9. Sigma force (moleculardynamics.c:590-647):
  *   X_f = CG(phi_f) with START_X0_NULL from the incoming X_f, Y_f = M^dag X_f,
