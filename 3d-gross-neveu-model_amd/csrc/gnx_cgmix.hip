@@ -119,14 +119,13 @@ GNX_DEVONLY double dnorm2(float2 v) { return (double)v.x * v.x + (double)v.y * v
 // Every kernel of a solve uses the same SPT (sites per thread), so the
 // per-block partial slots have one layout: nb = ceil(V / (SPT * MIX_BS)).
 // Thread sites: f = blockIdx.x * SPT * MIX_BS + i * MIX_BS + threadIdx.x.
-template <int SPT>
-GNX_DEVONLY int mix_site(const Geom& g, int i) {
-  const int f = blockIdx.x * SPT * MIX_BS + i * MIX_BS + (int)threadIdx.x;
-  return f < g.V ? f : -1;
+template <int SPT, bool XCD>
+GNX_DEVONLY int mix_site(const MixFix& a, int i) {
+  return SiteMap<SPT, XCD>(a.nb).f(a.g, i);
 }
 
 // imposed start: x = 0, r = p0 = (S)b, xs = 0, sm and b copied; partial |b|^2
-template <typename S, int SPT>
+template <typename S, int SPT, bool XCD>
 __global__ __launch_bounds__(MIX_BS) void k_mix_start_imposed(MixFix a, const double* __restrict__ sm_in,
                                                               const double2* __restrict__ b_in, double tol,
                                                               double delta, long max_iter) {
@@ -136,7 +135,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_start_imposed(MixFix a, const do
   double acc = 0.0;
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
-    const int f = mix_site<SPT>(a.g, i);
+    const int f = mix_site<SPT, XCD>(a, i);
     if (f < 0) break;
     double2 bb = b_in[off + f];
     a.b[off + f] = bb;
@@ -165,7 +164,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_start_imposed(MixFix a, const do
 
 // null start (conjugategradient.c:18-33) pass 1: u = M^dag x_in, x = x_in,
 // b and sm copied
-template <int SPT>
+template <int SPT, bool XCD>
 __global__ __launch_bounds__(MIX_BS) void k_mix_null_p1(MixFix a, const double* __restrict__ sm_in,
                                                          const double2* __restrict__ b_in,
                                                          const double2* __restrict__ xin) {
@@ -173,7 +172,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_null_p1(MixFix a, const double* 
   const double2* xv = xin + off;
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
-    const int f = mix_site<SPT>(a.g, i);
+    const int f = mix_site<SPT, XCD>(a, i);
     if (f < 0) break;
     a.u[off + f] = apply_Mdag(a.g, decode(a.g, f), sm_in[f], [&](int q) { return xv[q]; });
     a.x[off + f] = xv[f];
@@ -186,7 +185,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_null_p1(MixFix a, const double* 
 }
 
 // pass 2: partials |b - M u|^2 (slot 0) and |b|^2 (slot 1)
-template <int SPT>
+template <int SPT, bool XCD>
 __global__ __launch_bounds__(MIX_BS) void k_mix_null_p2(MixFix a) {
   __shared__ double lds[MIX_BS / 64];
   const int rhs = blockIdx.y;
@@ -195,7 +194,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_null_p2(MixFix a) {
   double ar = 0.0, ab = 0.0;
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
-    const int f = mix_site<SPT>(a.g, i);
+    const int f = mix_site<SPT, XCD>(a, i);
     if (f < 0) break;
     double2 bb = a.b[off + f];
     double2 rr = apply_sub_M(a.g, decode(a.g, f), a.sm[f], bb, [&](int q) { return u[q]; });
@@ -211,7 +210,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_null_p2(MixFix a) {
 }
 
 // pass 3: an early exit keeps x_in, otherwise as the imposed start (slot 1 kept)
-template <typename S, int SPT>
+template <typename S, int SPT, bool XCD>
 __global__ __launch_bounds__(MIX_BS) void k_mix_null_p3(MixFix a, double tol, double delta, long max_iter) {
   __shared__ double lds[MIX_BS / 64];
   const int rhs = blockIdx.y;
@@ -233,7 +232,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_null_p3(MixFix a, double tol, do
   if (early) return;
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
-    const int f = mix_site<SPT>(a.g, i);
+    const int f = mix_site<SPT, XCD>(a, i);
     if (f < 0) break;
     double2 bb = a.b[off + f];
     a.x[off + f] = make_double2(0.0, 0.0);
@@ -246,7 +245,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_null_p3(MixFix a, double tol, do
 
 // ---- iteration kernels -----------------------------------------------------------
 // odd: slot parity (p_old = odd ? p1 : p0); sin/sout state halves, pin/pout slots
-template <typename S, int SPT>
+template <typename S, int SPT, bool XCD>
 __global__ __launch_bounds__(MIX_BS) void k_mix_X(MixFix a, int odd, const MixState* __restrict__ sin,
                                                   MixState* __restrict__ sout, const double* __restrict__ pin,
                                                   double* __restrict__ pout) {
@@ -267,7 +266,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_X(MixFix a, int odd, const MixSt
   float smv[SPT];
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
-    fs[i] = mix_site<SPT>(g, i);
+    fs[i] = mix_site<SPT, XCD>(a, i);
     if (fs[i] < 0) continue;
     NbrF n = neighbours_f(g, decode(g, fs[i]));
     int q[7] = {fs[i], n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
@@ -353,7 +352,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_X(MixFix a, int odd, const MixSt
       pmv[d] = cadd(rv[i][4 + d], rmul(fb, pv[i][4 + d]));
     }
     S::st(pnew, fs[i], p0v);
-    float2 tt = apply_Mdag_vals(g, st, smv[i], p0v, ppv, pmv);
+    float2 tt = apply_Mdag_fast_vals(g, st, smv[i], p0v, ppv, pmv);
     S::st((T*)a.t + off, fs[i], tt);
     acc += dnorm2(tt);
   }
@@ -362,7 +361,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_X(MixFix a, int odd, const MixSt
   if (lead) put_mstate(out, 1, s.iters, PH_NORMAL, nrel, 0, 0, rho, rmax, s);
 }
 
-template <typename S, int SPT>
+template <typename S, int SPT, bool XCD>
 __global__ __launch_bounds__(MIX_BS) void k_mix_Y(MixFix a, int odd, const MixState* __restrict__ sin,
                                                   MixState* __restrict__ sout, const double* __restrict__ pin,
                                                   double* __restrict__ pout) {
@@ -385,7 +384,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Y(MixFix a, int odd, const MixSt
     const double2* u = a.u + off;
 #pragma unroll
     for (int i = 0; i < SPT; i++) {
-      const int f = mix_site<SPT>(g, i);
+      const int f = mix_site<SPT, XCD>(a, i);
       if (f < 0) break;
       float2 c = a.xs[off + f];
       a.x[off + f] = cadd(a.x[off + f], make_double2((double)c.x, (double)c.y));
@@ -407,7 +406,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Y(MixFix a, int odd, const MixSt
   float2 w[SPT], pv[SPT], xv[SPT], rv[SPT];
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
-    fs[i] = mix_site<SPT>(g, i);
+    fs[i] = mix_site<SPT, XCD>(a, i);
     if (fs[i] < 0) continue;
     w[i] = apply_M(g, decode(g, fs[i]), a.sm32[fs[i]], [&](int q) { return S::ld(t, q); });
     pv[i] = S::ld(pnew, fs[i]);
@@ -427,6 +426,273 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Y(MixFix a, int odd, const MixSt
   acc = block_sum(acc, lds);
   if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + blockIdx.x] = acc;
   if (lead) put_mstate(out, 1, s.iters + 1, PH_NORMAL, s.nrel, 0, 0, s.rho, s.rmax, s);
+}
+
+// ---- paired iteration kernels -------------------------------------------------------
+// Two systems per block (blockIdx.y = pair r0 = 2 y, r0 + 1): the site decode,
+// neighbours, phases and addresses are shared by the pair; each system keeps
+// its own state, scalars and phase, decided uniformly per block, so the
+// pair runs exactly the steps of k_mix_X / k_mix_Y on each system (same
+// partial sums, same order: a paired solve is bit-identical to an unpaired one
+// of the same sites-per-thread and mapping).
+enum MixMode { MM_OFF = 0, MM_NORMAL = 1, MM_UPDATE = 2 };
+
+// X-kernel decision of one system (the branches of k_mix_X); finished
+// systems get their final state and result written here
+GNX_DEVONLY int mix_x_decide(const MixFix& a, const MixState& s, int rhs, double rho, MixState* out, bool lead,
+                             double* beta, double* rmax, int* nrel) {
+  const double nr = sqrt(rho);
+  *beta = 0.0;
+  *rmax = s.rmax;
+  *nrel = s.nrel;
+  if (!s.active) {
+    if (lead) put_mstate(out, 0, s.iters, s.phase, s.nrel, s.converged, s.early, s.rho, s.rmax, s);
+    return MM_OFF;
+  }
+  if (s.phase == PH_UPDATED) {
+    *nrel = s.nrel + 1;
+    if (nr < s.tol || s.iters >= s.max_iter) {
+      const int conv = nr < s.tol ? 1 : 0;
+      if (lead) {
+        put_mstate(out, 0, s.iters, PH_NORMAL, *nrel, conv, 0, rho, nr, s);
+        CGDevResult res = {conv, 0, s.iters, *nrel, nr, {0, 0, 0}};
+        a.res[rhs] = res;
+      }
+      return MM_OFF;
+    }
+    *rmax = nr;
+    *beta = rho / s.rho;
+    return MM_NORMAL;
+  }
+  if (s.iters == 0) {
+    if (s.start == GNX_START_IMPOSED && nr < s.tol) {
+      if (lead) {
+        put_mstate(out, 0, 0, PH_NORMAL, 0, 1, 1, rho, nr, s);
+        CGDevResult res = {1, 1, 0, 0, nr, {0, 0, 0}};
+        a.res[rhs] = res;
+      }
+      return MM_OFF;
+    }
+    *rmax = nr;
+    return MM_NORMAL;
+  }
+  *beta = rho / s.rho;
+  return ((nr < s.delta * s.rmax) || (nr < s.tol) || (s.iters >= s.max_iter)) ? MM_UPDATE : MM_NORMAL;
+}
+
+template <typename S, int SPT, bool XCD>
+__global__ __launch_bounds__(MIX_BS) void k_mix_X2(MixFix a, int odd, const MixState* __restrict__ sin,
+                                                   MixState* __restrict__ sout, const double* __restrict__ pin,
+                                                   double* __restrict__ pout) {
+  __shared__ double lds[2 * (MIX_BS / 64)];
+  using T = typename S::T;
+  const Geom g = a.g;
+  const int r0 = 2 * blockIdx.y;
+  const bool two = r0 + 1 < a.nrhs;
+  const int rr[2] = {r0, two ? r0 + 1 : r0};
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const MixState s0 = sin[rr[0]];
+  const MixState s1 = sin[rr[1]];
+  const double sh0 = partial_share(pin + (size_t)rr[0] * a.nb, a.nb);
+  const double sh1 = two ? partial_share(pin + (size_t)rr[1] * a.nb, a.nb) : 0.0;
+  const T* r[2];
+  const T* pold[2];
+  T* pnew[2];
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const size_t off = (size_t)rr[k] * g.V;
+    r[k] = (const T*)a.r + off;
+    pold[k] = (const T*)(odd ? a.p1 : a.p0) + off;
+    pnew[k] = (T*)(odd ? a.p0 : a.p1) + off;
+  }
+  int fs[SPT];
+  float smv[SPT];
+  float2 rv[2][SPT][7], pv[2][SPT][7];
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    fs[i] = mix_site<SPT, XCD>(a, i);
+    if (fs[i] < 0) continue;
+    NbrF n = neighbours_f(g, decode(g, fs[i]));
+    int q[7] = {fs[i], n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
+    smv[i] = a.sm32[fs[i]];
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+#pragma unroll
+      for (int j = 0; j < 7; j++) {
+        rv[k][i][j] = S::ld(r[k], q[j]);
+        pv[k][i][j] = S::ld(pold[k], q[j]);
+      }
+  }
+  const double2 rho = block_sum2(sh0, sh1, lds);
+  const double rhok[2] = {rho.x, rho.y};
+  const MixState* sk[2] = {&s0, &s1};
+  int mode[2];
+  double beta[2], rmax[2];
+  int nrel[2];
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    if (k == 1 && !two) {
+      mode[1] = MM_OFF;
+      continue;
+    }
+    mode[k] = mix_x_decide(a, *sk[k], rr[k], rhok[k], sout + rr[k], lead, &beta[k], &rmax[k], &nrel[k]);
+  }
+  double acc[2] = {0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const size_t off = (size_t)rr[k] * g.V;
+    if (mode[k] == MM_UPDATE) {
+      // reliable update, part 1 (as k_mix_X)
+      const double2* x = a.x + off;
+      const float2* xs = a.xs + off;
+      auto ldx = [&](int q) {
+        float2 c = xs[q];
+        return cadd(x[q], make_double2((double)c.x, (double)c.y));
+      };
+#pragma unroll
+      for (int i = 0; i < SPT; i++) {
+        if (fs[i] < 0) continue;
+        a.u[off + fs[i]] = apply_Mdag(g, decode(g, fs[i]), a.sm[fs[i]], ldx);
+        S::st(pnew[k], fs[i], pv[k][i][0]);
+      }
+      if (lead) put_mstate(sout + rr[k], 1, sk[k]->iters, PH_UPDATE, nrel[k], 0, 0, sk[k]->rho, rmax[k], *sk[k]);
+    } else if (mode[k] == MM_NORMAL) {
+      const float fb = (float)beta[k];
+#pragma unroll
+      for (int i = 0; i < SPT; i++) {
+        if (fs[i] < 0) continue;
+        const Site st = decode(g, fs[i]);
+        float2 p0v = cadd(rv[k][i][0], rmul(fb, pv[k][i][0]));
+        float2 ppv[3], pmv[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          ppv[d] = cadd(rv[k][i][1 + d], rmul(fb, pv[k][i][1 + d]));
+          pmv[d] = cadd(rv[k][i][4 + d], rmul(fb, pv[k][i][4 + d]));
+        }
+        S::st(pnew[k], fs[i], p0v);
+        float2 tt = apply_Mdag_fast_vals(g, st, smv[i], p0v, ppv, pmv);
+        S::st((T*)a.t + off, fs[i], tt);
+        acc[k] += dnorm2(tt);
+      }
+    }
+  }
+  if (mode[0] != MM_NORMAL && mode[1] != MM_NORMAL) return;  // uniform
+  const double2 tot = block_sum2(acc[0], acc[1], lds);
+  const double totk[2] = {tot.x, tot.y};
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    if (mode[k] != MM_NORMAL) continue;
+    if (threadIdx.x == 0) pout[(size_t)rr[k] * a.nb + blockIdx.x] = totk[k];
+    if (lead) put_mstate(sout + rr[k], 1, sk[k]->iters, PH_NORMAL, nrel[k], 0, 0, rhok[k], rmax[k], *sk[k]);
+  }
+}
+
+template <typename S, int SPT, bool XCD>
+__global__ __launch_bounds__(MIX_BS) void k_mix_Y2(MixFix a, int odd, const MixState* __restrict__ sin,
+                                                   MixState* __restrict__ sout, const double* __restrict__ pin,
+                                                   double* __restrict__ pout) {
+  __shared__ double lds[2 * (MIX_BS / 64)];
+  using T = typename S::T;
+  const Geom g = a.g;
+  const int r0 = 2 * blockIdx.y;
+  const bool two = r0 + 1 < a.nrhs;
+  const int rr[2] = {r0, two ? r0 + 1 : r0};
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const MixState s0 = sin[rr[0]];
+  const MixState s1 = sin[rr[1]];
+  const double sh0 = partial_share(pin + (size_t)rr[0] * a.nb, a.nb);
+  const double sh1 = two ? partial_share(pin + (size_t)rr[1] * a.nb, a.nb) : 0.0;
+  const MixState* sk[2] = {&s0, &s1};
+  int mode[2];
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    if (k == 1 && !two) {
+      mode[1] = MM_OFF;
+      continue;
+    }
+    if (!sk[k]->active) {
+      if (lead)
+        put_mstate(sout + rr[k], 0, sk[k]->iters, sk[k]->phase, sk[k]->nrel, sk[k]->converged, sk[k]->early,
+                   sk[k]->rho, sk[k]->rmax, *sk[k]);
+      mode[k] = MM_OFF;
+    } else {
+      mode[k] = sk[k]->phase == PH_UPDATE ? MM_UPDATE : MM_NORMAL;
+    }
+  }
+  if (mode[0] == MM_OFF && mode[1] == MM_OFF) return;  // uniform
+  // normal-step loads (both systems), issued before the reduction
+  int fs[SPT];
+  Site st[SPT];
+  float2 w[2][SPT], pv[2][SPT], xv[2][SPT], rv[2][SPT];
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    fs[i] = mix_site<SPT, XCD>(a, i);
+    if (fs[i] < 0) continue;
+    st[i] = decode(g, fs[i]);
+    NbrF n = neighbours_f(g, st[i]);
+    const float smv = a.sm32[fs[i]];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      if (mode[k] != MM_NORMAL) continue;
+      const size_t off = (size_t)rr[k] * g.V;
+      const T* t = (const T*)a.t + off;
+      float2 vpp[3], vmm[3];
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        vpp[d] = S::ld(t, n.pp[d]);
+        vmm[d] = S::ld(t, n.mm[d]);
+      }
+      w[k][i] = apply_M_vals(g, st[i], smv, S::ld(t, fs[i]), vpp, vmm);
+      pv[k][i] = S::ld((const T*)(odd ? a.p0 : a.p1) + off, fs[i]);
+      xv[k][i] = a.xs[off + fs[i]];
+      rv[k][i] = S::ld((const T*)a.r + off, fs[i]);
+    }
+  }
+  const double2 gam = block_sum2(sh0, sh1, lds);
+  const double gamk[2] = {gam.x, gam.y};
+  double acc[2] = {0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const size_t off = (size_t)rr[k] * g.V;
+    if (mode[k] == MM_UPDATE) {
+      // reliable update, part 2 (as k_mix_Y)
+      const double2* u = a.u + off;
+#pragma unroll
+      for (int i = 0; i < SPT; i++) {
+        if (fs[i] < 0) continue;
+        const int f = fs[i];
+        float2 c = a.xs[off + f];
+        a.x[off + f] = cadd(a.x[off + f], make_double2((double)c.x, (double)c.y));
+        a.xs[off + f] = make_float2(0.f, 0.f);
+        double2 rn = apply_sub_M(g, st[i], a.sm[f], a.b[off + f], [&](int q) { return u[q]; });
+        S::st((T*)a.r + off, f, make_float2((float)rn.x, (float)rn.y));
+        acc[k] += cnorm2(rn);
+      }
+    } else if (mode[k] == MM_NORMAL) {
+      const float al = (float)(sk[k]->rho / gamk[k]);
+#pragma unroll
+      for (int i = 0; i < SPT; i++) {
+        if (fs[i] < 0) continue;
+        st_nt(&a.xs[off + fs[i]], make_float2(xv[k][i].x + al * pv[k][i].x, xv[k][i].y + al * pv[k][i].y));
+        float2 rn = make_float2(rv[k][i].x - al * w[k][i].x, rv[k][i].y - al * w[k][i].y);
+        S::st((T*)a.r + off, fs[i], rn);
+        acc[k] += dnorm2(rn);
+      }
+    }
+  }
+  const double2 tot = block_sum2(acc[0], acc[1], lds);
+  const double totk[2] = {tot.x, tot.y};
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    if (mode[k] == MM_OFF) continue;
+    if (threadIdx.x == 0) pout[(size_t)rr[k] * a.nb + blockIdx.x] = totk[k];
+    if (lead) {
+      if (mode[k] == MM_UPDATE)
+        put_mstate(sout + rr[k], 1, sk[k]->iters, PH_UPDATED, sk[k]->nrel, 0, 0, sk[k]->rho, sk[k]->rmax, *sk[k]);
+      else
+        put_mstate(sout + rr[k], 1, sk[k]->iters + 1, PH_NORMAL, sk[k]->nrel, 0, 0, sk[k]->rho, sk[k]->rmax, *sk[k]);
+    }
+  }
 }
 
 // ---- host side ---------------------------------------------------------------
@@ -492,6 +758,8 @@ static hipError_t mix_ensure(MixMachine* m, const Geom& g, int nrhs) {
 // launch shapes: sites per thread (the kernels of one solve share it)
 struct MixShape {
   int spt;
+  bool xcd;
+  bool pair;  // two systems per block (k_mix_X2 / k_mix_Y2)
   const void* X;
   const void* Y;
   const void* start_imposed;
@@ -500,16 +768,26 @@ struct MixShape {
   const void* null_p3;
 };
 
-#define GNX_MIX_SHAPE(S, N)                                                                              \
-  {N, (const void*)k_mix_X<S, N>, (const void*)k_mix_Y<S, N>, (const void*)k_mix_start_imposed<S, N>, \
-   (const void*)k_mix_null_p1<N>, (const void*)k_mix_null_p2<N>, (const void*)k_mix_null_p3<S, N>}
+#define GNX_MIX_SHAPE(S, N, X)                                                                     \
+  {N, X, false, (const void*)k_mix_X<S, N, X>, (const void*)k_mix_Y<S, N, X>,                     \
+   (const void*)k_mix_start_imposed<S, N, X>, (const void*)k_mix_null_p1<N, X>,                   \
+   (const void*)k_mix_null_p2<N, X>, (const void*)k_mix_null_p3<S, N, X>}
+#define GNX_MIX_PAIR(S, N, X)                                                                      \
+  {N, X, true, (const void*)k_mix_X2<S, N, X>, (const void*)k_mix_Y2<S, N, X>,                    \
+   (const void*)k_mix_start_imposed<S, N, X>, (const void*)k_mix_null_p1<N, X>,                   \
+   (const void*)k_mix_null_p2<N, X>, (const void*)k_mix_null_p3<S, N, X>}
+// shapes 0..5: one system per block; 6..9: two (batches of two or more)
 template <typename S>
 static const MixShape* mix_shapes() {
-  static const MixShape t[] = {GNX_MIX_SHAPE(S, 1), GNX_MIX_SHAPE(S, 2), GNX_MIX_SHAPE(S, 4)};
+  static const MixShape t[] = {GNX_MIX_SHAPE(S, 1, false), GNX_MIX_SHAPE(S, 1, true), GNX_MIX_SHAPE(S, 2, false),
+                               GNX_MIX_SHAPE(S, 2, true),  GNX_MIX_SHAPE(S, 4, false), GNX_MIX_SHAPE(S, 4, true),
+                               GNX_MIX_PAIR(S, 1, false),  GNX_MIX_PAIR(S, 1, true),   GNX_MIX_PAIR(S, 2, false),
+                               GNX_MIX_PAIR(S, 2, true)};
   return t;
 }
 #undef GNX_MIX_SHAPE
-constexpr int kMixShapes = 3;
+#undef GNX_MIX_PAIR
+constexpr int kMixShapes = 10;
 
 static MixFix mix_fix(MixMachine* m, const Geom& g, int nrhs, int spt) {
   MixFix a;
@@ -536,7 +814,7 @@ template <typename S>
 static hipError_t mix_graph(MixMachine* m, const Geom& g, int nrhs, int shape, int tag, hipGraphExec_t* exec) {
   const MixShape& sh = mix_shapes<S>()[shape];
   MixFix a = mix_fix(m, g, nrhs, sh.spt);
-  dim3 grid(a.nb, nrhs), blk(MIX_BS);
+  dim3 grid(a.nb, sh.pair ? (nrhs + 1) / 2 : nrhs), blk(MIX_BS);
   const MixState* s0 = a.state;
   const MixState* s1 = a.state + nrhs;
   MixState* w0 = a.state;
@@ -581,7 +859,7 @@ static hipError_t mix_begin(MixMachine* m, const Geom& g, const double* sm, cons
 }
 
 // Autotuner: two chunks of every shape on the caller's own system, fastest
-// kept per (batch size, storage format); GNX_MIX_SHAPE=<0..2> pins one.
+// kept per (batch size, storage format); GNX_MIX_SHAPE=<0..9> pins one.
 template <typename S>
 static hipError_t mix_tune(MixMachine* m, const Geom& g, const double* sm, const double2* b, double delta,
                            int nrhs, int tag, hipStream_t st, int* best) {
@@ -595,11 +873,12 @@ static hipError_t mix_tune(MixMachine* m, const Geom& g, const double* sm, const
   int best_shape = 0;
   hipError_t e = hipSuccess;
   const char* env = getenv("GNX_MIX_SHAPE");
-  if (env && atoi(env) >= 0 && atoi(env) < kMixShapes) {
+  if (env && atoi(env) >= 0 && atoi(env) < kMixShapes && (nrhs >= 2 || !mix_shapes<S>()[atoi(env)].pair)) {
     best_shape = atoi(env);
   } else {
     float best_ms = 1e30f;
     for (int sh = 0; sh < kMixShapes && e == hipSuccess; sh++) {
+      if (mix_shapes<S>()[sh].pair && nrhs < 2) continue;
       hipGraphExec_t exec;
       e = mix_graph<S>(m, g, nrhs, sh, tag, &exec);
       float ms_min = 1e30f;
@@ -663,12 +942,18 @@ hipError_t cgmix_run(void** machine_slot, const Geom& g, const double* sm, const
   return mix_run_t<StoreBF16>(m, g, sm, b, x, tol, delta, max_iter, start, nrhs, st, host_res, 101);
 }
 
-int cgmix_spt_of(void* machine, int nrhs, int prec) {
+int cgmix_shape_of(void* machine, int nrhs, int prec, int* spt, int* xcd, int* pair) {
   MixMachine* m = (MixMachine*)machine;
   if (!m) return -1;
   const int key = nrhs + 1000 * (prec == GNX_PREC_F32 ? 100 : 101);
   for (int i = 0; i < 8; i++)
-    if (m->eng.tuned_tag[i] >= 0 && m->eng.tuned_nrhs[i] == key) return mix_shapes<StoreF32>()[m->eng.tuned_tag[i]].spt;
+    if (m->eng.tuned_tag[i] >= 0 && m->eng.tuned_nrhs[i] == key) {
+      const MixShape& sh = mix_shapes<StoreF32>()[m->eng.tuned_tag[i]];
+      *spt = sh.spt;
+      *xcd = sh.xcd ? 1 : 0;
+      *pair = sh.pair ? 1 : 0;
+      return m->eng.tuned_tag[i];
+    }
   return -1;
 }
 

@@ -139,4 +139,21 @@ GNX_DEVONLY double block_sum(double v, double* lds) {
   return t;
 }
 
+// Two deterministic block sums at once (each bit-identical to block_sum of
+// its input); lds must hold 2*blockDim.x/64 doubles.  Contains __syncthreads().
+GNX_DEVONLY double2 block_sum2(double a, double b, double* lds) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  a = wave_sum(a);
+  b = wave_sum(b);
+  __syncthreads();
+  if (lane == 0) {
+    lds[wid] = a;
+    lds[nw + wid] = b;
+  }
+  __syncthreads();
+  const double ta = (lane < nw) ? lds[lane] : 0.0;
+  const double tb = (lane < nw) ? lds[nw + lane] : 0.0;
+  return make_double2(wave_sum(ta), wave_sum(tb));
+}
+
 }  // namespace gnx

@@ -48,7 +48,9 @@ hipError_t cgmix_run(void** machine_slot, const Geom& g, const double* sm, const
                      double delta, long max_iter, int start, int prec, int nrhs, hipStream_t st,
                      CGDevResult* host_res);
 void cgmix_release(void* machine);
-int cgmix_spt_of(void* machine, int nrhs, int prec);  // tuned sites per thread, -1 if none
+// tuned launch shape (sites per thread, XCD slab mapping, two systems per
+// block); -1 if none yet
+int cgmix_shape_of(void* machine, int nrhs, int prec, int* spt, int* xcd, int* pair);
 int cgm_shape_of(void* machine, int nrhs, int* spt, int* xcd);
 
 void set_err(const char* fmt, ...);

@@ -56,6 +56,20 @@ GNX_DEV T2 apply_M(const Geom& g, const Site& s, typename CT<T2>::R sm, LoadF lo
   return acc;
 }
 
+// M v from operand values already in registers, the order of apply_M
+template <typename T2>
+GNX_DEV T2 apply_M_vals(const Geom& g, const Site& s, typename CT<T2>::R sm, T2 v0, const T2 vpp[3],
+                        const T2 vmm[3]) {
+  const HalfEta e = half_eta(g, s);
+  T2 acc = rmul(sm, v0);
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    acc = cadd(acc, rmul(e.pp[d], vpp[d]));
+    acc = cadd(acc, rmul(-e.mm[d], vmm[d]));
+  }
+  return acc;
+}
+
 // r - M v, reference order (r -= sm*v, then -= pp, += mm per direction)
 template <typename LoadF, typename T2 = decltype(std::declval<LoadF>()(0))>
 GNX_DEV T2 apply_sub_M(const Geom& g, const Site& s, typename CT<T2>::R sm, T2 r, LoadF load) {
@@ -128,6 +142,23 @@ GNX_DEV T2 apply_Mdag_vals(const Geom& g, const Site& s, typename CT<T2>::R sm, 
     tpp[d] = rmul(cp[d], vpp[d]);
   }
   return accumulate_adj_order(g, s, rmul(sm, v0), tpp, tmm);
+}
+
+// M^+ in the plain gather form, M^+ = sm - D with D real antisymmetric:
+//   y = sm v - sum_d [0.5 eta_pp(x,d) v(x+d) - 0.5 eta_mm(x,d) v(x-d)]
+// For the reduced-precision inner solvers, which keep no reference order
+// (the exact-order form above costs about 30 selects per site).
+template <typename T2>
+GNX_DEV T2 apply_Mdag_fast_vals(const Geom& g, const Site& s, typename CT<T2>::R sm, T2 v0, const T2 vpp[3],
+                                const T2 vmm[3]) {
+  const HalfEta e = half_eta(g, s);
+  T2 acc = rmul(sm, v0);
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    acc = csub(acc, rmul(e.pp[d], vpp[d]));
+    acc = cadd(acc, rmul(e.mm[d], vmm[d]));
+  }
+  return acc;
 }
 
 template <typename LoadF, typename T2 = decltype(std::declval<LoadF>()(0))>

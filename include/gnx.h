@@ -131,8 +131,12 @@ gnx_status gnx_set_reliable_delta(gnx_ctx *ctx, double delta);
  * pins the shape instead of tuning. */
 gnx_status gnx_cg_shape(gnx_ctx *ctx, int nrhs, int *sites_per_thread, int *xcd_mapped);
 /* The same for the mixed-precision solver (prec GNX_PREC_F32 or GNX_PREC_BF16);
- * GNX_MIX_SHAPE=<0..2> pins 1, 2 or 4 sites per thread. */
-gnx_status gnx_cg_mixed_shape(gnx_ctx *ctx, int nrhs, gnx_prec prec, int *sites_per_thread);
+ * pair = 1: two systems share each block's site arithmetic.  GNX_MIX_SHAPE=<n>
+ * pins one: n = 0..5 one system per block with 1, 1, 2, 2, 4, 4 sites per
+ * thread, n = 6..9 two systems per block with 1, 1, 2, 2; odd n use the XCD
+ * slab mapping. */
+gnx_status gnx_cg_mixed_shape(gnx_ctx *ctx, int nrhs, gnx_prec prec, int *sites_per_thread, int *xcd_mapped,
+                              int *pair);
 
 /* A9. Sigma force (moleculardynamics.c:590-647):
  *   X_f = CG(phi_f) with START_X0_NULL from the incoming X_f, Y_f = M^dag X_f,

@@ -43,26 +43,39 @@ def test_mixed_vs_oracle(P, gnx_mod, prec, L):
     banner(f"  iterations fp64={st.last_iters} {prec}={res.iterations} updates={res.reliable_updates}")
 
 
+@pytest.mark.parametrize("shape", list(range(10)))
 @pytest.mark.parametrize("prec", ["f32", "bf16"])
-def test_mixed_batch_and_null_start(P, gnx_mod, prec):
-    banner(f"mixed {prec} batch of 3 and null-start early exit")
+def test_mixed_batch_and_null_start(P, gnx_mod, prec, shape, monkeypatch):
+    """Every launch shape: 0..5 one system per block (1/2/4 sites per thread,
+    with and without the XCD slab mapping), 6..9 two systems per block.  A
+    batch of 3 (the last pair half empty) is bit-identical to 3 single solves
+    of the same site mapping, and each solution meets the FP64 bar."""
+    banner(f"mixed {prec} shape {shape}: batch of 3 and null-start early exit")
     import torch
     L = 12
     lat = P.Lattice(L)
     r = P.Random(41)
     sigma = r.sigma(lat.V, P.sigma_variance(0.85))
-    sm = dev(lat.sigma_mass(sigma))
+    smh = lat.sigma_mass(sigma)
+    sm = dev(smh)
     bs = [r.z2_complex(lat.V) for _ in range(3)]
     ctx = gnx_mod.Context(L)
     p = gnx_mod.PREC_F32 if prec == "f32" else gnx_mod.PREC_BF16
+    monkeypatch.setenv("GNX_MIX_SHAPE", str(shape))
     X = ctx.vec(3)
     st, res = ctx.cg_batch(sm, dev(np.concatenate(bs)), X, 3, 1e-10, start=gnx_mod.START_IMPOSED, prec=p)
     assert st == 0
+    pair = shape >= 6
+    assert ctx.cg_mixed_shape(3, p) == {"sites_per_thread": [1, 1, 2, 2, 4, 4, 1, 1, 2, 2][shape],
+                                        "xcd_slab": shape % 2 == 1, "pair": pair}
+    monkeypatch.setenv("GNX_MIX_SHAPE", str(shape - 6 if pair else shape))
     for k in range(3):
         x = ctx.vec()
         s1, r1 = ctx.cg(sm, dev(bs[k]), x, 1e-10, start=gnx_mod.START_IMPOSED, prec=p)
         assert r1.iterations == res[k].iterations
-        assert np.array_equal(host(x), host(X[k * 2 * lat.V:(k + 1) * 2 * lat.V]))
+        xk = host(X[k * 2 * lat.V:(k + 1) * 2 * lat.V])
+        assert np.array_equal(host(x), xk)
+        assert true_res(lat, smh, bs[k], xk) < 1e-10
     # null start from the converged solution: the prologue exits, x untouched
     x = X[:2 * lat.V].clone()
     before = host(x).copy()
