@@ -70,7 +70,7 @@ struct gnx_ctx {
   double* host_out = nullptr;   // 16 pinned host scalars
   void* cgm = nullptr;          // graph-replayed CG state machine (gnx_cgm.hip)
   void* cgmix = nullptr;        // mixed-precision reliable-update CG (gnx_cgmix.hip)
-  double reliable_delta = 0.1;  // residual drop that triggers an FP64 reliable update
+  double reliable_delta = 0.0;  // residual drop that triggers an FP64 reliable update (0: per precision)
   double* bat_part = nullptr;   // per-vector partials of the batched trace terms
   size_t bat_cap = 0;           // doubles in bat_part
   double* bat_out = nullptr;    // 64 device scalars

@@ -45,7 +45,8 @@ def kernel_sources_sha():
 
 
 def shape_tag(shape):
-    return f"<{shape['sites_per_thread']}, {'true' if shape['xcd_slab'] else 'false'}>"
+    # kernel template arguments: <sites per thread, XCD slab mapping, link-coupled operator>
+    return f"<{shape['sites_per_thread']}, {'true' if shape['xcd_slab'] else 'false'}, false>"
 
 
 def recorded_traffic(L, shape):
@@ -285,7 +286,7 @@ def main():
     ap.add_argument("--L", type=int, default=64)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--batch", default="4,16", help="batch sizes of the batched FP64 CG leg ('' to skip)")
+    ap.add_argument("--batch", default="16,32", help="batch sizes of the batched FP64 CG leg ('' to skip)")
     ap.add_argument("--mixed", default="f32,bf16", help="precisions of the mixed-precision legs ('' to skip)")
     ap.add_argument("--condensate", default="48:64", metavar="L:N",
                     help="batched stochastic condensate leg: lattice L^3, N complex Z2 sources ('' to skip)")
