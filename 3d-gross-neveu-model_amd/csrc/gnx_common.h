@@ -113,47 +113,78 @@ GNX_DEV double cdot_im(double2 a, double2 b) { return a.x * b.y - a.y * b.x; }
 
 
 // ---------------------------------------------------------------- reductions
-// Deterministic wavefront sum (fixed butterfly order, 64 lanes).
-GNX_DEVONLY double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+// Deterministic wavefront sum.  Within each 16-lane row, four DPP exchanges
+// (quad xor 1, quad xor 2, half-row mirror, row mirror) each add a lane's
+// partial sum to its partner's; the partner adds the same two values in the
+// other operand order, so both hold the same bits.  The four row sums are
+// then read out and added as (r0 + r1) + (r2 + r3): the result is
+// wave-uniform.  DPP moves and lane reads are VALU operations, so no step
+// waits on the LDS crossbar (ds_bpermute), which __shfl_xor would use.
+template <int CTRL>
+GNX_DEVONLY double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+template <int CTRL>
+GNX_DEVONLY float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+// DPP controls: quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror
+constexpr int DPP_QXOR1 = 0xB1, DPP_QXOR2 = 0x4E, DPP_HALF_MIRROR = 0x141, DPP_MIRROR = 0x140;
+
+GNX_DEVONLY double lane_read(double v, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                          __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+GNX_DEVONLY float lane_read(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-GNX_DEVONLY float wave_sumf(float v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+template <typename T>
+GNX_DEVONLY T wave_sum_t(T v) {
+  v += dpp_move<DPP_QXOR1>(v);
+  v += dpp_move<DPP_QXOR2>(v);
+  v += dpp_move<DPP_HALF_MIRROR>(v);
+  v += dpp_move<DPP_MIRROR>(v);
+  return (lane_read(v, 0) + lane_read(v, 16)) + (lane_read(v, 32) + lane_read(v, 48));
 }
+GNX_DEVONLY double wave_sum(double v) { return wave_sum_t(v); }
+GNX_DEVONLY float wave_sumf(float v) { return wave_sum_t(v); }
 
-// Deterministic block sum; every thread gets the result. lds must hold
-// blockDim.x/64 doubles. Contains __syncthreads().
+// Deterministic block sum; every thread gets the same bits: the wave sums
+// added in wave order.  lds must hold blockDim.x/64 doubles.  Two
+// __syncthreads(): the first keeps a previous call's readers ahead of this
+// call's writes to lds.
 GNX_DEVONLY double block_sum(double v, double* lds) {
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
   v = wave_sum(v);
   __syncthreads();
-  if (lane == 0) lds[wid] = v;
+  if ((threadIdx.x & 63) == 0) lds[wid] = v;
   __syncthreads();
-  double t = (lane < nw) ? lds[lane] : 0.0;
-  t = wave_sum(t);
+  double t = lds[0];
+  for (int w = 1; w < nw; w++) t += lds[w];
   return t;
 }
 
 // Two deterministic block sums at once (each bit-identical to block_sum of
 // its input); lds must hold 2*blockDim.x/64 doubles.  Contains __syncthreads().
 GNX_DEVONLY double2 block_sum2(double a, double b, double* lds) {
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
   a = wave_sum(a);
   b = wave_sum(b);
   __syncthreads();
-  if (lane == 0) {
+  if ((threadIdx.x & 63) == 0) {
     lds[wid] = a;
     lds[nw + wid] = b;
   }
   __syncthreads();
-  const double ta = (lane < nw) ? lds[lane] : 0.0;
-  const double tb = (lane < nw) ? lds[nw + lane] : 0.0;
-  return make_double2(wave_sum(ta), wave_sum(tb));
+  double ta = lds[0], tb = lds[nw];
+  for (int w = 1; w < nw; w++) {
+    ta += lds[w];
+    tb += lds[nw + w];
+  }
+  return make_double2(ta, tb);
 }
 
 }  // namespace gnx
