@@ -25,9 +25,9 @@
 // of the same kernel writes.  Systems of a batch (blockIdx.y) are
 // independent: each keeps its alpha, beta and exit iteration.
 //
-// Latency: each kernel issues every load it needs (state, partials, the
-// stencil operands) before its first reduction, so one memory round trip is
-// on the critical path.  Kernel arguments are passed by value and point at
+// Latency: each kernel issues every load it needs before its first wait --
+// the state and the partials first, then the stencil operands, all vector
+// loads -- and reduces the partials while the operands are in flight.  Kernel arguments are passed by value and point at
 // buffers this solver owns (the caller's sigma mass and x are copied in and
 // the solution copied out), and the parity of p_old/p_new is baked into each
 // graph node (one slot pair = one iteration, a chunk holds an even count).
@@ -67,7 +67,7 @@ GNX_DEVONLY void put_state(CGMState* o, int active, int iters, int conv, int ear
   o->tol = cfg.tol;
   o->max_iter = cfg.max_iter;
   o->start = cfg.start;
-  o->pad = 0;
+  o->pad = cfg.pad;  // every loaded word stays live: a dead load destination is waited on early
 }
 
 // kernel arguments: fixed per solver instance, passed by value
@@ -220,9 +220,13 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_A(CGMFix a, int odd, const CGMSt
   const double2* pold = (odd ? a.p1 : a.p0) + off;
   double2* pnew = (odd ? a.p0 : a.p1) + off;
   const SiteMap<SPT, XCD> map(a.nb);
-  // issue every load before the reduction
-  const CGMState s = sin[rhs];
-  const double share = partial_share(pin + (size_t)rhs * a.nb, a.nb);
+  // loads in the order they are needed: state and partials, then the
+  // operands; the reduction waits for the first group only (loads return in
+  // issue order) and runs while the operands are in flight
+  const CGMState s = load_state(sin + rhs);
+  double pl[PART_K];
+  partial_issue(pin + (size_t)rhs * a.nb, a.nb, pl);
+  end_of_loads();
   int fs[SPT];
   double smv[SPT];
   double2 rv[SPT][7], pv[SPT][7];
@@ -230,18 +234,21 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_A(CGMFix a, int odd, const CGMSt
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
     fs[i] = map.f(g, i);
-    if (fs[i] < 0) continue;
-    NbrF n = neighbours_f(g, decode(g, fs[i]));
-    int q[7] = {fs[i], n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
-    smv[i] = a.sm[fs[i]];
-    if (LINK) load_links(a.U, fs[i], n, Uo[LINK ? i : 0], Um[LINK ? i : 0]);
+    // a thread past the last site loads site 0 and stores nothing: the loads
+    // stay one branch-free block, so none of them waits for another
+    const int fc = fs[i] < 0 ? 0 : fs[i];
+    NbrF n = neighbours_f(g, decode(g, fc));
+    int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
+    smv[i] = a.sm[fc];
+    if (LINK) load_links(a.U, fc, n, Uo[LINK ? i : 0], Um[LINK ? i : 0]);
 #pragma unroll
     for (int k = 0; k < 7; k++) {
       rv[i][k] = r[q[k]];
       pv[i][k] = pold[q[k]];
     }
   }
-  const double rho = block_sum(share, lds);
+  end_of_loads();
+  const double rho = block_sum(partial_finish(pin + (size_t)rhs * a.nb, a.nb, pl), lds);
   CGMState* out = sout + rhs;
   if (!s.active) {
     if (lead) put_state(out, 0, s.iters, s.converged, s.early, s.rho, s);
@@ -310,22 +317,33 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_B(CGMFix a, int odd, const CGMSt
   double2* r = a.r + off;
   double2* x = a.x + off;
   const SiteMap<SPT, XCD> map(a.nb);
-  const CGMState s = sin[rhs];
-  const double share = partial_share(pin + (size_t)rhs * a.nb, a.nb);
+  // loads in the order they are needed (as in A)
+  const CGMState s = load_state(sin + rhs);
+  double pl[PART_K];
+  partial_issue(pin + (size_t)rhs * a.nb, a.nb, pl);
+  end_of_loads();
   int fs[SPT];
-  double2 w[SPT], pv[SPT], xv[SPT], rv[SPT];
+  double smv[SPT];
+  double2 tv[SPT][7], pv[SPT], xv[SPT], rv[SPT];
+  double2 Uo[LINK ? SPT : 1][3], Um[LINK ? SPT : 1][3];
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
     fs[i] = map.f(g, i);
-    if (fs[i] < 0) continue;
-    w[i] = LINK ? apply_link(g, decode(g, fs[i]), 0, a.sm[fs[i]], make_double2(0.0, 0.0), a.U,
-                             [&](int q) { return t[q]; })
-                : apply_M(g, decode(g, fs[i]), a.sm[fs[i]], [&](int q) { return t[q]; });
-    pv[i] = pnew[fs[i]];
-    xv[i] = x[fs[i]];
-    rv[i] = r[fs[i]];
+    // a thread past the last site loads site 0 and stores nothing: the loads
+    // stay one branch-free block, so none of them waits for another
+    const int fc = fs[i] < 0 ? 0 : fs[i];
+    NbrF n = neighbours_f(g, decode(g, fc));
+    int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
+    smv[i] = a.sm[fc];
+    if (LINK) load_links(a.U, fc, n, Uo[LINK ? i : 0], Um[LINK ? i : 0]);
+#pragma unroll
+    for (int k = 0; k < 7; k++) tv[i][k] = t[q[k]];
+    pv[i] = pnew[fc];
+    xv[i] = x[fc];
+    rv[i] = r[fc];
   }
-  const double gamma = block_sum(share, lds);
+  end_of_loads();
+  const double gamma = block_sum(partial_finish(pin + (size_t)rhs * a.nb, a.nb, pl), lds);
   CGMState* out = sout + rhs;
   if (!s.active) {
     if (lead) put_state(out, 0, s.iters, s.converged, s.early, s.rho, s);
@@ -336,8 +354,13 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_B(CGMFix a, int odd, const CGMSt
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
     if (fs[i] < 0) continue;
+    // M t from the preloaded operands, the order of apply_M / apply_link
+    const Site st = decode(g, fs[i]);
+    const double2 m0 = rmul(smv[i], tv[i][0]);
+    const double2 w = LINK ? link_hops<1>(g, st, m0, &tv[i][1], &tv[i][4], Uo[LINK ? i : 0], Um[LINK ? i : 0])
+                           : apply_M_vals(g, st, smv[i], tv[i][0], &tv[i][1], &tv[i][4]);
     st_nt(&x[fs[i]], cadd(xv[i], rmul(al, pv[i])));
-    double2 rr = cadd(rv[i], rmul(-al, w[i]));
+    double2 rr = cadd(rv[i], rmul(-al, w));
     st_nt(&r[fs[i]], rr);
     acc += cnorm2(rr);
   }

@@ -46,22 +46,60 @@ GNX_DEVONLY void st_nt(float2* p, float2 v) {
 }
 GNX_DEVONLY void st_nt(unsigned int* p, unsigned int v) { __builtin_nontemporal_store(v, p); }
 
-// per-thread share of n partials, fixed order; the loads of a round are
-// issued together (one memory latency per 8*BS partials, not one per load)
-GNX_DEVONLY double partial_share(const double* __restrict__ src, int n) {
-  constexpr int K = 8;
-  double acc = 0.0;
-  for (int base = 0; base < n; base += K * ENG_BS) {
-    double v[K];
+// Per-thread share of n >= 1 partials, fixed order, in two steps so that a
+// kernel can issue every load it needs before it waits for any:
+// partial_issue loads the first round (up to 4*BS partials) with no branch
+// -- every lane reads an in-bounds index, clamped to n-1 -- and
+// partial_finish masks the surplus, sums, and continues in a loop when n is
+// larger.
+constexpr int PART_K = 4;
+GNX_DEVONLY void partial_issue(const double* __restrict__ src, int n, double v[PART_K]) {
 #pragma unroll
-    for (int k = 0; k < K; k++) {
+  for (int k = 0; k < PART_K; k++) {
+    const int i = k * ENG_BS + (int)threadIdx.x;
+    v[k] = src[i < n ? i : n - 1];
+  }
+}
+GNX_DEVONLY double partial_finish(const double* __restrict__ src, int n, const double v[PART_K]) {
+  double acc = 0.0;
+#pragma unroll
+  for (int k = 0; k < PART_K; k++) acc += (k * ENG_BS + (int)threadIdx.x < n) ? v[k] : 0.0;
+  for (int base = PART_K * ENG_BS; base < n; base += PART_K * ENG_BS) {
+    double w[PART_K];
+#pragma unroll
+    for (int k = 0; k < PART_K; k++) {
       const int i = base + k * ENG_BS + (int)threadIdx.x;
-      v[k] = i < n ? src[i] : 0.0;
+      w[k] = i < n ? src[i] : 0.0;
     }
 #pragma unroll
-    for (int k = 0; k < K; k++) acc += v[k];
+    for (int k = 0; k < PART_K; k++) acc += w[k];
   }
   return acc;
+}
+GNX_DEVONLY double partial_share(const double* __restrict__ src, int n) {
+  double v[PART_K];
+  partial_issue(src, n, v);
+  return partial_finish(src, n, v);
+}
+
+// Marks the end of a kernel's load phase: the scheduler moves no
+// instruction across it, so no arithmetic on a loaded value (and no wait for
+// one) lands between the loads.
+GNX_DEVONLY void end_of_loads() { __builtin_amdgcn_sched_barrier(0); }
+
+// Per-system solver state, written by the previous kernel, read with cached
+// vector loads.  A uniform address would be read with a scalar load, waited
+// on with lgkmcnt(0) -- a wait that also covers every later scalar load of a
+// kernel argument -- so one more memory round trip would sit ahead of the
+// operand loads.  mbcnt_lo(0, 0) is 0 in every lane, but the compiler cannot
+// treat the index as uniform, so the fields come in as vector loads that
+// return with the operands.  (Atomic loads would also be vector loads, but
+// at device scope they bypass the caches and every wave of the grid would
+// queue on one line.)
+template <typename T>
+GNX_DEVONLY T load_state(const T* p) {
+  const int zero = (int)__builtin_amdgcn_mbcnt_lo(0u, 0u);
+  return p[zero];
 }
 
 }  // namespace gnx
