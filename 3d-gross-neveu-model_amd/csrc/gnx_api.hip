@@ -255,9 +255,10 @@ gnx_status gnx_cg_shape(gnx_ctx* c, int nrhs, int* spt, int* xcd) {
   return GNX_OK;
 }
 
-gnx_status gnx_cg_mixed_shape(gnx_ctx* c, int nrhs, gnx_prec prec, int* spt, int* xcd, int* pair) {
-  CHECK_ARG(c && spt && xcd && pair, "gnx_cg_mixed_shape: null argument");
-  if (cgmix_shape_of(c->cgmix, nrhs, (int)prec, spt, xcd, pair) < 0) {
+gnx_status gnx_cg_mixed_shape(gnx_ctx* c, int nrhs, gnx_prec prec, int* spt, int* xcd, int* pair,
+                              int* loads_first) {
+  CHECK_ARG(c && spt && xcd && pair && loads_first, "gnx_cg_mixed_shape: null argument");
+  if (cgmix_shape_of(c->cgmix, nrhs, (int)prec, spt, xcd, pair, loads_first) < 0) {
     set_err("gnx_cg_mixed_shape: no tuned shape for nrhs=%d yet", nrhs);
     return GNX_ERR_STATE;
   }

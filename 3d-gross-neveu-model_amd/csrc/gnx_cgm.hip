@@ -16,9 +16,11 @@
 //   B(k): gamma from A's partials; alpha = rho_k / gamma; x += alpha p_new;
 //         r += (-alpha) M t; partial |r|^2.
 //
-// The per-site arithmetic is the reference's (x += alpha p, r -= alpha M t,
-// p = r + beta p, M and M^dag in reference order); only the global sums are
-// taken in a fixed tree order.  Every block reduces the previous kernel's
+// The per-site updates are the reference's (x += alpha p, r -= alpha M t,
+// p = r + beta p, M in reference order); M^dag is applied in its gather form
+// (sm v minus the hops, no reordering selects) and the global sums are taken
+// in a fixed tree order, so solutions agree with the reference to the
+// solver tolerance.  Every block reduces the previous kernel's
 // per-block partials in the same order, so all blocks hold bit-identical
 // scalars.  Per-system state is double-buffered: A reads state[0] and its
 // block 0 writes state[1]; B the reverse; no block reads what another block
@@ -211,7 +213,7 @@ template <int SPT, bool XCD, bool LINK>
 __global__ __launch_bounds__(CGM_BS) void k_cgm_A(CGMFix a, int odd, const CGMState* __restrict__ sin,
                                                    CGMState* __restrict__ sout, const double* __restrict__ pin,
                                                    double* __restrict__ pout) {
-  __shared__ double lds[CGM_BS / 64];
+  __shared__ double lds[CGM_BS / 64], lds2[CGM_BS / 64];  // head and tail reductions
   const Geom g = a.g;
   const int rhs = blockIdx.y;
   const size_t off = (size_t)rhs * g.V;
@@ -239,16 +241,16 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_A(CGMFix a, int odd, const CGMSt
     const int fc = fs[i] < 0 ? 0 : fs[i];
     NbrF n = neighbours_f(g, decode(g, fc));
     int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
-    smv[i] = a.sm[fc];
+    smv[i] = *elem(a.sm, fc);
     if (LINK) load_links(a.U, fc, n, Uo[LINK ? i : 0], Um[LINK ? i : 0]);
 #pragma unroll
     for (int k = 0; k < 7; k++) {
-      rv[i][k] = r[q[k]];
-      pv[i][k] = pold[q[k]];
+      rv[i][k] = *elem(r, q[k]);
+      pv[i][k] = *elem(pold, q[k]);
     }
   }
   end_of_loads();
-  const double rho = block_sum(partial_finish(pin + (size_t)rhs * a.nb, a.nb, pl), lds);
+  const double rho = block_sum_once(partial_finish(pin + (size_t)rhs * a.nb, a.nb, pl), lds);
   CGMState* out = sout + rhs;
   if (!s.active) {
     if (lead) put_state(out, 0, s.iters, s.converged, s.early, s.rho, s);
@@ -292,13 +294,13 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_A(CGMFix a, int odd, const CGMSt
       ppv[d] = cadd(rv[i][1 + d], rmul(beta, pv[i][1 + d]));
       pmv[d] = cadd(rv[i][4 + d], rmul(beta, pv[i][4 + d]));
     }
-    st_nt(&pnew[fs[i]], p0v);
+    st_nt(elem(pnew, fs[i]), p0v);
     double2 tt = LINK ? link_hops<-1>(g, st, rmul(smv[i], p0v), ppv, pmv, Uo[LINK ? i : 0], Um[LINK ? i : 0])
-                      : apply_Mdag_vals(g, st, smv[i], p0v, ppv, pmv);
-    st_nt(&a.t[off + fs[i]], tt);
+                      : apply_Mdag_fast_vals(g, st, smv[i], p0v, ppv, pmv);
+    st_nt(elem(a.t + off, fs[i]), tt);
     acc += cnorm2(tt);
   }
-  acc = block_sum(acc, lds);
+  acc = block_sum_once(acc, lds2);
   if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + blockIdx.x] = acc;
   if (lead) put_state(out, 1, s.iters, 0, 0, rho, s);
 }
@@ -307,7 +309,7 @@ template <int SPT, bool XCD, bool LINK>
 __global__ __launch_bounds__(CGM_BS) void k_cgm_B(CGMFix a, int odd, const CGMState* __restrict__ sin,
                                                    CGMState* __restrict__ sout, const double* __restrict__ pin,
                                                    double* __restrict__ pout) {
-  __shared__ double lds[CGM_BS / 64];
+  __shared__ double lds[CGM_BS / 64], lds2[CGM_BS / 64];  // head and tail reductions
   const Geom g = a.g;
   const int rhs = blockIdx.y;
   const size_t off = (size_t)rhs * g.V;
@@ -334,16 +336,16 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_B(CGMFix a, int odd, const CGMSt
     const int fc = fs[i] < 0 ? 0 : fs[i];
     NbrF n = neighbours_f(g, decode(g, fc));
     int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
-    smv[i] = a.sm[fc];
+    smv[i] = *elem(a.sm, fc);
     if (LINK) load_links(a.U, fc, n, Uo[LINK ? i : 0], Um[LINK ? i : 0]);
 #pragma unroll
-    for (int k = 0; k < 7; k++) tv[i][k] = t[q[k]];
-    pv[i] = pnew[fc];
-    xv[i] = x[fc];
-    rv[i] = r[fc];
+    for (int k = 0; k < 7; k++) tv[i][k] = *elem(t, q[k]);
+    pv[i] = *elem(pnew, fc);
+    xv[i] = *elem(x, fc);
+    rv[i] = *elem(r, fc);
   }
   end_of_loads();
-  const double gamma = block_sum(partial_finish(pin + (size_t)rhs * a.nb, a.nb, pl), lds);
+  const double gamma = block_sum_once(partial_finish(pin + (size_t)rhs * a.nb, a.nb, pl), lds);
   CGMState* out = sout + rhs;
   if (!s.active) {
     if (lead) put_state(out, 0, s.iters, s.converged, s.early, s.rho, s);
@@ -359,12 +361,12 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_B(CGMFix a, int odd, const CGMSt
     const double2 m0 = rmul(smv[i], tv[i][0]);
     const double2 w = LINK ? link_hops<1>(g, st, m0, &tv[i][1], &tv[i][4], Uo[LINK ? i : 0], Um[LINK ? i : 0])
                            : apply_M_vals(g, st, smv[i], tv[i][0], &tv[i][1], &tv[i][4]);
-    st_nt(&x[fs[i]], cadd(xv[i], rmul(al, pv[i])));
+    st_nt(elem(x, fs[i]), cadd(xv[i], rmul(al, pv[i])));
     double2 rr = cadd(rv[i], rmul(-al, w));
-    st_nt(&r[fs[i]], rr);
+    st_nt(elem(r, fs[i]), rr);
     acc += cnorm2(rr);
   }
-  acc = block_sum(acc, lds);
+  acc = block_sum_once(acc, lds2);
   if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + blockIdx.x] = acc;
   if (lead) put_state(out, 1, s.iters + 1, 0, 0, s.rho, s);
 }

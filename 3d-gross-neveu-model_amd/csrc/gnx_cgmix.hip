@@ -20,8 +20,9 @@
 //      p = r + beta p, t = M^dag p.
 //   Y: finish an update (x += xs, xs = 0, r = (S)(b - M u), partial |r|^2 in
 //      FP64) or the normal step (xs += alpha p, r -= alpha M t, partial |r|^2).
-// Loads are issued before each kernel's reduction and kernel arguments are
-// passed by value, as in the FP64 solver.
+// As in the FP64 solver, each kernel issues its loads (state and partials,
+// then the operands) before its first wait, and kernel arguments are passed
+// by value.
 #include <stdlib.h>
 #include <string.h>
 
@@ -34,8 +35,8 @@ namespace gnx {
 // ---- storage formats --------------------------------------------------------
 struct StoreF32 {
   using T = float2;
-  GNX_DEVONLY static float2 ld(const T* p, int i) { return p[i]; }
-  GNX_DEVONLY static void st(T* p, int i, float2 v) { st_nt(&p[i], v); }
+  GNX_DEVONLY static float2 ld(const T* p, int i) { return *elem(p, i); }
+  GNX_DEVONLY static void st(T* p, int i, float2 v) { st_nt(elem(p, i), v); }
 };
 
 // bfloat16 pair (re in the low half), round-to-nearest-even on store
@@ -48,10 +49,10 @@ struct StoreBF16 {
     return u >> 16;
   }
   GNX_DEVONLY static float2 ld(const T* p, int i) {
-    unsigned int v = p[i];
+    unsigned int v = *elem(p, i);
     return make_float2(up(v & 0xFFFFu), up(v >> 16));
   }
-  GNX_DEVONLY static void st(T* p, int i, float2 v) { st_nt(&p[i], down(v.x) | (down(v.y) << 16)); }
+  GNX_DEVONLY static void st(T* p, int i, float2 v) { st_nt(elem(p, i), down(v.x) | (down(v.y) << 16)); }
 };
 
 constexpr int MIX_BS = ENG_BS;
@@ -83,7 +84,7 @@ GNX_DEVONLY void put_mstate(MixState* o, int active, int iters, int phase, int n
   o->converged = conv;
   o->early = early;
   o->start = cfg.start;
-  o->pad = 0;
+  o->pad = cfg.pad;  // every loaded word stays live: a dead load destination is waited on early
   o->rho = rho;
   o->rmax = rmax;
   o->tol = cfg.tol;
@@ -428,6 +429,213 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Y(MixFix a, int odd, const MixSt
   if (lead) put_mstate(out, 1, s.iters + 1, PH_NORMAL, s.nrel, 0, 0, s.rho, s.rmax, s);
 }
 
+// ---- iteration kernels, loads-first form ------------------------------------------
+// The same steps as k_mix_X / k_mix_Y, but every load of a kernel (state,
+// partials, then all operands of the normal step, whatever the phase) is
+// issued before its first wait, and the partial reduction runs while the
+// operands are in flight.  This holds every operand in registers at once
+// (lower occupancy), so which form is faster depends on the storage format
+// and batch size: the tuner times both.
+template <typename S, int SPT, bool XCD>
+__global__ __launch_bounds__(MIX_BS) void k_mix_Xe(MixFix a, int odd, const MixState* __restrict__ sin,
+                                                  MixState* __restrict__ sout, const double* __restrict__ pin,
+                                                  double* __restrict__ pout) {
+  __shared__ double lds[MIX_BS / 64], lds2[MIX_BS / 64];  // head and tail reductions
+  using T = typename S::T;
+  const Geom g = a.g;
+  const int rhs = blockIdx.y;
+  const size_t off = (size_t)rhs * g.V;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const T* r = (const T*)a.r + off;
+  const T* pold = (const T*)(odd ? a.p1 : a.p0) + off;
+  T* pnew = (T*)(odd ? a.p0 : a.p1) + off;
+  // loads in the order they are needed (state and partials, then the
+  // normal step's operands), all issued before the first wait; the
+  // reduction runs while the operands are in flight
+  const MixState s = load_state(sin + rhs);
+  double pl[PART_K];
+  partial_issue(pin + (size_t)rhs * a.nb, a.nb, pl);
+  end_of_loads();
+  int fs[SPT];
+  float2 rv[SPT][7], pv[SPT][7];
+  float smv[SPT];
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    fs[i] = mix_site<SPT, XCD>(a, i);
+    const int fc = fs[i] < 0 ? 0 : fs[i];  // past the last site: load site 0, store nothing
+    NbrF n = neighbours_f(g, decode(g, fc));
+    int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
+    smv[i] = *elem(a.sm32, fc);
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+      rv[i][k] = S::ld(r, q[k]);
+      pv[i][k] = S::ld(pold, q[k]);
+    }
+  }
+  end_of_loads();
+  const double rho = block_sum_once(partial_finish(pin + (size_t)rhs * a.nb, a.nb, pl), lds);
+  const double nr = sqrt(rho);
+  MixState* out = sout + rhs;
+  if (!s.active) {
+    if (lead) put_mstate(out, 0, s.iters, s.phase, s.nrel, s.converged, s.early, s.rho, s.rmax, s);
+    return;
+  }
+  double beta = 0.0;
+  double rmax = s.rmax;
+  int nrel = s.nrel;
+  bool update = false;
+  if (s.phase == PH_UPDATED) {
+    // rho is the true FP64 residual norm^2 after a reliable update
+    nrel = s.nrel + 1;
+    if (nr < s.tol || s.iters >= s.max_iter) {
+      const int conv = nr < s.tol ? 1 : 0;
+      if (lead) {
+        put_mstate(out, 0, s.iters, PH_NORMAL, nrel, conv, 0, rho, nr, s);
+        CGDevResult res = {conv, 0, s.iters, nrel, nr, {0, 0, 0}};
+        a.res[rhs] = res;
+      }
+      return;
+    }
+    rmax = nr;
+    beta = rho / s.rho;
+  } else if (s.iters == 0) {
+    if (s.start == GNX_START_IMPOSED && nr < s.tol) {  // conjugategradient.c:123-128
+      if (lead) {
+        put_mstate(out, 0, 0, PH_NORMAL, 0, 1, 1, rho, nr, s);
+        CGDevResult res = {1, 1, 0, 0, nr, {0, 0, 0}};
+        a.res[rhs] = res;
+      }
+      return;
+    }
+    rmax = nr;
+  } else {
+    // fold the correction and test the true residual when the iterated one
+    // has dropped by delta, claims convergence, or the iteration cap is hit
+    update = (nr < s.delta * s.rmax) || (nr < s.tol) || (s.iters >= s.max_iter);
+    beta = rho / s.rho;
+  }
+  if (update) {
+    // reliable update, part 1: u = M^dag (x + xs) in FP64 (x and xs are not
+    // written in this kernel, so neighbours read consistent values); the
+    // direction is carried into the other parity buffer unchanged
+    const double2* x = a.x + off;
+    const float2* xs = a.xs + off;
+    auto ldx = [&](int q) {
+      float2 c = xs[q];
+      return cadd(x[q], make_double2((double)c.x, (double)c.y));
+    };
+#pragma unroll
+    for (int i = 0; i < SPT; i++) {
+      if (fs[i] < 0) continue;
+      a.u[off + fs[i]] = apply_Mdag(g, decode(g, fs[i]), a.sm[fs[i]], ldx);
+      S::st(pnew, fs[i], pv[i][0]);
+    }
+    if (lead) put_mstate(out, 1, s.iters, PH_UPDATE, nrel, 0, 0, s.rho, rmax, s);
+    return;
+  }
+  // normal step: p = r + beta p, t = M^dag p
+  const float fb = (float)beta;
+  double acc = 0.0;
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    if (fs[i] < 0) continue;
+    const Site st = decode(g, fs[i]);
+    float2 p0v = cadd(rv[i][0], rmul(fb, pv[i][0]));
+    float2 ppv[3], pmv[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      ppv[d] = cadd(rv[i][1 + d], rmul(fb, pv[i][1 + d]));
+      pmv[d] = cadd(rv[i][4 + d], rmul(fb, pv[i][4 + d]));
+    }
+    S::st(pnew, fs[i], p0v);
+    float2 tt = apply_Mdag_fast_vals(g, st, smv[i], p0v, ppv, pmv);
+    S::st((T*)a.t + off, fs[i], tt);
+    acc += dnorm2(tt);
+  }
+  acc = block_sum_once(acc, lds2);
+  if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + blockIdx.x] = acc;
+  if (lead) put_mstate(out, 1, s.iters, PH_NORMAL, nrel, 0, 0, rho, rmax, s);
+}
+
+template <typename S, int SPT, bool XCD>
+__global__ __launch_bounds__(MIX_BS) void k_mix_Ye(MixFix a, int odd, const MixState* __restrict__ sin,
+                                                  MixState* __restrict__ sout, const double* __restrict__ pin,
+                                                  double* __restrict__ pout) {
+  __shared__ double lds[MIX_BS / 64], lds2[MIX_BS / 64];  // head and tail reductions
+  using T = typename S::T;
+  const Geom g = a.g;
+  const int rhs = blockIdx.y;
+  const size_t off = (size_t)rhs * g.V;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const T* pnew = (const T*)(odd ? a.p0 : a.p1) + off;
+  const T* t = (const T*)a.t + off;
+  T* r = (T*)a.r + off;
+  // loads as in k_mix_X: state and partials, then the normal step's
+  // operands, issued whatever the phase (an update or a finished system
+  // ignores them), so that none waits for the state
+  const MixState s = load_state(sin + rhs);
+  double pl[PART_K];
+  partial_issue(pin + (size_t)rhs * a.nb, a.nb, pl);
+  end_of_loads();
+  int fs[SPT];
+  float smv[SPT];
+  float2 tv[SPT][7], pv[SPT], xv[SPT], rv[SPT];
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    fs[i] = mix_site<SPT, XCD>(a, i);
+    const int fc = fs[i] < 0 ? 0 : fs[i];
+    NbrF n = neighbours_f(g, decode(g, fc));
+    int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
+    smv[i] = *elem(a.sm32, fc);
+#pragma unroll
+    for (int k = 0; k < 7; k++) tv[i][k] = S::ld(t, q[k]);
+    pv[i] = S::ld(pnew, fc);
+    xv[i] = *elem(a.xs + off, fc);
+    rv[i] = S::ld(r, fc);
+  }
+  end_of_loads();
+  const double gamma = block_sum_once(partial_finish(pin + (size_t)rhs * a.nb, a.nb, pl), lds);
+  MixState* out = sout + rhs;
+  if (!s.active) {
+    if (lead) put_mstate(out, 0, s.iters, s.phase, s.nrel, s.converged, s.early, s.rho, s.rmax, s);
+    return;
+  }
+  double acc = 0.0;
+  if (s.phase == PH_UPDATE) {
+    // reliable update, part 2: x += xs, xs = 0, r = (S)(b - M u)
+    const double2* u = a.u + off;
+#pragma unroll
+    for (int i = 0; i < SPT; i++) {
+      const int f = fs[i];
+      if (f < 0) break;
+      float2 c = a.xs[off + f];
+      a.x[off + f] = cadd(a.x[off + f], make_double2((double)c.x, (double)c.y));
+      a.xs[off + f] = make_float2(0.f, 0.f);
+      double2 rr = apply_sub_M(g, decode(g, f), a.sm[f], a.b[off + f], [&](int q) { return u[q]; });
+      S::st(r, f, make_float2((float)rr.x, (float)rr.y));
+      acc += cnorm2(rr);
+    }
+    acc = block_sum_once(acc, lds2);
+    if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + blockIdx.x] = acc;
+    if (lead) put_mstate(out, 1, s.iters, PH_UPDATED, s.nrel, 0, 0, s.rho, s.rmax, s);
+    return;
+  }
+  // normal step: xs += alpha p, r -= alpha M t
+  const float al = (float)(s.rho / gamma);
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    if (fs[i] < 0) continue;
+    const float2 w = apply_M_vals(g, decode(g, fs[i]), smv[i], tv[i][0], &tv[i][1], &tv[i][4]);
+    st_nt(elem(a.xs + off, fs[i]), make_float2(xv[i].x + al * pv[i].x, xv[i].y + al * pv[i].y));
+    float2 rn = make_float2(rv[i].x - al * w.x, rv[i].y - al * w.y);
+    S::st(r, fs[i], rn);
+    acc += dnorm2(rn);
+  }
+  acc = block_sum_once(acc, lds2);
+  if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + blockIdx.x] = acc;
+  if (lead) put_mstate(out, 1, s.iters + 1, PH_NORMAL, s.nrel, 0, 0, s.rho, s.rmax, s);
+}
+
 // ---- paired iteration kernels -------------------------------------------------------
 // Two systems per block (blockIdx.y = pair r0 = 2 y, r0 + 1): the site decode,
 // neighbours, phases and addresses are shared by the pair; each system keeps
@@ -759,7 +967,8 @@ static hipError_t mix_ensure(MixMachine* m, const Geom& g, int nrhs) {
 struct MixShape {
   int spt;
   bool xcd;
-  bool pair;  // two systems per block (k_mix_X2 / k_mix_Y2)
+  bool pair;   // two systems per block (k_mix_X2 / k_mix_Y2)
+  bool early;  // loads-first kernels (k_mix_Xe / k_mix_Ye)
   const void* X;
   const void* Y;
   const void* start_imposed;
@@ -769,25 +978,33 @@ struct MixShape {
 };
 
 #define GNX_MIX_SHAPE(S, N, X)                                                                     \
-  {N, X, false, (const void*)k_mix_X<S, N, X>, (const void*)k_mix_Y<S, N, X>,                     \
+  {N, X, false, false, (const void*)k_mix_X<S, N, X>, (const void*)k_mix_Y<S, N, X>,              \
    (const void*)k_mix_start_imposed<S, N, X>, (const void*)k_mix_null_p1<N, X>,                   \
    (const void*)k_mix_null_p2<N, X>, (const void*)k_mix_null_p3<S, N, X>}
 #define GNX_MIX_PAIR(S, N, X)                                                                      \
-  {N, X, true, (const void*)k_mix_X2<S, N, X>, (const void*)k_mix_Y2<S, N, X>,                    \
+  {N, X, true, false, (const void*)k_mix_X2<S, N, X>, (const void*)k_mix_Y2<S, N, X>,             \
    (const void*)k_mix_start_imposed<S, N, X>, (const void*)k_mix_null_p1<N, X>,                   \
    (const void*)k_mix_null_p2<N, X>, (const void*)k_mix_null_p3<S, N, X>}
-// shapes 0..5: one system per block; 6..9: two (batches of two or more)
+#define GNX_MIX_EARLY(S, N, X)                                                                     \
+  {N, X, false, true, (const void*)k_mix_Xe<S, N, X>, (const void*)k_mix_Ye<S, N, X>,             \
+   (const void*)k_mix_start_imposed<S, N, X>, (const void*)k_mix_null_p1<N, X>,                   \
+   (const void*)k_mix_null_p2<N, X>, (const void*)k_mix_null_p3<S, N, X>}
+// shapes 0..5: one system per block; 6..9: two (batches of two or more);
+// 10..15: one system per block, loads-first kernels
 template <typename S>
 static const MixShape* mix_shapes() {
   static const MixShape t[] = {GNX_MIX_SHAPE(S, 1, false), GNX_MIX_SHAPE(S, 1, true), GNX_MIX_SHAPE(S, 2, false),
                                GNX_MIX_SHAPE(S, 2, true),  GNX_MIX_SHAPE(S, 4, false), GNX_MIX_SHAPE(S, 4, true),
                                GNX_MIX_PAIR(S, 1, false),  GNX_MIX_PAIR(S, 1, true),   GNX_MIX_PAIR(S, 2, false),
-                               GNX_MIX_PAIR(S, 2, true)};
+                               GNX_MIX_PAIR(S, 2, true),   GNX_MIX_EARLY(S, 1, false), GNX_MIX_EARLY(S, 1, true),
+                               GNX_MIX_EARLY(S, 2, false), GNX_MIX_EARLY(S, 2, true),  GNX_MIX_EARLY(S, 4, false),
+                               GNX_MIX_EARLY(S, 4, true)};
   return t;
 }
 #undef GNX_MIX_SHAPE
 #undef GNX_MIX_PAIR
-constexpr int kMixShapes = 10;
+#undef GNX_MIX_EARLY
+constexpr int kMixShapes = 16;
 
 static MixFix mix_fix(MixMachine* m, const Geom& g, int nrhs, int spt) {
   MixFix a;
@@ -859,7 +1076,7 @@ static hipError_t mix_begin(MixMachine* m, const Geom& g, const double* sm, cons
 }
 
 // Autotuner: two chunks of every shape on the caller's own system, fastest
-// kept per (batch size, storage format); GNX_MIX_SHAPE=<0..9> pins one.
+// kept per (batch size, storage format); GNX_MIX_SHAPE=<0..15> pins one.
 template <typename S>
 static hipError_t mix_tune(MixMachine* m, const Geom& g, const double* sm, const double2* b, double delta,
                            int nrhs, int tag, hipStream_t st, int* best) {
@@ -942,7 +1159,7 @@ hipError_t cgmix_run(void** machine_slot, const Geom& g, const double* sm, const
   return mix_run_t<StoreBF16>(m, g, sm, b, x, tol, delta, max_iter, start, nrhs, st, host_res, 101);
 }
 
-int cgmix_shape_of(void* machine, int nrhs, int prec, int* spt, int* xcd, int* pair) {
+int cgmix_shape_of(void* machine, int nrhs, int prec, int* spt, int* xcd, int* pair, int* early) {
   MixMachine* m = (MixMachine*)machine;
   if (!m) return -1;
   const int key = nrhs + 1000 * (prec == GNX_PREC_F32 ? 100 : 101);
@@ -952,6 +1169,7 @@ int cgmix_shape_of(void* machine, int nrhs, int prec, int* spt, int* xcd, int* p
       *spt = sh.spt;
       *xcd = sh.xcd ? 1 : 0;
       *pair = sh.pair ? 1 : 0;
+      *early = sh.early ? 1 : 0;
       return m->eng.tuned_tag[i];
     }
   return -1;

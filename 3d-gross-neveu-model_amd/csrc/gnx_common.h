@@ -167,6 +167,18 @@ GNX_DEVONLY double block_sum(double v, double* lds) {
   return t;
 }
 
+// block_sum for an lds array that the kernel uses only once: no leading
+// barrier (no earlier reader of lds can be overtaken).
+GNX_DEVONLY double block_sum_once(double v, double* lds) {
+  const int wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) lds[wid] = v;
+  __syncthreads();
+  double t = lds[0];
+  for (int w = 1; w < nw; w++) t += lds[w];
+  return t;
+}
+
 // Two deterministic block sums at once (each bit-identical to block_sum of
 // its input); lds must hold 2*blockDim.x/64 doubles.  Contains __syncthreads().
 GNX_DEVONLY double2 block_sum2(double a, double b, double* lds) {

@@ -17,7 +17,7 @@
 namespace gnx {
 
 constexpr int ENG_BS = 256;     // threads per block of every solver kernel
-constexpr int ENG_CHUNK = 16;   // iterations per graph replay
+constexpr int ENG_CHUNK = 32;   // iterations per graph replay
 
 // launches ENG_CHUNK iterations (every X and Y kernel) into the capture stream
 using ChunkBody = std::function<void(hipStream_t)>;

@@ -49,8 +49,8 @@ hipError_t cgmix_run(void** machine_slot, const Geom& g, const double* sm, const
                      CGDevResult* host_res);
 void cgmix_release(void* machine);
 // tuned launch shape (sites per thread, XCD slab mapping, two systems per
-// block); -1 if none yet
-int cgmix_shape_of(void* machine, int nrhs, int prec, int* spt, int* xcd, int* pair);
+// block, loads-first kernels); the shape index, -1 if none yet
+int cgmix_shape_of(void* machine, int nrhs, int prec, int* spt, int* xcd, int* pair, int* early);
 int cgm_shape_of(void* machine, int nrhs, int* spt, int* xcd);
 
 void set_err(const char* fmt, ...);

@@ -29,6 +29,21 @@ struct SiteMap {
   }
 };
 
+// ---- addressing ------------------------------------------------------------------
+// Address of element i of a vector whose base is wave-uniform, as the base
+// plus a 32-bit byte offset: the compiler then uses the scalar-base +
+// vector-offset address form, one instruction per address instead of a
+// 64-bit shift-and-add.  Volumes are capped at 2^26 sites, so offsets of
+// vectors of up to 64 bytes per site stay below 2^32.
+template <typename T>
+GNX_DEVONLY T* elem(T* base, int i) {
+  return (T*)((char*)base + (unsigned)i * (unsigned)sizeof(T));
+}
+template <typename T>
+GNX_DEVONLY const T* elem(const T* base, int i) {
+  return (const T*)((const char*)base + (unsigned)i * (unsigned)sizeof(T));
+}
+
 // ---- streaming stores ----------------------------------------------------------
 // Vector outputs of a solver kernel are not re-read by the kernel that writes
 // them, and the next kernel (on any XCD) reads them from memory anyway, so

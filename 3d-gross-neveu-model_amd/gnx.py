@@ -72,7 +72,7 @@ def lib():
                                            C.POINTER(C.c_double), C.POINTER(CGResult)]
         L.gnx_cg_shape.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.gnx_cg_mixed_shape.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
-                                         C.POINTER(C.c_int)]
+                                         C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.gnx_sigma_force.argtypes = [vp, vp, vp, C.c_double, C.c_int, C.c_double, C.c_long,
                                       vp * 2, vp * 2, vp * 2, vp, CGResult * 2]
         L.gnx_sigma_force_from_solutions.argtypes = [vp, vp, C.c_double, C.c_int, vp, vp, vp, vp, vp]
@@ -263,10 +263,11 @@ class Context:
         return None if st != 0 else {"sites_per_thread": spt.value, "xcd_slab": bool(xcd.value)}
 
     def cg_mixed_shape(self, nrhs, prec):
-        spt, xcd, pair = C.c_int(), C.c_int(), C.c_int()
-        st = lib().gnx_cg_mixed_shape(self.h, nrhs, prec, C.byref(spt), C.byref(xcd), C.byref(pair))
+        spt, xcd, pair, early = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        st = lib().gnx_cg_mixed_shape(self.h, nrhs, prec, C.byref(spt), C.byref(xcd), C.byref(pair),
+                                      C.byref(early))
         return None if st != 0 else {"sites_per_thread": spt.value, "xcd_slab": bool(xcd.value),
-                                     "pair": bool(pair.value)}
+                                     "pair": bool(pair.value), "loads_first": bool(early.value)}
 
     # -- A9/A10 -----------------------------------------------------------
     def sigma_force(self, sigma, sm, beta_s, tol, phi, X, Y, force, max_iter=0, nflav=4):

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define GNX_ABI_VERSION 1
+#define GNX_ABI_VERSION 2
 
 typedef struct gnx_ctx gnx_ctx;
 
@@ -132,12 +132,14 @@ gnx_status gnx_set_reliable_delta(gnx_ctx *ctx, double delta);
  * pins the shape instead of tuning. */
 gnx_status gnx_cg_shape(gnx_ctx *ctx, int nrhs, int *sites_per_thread, int *xcd_mapped);
 /* The same for the mixed-precision solver (prec GNX_PREC_F32 or GNX_PREC_BF16);
- * pair = 1: two systems share each block's site arithmetic.  GNX_MIX_SHAPE=<n>
- * pins one: n = 0..5 one system per block with 1, 1, 2, 2, 4, 4 sites per
- * thread, n = 6..9 two systems per block with 1, 1, 2, 2; odd n use the XCD
+ * pair = 1: two systems share each block's site arithmetic; loads_first = 1:
+ * kernels that issue every load before their first wait (more registers,
+ * less latency).  GNX_MIX_SHAPE=<n> pins one: n = 0..5 one system per block
+ * with 1, 1, 2, 2, 4, 4 sites per thread, n = 6..9 two systems per block with
+ * 1, 1, 2, 2, n = 10..15 as 0..5 with loads-first kernels; odd n use the XCD
  * slab mapping. */
 gnx_status gnx_cg_mixed_shape(gnx_ctx *ctx, int nrhs, gnx_prec prec, int *sites_per_thread, int *xcd_mapped,
-                              int *pair);
+                              int *pair, int *loads_first);
 
 /* A9. Sigma force (moleculardynamics.c:590-647):
  *   X_f = CG(phi_f) with START_X0_NULL from the incoming X_f, Y_f = M^dag X_f,

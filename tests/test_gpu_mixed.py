@@ -43,13 +43,15 @@ def test_mixed_vs_oracle(P, gnx_mod, prec, L):
     banner(f"  iterations fp64={st.last_iters} {prec}={res.iterations} updates={res.reliable_updates}")
 
 
-@pytest.mark.parametrize("shape", list(range(10)))
+@pytest.mark.parametrize("shape", list(range(16)))
 @pytest.mark.parametrize("prec", ["f32", "bf16"])
 def test_mixed_batch_and_null_start(P, gnx_mod, prec, shape, monkeypatch):
     """Every launch shape: 0..5 one system per block (1/2/4 sites per thread,
-    with and without the XCD slab mapping), 6..9 two systems per block.  A
-    batch of 3 (the last pair half empty) is bit-identical to 3 single solves
-    of the same site mapping, and each solution meets the FP64 bar."""
+    with and without the XCD slab mapping), 6..9 two systems per block,
+    10..15 as 0..5 with the loads-first kernels.  A batch of 3 (the last pair
+    half empty) is bit-identical to 3 single solves of the same site mapping
+    (and the two kernel forms to each other), and each solution meets the
+    FP64 bar."""
     banner(f"mixed {prec} shape {shape}: batch of 3 and null-start early exit")
     import torch
     L = 12
@@ -65,10 +67,12 @@ def test_mixed_batch_and_null_start(P, gnx_mod, prec, shape, monkeypatch):
     X = ctx.vec(3)
     st, res = ctx.cg_batch(sm, dev(np.concatenate(bs)), X, 3, 1e-10, start=gnx_mod.START_IMPOSED, prec=p)
     assert st == 0
-    pair = shape >= 6
-    assert ctx.cg_mixed_shape(3, p) == {"sites_per_thread": [1, 1, 2, 2, 4, 4, 1, 1, 2, 2][shape],
-                                        "xcd_slab": shape % 2 == 1, "pair": pair}
-    monkeypatch.setenv("GNX_MIX_SHAPE", str(shape - 6 if pair else shape))
+    pair = 6 <= shape < 10
+    early = shape >= 10
+    assert ctx.cg_mixed_shape(3, p) == {"sites_per_thread": [1, 1, 2, 2, 4, 4, 1, 1, 2, 2][shape % 10],
+                                        "xcd_slab": shape % 2 == 1, "pair": pair, "loads_first": early}
+    # single solves with the other kernel form of the same site mapping
+    monkeypatch.setenv("GNX_MIX_SHAPE", str(shape - 6 if pair else (shape - 10 if early else shape + 10)))
     for k in range(3):
         x = ctx.vec()
         s1, r1 = ctx.cg(sm, dev(bs[k]), x, 1e-10, start=gnx_mod.START_IMPOSED, prec=p)
