@@ -123,7 +123,7 @@ gnx_status gnx_condensate_batch(gnx_ctx *ctx, const double *sm, const double *no
 
 /* Mixed precision: residual reduction factor since the last FP64 reliable
  * update that triggers the next one.  0 (the default) selects it per
- * precision: 0.05 for FP32, 0.25 for BF16. */
+ * precision: 0.05 for FP32, 0.2 for BF16. */
 gnx_status gnx_set_reliable_delta(gnx_ctx *ctx, double delta);
 
 /* Launch shape the CG autotuner picked for batches of nrhs systems on this
