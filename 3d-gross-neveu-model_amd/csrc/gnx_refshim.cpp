@@ -157,6 +157,9 @@ bool force_terms(const Lattice* lat, const Info* info, Forcesigma* force, Fermio
 
 }  // namespace
 
+// lattice volume bound by GnxRefInit, for the checkpoint code (gnx_refio.cpp)
+long gnx_ref_bound_volume() { return g_shim.V; }
+
 extern "C" {
 
 int GnxRefInit(int lspace, int ltime, int device, double (*gauss1)(void), double (*casuale)(void)) {

@@ -139,6 +139,26 @@ struct Force { double *force; };
 struct Momentasigma { double *momenta; };
 struct Forcesigma { double *force; };
 struct RandomVec { struct Fermion *vec; };
+struct Momentapi;   /* pi.h: not used by the drop-in (PI_INTERACTION off) */
+struct Forcepi;
+
+/* simulation.h:14-31 */
+struct Simulation {
+    struct Lattice *lattice;
+    struct Info info;
+    struct Fermioncoor *fcor;
+    struct Fermionkit *kit;
+    struct Force *force;
+    struct Momenta *momenta;
+    struct Momentasigma *momentasigma;
+    struct Forcesigma *forcesigma;
+    struct Momentapi *momentapi;
+    struct Forcepi *forcepi;
+    char observables_file[200];
+    char acceptance_file[200];
+    char terma_file[200];
+    char terma_file_copy[200];
+};
 
 /* ---- set-up (no counterpart in the reference: it fixes L at compile time) -- */
 /* Bind the lattice size (macro.h:21-24 LSPACE/LTIME), the HIP device and the
@@ -213,6 +233,18 @@ void ComputeCondensateDirac(const struct Lattice *lat, double *condensate_tr1, d
                             struct RandomVec *vec, struct Fermion *target_inverted, const struct Info *info,
                             const struct Fermioncoor *fermioncoor, double inversion_precision,
                             struct Fermionkit *kit, int f);
+
+/* simulation.h:39-41 (simulation.c:104-401): the terma checkpoint, byte for
+ * byte the reference's file (SIGMA_INTERACTION on, PI_INTERACTION off, as
+ * shipped).  WriteBinaryLattice renames an existing terma_file to
+ * terma_file_copy first; ReadBinaryLattice falls back to the copy on a
+ * missing or damaged file and, like the reference, prints an error and exits
+ * with status 1 when the file it must read (try_copy false) is damaged, or
+ * when the file cannot be opened for writing.  ReadBinaryLattice recomputes
+ * the sigma mass with ComputeFullSigmaMass above. */
+double TestWriteReadLattice(const struct Simulation *simulation);
+void WriteBinaryLattice(const struct Simulation *simulation);
+void ReadBinaryLattice(struct Simulation *simulation, const char *terma_file, bool try_copy);
 
 #ifdef __cplusplus
 }

@@ -8,7 +8,7 @@
  * hot-path functions called directly.
  *
  * Output: a flat binary file of named records
- *   char name[32] | char kind ('d' or 'i') | pad[7] | int64 count | data
+ *   char name[32] | char kind ('d' double, 'i' int, 'b' byte) | pad[7] | int64 count | data
  * read by tests/golden/make_golden.py.
  *
  * TEST INFRASTRUCTURE ONLY.
@@ -32,7 +32,46 @@ static void rec(const char *name, char kind, const void *data, int64_t count)
     hdr[32] = kind;
     fwrite(hdr, 1, 40, g_out);
     fwrite(&count, sizeof(int64_t), 1, g_out);
-    fwrite(data, kind == 'd' ? sizeof(double) : sizeof(int), (size_t)count, g_out);
+    fwrite(data, kind == 'd' ? sizeof(double) : kind == 'i' ? sizeof(int) : 1, (size_t)count, g_out);
+}
+
+/* a whole file as a byte record ('b') */
+static void rec_file(const char *name, const char *path)
+{
+    static unsigned char buf[1 << 20];
+    FILE *f = fopen(path, "rb");
+    size_t n = f ? fread(buf, 1, sizeof(buf), f) : 0;
+    if (f) fclose(f);
+    rec(name, 'b', buf, (int64_t)n);
+}
+
+/* the checkpoint fields ReadBinaryLattice restores, as doubles */
+static void rec_terma_state(const char *prefix, const struct Simulation *s)
+{
+    char name[64];
+    const struct Info *in = &(s->info);
+    double st[15] = {in->epsilon_link_HMC, in->story_link, in->hybrid_accepted_link, in->hybrid_done_link,
+                     in->hybrid_done_acceptance_link, in->story_sigma, in->hybrid_accepted_sigma,
+                     in->hybrid_done_sigma, in->hybrid_done_acceptance_sigma, in->epsilon_pi_HMC, in->story_pi,
+                     in->hybrid_accepted_pi, in->hybrid_done_pi, in->hybrid_done_acceptance_pi, in->measure};
+    snprintf(name, sizeof(name), "%s_info", prefix); rec(name, 'd', st, 15);
+    snprintf(name, sizeof(name), "%s_U", prefix); rec(name, 'd', s->lattice->dlink, 2 * DIM * VOLUME);
+    snprintf(name, sizeof(name), "%s_sigma", prefix); rec(name, 'd', s->lattice->sigma, VOLUME);
+    snprintf(name, sizeof(name), "%s_sigmamass", prefix); rec(name, 'd', s->lattice->sigmamass, VOLUME);
+}
+
+/* a Simulation with fresh, zeroed lattice fields sharing fcor (for reads) */
+static void fresh_sim(struct Simulation *s, const struct Simulation *from)
+{
+    memset(s, 0, sizeof(*s));
+    s->lattice = malloc(sizeof(Lattice));
+    AllocateLattice(s->lattice);
+    memset(s->lattice->dlink, 0, sizeof(struct Link) * DIM * VOLUME);
+    memset(s->lattice->sigma, 0, sizeof(double) * VOLUME);
+    memset(s->lattice->sigmamass, 0, sizeof(double) * VOLUME);
+    s->fcor = from->fcor;
+    strcpy(s->terma_file, from->terma_file);
+    strcpy(s->terma_file_copy, from->terma_file_copy);
 }
 
 static void rec_f(const char *name, const struct Fermion *f)
@@ -328,6 +367,56 @@ int main(int argc, char **argv)
     rec("link_traj_acc", 'd', lacc, 3);
     rec("link_traj_U", 'd', lat->dlink, 2 * DIM * VOLUME);
     rec("link_traj_stat", 'd', lst, 2);
+
+    /* ---------------- terma checkpoint (simulation.c:110-401) ---------------- */
+    /* Links of seed 555 and sigma of seed 4343 (as drawn above), distinct
+     * counters.  Info is zeroed first: the story fields are ints written and
+     * read as 8-byte doubles (simulation.c:169,173,178), so each slot also
+     * carries the next field or the struct's tail padding. */
+    Initrand(555);
+    for (int i = 0; i < DIM * VOLUME; i++) lat->dlink[i].link = cexp(I_UNIT * 2.0 * PI * Casuale());
+    Initrand(4343);
+    draw_sigma(lat, fcor, 0.85);
+    memset(&(sim.info), 0, sizeof(sim.info));
+    sim.info.epsilon_link_HMC = 0.02; sim.info.story_link = 3;
+    sim.info.hybrid_accepted_link = 5; sim.info.hybrid_done_link = 7;
+    sim.info.story_sigma = 11; sim.info.hybrid_accepted_sigma = 13; sim.info.hybrid_done_sigma = 17;
+    sim.info.epsilon_pi_HMC = 0.05; sim.info.story_pi = 19;
+    sim.info.hybrid_accepted_pi = 23; sim.info.hybrid_done_pi = 29;
+    sim.info.measure = 31;
+    snprintf(sim.terma_file, sizeof(sim.terma_file), "%s.terma", accfile);
+    snprintf(sim.terma_file_copy, sizeof(sim.terma_file_copy), "%s.termacopy", accfile);
+    remove(sim.terma_file);
+    remove(sim.terma_file_copy);
+    rec_scalar("terma_test", TestWriteReadLattice(&sim));
+    WriteBinaryLattice(&sim);                       /* no earlier file: main only */
+    rec_file("terma_first", sim.terma_file);
+    sim.info.measure = 32;
+    sim.info.hybrid_done_sigma = 18;
+    WriteBinaryLattice(&sim);                       /* the first file becomes the copy */
+    rec_file("terma_main", sim.terma_file);
+    rec_file("terma_copy", sim.terma_file_copy);
+    struct Simulation rd;
+    fresh_sim(&rd, &sim);
+    ReadBinaryLattice(&rd, rd.terma_file, true);    /* intact main file */
+    rec_terma_state("terma_read", &rd);
+    {   /* main file cut short: the reader falls back to the copy */
+        static unsigned char buf[1 << 20];
+        FILE *f = fopen(sim.terma_file, "rb");
+        size_t n = fread(buf, 1, sizeof(buf), f);
+        fclose(f);
+        f = fopen(sim.terma_file, "wb");
+        fwrite(buf, 1, n / 2, f);
+        fclose(f);
+    }
+    fresh_sim(&rd, &sim);
+    ReadBinaryLattice(&rd, rd.terma_file, true);
+    rec_terma_state("terma_fallback", &rd);
+    remove(sim.terma_file);                         /* main file missing: the copy is read */
+    fresh_sim(&rd, &sim);
+    ReadBinaryLattice(&rd, rd.terma_file, true);
+    rec_terma_state("terma_missing", &rd);
+    remove(sim.terma_file_copy);
 
     fclose(g_out);
     return 0;

@@ -31,7 +31,7 @@ def read_records(path):
             name = hdr[:32].split(b"\0", 1)[0].decode()
             kind = chr(hdr[32])
             (count,) = struct.unpack("<q", f.read(8))
-            dt = np.float64 if kind == "d" else np.int32
+            dt = {"d": np.float64, "i": np.int32, "b": np.uint8}[kind]
             data = np.frombuffer(f.read(count * np.dtype(dt).itemsize), dtype=dt).copy()
             recs[name] = data
     return recs

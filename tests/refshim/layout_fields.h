@@ -13,6 +13,8 @@
     X(Info, beta_scalar) X(Info, measure) X(Info, epsilon_sigma_HMC) \
     X(Info, steps_sigma_HMC) X(Info, hybrid_accepted_sigma)   \
     X(Info, hybrid_done_acceptance_sigma) X(Info, local_acceptance_sigma) \
-    X(Info, delta_H_sigma) X(Info, story_sigma) X(Info, story_pi)
+    X(Info, delta_H_sigma) X(Info, story_sigma) X(Info, story_pi) \
+    X(Simulation, info) X(Simulation, fcor) X(Simulation, observables_file) \
+    X(Simulation, terma_file) X(Simulation, terma_file_copy)
 #define GNX_LAYOUT_TYPES(X) X(Lattice) X(Fermioncoor) X(Fermion) X(Fermionkit) X(Info) \
-    X(Momentasigma) X(Forcesigma) X(RandomVec) X(Link)
+    X(Momentasigma) X(Forcesigma) X(RandomVec) X(Link) X(Simulation)

@@ -35,7 +35,44 @@ static void rec(const char *name, char kind, const void *data, int64_t count)
     hdr[32] = kind;
     fwrite(hdr, 1, 40, g_out);
     fwrite(&count, sizeof(int64_t), 1, g_out);
-    fwrite(data, kind == 'd' ? sizeof(double) : sizeof(int), (size_t)count, g_out);
+    fwrite(data, kind == 'd' ? sizeof(double) : kind == 'i' ? sizeof(int) : 1, (size_t)count, g_out);
+}
+
+/* a whole file as a byte record ('b') */
+static void rec_file(const char *name, const char *path)
+{
+    static unsigned char buf[1 << 20];
+    FILE *f = fopen(path, "rb");
+    size_t n = f ? fread(buf, 1, sizeof(buf), f) : 0;
+    if (f) fclose(f);
+    rec(name, 'b', buf, (int64_t)n);
+}
+
+/* the checkpoint fields ReadBinaryLattice restores, as doubles */
+static void rec_terma_state(const char *prefix, const struct Simulation *s)
+{
+    char name[64];
+    const struct Info *in = &(s->info);
+    double st[15] = {in->epsilon_link_HMC, in->story_link, in->hybrid_accepted_link, in->hybrid_done_link,
+                     in->hybrid_done_acceptance_link, in->story_sigma, in->hybrid_accepted_sigma,
+                     in->hybrid_done_sigma, in->hybrid_done_acceptance_sigma, in->epsilon_pi_HMC, in->story_pi,
+                     in->hybrid_accepted_pi, in->hybrid_done_pi, in->hybrid_done_acceptance_pi, in->measure};
+    snprintf(name, sizeof(name), "%s_info", prefix); rec(name, 'd', st, 15);
+    snprintf(name, sizeof(name), "%s_U", prefix); rec(name, 'd', s->lattice->dlink, 6 * V);
+    snprintf(name, sizeof(name), "%s_sigma", prefix); rec(name, 'd', s->lattice->sigma, V);
+    snprintf(name, sizeof(name), "%s_sigmamass", prefix); rec(name, 'd', s->lattice->sigmamass, V);
+}
+
+/* a Simulation with fresh, zeroed lattice fields (for reads) */
+static void fresh_sim(struct Simulation *s, const struct Simulation *from)
+{
+    memset(s, 0, sizeof(*s));
+    s->lattice = (struct Lattice *)calloc(1, sizeof(struct Lattice));
+    s->lattice->dlink = (struct Link *)calloc(3 * V, sizeof(struct Link));
+    s->lattice->sigma = (double *)calloc(V, sizeof(double));
+    s->lattice->sigmamass = (double *)calloc(V, sizeof(double));
+    strcpy(s->terma_file, from->terma_file);
+    strcpy(s->terma_file_copy, from->terma_file_copy);
 }
 
 static void rec_f(const char *name, const struct Fermion *f) { rec(name, 'd', f->fermion, 2 * V); }
@@ -262,6 +299,55 @@ int main(int argc, char **argv)
     rec("link_traj_acc", 'd', lacc, 3);
     rec("link_traj_U", 'd', lat.dlink, 6 * V);
     rec("link_traj_stat", 'd', lst, 2);
+
+    /* ---------------- terma checkpoint through the drop-in (as the harness) ---------------- */
+    orc_rng_init(&R, 555);
+    for (int i = 0; i < 3 * V; i++) lat.dlink[i].link = cexp(I * 2.0 * M_PI * drv_casuale());
+    orc_rng_init(&R, 4343);
+    draw_sigma(&lat);
+    struct Simulation sim;
+    memset(&sim, 0, sizeof(sim));
+    sim.lattice = &lat;
+    sim.info.epsilon_link_HMC = 0.02; sim.info.story_link = 3;
+    sim.info.hybrid_accepted_link = 5; sim.info.hybrid_done_link = 7;
+    sim.info.story_sigma = 11; sim.info.hybrid_accepted_sigma = 13; sim.info.hybrid_done_sigma = 17;
+    sim.info.epsilon_pi_HMC = 0.05; sim.info.story_pi = 19;
+    sim.info.hybrid_accepted_pi = 23; sim.info.hybrid_done_pi = 29;
+    sim.info.measure = 31;
+    snprintf(sim.terma_file, sizeof(sim.terma_file), "%s.terma", accfile);
+    snprintf(sim.terma_file_copy, sizeof(sim.terma_file_copy), "%s.termacopy", accfile);
+    remove(sim.terma_file);
+    remove(sim.terma_file_copy);
+    double tv = TestWriteReadLattice(&sim);
+    rec("terma_test", 'd', &tv, 1);
+    WriteBinaryLattice(&sim);
+    rec_file("terma_first", sim.terma_file);
+    sim.info.measure = 32;
+    sim.info.hybrid_done_sigma = 18;
+    WriteBinaryLattice(&sim);
+    rec_file("terma_main", sim.terma_file);
+    rec_file("terma_copy", sim.terma_file_copy);
+    struct Simulation rd;
+    fresh_sim(&rd, &sim);
+    ReadBinaryLattice(&rd, rd.terma_file, true);
+    rec_terma_state("terma_read", &rd);
+    {
+        static unsigned char buf[1 << 20];
+        FILE *f = fopen(sim.terma_file, "rb");
+        size_t n = fread(buf, 1, sizeof(buf), f);
+        fclose(f);
+        f = fopen(sim.terma_file, "wb");
+        fwrite(buf, 1, n / 2, f);
+        fclose(f);
+    }
+    fresh_sim(&rd, &sim);
+    ReadBinaryLattice(&rd, rd.terma_file, true);
+    rec_terma_state("terma_fallback", &rd);
+    remove(sim.terma_file);
+    fresh_sim(&rd, &sim);
+    ReadBinaryLattice(&rd, rd.terma_file, true);
+    rec_terma_state("terma_missing", &rd);
+    remove(sim.terma_file_copy);
 
     GnxRefFinalize();
     fclose(g_out);

@@ -166,3 +166,48 @@ def test_leapfrog2link_golden(golden, P):
     assert np.array_equal(h.U, golden["link_traj_U"])
     assert h.kit.st.iterations == golden["link_traj_stat"][0]
     assert h.kit.st.cg_call == golden["link_traj_stat"][1]
+
+
+def test_terma_checkpoint_format_golden(golden, P):
+    """The terma checkpoint the reference wrote (simulation.c:129-185), parsed
+    with the byte layout the drop-in writes: the links of seed 555 in lex
+    order, the sigma of seed 4343 in f-order, the check value
+    sum_x sum_d (carg U_d(x) + sm[x]/3) recomputed here bit for bit from the
+    oracle's sigma mass, then the counters, the int story fields in 8-byte
+    slots (each carrying the next field, the last one the zeroed tail padding
+    of struct Info).  The second write made the first file the copy."""
+    import math
+    import struct
+    lat = P.Lattice(8)
+    V = lat.V
+    b = golden["terma_first"].tobytes()
+    assert len(b) == 48 * V + 8 * V + 8 + 68
+    U = np.frombuffer(b, np.complex128, 3 * V, 0)
+    assert np.array_equal(U.view(np.float64), golden["link_U"])
+    sigma = np.frombuffer(b, np.float64, V, 48 * V)
+    assert np.array_equal(sigma, P.Random(4343).sigma(V, P.sigma_variance(0.85)))
+    sm = lat.sigma_mass(sigma)
+    assert np.array_equal(sm, golden["link_sigmamass"])
+    total = 0.0
+    for x in range(V):
+        for d in range(3):
+            u = U[3 * x + d]
+            total += math.atan2(u.imag, u.real)
+            total += sm[x] / 3
+    tail = struct.unpack_from("<ddiiiiiiiidiiiii", b, 56 * V)
+    assert tail[0] == total == golden["terma_test"][0]
+    assert tail[1:] == (0.02, 3, 11, 5, 7, 11, 19, 13, 17, 0.05, 19, 0, 23, 29, 31)
+    m = golden["terma_main"].tobytes()
+    assert m[:56 * V + 8] == b[:56 * V + 8]
+    assert struct.unpack_from("<ddiiiiiiiidiiiii", m, 56 * V)[1:] == \
+        (0.02, 3, 11, 5, 7, 11, 19, 13, 18, 0.05, 19, 0, 23, 29, 32)
+    assert golden["terma_copy"].tobytes() == b
+    # what the reference's reader restored from each file
+    assert np.array_equal(golden["terma_read_sigma"], sigma)
+    assert np.array_equal(golden["terma_read_sigmamass"], sm)
+    np.testing.assert_array_equal(golden["terma_read_info"],
+                                  [0.02, 3, 5, 7, 5 / 7, 11, 13, 18, 13 / 18, 0.05, 19, 23, 29, 23 / 29, 32])
+    for case in ("fallback", "missing"):
+        np.testing.assert_array_equal(golden[f"terma_{case}_info"],
+                                      [0.02, 3, 5, 7, 5 / 7, 11, 13, 17, 13 / 17, 0.05, 19, 23, 29, 23 / 29, 31])
+        assert np.array_equal(golden[f"terma_{case}_U"], golden["link_U"])

@@ -29,7 +29,7 @@ DRIVER = os.path.join(SHIM, "shim_driver")
 def _declared():
     txt = open(HDR).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"^(?:bool|void|int)\s+(\w+)\s*\(", txt, flags=re.M)))
+    return sorted(set(re.findall(r"^(?:bool|void|int|double)\s+(\w+)\s*\(", txt, flags=re.M)))
 
 
 def test_struct_layout_matches_reference(golden, tmp_path):
@@ -59,7 +59,7 @@ def _read_records(path):
             if not hdr:
                 break
             name = hdr[:32].split(b"\0", 1)[0].decode()
-            dt = np.float64 if chr(hdr[32]) == "d" else np.int32
+            dt = {"d": np.float64, "i": np.int32, "b": np.uint8}[chr(hdr[32])]
             (count,) = struct.unpack("<q", f.read(8))
             recs[name] = np.frombuffer(f.read(count * np.dtype(dt).itemsize), dtype=dt).copy()
     return recs
@@ -154,3 +154,24 @@ def test_shim_link_functions(shim, golden):
     # with SIGMA_INTERACTION on only the starting energy is written (moleculardynamics.c:454, 523-526)
     lines = open(str(acc) + ".link").read().strip().splitlines()
     assert len(lines) == 3 and all(l.startswith("# Slink\t") for l in lines)
+
+
+@pytest.mark.gpu
+def test_shim_terma_checkpoint(shim, golden):
+    """WriteBinaryLattice / ReadBinaryLattice / TestWriteReadLattice through
+    the drop-in against the reference's own (simulation.c:104-401), on the
+    same links and sigma: the check value bit for bit; the first file, the
+    second file and the copy it made byte for byte; the fields read back
+    from the intact file, from a file cut short (the reader falls back to the
+    copy) and with the main file missing (the copy is read) exactly equal."""
+    recs, _ = shim
+    assert recs["terma_test"][0] == golden["terma_test"][0]
+    for k in ("terma_first", "terma_main", "terma_copy"):
+        assert len(recs[k]) == len(golden[k]) == 28748, k
+        np.testing.assert_array_equal(recs[k], golden[k], err_msg=k)
+    for case in ("read", "fallback", "missing"):
+        for f in ("info", "U", "sigma", "sigmamass"):
+            k = f"terma_{case}_{f}"
+            np.testing.assert_array_equal(recs[k], golden[k], err_msg=k)
+    # the copy holds the first checkpoint (measure 31), the main file the second
+    assert golden["terma_read_info"][14] == 32 and golden["terma_fallback_info"][14] == 31
