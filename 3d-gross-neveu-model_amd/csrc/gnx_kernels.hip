@@ -4,6 +4,7 @@
 // Every kernel is a memory-bound stencil or sweep: one site per thread,
 // coalesced double2 loads in f-order, no tables read from memory.
 #include "gnx_internal.h"
+#include "gnx_solver.h"  // elem()
 
 namespace gnx {
 
@@ -33,20 +34,20 @@ __global__ void k_evolve_momenta(int V, double* __restrict__ mom, const double* 
 }
 
 // ------------------------------------------------------------------ Dirac
-__global__ void k_dirac(Geom g, int op, const double* __restrict__ sm, const double2* __restrict__ in,
-                        double2* __restrict__ out) {
-  int f = blockIdx.x * blockDim.x + threadIdx.x;
+template <int OP>
+__global__ __launch_bounds__(256) void k_dirac(Geom g, const double* __restrict__ sm, const double2* __restrict__ in,
+                                               double2* __restrict__ out) {
+  const int f = blockIdx.x * 256 + threadIdx.x;
   if (f >= g.V) return;
-  Site s = decode(g, f);
-  auto ld = [&](int j) { return in[j]; };
+  const Site s = decode(g, f);
+  auto ld = [&](int j) { return *elem(in, j); };
+  const double smv = *elem(sm, f);
   double2 y;
-  switch (op) {
-    case GNX_OP_M: y = apply_M(g, s, sm[f], ld); break;
-    case GNX_OP_MDAG: y = apply_Mdag(g, s, sm[f], ld); break;
-    case GNX_OP_SUB_M: y = apply_sub_M(g, s, sm[f], out[f], ld); break;
-    default: y = apply_sub_Mdag(g, s, sm[f], out[f], ld); break;
-  }
-  out[f] = y;
+  if (OP == GNX_OP_M) y = apply_M(g, s, smv, ld);
+  else if (OP == GNX_OP_MDAG) y = apply_Mdag(g, s, smv, ld);
+  else if (OP == GNX_OP_SUB_M) y = apply_sub_M(g, s, smv, *elem(out, f), ld);
+  else y = apply_sub_Mdag(g, s, smv, *elem(out, f), ld);
+  *elem(out, f) = y;
 }
 
 // ------------------------------------------------------------------ sigma force
@@ -188,18 +189,24 @@ hipError_t launch_dirac(const Geom& g, int op, const double* sm, const double* i
   dim3 grid(nblk(g.V, 256)), blk(256);
   switch (op) {
     case GNX_OP_M:
+      hipLaunchKernelGGL(k_dirac<GNX_OP_M>, grid, blk, 0, st, g, sm, vin, vout);
+      break;
     case GNX_OP_MDAG:
+      hipLaunchKernelGGL(k_dirac<GNX_OP_MDAG>, grid, blk, 0, st, g, sm, vin, vout);
+      break;
     case GNX_OP_SUB_M:
+      hipLaunchKernelGGL(k_dirac<GNX_OP_SUB_M>, grid, blk, 0, st, g, sm, vin, vout);
+      break;
     case GNX_OP_SUB_MDAG:
-      hipLaunchKernelGGL(k_dirac, grid, blk, 0, st, g, op, sm, vin, vout);
+      hipLaunchKernelGGL(k_dirac<GNX_OP_SUB_MDAG>, grid, blk, 0, st, g, sm, vin, vout);
       break;
     case GNX_OP_MMDAG:
-      hipLaunchKernelGGL(k_dirac, grid, blk, 0, st, g, (int)GNX_OP_MDAG, sm, vin, (double2*)tmp);
-      hipLaunchKernelGGL(k_dirac, grid, blk, 0, st, g, (int)GNX_OP_M, sm, (const double2*)tmp, vout);
+      hipLaunchKernelGGL(k_dirac<GNX_OP_MDAG>, grid, blk, 0, st, g, sm, vin, (double2*)tmp);
+      hipLaunchKernelGGL(k_dirac<GNX_OP_M>, grid, blk, 0, st, g, sm, (const double2*)tmp, vout);
       break;
     case GNX_OP_MDAGM:
-      hipLaunchKernelGGL(k_dirac, grid, blk, 0, st, g, (int)GNX_OP_M, sm, vin, (double2*)tmp);
-      hipLaunchKernelGGL(k_dirac, grid, blk, 0, st, g, (int)GNX_OP_MDAG, sm, (const double2*)tmp, vout);
+      hipLaunchKernelGGL(k_dirac<GNX_OP_M>, grid, blk, 0, st, g, sm, vin, (double2*)tmp);
+      hipLaunchKernelGGL(k_dirac<GNX_OP_MDAG>, grid, blk, 0, st, g, sm, (const double2*)tmp, vout);
       break;
     default:
       return hipErrorInvalidValue;
