@@ -116,7 +116,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_start_imposed(CGMFix a, const do
     acc += cnorm2(bb);
   }
   acc = block_sum(acc, lds);
-  if (threadIdx.x == 0) slot(a, 1, rhs)[blockIdx.x] = acc;
+  if (threadIdx.x == 0) slot(a, 1, rhs)[partial_slot(blockIdx.x, a.nb)] = acc;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     CGMState s = {1, 0, 0, 0, 0.0, tol, max_iter, GNX_START_IMPOSED, 0};
     a.state[rhs] = s;
@@ -168,8 +168,8 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_null_p2(CGMFix a, const double2*
   ar = block_sum(ar, lds);
   ab = block_sum(ab, lds);
   if (threadIdx.x == 0) {
-    slot(a, 0, rhs)[blockIdx.x] = ar;
-    slot(a, 1, rhs)[blockIdx.x] = ab;
+    slot(a, 0, rhs)[partial_slot(blockIdx.x, a.nb)] = ar;
+    slot(a, 1, rhs)[partial_slot(blockIdx.x, a.nb)] = ab;
   }
 }
 
@@ -301,7 +301,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_A(CGMFix a, int odd, const CGMSt
     acc += cnorm2(tt);
   }
   acc = block_sum_once(acc, lds2);
-  if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + blockIdx.x] = acc;
+  if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + partial_slot(blockIdx.x, a.nb)] = acc;
   if (lead) put_state(out, 1, s.iters, 0, 0, rho, s);
 }
 
@@ -367,7 +367,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_B(CGMFix a, int odd, const CGMSt
     acc += cnorm2(rr);
   }
   acc = block_sum_once(acc, lds2);
-  if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + blockIdx.x] = acc;
+  if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + partial_slot(blockIdx.x, a.nb)] = acc;
   if (lead) put_state(out, 1, s.iters + 1, 0, 0, s.rho, s);
 }
 

@@ -152,7 +152,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_start_imposed(MixFix a, const do
     acc += cnorm2(bb);
   }
   acc = block_sum(acc, lds);
-  if (threadIdx.x == 0) mslot(a, 1, rhs)[blockIdx.x] = acc;
+  if (threadIdx.x == 0) mslot(a, 1, rhs)[partial_slot(blockIdx.x, a.nb)] = acc;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     MixState cfg;
     cfg.start = GNX_START_IMPOSED;
@@ -205,8 +205,8 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_null_p2(MixFix a) {
   ar = block_sum(ar, lds);
   ab = block_sum(ab, lds);
   if (threadIdx.x == 0) {
-    mslot(a, 0, rhs)[blockIdx.x] = ar;
-    mslot(a, 1, rhs)[blockIdx.x] = ab;
+    mslot(a, 0, rhs)[partial_slot(blockIdx.x, a.nb)] = ar;
+    mslot(a, 1, rhs)[partial_slot(blockIdx.x, a.nb)] = ab;
   }
 }
 
@@ -358,7 +358,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_X(MixFix a, int odd, const MixSt
     acc += dnorm2(tt);
   }
   acc = block_sum(acc, lds);
-  if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + blockIdx.x] = acc;
+  if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + partial_slot(blockIdx.x, a.nb)] = acc;
   if (lead) put_mstate(out, 1, s.iters, PH_NORMAL, nrel, 0, 0, rho, rmax, s);
 }
 
@@ -395,7 +395,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Y(MixFix a, int odd, const MixSt
       acc += cnorm2(rr);
     }
     acc = block_sum(acc, lds);
-    if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + blockIdx.x] = acc;
+    if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + partial_slot(blockIdx.x, a.nb)] = acc;
     if (lead) put_mstate(out, 1, s.iters, PH_UPDATED, s.nrel, 0, 0, s.rho, s.rmax, s);
     return;
   }
@@ -425,7 +425,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Y(MixFix a, int odd, const MixSt
     acc += dnorm2(rn);
   }
   acc = block_sum(acc, lds);
-  if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + blockIdx.x] = acc;
+  if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + partial_slot(blockIdx.x, a.nb)] = acc;
   if (lead) put_mstate(out, 1, s.iters + 1, PH_NORMAL, s.nrel, 0, 0, s.rho, s.rmax, s);
 }
 
@@ -553,7 +553,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Xe(MixFix a, int odd, const MixS
     acc += dnorm2(tt);
   }
   acc = block_sum_once(acc, lds2);
-  if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + blockIdx.x] = acc;
+  if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + partial_slot(blockIdx.x, a.nb)] = acc;
   if (lead) put_mstate(out, 1, s.iters, PH_NORMAL, nrel, 0, 0, rho, rmax, s);
 }
 
@@ -616,7 +616,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Ye(MixFix a, int odd, const MixS
       acc += cnorm2(rr);
     }
     acc = block_sum_once(acc, lds2);
-    if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + blockIdx.x] = acc;
+    if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + partial_slot(blockIdx.x, a.nb)] = acc;
     if (lead) put_mstate(out, 1, s.iters, PH_UPDATED, s.nrel, 0, 0, s.rho, s.rmax, s);
     return;
   }
@@ -632,7 +632,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Ye(MixFix a, int odd, const MixS
     acc += dnorm2(rn);
   }
   acc = block_sum_once(acc, lds2);
-  if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + blockIdx.x] = acc;
+  if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + partial_slot(blockIdx.x, a.nb)] = acc;
   if (lead) put_mstate(out, 1, s.iters + 1, PH_NORMAL, s.nrel, 0, 0, s.rho, s.rmax, s);
 }
 
@@ -790,7 +790,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_X2(MixFix a, int odd, const MixS
 #pragma unroll
   for (int k = 0; k < 2; k++) {
     if (mode[k] != MM_NORMAL) continue;
-    if (threadIdx.x == 0) pout[(size_t)rr[k] * a.nb + blockIdx.x] = totk[k];
+    if (threadIdx.x == 0) pout[(size_t)rr[k] * a.nb + partial_slot(blockIdx.x, a.nb)] = totk[k];
     if (lead) put_mstate(sout + rr[k], 1, sk[k]->iters, PH_NORMAL, nrel[k], 0, 0, rhok[k], rmax[k], *sk[k]);
   }
 }
@@ -893,7 +893,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Y2(MixFix a, int odd, const MixS
 #pragma unroll
   for (int k = 0; k < 2; k++) {
     if (mode[k] == MM_OFF) continue;
-    if (threadIdx.x == 0) pout[(size_t)rr[k] * a.nb + blockIdx.x] = totk[k];
+    if (threadIdx.x == 0) pout[(size_t)rr[k] * a.nb + partial_slot(blockIdx.x, a.nb)] = totk[k];
     if (lead) {
       if (mode[k] == MM_UPDATE)
         put_mstate(sout + rr[k], 1, sk[k]->iters, PH_UPDATED, sk[k]->nrel, 0, 0, sk[k]->rho, sk[k]->rmax, *sk[k]);

@@ -44,6 +44,17 @@ GNX_DEVONLY const T* elem(const T* base, int i) {
   return (const T*)((const char*)base + (unsigned)i * (unsigned)sizeof(T));
 }
 
+// ---- partial slots --------------------------------------------------------------
+// Slot of block b's partial sum among nb.  Workgroups are dispatched
+// round-robin over the XCDs, so with slot = b the 16 partials of one 128-byte
+// line come from all eight XCDs and every XCD writes back a partial line;
+// grouping the blocks of one XCD (slot = (b mod 8) * nb/8 + b/8, when 8
+// divides nb) gives each line one writer.  Every reader sums all nb slots in
+// slot order, so the sums stay deterministic.
+GNX_DEVONLY int partial_slot(int b, int nb) {
+  return (nb % NUM_XCD == 0) ? (b % NUM_XCD) * (nb / NUM_XCD) + b / NUM_XCD : b;
+}
+
 // ---- streaming stores ----------------------------------------------------------
 // Vector outputs of a solver kernel are not re-read by the kernel that writes
 // them, and the next kernel (on any XCD) reads them from memory anyway, so
