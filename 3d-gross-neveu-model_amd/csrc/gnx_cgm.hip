@@ -36,6 +36,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "gnx_cgstate.h"
 #include "gnx_engine.h"
 #include "gnx_internal.h"
 #include "gnx_solver.h"
@@ -43,34 +44,6 @@
 namespace gnx {
 
 constexpr int CGM_BS = ENG_BS;
-
-// per-system state; `active` and `iters` lead, as the replay engine polls them
-struct CGMState {
-  int active;
-  int iters;
-  int converged;
-  int early;
-  double rho;
-  double tol;
-  long max_iter;
-  int start;
-  int pad;
-};
-
-// field-wise state write (an aggregate copy makes the compiler stage the
-// struct through LDS)
-GNX_DEVONLY void put_state(CGMState* o, int active, int iters, int conv, int early, double rho,
-                           const CGMState& cfg) {
-  o->active = active;
-  o->iters = iters;
-  o->converged = conv;
-  o->early = early;
-  o->rho = rho;
-  o->tol = cfg.tol;
-  o->max_iter = cfg.max_iter;
-  o->start = cfg.start;
-  o->pad = cfg.pad;  // every loaded word stays live: a dead load destination is waited on early
-}
 
 // kernel arguments: fixed per solver instance, passed by value
 struct CGMFix {

@@ -1,0 +1,499 @@
+// gnx_cgu.hip -- FP64 CG on A = M M^dag (conjugategradient.c:13-182) for n
+// independent systems on lattices of different sizes, each with its own sigma
+// mass, in one launch per kernel: the graph-replayed two-kernel state machine
+// of gnx_cgm.hip with the geometry taken per system from a descriptor table.
+//
+//   grid = (largest system's block count, n); block (b, k) works on system k
+//   when b < nb_k and leaves at once otherwise.  A system that has left the
+//   loop leaves at its state read, before any operand load, so systems whose
+//   iteration counts differ do not pay for each other's iterations beyond one
+//   state read per block.
+//
+// Per system the steps, the launch partition (one site per thread, 256-thread
+// blocks) and the order of every sum are those of gnx_cgm.hip's shape 0, so
+// a system's solution, iteration count and exit are bit-identical to its
+// single solve with that shape.  Vectors of all systems are concatenated in
+// solver-owned buffers (the caller's sigma mass, b and x_in are copied in,
+// the solution copied out); partial sums of system k live at offset poff_k
+// of each of the two slots.
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "gnx_cgstate.h"
+#include "gnx_engine.h"
+#include "gnx_internal.h"
+#include "gnx_solver.h"
+
+namespace gnx {
+
+constexpr int CGU_BS = ENG_BS;
+
+// one system of an uneven batch
+struct CGUSys {
+  Geom g;
+  int nb;     // its blocks (ceil(V / CGU_BS))
+  long voff;  // offset of its vectors, complex elements
+  long soff;  // offset of its sigma mass, doubles
+  long poff;  // offset of its partials within a slot
+};
+
+struct CGUFix {
+  const CGUSys* sys;  // [n]
+  double* sm;         // concatenated sigma masses
+  double2* b;         // concatenated right-hand sides
+  double2* x;         // concatenated solutions (x_in for the null start)
+  double2* r;
+  double2* p0;
+  double2* p1;
+  double2* t;
+  double* part;       // [2 slots][ptotal]
+  CGMState* state;    // [2][n]
+  CGDevResult* res;   // [n]
+  long ptotal;
+};
+
+// the system of this block; false if the block lies past the system's sites
+GNX_DEVONLY bool cgu_sys(const CGUFix& a, CGUSys* d) {
+  *d = a.sys[blockIdx.y];
+  return (int)blockIdx.x < d->nb;
+}
+
+GNX_DEVONLY int cgu_site(const CGUSys& d) {
+  const int f = (int)blockIdx.x * CGU_BS + (int)threadIdx.x;
+  return f < d.g.V ? f : -1;
+}
+
+// ---- start kernels (per solve, outside the graph) -------------------------------
+// imposed start (conjugategradient.c:121-128): x = 0, r = p0 = b, partial |b|^2 (slot 1)
+__global__ __launch_bounds__(CGU_BS) void k_cgu_start_imposed(CGUFix a, double tol, const long* __restrict__ max_iter) {
+  __shared__ double lds[CGU_BS / 64];
+  CGUSys d;
+  if (!cgu_sys(a, &d)) return;
+  const int f = cgu_site(d);
+  double acc = 0.0;
+  if (f >= 0) {
+    const double2 bb = *elem(a.b + d.voff, f);
+    *elem(a.x + d.voff, f) = make_double2(0.0, 0.0);
+    *elem(a.r + d.voff, f) = bb;
+    *elem(a.p0 + d.voff, f) = bb;
+    acc = cnorm2(bb);
+  }
+  acc = block_sum(acc, lds);
+  if (threadIdx.x == 0) a.part[a.ptotal + d.poff + partial_slot(blockIdx.x, d.nb)] = acc;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    CGMState s = {1, 0, 0, 0, 0.0, tol, max_iter[blockIdx.y], GNX_START_IMPOSED, 0};
+    a.state[blockIdx.y] = s;
+  }
+}
+
+// null start, pass 1 (conjugategradient.c:20): t = M^dag x_in
+__global__ __launch_bounds__(CGU_BS) void k_cgu_null_p1(CGUFix a) {
+  CGUSys d;
+  if (!cgu_sys(a, &d)) return;
+  const int f = cgu_site(d);
+  if (f < 0) return;
+  const double2* xv = a.x + d.voff;
+  *elem(a.t + d.voff, f) =
+      apply_Mdag(d.g, decode(d.g, f), *elem(a.sm + d.soff, f), [&](int q) { return *elem(xv, q); });
+}
+
+// null start, pass 2 (conjugategradient.c:19-21): r = b - M t; partials |r|^2
+// (slot 0) and |b|^2 (slot 1)
+__global__ __launch_bounds__(CGU_BS) void k_cgu_null_p2(CGUFix a) {
+  __shared__ double lds[CGU_BS / 64];
+  CGUSys d;
+  if (!cgu_sys(a, &d)) return;
+  const int f = cgu_site(d);
+  double ar = 0.0, ab = 0.0;
+  if (f >= 0) {
+    const double2* t = a.t + d.voff;
+    const double2 bb = *elem(a.b + d.voff, f);
+    const double2 rr =
+        apply_sub_M(d.g, decode(d.g, f), *elem(a.sm + d.soff, f), bb, [&](int q) { return *elem(t, q); });
+    *elem(a.r + d.voff, f) = rr;
+    ar = cnorm2(rr);
+    ab = cnorm2(bb);
+  }
+  ar = block_sum(ar, lds);
+  ab = block_sum(ab, lds);
+  if (threadIdx.x == 0) {
+    a.part[d.poff + partial_slot(blockIdx.x, d.nb)] = ar;
+    a.part[a.ptotal + d.poff + partial_slot(blockIdx.x, d.nb)] = ab;
+  }
+}
+
+// null start, pass 3 (conjugategradient.c:22-33): an early exit keeps x_in;
+// otherwise x = 0, r = p0 = b.  The |b|^2 partials stay in slot 1 for A(0).
+__global__ __launch_bounds__(CGU_BS) void k_cgu_null_p3(CGUFix a, double tol, const long* __restrict__ max_iter) {
+  __shared__ double lds[CGU_BS / 64];
+  CGUSys d;
+  if (!cgu_sys(a, &d)) return;
+  const double nr = block_sum(partial_share(a.part + d.poff, d.nb), lds);
+  const bool early = sqrt(nr) < tol;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    CGMState s = {early ? 0 : 1, 0, early ? 1 : 0, early ? 1 : 0, 0.0, tol, max_iter[blockIdx.y], GNX_START_NULL, 0};
+    a.state[blockIdx.y] = s;
+    if (early) {
+      CGDevResult r = {1, 1, 0, 0, sqrt(nr), {0, 0, 0}};
+      a.res[blockIdx.y] = r;
+    }
+  }
+  if (early) return;
+  const int f = cgu_site(d);
+  if (f < 0) return;
+  const double2 bb = *elem(a.b + d.voff, f);
+  *elem(a.x + d.voff, f) = make_double2(0.0, 0.0);
+  *elem(a.r + d.voff, f) = bb;
+  *elem(a.p0 + d.voff, f) = bb;
+}
+
+// ---- iteration kernels (the steps of k_cgm_A / k_cgm_B) -------------------------
+// A: rho from the previous partials, exit test, beta; p_new = r + beta p_old at
+// the site and its neighbours; t = M^dag p_new; partial |t|^2
+__global__ __launch_bounds__(CGU_BS) void k_cgu_A(CGUFix a, int odd, const CGMState* __restrict__ sin,
+                                                  CGMState* __restrict__ sout, const double* __restrict__ pin,
+                                                  double* __restrict__ pout) {
+  __shared__ double lds[CGU_BS / 64], lds2[CGU_BS / 64];
+  CGUSys d;
+  if (!cgu_sys(a, &d)) return;
+  const int k = blockIdx.y;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const CGMState s = sin[k];
+  CGMState* out = sout + k;
+  if (!s.active) {  // uniform per block: a finished system loads nothing more
+    if (lead) put_state(out, 0, s.iters, s.converged, s.early, s.rho, s);
+    return;
+  }
+  const Geom g = d.g;
+  const double2* r = a.r + d.voff;
+  const double2* pold = (odd ? a.p1 : a.p0) + d.voff;
+  double2* pnew = (odd ? a.p0 : a.p1) + d.voff;
+  double pl[PART_K];
+  partial_issue(pin + d.poff, d.nb, pl);
+  end_of_loads();
+  const int f = cgu_site(d);
+  const int fc = f < 0 ? 0 : f;  // past the last site: load site 0, store nothing
+  const NbrF n = neighbours_f(g, decode(g, fc));
+  const int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
+  const double smv = *elem(a.sm + d.soff, fc);
+  double2 rv[7], pv[7];
+#pragma unroll
+  for (int j = 0; j < 7; j++) {
+    rv[j] = *elem(r, q[j]);
+    pv[j] = *elem(pold, q[j]);
+  }
+  end_of_loads();
+  const double rho = block_sum_once(partial_finish(pin + d.poff, d.nb, pl), lds);
+  double beta = 0.0;
+  if (s.iters == 0) {
+    if (s.start == GNX_START_IMPOSED && sqrt(rho) < s.tol) {  // conjugategradient.c:123-128
+      if (lead) {
+        put_state(out, 0, 0, 1, 1, rho, s);
+        CGDevResult res = {1, 1, 0, 0, sqrt(rho), {0, 0, 0}};
+        a.res[k] = res;
+      }
+      return;
+    }
+  } else {
+    const bool conv = sqrt(rho) < s.tol;
+    if (conv || s.iters >= s.max_iter) {
+      if (lead) {
+        put_state(out, 0, s.iters, conv ? 1 : 0, 0, rho, s);
+        CGDevResult res = {conv ? 1 : 0, 0, s.iters, 0, sqrt(rho), {0, 0, 0}};
+        a.res[k] = res;
+      }
+      return;
+    }
+    beta = rho / s.rho;
+  }
+  double acc = 0.0;
+  if (f >= 0) {
+    const Site st = decode(g, f);
+    const double2 p0v = cadd(rv[0], rmul(beta, pv[0]));
+    double2 ppv[3], pmv[3];
+#pragma unroll
+    for (int dd = 0; dd < 3; dd++) {
+      ppv[dd] = cadd(rv[1 + dd], rmul(beta, pv[1 + dd]));
+      pmv[dd] = cadd(rv[4 + dd], rmul(beta, pv[4 + dd]));
+    }
+    st_nt(elem(pnew, f), p0v);
+    const double2 tt = apply_Mdag_fast_vals(g, st, smv, p0v, ppv, pmv);
+    st_nt(elem(a.t + d.voff, f), tt);
+    acc = cnorm2(tt);
+  }
+  acc = block_sum_once(acc, lds2);
+  if (threadIdx.x == 0) pout[d.poff + partial_slot(blockIdx.x, d.nb)] = acc;
+  if (lead) put_state(out, 1, s.iters, 0, 0, rho, s);
+}
+
+// B: gamma from A's partials; alpha = rho / gamma; x += alpha p_new;
+// r -= alpha M t; partial |r|^2
+__global__ __launch_bounds__(CGU_BS) void k_cgu_B(CGUFix a, int odd, const CGMState* __restrict__ sin,
+                                                  CGMState* __restrict__ sout, const double* __restrict__ pin,
+                                                  double* __restrict__ pout) {
+  __shared__ double lds[CGU_BS / 64], lds2[CGU_BS / 64];
+  CGUSys d;
+  if (!cgu_sys(a, &d)) return;
+  const int k = blockIdx.y;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const CGMState s = sin[k];
+  CGMState* out = sout + k;
+  if (!s.active) {
+    if (lead) put_state(out, 0, s.iters, s.converged, s.early, s.rho, s);
+    return;
+  }
+  const Geom g = d.g;
+  const double2* pnew = (odd ? a.p0 : a.p1) + d.voff;
+  const double2* t = a.t + d.voff;
+  double2* r = a.r + d.voff;
+  double2* x = a.x + d.voff;
+  double pl[PART_K];
+  partial_issue(pin + d.poff, d.nb, pl);
+  end_of_loads();
+  const int f = cgu_site(d);
+  const int fc = f < 0 ? 0 : f;
+  const NbrF n = neighbours_f(g, decode(g, fc));
+  const int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
+  const double smv = *elem(a.sm + d.soff, fc);
+  double2 tv[7];
+#pragma unroll
+  for (int j = 0; j < 7; j++) tv[j] = *elem(t, q[j]);
+  const double2 pv = *elem(pnew, fc);
+  const double2 xv = *elem(x, fc);
+  const double2 rv = *elem(r, fc);
+  end_of_loads();
+  const double gamma = block_sum_once(partial_finish(pin + d.poff, d.nb, pl), lds);
+  const double al = s.rho / gamma;
+  double acc = 0.0;
+  if (f >= 0) {
+    const double2 w = apply_M_vals(g, decode(g, f), smv, tv[0], &tv[1], &tv[4]);
+    st_nt(elem(x, f), cadd(xv, rmul(al, pv)));
+    const double2 rr = cadd(rv, rmul(-al, w));
+    st_nt(elem(r, f), rr);
+    acc = cnorm2(rr);
+  }
+  acc = block_sum_once(acc, lds2);
+  if (threadIdx.x == 0) pout[d.poff + partial_slot(blockIdx.x, d.nb)] = acc;
+  if (lead) put_state(out, 1, s.iters + 1, 0, 0, s.rho, s);
+}
+
+}  // namespace gnx
+
+// ---- host side ----------------------------------------------------------------
+struct gnx_multi {
+  int device = 0;
+  int n = 0;
+  hipStream_t stream = nullptr;
+  std::vector<gnx::CGUSys> sys;  // host copy of the descriptors
+  long vtotal = 0;               // complex elements over all systems
+  long stotal = 0;               // sigma-mass doubles over all systems
+  long ptotal = 0;               // partials per slot over all systems
+  int nbmax = 0;
+  gnx::GraphEngine eng;
+  gnx::CGUSys* dsys = nullptr;
+  long* dmax_iter = nullptr;     // [n]
+  long* hmax_iter = nullptr;     // [n], pinned
+  double* sm = nullptr;
+  double2 *b = nullptr, *x = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr, *t = nullptr;
+  gnx::CGDevResult* res = nullptr;
+  gnx::CGDevResult* hres = nullptr;  // pinned
+};
+
+namespace gnx {
+
+static void cgu_free(gnx_multi* m) {
+  void* dev[] = {m->dsys, m->dmax_iter, m->sm, m->b, m->x, m->r, m->p0, m->p1, m->t, m->res};
+  for (void* p : dev)
+    if (p) (void)hipFree(p);
+  if (m->hmax_iter) (void)hipHostFree(m->hmax_iter);
+  if (m->hres) (void)hipHostFree(m->hres);
+}
+
+static CGUFix cgu_fix(gnx_multi* m) {
+  CGUFix a;
+  a.sys = m->dsys;
+  a.sm = m->sm;
+  a.b = m->b;
+  a.x = m->x;
+  a.r = m->r;
+  a.p0 = m->p0;
+  a.p1 = m->p1;
+  a.t = m->t;
+  a.part = m->eng.dev_part();
+  a.state = (CGMState*)m->eng.dev_state();
+  a.res = m->res;
+  a.ptotal = m->ptotal;
+  return a;
+}
+
+// the chunk graph: ENG_CHUNK (A, B) pairs over all systems
+static hipError_t cgu_graph(gnx_multi* m, hipGraphExec_t* exec) {
+  CGUFix a = cgu_fix(m);
+  const int n = m->n;
+  dim3 grid(m->nbmax, n), blk(CGU_BS);
+  const CGMState* s0 = a.state;
+  const CGMState* s1 = a.state + n;
+  CGMState* w0 = a.state;
+  CGMState* w1 = a.state + n;
+  const double* q0 = a.part;
+  const double* q1 = a.part + a.ptotal;
+  double* o0 = a.part;
+  double* o1 = a.part + a.ptotal;
+  auto body = [&](hipStream_t cs) {
+    for (int i = 0; i < ENG_CHUNK; i++) {
+      int odd = i & 1;
+      // A: state[0] -> state[1], reduces slot 1, writes slot 0; B the reverse
+      void* argsA[] = {(void*)&a, (void*)&odd, (void*)&s0, (void*)&w1, (void*)&q1, (void*)&o0};
+      void* argsB[] = {(void*)&a, (void*)&odd, (void*)&s1, (void*)&w0, (void*)&q0, (void*)&o1};
+      (void)hipLaunchKernel((const void*)k_cgu_A, grid, blk, argsA, 0, cs);
+      (void)hipLaunchKernel((const void*)k_cgu_B, grid, blk, argsB, 0, cs);
+    }
+  };
+  return m->eng.graph(0, n, body, exec);
+}
+
+}  // namespace gnx
+
+extern "C" {
+
+gnx_status gnx_multi_create(int n, const int* lspace, const int* ltime, int device, gnx_multi** out) {
+  using namespace gnx;
+  CHECK_ARG(out != nullptr && lspace != nullptr && ltime != nullptr, "gnx_multi_create: null argument");
+  *out = nullptr;
+  CHECK_ARG(n >= 1 && n <= 1024, "gnx_multi_create: n must be in 1..1024");
+  for (int k = 0; k < n; k++) {
+    CHECK_ARG(lspace[k] >= 4 && ltime[k] >= 4 && lspace[k] % 2 == 0 && ltime[k] % 2 == 0,
+              "gnx_multi_create: every L and LT must be even and >= 4");
+    if ((long)lspace[k] * lspace[k] * ltime[k] > (1L << 26)) {
+      set_err("gnx_multi_create: system %d exceeds 2^26 sites", k);
+      return GNX_ERR_SIZE;
+    }
+  }
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  CHECK_ARG(device >= 0 && device < ndev, "gnx_multi_create: no such device");
+  HIP_TRY(hipSetDevice(device));
+  gnx_multi* m = new gnx_multi();
+  m->device = device;
+  m->n = n;
+  m->sys.resize(n);
+  for (int k = 0; k < n; k++) {
+    CGUSys& d = m->sys[k];
+    d.g = make_geom(lspace[k], ltime[k]);
+    d.nb = (d.g.V + CGU_BS - 1) / CGU_BS;
+    d.voff = m->vtotal;
+    d.soff = m->stotal;
+    d.poff = m->ptotal;
+    m->vtotal += d.g.V;
+    m->stotal += d.g.V;
+    m->ptotal += d.nb;
+    if (d.nb > m->nbmax) m->nbmax = d.nb;
+  }
+  const size_t vb = sizeof(double2) * (size_t)m->vtotal;
+  hipError_t e = m->eng.ensure(n, 1, sizeof(CGMState), 2 * (size_t)m->ptotal);
+  if (e == hipSuccess) e = hipMalloc(&m->dsys, sizeof(CGUSys) * n);
+  if (e == hipSuccess) e = hipMalloc(&m->dmax_iter, sizeof(long) * n);
+  if (e == hipSuccess) e = hipHostMalloc(&m->hmax_iter, sizeof(long) * n, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc(&m->sm, sizeof(double) * (size_t)m->stotal);
+  if (e == hipSuccess) e = hipMalloc(&m->b, vb);
+  if (e == hipSuccess) e = hipMalloc(&m->x, vb);
+  if (e == hipSuccess) e = hipMalloc(&m->r, vb);
+  if (e == hipSuccess) e = hipMalloc(&m->p0, vb);
+  if (e == hipSuccess) e = hipMalloc(&m->p1, vb);
+  if (e == hipSuccess) e = hipMalloc(&m->t, vb);
+  if (e == hipSuccess) e = hipMalloc(&m->res, sizeof(CGDevResult) * n);
+  if (e == hipSuccess) e = hipHostMalloc(&m->hres, sizeof(CGDevResult) * n, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMemcpy(m->dsys, m->sys.data(), sizeof(CGUSys) * n, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    set_err("gnx_multi_create: allocation failed: %s", hipGetErrorString(e));
+    cgu_free(m);
+    delete m;
+    return GNX_ERR_HIP;
+  }
+  *out = m;
+  return GNX_OK;
+}
+
+gnx_status gnx_multi_destroy(gnx_multi* m) {
+  if (!m) return GNX_OK;
+  (void)hipSetDevice(m->device);
+  (void)hipDeviceSynchronize();
+  gnx::cgu_free(m);
+  delete m;
+  return GNX_OK;
+}
+
+gnx_status gnx_multi_set_stream(gnx_multi* m, void* stream) {
+  CHECK_ARG(m != nullptr, "gnx_multi_set_stream: null handle");
+  m->stream = (hipStream_t)stream;
+  return GNX_OK;
+}
+
+gnx_status gnx_multi_cg(gnx_multi* m, const double* const* sm, const double* const* b, double* const* x, double tol,
+                        long max_iter, gnx_cg_start start, gnx_cg_result* res) {
+  using namespace gnx;
+  CHECK_ARG(m && sm && b && x, "gnx_multi_cg: null argument");
+  CHECK_ARG(tol >= 0.0, "gnx_multi_cg: negative tolerance");
+  CHECK_ARG(start == GNX_START_NULL || start == GNX_START_IMPOSED, "gnx_multi_cg: bad start");
+  for (int k = 0; k < m->n; k++) {
+    CHECK_ARG(sm[k] && b[k] && x[k], "gnx_multi_cg: null vector");
+    CHECK_ARG((const void*)b[k] != (const void*)x[k], "gnx_multi_cg: b and x alias");
+  }
+  HIP_TRY(hipSetDevice(m->device));
+  hipStream_t st = m->stream;
+  const int n = m->n;
+  long cap_max = 0;
+  for (int k = 0; k < n; k++) {
+    const CGUSys& d = m->sys[k];
+    // the reference's cap V+1 (macro.h:134) of each system when max_iter <= 0
+    m->hmax_iter[k] = max_iter > 0 ? max_iter : (long)d.g.V + 1;
+    if (m->hmax_iter[k] > cap_max) cap_max = m->hmax_iter[k];
+    const size_t vb = sizeof(double2) * (size_t)d.g.V;
+    HIP_TRY(hipMemcpyAsync(m->sm + d.soff, sm[k], sizeof(double) * (size_t)d.g.V, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(m->b + d.voff, b[k], vb, hipMemcpyDeviceToDevice, st));
+    if (start == GNX_START_NULL) HIP_TRY(hipMemcpyAsync(m->x + d.voff, x[k], vb, hipMemcpyDeviceToDevice, st));
+  }
+  HIP_TRY(hipMemcpyAsync(m->dmax_iter, m->hmax_iter, sizeof(long) * n, hipMemcpyHostToDevice, st));
+  CGUFix a = cgu_fix(m);
+  dim3 grid(m->nbmax, n), blk(CGU_BS);
+  const long* mi = m->dmax_iter;
+  if (start == GNX_START_NULL) {
+    void* a1[] = {(void*)&a};
+    HIP_TRY(hipLaunchKernel((const void*)k_cgu_null_p1, grid, blk, a1, 0, st));
+    HIP_TRY(hipLaunchKernel((const void*)k_cgu_null_p2, grid, blk, a1, 0, st));
+    void* a3[] = {(void*)&a, (void*)&tol, (void*)&mi};
+    HIP_TRY(hipLaunchKernel((const void*)k_cgu_null_p3, grid, blk, a3, 0, st));
+  } else {
+    void* a0[] = {(void*)&a, (void*)&tol, (void*)&mi};
+    HIP_TRY(hipLaunchKernel((const void*)k_cgu_start_imposed, grid, blk, a0, 0, st));
+  }
+  hipGraphExec_t exec;
+  HIP_TRY(cgu_graph(m, &exec));
+  HIP_TRY(m->eng.run(exec, n, cap_max, st, nullptr));
+  for (int k = 0; k < n; k++) {
+    const CGUSys& d = m->sys[k];
+    HIP_TRY(hipMemcpyAsync(x[k], m->x + d.voff, sizeof(double2) * (size_t)d.g.V, hipMemcpyDeviceToDevice, st));
+  }
+  HIP_TRY(hipMemcpyAsync(m->hres, m->res, sizeof(CGDevResult) * n, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  int all = 1;
+  for (int k = 0; k < n; k++) {
+    if (res) {
+      res[k].converged = m->hres[k].converged;
+      res[k].early_exit = m->hres[k].early_exit;
+      res[k].iterations = m->hres[k].iterations;
+      res[k].reliable_updates = 0;
+      res[k].residual = m->hres[k].residual;
+    }
+    all &= m->hres[k].converged;
+  }
+  if (!all) {
+    set_err("gnx_multi_cg: a system reached its iteration cap");
+    return GNX_ERR_NOCONV;
+  }
+  return GNX_OK;
+}
+
+}  // extern "C"

@@ -114,6 +114,12 @@ def lib():
         L.gnx_link_hmc_accept.argtypes = [vp, C.c_int]
         L.gnx_link_hmc_energy.argtypes = [vp, dp]
         L.gnx_link_hmc_cg_stats.argtypes = [vp, dp, C.POINTER(C.c_int)]
+        ip = C.POINTER(C.c_int)
+        L.gnx_multi_create.argtypes = [C.c_int, ip, ip, C.c_int, C.POINTER(vp)]
+        L.gnx_multi_destroy.argtypes = [vp]
+        L.gnx_multi_set_stream.argtypes = [vp, vp]
+        L.gnx_multi_cg.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.c_double, C.c_long,
+                                   C.c_int, C.POINTER(CGResult)]
         _lib = L
     return _lib
 
@@ -384,6 +390,56 @@ class HMC:
         calls = C.c_int()
         _check(lib().gnx_hmc_cg_stats(self.h, C.byref(it), C.byref(calls)))
         return it.value, calls.value
+
+
+class Multi:
+    """Uneven batch (gnx_multi_*): n systems on lattices of different sizes,
+    solved together by one graph-replayed FP64 CG."""
+
+    def __init__(self, sizes, device=0, stream=None):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("gnx needs a ROCm GPU; no device is visible")
+        self.sizes = [(L, LT or L) for L, LT in ((s if isinstance(s, tuple) else (s, s)) for s in sizes)]
+        n = len(self.sizes)
+        Ls = (C.c_int * n)(*[L for L, _ in self.sizes])
+        LTs = (C.c_int * n)(*[LT for _, LT in self.sizes])
+        h = C.c_void_p()
+        _check(lib().gnx_multi_create(n, Ls, LTs, device, C.byref(h)))
+        self.h = h
+        self.n = n
+        self.V = [L * L * LT for L, LT in self.sizes]
+        self.device = torch.device("cuda", device)
+        s = stream if stream is not None else torch.cuda.current_stream(self.device)
+        self.stream = s
+        _check(lib().gnx_multi_set_stream(self.h, C.c_void_p(s.cuda_stream)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gnx_multi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def cg(self, sms, bs, xs, tol, max_iter=0, start=START_NULL):
+        """sms[k] (V_k), bs[k], xs[k] (2 V_k) device tensors; returns (status, [CGResult])."""
+        vp = C.c_void_p
+        n = self.n
+        assert len(sms) == len(bs) == len(xs) == n
+        for k in range(n):
+            assert sms[k].numel() == self.V[k] and as_real(bs[k]).numel() == 2 * self.V[k]
+            assert as_real(xs[k]).numel() == 2 * self.V[k]
+        res = (CGResult * n)()
+        st = lib().gnx_multi_cg(self.h, (vp * n)(*[_ptr(t) for t in sms]),
+                                (vp * n)(*[_ptr(as_real(t)) for t in bs]),
+                                (vp * n)(*[_ptr(as_real(t)) for t in xs]), tol, max_iter, start, res)
+        if st not in (0, 3):
+            _check(st)
+        return st, list(res)
 
 
 class LinkHMC:

@@ -31,6 +31,7 @@ extern "C" {
 #define GNX_ABI_VERSION 2
 
 typedef struct gnx_ctx gnx_ctx;
+typedef struct gnx_multi gnx_multi;
 
 typedef enum gnx_status {
     GNX_OK = 0,
@@ -110,6 +111,23 @@ gnx_status gnx_cg(gnx_ctx *ctx, const double *sm, const double *b, double *x, do
 gnx_status gnx_cg_batch(gnx_ctx *ctx, const double *sm, const double *b, double *x, int nrhs,
                         double tol, long max_iter, gnx_cg_start start, gnx_prec prec,
                         gnx_cg_result *res);
+
+/* Uneven batches (extension 2: independent chains on lattices of uneven
+ * size).  n systems, system k on an lspace[k]^2 x ltime[k] lattice with its
+ * own sigma mass, solved by the FP64 CG of gnx_cg (both starts, the same exit
+ * rules, the cap V_k + 1 when max_iter <= 0) in one launch per kernel: one
+ * graph-replayed kernel pair per iteration over all systems, each block
+ * reading its system's geometry from a descriptor table; a system that has
+ * left the loop costs one state read per block per kernel.  Each system's
+ * solution and iteration count are bit-identical to its gnx_cg solve with
+ * the one-site-per-thread launch shape (GNX_CG_SHAPE=0).  sm[k], b[k], x[k]
+ * are device pointers (V_k doubles, 2 V_k doubles); res[n].  Returns
+ * GNX_ERR_NOCONV when a system reaches its cap. */
+gnx_status gnx_multi_create(int n, const int *lspace, const int *ltime, int device, gnx_multi **out);
+gnx_status gnx_multi_destroy(gnx_multi *m);
+gnx_status gnx_multi_set_stream(gnx_multi *m, void *stream);
+gnx_status gnx_multi_cg(gnx_multi *m, const double *const *sm, const double *const *b, double *const *x,
+                        double tol, long max_iter, gnx_cg_start start, gnx_cg_result *res);
 
 /* A12, batched.  Stochastic trace term of the condensate for nsrc noise
  * vectors at once (condensate.c:127-145 is this for one vector at a time):
