@@ -32,7 +32,8 @@ BYTES_CG_ITER_F64 = 144   # 8*16 + 2*8 per site per CG iteration
 PROFILES = os.path.join(ROOT, "profiles")
 TRAFFIC_JSON = os.path.join(PROFILES, "cg_f64_L64_traffic.json")
 # sources whose change invalidates a recorded traffic measurement
-KERNEL_SOURCES = ["gnx_cgm.hip", "gnx_solver.h", "gnx_stencil.h", "gnx_common.h", "gnx_engine.h", "gnx_engine.hip"]
+KERNEL_SOURCES = ["gnx_cgm.hip", "gnx_cgstate.h", "gnx_solver.h", "gnx_stencil.h", "gnx_common.h", "gnx_engine.h",
+                  "gnx_engine.hip"]
 
 
 def kernel_sources_sha():
@@ -278,6 +279,76 @@ def ensemble_leg(gnx, sizes, ntraj, betas=(0.6, 0.75, 0.85, 1.0), eps=0.02, step
     return out
 
 
+def uneven_leg(gnx, torch, sizes, tol, betas=(0.6, 0.75, 0.85, 1.0)):
+    """Independent systems on lattices of uneven size, one per (L, beta_s):
+    sigma iid N(0, 2/(4 beta_s)), a complex Z2 source, FP64 CG to tol
+    (imposed start).  Solved three ways: all in one uneven batch
+    (gnx_multi_cg: one graph-replayed kernel pair per iteration over every
+    system), one after another on per-size contexts, and concurrently, each
+    on its own context, stream and host thread."""
+    import concurrent.futures as cf
+    dev = torch.device("cuda", 0)
+    systems, ctx_of = [], {}
+    for L in sizes:
+        ctx_of[L] = gnx.Context(L)
+        for bs in betas:
+            V = L ** 3
+            rng = np.random.default_rng(7000 + 10 * L + int(bs * 100))
+            sm = ctx_of[L].field()
+            ctx_of[L].sigma_mass(torch.from_numpy(rng.normal(0.0, np.sqrt(2.0 / (4 * bs)), V)).to(dev), sm)
+            b = torch.from_numpy(np.where(rng.random(2 * V) <= 0.5, 1.0, -1.0)).to(dev)
+            systems.append({"L": L, "beta_s": bs, "V": V, "sm": sm, "b": b})
+    n = len(systems)
+    out = {"workload": f"{n} systems on {sizes}^3 x beta_s {list(betas)}, own sigma each, "
+                       f"complex Z2 source, FP64 to tol {tol:g}"}
+    # one uneven batch
+    m = gnx.Multi([s["L"] for s in systems])
+    xs = [torch.zeros(2 * s["V"], dtype=torch.float64, device=dev) for s in systems]
+    args = ([s["sm"] for s in systems], [s["b"] for s in systems], xs, tol)
+    m.cg(*args, start=gnx.START_IMPOSED)  # graph capture + warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st, res = m.cg(*args, start=gnx.START_IMPOSED)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out["one_batch"] = {"rhs_solves_per_s": n / dt, "seconds": dt, "all_converged": st == 0,
+                        "iterations": [r.iterations for r in res]}
+    # one after another
+    x1 = [torch.zeros_like(x) for x in xs]
+    for s, x in zip(systems, x1):  # tune + warm each size
+        ctx_of[s["L"]].cg(s["sm"], s["b"], x, tol, start=gnx.START_IMPOSED)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s, x in zip(systems, x1):
+        ctx_of[s["L"]].cg(s["sm"], s["b"], x, tol, start=gnx.START_IMPOSED)
+    torch.cuda.synchronize()
+    dt1 = time.perf_counter() - t0
+    out["one_after_another"] = {"rhs_solves_per_s": n / dt1, "seconds": dt1}
+    # concurrently: a context, stream and host thread per system
+    ctxs = [gnx.Context(s["L"], stream=torch.cuda.Stream(dev)) for s in systems]
+    x2 = [torch.zeros_like(x) for x in xs]
+
+    def solve(k):
+        ctxs[k].cg(systems[k]["sm"], systems[k]["b"], x2[k], tol, start=gnx.START_IMPOSED)
+        ctxs[k].synchronize()
+    with cf.ThreadPoolExecutor(max_workers=n) as ex:
+        list(ex.map(solve, range(n)))  # tune + warm
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=n) as ex:
+        list(ex.map(solve, range(n)))
+    torch.cuda.synchronize()
+    dt2 = time.perf_counter() - t0
+    out["concurrent_streams"] = {"rhs_solves_per_s": n / dt2, "seconds": dt2}
+    out["speedup_vs_one_after_another"] = dt1 / dt
+    out["speedup_vs_concurrent_streams"] = dt2 / dt
+    out["max_rel_diff_vs_single"] = max(float((a - b).norm() / b.norm()) for a, b in zip(xs, x1))
+    m.close()
+    for c in ctxs + list(ctx_of.values()):
+        c.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -478,6 +549,8 @@ def main():
     if args.chains:
         result["ensembles_with_links"] = ensemble_leg(gnx=gnx, sizes=[int(v) for v in args.chains.split(",")],
                                                       ntraj=args.chain_trajectories)
+        result["uneven_batch"] = uneven_leg(gnx=gnx, torch=torch, sizes=[int(v) for v in args.chains.split(",")],
+                                            tol=args.tol)
     if args.copy_calibration > 0:
         n = args.copy_calibration * (1 << 20) // 8
         src = torch.ones(n, dtype=torch.float64, device=dev)
