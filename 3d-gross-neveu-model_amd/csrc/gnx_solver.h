@@ -11,7 +11,11 @@ constexpr int NUM_XCD = 8;
 // ---- launch shapes ----------------------------------------------------------
 // SPT sites per thread; XCD = remap blocks so that each XCD (workgroups are
 // dispatched round-robin over the 8 XCDs) sweeps one contiguous lex slab of
-// both parities, keeping a slab's halo in its own L2.
+// both parities, keeping a slab's halo in its own L2.  Inside the slab the
+// thread-site index j runs in groups of 128: the first 64 are even sites, the
+// next 64 the odd sites at the same positions, so every wave reads one
+// parity's contiguous run (lanes alternating between the two halves, 2 MB
+// apart at 64^3, cost the L1 half its throughput).
 GNX_DEVONLY int xcd_block(int b, int nb) {
   int q = nb / NUM_XCD, rem = nb % NUM_XCD, x = b % NUM_XCD;
   return x * q + (x < rem ? x : rem) + b / NUM_XCD;
@@ -25,7 +29,10 @@ struct SiteMap {
   GNX_DEVONLY int f(const Geom& g, int i) const {
     int j = base + i * ENG_BS + (int)threadIdx.x;
     if (j >= g.V) return -1;
-    return XCD ? (j >> 1) + (j & 1) * g.half : j;
+    if (!XCD) return j;
+    const int G = j >> 7, t = j & 127;
+    const int r = min(64, g.half - 64 * G);  // sites per parity in this group (< 64 only in the last)
+    return 64 * G + (t < r ? t : t - r + g.half);
   }
 };
 
