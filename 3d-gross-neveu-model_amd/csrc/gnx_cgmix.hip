@@ -39,20 +39,21 @@ struct StoreF32 {
   GNX_DEVONLY static void st(T* p, int i, float2 v) { st_nt(elem(p, i), v); }
 };
 
-// bfloat16 pair (re in the low half), round-to-nearest-even on store
+// bfloat16 pair (re in the low half), round-to-nearest-even on store: one
+// v_cvt_pk_bf16_f32 (gfx950) for the pair
+typedef __bf16 gnx_bf16x2 __attribute__((ext_vector_type(2)));
+typedef float gnx_f32x2 __attribute__((ext_vector_type(2)));
 struct StoreBF16 {
   using T = unsigned int;
   GNX_DEVONLY static float up(unsigned int h) { return __uint_as_float(h << 16); }
-  GNX_DEVONLY static unsigned int down(float f) {
-    unsigned int u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return u >> 16;
-  }
   GNX_DEVONLY static float2 ld(const T* p, int i) {
     unsigned int v = *elem(p, i);
     return make_float2(up(v & 0xFFFFu), up(v >> 16));
   }
-  GNX_DEVONLY static void st(T* p, int i, float2 v) { st_nt(elem(p, i), down(v.x) | (down(v.y) << 16)); }
+  GNX_DEVONLY static void st(T* p, int i, float2 v) {
+    const gnx_f32x2 w = {v.x, v.y};
+    st_nt(elem(p, i), __builtin_bit_cast(unsigned int, __builtin_convertvector(w, gnx_bf16x2)));
+  }
 };
 
 constexpr int MIX_BS = ENG_BS;
@@ -73,6 +74,10 @@ struct MixState {
   double tol;
   double delta;
   long max_iter;
+  // loads-first form: step lengths whose alpha p terms are not yet in xs
+  // (alpha_a for the newest direction, alpha_b for the one before)
+  double alpha_a;
+  double alpha_b;
 };
 
 GNX_DEVONLY void put_mstate(MixState* o, int active, int iters, int phase, int nrel, int conv, int early,
@@ -90,6 +95,12 @@ GNX_DEVONLY void put_mstate(MixState* o, int active, int iters, int phase, int n
   o->tol = cfg.tol;
   o->delta = cfg.delta;
   o->max_iter = cfg.max_iter;
+  o->alpha_a = cfg.alpha_a;
+  o->alpha_b = cfg.alpha_b;
+}
+GNX_DEVONLY void put_alphas(MixState* o, double alpha_a, double alpha_b) {
+  o->alpha_a = alpha_a;
+  o->alpha_b = alpha_b;
 }
 
 // kernel arguments, fixed per solver instance (buffers the solver owns)
@@ -102,7 +113,7 @@ struct MixFix {
   double2* u;       // FP64 M^dag (x + xs) of a reliable update
   float2* xs;       // FP32 solution correction
   void* r;          // storage-format residual
-  void* p0;
+  void* p0;  // direction buffers, slot parity: X reads p[sl], writes p[sl+1]
   void* p1;
   void* t;
   double* part;     // [2 slots][nrhs][nb]
@@ -113,6 +124,8 @@ struct MixFix {
 };
 
 GNX_DEV double* mslot(const MixFix& a, int s, int rhs) { return a.part + ((size_t)s * a.nrhs + rhs) * a.nb; }
+
+GNX_DEVONLY void* mix_p(const MixFix& a, int k) { return (k & 1) ? a.p1 : a.p0; }
 
 GNX_DEVONLY double dnorm2(float2 v) { return (double)v.x * v.x + (double)v.y * v.y; }
 
@@ -144,6 +157,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_start_imposed(MixFix a, const do
     float2 bs = make_float2((float)bb.x, (float)bb.y);
     S::st((typename S::T*)a.r + off, f, bs);
     S::st((typename S::T*)a.p0 + off, f, bs);
+    S::st((typename S::T*)a.p1 + off, f, make_float2(0.f, 0.f));
     a.xs[off + f] = make_float2(0.f, 0.f);
     if (rhs == 0) {
       a.sm[f] = sm_in[f];
@@ -154,7 +168,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_start_imposed(MixFix a, const do
   acc = block_sum(acc, lds);
   if (threadIdx.x == 0) mslot(a, 1, rhs)[partial_slot(blockIdx.x, a.nb)] = acc;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    MixState cfg;
+    MixState cfg = {};
     cfg.start = GNX_START_IMPOSED;
     cfg.tol = tol;
     cfg.delta = delta;
@@ -219,7 +233,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_null_p3(MixFix a, double tol, do
   const double nr = block_sum(partial_share(mslot(a, 0, rhs), a.nb), lds);
   const bool early = sqrt(nr) < tol;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    MixState cfg;
+    MixState cfg = {};
     cfg.start = GNX_START_NULL;
     cfg.tol = tol;
     cfg.delta = delta;
@@ -240,6 +254,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_null_p3(MixFix a, double tol, do
     float2 bs = make_float2((float)bb.x, (float)bb.y);
     S::st((typename S::T*)a.r + off, f, bs);
     S::st((typename S::T*)a.p0 + off, f, bs);
+    S::st((typename S::T*)a.p1 + off, f, make_float2(0.f, 0.f));
     a.xs[off + f] = make_float2(0.f, 0.f);
   }
 }
@@ -345,12 +360,12 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_X(MixFix a, int odd, const MixSt
   for (int i = 0; i < SPT; i++) {
     if (fs[i] < 0) continue;
     const Site st = decode(g, fs[i]);
-    float2 p0v = cadd(rv[i][0], rmul(fb, pv[i][0]));
+    float2 p0v = cfma(fb, pv[i][0], rv[i][0]);
     float2 ppv[3], pmv[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      ppv[d] = cadd(rv[i][1 + d], rmul(fb, pv[i][1 + d]));
-      pmv[d] = cadd(rv[i][4 + d], rmul(fb, pv[i][4 + d]));
+      ppv[d] = cfma(fb, pv[i][1 + d], rv[i][1 + d]);
+      pmv[d] = cfma(fb, pv[i][4 + d], rv[i][4 + d]);
     }
     S::st(pnew, fs[i], p0v);
     float2 tt = apply_Mdag_fast_vals(g, st, smv[i], p0v, ppv, pmv);
@@ -419,8 +434,8 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Y(MixFix a, int odd, const MixSt
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
     if (fs[i] < 0) continue;
-    st_nt(&a.xs[off + fs[i]], make_float2(xv[i].x + al * pv[i].x, xv[i].y + al * pv[i].y));
-    float2 rn = make_float2(rv[i].x - al * w[i].x, rv[i].y - al * w[i].y);
+    st_nt(&a.xs[off + fs[i]], cfma(al, pv[i], xv[i]));
+    float2 rn = cfma(-al, w[i], rv[i]);
     S::st(r, fs[i], rn);
     acc += dnorm2(rn);
   }
@@ -430,14 +445,27 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Y(MixFix a, int odd, const MixSt
 }
 
 // ---- iteration kernels, loads-first form ------------------------------------------
-// The same steps as k_mix_X / k_mix_Y, but every load of a kernel (state,
-// partials, then all operands of the normal step, whatever the phase) is
-// issued before its first wait, and the partial reduction runs while the
-// operands are in flight.  This holds every operand in registers at once
-// (lower occupancy), so which form is faster depends on the storage format
-// and batch size: the tuner times both.
-template <typename S, int SPT, bool XCD>
-__global__ __launch_bounds__(MIX_BS) void k_mix_Xe(MixFix a, int odd, const MixState* __restrict__ sin,
+// The steps of k_mix_X / k_mix_Y with two changes.
+// (1) Every load of a kernel (state, partials, then all operands of the
+// normal step, whatever the phase) is issued before its first wait, and the
+// partial reduction runs while the operands are in flight.  This holds every
+// operand in registers at once (lower occupancy), so which form is faster
+// depends on the storage format and batch size: the tuner times both.
+// (2) The FP32 correction xs is not touched by the Y kernel: Y leaves its
+// step length alpha pending in the state, and the X kernel of every even
+// slot adds both pending terms, xs += alpha_b p_prev + alpha_a p_old, in
+// that order, while it reads p_old anyway.  Y reads only t and r (the
+// 16-byte xs read-modify-write and the p read leave the every-slot path),
+// X gains xs and p_prev on every other slot.  A reliable update folds the
+// pending terms into x: its u = M^dag (x + c) and the following x += c use
+// the same c = xs + alpha_b p_prev + alpha_a p_old, computed alike.
+// Two direction buffers, by slot parity: p_prev is p[sl+1], the buffer X
+// writes, so it is read only at the thread's own sites, before that thread's
+// store (a third and fourth buffer would push a batch of 32 at 64^3 past the
+// 256 MB Infinity Cache); an update's X gathers p_prev at the neighbours and
+// so writes no direction, its Y carries p_old into p[sl+1] after the fold.
+template <typename S, int SPT, bool XCD, bool FLUSH>
+__global__ __launch_bounds__(MIX_BS) void k_mix_Xe(MixFix a, int sl, const MixState* __restrict__ sin,
                                                   MixState* __restrict__ sout, const double* __restrict__ pin,
                                                   double* __restrict__ pout) {
   __shared__ double lds[MIX_BS / 64], lds2[MIX_BS / 64];  // head and tail reductions
@@ -447,8 +475,9 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Xe(MixFix a, int odd, const MixS
   const size_t off = (size_t)rhs * g.V;
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   const T* r = (const T*)a.r + off;
-  const T* pold = (const T*)(odd ? a.p1 : a.p0) + off;
-  T* pnew = (T*)(odd ? a.p0 : a.p1) + off;
+  const T* pold = (const T*)mix_p(a, sl) + off;
+  const T* pprev = (const T*)mix_p(a, sl + 1) + off;  // = pnew: own sites only, read before the store
+  T* pnew = (T*)mix_p(a, sl + 1) + off;
   // loads in the order they are needed (state and partials, then the
   // normal step's operands), all issued before the first wait; the
   // reduction runs while the operands are in flight
@@ -457,7 +486,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Xe(MixFix a, int odd, const MixS
   partial_issue(pin + (size_t)rhs * a.nb, a.nb, pl);
   end_of_loads();
   int fs[SPT];
-  float2 rv[SPT][7], pv[SPT][7];
+  float2 rv[SPT][7], pv[SPT][7], xsv[SPT], ppr[SPT];
   float smv[SPT];
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
@@ -470,6 +499,10 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Xe(MixFix a, int odd, const MixS
     for (int k = 0; k < 7; k++) {
       rv[i][k] = S::ld(r, q[k]);
       pv[i][k] = S::ld(pold, q[k]);
+    }
+    if (FLUSH) {
+      xsv[i] = *elem(a.xs + off, fc);
+      ppr[i] = S::ld(pprev, fc);
     }
   }
   end_of_loads();
@@ -514,38 +547,40 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Xe(MixFix a, int odd, const MixS
     update = (nr < s.delta * s.rmax) || (nr < s.tol) || (s.iters >= s.max_iter);
     beta = rho / s.rho;
   }
+  const float fa = (float)s.alpha_a, fp = (float)s.alpha_b;
   if (update) {
-    // reliable update, part 1: u = M^dag (x + xs) in FP64 (x and xs are not
-    // written in this kernel, so neighbours read consistent values); the
-    // direction is carried into the other parity buffer unchanged
+    // reliable update, part 1: u = M^dag (x + c) in FP64 with the pending
+    // terms in c (x, xs and both directions are not written in this kernel,
+    // so neighbours read consistent values); the pending alphas stay for Y
     const double2* x = a.x + off;
     const float2* xs = a.xs + off;
     auto ldx = [&](int q) {
-      float2 c = xs[q];
+      float2 c = cfma(fa, S::ld(pold, q), cfma(fp, S::ld(pprev, q), xs[q]));
       return cadd(x[q], make_double2((double)c.x, (double)c.y));
     };
 #pragma unroll
     for (int i = 0; i < SPT; i++) {
       if (fs[i] < 0) continue;
       a.u[off + fs[i]] = apply_Mdag(g, decode(g, fs[i]), a.sm[fs[i]], ldx);
-      S::st(pnew, fs[i], pv[i][0]);
     }
     if (lead) put_mstate(out, 1, s.iters, PH_UPDATE, nrel, 0, 0, s.rho, rmax, s);
     return;
   }
-  // normal step: p = r + beta p, t = M^dag p
+  // normal step: (even slots) xs += alpha_b p_prev + alpha_a p_old;
+  // p = r + beta p, t = M^dag p
   const float fb = (float)beta;
   double acc = 0.0;
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
     if (fs[i] < 0) continue;
+    if (FLUSH) st_nt(elem(a.xs + off, fs[i]), cfma(fa, pv[i][0], cfma(fp, ppr[i], xsv[i])));
     const Site st = decode(g, fs[i]);
-    float2 p0v = cadd(rv[i][0], rmul(fb, pv[i][0]));
+    float2 p0v = cfma(fb, pv[i][0], rv[i][0]);
     float2 ppv[3], pmv[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      ppv[d] = cadd(rv[i][1 + d], rmul(fb, pv[i][1 + d]));
-      pmv[d] = cadd(rv[i][4 + d], rmul(fb, pv[i][4 + d]));
+      ppv[d] = cfma(fb, pv[i][1 + d], rv[i][1 + d]);
+      pmv[d] = cfma(fb, pv[i][4 + d], rv[i][4 + d]);
     }
     S::st(pnew, fs[i], p0v);
     float2 tt = apply_Mdag_fast_vals(g, st, smv[i], p0v, ppv, pmv);
@@ -554,11 +589,14 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Xe(MixFix a, int odd, const MixS
   }
   acc = block_sum_once(acc, lds2);
   if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + partial_slot(blockIdx.x, a.nb)] = acc;
-  if (lead) put_mstate(out, 1, s.iters, PH_NORMAL, nrel, 0, 0, rho, rmax, s);
+  if (lead) {
+    put_mstate(out, 1, s.iters, PH_NORMAL, nrel, 0, 0, rho, rmax, s);
+    if (FLUSH) put_alphas(out, 0.0, 0.0);
+  }
 }
 
 template <typename S, int SPT, bool XCD>
-__global__ __launch_bounds__(MIX_BS) void k_mix_Ye(MixFix a, int odd, const MixState* __restrict__ sin,
+__global__ __launch_bounds__(MIX_BS) void k_mix_Ye(MixFix a, int sl, const MixState* __restrict__ sin,
                                                   MixState* __restrict__ sout, const double* __restrict__ pin,
                                                   double* __restrict__ pout) {
   __shared__ double lds[MIX_BS / 64], lds2[MIX_BS / 64];  // head and tail reductions
@@ -567,10 +605,9 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Ye(MixFix a, int odd, const MixS
   const int rhs = blockIdx.y;
   const size_t off = (size_t)rhs * g.V;
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-  const T* pnew = (const T*)(odd ? a.p0 : a.p1) + off;
   const T* t = (const T*)a.t + off;
   T* r = (T*)a.r + off;
-  // loads as in k_mix_X: state and partials, then the normal step's
+  // loads as in k_mix_Xe: state and partials, then the normal step's
   // operands, issued whatever the phase (an update or a finished system
   // ignores them), so that none waits for the state
   const MixState s = load_state(sin + rhs);
@@ -579,7 +616,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Ye(MixFix a, int odd, const MixS
   end_of_loads();
   int fs[SPT];
   float smv[SPT];
-  float2 tv[SPT][7], pv[SPT], xv[SPT], rv[SPT];
+  float2 tv[SPT][7], rv[SPT];
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
     fs[i] = mix_site<SPT, XCD>(a, i);
@@ -589,8 +626,6 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Ye(MixFix a, int odd, const MixS
     smv[i] = *elem(a.sm32, fc);
 #pragma unroll
     for (int k = 0; k < 7; k++) tv[i][k] = S::ld(t, q[k]);
-    pv[i] = S::ld(pnew, fc);
-    xv[i] = *elem(a.xs + off, fc);
     rv[i] = S::ld(r, fc);
   }
   end_of_loads();
@@ -602,13 +637,20 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Ye(MixFix a, int odd, const MixS
   }
   double acc = 0.0;
   if (s.phase == PH_UPDATE) {
-    // reliable update, part 2: x += xs, xs = 0, r = (S)(b - M u)
+    // reliable update, part 2: x += c (the c of part 1), xs = 0,
+    // r = (S)(b - M u); the direction is carried into p[sl+1] (p_prev's
+    // buffer, read first) unchanged
     const double2* u = a.u + off;
+    const T* pold = (const T*)mix_p(a, sl) + off;
+    T* pnext = (T*)mix_p(a, sl + 1) + off;
+    const float fa = (float)s.alpha_a, fp = (float)s.alpha_b;
 #pragma unroll
     for (int i = 0; i < SPT; i++) {
       const int f = fs[i];
       if (f < 0) break;
-      float2 c = a.xs[off + f];
+      const float2 po = S::ld(pold, f);
+      const float2 c = cfma(fa, po, cfma(fp, S::ld(pnext, f), a.xs[off + f]));
+      S::st(pnext, f, po);
       a.x[off + f] = cadd(a.x[off + f], make_double2((double)c.x, (double)c.y));
       a.xs[off + f] = make_float2(0.f, 0.f);
       double2 rr = apply_sub_M(g, decode(g, f), a.sm[f], a.b[off + f], [&](int q) { return u[q]; });
@@ -617,23 +659,28 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Ye(MixFix a, int odd, const MixS
     }
     acc = block_sum_once(acc, lds2);
     if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + partial_slot(blockIdx.x, a.nb)] = acc;
-    if (lead) put_mstate(out, 1, s.iters, PH_UPDATED, s.nrel, 0, 0, s.rho, s.rmax, s);
+    if (lead) {
+      put_mstate(out, 1, s.iters, PH_UPDATED, s.nrel, 0, 0, s.rho, s.rmax, s);
+      put_alphas(out, 0.0, 0.0);
+    }
     return;
   }
-  // normal step: xs += alpha p, r -= alpha M t
+  // normal step: r -= alpha M t; alpha p is left pending for the X kernel
   const float al = (float)(s.rho / gamma);
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
     if (fs[i] < 0) continue;
     const float2 w = apply_M_vals(g, decode(g, fs[i]), smv[i], tv[i][0], &tv[i][1], &tv[i][4]);
-    st_nt(elem(a.xs + off, fs[i]), make_float2(xv[i].x + al * pv[i].x, xv[i].y + al * pv[i].y));
-    float2 rn = make_float2(rv[i].x - al * w.x, rv[i].y - al * w.y);
+    float2 rn = cfma(-al, w, rv[i]);
     S::st(r, fs[i], rn);
     acc += dnorm2(rn);
   }
   acc = block_sum_once(acc, lds2);
   if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + partial_slot(blockIdx.x, a.nb)] = acc;
-  if (lead) put_mstate(out, 1, s.iters + 1, PH_NORMAL, s.nrel, 0, 0, s.rho, s.rmax, s);
+  if (lead) {
+    put_mstate(out, 1, s.iters + 1, PH_NORMAL, s.nrel, 0, 0, s.rho, s.rmax, s);
+    put_alphas(out, (double)al, s.alpha_a);
+  }
 }
 
 // ---- paired iteration kernels -------------------------------------------------------
@@ -770,12 +817,12 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_X2(MixFix a, int odd, const MixS
       for (int i = 0; i < SPT; i++) {
         if (fs[i] < 0) continue;
         const Site st = decode(g, fs[i]);
-        float2 p0v = cadd(rv[k][i][0], rmul(fb, pv[k][i][0]));
+        float2 p0v = cfma(fb, pv[k][i][0], rv[k][i][0]);
         float2 ppv[3], pmv[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-          ppv[d] = cadd(rv[k][i][1 + d], rmul(fb, pv[k][i][1 + d]));
-          pmv[d] = cadd(rv[k][i][4 + d], rmul(fb, pv[k][i][4 + d]));
+          ppv[d] = cfma(fb, pv[k][i][1 + d], rv[k][i][1 + d]);
+          pmv[d] = cfma(fb, pv[k][i][4 + d], rv[k][i][4 + d]);
         }
         S::st(pnew[k], fs[i], p0v);
         float2 tt = apply_Mdag_fast_vals(g, st, smv[i], p0v, ppv, pmv);
@@ -881,8 +928,8 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Y2(MixFix a, int odd, const MixS
 #pragma unroll
       for (int i = 0; i < SPT; i++) {
         if (fs[i] < 0) continue;
-        st_nt(&a.xs[off + fs[i]], make_float2(xv[k][i].x + al * pv[k][i].x, xv[k][i].y + al * pv[k][i].y));
-        float2 rn = make_float2(rv[k][i].x - al * w[k][i].x, rv[k][i].y - al * w[k][i].y);
+        st_nt(&a.xs[off + fs[i]], cfma(al, pv[k][i], xv[k][i]));
+        float2 rn = cfma(-al, w[k][i], rv[k][i]);
         S::st((T*)a.r + off, fs[i], rn);
         acc[k] += dnorm2(rn);
       }
@@ -969,7 +1016,8 @@ struct MixShape {
   bool xcd;
   bool pair;   // two systems per block (k_mix_X2 / k_mix_Y2)
   bool early;  // loads-first kernels (k_mix_Xe / k_mix_Ye)
-  const void* X;
+  const void* X;   // X kernel of odd slots
+  const void* Xf;  // X kernel of even slots (the loads-first form's xs flush)
   const void* Y;
   const void* start_imposed;
   const void* null_p1;
@@ -978,15 +1026,18 @@ struct MixShape {
 };
 
 #define GNX_MIX_SHAPE(S, N, X)                                                                     \
-  {N, X, false, false, (const void*)k_mix_X<S, N, X>, (const void*)k_mix_Y<S, N, X>,              \
+  {N, X, false, false, (const void*)k_mix_X<S, N, X>, (const void*)k_mix_X<S, N, X>,             \
+   (const void*)k_mix_Y<S, N, X>,                                                                 \
    (const void*)k_mix_start_imposed<S, N, X>, (const void*)k_mix_null_p1<N, X>,                   \
    (const void*)k_mix_null_p2<N, X>, (const void*)k_mix_null_p3<S, N, X>}
 #define GNX_MIX_PAIR(S, N, X)                                                                      \
-  {N, X, true, false, (const void*)k_mix_X2<S, N, X>, (const void*)k_mix_Y2<S, N, X>,             \
+  {N, X, true, false, (const void*)k_mix_X2<S, N, X>, (const void*)k_mix_X2<S, N, X>,            \
+   (const void*)k_mix_Y2<S, N, X>,                                                                \
    (const void*)k_mix_start_imposed<S, N, X>, (const void*)k_mix_null_p1<N, X>,                   \
    (const void*)k_mix_null_p2<N, X>, (const void*)k_mix_null_p3<S, N, X>}
 #define GNX_MIX_EARLY(S, N, X)                                                                     \
-  {N, X, false, true, (const void*)k_mix_Xe<S, N, X>, (const void*)k_mix_Ye<S, N, X>,             \
+  {N, X, false, true, (const void*)k_mix_Xe<S, N, X, false>, (const void*)k_mix_Xe<S, N, X, true>, \
+   (const void*)k_mix_Ye<S, N, X>,                                                                \
    (const void*)k_mix_start_imposed<S, N, X>, (const void*)k_mix_null_p1<N, X>,                   \
    (const void*)k_mix_null_p2<N, X>, (const void*)k_mix_null_p3<S, N, X>}
 // shapes 0..5: one system per block; 6..9: two (batches of two or more);
@@ -1041,12 +1092,13 @@ static hipError_t mix_graph(MixMachine* m, const Geom& g, int nrhs, int shape, i
   double* o0 = mslot(a, 0, 0);
   double* o1 = mslot(a, 1, 0);
   auto body = [&](hipStream_t cs) {
+    static_assert(ENG_CHUNK % 2 == 0, "the direction buffers alternate by slot parity");
     for (int i = 0; i < ENG_CHUNK; i++) {
-      int odd = i & 1;
+      int sl = i & 1;  // slot parity
       // X: state[0] -> state[1], reduces slot 1, writes slot 0; Y the reverse
-      void* argsX[] = {(void*)&a, (void*)&odd, (void*)&s0, (void*)&w1, (void*)&q1, (void*)&o0};
-      void* argsY[] = {(void*)&a, (void*)&odd, (void*)&s1, (void*)&w0, (void*)&q0, (void*)&o1};
-      (void)hipLaunchKernel(sh.X, grid, blk, argsX, 0, cs);
+      void* argsX[] = {(void*)&a, (void*)&sl, (void*)&s0, (void*)&w1, (void*)&q1, (void*)&o0};
+      void* argsY[] = {(void*)&a, (void*)&sl, (void*)&s1, (void*)&w0, (void*)&q0, (void*)&o1};
+      (void)hipLaunchKernel((i & 1) ? sh.X : sh.Xf, grid, blk, argsX, 0, cs);
       (void)hipLaunchKernel(sh.Y, grid, blk, argsY, 0, cs);
     }
   };

@@ -161,6 +161,38 @@ GNX_DEV T2 apply_Mdag_fast_vals(const Geom& g, const Site& s, typename CT<T2>::R
   return acc;
 }
 
+// FP32 forms for the reduced-precision inner solvers.  Each hop term
+// +-0.5 v is exact, so a fused multiply-add rounds like the separate product
+// and sum, in one instruction; the FP64 forms above keep product and sum
+// apart (they are built without contraction, for the bit-exact operators).
+GNX_DEV float2 cfma(float a, float2 x, float2 y) { return make_float2(fmaf(a, x.x, y.x), fmaf(a, x.y, y.y)); }
+
+template <>
+GNX_DEV float2 apply_M_vals<float2>(const Geom& g, const Site& s, float sm, float2 v0, const float2 vpp[3],
+                                    const float2 vmm[3]) {
+  const HalfEta e = half_eta(g, s);
+  float2 acc = rmul(sm, v0);
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    acc = cfma((float)e.pp[d], vpp[d], acc);
+    acc = cfma(-(float)e.mm[d], vmm[d], acc);
+  }
+  return acc;
+}
+
+template <>
+GNX_DEV float2 apply_Mdag_fast_vals<float2>(const Geom& g, const Site& s, float sm, float2 v0, const float2 vpp[3],
+                                            const float2 vmm[3]) {
+  const HalfEta e = half_eta(g, s);
+  float2 acc = rmul(sm, v0);
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    acc = cfma(-(float)e.pp[d], vpp[d], acc);
+    acc = cfma((float)e.mm[d], vmm[d], acc);
+  }
+  return acc;
+}
+
 template <typename LoadF, typename T2 = decltype(std::declval<LoadF>()(0))>
 GNX_DEV T2 apply_Mdag(const Geom& g, const Site& s, typename CT<T2>::R sm, LoadF load) {
   NbrF n = neighbours_f(g, s);

@@ -48,10 +48,11 @@ def test_mixed_vs_oracle(P, gnx_mod, prec, L):
 def test_mixed_batch_and_null_start(P, gnx_mod, prec, shape, monkeypatch):
     """Every launch shape: 0..5 one system per block (1/2/4 sites per thread,
     with and without the XCD slab mapping), 6..9 two systems per block,
-    10..15 as 0..5 with the loads-first kernels.  A batch of 3 (the last pair
-    half empty) is bit-identical to 3 single solves of the same site mapping
-    (and the two kernel forms to each other), and each solution meets the
-    FP64 bar."""
+    10..15 as 0..5 with the loads-first kernels (which also defer the
+    correction update to every other slot).  A batch of 3 (the last pair half
+    empty) is bit-identical to 3 single solves with the same steps: a paired
+    batch to the unpaired kernels of its site mapping, the others to their own
+    shape; each solution meets the FP64 bar."""
     banner(f"mixed {prec} shape {shape}: batch of 3 and null-start early exit")
     import torch
     L = 12
@@ -71,8 +72,9 @@ def test_mixed_batch_and_null_start(P, gnx_mod, prec, shape, monkeypatch):
     early = shape >= 10
     assert ctx.cg_mixed_shape(3, p) == {"sites_per_thread": [1, 1, 2, 2, 4, 4, 1, 1, 2, 2][shape % 10],
                                         "xcd_slab": shape % 2 == 1, "pair": pair, "loads_first": early}
-    # single solves with the other kernel form of the same site mapping
-    monkeypatch.setenv("GNX_MIX_SHAPE", str(shape - 6 if pair else (shape - 10 if early else shape + 10)))
+    # single solves: a paired batch against the unpaired kernels of the same
+    # site mapping, the other shapes against themselves
+    monkeypatch.setenv("GNX_MIX_SHAPE", str(shape - 6 if pair else shape))
     for k in range(3):
         x = ctx.vec()
         s1, r1 = ctx.cg(sm, dev(bs[k]), x, 1e-10, start=gnx_mod.START_IMPOSED, prec=p)
@@ -92,3 +94,38 @@ def test_mixed_batch_and_null_start(P, gnx_mod, prec, shape, monkeypatch):
     s3, r3 = ctx.cg(sm, dev(bs[1]), x, 1e-10, start=gnx_mod.START_NULL, prec=p)
     assert s3 == 0 and r3.converged and r3.early_exit == 0
     assert np.array_equal(host(x), host(X[2 * lat.V:4 * lat.V]))
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16"])
+@pytest.mark.parametrize("cap", [5, 6, 13])
+def test_mixed_iteration_cap_folds_pending(P, gnx_mod, prec, cap, monkeypatch):
+    """Iteration cap reached on an odd and on an even slot.  The loads-first
+    kernels keep up to two step lengths pending between their correction
+    flushes; the cap's closing reliable update must fold them into x.  Checked
+    three ways: the returned x agrees with the two-buffer kernels' (which
+    update the correction every slot) to FP32 rounding, the reported residual
+    is the true residual of the returned x (oracle, FP64), and the cap is
+    reported as not converged after exactly `cap` iterations."""
+    banner(f"mixed {prec} iteration cap {cap}")
+    import torch
+    L = 8
+    lat = P.Lattice(L)
+    r = P.Random(77)
+    sigma = r.sigma(lat.V, P.sigma_variance(0.85))
+    smh = lat.sigma_mass(sigma)
+    sm = dev(smh)
+    b = r.z2_complex(lat.V)
+    p = gnx_mod.PREC_F32 if prec == "f32" else gnx_mod.PREC_BF16
+    xs = {}
+    for shape in (0, 10):
+        monkeypatch.setenv("GNX_MIX_SHAPE", str(shape))
+        c = gnx_mod.Context(L)
+        x = c.vec()
+        st, res = c.cg(sm, dev(b), x, 1e-30, max_iter=cap, start=gnx_mod.START_IMPOSED, prec=p)
+        torch.cuda.synchronize()
+        assert st == 3 and not res.converged and res.iterations == cap, (st, res.as_dict())
+        xh = host(x)
+        tr = true_res(lat, smh, b, xh)
+        assert abs(tr - res.residual) <= 1e-9 * np.linalg.norm(b), (tr, res.residual)
+        xs[shape] = xh
+    assert rel(xs[10], xs[0]) < 1e-4, rel(xs[10], xs[0])
