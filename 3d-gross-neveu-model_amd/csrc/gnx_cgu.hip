@@ -302,6 +302,18 @@ struct gnx_multi {
 };
 
 namespace gnx {
+// systems of an uneven solver and their lattices (for the ensemble drivers)
+int multi_count(const gnx_multi* m) { return m ? m->n : 0; }
+bool multi_system(const gnx_multi* m, int k, int* L, int* LT, int* device) {
+  if (!m || k < 0 || k >= m->n) return false;
+  *L = m->sys[k].g.L;
+  *LT = m->sys[k].g.LT;
+  *device = m->device;
+  return true;
+}
+}  // namespace gnx
+
+namespace gnx {
 
 static void cgu_free(gnx_multi* m) {
   void* dev[] = {m->dsys, m->dmax_iter, m->sm, m->b, m->x, m->r, m->p0, m->p1, m->t, m->res};

@@ -5,6 +5,8 @@
 #include <complex>
 #include <string.h>
 
+#include <vector>
+
 #include "gnx_internal.h"
 
 using namespace gnx;
@@ -110,12 +112,11 @@ gnx_status hmc_cg(gnx_hmc* h, const double2* b, double2* x, double tol, long max
   return GNX_OK;
 }
 
-// ComputeSigmaForce (moleculardynamics.c:631-647) on the device state
-gnx_status hmc_force(gnx_hmc* h, double tol, long max_iter, bool* ok) {
+// the force from the solutions X (moleculardynamics.c:601-619): Y = M^dag X
+// per flavour, then the corner sums and the sigma term
+gnx_status hmc_force_post(gnx_hmc* h) {
   gnx_ctx* c = h->c;
   const long V = c->V;
-  gnx_status s = hmc_cg(h, h->phi, h->X, tol, max_iter, GNX_START_NULL, 2, ok);
-  if (s != GNX_OK || !*ok) return s;
   for (int f = 0; f < 2; f++)
     HIP_TRY(launch_dirac(c->g, GNX_OP_MDAG, h->sm, (const double*)(h->X + f * V), (double*)(h->Y + f * V),
                          c->tmp, c->stream));
@@ -123,6 +124,14 @@ gnx_status hmc_force(gnx_hmc* h, double tol, long max_iter, bool* ok) {
                              (const double*)(h->X + V), (const double*)(h->Y + V), h->force, c->stream));
   return GNX_OK;
 }
+
+// ComputeSigmaForce (moleculardynamics.c:631-647) on the device state
+gnx_status hmc_force(gnx_hmc* h, double tol, long max_iter, bool* ok) {
+  gnx_status s = hmc_cg(h, h->phi, h->X, tol, max_iter, GNX_START_NULL, 2, ok);
+  if (s != GNX_OK || !*ok) return s;
+  return hmc_force_post(h);
+}
+
 
 gnx_status hmc_energy(gnx_hmc* h, double* H) {
   gnx_ctx* c = h->c;
@@ -355,6 +364,124 @@ gnx_status gnx_hmc_accept(gnx_hmc* h, int accepted) {
   CHECK_ARG(h, "gnx_hmc_accept: null argument");
   HIP_TRY(hipSetDevice(h->c->device));
   if (!accepted) return restore_sigma(h);
+  return GNX_OK;
+}
+
+// n chains in lockstep (moleculardynamics.c:1234-1434 for each): the steps
+// of gnx_hmc_trajectory, chain by chain on each chain's stream, except that
+// every solve point of the integrator is one uneven-batch solve over all
+// live chains (systems 2k, 2k+1 of m: chain k's two flavours).  A chain
+// whose solve fails leaves the lockstep as gnx_hmc_trajectory would (sigma
+// restored, ok = 0); the others go on.
+gnx_status gnx_hmc_ensemble_trajectory(gnx_multi* m, int n, gnx_hmc* const* hs, const double* const* mom_host,
+                                       const double* const* chi0_host, const double* const* chi1_host,
+                                       double epsilon, int step, double tol_high, double tol_medium,
+                                       long max_iter, gnx_traj_result* res) {
+  CHECK_ARG(m && hs && mom_host && chi0_host && chi1_host && res && n >= 1,
+            "gnx_hmc_ensemble_trajectory: null argument");
+  CHECK_ARG(step >= 2, "gnx_hmc_ensemble_trajectory: the integrator needs at least 2 steps (info.c:66)");
+  CHECK_ARG(multi_count(m) == 2 * n, "gnx_hmc_ensemble_trajectory: the solver must hold two systems per chain");
+  for (int k = 0; k < n; k++) {
+    CHECK_ARG(hs[k] && mom_host[k] && chi0_host[k] && chi1_host[k], "gnx_hmc_ensemble_trajectory: null argument");
+    for (int f = 0; f < 2; f++) {
+      int L = 0, LT = 0, dev = -1;
+      multi_system(m, 2 * k + f, &L, &LT, &dev);
+      CHECK_ARG(L == hs[k]->c->g.L && LT == hs[k]->c->g.LT && dev == hs[k]->c->device,
+                "gnx_hmc_ensemble_trajectory: system 2k, 2k+1 of the solver must have chain k's lattice and device");
+    }
+  }
+  HIP_TRY(hipSetDevice(hs[0]->c->device));
+  const double lam = 0.1931833275037836;  // MAGIC_LAMBDA, macro.h:169
+  std::vector<char> live(n, 1), ok(n, 1);
+  std::vector<double> it0(n), H0(n, 0.0);
+  std::vector<int> calls0(n);
+  gnx_status s;
+  for (int k = 0; k < n; k++) {
+    gnx_hmc* h = hs[k];
+    gnx_ctx* c = h->c;
+    const long V = c->V;
+    memset(&res[k], 0, sizeof(res[k]));
+    it0[k] = h->iterations;
+    calls0[k] = h->calls;
+    if ((s = upload_real(h, mom_host[k], h->mom)) != GNX_OK) return s;
+    const double* chis[2] = {chi0_host[k], chi1_host[k]};
+    for (int f = 0; f < 2; f++) {
+      if ((s = upload_real_as_complex(h, chis[f], h->chi + f * V)) != GNX_OK) return s;
+      HIP_TRY(launch_dirac(c->g, GNX_OP_M, h->sm, (const double*)(h->chi + f * V), (double*)(h->phi + f * V),
+                           c->tmp, c->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(h->sigma_copy, h->sigma, sizeof(double) * V, hipMemcpyDeviceToDevice, c->stream));
+  }
+  // a failed solve: the chain leaves as gnx_hmc_trajectory's TRY_CG does
+  auto solve = [&](double tol, int start) -> gnx_status {
+    gnx_status q = ens_solve(m, n, hs, live, tol, max_iter, start, ok);
+    if (q != GNX_OK) return q;
+    for (int k = 0; k < n; k++)
+      if (live[k] && !ok[k]) {
+        live[k] = 0;
+        if ((q = restore_sigma(hs[k])) != GNX_OK) return q;
+        res[k].ok = 0;
+      }
+    return GNX_OK;
+  };
+  auto force = [&](double tol) -> gnx_status {
+    gnx_status q = solve(tol, GNX_START_NULL);
+    for (int k = 0; k < n && q == GNX_OK; k++)
+      if (live[k]) q = hmc_force_post(hs[k]);
+    return q;
+  };
+  auto pq = [&](double pc, double qc) -> gnx_status {
+    for (int k = 0; k < n; k++) {
+      if (!live[k]) continue;
+      gnx_ctx* c = hs[k]->c;
+      HIP_TRY(launch_evolve_momenta(c->g, hs[k]->mom, hs[k]->force, pc, c->stream));
+      if (qc != 0.0) HIP_TRY(launch_evolve_sigma(c->g, hs[k]->sigma, hs[k]->sm, hs[k]->mom, qc, c->stream));
+    }
+    return GNX_OK;
+  };
+#define ENS_TRY(call)                         \
+  do {                                        \
+    if ((s = (call)) != GNX_OK) return s;     \
+  } while (0)
+  ENS_TRY(solve(tol_high, GNX_START_IMPOSED));
+  for (int k = 0; k < n; k++)
+    if (live[k]) ENS_TRY(hmc_energy(hs[k], &H0[k]));
+  ENS_TRY(force(tol_medium));
+  ENS_TRY(pq(lam * epsilon, epsilon / 2.0));
+  ENS_TRY(force(tol_medium));
+  ENS_TRY(pq((1.0 - (2.0 * lam)) * epsilon, epsilon / 2.0));
+  for (int st = 1; st < step - 1; st++) {
+    ENS_TRY(force(tol_medium));
+    ENS_TRY(pq(2.0 * lam * epsilon, epsilon / 2.0));
+    ENS_TRY(force(tol_medium));
+    ENS_TRY(pq((1.0 - (2.0 * lam)) * epsilon, epsilon / 2.0));
+  }
+  ENS_TRY(force(tol_medium));
+  ENS_TRY(pq(2.0 * lam * epsilon, epsilon / 2.0));
+  ENS_TRY(force(tol_medium));
+  ENS_TRY(pq((1.0 - (2.0 * lam)) * epsilon, epsilon / 2.0));
+  ENS_TRY(force(tol_high));
+  ENS_TRY(pq(lam * epsilon, 0.0));
+  bool all = true;
+  for (int k = 0; k < n; k++) {
+    if (!live[k]) {
+      all = false;
+      continue;
+    }
+    double H1 = 0.0;
+    ENS_TRY(hmc_energy(hs[k], &H1));
+    res[k].H0 = H0[k];
+    res[k].H1 = H1;
+    res[k].dH = H1 - H0[k];
+    res[k].ok = 1;
+    res[k].cg_solves = hs[k]->calls - calls0[k];
+    res[k].cg_iterations = (long)(hs[k]->iterations - it0[k]);
+  }
+#undef ENS_TRY
+  if (!all) {
+    set_err("gnx_hmc_ensemble_trajectory: CG did not converge on a chain (its ok = 0)");
+    return GNX_ERR_NOCONV;
+  }
   return GNX_OK;
 }
 

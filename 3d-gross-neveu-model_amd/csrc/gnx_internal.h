@@ -1,6 +1,8 @@
 // gnx_internal.h -- declarations shared by the gnx translation units.
 #pragma once
 
+#include <vector>
+
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -51,6 +53,10 @@ void cgmix_release(void* machine);
 // tuned launch shape (sites per thread, XCD slab mapping, two systems per
 // block, loads-first kernels); the shape index, -1 if none yet
 int cgmix_shape_of(void* machine, int nrhs, int prec, int* spt, int* xcd, int* pair, int* early);
+
+// uneven-batch solver (gnx_cgu.hip): system count and the lattice of system k
+int multi_count(const gnx_multi* m);
+bool multi_system(const gnx_multi* m, int k, int* L, int* LT, int* device);
 int cgm_shape_of(void* machine, int nrhs, int* spt, int* xcd);
 
 void set_err(const char* fmt, ...);
@@ -103,3 +109,50 @@ hipError_t cg_run_link(gnx_ctx* c, const double* sm, const double2* U, const dou
     }                            \
   } while (0)
 
+namespace gnx {
+
+// One uneven-batch solve of both flavours of every live chain (systems 2k,
+// 2k+1 of m).  A chain that is no longer live has its phi and X cleared, so
+// its two systems leave in the prologue; ok[k] and the chain's CG counters
+// follow the single-chain solve (hmc_cg, lh_cg).  H: gnx_hmc or
+// gnx_link_hmc (fields c, sm, phi, X, iterations, calls).
+template <typename H>
+inline gnx_status ens_solve(gnx_multi* m, int n, H* const* hs, const std::vector<char>& live, double tol,
+                     long max_iter, int start, std::vector<char>& ok) {
+  std::vector<const double*> sms(2 * n), bs(2 * n);
+  std::vector<double*> xs(2 * n);
+  for (int k = 0; k < n; k++) {
+    H* h = hs[k];
+    gnx_ctx* c = h->c;
+    const long V = c->V;
+    if (!live[k]) {
+      HIP_TRY(hipMemsetAsync(h->phi, 0, sizeof(double2) * 2 * V, c->stream));
+      HIP_TRY(hipMemsetAsync(h->X, 0, sizeof(double2) * 2 * V, c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    sms[2 * k] = sms[2 * k + 1] = h->sm;
+    bs[2 * k] = (const double*)h->phi;
+    bs[2 * k + 1] = (const double*)(h->phi + V);
+    xs[2 * k] = (double*)h->X;
+    xs[2 * k + 1] = (double*)(h->X + V);
+  }
+  std::vector<gnx_cg_result> r(2 * n);
+  const gnx_status s = gnx_multi_cg(m, sms.data(), bs.data(), xs.data(), tol, max_iter, (gnx_cg_start)start, r.data());
+  if (s != GNX_OK && s != GNX_ERR_NOCONV) return s;
+  for (int k = 0; k < n; k++) {
+    ok[k] = 0;
+    if (!live[k]) continue;
+    ok[k] = 1;
+    for (int f = 0; f < 2; f++) {
+      const gnx_cg_result& q = r[2 * k + f];
+      if (q.converged && !q.early_exit) {
+        hs[k]->iterations += q.iterations;
+        hs[k]->calls += 1;
+      }
+      if (!q.converged) ok[k] = 0;
+    }
+  }
+  return GNX_OK;
+}
+
+}  // namespace gnx

@@ -245,14 +245,17 @@ gnx_status lh_cg(gnx_link_hmc* h, double tol, long max_iter, bool* ok) {
   return GNX_OK;
 }
 
-// ComputeForce (moleculardynamics.c:203-222)
-gnx_status lh_force(gnx_link_hmc* h, double tol, long max_iter, bool* ok) {
+// ComputeForce (moleculardynamics.c:203-222), in three parts: the gauge
+// force, the solve, and the fermion force from the solutions
+void lh_force_pre(gnx_link_hmc* h) {
+  gnx_ctx* c = h->c;
+  if (h->pure_gauge)
+    hipLaunchKernelGGL(k_gauge_force, dim3(nblk(c->V, BS)), dim3(BS), 0, c->stream, c->g, h->U, h->beta, h->force);
+}
+
+gnx_status lh_force_post(gnx_link_hmc* h) {
   gnx_ctx* c = h->c;
   const long V = c->V;
-  if (h->pure_gauge)
-    hipLaunchKernelGGL(k_gauge_force, dim3(nblk(V, BS)), dim3(BS), 0, c->stream, c->g, h->U, h->beta, h->force);
-  gnx_status s = lh_cg(h, tol, max_iter, ok);
-  if (s != GNX_OK || !*ok) return s;
   for (int f = 0; f < 2; f++) {
     if (h->linked)
       HIP_TRY(link_dirac(c, 1, h->sm, h->U, h->X + f * V, h->Y + f * V));
@@ -265,6 +268,13 @@ gnx_status lh_force(gnx_link_hmc* h, double tol, long max_iter, bool* ok) {
                      (const double2*)(h->Y + V), h->force, h->pure_gauge);
   HIP_TRY(hipGetLastError());
   return GNX_OK;
+}
+
+gnx_status lh_force(gnx_link_hmc* h, double tol, long max_iter, bool* ok) {
+  lh_force_pre(h);
+  gnx_status s = lh_cg(h, tol, max_iter, ok);
+  if (s != GNX_OK || !*ok) return s;
+  return lh_force_post(h);
 }
 
 // ComputeFullEnergyWithPhiWithoutSigma (moleculardynamics.c:268-294)
@@ -562,6 +572,127 @@ gnx_status gnx_link_hmc_trajectory(gnx_link_hmc* h, const double* mom_host, cons
   res->ok = 1;
   res->cg_iterations = h->iterations - it0;
   res->cg_solves = h->calls - calls0;
+  return GNX_OK;
+}
+
+// n chains in lockstep (moleculardynamics.c:419-542 for each), the live
+// operator only: the steps of gnx_link_hmc_trajectory chain by chain, except
+// that every solve point is one uneven-batch solve over all live chains
+// (systems 2k, 2k+1 of m: chain k's two flavours).  A chain whose solve
+// fails leaves as gnx_link_hmc_trajectory would (links restored, ok = 0).
+gnx_status gnx_link_hmc_ensemble_trajectory(gnx_multi* m, int n, gnx_link_hmc* const* hs,
+                                            const double* const* mom_host, const double* const* chi0_host,
+                                            const double* const* chi1_host, double eps, int steps, double tol,
+                                            long max_iter, gnx_traj_result* res) {
+  CHECK_ARG(m && hs && mom_host && chi0_host && chi1_host && res && n >= 1,
+            "gnx_link_hmc_ensemble_trajectory: null argument");
+  CHECK_ARG(steps >= 1, "gnx_link_hmc_ensemble_trajectory: steps must be >= 1");
+  CHECK_ARG(multi_count(m) == 2 * n, "gnx_link_hmc_ensemble_trajectory: the solver must hold two systems per chain");
+  for (int k = 0; k < n; k++) {
+    CHECK_ARG(hs[k] && mom_host[k] && chi0_host[k] && chi1_host[k], "gnx_link_hmc_ensemble_trajectory: null argument");
+    CHECK_ARG(!hs[k]->linked,
+              "gnx_link_hmc_ensemble_trajectory: the shared solve is the live operator (chains created with linked = 0)");
+    for (int f = 0; f < 2; f++) {
+      int L = 0, LT = 0, dev = -1;
+      multi_system(m, 2 * k + f, &L, &LT, &dev);
+      CHECK_ARG(L == hs[k]->c->g.L && LT == hs[k]->c->g.LT && dev == hs[k]->c->device,
+                "gnx_link_hmc_ensemble_trajectory: system 2k, 2k+1 of the solver must have chain k's lattice and device");
+    }
+  }
+  HIP_TRY(hipSetDevice(hs[0]->c->device));
+  std::vector<char> live(n, 1), ok(n, 1);
+  std::vector<double> it0(n), H0(n, 0.0);
+  std::vector<int> calls0(n);
+  gnx_status s;
+  for (int k = 0; k < n; k++) {
+    gnx_link_hmc* h = hs[k];
+    gnx_ctx* c = h->c;
+    const long V = c->V;
+    memset(&res[k], 0, sizeof(res[k]));
+    it0[k] = h->iterations;
+    calls0[k] = h->calls;
+    if ((s = lh_upload(h, mom_host[k], h->mom, 3 * (size_t)V)) != GNX_OK) return s;
+    const double* chis[2] = {chi0_host[k], chi1_host[k]};
+    for (int f = 0; f < 2; f++) {
+      if ((s = lh_upload(h, chis[f], (double*)h->Ulex, (size_t)V)) != GNX_OK) return s;
+      hipLaunchKernelGGL(k_real_to_complex_l, dim3(nblk(V, BS)), dim3(BS), 0, c->stream, (int)V,
+                         (const double*)h->Ulex, h->chi + f * V);
+      HIP_TRY(launch_dirac(c->g, GNX_OP_M, h->sm, (const double*)(h->chi + f * V), (double*)(h->phi + f * V),
+                           c->tmp, c->stream));
+      HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(h->U_copy, h->U, sizeof(double2) * 3 * V, hipMemcpyDeviceToDevice, c->stream));
+  }
+  auto solve = [&]() -> gnx_status {
+    gnx_status q = ens_solve(m, n, hs, live, tol, max_iter, GNX_START_NULL, ok);
+    if (q != GNX_OK) return q;
+    for (int k = 0; k < n; k++)
+      if (live[k] && !ok[k]) {
+        live[k] = 0;
+        gnx_ctx* c = hs[k]->c;
+        HIP_TRY(hipMemcpyAsync(hs[k]->U, hs[k]->U_copy, sizeof(double2) * 3 * c->V, hipMemcpyDeviceToDevice,
+                               c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        res[k].ok = 0;
+        res[k].cg_iterations = hs[k]->iterations - it0[k];
+        res[k].cg_solves = hs[k]->calls - calls0[k];
+      }
+    return GNX_OK;
+  };
+  auto force = [&]() -> gnx_status {
+    for (int k = 0; k < n; k++)
+      if (live[k]) lh_force_pre(hs[k]);
+    gnx_status q = solve();
+    for (int k = 0; k < n && q == GNX_OK; k++)
+      if (live[k]) q = lh_force_post(hs[k]);
+    return q;
+  };
+  auto mom = [&](double coef) {
+    for (int k = 0; k < n; k++)
+      if (live[k]) {
+        const int n3 = (int)(3 * hs[k]->c->V);
+        hipLaunchKernelGGL(k_mom_evolve, dim3(nblk(n3, BS)), dim3(BS), 0, hs[k]->c->stream, n3, hs[k]->mom,
+                           (const double*)hs[k]->force, coef);
+      }
+  };
+  auto links = [&](double coef) {
+    for (int k = 0; k < n; k++)
+      if (live[k]) {
+        const int n3 = (int)(3 * hs[k]->c->V);
+        hipLaunchKernelGGL(k_link_evolve, dim3(nblk(n3, BS)), dim3(BS), 0, hs[k]->c->stream, n3, hs[k]->U,
+                           (const double*)hs[k]->mom, coef);
+      }
+  };
+#define ENS_TRY(call)                     \
+  do {                                    \
+    if ((s = (call)) != GNX_OK) return s; \
+  } while (0)
+  ENS_TRY(solve());
+  for (int k = 0; k < n; k++)
+    if (live[k]) ENS_TRY(lh_energy(hs[k], &H0[k]));
+  ENS_TRY(force());
+  mom(eps / 2.0);
+  for (int st = 0; st < steps - 1; st++) {
+    links(eps);
+    ENS_TRY(force());
+    mom(eps);
+  }
+  links(eps);
+  ENS_TRY(force());
+  mom(eps / 2.0);
+  HIP_TRY(hipGetLastError());
+  for (int k = 0; k < n; k++) {
+    if (!live[k]) continue;
+    double H1 = 0.0;
+    ENS_TRY(lh_energy(hs[k], &H1));
+    res[k].H0 = H0[k];
+    res[k].H1 = H1;
+    res[k].dH = H1 - H0[k];
+    res[k].ok = 1;
+    res[k].cg_iterations = hs[k]->iterations - it0[k];
+    res[k].cg_solves = hs[k]->calls - calls0[k];
+  }
+#undef ENS_TRY
   return GNX_OK;
 }
 

@@ -120,6 +120,12 @@ def lib():
         L.gnx_multi_set_stream.argtypes = [vp, vp]
         L.gnx_multi_cg.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.c_double, C.c_long,
                                    C.c_int, C.POINTER(CGResult)]
+        L.gnx_hmc_ensemble_trajectory.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(dp), C.POINTER(dp),
+                                                  C.POINTER(dp), C.c_double, C.c_int, C.c_double, C.c_double,
+                                                  C.c_long, C.POINTER(TrajResult)]
+        L.gnx_link_hmc_ensemble_trajectory.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(dp), C.POINTER(dp),
+                                                       C.POINTER(dp), C.c_double, C.c_int, C.c_double, C.c_long,
+                                                       C.POINTER(TrajResult)]
         _lib = L
     return _lib
 
@@ -517,3 +523,51 @@ class LinkHMC:
         calls = C.c_int()
         _check(lib().gnx_link_hmc_cg_stats(self.h, C.byref(it), C.byref(calls)))
         return it.value, calls.value
+
+
+class Ensemble:
+    """Chains on lattices of uneven size stepped in lockstep: every solve point
+    of the integrator is one uneven-batch launch per CG kernel over all
+    chains (gnx_hmc_ensemble_trajectory / gnx_link_hmc_ensemble_trajectory).
+    chains: HMC or LinkHMC objects (one kind per ensemble)."""
+
+    def __init__(self, chains, device=0):
+        self.chains = list(chains)
+        sizes = []
+        for c in self.chains:
+            sizes += [(c.ctx.L, c.ctx.LT)] * 2
+        self.multi = Multi(sizes, device)
+
+    def close(self):
+        self.multi.close()
+
+    def _draws(self, arrs):
+        keep = [_dptr(a) for a in arrs]
+        return keep, (C.POINTER(C.c_double) * len(keep))(*[p for _, p in keep])
+
+    def sigma_trajectory(self, moms, chi0s, chi1s, eps, steps, tol_high, tol_medium, max_iter=0):
+        """One gnx_hmc_trajectory per chain, in lockstep; returns (status, [TrajResult])."""
+        n = len(self.chains)
+        k0, m = self._draws(moms)
+        k1, a = self._draws(chi0s)
+        k2, b = self._draws(chi1s)
+        res = (TrajResult * n)()
+        hs = (C.c_void_p * n)(*[c.h for c in self.chains])
+        st = lib().gnx_hmc_ensemble_trajectory(self.multi.h, n, hs, m, a, b, eps, steps, tol_high, tol_medium,
+                                               max_iter, res)
+        if st not in (0, 3):
+            _check(st)
+        return st, list(res)
+
+    def link_trajectory(self, moms, chi0s, chi1s, eps, steps, tol, max_iter=0):
+        """One gnx_link_hmc_trajectory per chain (live operator), in lockstep."""
+        n = len(self.chains)
+        k0, m = self._draws(moms)
+        k1, a = self._draws(chi0s)
+        k2, b = self._draws(chi1s)
+        res = (TrajResult * n)()
+        hs = (C.c_void_p * n)(*[c.h for c in self.chains])
+        st = lib().gnx_link_hmc_ensemble_trajectory(self.multi.h, n, hs, m, a, b, eps, steps, tol, max_iter, res)
+        if st not in (0, 3):
+            _check(st)
+        return st, list(res)

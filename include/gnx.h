@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define GNX_ABI_VERSION 2
+#define GNX_ABI_VERSION 3
 
 typedef struct gnx_ctx gnx_ctx;
 typedef struct gnx_multi gnx_multi;
@@ -240,6 +240,18 @@ gnx_status gnx_hmc_condensate(gnx_hmc *h, const double *z2_host, double tol, lon
 gnx_status gnx_hmc_condensate_sigma(gnx_hmc *h, double *out);
 /* accumulated CG statistics (kit->iterations, kit->CG_call) */
 gnx_status gnx_hmc_cg_stats(gnx_hmc *h, double *iterations, int *calls);
+/* n independent chains (configuration 5: lattices of uneven size) in
+ * lockstep: gnx_hmc_trajectory for every chain, with each solve point of the
+ * integrator one launch per CG kernel over all live chains.  m: an uneven
+ * solver (gnx_multi_create) holding 2n systems, systems 2k and 2k+1 on chain
+ * k's lattice (its two flavours).  Per-chain arguments and results as
+ * gnx_hmc_trajectory; a chain whose solve fails gets res[k].ok = 0 and its
+ * sigma restored while the others go on (status GNX_ERR_NOCONV).  No
+ * counterpart in the reference, which runs one chain per process. */
+gnx_status gnx_hmc_ensemble_trajectory(gnx_multi *m, int n, gnx_hmc *const *hs, const double *const *mom_host,
+                                       const double *const *chi0_host, const double *const *chi1_host,
+                                       double eps, int steps, double tol_high, double tol_medium,
+                                       long max_iter, gnx_traj_result *res);
 
 /* ------------------------------------------------------------------------
  * A11: the U(1) link sector.  The reference's live operator (dirac.c)
@@ -298,6 +310,14 @@ gnx_status gnx_link_hmc_accept(gnx_link_hmc *h, int accepted);
 /* ComputeFullEnergyWithPhiWithoutSigma (moleculardynamics.c:268-294) on the current fields */
 gnx_status gnx_link_hmc_energy(gnx_link_hmc *h, double *H);
 gnx_status gnx_link_hmc_cg_stats(gnx_link_hmc *h, double *iterations, int *calls);
+/* n chains' LeapFrog2Link trajectories in lockstep, each solve point one
+ * launch per CG kernel over all live chains (as gnx_hmc_ensemble_trajectory;
+ * chains created with linked = 0, the live operator).  No counterpart in the
+ * reference. */
+gnx_status gnx_link_hmc_ensemble_trajectory(gnx_multi *m, int n, gnx_link_hmc *const *hs,
+                                            const double *const *mom_host, const double *const *chi0_host,
+                                            const double *const *chi1_host, double eps, int steps, double tol,
+                                            long max_iter, gnx_traj_result *res);
 
 #ifdef __cplusplus
 }
