@@ -218,24 +218,30 @@ def ensemble_leg(gnx, sizes, ntraj, betas=(0.6, 0.75, 0.85, 1.0), eps=0.02, step
     Metropolis test.  Links start uniform in [0, 2 pi), sigma iid
     N(0, 2/(4 beta_s)).  The link trajectory follows the reference as
     written: solves with the live operator, force from the links
-    (moleculardynamics.c:419-542).  Every chain has its own context and
-    stream and is driven by its own host thread, so kernels of different
-    chains run concurrently on the GPU."""
+    (moleculardynamics.c:419-542).  Run twice on identical chains and draws:
+    every chain with its own context, stream and host thread (kernels of
+    different chains run concurrently on the GPU), then all chains in
+    lockstep (gnx.Ensemble: each solve point of an integrator one
+    uneven-batch launch over all chains)."""
     import concurrent.futures as cf
-    chains = []
-    for L in sizes:
-        for bs in betas:
-            V = L ** 3
-            seed = 5000 + 10 * L + int(bs * 100)
-            rng = np.random.default_rng(seed)
-            ctx = gnx.Context(L)
-            h = gnx.HMC(ctx, beta_s=bs)
-            h.set_sigma(rng.normal(0.0, np.sqrt(2.0 / (4 * bs)), V))
-            lh = gnx.LinkHMC(ctx, beta=1.0, pure_gauge=True, linked=False)
-            _, sm = h.get_sigma()
-            lh.set_fields(np.exp(1j * rng.uniform(0.0, 2 * np.pi, 3 * V)), sm)
-            chains.append({"L": L, "beta_s": bs, "V": V, "ctx": ctx, "h": h, "lh": lh, "rng": rng,
-                           "dH_sigma": [], "dH_link": [], "acc_sigma": 0, "acc_link": 0, "cg_iterations": 0})
+    def make_chains():
+        out = []
+        for L in sizes:
+            for bs in betas:
+                V = L ** 3
+                seed = 5000 + 10 * L + int(bs * 100)
+                rng = np.random.default_rng(seed)
+                ctx = gnx.Context(L)
+                h = gnx.HMC(ctx, beta_s=bs)
+                h.set_sigma(rng.normal(0.0, np.sqrt(2.0 / (4 * bs)), V))
+                lh = gnx.LinkHMC(ctx, beta=1.0, pure_gauge=True, linked=False)
+                _, sm = h.get_sigma()
+                lh.set_fields(np.exp(1j * rng.uniform(0.0, 2 * np.pi, 3 * V)), sm)
+                out.append({"L": L, "beta_s": bs, "V": V, "ctx": ctx, "h": h, "lh": lh, "rng": rng,
+                            "dH_sigma": [], "dH_link": [], "acc_sigma": 0, "acc_link": 0, "cg_iterations": 0})
+        return out
+
+    chains = make_chains()
 
     def run(c, n):
         V, rng = c["V"], c["rng"]
@@ -269,11 +275,65 @@ def ensemble_leg(gnx, sizes, ntraj, betas=(0.6, 0.75, 0.85, 1.0), eps=0.02, step
     dt = time.perf_counter() - t0
     out = {"workload": f"{len(chains)} chains on {sizes}^3 x beta_s {list(betas)}, links on (beta 1), "
                        f"{ntraj} sigma + link trajectories each, {steps} steps of {eps}",
+           "mode": "one context, stream and host thread per chain",
            "trajectories_per_s": len(chains) * ntraj / dt, "seconds": dt,
            "chains": [{"L": c["L"], "beta_s": c["beta_s"], "dH_sigma": c["dH_sigma"], "dH_link": c["dH_link"],
                        "accepted_sigma": c["acc_sigma"], "accepted_link": c["acc_link"],
                        "cg_iterations": c["cg_iterations"]} for c in chains]}
-    for c in chains:
+
+    # the same chains and draws in lockstep: every solve point of both
+    # integrators one uneven-batch launch over all chains
+    lock = make_chains()
+    es = gnx.Ensemble([c["h"] for c in lock])
+    el = gnx.Ensemble([c["lh"] for c in lock])
+
+    def run_lockstep(n):
+        for _ in range(n):
+            dr = [(c["rng"].standard_normal(c["V"]), c["rng"].standard_normal(c["V"]),
+                   c["rng"].standard_normal(c["V"])) for c in lock]
+            _, ts = es.sigma_trajectory([d[0] for d in dr], [d[1] for d in dr], [d[2] for d in dr],
+                                        eps, steps, 1e-10, 1e-3)
+            for c, t in zip(lock, ts):
+                a = bool(t.ok) and c["rng"].random() <= np.exp(-t.dH)
+                c["h"].accept(a)
+                _, sm = c["h"].get_sigma()
+                c["lh"].set_fields(None, sm)
+                c["dH_sigma"].append(t.dH)
+                c["acc_sigma"] += int(a)
+                c["cg_iterations"] += t.cg_iterations
+            dl = [(c["rng"].standard_normal(3 * c["V"]), c["rng"].standard_normal(c["V"]),
+                   c["rng"].standard_normal(c["V"])) for c in lock]
+            _, tls = el.link_trajectory([d[0] for d in dl], [d[1] for d in dl], [d[2] for d in dl],
+                                        eps, steps, 1e-10)
+            for c, tl in zip(lock, tls):
+                al = bool(tl.ok) and c["rng"].random() <= np.exp(-tl.dH)
+                c["lh"].accept(al)
+                c["dH_link"].append(tl.dH)
+                c["acc_link"] += int(al)
+                c["cg_iterations"] += tl.cg_iterations
+
+    run_lockstep(1)  # warm-up, as above
+    for c in lock:
+        c["dH_sigma"].clear(), c["dH_link"].clear()
+        c["acc_sigma"] = c["acc_link"] = c["cg_iterations"] = 0
+    t0 = time.perf_counter()
+    run_lockstep(ntraj)
+    dtl = time.perf_counter() - t0
+    # the two runs tune their own CG launch shapes, so rounding differs; a
+    # force solve to 1e-3 can then stop one iteration apart, which moves its
+    # solution by up to the tolerance
+    ds = [abs(a - b) for c, d in zip(chains, lock) for a, b in zip(c["dH_sigma"], d["dH_sigma"])]
+    dl = [abs(a - b) / max(1.0, abs(a)) for c, d in zip(chains, lock) for a, b in zip(c["dH_link"], d["dH_link"])]
+    out["lockstep"] = {"mode": "all chains in lockstep, one uneven-batch solve per integrator solve point",
+                       "trajectories_per_s": len(lock) * ntraj / dtl, "seconds": dtl,
+                       "speedup_vs_thread_per_chain": dt / dtl,
+                       "max_abs_dH_sigma_diff_vs_thread_per_chain": max(ds),
+                       "max_rel_dH_link_diff_vs_thread_per_chain": max(dl),
+                       "same_accept_decisions": all(c["acc_sigma"] == d["acc_sigma"] and c["acc_link"] == d["acc_link"]
+                                                    for c, d in zip(chains, lock))}
+    es.close()
+    el.close()
+    for c in chains + lock:
         c["lh"].close()
         c["h"].close()
     return out
