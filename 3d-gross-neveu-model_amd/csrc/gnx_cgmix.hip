@@ -20,6 +20,8 @@
 //      p = r + beta p, t = M^dag p.
 //   Y: finish an update (x += xs, xs = 0, r = (S)(b - M u), partial |r|^2 in
 //      FP64) or the normal step (xs += alpha p, r -= alpha M t, partial |r|^2).
+// The loads-first kernels (k_mix_Xe / k_mix_Ye) move the xs update out of Y:
+// it is folded into the X kernel of every even slot and into the update.
 // As in the FP64 solver, each kernel issues its loads (state and partials,
 // then the operands) before its first wait, and kernel arguments are passed
 // by value.
@@ -1017,7 +1019,7 @@ struct MixShape {
   bool pair;   // two systems per block (k_mix_X2 / k_mix_Y2)
   bool early;  // loads-first kernels (k_mix_Xe / k_mix_Ye)
   const void* X;   // X kernel of odd slots
-  const void* Xf;  // X kernel of even slots (the loads-first form's xs flush)
+  const void* Xf;  // X kernel of even slots (differs only in the loads-first form: the xs flush)
   const void* Y;
   const void* start_imposed;
   const void* null_p1;
