@@ -182,7 +182,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_null_p3(CGMFix a, const double2*
 // sin/sout: this kernel's input and output state halves; pin/pout: the slot
 // it reduces and the slot it writes (separate restrict pointers, so the
 // compiler may keep the reads ahead of the writes)
-template <int SPT, bool XCD, bool LINK>
+template <int SPT, bool XCD, bool LINK, bool WH>
 __global__ __launch_bounds__(CGM_BS) void k_cgm_A(CGMFix a, int odd, const CGMState* __restrict__ sin,
                                                    CGMState* __restrict__ sout, const double* __restrict__ pin,
                                                    double* __restrict__ pout) {
@@ -199,8 +199,8 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_A(CGMFix a, int odd, const CGMSt
   // operands; the reduction waits for the first group only (loads return in
   // issue order) and runs while the operands are in flight
   const CGMState s = load_state(sin + rhs);
-  double pl[PART_K];
-  partial_issue(pin + (size_t)rhs * a.nb, a.nb, pl);
+  double pl[HEAD_W];
+  head_issue<WH>(pin + (size_t)rhs * a.nb, a.nb, pl);
   end_of_loads();
   int fs[SPT];
   double smv[SPT];
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_A(CGMFix a, int odd, const CGMSt
     }
   }
   end_of_loads();
-  const double rho = block_sum_once(partial_finish(pin + (size_t)rhs * a.nb, a.nb, pl), lds);
+  const double rho = head_finish<WH>(pin + (size_t)rhs * a.nb, a.nb, pl, lds);
   CGMState* out = sout + rhs;
   if (!s.active) {
     if (lead) put_state(out, 0, s.iters, s.converged, s.early, s.rho, s);
@@ -278,7 +278,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_A(CGMFix a, int odd, const CGMSt
   if (lead) put_state(out, 1, s.iters, 0, 0, rho, s);
 }
 
-template <int SPT, bool XCD, bool LINK>
+template <int SPT, bool XCD, bool LINK, bool WH>
 __global__ __launch_bounds__(CGM_BS) void k_cgm_B(CGMFix a, int odd, const CGMState* __restrict__ sin,
                                                    CGMState* __restrict__ sout, const double* __restrict__ pin,
                                                    double* __restrict__ pout) {
@@ -294,8 +294,8 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_B(CGMFix a, int odd, const CGMSt
   const SiteMap<SPT, XCD> map(a.nb);
   // loads in the order they are needed (as in A)
   const CGMState s = load_state(sin + rhs);
-  double pl[PART_K];
-  partial_issue(pin + (size_t)rhs * a.nb, a.nb, pl);
+  double pl[HEAD_W];
+  head_issue<WH>(pin + (size_t)rhs * a.nb, a.nb, pl);
   end_of_loads();
   int fs[SPT];
   double smv[SPT];
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_B(CGMFix a, int odd, const CGMSt
     rv[i] = *elem(r, fc);
   }
   end_of_loads();
-  const double gamma = block_sum_once(partial_finish(pin + (size_t)rhs * a.nb, a.nb, pl), lds);
+  const double gamma = head_finish<WH>(pin + (size_t)rhs * a.nb, a.nb, pl, lds);
   CGMState* out = sout + rhs;
   if (!s.active) {
     if (lead) put_state(out, 0, s.iters, s.converged, s.early, s.rho, s);
@@ -349,8 +349,10 @@ struct CGMShape {
   int spt;
   bool xcd;
   bool link;
-  const void* A;
+  const void* A;   // block-wide head reduction
   const void* B;
+  const void* Aw;  // per-wave head reduction (up to 512 partials)
+  const void* Bw;
   const void* start_imposed;
   const void* null_p1;
   const void* null_p2;
@@ -358,7 +360,8 @@ struct CGMShape {
 };
 
 #define GNX_SHAPE(S, X, K)                                                                                  \
-  {S, X, K, (const void*)k_cgm_A<S, X, K>, (const void*)k_cgm_B<S, X, K>, (const void*)k_cgm_start_imposed<S, X>, \
+  {S, X, K, (const void*)k_cgm_A<S, X, K, false>, (const void*)k_cgm_B<S, X, K, false>,                      \
+   (const void*)k_cgm_A<S, X, K, true>, (const void*)k_cgm_B<S, X, K, true>, (const void*)k_cgm_start_imposed<S, X>, \
    (const void*)k_cgm_null_p1<S, X, K>, (const void*)k_cgm_null_p2<S, X, K>, (const void*)k_cgm_null_p3<S, X>}
 // shapes 0..5: the live operator; 6..11: the link-coupled operator
 static const CGMShape kShapes[] = {
@@ -476,8 +479,9 @@ static hipError_t cgm_graph(CGMachine* m, const Geom& g, int nrhs, int shape, hi
       // A: state[0] -> state[1], reduces slot 1, writes slot 0; B the reverse
       void* argsA[] = {(void*)&a, (void*)&odd, (void*)&s0, (void*)&w1, (void*)&q1, (void*)&o0};
       void* argsB[] = {(void*)&a, (void*)&odd, (void*)&s1, (void*)&w0, (void*)&q0, (void*)&o1};
-      (void)hipLaunchKernel(kShapes[shape].A, grid, blk, argsA, 0, cs);
-      (void)hipLaunchKernel(kShapes[shape].B, grid, blk, argsB, 0, cs);
+      const bool wh = head_wave(a.nb);
+      (void)hipLaunchKernel(wh ? kShapes[shape].Aw : kShapes[shape].A, grid, blk, argsA, 0, cs);
+      (void)hipLaunchKernel(wh ? kShapes[shape].Bw : kShapes[shape].B, grid, blk, argsB, 0, cs);
     }
   };
   return m->eng.graph(shape, nrhs, body, exec);

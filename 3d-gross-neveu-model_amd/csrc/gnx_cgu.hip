@@ -152,6 +152,7 @@ __global__ __launch_bounds__(CGU_BS) void k_cgu_null_p3(CGUFix a, double tol, co
 // ---- iteration kernels (the steps of k_cgm_A / k_cgm_B) -------------------------
 // A: rho from the previous partials, exit test, beta; p_new = r + beta p_old at
 // the site and its neighbours; t = M^dag p_new; partial |t|^2
+template <bool WH>
 __global__ __launch_bounds__(CGU_BS) void k_cgu_A(CGUFix a, int odd, const CGMState* __restrict__ sin,
                                                   CGMState* __restrict__ sout, const double* __restrict__ pin,
                                                   double* __restrict__ pout) {
@@ -170,8 +171,8 @@ __global__ __launch_bounds__(CGU_BS) void k_cgu_A(CGUFix a, int odd, const CGMSt
   const double2* r = a.r + d.voff;
   const double2* pold = (odd ? a.p1 : a.p0) + d.voff;
   double2* pnew = (odd ? a.p0 : a.p1) + d.voff;
-  double pl[PART_K];
-  partial_issue(pin + d.poff, d.nb, pl);
+  double pl[HEAD_W];
+  head_issue<WH>(pin + d.poff, d.nb, pl);
   end_of_loads();
   const int f = cgu_site(d);
   const int fc = f < 0 ? 0 : f;  // past the last site: load site 0, store nothing
@@ -185,7 +186,7 @@ __global__ __launch_bounds__(CGU_BS) void k_cgu_A(CGUFix a, int odd, const CGMSt
     pv[j] = *elem(pold, q[j]);
   }
   end_of_loads();
-  const double rho = block_sum_once(partial_finish(pin + d.poff, d.nb, pl), lds);
+  const double rho = head_finish<WH>(pin + d.poff, d.nb, pl, lds);
   double beta = 0.0;
   if (s.iters == 0) {
     if (s.start == GNX_START_IMPOSED && sqrt(rho) < s.tol) {  // conjugategradient.c:123-128
@@ -230,6 +231,7 @@ __global__ __launch_bounds__(CGU_BS) void k_cgu_A(CGUFix a, int odd, const CGMSt
 
 // B: gamma from A's partials; alpha = rho / gamma; x += alpha p_new;
 // r -= alpha M t; partial |r|^2
+template <bool WH>
 __global__ __launch_bounds__(CGU_BS) void k_cgu_B(CGUFix a, int odd, const CGMState* __restrict__ sin,
                                                   CGMState* __restrict__ sout, const double* __restrict__ pin,
                                                   double* __restrict__ pout) {
@@ -249,8 +251,8 @@ __global__ __launch_bounds__(CGU_BS) void k_cgu_B(CGUFix a, int odd, const CGMSt
   const double2* t = a.t + d.voff;
   double2* r = a.r + d.voff;
   double2* x = a.x + d.voff;
-  double pl[PART_K];
-  partial_issue(pin + d.poff, d.nb, pl);
+  double pl[HEAD_W];
+  head_issue<WH>(pin + d.poff, d.nb, pl);
   end_of_loads();
   const int f = cgu_site(d);
   const int fc = f < 0 ? 0 : f;
@@ -264,7 +266,7 @@ __global__ __launch_bounds__(CGU_BS) void k_cgu_B(CGUFix a, int odd, const CGMSt
   const double2 xv = *elem(x, fc);
   const double2 rv = *elem(r, fc);
   end_of_loads();
-  const double gamma = block_sum_once(partial_finish(pin + d.poff, d.nb, pl), lds);
+  const double gamma = head_finish<WH>(pin + d.poff, d.nb, pl, lds);
   const double al = s.rho / gamma;
   double acc = 0.0;
   if (f >= 0) {
@@ -359,8 +361,11 @@ static hipError_t cgu_graph(gnx_multi* m, hipGraphExec_t* exec) {
       // A: state[0] -> state[1], reduces slot 1, writes slot 0; B the reverse
       void* argsA[] = {(void*)&a, (void*)&odd, (void*)&s0, (void*)&w1, (void*)&q1, (void*)&o0};
       void* argsB[] = {(void*)&a, (void*)&odd, (void*)&s1, (void*)&w0, (void*)&q0, (void*)&o1};
-      (void)hipLaunchKernel((const void*)k_cgu_A, grid, blk, argsA, 0, cs);
-      (void)hipLaunchKernel((const void*)k_cgu_B, grid, blk, argsB, 0, cs);
+      // the per-wave head when every system has at most 512 partials (the
+      // form a single-lattice solve of each system then also uses)
+      const bool wh = head_wave(m->nbmax);
+      (void)hipLaunchKernel(wh ? (const void*)k_cgu_A<true> : (const void*)k_cgu_A<false>, grid, blk, argsA, 0, cs);
+      (void)hipLaunchKernel(wh ? (const void*)k_cgu_B<true> : (const void*)k_cgu_B<false>, grid, blk, argsB, 0, cs);
     }
   };
   return m->eng.graph(0, n, body, exec);

@@ -115,6 +115,45 @@ GNX_DEVONLY double partial_share(const double* __restrict__ src, int n) {
   return partial_finish(src, n, v);
 }
 
+// Head reduction of the iteration kernels, in two forms chosen per launch
+// (a template argument: a branch between them would join two load paths,
+// and the compiler then waits for every load at the join, before the
+// operand loads).  WH (up to HEAD_W * 64 = 512 partials): every wave loads
+// all of them itself (lane l: l, l + 64, ..., clamped, branch-free) and sums
+// them with DPP -- no LDS and no barrier, the same order in every wave of
+// every block; a block-wide sum of those partials costs a barrier that holds
+// every wave to the slowest (11.34 against 11.17 us per FP64 iteration at
+// 64^3, 11.74 against 11.11 with 4 sites per thread).  Otherwise the
+// block-wide form above (8 loads per lane would leave a second round of
+// loads behind the first wait).
+constexpr int HEAD_W = 8;
+GNX_DEV bool head_wave(int n) { return n <= HEAD_W * 64; }
+template <bool WH>
+GNX_DEVONLY void head_issue(const double* __restrict__ src, int n, double v[HEAD_W]) {
+  if constexpr (WH) {
+    const int lane = (int)(threadIdx.x & 63);
+#pragma unroll
+    for (int k = 0; k < HEAD_W; k++) {
+      const int i = k * 64 + lane;
+      v[k] = src[i < n ? i : n - 1];
+    }
+  } else {
+    partial_issue(src, n, v);
+  }
+}
+template <bool WH>
+GNX_DEVONLY double head_finish(const double* __restrict__ src, int n, const double v[HEAD_W], double* lds) {
+  if constexpr (WH) {
+    const int lane = (int)(threadIdx.x & 63);
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < HEAD_W; k++) acc += (k * 64 + lane < n) ? v[k] : 0.0;
+    return wave_sum(acc);
+  } else {
+    return block_sum_once(partial_finish(src, n, v), lds);
+  }
+}
+
 // Marks the end of a kernel's load phase: the scheduler moves no
 // instruction across it, so no arithmetic on a loaded value (and no wait for
 // one) lands between the loads.

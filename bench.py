@@ -110,7 +110,9 @@ def summarize_rocprof(kt_dir, fetch_dir, write_dir, L):
         wk = statistics.median(counters["WRITE_SIZE"][0][n])
         mb = (fk * scale["FETCH_SIZE"] + wk * scale["WRITE_SIZE"]) * 1024 / 1e6
         out.append(f"| `{n}` | {len(counters['FETCH_SIZE'][0][n])} | {fk:.0f} | {wk:.0f} | {mb:.2f} |")
-        tag = n[n.index("<"):]
+        # <sites per thread, XCD slab, link-coupled, per-wave head>: the head
+        # form follows from the block count, so the launch shape is the key
+        tag = "<" + ", ".join(n[n.index("<") + 1:n.rindex(">")].split(", ")[:3]) + ">"
         bpi[tag] = bpi.get(tag, 0.0) + mb * 1e6
     alg = BYTES_CG_ITER_F64 * L ** 3
     out.append("")
