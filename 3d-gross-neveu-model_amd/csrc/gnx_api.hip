@@ -173,13 +173,13 @@ gnx_status gnx_cg_batch(gnx_ctx* c, const double* sm, const double* b, double* x
   CHECK_ARG((const void*)b != (const void*)x, "gnx_cg: b and x alias");
   if (max_iter <= 0) max_iter = c->V + 1;
   HIP_TRY(hipSetDevice(c->device));
-  if (prec == GNX_PREC_F32 || prec == GNX_PREC_BF16) {
+  if (prec == GNX_PREC_F32 || prec == GNX_PREC_BF16 || prec == GNX_PREC_F16) {
     CGDevResult hr[64];
     // default drop factors measured at 64^3 x 32 sources, tol 1e-10: FP32
     // needs the same iterations for any delta, so few updates (0.05); BF16
     // loses direction quality fast and needs frequent true-residual
     // refreshes (0.2: 447 iterations, 19 updates, against 549 at 0.1; the
-    // rate peaks at 0.15-0.2)
+    // rate peaks at 0.15-0.2); FP16, three mantissa bits more, as FP32
     const double delta = c->reliable_delta > 0.0 ? c->reliable_delta : (prec == GNX_PREC_BF16 ? 0.2 : 0.05);
     HIP_TRY(cgmix_run(&c->cgmix, c->g, sm, (const double2*)b, (double2*)x, tol, delta, max_iter,
                       (int)start, (int)prec, nrhs, c->stream, hr));

@@ -3,7 +3,7 @@
 // lattice Dirac solvers).
 //
 // Inner iterations keep r, p, t in reduced-precision storage (FP32, or BF16
-// with FP32 arithmetic) and the solution correction xs in FP32; every global
+// or FP16 with FP32 arithmetic) and the solution correction xs in FP32; every global
 // sum is accumulated in FP64.  When the iterated residual has dropped by a
 // factor `delta` since the last update, claims convergence or reaches the
 // iteration cap, a reliable update folds xs into the FP64 solution and
@@ -37,6 +37,7 @@ namespace gnx {
 // ---- storage formats --------------------------------------------------------
 struct StoreF32 {
   using T = float2;
+  static constexpr bool kScaled = false;
   GNX_DEVONLY static float2 ld(const T* p, int i) { return *elem(p, i); }
   GNX_DEVONLY static void st(T* p, int i, float2 v) { st_nt(elem(p, i), v); }
 };
@@ -47,6 +48,7 @@ typedef __bf16 gnx_bf16x2 __attribute__((ext_vector_type(2)));
 typedef float gnx_f32x2 __attribute__((ext_vector_type(2)));
 struct StoreBF16 {
   using T = unsigned int;
+  static constexpr bool kScaled = false;
   GNX_DEVONLY static float up(unsigned int h) { return __uint_as_float(h << 16); }
   GNX_DEVONLY static float2 ld(const T* p, int i) {
     unsigned int v = *elem(p, i);
@@ -55,6 +57,26 @@ struct StoreBF16 {
   GNX_DEVONLY static void st(T* p, int i, float2 v) {
     const gnx_f32x2 w = {v.x, v.y};
     st_nt(elem(p, i), __builtin_bit_cast(unsigned int, __builtin_convertvector(w, gnx_bf16x2)));
+  }
+};
+
+// IEEE half pair (re in the low half), round-to-nearest-even on store (one
+// v_cvt_pk_f16_f32).  Three more mantissa bits than BF16 in the same four
+// bytes, but a range of 6e-5 .. 65504, so r, p, t are stored times a
+// per-system scale s = sqrt(V) / ||r|| set at every reliable update (the
+// first slot of a solve is one): elements of r are then O(1).  Only the
+// loads-first kernels implement the scale.
+typedef _Float16 gnx_f16x2 __attribute__((ext_vector_type(2)));
+struct StoreF16 {
+  using T = unsigned int;
+  static constexpr bool kScaled = true;
+  GNX_DEVONLY static float2 ld(const T* p, int i) {
+    const gnx_f32x2 w = __builtin_convertvector(__builtin_bit_cast(gnx_f16x2, *elem(p, i)), gnx_f32x2);
+    return make_float2(w.x, w.y);
+  }
+  GNX_DEVONLY static void st(T* p, int i, float2 v) {
+    const gnx_f32x2 w = {v.x, v.y};
+    st_nt(elem(p, i), __builtin_bit_cast(unsigned int, __builtin_convertvector(w, gnx_f16x2)));
   }
 };
 
@@ -80,6 +102,10 @@ struct MixState {
   // (alpha_a for the newest direction, alpha_b for the one before)
   double alpha_a;
   double alpha_b;
+  // scaled storage (FP16): r, p, t hold s times their values; an update's
+  // X kernel sets scale_next for its Y kernel, which rescales to it
+  double scale;
+  double scale_next;
 };
 
 GNX_DEVONLY void put_mstate(MixState* o, int active, int iters, int phase, int nrel, int conv, int early,
@@ -99,6 +125,8 @@ GNX_DEVONLY void put_mstate(MixState* o, int active, int iters, int phase, int n
   o->max_iter = cfg.max_iter;
   o->alpha_a = cfg.alpha_a;
   o->alpha_b = cfg.alpha_b;
+  o->scale = cfg.scale;
+  o->scale_next = cfg.scale_next;
 }
 GNX_DEVONLY void put_alphas(MixState* o, double alpha_a, double alpha_b) {
   o->alpha_a = alpha_a;
@@ -156,7 +184,9 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_start_imposed(MixFix a, const do
     double2 bb = b_in[off + f];
     a.b[off + f] = bb;
     a.x[off + f] = make_double2(0.0, 0.0);
-    float2 bs = make_float2((float)bb.x, (float)bb.y);
+    // a scaled format stores r = s b in the first slot (a reliable update),
+    // once ||b|| is known; zeros here keep every loaded value finite
+    float2 bs = S::kScaled ? make_float2(0.f, 0.f) : make_float2((float)bb.x, (float)bb.y);
     S::st((typename S::T*)a.r + off, f, bs);
     S::st((typename S::T*)a.p0 + off, f, bs);
     S::st((typename S::T*)a.p1 + off, f, make_float2(0.f, 0.f));
@@ -171,6 +201,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_start_imposed(MixFix a, const do
   if (threadIdx.x == 0) mslot(a, 1, rhs)[partial_slot(blockIdx.x, a.nb)] = acc;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     MixState cfg = {};
+    cfg.scale = cfg.scale_next = 1.0;
     cfg.start = GNX_START_IMPOSED;
     cfg.tol = tol;
     cfg.delta = delta;
@@ -236,6 +267,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_null_p3(MixFix a, double tol, do
   const bool early = sqrt(nr) < tol;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     MixState cfg = {};
+    cfg.scale = cfg.scale_next = 1.0;
     cfg.start = GNX_START_NULL;
     cfg.tol = tol;
     cfg.delta = delta;
@@ -253,7 +285,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_null_p3(MixFix a, double tol, do
     if (f < 0) break;
     double2 bb = a.b[off + f];
     a.x[off + f] = make_double2(0.0, 0.0);
-    float2 bs = make_float2((float)bb.x, (float)bb.y);
+    float2 bs = S::kScaled ? make_float2(0.f, 0.f) : make_float2((float)bb.x, (float)bb.y);
     S::st((typename S::T*)a.r + off, f, bs);
     S::st((typename S::T*)a.p0 + off, f, bs);
     S::st((typename S::T*)a.p1 + off, f, make_float2(0.f, 0.f));
@@ -509,7 +541,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Xe(MixFix a, int sl, const MixSt
   }
   end_of_loads();
   const double rho = block_sum_once(partial_finish(pin + (size_t)rhs * a.nb, a.nb, pl), lds);
-  const double nr = sqrt(rho);
+  const double nr = S::kScaled ? sqrt(rho) / s.scale : sqrt(rho);  // the unscaled norm
   MixState* out = sout + rhs;
   if (!s.active) {
     if (lead) put_mstate(out, 0, s.iters, s.phase, s.nrel, s.converged, s.early, s.rho, s.rmax, s);
@@ -532,7 +564,8 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Xe(MixFix a, int sl, const MixSt
       return;
     }
     rmax = nr;
-    beta = rho / s.rho;
+    // a scaled format's first update is the start (the direction is zero)
+    beta = (S::kScaled && s.iters == 0 && s.nrel == 0) ? 0.0 : rho / s.rho;
   } else if (s.iters == 0) {
     if (s.start == GNX_START_IMPOSED && nr < s.tol) {  // conjugategradient.c:123-128
       if (lead) {
@@ -543,6 +576,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Xe(MixFix a, int sl, const MixSt
       return;
     }
     rmax = nr;
+    update = S::kScaled;  // scaled storage: the first slot stores r = s b as an update
   } else {
     // fold the correction and test the true residual when the iterated one
     // has dropped by delta, claims convergence, or the iteration cap is hit
@@ -565,7 +599,11 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Xe(MixFix a, int sl, const MixSt
       if (fs[i] < 0) continue;
       a.u[off + fs[i]] = apply_Mdag(g, decode(g, fs[i]), a.sm[fs[i]], ldx);
     }
-    if (lead) put_mstate(out, 1, s.iters, PH_UPDATE, nrel, 0, 0, s.rho, rmax, s);
+    if (lead) {
+      put_mstate(out, 1, s.iters, PH_UPDATE, nrel, 0, 0, s.rho, rmax, s);
+      // scale of the coming r: its elements O(1) if ||r|| stays near ||r_it||
+      if (S::kScaled) out->scale_next = nr > 0.0 ? sqrt((double)g.V) / nr : s.scale;
+    }
     return;
   }
   // normal step: (even slots) xs += alpha_b p_prev + alpha_a p_old;
@@ -646,24 +684,31 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Ye(MixFix a, int sl, const MixSt
     const T* pold = (const T*)mix_p(a, sl) + off;
     T* pnext = (T*)mix_p(a, sl + 1) + off;
     const float fa = (float)s.alpha_a, fp = (float)s.alpha_b;
+    // scaled storage: r is stored at the new scale, the direction and the
+    // previous rho are converted to it
+    const double sn = S::kScaled ? s.scale_next : 1.0;
+    const double ratio = S::kScaled ? s.scale_next / s.scale : 1.0;
+    const float fr = (float)ratio;
 #pragma unroll
     for (int i = 0; i < SPT; i++) {
       const int f = fs[i];
       if (f < 0) break;
       const float2 po = S::ld(pold, f);
       const float2 c = cfma(fa, po, cfma(fp, S::ld(pnext, f), a.xs[off + f]));
-      S::st(pnext, f, po);
+      S::st(pnext, f, S::kScaled ? make_float2(fr * po.x, fr * po.y) : po);
       a.x[off + f] = cadd(a.x[off + f], make_double2((double)c.x, (double)c.y));
       a.xs[off + f] = make_float2(0.f, 0.f);
       double2 rr = apply_sub_M(g, decode(g, f), a.sm[f], a.b[off + f], [&](int q) { return u[q]; });
+      if (S::kScaled) rr = make_double2(sn * rr.x, sn * rr.y);
       S::st(r, f, make_float2((float)rr.x, (float)rr.y));
       acc += cnorm2(rr);
     }
     acc = block_sum_once(acc, lds2);
     if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + partial_slot(blockIdx.x, a.nb)] = acc;
     if (lead) {
-      put_mstate(out, 1, s.iters, PH_UPDATED, s.nrel, 0, 0, s.rho, s.rmax, s);
+      put_mstate(out, 1, s.iters, PH_UPDATED, s.nrel, 0, 0, s.rho * ratio * ratio, s.rmax, s);
       put_alphas(out, 0.0, 0.0);
+      if (S::kScaled) out->scale = sn;
     }
     return;
   }
@@ -681,7 +726,8 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Ye(MixFix a, int sl, const MixSt
   if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + partial_slot(blockIdx.x, a.nb)] = acc;
   if (lead) {
     put_mstate(out, 1, s.iters + 1, PH_NORMAL, s.nrel, 0, 0, s.rho, s.rmax, s);
-    put_alphas(out, (double)al, s.alpha_a);
+    // pending step length of the unscaled direction: alpha / s
+    put_alphas(out, S::kScaled ? (double)al / s.scale : (double)al, s.alpha_a);
   }
 }
 
@@ -1144,12 +1190,14 @@ static hipError_t mix_tune(MixMachine* m, const Geom& g, const double* sm, const
   int best_shape = 0;
   hipError_t e = hipSuccess;
   const char* env = getenv("GNX_MIX_SHAPE");
-  if (env && atoi(env) >= 0 && atoi(env) < kMixShapes && (nrhs >= 2 || !mix_shapes<S>()[atoi(env)].pair)) {
+  if (env && atoi(env) >= 0 && atoi(env) < kMixShapes && (nrhs >= 2 || !mix_shapes<S>()[atoi(env)].pair) &&
+      (!S::kScaled || mix_shapes<S>()[atoi(env)].early)) {
     best_shape = atoi(env);
   } else {
     float best_ms = 1e30f;
     for (int sh = 0; sh < kMixShapes && e == hipSuccess; sh++) {
       if (mix_shapes<S>()[sh].pair && nrhs < 2) continue;
+      if (S::kScaled && !mix_shapes<S>()[sh].early) continue;  // the scale lives in the loads-first form
       hipGraphExec_t exec;
       e = mix_graph<S>(m, g, nrhs, sh, tag, &exec);
       float ms_min = 1e30f;
@@ -1210,13 +1258,15 @@ hipError_t cgmix_run(void** machine_slot, const Geom& g, const double* sm, const
   MixMachine* m = (MixMachine*)*machine_slot;
   if (prec == GNX_PREC_F32)
     return mix_run_t<StoreF32>(m, g, sm, b, x, tol, delta, max_iter, start, nrhs, st, host_res, 100);
+  if (prec == GNX_PREC_F16)
+    return mix_run_t<StoreF16>(m, g, sm, b, x, tol, delta, max_iter, start, nrhs, st, host_res, 102);
   return mix_run_t<StoreBF16>(m, g, sm, b, x, tol, delta, max_iter, start, nrhs, st, host_res, 101);
 }
 
 int cgmix_shape_of(void* machine, int nrhs, int prec, int* spt, int* xcd, int* pair, int* early) {
   MixMachine* m = (MixMachine*)machine;
   if (!m) return -1;
-  const int key = nrhs + 1000 * (prec == GNX_PREC_F32 ? 100 : 101);
+  const int key = nrhs + 1000 * (prec == GNX_PREC_F32 ? 100 : prec == GNX_PREC_F16 ? 102 : 101);
   for (int i = 0; i < 8; i++)
     if (m->eng.tuned_tag[i] >= 0 && m->eng.tuned_nrhs[i] == key) {
       const MixShape& sh = mix_shapes<StoreF32>()[m->eng.tuned_tag[i]];

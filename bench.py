@@ -420,7 +420,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch", default="16,32", help="batch sizes of the batched FP64 CG leg ('' to skip)")
-    ap.add_argument("--mixed", default="f32,bf16", help="precisions of the mixed-precision legs ('' to skip)")
+    ap.add_argument("--mixed", default="f32,bf16,f16", help="precisions of the mixed-precision legs ('' to skip)")
     ap.add_argument("--condensate", default="48:64", metavar="L:N",
                     help="batched stochastic condensate leg: lattice L^3, N complex Z2 sources ('' to skip)")
     ap.add_argument("--chains", default="16,24,32,48", metavar="L,L,...",
@@ -569,7 +569,7 @@ def main():
 
     # ---------------- mixed precision (FP64 reliable updates) ----------------
     mixed = {}
-    precs = {"f32": gnx.PREC_F32, "bf16": gnx.PREC_BF16}
+    precs = {"f32": gnx.PREC_F32, "bf16": gnx.PREC_BF16, "f16": gnx.PREC_F16}
     for pname in [v for v in args.mixed.split(",") if v]:
         pcode = precs[pname]
         for nb in [1] + [int(v) for v in args.batch.split(",") if v]:

@@ -64,8 +64,12 @@ typedef enum gnx_cg_start {
 typedef enum gnx_prec {
     GNX_PREC_F64 = 0,       /* plain FP64 drop-in                                */
     GNX_PREC_F32 = 1,       /* FP32 inner iterations, FP64 reliable updates      */
-    GNX_PREC_BF16 = 2       /* BF16 storage / FP32 arithmetic inner iterations,
+    GNX_PREC_BF16 = 2,      /* BF16 storage / FP32 arithmetic inner iterations,
                                FP64 reliable updates                             */
+    GNX_PREC_F16 = 3        /* FP16 storage (scaled per system at every reliable
+                               update) / FP32 arithmetic inner iterations, FP64
+                               reliable updates; the first slot of a solve is an
+                               update, counted in reliable_updates           */
 } gnx_prec;
 
 typedef struct gnx_cg_result {
@@ -141,7 +145,7 @@ gnx_status gnx_condensate_batch(gnx_ctx *ctx, const double *sm, const double *no
 
 /* Mixed precision: residual reduction factor since the last FP64 reliable
  * update that triggers the next one.  0 (the default) selects it per
- * precision: 0.05 for FP32, 0.2 for BF16. */
+ * precision: 0.05 for FP32 and FP16, 0.2 for BF16. */
 gnx_status gnx_set_reliable_delta(gnx_ctx *ctx, double delta);
 
 /* Launch shape the CG autotuner picked for batches of nrhs systems on this
