@@ -161,7 +161,8 @@ def cpu_baseline(L, sigma, b, iters_full, budget_s=15.0):
 def condensate_leg(ctx_factory, torch, gnx, spec, tol):
     """Chiral condensate Tr M^-1 / V from N complex Z2 sources on one sigma
     field (condensate.c:127-145 per source): one batched solve in FP64 and in
-    mixed FP32, against N single FP64 solves of the same sources.  Complex Z2
+    mixed FP32 and FP16 storage, against N single FP64 solves of the same
+    sources.  Complex Z2
     has E|v|^2 = 2, so each source's estimate is Re(v^dag M^-1 v) / (2V)."""
     Lc, n = (int(v) for v in spec.split(":"))
     V = Lc ** 3
@@ -175,7 +176,7 @@ def condensate_leg(ctx_factory, torch, gnx, spec, tol):
     noise = torch.from_numpy(noise_h).to(dev)
     X = ctx.vec(n)
     out = {"workload": f"{n} complex Z2 sources at {Lc}^3, tol {tol:g}"}
-    for name, prec in (("f64", gnx.PREC_F64), ("f32", gnx.PREC_F32)):
+    for name, prec in (("f64", gnx.PREC_F64), ("f32", gnx.PREC_F32), ("f16", gnx.PREC_F16)):
         ctx.condensate_batch(sm, noise, n, tol, X, prec=prec)  # tune + warm
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -205,7 +206,7 @@ def condensate_leg(ctx_factory, torch, gnx, spec, tol):
                              "noise_variance_of_mean": float(est1.var(ddof=1) / n),
                              "seconds": dt, "rhs_solves_per_s": n / dt}
     sem = float(np.sqrt(est1.var(ddof=1) / n))
-    for name in ("f64", "f32"):
+    for name in ("f64", "f32", "f16"):
         b = out[f"batched_{name}"]
         b["diff_vs_per_source"] = abs(b["estimate"] - est1.mean())
         b["diff_in_statistical_errors"] = b["diff_vs_per_source"] / sem
