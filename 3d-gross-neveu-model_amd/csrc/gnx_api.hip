@@ -156,6 +156,20 @@ gnx_status gnx_dirac(gnx_ctx* c, gnx_op op, const double* sm, const double* in, 
   return GNX_OK;
 }
 
+gnx_status gnx_dirac_batch(gnx_ctx* c, gnx_op op, const double* sm, const double* in, double* out, int nvec) {
+  CHECK_ARG(c && sm && in && out, "gnx_dirac_batch: null argument");
+  CHECK_ARG(nvec >= 1 && nvec <= 64, "gnx_dirac_batch: nvec must be in [1, 64]");
+  CHECK_ARG(op == GNX_OP_M || op == GNX_OP_MDAG || op == GNX_OP_SUB_M || op == GNX_OP_SUB_MDAG,
+            "gnx_dirac_batch: op must be one operator (M, M^dag, r -= M v, r -= M^dag v)");
+  const char* ib = (const char*)in;
+  const char* ob = (const char*)out;
+  const size_t bytes = sizeof(double2) * (size_t)c->V * nvec;
+  CHECK_ARG(ib + bytes <= ob || ob + bytes <= ib, "gnx_dirac_batch: in and out overlap");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(launch_dirac(c->g, (int)op, sm, in, out, c->tmp, c->stream, nvec));
+  return GNX_OK;
+}
+
 static void fill_result(gnx_cg_result* r, const CGDevResult& d) {
   r->converged = d.converged;
   r->early_exit = d.early_exit;

@@ -34,11 +34,20 @@ __global__ void k_evolve_momenta(int V, double* __restrict__ mom, const double* 
 }
 
 // ------------------------------------------------------------------ Dirac
+// blockIdx.y: vector of a batch (the same sigma mass for all).  Sites are
+// mapped as the CG kernels' XCD slab shape: each XCD sweeps one lex slab of
+// both parities, so a site's x0 neighbours are mostly in its own L2 (with
+// the plain mapping they sit L*LT/2 sites = L*LT/512 blocks away, on another
+// XCD unless that count is a multiple of 8, and every element is fetched by
+// three L2s once the vectors outgrow the Infinity Cache).
 template <int OP>
 __global__ __launch_bounds__(256) void k_dirac(Geom g, const double* __restrict__ sm, const double2* __restrict__ in,
                                                double2* __restrict__ out) {
-  const int f = blockIdx.x * 256 + threadIdx.x;
-  if (f >= g.V) return;
+  static_assert(ENG_BS == 256, "the site map assumes 256-thread blocks");
+  const int f = SiteMap<1, true>((int)gridDim.x).f(g, 0);
+  if (f < 0) return;
+  in += (size_t)blockIdx.y * g.V;
+  out += (size_t)blockIdx.y * g.V;
   const Site s = decode(g, f);
   auto ld = [&](int j) { return *elem(in, j); };
   const double smv = *elem(sm, f);
@@ -183,10 +192,12 @@ hipError_t launch_evolve_momenta(const Geom& g, double* mom, const double* force
 }
 
 hipError_t launch_dirac(const Geom& g, int op, const double* sm, const double* in, double* out, double* tmp,
-                        hipStream_t st) {
+                        hipStream_t st, int nvec) {
   const double2* vin = (const double2*)in;
   double2* vout = (double2*)out;
-  dim3 grid(nblk(g.V, 256)), blk(256);
+  dim3 grid(nblk(g.V, 256), nvec), blk(256);
+  // the two-operator products stage one vector in tmp
+  if (nvec != 1 && (op == GNX_OP_MMDAG || op == GNX_OP_MDAGM)) return hipErrorInvalidValue;
   switch (op) {
     case GNX_OP_M:
       hipLaunchKernelGGL(k_dirac<GNX_OP_M>, grid, blk, 0, st, g, sm, vin, vout);

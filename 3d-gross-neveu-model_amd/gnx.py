@@ -64,6 +64,7 @@ def lib():
         L.gnx_geometry_tables.argtypes = [vp, vp, vp, vp, vp, vp]
         L.gnx_sigma_mass.argtypes = [vp, vp, vp]
         L.gnx_dirac.argtypes = [vp, C.c_int, vp, vp, vp]
+        L.gnx_dirac_batch.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int]
         L.gnx_cg.argtypes = [vp, vp, vp, vp, C.c_double, C.c_long, C.c_int, C.c_int, C.POINTER(CGResult)]
         L.gnx_cg_batch.argtypes = [vp, vp, vp, vp, C.c_int, C.c_double, C.c_long, C.c_int, C.c_int,
                                    C.POINTER(CGResult)]
@@ -207,6 +208,10 @@ class Context:
 
     def dirac(self, op, sm, vin, vout):
         _check(lib().gnx_dirac(self.h, op, _ptr(sm), _ptr(as_real(vin)), _ptr(as_real(vout))))
+
+    def dirac_batch(self, op, sm, vin, vout, nvec):
+        assert as_real(vin).numel() == 2 * self.V * nvec and as_real(vout).numel() == 2 * self.V * nvec
+        _check(lib().gnx_dirac_batch(self.h, op, _ptr(sm), _ptr(as_real(vin)), _ptr(as_real(vout)), nvec))
 
     # -- A6/A7 --------------------------------------------------------------
     def cg(self, sm, b, x, tol, max_iter=0, start=START_NULL, prec=PREC_F64, check=True):

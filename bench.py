@@ -476,6 +476,26 @@ def main():
     e1.synchronize()
     t_app = e0.elapsed_time(e1) * 1e-3 / napp
     dirac_gbs = BYTES_DIRAC_F64 * V / t_app / 1e9
+    # batched: 16 vectors per launch, one sigma mass (2 s_v + s_m / n per site)
+    nvec = 16
+    vb = torch.from_numpy(np.where(np.random.default_rng(77).random(2 * V * nvec) <= 0.5, 1.0, -1.0)).to(dev)
+    ob = ctx.vec(nvec)
+    wb = ctx.vec(nvec)
+    for _ in range(4):
+        ctx.dirac_batch(gnx.OP_M, sm, vb, ob, nvec)
+    torch.cuda.synchronize()
+    nb_launch = 100
+    e0.record(stream)
+    for _ in range(nb_launch // 2):
+        ctx.dirac_batch(gnx.OP_M, sm, vb, ob, nvec)
+        ctx.dirac_batch(gnx.OP_MDAG, sm, ob, wb, nvec)
+    e1.record(stream)
+    e1.synchronize()
+    t_b = e0.elapsed_time(e1) * 1e-3 / (nb_launch * nvec)
+    bytes_b = 32 + 8.0 / nvec
+    dirac_batched = {"vectors_per_launch": nvec, "apps_per_s": 1.0 / t_b, "bytes_per_site": bytes_b,
+                     "GB_s": bytes_b * V / t_b / 1e9, "frac_hbm": bytes_b * V / t_b / 1e9 / HBM_PEAK_GBS}
+    del vb, ob, wb
 
     # ---------------- FP64 CG drop-in ----------------
     for _ in range(args.warmup):
@@ -522,6 +542,7 @@ def main():
                    "true_residual": true_res, "cg_shape": ctx.cg_shape(1)},
         "dirac": {"apps_per_s": 1.0 / t_app, "GB_s": dirac_gbs, "frac_hbm": dirac_gbs / HBM_PEAK_GBS,
                   "bytes_per_site": BYTES_DIRAC_F64},
+        "dirac_batched": dirac_batched,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": recorded_traffic(L, ctx.cg_shape(1)),

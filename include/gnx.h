@@ -102,6 +102,11 @@ gnx_status gnx_sigma_mass(gnx_ctx *ctx, const double *sigma, double *sm);
 
 /* A3/A4/A5 and the normal operators.  `in` and `out` must not alias. */
 gnx_status gnx_dirac(gnx_ctx *ctx, gnx_op op, const double *sm, const double *in, double *out);
+/* The same operator on nvec vectors (in, out: [nvec][V] complex, not
+ * overlapping) with one sigma mass, one launch; op is GNX_OP_M, GNX_OP_MDAG,
+ * GNX_OP_SUB_M or GNX_OP_SUB_MDAG.  Each vector's result is bit-identical to
+ * its own gnx_dirac.  No counterpart in the reference (one vector per call). */
+gnx_status gnx_dirac_batch(gnx_ctx *ctx, gnx_op op, const double *sm, const double *in, double *out, int nvec);
 
 /* A6/A7. Solve (M M^dag) x = b to ||r||_2 < tol (absolute).
  * max_iter <= 0 selects the reference cap V+1 (macro.h:134).

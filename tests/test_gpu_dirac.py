@@ -94,3 +94,33 @@ def test_ops_vs_oracle(torch_dev, P, gnx_mod, L, LT):
     ctx.dirac(gnx_mod.OP_MDAGM, sm, bd, out)
     ctx.synchronize()
     assert np.array_equal(host(out), lat.Mdag(sm_ref, lat.M(sm_ref, b)))
+
+
+@pytest.mark.parametrize("L,LT", [(8, 8), (6, 10)])
+def test_dirac_batch_matches_single(torch_dev, P, gnx_mod, L, LT):
+    """gnx_dirac_batch: each of 5 vectors bit-identical to its own gnx_dirac,
+    for the four single-operator forms (the subtracting ones read out), and
+    the batch against the oracle."""
+    banner(f"dirac batch {L}x{L}x{LT}")
+    lat = P.Lattice(L, LT)
+    V = lat.V
+    r = P.Random(71)
+    smh = lat.sigma_mass(r.sigma(V, P.sigma_variance(0.85)))
+    sm = dev(smh)
+    n = 5
+    vs = [r.z2_complex(V) for _ in range(n)]
+    ts = [r.z2_complex(V) for _ in range(n)]
+    ctx = gnx_mod.Context(L, LT)
+    for op, ref in ((gnx_mod.OP_M, lambda v, t: lat.M(smh, v)), (gnx_mod.OP_MDAG, lambda v, t: lat.Mdag(smh, v)),
+                    (gnx_mod.OP_SUB_M, lambda v, t: lat.sub_M(smh, v, t)),
+                    (gnx_mod.OP_SUB_MDAG, lambda v, t: lat.sub_Mdag(smh, v, t))):
+        vin = dev(np.concatenate(vs))
+        out = dev(np.concatenate(ts))
+        ctx.dirac_batch(op, sm, vin, out, n)
+        outh = host(out)
+        for k in range(n):
+            o1 = dev(ts[k])
+            ctx.dirac(op, sm, dev(vs[k]), o1)
+            seg = outh[2 * V * k:2 * V * (k + 1)]
+            assert np.array_equal(seg, host(o1)), (op, k)
+            assert np.array_equal(seg, ref(vs[k], ts[k])), (op, k)
