@@ -92,8 +92,11 @@ def summarize_rocprof(kt_dir, fetch_dir, write_dir, L):
                 copy.append((int(r["Grid_Size"]), v))
             elif n.startswith(("k_cgm_A", "k_cgm_B")) and int(r["Grid_Size"]) * int(n.split("<")[1][0]) == L ** 3:
                 per.setdefault(n, []).append(v)
+        # the calibration's copies: the largest transfers of the largest
+        # grid (a batched leg's copy-out runs on the same grid, smaller)
         big = max(g for g, _ in copy)
-        copy_kib = statistics.median(v for g, v in copy if g == big)
+        top = max(v for g, v in copy if g == big)
+        copy_kib = statistics.median(v for g, v in copy if g == big and v >= 0.75 * top)
         counters[ctr] = (per, copy_kib)
     calib_kib = 256 * 1024  # --copy-calibration 256: 256 MiB read and 256 MiB written
     scale = {c: calib_kib / counters[c][1] for c in counters}
