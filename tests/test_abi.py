@@ -31,3 +31,27 @@ def test_library_does_not_link_oracle():
     assert "oracle" not in out
     syms = subprocess.run(["nm", "-D", LIB], capture_output=True, text=True).stdout
     assert "orc_" not in syms
+
+
+def test_argument_errors_without_a_gpu():
+    """Argument checks come before any HIP call: null handles and bad sizes
+    return GNX_ERR_ARG (1) with a message naming the entry point."""
+    lib = ctypes.CDLL(LIB)
+    lib.gnx_last_error.restype = ctypes.c_char_p
+    vp, dp = ctypes.c_void_p, ctypes.c_double
+
+    def err(status, name):
+        assert status == 1, (name, status)
+        assert name in lib.gnx_last_error().decode(), lib.gnx_last_error()
+
+    err(lib.gnx_dirac_batch(vp(None), 0, vp(None), vp(None), vp(None), 4), "gnx_dirac_batch")
+    err(lib.gnx_multi_cg(vp(None), vp(None), vp(None), vp(None), dp(1e-10), ctypes.c_long(0), 1, vp(None)),
+        "gnx_multi_cg")
+    err(lib.gnx_hmc_ensemble_trajectory(vp(None), 1, vp(None), vp(None), vp(None), vp(None), dp(0.1), 2,
+                                        dp(1e-10), dp(1e-3), ctypes.c_long(0), vp(None)),
+        "gnx_hmc_ensemble_trajectory")
+    err(lib.gnx_link_hmc_ensemble_trajectory(vp(None), 1, vp(None), vp(None), vp(None), vp(None), dp(0.02), 2,
+                                             dp(1e-10), ctypes.c_long(0), vp(None)),
+        "gnx_link_hmc_ensemble_trajectory")
+    n = ctypes.c_int(0)
+    err(lib.gnx_multi_create(0, ctypes.byref(n), ctypes.byref(n), 0, ctypes.byref(vp())), "gnx_multi_create")
