@@ -151,6 +151,13 @@ def cpu_baseline(L, sigma, b, iters_full, budget_s=15.0):
     lat.cg(sm, b, None, tol=0.0, max_iter=n, start="imposed")
     dt = time.perf_counter() - t0
     per_iter = dt / n
+    # the serial Dirac application (M and M^dag alternating, 100 of them)
+    v = b
+    t0 = time.perf_counter()
+    for k in range(50):
+        w = lat.M(sm, v)
+        v = lat.Mdag(sm, w)
+    t_app = (time.perf_counter() - t0) / 100
     return {
         "value": 1.0 / (per_iter * iters_full),
         "unit": "RHS-solves/s",
@@ -158,6 +165,8 @@ def cpu_baseline(L, sigma, b, iters_full, budget_s=15.0):
         "kind": "port",
         "sample": f"{n} CG iterations of the serial oracle at {L}^3 ({dt:.1f} s), "
                   f"scaled to the {iters_full} iterations of a full solve; per-iteration {per_iter*1e3:.2f} ms",
+        "dirac": {"apps_per_s": 1.0 / t_app, "GB_s": BYTES_DIRAC_F64 * L ** 3 / t_app / 1e9,
+                  "sample": "100 serial applications (M and M^dag alternating) at the same size"},
     }
 
 
