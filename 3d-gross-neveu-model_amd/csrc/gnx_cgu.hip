@@ -9,10 +9,11 @@
 //   iteration counts differ do not pay for each other's iterations beyond one
 //   state read per block.
 //
-// Per system the steps, the launch partition (one site per thread, 256-thread
-// blocks) and the order of every sum are those of gnx_cgm.hip's shape 0, so
-// a system's solution, iteration count and exit are bit-identical to its
-// single solve with that shape.  Vectors of all systems are concatenated in
+// Per system the steps, the launch partition (one site per thread in the XCD
+// slab mapping, 256-thread blocks) and the order of every sum are those of
+// gnx_cgm.hip's shape 1, so a system's solution, iteration count and exit
+// are bit-identical to its single solve with that shape (while every system
+// picks the same head-reduction form: all at most 512 blocks, or all more).  Vectors of all systems are concatenated in
 // solver-owned buffers (the caller's sigma mass, b and x_in are copied in,
 // the solution copied out); partial sums of system k live at offset poff_k
 // of each of the two slots.
@@ -60,9 +61,12 @@ GNX_DEVONLY bool cgu_sys(const CGUFix& a, CGUSys* d) {
   return (int)blockIdx.x < d->nb;
 }
 
+// one site per thread in the XCD slab mapping of the single-lattice solver
+// (its shape 1); the grid's x extent is a multiple of 8, so block x of every
+// system runs on XCD x mod 8 and each system's slabs stay on their XCDs
 GNX_DEVONLY int cgu_site(const CGUSys& d) {
-  const int f = (int)blockIdx.x * CGU_BS + (int)threadIdx.x;
-  return f < d.g.V ? f : -1;
+  static_assert(CGU_BS == ENG_BS, "the site map assumes the engine's block size");
+  return SiteMap<1, true>(d.nb).f(d.g, 0);
 }
 
 // ---- start kernels (per solve, outside the graph) -------------------------------
@@ -346,7 +350,7 @@ static CGUFix cgu_fix(gnx_multi* m) {
 static hipError_t cgu_graph(gnx_multi* m, hipGraphExec_t* exec) {
   CGUFix a = cgu_fix(m);
   const int n = m->n;
-  dim3 grid(m->nbmax, n), blk(CGU_BS);
+  dim3 grid((m->nbmax + NUM_XCD - 1) / NUM_XCD * NUM_XCD, n), blk(CGU_BS);
   const CGMState* s0 = a.state;
   const CGMState* s1 = a.state + n;
   CGMState* w0 = a.state;
@@ -474,7 +478,7 @@ gnx_status gnx_multi_cg(gnx_multi* m, const double* const* sm, const double* con
   }
   HIP_TRY(hipMemcpyAsync(m->dmax_iter, m->hmax_iter, sizeof(long) * n, hipMemcpyHostToDevice, st));
   CGUFix a = cgu_fix(m);
-  dim3 grid(m->nbmax, n), blk(CGU_BS);
+  dim3 grid((m->nbmax + NUM_XCD - 1) / NUM_XCD * NUM_XCD, n), blk(CGU_BS);
   const long* mi = m->dmax_iter;
   if (start == GNX_START_NULL) {
     void* a1[] = {(void*)&a};

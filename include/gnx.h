@@ -129,7 +129,8 @@ gnx_status gnx_cg_batch(gnx_ctx *ctx, const double *sm, const double *b, double 
  * reading its system's geometry from a descriptor table; a system that has
  * left the loop costs one state read per block per kernel.  Each system's
  * solution and iteration count are bit-identical to its gnx_cg solve with
- * the one-site-per-thread launch shape (GNX_CG_SHAPE=0).  sm[k], b[k], x[k]
+ * one site per thread in the XCD slab mapping (GNX_CG_SHAPE=1) when every
+ * system has at most 512 blocks of 256 sites, or every one more.  sm[k], b[k], x[k]
  * are device pointers (V_k doubles, 2 V_k doubles); res[n].  Returns
  * GNX_ERR_NOCONV when a system reaches its cap. */
 gnx_status gnx_multi_create(int n, const int *lspace, const int *ltime, int device, gnx_multi **out);

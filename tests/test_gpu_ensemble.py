@@ -3,7 +3,7 @@ gnx_link_hmc_ensemble_trajectory): every solve point of the integrator is
 one uneven-batch launch over all chains.
 
 Bars (written here):
-  - with the one-site-per-thread CG shape pinned (GNX_CG_SHAPE=0, the
+  - with the one-site-per-thread slab CG shape pinned (GNX_CG_SHAPE=1, the
     partition and order of sums of the uneven solver), every chain's
     trajectory is bit-identical to its own single-chain trajectory on the
     same draws: dH, H0, H1, the CG counters, and the field after it;
@@ -39,7 +39,7 @@ def _same(a, b, what):
 
 def test_sigma_ensemble_matches_single_chains(P, gnx_mod, monkeypatch):
     banner("sigma ensemble (4, 8, 6, 10)^3 vs single chains")
-    monkeypatch.setenv("GNX_CG_SHAPE", "0")
+    monkeypatch.setenv("GNX_CG_SHAPE", "1")
     single, ens = [], []
     for i, (L, bs) in enumerate(CHAINS):
         sig = _sigma(P, L, bs, 900 + i)
@@ -69,7 +69,7 @@ def test_sigma_ensemble_matches_single_chains(P, gnx_mod, monkeypatch):
 
 def test_link_ensemble_matches_single_chains(P, gnx_mod, monkeypatch):
     banner("link ensemble (4, 8, 6, 10)^3 vs single chains")
-    monkeypatch.setenv("GNX_CG_SHAPE", "0")
+    monkeypatch.setenv("GNX_CG_SHAPE", "1")
     single, ens = [], []
     for i, (L, bs) in enumerate(CHAINS):
         V = L ** 3
@@ -103,7 +103,7 @@ def test_ensemble_chain_leaves_on_iteration_cap(P, gnx_mod, monkeypatch):
     restores sigma and leaves; the small chain's trajectory is unchanged."""
     banner("sigma ensemble, one chain at the iteration cap")
     import torch
-    monkeypatch.setenv("GNX_CG_SHAPE", "0")
+    monkeypatch.setenv("GNX_CG_SHAPE", "1")
     chains = [(4, 0.85), (12, 0.85)]
     sig = [_sigma(P, L, bs, 990 + i) for i, (L, bs) in enumerate(chains)]
     draws = [_draws(17 + i, L ** 3, L ** 3) for i, (L, _) in enumerate(chains)]

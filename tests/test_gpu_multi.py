@@ -3,8 +3,9 @@ each with its own sigma mass, solved together in one launch per kernel.
 
 Bars (written here):
   - every system's solution bit-identical, and its iteration count and exit
-    identical, to its own gnx_cg solve with the one-site-per-thread shape
-    (GNX_CG_SHAPE=0: the same partition, arithmetic and order of sums);
+    identical, to its own gnx_cg solve with one site per thread in the XCD
+    slab mapping (GNX_CG_SHAPE=1: the same partition, arithmetic and order
+    of sums);
   - against the oracle (the reference's CG restated): iterations within +-1,
     solution within 1e-8 relative at tol 1e-10; both starts;
   - null start: a system handed its own solution leaves in the prologue
@@ -42,7 +43,7 @@ def _single(gnx_mod, lat, sm, b, x0, tol, start, max_iter=0):
 @pytest.mark.parametrize("start", ["imposed", "null"])
 def test_multi_matches_single_and_oracle(P, gnx_mod, start, monkeypatch):
     banner(f"uneven batch {SIZES} {start}")
-    monkeypatch.setenv("GNX_CG_SHAPE", "0")
+    monkeypatch.setenv("GNX_CG_SHAPE", "1")
     sy = _systems(P, 700)
     code = gnx_mod.START_IMPOSED if start == "imposed" else gnx_mod.START_NULL
     m = gnx_mod.Multi(SIZES)
@@ -61,7 +62,7 @@ def test_multi_matches_single_and_oracle(P, gnx_mod, start, monkeypatch):
 
 def test_multi_null_start_prologue_exit(P, gnx_mod, monkeypatch):
     banner("uneven batch: one system starts from its solution")
-    monkeypatch.setenv("GNX_CG_SHAPE", "0")
+    monkeypatch.setenv("GNX_CG_SHAPE", "1")
     sy = _systems(P, 800)
     m = gnx_mod.Multi(SIZES)
     sms = [dev(sm) for _, sm, _, _ in sy]
@@ -88,7 +89,7 @@ def test_multi_null_start_prologue_exit(P, gnx_mod, monkeypatch):
 
 def test_multi_iteration_cap(P, gnx_mod, monkeypatch):
     banner("uneven batch: iteration cap")
-    monkeypatch.setenv("GNX_CG_SHAPE", "0")
+    monkeypatch.setenv("GNX_CG_SHAPE", "1")
     sy = _systems(P, 900)
     m = gnx_mod.Multi(SIZES)
     xs = [dev(x0) for _, _, _, x0 in sy]
