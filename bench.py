@@ -488,6 +488,20 @@ def main():
     e1.synchronize()
     t_app = e0.elapsed_time(e1) * 1e-3 / napp
     dirac_gbs = BYTES_DIRAC_F64 * V / t_app / 1e9
+    # the normal operators as the drop-in applies them (two launches each):
+    # M M^dag (the CG's operator) and M^dag M (offered beside it)
+    prod = {}
+    for name, op in (("M_Mdag", gnx.OP_MMDAG), ("Mdag_M", gnx.OP_MDAGM)):
+        for _ in range(10):
+            ctx.dirac(op, sm, b, out)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(100):
+            ctx.dirac(op, sm, b, out)
+        e1.record(stream)
+        e1.synchronize()
+        tp = e0.elapsed_time(e1) * 1e-3 / 100
+        prod[name] = {"apps_per_s": 1.0 / tp, "GB_s": 2 * BYTES_DIRAC_F64 * V / tp / 1e9, "launches": 2}
     # batched: 16 vectors per launch, one sigma mass (2 s_v + s_m / n per site)
     nvec = 16
     vb = torch.from_numpy(np.where(np.random.default_rng(77).random(2 * V * nvec) <= 0.5, 1.0, -1.0)).to(dev)
@@ -551,10 +565,13 @@ def main():
         "data": "synthetic: sigma iid N(0, 2/(4*0.85)), complex Z2 source, numpy seed 1231",
         "config": {"workload": f"single-RHS FP64 CG on M M^dag to |r|<{args.tol:g} at {L}^3",
                    "L": L, "V": V, "tol": args.tol, "start": "imposed", "cg_iterations": it,
-                   "true_residual": true_res, "cg_shape": ctx.cg_shape(1)},
+                   "true_residual": true_res,
+                   "relative_residual": true_res / float(torch.linalg.norm(b).item()),
+                   "cg_shape": ctx.cg_shape(1)},
         "dirac": {"apps_per_s": 1.0 / t_app, "GB_s": dirac_gbs, "frac_hbm": dirac_gbs / HBM_PEAK_GBS,
                   "bytes_per_site": BYTES_DIRAC_F64},
         "dirac_batched": dirac_batched,
+        "normal_operators": prod,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": recorded_traffic(L, ctx.cg_shape(1)),
