@@ -69,6 +69,36 @@ GNX_DEVONLY int cgu_site(const CGUSys& d) {
   return SiteMap<1, true>(d.nb).f(d.g, 0);
 }
 
+// ---- the caller's vectors in and out (one launch each way) ------------------------
+// Per system the caller's sigma mass, right-hand side and (null start) x_in
+// are gathered into the concatenated buffers, and the solution scattered
+// back, by one kernel each instead of one copy per system and vector (three
+// copies a system per solve, each its own dispatch).
+struct CGUPtrs {
+  const double* sm;
+  const double2* b;
+  double2* x;
+};
+
+__global__ __launch_bounds__(CGU_BS) void k_cgu_gather(CGUFix a, const CGUPtrs* __restrict__ ptr, int with_x) {
+  CGUSys d;
+  if (!cgu_sys(a, &d)) return;
+  const int f = (int)blockIdx.x * CGU_BS + (int)threadIdx.x;
+  if (f >= d.g.V) return;
+  const CGUPtrs p = ptr[blockIdx.y];
+  a.sm[d.soff + f] = p.sm[f];
+  a.b[d.voff + f] = p.b[f];
+  if (with_x) a.x[d.voff + f] = p.x[f];
+}
+
+__global__ __launch_bounds__(CGU_BS) void k_cgu_scatter(CGUFix a, const CGUPtrs* __restrict__ ptr) {
+  CGUSys d;
+  if (!cgu_sys(a, &d)) return;
+  const int f = (int)blockIdx.x * CGU_BS + (int)threadIdx.x;
+  if (f >= d.g.V) return;
+  ptr[blockIdx.y].x[f] = a.x[d.voff + f];
+}
+
 // ---- start kernels (per solve, outside the graph) -------------------------------
 // imposed start (conjugategradient.c:121-128): x = 0, r = p0 = b, partial |b|^2 (slot 1)
 __global__ __launch_bounds__(CGU_BS) void k_cgu_start_imposed(CGUFix a, double tol, const long* __restrict__ max_iter) {
@@ -301,6 +331,8 @@ struct gnx_multi {
   gnx::CGUSys* dsys = nullptr;
   long* dmax_iter = nullptr;     // [n]
   long* hmax_iter = nullptr;     // [n], pinned
+  gnx::CGUPtrs* dptr = nullptr;  // [n] the caller's vectors of the current solve
+  gnx::CGUPtrs* hptr = nullptr;  // [n], pinned
   double* sm = nullptr;
   double2 *b = nullptr, *x = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr, *t = nullptr;
   gnx::CGDevResult* res = nullptr;
@@ -322,10 +354,11 @@ bool multi_system(const gnx_multi* m, int k, int* L, int* LT, int* device) {
 namespace gnx {
 
 static void cgu_free(gnx_multi* m) {
-  void* dev[] = {m->dsys, m->dmax_iter, m->sm, m->b, m->x, m->r, m->p0, m->p1, m->t, m->res};
+  void* dev[] = {m->dsys, m->dmax_iter, m->dptr, m->sm, m->b, m->x, m->r, m->p0, m->p1, m->t, m->res};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   if (m->hmax_iter) (void)hipHostFree(m->hmax_iter);
+  if (m->hptr) (void)hipHostFree(m->hptr);
   if (m->hres) (void)hipHostFree(m->hres);
 }
 
@@ -417,6 +450,8 @@ gnx_status gnx_multi_create(int n, const int* lspace, const int* ltime, int devi
   if (e == hipSuccess) e = hipMalloc(&m->dsys, sizeof(CGUSys) * n);
   if (e == hipSuccess) e = hipMalloc(&m->dmax_iter, sizeof(long) * n);
   if (e == hipSuccess) e = hipHostMalloc(&m->hmax_iter, sizeof(long) * n, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc(&m->dptr, sizeof(gnx::CGUPtrs) * n);
+  if (e == hipSuccess) e = hipHostMalloc(&m->hptr, sizeof(gnx::CGUPtrs) * n, hipHostMallocDefault);
   if (e == hipSuccess) e = hipMalloc(&m->sm, sizeof(double) * (size_t)m->stotal);
   if (e == hipSuccess) e = hipMalloc(&m->b, vb);
   if (e == hipSuccess) e = hipMalloc(&m->x, vb);
@@ -471,14 +506,20 @@ gnx_status gnx_multi_cg(gnx_multi* m, const double* const* sm, const double* con
     // the reference's cap V+1 (macro.h:134) of each system when max_iter <= 0
     m->hmax_iter[k] = max_iter > 0 ? max_iter : (long)d.g.V + 1;
     if (m->hmax_iter[k] > cap_max) cap_max = m->hmax_iter[k];
-    const size_t vb = sizeof(double2) * (size_t)d.g.V;
-    HIP_TRY(hipMemcpyAsync(m->sm + d.soff, sm[k], sizeof(double) * (size_t)d.g.V, hipMemcpyDeviceToDevice, st));
-    HIP_TRY(hipMemcpyAsync(m->b + d.voff, b[k], vb, hipMemcpyDeviceToDevice, st));
-    if (start == GNX_START_NULL) HIP_TRY(hipMemcpyAsync(m->x + d.voff, x[k], vb, hipMemcpyDeviceToDevice, st));
+    m->hptr[k] = CGUPtrs{sm[k], (const double2*)b[k], (double2*)x[k]};
   }
+  // the pinned tables are rewritten only after the previous solve's
+  // copies are complete (every solve ends with a stream synchronisation)
   HIP_TRY(hipMemcpyAsync(m->dmax_iter, m->hmax_iter, sizeof(long) * n, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(m->dptr, m->hptr, sizeof(CGUPtrs) * n, hipMemcpyHostToDevice, st));
   CGUFix a = cgu_fix(m);
   dim3 grid((m->nbmax + NUM_XCD - 1) / NUM_XCD * NUM_XCD, n), blk(CGU_BS);
+  {
+    const CGUPtrs* dp = m->dptr;
+    int with_x = start == GNX_START_NULL ? 1 : 0;
+    void* ag[] = {(void*)&a, (void*)&dp, (void*)&with_x};
+    HIP_TRY(hipLaunchKernel((const void*)k_cgu_gather, grid, blk, ag, 0, st));
+  }
   const long* mi = m->dmax_iter;
   if (start == GNX_START_NULL) {
     void* a1[] = {(void*)&a};
@@ -493,9 +534,10 @@ gnx_status gnx_multi_cg(gnx_multi* m, const double* const* sm, const double* con
   hipGraphExec_t exec;
   HIP_TRY(cgu_graph(m, &exec));
   HIP_TRY(m->eng.run(exec, n, cap_max, st, nullptr));
-  for (int k = 0; k < n; k++) {
-    const CGUSys& d = m->sys[k];
-    HIP_TRY(hipMemcpyAsync(x[k], m->x + d.voff, sizeof(double2) * (size_t)d.g.V, hipMemcpyDeviceToDevice, st));
+  {
+    const CGUPtrs* dp = m->dptr;
+    void* as[] = {(void*)&a, (void*)&dp};
+    HIP_TRY(hipLaunchKernel((const void*)k_cgu_scatter, grid, blk, as, 0, st));
   }
   HIP_TRY(hipMemcpyAsync(m->hres, m->res, sizeof(CGDevResult) * n, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
