@@ -499,6 +499,9 @@ gnx_status gnx_multi_cg(gnx_multi* m, const double* const* sm, const double* con
   }
   HIP_TRY(hipSetDevice(m->device));
   hipStream_t st = m->stream;
+  // the pinned tables below are rewritten here: no copy of an earlier call
+  // (one that returned early on an error) may still be reading them
+  HIP_TRY(hipStreamSynchronize(st));
   const int n = m->n;
   long cap_max = 0;
   for (int k = 0; k < n; k++) {
@@ -508,8 +511,6 @@ gnx_status gnx_multi_cg(gnx_multi* m, const double* const* sm, const double* con
     if (m->hmax_iter[k] > cap_max) cap_max = m->hmax_iter[k];
     m->hptr[k] = CGUPtrs{sm[k], (const double2*)b[k], (double2*)x[k]};
   }
-  // the pinned tables are rewritten only after the previous solve's
-  // copies are complete (every solve ends with a stream synchronisation)
   HIP_TRY(hipMemcpyAsync(m->dmax_iter, m->hmax_iter, sizeof(long) * n, hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(m->dptr, m->hptr, sizeof(CGUPtrs) * n, hipMemcpyHostToDevice, st));
   CGUFix a = cgu_fix(m);
