@@ -35,11 +35,21 @@
 namespace gnx {
 
 // ---- storage formats --------------------------------------------------------
+// Vectors of the inner iterations are written with ordinary stores: a batch's
+// working set (r, t, two p, xs: 201 MB for 32 systems at 64^3 in FP16 or
+// BF16) stays in the 256 MB Infinity Cache for the next kernel, where
+// non-temporal stores (kept by the FP64 kernels, 11.05 against 11.8 us per
+// iteration there) cost the FP16 batch of 32 12 % (824-832 against 738
+// RHS-solves/s; BF16 714 against 652, FP32 x16 497 against 444).  FP32 x32
+// (335 MB, past the cache) loses 5 % with them (408 against 430).
+GNX_DEVONLY void st_mix(float2* p, float2 v) { *p = v; }
+GNX_DEVONLY void st_mix(unsigned int* p, unsigned int v) { *p = v; }
+
 struct StoreF32 {
   using T = float2;
   static constexpr bool kScaled = false;
   GNX_DEVONLY static float2 ld(const T* p, int i) { return *elem(p, i); }
-  GNX_DEVONLY static void st(T* p, int i, float2 v) { st_nt(elem(p, i), v); }
+  GNX_DEVONLY static void st(T* p, int i, float2 v) { st_mix(elem(p, i), v); }
 };
 
 // bfloat16 pair (re in the low half), round-to-nearest-even on store: one
@@ -56,7 +66,7 @@ struct StoreBF16 {
   }
   GNX_DEVONLY static void st(T* p, int i, float2 v) {
     const gnx_f32x2 w = {v.x, v.y};
-    st_nt(elem(p, i), __builtin_bit_cast(unsigned int, __builtin_convertvector(w, gnx_bf16x2)));
+    st_mix(elem(p, i), __builtin_bit_cast(unsigned int, __builtin_convertvector(w, gnx_bf16x2)));
   }
 };
 
@@ -76,7 +86,7 @@ struct StoreF16 {
   }
   GNX_DEVONLY static void st(T* p, int i, float2 v) {
     const gnx_f32x2 w = {v.x, v.y};
-    st_nt(elem(p, i), __builtin_bit_cast(unsigned int, __builtin_convertvector(w, gnx_f16x2)));
+    st_mix(elem(p, i), __builtin_bit_cast(unsigned int, __builtin_convertvector(w, gnx_f16x2)));
   }
 };
 
@@ -468,7 +478,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Y(MixFix a, int odd, const MixSt
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
     if (fs[i] < 0) continue;
-    st_nt(&a.xs[off + fs[i]], cfma(al, pv[i], xv[i]));
+    st_mix(&a.xs[off + fs[i]], cfma(al, pv[i], xv[i]));
     float2 rn = cfma(-al, w[i], rv[i]);
     S::st(r, fs[i], rn);
     acc += dnorm2(rn);
@@ -613,7 +623,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Xe(MixFix a, int sl, const MixSt
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
     if (fs[i] < 0) continue;
-    if (FLUSH) st_nt(elem(a.xs + off, fs[i]), cfma(fa, pv[i][0], cfma(fp, ppr[i], xsv[i])));
+    if (FLUSH) st_mix(elem(a.xs + off, fs[i]), cfma(fa, pv[i][0], cfma(fp, ppr[i], xsv[i])));
     const Site st = decode(g, fs[i]);
     float2 p0v = cfma(fb, pv[i][0], rv[i][0]);
     float2 ppv[3], pmv[3];
@@ -976,7 +986,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Y2(MixFix a, int odd, const MixS
 #pragma unroll
       for (int i = 0; i < SPT; i++) {
         if (fs[i] < 0) continue;
-        st_nt(&a.xs[off + fs[i]], cfma(al, pv[k][i], xv[k][i]));
+        st_mix(&a.xs[off + fs[i]], cfma(al, pv[k][i], xv[k][i]));
         float2 rn = cfma(-al, w[k][i], rv[k][i]);
         S::st((T*)a.r + off, fs[i], rn);
         acc[k] += dnorm2(rn);
