@@ -35,21 +35,31 @@
 namespace gnx {
 
 // ---- storage formats --------------------------------------------------------
-// Vectors of the inner iterations are written with ordinary stores: a batch's
-// working set (r, t, two p, xs: 201 MB for 32 systems at 64^3 in FP16 or
-// BF16) stays in the 256 MB Infinity Cache for the next kernel, where
-// non-temporal stores (kept by the FP64 kernels, 11.05 against 11.8 us per
-// iteration there) cost the FP16 batch of 32 12 % (824-832 against 738
-// RHS-solves/s; BF16 714 against 652, FP32 x16 497 against 444).  FP32 x32
-// (335 MB, past the cache) loses 5 % with them (408 against 430).
-GNX_DEVONLY void st_mix(float2* p, float2 v) { *p = v; }
-GNX_DEVONLY void st_mix(unsigned int* p, unsigned int v) { *p = v; }
+// Store policy of the inner iterations, per solve (nt, a kernel argument):
+// while a batch's working set (r, t, two p, xs) fits the 256 MB Infinity
+// Cache, ordinary stores keep it there for the next kernel (FP16 x32 at 64^3,
+// 201 MB: 824-832 against 738 RHS-solves/s with non-temporal stores; BF16
+// 714 against 652; FP32 x16 497 against 444); past it (FP32 x32 at 64^3, 335
+// MB; 96^3 batches) non-temporal stores win by 2-5 %, as for the FP64 kernels.
+// The branch is uniform and follows every load of a kernel.
+GNX_DEVONLY void st_mix(float2* p, float2 v, int nt) {
+  if (nt) {
+    gnx_f2v w = {v.x, v.y};
+    __builtin_nontemporal_store(w, (gnx_f2v*)p);
+  } else {
+    *p = v;
+  }
+}
+GNX_DEVONLY void st_mix(unsigned int* p, unsigned int v, int nt) {
+  if (nt) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
 
 struct StoreF32 {
   using T = float2;
   static constexpr bool kScaled = false;
   GNX_DEVONLY static float2 ld(const T* p, int i) { return *elem(p, i); }
-  GNX_DEVONLY static void st(T* p, int i, float2 v) { st_mix(elem(p, i), v); }
+  GNX_DEVONLY static void st(T* p, int i, float2 v, int nt) { st_mix(elem(p, i), v, nt); }
 };
 
 // bfloat16 pair (re in the low half), round-to-nearest-even on store: one
@@ -64,9 +74,9 @@ struct StoreBF16 {
     unsigned int v = *elem(p, i);
     return make_float2(up(v & 0xFFFFu), up(v >> 16));
   }
-  GNX_DEVONLY static void st(T* p, int i, float2 v) {
+  GNX_DEVONLY static void st(T* p, int i, float2 v, int nt) {
     const gnx_f32x2 w = {v.x, v.y};
-    st_mix(elem(p, i), __builtin_bit_cast(unsigned int, __builtin_convertvector(w, gnx_bf16x2)));
+    st_mix(elem(p, i), __builtin_bit_cast(unsigned int, __builtin_convertvector(w, gnx_bf16x2)), nt);
   }
 };
 
@@ -84,9 +94,9 @@ struct StoreF16 {
     const gnx_f32x2 w = __builtin_convertvector(__builtin_bit_cast(gnx_f16x2, *elem(p, i)), gnx_f32x2);
     return make_float2(w.x, w.y);
   }
-  GNX_DEVONLY static void st(T* p, int i, float2 v) {
+  GNX_DEVONLY static void st(T* p, int i, float2 v, int nt) {
     const gnx_f32x2 w = {v.x, v.y};
-    st_mix(elem(p, i), __builtin_bit_cast(unsigned int, __builtin_convertvector(w, gnx_f16x2)));
+    st_mix(elem(p, i), __builtin_bit_cast(unsigned int, __builtin_convertvector(w, gnx_f16x2)), nt);
   }
 };
 
@@ -161,6 +171,7 @@ struct MixFix {
   CGDevResult* res;
   int nrhs;
   int nb;
+  int nt;           // non-temporal stores: the working set exceeds the Infinity Cache
 };
 
 GNX_DEV double* mslot(const MixFix& a, int s, int rhs) { return a.part + ((size_t)s * a.nrhs + rhs) * a.nb; }
@@ -197,9 +208,9 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_start_imposed(MixFix a, const do
     // a scaled format stores r = s b in the first slot (a reliable update),
     // once ||b|| is known; zeros here keep every loaded value finite
     float2 bs = S::kScaled ? make_float2(0.f, 0.f) : make_float2((float)bb.x, (float)bb.y);
-    S::st((typename S::T*)a.r + off, f, bs);
-    S::st((typename S::T*)a.p0 + off, f, bs);
-    S::st((typename S::T*)a.p1 + off, f, make_float2(0.f, 0.f));
+    S::st((typename S::T*)a.r + off, f, bs, a.nt);
+    S::st((typename S::T*)a.p0 + off, f, bs, a.nt);
+    S::st((typename S::T*)a.p1 + off, f, make_float2(0.f, 0.f), a.nt);
     a.xs[off + f] = make_float2(0.f, 0.f);
     if (rhs == 0) {
       a.sm[f] = sm_in[f];
@@ -296,9 +307,9 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_null_p3(MixFix a, double tol, do
     double2 bb = a.b[off + f];
     a.x[off + f] = make_double2(0.0, 0.0);
     float2 bs = S::kScaled ? make_float2(0.f, 0.f) : make_float2((float)bb.x, (float)bb.y);
-    S::st((typename S::T*)a.r + off, f, bs);
-    S::st((typename S::T*)a.p0 + off, f, bs);
-    S::st((typename S::T*)a.p1 + off, f, make_float2(0.f, 0.f));
+    S::st((typename S::T*)a.r + off, f, bs, a.nt);
+    S::st((typename S::T*)a.p0 + off, f, bs, a.nt);
+    S::st((typename S::T*)a.p1 + off, f, make_float2(0.f, 0.f), a.nt);
     a.xs[off + f] = make_float2(0.f, 0.f);
   }
 }
@@ -392,7 +403,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_X(MixFix a, int odd, const MixSt
     for (int i = 0; i < SPT; i++) {
       if (fs[i] < 0) continue;
       a.u[off + fs[i]] = apply_Mdag(g, decode(g, fs[i]), a.sm[fs[i]], ldx);
-      S::st(pnew, fs[i], pv[i][0]);
+      S::st(pnew, fs[i], pv[i][0], a.nt);
     }
     if (lead) put_mstate(out, 1, s.iters, PH_UPDATE, nrel, 0, 0, s.rho, rmax, s);
     return;
@@ -411,9 +422,9 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_X(MixFix a, int odd, const MixSt
       ppv[d] = cfma(fb, pv[i][1 + d], rv[i][1 + d]);
       pmv[d] = cfma(fb, pv[i][4 + d], rv[i][4 + d]);
     }
-    S::st(pnew, fs[i], p0v);
+    S::st(pnew, fs[i], p0v, a.nt);
     float2 tt = apply_Mdag_fast_vals(g, st, smv[i], p0v, ppv, pmv);
-    S::st((T*)a.t + off, fs[i], tt);
+    S::st((T*)a.t + off, fs[i], tt, a.nt);
     acc += dnorm2(tt);
   }
   acc = block_sum(acc, lds);
@@ -450,7 +461,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Y(MixFix a, int odd, const MixSt
       a.x[off + f] = cadd(a.x[off + f], make_double2((double)c.x, (double)c.y));
       a.xs[off + f] = make_float2(0.f, 0.f);
       double2 rr = apply_sub_M(g, decode(g, f), a.sm[f], a.b[off + f], [&](int q) { return u[q]; });
-      S::st((T*)a.r + off, f, make_float2((float)rr.x, (float)rr.y));
+      S::st((T*)a.r + off, f, make_float2((float)rr.x, (float)rr.y), a.nt);
       acc += cnorm2(rr);
     }
     acc = block_sum(acc, lds);
@@ -478,9 +489,9 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Y(MixFix a, int odd, const MixSt
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
     if (fs[i] < 0) continue;
-    st_mix(&a.xs[off + fs[i]], cfma(al, pv[i], xv[i]));
+    st_mix(&a.xs[off + fs[i]], cfma(al, pv[i], xv[i]), a.nt);
     float2 rn = cfma(-al, w[i], rv[i]);
-    S::st(r, fs[i], rn);
+    S::st(r, fs[i], rn, a.nt);
     acc += dnorm2(rn);
   }
   acc = block_sum(acc, lds);
@@ -623,7 +634,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Xe(MixFix a, int sl, const MixSt
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
     if (fs[i] < 0) continue;
-    if (FLUSH) st_mix(elem(a.xs + off, fs[i]), cfma(fa, pv[i][0], cfma(fp, ppr[i], xsv[i])));
+    if (FLUSH) st_mix(elem(a.xs + off, fs[i]), cfma(fa, pv[i][0], cfma(fp, ppr[i], xsv[i])), a.nt);
     const Site st = decode(g, fs[i]);
     float2 p0v = cfma(fb, pv[i][0], rv[i][0]);
     float2 ppv[3], pmv[3];
@@ -632,9 +643,9 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Xe(MixFix a, int sl, const MixSt
       ppv[d] = cfma(fb, pv[i][1 + d], rv[i][1 + d]);
       pmv[d] = cfma(fb, pv[i][4 + d], rv[i][4 + d]);
     }
-    S::st(pnew, fs[i], p0v);
+    S::st(pnew, fs[i], p0v, a.nt);
     float2 tt = apply_Mdag_fast_vals(g, st, smv[i], p0v, ppv, pmv);
-    S::st((T*)a.t + off, fs[i], tt);
+    S::st((T*)a.t + off, fs[i], tt, a.nt);
     acc += dnorm2(tt);
   }
   acc = block_sum_once(acc, lds2);
@@ -705,12 +716,12 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Ye(MixFix a, int sl, const MixSt
       if (f < 0) break;
       const float2 po = S::ld(pold, f);
       const float2 c = cfma(fa, po, cfma(fp, S::ld(pnext, f), a.xs[off + f]));
-      S::st(pnext, f, S::kScaled ? make_float2(fr * po.x, fr * po.y) : po);
+      S::st(pnext, f, S::kScaled ? make_float2(fr * po.x, fr * po.y) : po, a.nt);
       a.x[off + f] = cadd(a.x[off + f], make_double2((double)c.x, (double)c.y));
       a.xs[off + f] = make_float2(0.f, 0.f);
       double2 rr = apply_sub_M(g, decode(g, f), a.sm[f], a.b[off + f], [&](int q) { return u[q]; });
       if (S::kScaled) rr = make_double2(sn * rr.x, sn * rr.y);
-      S::st(r, f, make_float2((float)rr.x, (float)rr.y));
+      S::st(r, f, make_float2((float)rr.x, (float)rr.y), a.nt);
       acc += cnorm2(rr);
     }
     acc = block_sum_once(acc, lds2);
@@ -729,7 +740,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Ye(MixFix a, int sl, const MixSt
     if (fs[i] < 0) continue;
     const float2 w = apply_M_vals(g, decode(g, fs[i]), smv[i], tv[i][0], &tv[i][1], &tv[i][4]);
     float2 rn = cfma(-al, w, rv[i]);
-    S::st(r, fs[i], rn);
+    S::st(r, fs[i], rn, a.nt);
     acc += dnorm2(rn);
   }
   acc = block_sum_once(acc, lds2);
@@ -866,7 +877,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_X2(MixFix a, int odd, const MixS
       for (int i = 0; i < SPT; i++) {
         if (fs[i] < 0) continue;
         a.u[off + fs[i]] = apply_Mdag(g, decode(g, fs[i]), a.sm[fs[i]], ldx);
-        S::st(pnew[k], fs[i], pv[k][i][0]);
+        S::st(pnew[k], fs[i], pv[k][i][0], a.nt);
       }
       if (lead) put_mstate(sout + rr[k], 1, sk[k]->iters, PH_UPDATE, nrel[k], 0, 0, sk[k]->rho, rmax[k], *sk[k]);
     } else if (mode[k] == MM_NORMAL) {
@@ -882,9 +893,9 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_X2(MixFix a, int odd, const MixS
           ppv[d] = cfma(fb, pv[k][i][1 + d], rv[k][i][1 + d]);
           pmv[d] = cfma(fb, pv[k][i][4 + d], rv[k][i][4 + d]);
         }
-        S::st(pnew[k], fs[i], p0v);
+        S::st(pnew[k], fs[i], p0v, a.nt);
         float2 tt = apply_Mdag_fast_vals(g, st, smv[i], p0v, ppv, pmv);
-        S::st((T*)a.t + off, fs[i], tt);
+        S::st((T*)a.t + off, fs[i], tt, a.nt);
         acc[k] += dnorm2(tt);
       }
     }
@@ -978,7 +989,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Y2(MixFix a, int odd, const MixS
         a.x[off + f] = cadd(a.x[off + f], make_double2((double)c.x, (double)c.y));
         a.xs[off + f] = make_float2(0.f, 0.f);
         double2 rn = apply_sub_M(g, st[i], a.sm[f], a.b[off + f], [&](int q) { return u[q]; });
-        S::st((T*)a.r + off, f, make_float2((float)rn.x, (float)rn.y));
+        S::st((T*)a.r + off, f, make_float2((float)rn.x, (float)rn.y), a.nt);
         acc[k] += cnorm2(rn);
       }
     } else if (mode[k] == MM_NORMAL) {
@@ -986,9 +997,9 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Y2(MixFix a, int odd, const MixS
 #pragma unroll
       for (int i = 0; i < SPT; i++) {
         if (fs[i] < 0) continue;
-        st_mix(&a.xs[off + fs[i]], cfma(al, pv[k][i], xv[k][i]));
+        st_mix(&a.xs[off + fs[i]], cfma(al, pv[k][i], xv[k][i]), a.nt);
         float2 rn = cfma(-al, w[k][i], rv[k][i]);
-        S::st((T*)a.r + off, fs[i], rn);
+        S::st((T*)a.r + off, fs[i], rn, a.nt);
         acc[k] += dnorm2(rn);
       }
     }
@@ -1115,7 +1126,7 @@ static const MixShape* mix_shapes() {
 #undef GNX_MIX_EARLY
 constexpr int kMixShapes = 16;
 
-static MixFix mix_fix(MixMachine* m, const Geom& g, int nrhs, int spt) {
+static MixFix mix_fix(MixMachine* m, const Geom& g, int nrhs, int spt, int store_bytes) {
   MixFix a;
   a.g = g;
   a.sm = m->sm;
@@ -1133,13 +1144,16 @@ static MixFix mix_fix(MixMachine* m, const Geom& g, int nrhs, int spt) {
   a.res = m->res;
   a.nrhs = nrhs;
   a.nb = (g.V + spt * MIX_BS - 1) / (spt * MIX_BS);
+  // r, t, two p in the storage format and xs in FP32, per site and system
+  const double ws = (double)g.V * nrhs * (4.0 * store_bytes + 8.0);
+  a.nt = ws > 240.0 * (1 << 20) ? 1 : 0;
   return a;
 }
 
 template <typename S>
 static hipError_t mix_graph(MixMachine* m, const Geom& g, int nrhs, int shape, int tag, hipGraphExec_t* exec) {
   const MixShape& sh = mix_shapes<S>()[shape];
-  MixFix a = mix_fix(m, g, nrhs, sh.spt);
+  MixFix a = mix_fix(m, g, nrhs, sh.spt, (int)sizeof(typename S::T));
   dim3 grid(a.nb, sh.pair ? (nrhs + 1) / 2 : nrhs), blk(MIX_BS);
   const MixState* s0 = a.state;
   const MixState* s1 = a.state + nrhs;
@@ -1168,7 +1182,7 @@ static hipError_t mix_begin(MixMachine* m, const Geom& g, const double* sm, cons
                             double tol, double delta, long max_iter, int start, int nrhs, int shape,
                             hipStream_t st) {
   const MixShape& sh = mix_shapes<S>()[shape];
-  MixFix a = mix_fix(m, g, nrhs, sh.spt);
+  MixFix a = mix_fix(m, g, nrhs, sh.spt, (int)sizeof(typename S::T));
   dim3 grid(a.nb, nrhs), blk(MIX_BS);
   hipError_t e;
   if (start == GNX_START_NULL) {
