@@ -66,6 +66,12 @@ GNX_DEV double* slot(const CGMFix& a, int s, int rhs) {
   return a.part + ((size_t)s * a.nrhs + rhs) * a.nb;
 }
 
+// the one-reduction form's partials: (|r|^2, |q|^2) pairs, [nrhs][nb], in
+// the space of the two slots
+GNX_DEV double2* pair_slots(const CGMFix& a, int rhs) {
+  return (double2*)a.part + (size_t)rhs * a.nb;
+}
+
 // ---- start kernels (launched per solve, outside the graph) ------------------
 // imposed start (conjugategradient.c:121-128): x = 0, r = p0 = b, partial |b|^2
 template <int SPT, bool XCD>
@@ -344,11 +350,184 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_B(CGMFix a, int odd, const CGMSt
   if (lead) put_state(out, 1, s.iters + 1, 0, 0, s.rho, s);
 }
 
+// ---- one-reduction form (Chronopoulos-Gear CG) ------------------------------
+// The same iterates as the kernels above in exact arithmetic, with both dot
+// products of an iteration taken at one point.  With A = M M^dag, s = A p
+// kept as a vector and q = M^dag r:
+//
+//   X(k): q = M^dag r; partials (|r|^2, |q|^2 = (r, A r)) as one pair per
+//         block -- a stencil with no reduction ahead of it.
+//   Y(k): gamma_k = |r|^2 and delta_k = (r, A r) from X's partials; exit
+//         test sqrt(gamma_k) < tol (or the iteration cap); beta_k = gamma_k /
+//         gamma_{k-1}, alpha_k = gamma_k / (delta_k - beta_k gamma_k /
+//         alpha_{k-1}) (beta_0 = 0, alpha_0 = gamma_0 / delta_0); then, at
+//         the site only: s = M q + beta s, p = r + beta p, x += alpha p,
+//         r -= alpha s.
+//
+// One head reduction per iteration instead of two, 18 operand loads per site
+// instead of 24, and no vector is read at a neighbour by the kernel that
+// writes it, so r, s, p, x are updated in place.  Per-site arithmetic: x
+// and p as in conjugategradient.c (x += alpha p, p = r + beta p); r by the
+// recurrence r -= alpha s instead of r -= alpha M t.  Buffers: q in t, s in
+// p1.  The start kernels are the two-reduction form's (x, r = b); their
+// partials are not read.  X reads state[0] and writes state[1], Y the
+// reverse.
+
+template <int SPT, bool XCD, bool LINK, bool WH>
+__global__ __launch_bounds__(CGM_BS) void k_cg1_X(CGMFix a, int odd, const CGMState* __restrict__ sin,
+                                                   CGMState* __restrict__ sout) {
+  __shared__ double lds[2 * CGM_BS / 64];
+  (void)odd;
+  const Geom g = a.g;
+  const int rhs = blockIdx.y;
+  const size_t off = (size_t)rhs * g.V;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const double2* r = a.r + off;
+  double2* qo = a.t + off;
+  const SiteMap<SPT, XCD> map(a.nb);
+  const CGMState s = load_state(sin + rhs);
+  end_of_loads();
+  int fs[SPT];
+  double smv[SPT];
+  double2 rv[SPT][7];
+  double2 Uo[LINK ? SPT : 1][3], Um[LINK ? SPT : 1][3];
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    fs[i] = map.f(g, i);
+    // a thread past the last site loads site 0 and stores nothing
+    const int fc = fs[i] < 0 ? 0 : fs[i];
+    NbrF n = neighbours_f(g, decode(g, fc));
+    int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
+    smv[i] = *elem(a.sm, fc);
+    if (LINK) load_links(a.U, fc, n, Uo[LINK ? i : 0], Um[LINK ? i : 0]);
+#pragma unroll
+    for (int k = 0; k < 7; k++) rv[i][k] = *elem(r, q[k]);
+  }
+  end_of_loads();
+  CGMState* out = sout + rhs;
+  if (!s.active) {
+    if (lead) put_state1(out, 0, s.iters, s.converged, s.early, s.rho, s.alpha, s.beta, s);
+    return;
+  }
+  double ag = 0.0, ad = 0.0;
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    if (fs[i] < 0) continue;
+    const Site st = decode(g, fs[i]);
+    const double2 qq = LINK ? link_hops<-1>(g, st, rmul(smv[i], rv[i][0]), &rv[i][1], &rv[i][4], Uo[LINK ? i : 0],
+                                            Um[LINK ? i : 0])
+                            : apply_Mdag_fast_vals(g, st, smv[i], rv[i][0], &rv[i][1], &rv[i][4]);
+    st_nt(elem(qo, fs[i]), qq);
+    ag += cnorm2(rv[i][0]);
+    ad += cnorm2(qq);
+  }
+  const double2 t = block_sum2(ag, ad, lds);
+  if (threadIdx.x == 0) pair_slots(a, rhs)[partial_slot(blockIdx.x, a.nb)] = t;
+  if (lead) put_state1(out, 1, s.iters, 0, 0, s.rho, s.alpha, s.beta, s);
+}
+
+template <int SPT, bool XCD, bool LINK, bool WH>
+__global__ __launch_bounds__(CGM_BS) void k_cg1_Y(CGMFix a, int odd, const CGMState* __restrict__ sin,
+                                                   CGMState* __restrict__ sout) {
+  __shared__ double lds[2 * CGM_BS / 64];
+  (void)odd;
+  const Geom g = a.g;
+  const int rhs = blockIdx.y;
+  const size_t off = (size_t)rhs * g.V;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const double2* qv = a.t + off;
+  double2* r = a.r + off;
+  double2* x = a.x + off;
+  double2* p = a.p0 + off;
+  double2* sv = a.p1 + off;
+  const double2* pp = pair_slots(a, rhs);
+  const SiteMap<SPT, XCD> map(a.nb);
+  // loads in the order they are needed: state and partials, then the operands
+  const CGMState s = load_state(sin + rhs);
+  double2 pv2[HEAD_W];
+  head_issue2<WH>(pp, a.nb, pv2);
+  end_of_loads();
+  int fs[SPT];
+  double smv[SPT];
+  double2 tv[SPT][7], rv[SPT], xv[SPT], pv[SPT], so[SPT];
+  double2 Uo[LINK ? SPT : 1][3], Um[LINK ? SPT : 1][3];
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    fs[i] = map.f(g, i);
+    const int fc = fs[i] < 0 ? 0 : fs[i];
+    NbrF n = neighbours_f(g, decode(g, fc));
+    int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
+    smv[i] = *elem(a.sm, fc);
+    if (LINK) load_links(a.U, fc, n, Uo[LINK ? i : 0], Um[LINK ? i : 0]);
+#pragma unroll
+    for (int k = 0; k < 7; k++) tv[i][k] = *elem(qv, q[k]);
+    rv[i] = *elem(r, fc);
+    xv[i] = *elem(x, fc);
+    pv[i] = *elem(p, fc);
+    so[i] = *elem(sv, fc);
+  }
+  end_of_loads();
+  const double2 gd = head_finish2<WH>(pp, a.nb, pv2, lds);
+  const double gam = gd.x, del = gd.y;
+  CGMState* out = sout + rhs;
+  if (!s.active) {
+    if (lead) put_state1(out, 0, s.iters, s.converged, s.early, s.rho, s.alpha, s.beta, s);
+    return;
+  }
+  const int it = s.iters;
+  double al, be;
+  if (it == 0) {
+    // ||b|| < tol returns before the loop only in the imposed start
+    // (conjugategradient.c:123-128); the null start tested its prologue
+    // residual instead (:22)
+    if (s.start == GNX_START_IMPOSED && sqrt(gam) < s.tol) {
+      if (lead) {
+        put_state1(out, 0, 0, 1, 1, gam, 0.0, 0.0, s);
+        CGDevResult res = {1, 1, 0, 0, sqrt(gam), {0, 0, 0}};
+        a.res[rhs] = res;
+      }
+      return;
+    }
+    al = gam / del;
+    be = 0.0;
+  } else {
+    const bool conv = sqrt(gam) < s.tol;
+    if (conv || it >= s.max_iter) {
+      if (lead) {
+        put_state1(out, 0, it, conv ? 1 : 0, 0, gam, s.alpha, s.beta, s);
+        CGDevResult res = {conv ? 1 : 0, 0, it, 0, sqrt(gam), {0, 0, 0}};
+        a.res[rhs] = res;
+      }
+      return;
+    }
+    be = gam / s.rho;
+    al = gam / (del - be * gam / s.alpha);
+  }
+  // the first iteration has no previous direction: p = r, s = A r
+  const bool first = it == 0;
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    if (fs[i] < 0) continue;
+    const Site st = decode(g, fs[i]);
+    const double2 m0 = rmul(smv[i], tv[i][0]);
+    const double2 w = LINK ? link_hops<1>(g, st, m0, &tv[i][1], &tv[i][4], Uo[LINK ? i : 0], Um[LINK ? i : 0])
+                           : apply_M_vals(g, st, smv[i], tv[i][0], &tv[i][1], &tv[i][4]);
+    const double2 sn = first ? w : cadd(w, rmul(be, so[i]));
+    const double2 pn = first ? rv[i] : cadd(rv[i], rmul(be, pv[i]));
+    st_nt(elem(sv, fs[i]), sn);
+    st_nt(elem(p, fs[i]), pn);
+    st_nt(elem(x, fs[i]), cadd(xv[i], rmul(al, pn)));
+    st_nt(elem(r, fs[i]), cadd(rv[i], rmul(-al, sn)));
+  }
+  if (lead) put_state1(out, 1, it + 1, 0, 0, gam, al, be, s);
+}
+
 // ---- shapes the tuner chooses from -----------------------------------------------
 struct CGMShape {
   int spt;
   bool xcd;
   bool link;
+  bool one;        // one-reduction form: A = k_cg1_X, B = k_cg1_Y
   const void* A;   // block-wide head reduction
   const void* B;
   const void* Aw;  // per-wave head reduction (up to 512 partials)
@@ -360,16 +539,27 @@ struct CGMShape {
 };
 
 #define GNX_SHAPE(S, X, K)                                                                                  \
-  {S, X, K, (const void*)k_cgm_A<S, X, K, false>, (const void*)k_cgm_B<S, X, K, false>,                      \
+  {S, X, K, false, (const void*)k_cgm_A<S, X, K, false>, (const void*)k_cgm_B<S, X, K, false>,               \
    (const void*)k_cgm_A<S, X, K, true>, (const void*)k_cgm_B<S, X, K, true>, (const void*)k_cgm_start_imposed<S, X>, \
    (const void*)k_cgm_null_p1<S, X, K>, (const void*)k_cgm_null_p2<S, X, K>, (const void*)k_cgm_null_p3<S, X>}
-// shapes 0..5: the live operator; 6..11: the link-coupled operator
+#define GNX_SHAPE1(S, X, K)                                                                                 \
+  {S, X, K, true, (const void*)k_cg1_X<S, X, K, false>, (const void*)k_cg1_Y<S, X, K, false>,                \
+   (const void*)k_cg1_X<S, X, K, true>, (const void*)k_cg1_Y<S, X, K, true>, (const void*)k_cgm_start_imposed<S, X>, \
+   (const void*)k_cgm_null_p1<S, X, K>, (const void*)k_cgm_null_p2<S, X, K>, (const void*)k_cgm_null_p3<S, X>}
+// per operator (the live operator first, then the link-coupled one): 0..5
+// the two-reduction kernels, 6..11 the one-reduction kernels, each with 1, 1,
+// 2, 2, 4, 4 sites per thread, odd indices with the XCD slab mapping
 static const CGMShape kShapes[] = {
-    GNX_SHAPE(1, false, false), GNX_SHAPE(1, true, false), GNX_SHAPE(2, false, false),
-    GNX_SHAPE(2, true, false),  GNX_SHAPE(4, false, false), GNX_SHAPE(4, true, false),
-    GNX_SHAPE(1, false, true),  GNX_SHAPE(1, true, true),   GNX_SHAPE(2, false, true),
-    GNX_SHAPE(2, true, true),   GNX_SHAPE(4, false, true),  GNX_SHAPE(4, true, true)};
+    GNX_SHAPE(1, false, false),  GNX_SHAPE(1, true, false),  GNX_SHAPE(2, false, false),
+    GNX_SHAPE(2, true, false),   GNX_SHAPE(4, false, false), GNX_SHAPE(4, true, false),
+    GNX_SHAPE1(1, false, false), GNX_SHAPE1(1, true, false), GNX_SHAPE1(2, false, false),
+    GNX_SHAPE1(2, true, false),  GNX_SHAPE1(4, false, false), GNX_SHAPE1(4, true, false),
+    GNX_SHAPE(1, false, true),   GNX_SHAPE(1, true, true),   GNX_SHAPE(2, false, true),
+    GNX_SHAPE(2, true, true),    GNX_SHAPE(4, false, true),  GNX_SHAPE(4, true, true),
+    GNX_SHAPE1(1, false, true),  GNX_SHAPE1(1, true, true),  GNX_SHAPE1(2, false, true),
+    GNX_SHAPE1(2, true, true),   GNX_SHAPE1(4, false, true), GNX_SHAPE1(4, true, true)};
 #undef GNX_SHAPE
+#undef GNX_SHAPE1
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 constexpr int kShapesPerOp = kNumShapes / 2;
 
@@ -473,15 +663,17 @@ static hipError_t cgm_graph(CGMachine* m, const Geom& g, int nrhs, int shape, hi
   const double* q1 = slot(a, 1, 0);
   double* o0 = a.part;
   double* o1 = a.part + (size_t)nrhs * a.nb;
+  const CGMShape& sh = kShapes[shape];
+  const bool wh = head_wave(a.nb);
   auto body = [&](hipStream_t cs) {
     for (int i = 0; i < ENG_CHUNK; i++) {
       int odd = i & 1;
-      // A: state[0] -> state[1], reduces slot 1, writes slot 0; B the reverse
+      // A: state[0] -> state[1], reduces slot 1, writes slot 0; B the reverse.
+      // The one-reduction kernels take (a, odd, state in, state out) only.
       void* argsA[] = {(void*)&a, (void*)&odd, (void*)&s0, (void*)&w1, (void*)&q1, (void*)&o0};
       void* argsB[] = {(void*)&a, (void*)&odd, (void*)&s1, (void*)&w0, (void*)&q0, (void*)&o1};
-      const bool wh = head_wave(a.nb);
-      (void)hipLaunchKernel(wh ? kShapes[shape].Aw : kShapes[shape].A, grid, blk, argsA, 0, cs);
-      (void)hipLaunchKernel(wh ? kShapes[shape].Bw : kShapes[shape].B, grid, blk, argsB, 0, cs);
+      (void)hipLaunchKernel(wh ? sh.Aw : sh.A, grid, blk, argsA, 0, cs);
+      (void)hipLaunchKernel(wh ? sh.Bw : sh.B, grid, blk, argsB, 0, cs);
     }
   };
   return m->eng.graph(shape, nrhs, body, exec);
@@ -592,7 +784,7 @@ int cgm_shape_of(void* machine, int nrhs, int* spt, int* xcd) {
     if (m->eng.tuned_tag[i] >= 0 && m->eng.tuned_nrhs[i] == nrhs) {
       int sh = m->eng.tuned_tag[i];
       if (spt) *spt = kShapes[sh].spt;
-      if (xcd) *xcd = kShapes[sh].xcd ? 1 : 0;
+      if (xcd) *xcd = (kShapes[sh].xcd ? 1 : 0) | (kShapes[sh].one ? 2 : 0);
       return sh;
     }
   return -1;

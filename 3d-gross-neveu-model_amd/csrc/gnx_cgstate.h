@@ -17,6 +17,11 @@ struct CGMState {
   long max_iter;
   int start;
   int pad;
+  // one-reduction form only: the step length and direction coefficient of
+  // the next iteration (its own 16-byte line: the two-reduction kernels
+  // never load it)
+  double alpha;
+  double beta;
 };
 
 // field-wise state write (an aggregate copy makes the compiler stage the
@@ -32,6 +37,13 @@ GNX_DEVONLY void put_state(CGMState* o, int active, int iters, int conv, int ear
   o->max_iter = cfg.max_iter;
   o->start = cfg.start;
   o->pad = cfg.pad;  // every loaded word stays live: a dead load destination is waited on early
+}
+
+GNX_DEVONLY void put_state1(CGMState* o, int active, int iters, int conv, int early, double rho, double alpha,
+                            double beta, const CGMState& cfg) {
+  put_state(o, active, iters, conv, early, rho, cfg);
+  o->alpha = alpha;
+  o->beta = beta;
 }
 
 }  // namespace gnx

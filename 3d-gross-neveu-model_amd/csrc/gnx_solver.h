@@ -154,6 +154,62 @@ GNX_DEVONLY double head_finish(const double* __restrict__ src, int n, const doub
   }
 }
 
+// The same for n pairs of partials (two sums taken at one point), one
+// 16-byte load per pair; each component is summed in the order of
+// head_issue/head_finish.  The block form's lds must hold 2 * ENG_BS / 64
+// doubles.
+template <bool WH>
+GNX_DEVONLY void head_issue2(const double2* __restrict__ src, int n, double2 v[HEAD_W]) {
+  if constexpr (WH) {
+    const int lane = (int)(threadIdx.x & 63);
+#pragma unroll
+    for (int k = 0; k < HEAD_W; k++) {
+      const int i = k * 64 + lane;
+      v[k] = src[i < n ? i : n - 1];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < PART_K; k++) {
+      const int i = k * ENG_BS + (int)threadIdx.x;
+      v[k] = src[i < n ? i : n - 1];
+    }
+  }
+}
+template <bool WH>
+GNX_DEVONLY double2 head_finish2(const double2* __restrict__ src, int n, const double2 v[HEAD_W], double* lds) {
+  double ax = 0.0, ay = 0.0;
+  if constexpr (WH) {
+    const int lane = (int)(threadIdx.x & 63);
+#pragma unroll
+    for (int k = 0; k < HEAD_W; k++) {
+      const bool in = k * 64 + lane < n;
+      ax += in ? v[k].x : 0.0;
+      ay += in ? v[k].y : 0.0;
+    }
+    return make_double2(wave_sum(ax), wave_sum(ay));
+  } else {
+#pragma unroll
+    for (int k = 0; k < PART_K; k++) {
+      const bool in = k * ENG_BS + (int)threadIdx.x < n;
+      ax += in ? v[k].x : 0.0;
+      ay += in ? v[k].y : 0.0;
+    }
+    for (int base = PART_K * ENG_BS; base < n; base += PART_K * ENG_BS) {
+      double2 w[PART_K];
+#pragma unroll
+      for (int k = 0; k < PART_K; k++) {
+        const int i = base + k * ENG_BS + (int)threadIdx.x;
+        w[k] = i < n ? src[i] : make_double2(0.0, 0.0);
+      }
+#pragma unroll
+      for (int k = 0; k < PART_K; k++) {
+        ax += w[k].x;
+        ay += w[k].y;
+      }
+    }
+    return block_sum2(ax, ay, lds);
+  }
+}
 // Marks the end of a kernel's load phase: the scheduler moves no
 // instruction across it, so no arithmetic on a loaded value (and no wait for
 // one) lands between the loads.

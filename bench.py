@@ -46,8 +46,19 @@ def kernel_sources_sha():
 
 
 def shape_tag(shape):
-    # kernel template arguments: <sites per thread, XCD slab mapping, link-coupled operator>
-    return f"<{shape['sites_per_thread']}, {'true' if shape['xcd_slab'] else 'false'}, false>"
+    # kernel template arguments: <sites per thread, XCD slab mapping, link-coupled operator>,
+    # prefixed "cg1" for the one-reduction kernels (k_cg1_X, k_cg1_Y)
+    return (("cg1" if shape.get("one_reduction") else "")
+            + f"<{shape['sites_per_thread']}, {'true' if shape['xcd_slab'] else 'false'}, false>")
+
+
+CG_KERNELS = ("k_cgm_A", "k_cgm_B", "k_cg1_X", "k_cg1_Y")
+
+
+def cg_kernel_label(shape):
+    if shape and shape.get("one_reduction"):
+        return "k_cg1_X + k_cg1_Y (one CG iteration, one-reduction form)"
+    return "k_cgm_A + k_cgm_B (one CG iteration)"
 
 
 def recorded_traffic(L, shape):
@@ -90,7 +101,7 @@ def summarize_rocprof(kt_dir, fetch_dir, write_dir, L):
             v = float(r["Counter_Value"])
             if n == "__amd_rocclr_copyBuffer":
                 copy.append((int(r["Grid_Size"]), v))
-            elif n.startswith(("k_cgm_A", "k_cgm_B")) and int(r["Grid_Size"]) * int(n.split("<")[1][0]) == L ** 3:
+            elif n.startswith(CG_KERNELS) and int(r["Grid_Size"]) * int(n.split("<")[1][0]) == L ** 3:
                 per.setdefault(n, []).append(v)
         # the calibration's copies: the largest transfers of the largest
         # grid (a batched leg's copy-out runs on the same grid, smaller)
@@ -115,12 +126,13 @@ def summarize_rocprof(kt_dir, fetch_dir, write_dir, L):
         out.append(f"| `{n}` | {len(counters['FETCH_SIZE'][0][n])} | {fk:.0f} | {wk:.0f} | {mb:.2f} |")
         # <sites per thread, XCD slab, link-coupled, per-wave head>: the head
         # form follows from the block count, so the launch shape is the key
-        tag = "<" + ", ".join(n[n.index("<") + 1:n.rindex(">")].split(", ")[:3]) + ">"
+        tag = (("cg1" if n.startswith("k_cg1") else "")
+               + "<" + ", ".join(n[n.index("<") + 1:n.rindex(">")].split(", ")[:3]) + ">")
         bpi[tag] = bpi.get(tag, 0.0) + mb * 1e6
     alg = BYTES_CG_ITER_F64 * L ** 3
     out.append("")
     for tag, b in sorted(bpi.items()):
-        out.append(f"Shape {tag}: {b / 1e6:.2f} MB per CG iteration (A + B) = {b / alg:.2f} x the "
+        out.append(f"Shape {tag}: {b / 1e6:.2f} MB per CG iteration (both kernels) = {b / alg:.2f} x the "
                    f"{alg / 1e6:.2f} MB algorithmic ({BYTES_CG_ITER_F64} B/site).")
     rec = {"L": L, "sources_sha256": kernel_sources_sha(), "bytes_per_iteration": bpi,
            "fetch_scale": scale["FETCH_SIZE"], "write_scale": scale["WRITE_SIZE"]}
@@ -619,7 +631,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": recorded_traffic(L, ctx.cg_shape(1)),
-                     "kernel": "k_cgm_A + k_cgm_B (one CG iteration)",
+                     "kernel": cg_kernel_label(ctx.cg_shape(1)),
                      "us_per_iteration": avg_k / it * 1e6, "solve_ms": avg_k * 1e3,
                      "bytes_per_site_iter": BYTES_CG_ITER_F64},
     }

@@ -155,9 +155,12 @@ gnx_status gnx_condensate_batch(gnx_ctx *ctx, const double *sm, const double *no
 gnx_status gnx_set_reliable_delta(gnx_ctx *ctx, double delta);
 
 /* Launch shape the CG autotuner picked for batches of nrhs systems on this
- * context (sites per thread, XCD-slab block mapping); GNX_ERR_STATE if no
- * solve of that batch size has run yet.  GNX_CG_SHAPE=<n> in the environment
- * pins the shape instead of tuning. */
+ * context (sites per thread; xcd_mapped bit 0: XCD-slab block mapping, bit 1:
+ * the one-reduction (Chronopoulos-Gear) kernels); GNX_ERR_STATE if no solve
+ * of that batch size has run yet.  GNX_CG_SHAPE=<n> in the environment pins
+ * the shape instead of tuning: n = 0..5 the two-reduction kernels with 1, 1,
+ * 2, 2, 4, 4 sites per thread, n = 6..11 the one-reduction kernels likewise;
+ * odd n use the XCD slab mapping. */
 gnx_status gnx_cg_shape(gnx_ctx *ctx, int nrhs, int *sites_per_thread, int *xcd_mapped);
 /* The same for the mixed-precision solver (prec GNX_PREC_F32 or GNX_PREC_BF16);
  * pair = 1: two systems share each block's site arithmetic; loads_first = 1:

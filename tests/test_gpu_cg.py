@@ -28,9 +28,14 @@ def true_residual(lat, sm, b, x):
     return np.linalg.norm(b - lat.M(sm, lat.Mdag(sm, x)))
 
 
+@pytest.mark.parametrize("shape", [None, "1", "7"])
 @pytest.mark.parametrize("tol", ["1e-10", "1e-6", "1e-3"])
-def test_golden_L8(torch_dev, golden, P, gnx_mod, tol):
-    banner(f"cg golden 8^3 tol={tol}")
+def test_golden_L8(torch_dev, golden, P, gnx_mod, tol, shape, monkeypatch):
+    """The tuned shape, and pinned: the two-reduction (1) and the
+    one-reduction (7) kernels, both starts and the prologue early exit."""
+    banner(f"cg golden 8^3 tol={tol} shape={shape}")
+    if shape is not None:
+        monkeypatch.setenv("GNX_CG_SHAPE", shape)
     lat = P.Lattice(8)
     ctx = gnx_mod.Context(8)
     sm = dev(golden["op_sigmamass"])
@@ -94,10 +99,11 @@ def test_noconv_and_determinism(torch_dev, P, gnx_mod):
     assert np.array_equal(host(x1), host(x2))
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("shape", list(range(12)))
 def test_batch_matches_single(torch_dev, P, gnx_mod, shape, monkeypatch):
     """Every launch shape (sites per thread 1/2/4, with and without the XCD
-    slab mapping) against the oracle; with the shape fixed, a batch of 3 is
+    slab mapping; 0..5 the two-reduction kernels, 6..11 the one-reduction
+    kernels) against the oracle; with the shape fixed, a batch of 3 is
     bit-identical to 3 single solves (same partial sums, same order)."""
     banner(f"cg batch of 3 == 3 single solves, shape {shape}")
     monkeypatch.setenv("GNX_CG_SHAPE", str(shape))
@@ -113,7 +119,8 @@ def test_batch_matches_single(torch_dev, P, gnx_mod, shape, monkeypatch):
     X = ctx.vec(3)
     st, res = ctx.cg_batch(sm, B, X, 3, 1e-10, start=gnx_mod.START_IMPOSED)
     assert st == 0
-    assert ctx.cg_shape(3) == {"sites_per_thread": [1, 1, 2, 2, 4, 4][shape], "xcd_slab": shape % 2 == 1}
+    assert ctx.cg_shape(3) == {"sites_per_thread": [1, 1, 2, 2, 4, 4][shape % 6], "xcd_slab": shape % 2 == 1,
+                               "one_reduction": shape >= 6}
     for k in range(3):
         x = ctx.vec()
         s1, r1 = ctx.cg(sm, dev(bs[k]), x, 1e-10, start=gnx_mod.START_IMPOSED)

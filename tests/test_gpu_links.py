@@ -84,9 +84,13 @@ def test_link_forces_vs_oracle(P, gnx_mod):
     assert abs(p - po) <= 1e-13 * max(1.0, abs(po))
 
 
+@pytest.mark.parametrize("shape", [None, "7"])
 @pytest.mark.parametrize("start", ["null", "imposed"])
-def test_link_cg_vs_oracle(P, gnx_mod, start):
-    banner(f"link cg 8^3 {start}")
+def test_link_cg_vs_oracle(P, gnx_mod, start, shape, monkeypatch):
+    """The tuned shape and, pinned, the one-reduction kernels (shape 7)."""
+    banner(f"link cg 8^3 {start} shape={shape}")
+    if shape is not None:
+        monkeypatch.setenv("GNX_CG_SHAPE", shape)
     import torch
     lat, r, sm, U = setup(P, 8, 8, 33)
     V = lat.V
