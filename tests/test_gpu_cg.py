@@ -80,8 +80,14 @@ def test_vs_oracle(torch_dev, P, gnx_mod, L):
     assert true_residual(lat, sm, b, host(x)) <= max(1e-9, 10 * true_residual(lat, sm, b, xr))
 
 
-def test_noconv_and_determinism(torch_dev, P, gnx_mod):
-    banner("cg max_iter cap and determinism")
+@pytest.mark.parametrize("shape", [None, "9"])
+def test_noconv_and_determinism(torch_dev, P, gnx_mod, shape, monkeypatch):
+    """The tuned shape and, pinned, the one-reduction kernels (shape 9): the
+    cap stops after exactly max_iter iterations, repeated solves agree
+    bit-for-bit."""
+    banner(f"cg max_iter cap and determinism, shape={shape}")
+    if shape is not None:
+        monkeypatch.setenv("GNX_CG_SHAPE", shape)
     L = 16
     lat = P.Lattice(L)
     r = P.Random(5)
