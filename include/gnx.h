@@ -28,7 +28,8 @@
 extern "C" {
 #endif
 
-#define GNX_ABI_VERSION 3
+/* 4: gnx_cg_shape reports the one-reduction kernels in bit 1 of xcd_mapped */
+#define GNX_ABI_VERSION 4
 
 typedef struct gnx_ctx gnx_ctx;
 typedef struct gnx_multi gnx_multi;
