@@ -23,9 +23,14 @@ def relerr(a, b):
     return np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))
 
 
+@pytest.mark.parametrize("shape", [None, "7"])
 @pytest.mark.parametrize("tag,eps,steps,ntraj", [("hmc_e0.1", 0.1, 2, 20), ("hmc_e0.05s4", 0.05, 4, 5)])
-def test_hmc_golden(golden, P, gnx_mod, tag, eps, steps, ntraj):
-    banner(f"hmc {tag}")
+def test_hmc_golden(golden, P, gnx_mod, tag, eps, steps, ntraj, shape, monkeypatch):
+    """The tuned CG shapes and, pinned, the one-reduction kernels (shape 7)
+    in every solve of the trajectory."""
+    banner(f"hmc {tag} shape={shape}")
+    if shape is not None:
+        monkeypatch.setenv("GNX_CG_SHAPE", shape)
     import torch
     assert torch.cuda.is_available()
     L = 8
