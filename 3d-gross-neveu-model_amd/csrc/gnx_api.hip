@@ -20,20 +20,6 @@ void set_err(const char* fmt, ...) {
 }
 }  // namespace gnx
 
-namespace gnx {
-Geom make_geom(int L, int LT) {
-  Geom g;
-  g.L = L;
-  g.LT = LT;
-  g.V = L * L * LT;
-  g.half = g.V / 2;
-  g.LLT = L * LT;
-  g.magic_LT = (unsigned)((((unsigned long long)1 << 32) + LT - 1) / LT);
-  g.magic_L = (unsigned)((((unsigned long long)1 << 32) + L - 1) / L);
-  return g;
-}
-}  // namespace gnx
-
 // geometry table export (f-order, reference layout)
 __global__ void k_geometry_tables(Geom g, int* fpp, int* fmm, double* eta_pp, double* eta_mm, int* f2lex) {
   int f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -73,6 +59,10 @@ gnx_status gnx_create(int lspace, int ltime, int device, gnx_ctx** out) {
   long V = (long)lspace * lspace * ltime;
   if (V > (1L << 26)) {
     set_err("gnx_create: volume %ld exceeds 2^26 sites", V);
+    return GNX_ERR_SIZE;
+  }
+  if (!geom_decodable(lspace, ltime)) {
+    set_err("gnx_create: %d^2 x %d is outside the exact range of the in-kernel site decode", lspace, ltime);
     return GNX_ERR_SIZE;
   }
   int ndev = 0;
@@ -264,9 +254,23 @@ gnx_status gnx_set_reliable_delta(gnx_ctx* c, double delta) {
 gnx_status gnx_cg_shape(gnx_ctx* c, int nrhs, int* spt, int* xcd) {
   CHECK_ARG(c, "gnx_cg_shape: null context");
   if (cgm_shape_of(c->cgm, nrhs, spt, xcd) < 0) {
-    set_err("gnx_cg_shape: no tuned shape for nrhs=%d yet", nrhs);
+    set_err("gnx_cg_shape: no shape chosen for nrhs=%d yet", nrhs);
     return GNX_ERR_STATE;
   }
+  return GNX_OK;
+}
+
+gnx_status gnx_set_autotune(gnx_ctx* c, int on) {
+  CHECK_ARG(c, "gnx_set_autotune: null context");
+  cgm_set_autotune(&c->cgm, on ? 1 : 0);
+  cgmix_set_autotune(&c->cgmix, on ? 1 : 0);
+  return GNX_OK;
+}
+
+gnx_status gnx_tune_runs(gnx_ctx* c, long* fp64, long* mixed) {
+  CHECK_ARG(c && fp64 && mixed, "gnx_tune_runs: null argument");
+  *fp64 = cgm_tune_runs(c->cgm);
+  *mixed = cgmix_tune_runs(c->cgmix);
   return GNX_OK;
 }
 
@@ -274,7 +278,7 @@ gnx_status gnx_cg_mixed_shape(gnx_ctx* c, int nrhs, gnx_prec prec, int* spt, int
                               int* loads_first) {
   CHECK_ARG(c && spt && xcd && pair && loads_first, "gnx_cg_mixed_shape: null argument");
   if (cgmix_shape_of(c->cgmix, nrhs, (int)prec, spt, xcd, pair, loads_first) < 0) {
-    set_err("gnx_cg_mixed_shape: no tuned shape for nrhs=%d yet", nrhs);
+    set_err("gnx_cg_mixed_shape: no shape chosen for nrhs=%d yet", nrhs);
     return GNX_ERR_STATE;
   }
   return GNX_OK;

@@ -581,6 +581,7 @@ struct CGMachine {
   CGDevResult* res = nullptr;
   long cap = 0;          // complex elements per vector
   int forced_shape = -1; // GNX_CG_SHAPE (index within an operator's shapes)
+  int autotune = 0;      // 1: choose shapes by timing (GNX_AUTOTUNE=1, gnx_set_autotune)
   void free_vectors() {
     void* bufs[] = {sm, U, x, r, p0, p1, t, res};
     for (void* p : bufs)
@@ -604,6 +605,8 @@ static CGMachine* cgm_get(void** slotp) {
       int v = atoi(env);
       if (v >= 0 && v < kShapesPerOp) m->forced_shape = v;
     }
+    const char* at = getenv("GNX_AUTOTUNE");
+    if (at && atoi(at) != 0) m->autotune = 1;
     *slotp = m;
   }
   return (CGMachine*)*slotp;
@@ -704,25 +707,42 @@ static hipError_t cgm_begin(CGMachine* m, const Geom& g, const double* sm, const
   return e;
 }
 
-// Autotuner: time every launch shape on 2*CHUNK iterations of the caller's
-// own system (imposed start into the solver's buffers; the caller's x is not
-// touched) and keep the fastest per batch size.
-static hipError_t cgm_tune(CGMachine* m, const Geom& g, const double* sm, const double2* U, const double2* b,
-                           int nrhs, hipStream_t st, int* best) {
+// Launch shape of a solve.  By default a fixed rule, so that a solve's
+// iterates (which depend on the form and on the partial-sum partition) are
+// the same in every process: the one-reduction kernels in the XCD slab
+// mapping with the most sites per thread (4, 2, 1) that still gives the
+// batch 512 blocks of at least 8 per system -- the shape the timing tuner
+// picks at 64^3 single and batched.  GNX_CG_SHAPE pins a shape; autotune
+// (GNX_AUTOTUNE=1 or gnx_set_autotune) times every shape on 2*CHUNK
+// iterations of the caller's own system (imposed start into the solver's
+// buffers; the caller's x is not touched) and keeps the fastest per key.
+static int cgm_rule(const Geom& g, int nrhs) {
+  for (int k = 2; k >= 1; k--) {
+    const long nb = (g.V + (long)CGM_BS * (1 << k) - 1) / ((long)CGM_BS * (1 << k));
+    if (nb >= NUM_XCD && nb * nrhs >= 512) return 7 + 2 * k;
+  }
+  return 7;
+}
+
+static int cgm_key(int nrhs, bool link) { return nrhs + (link ? 1000 : 0); }
+
+static hipError_t cgm_choose(CGMachine* m, const Geom& g, const double* sm, const double2* U, const double2* b,
+                             int nrhs, hipStream_t st, int* best) {
   GraphEngine& E = m->eng;
   const bool link = U != nullptr;
-  const int key = nrhs + (link ? 1000 : 0);
+  const int key = cgm_key(nrhs, link);
   const int first = link ? kShapesPerOp : 0;
-  for (int i = 0; i < 8; i++)
-    if (E.tuned_tag[i] >= 0 && E.tuned_nrhs[i] == key) {
-      *best = E.tuned_tag[i];
-      return hipSuccess;
-    }
-  int best_shape = first;
+  const int cached = E.shapes.find(key);
+  if (cached >= 0) {
+    *best = cached;
+    return hipSuccess;
+  }
+  int best_shape = first + cgm_rule(g, nrhs);
   hipError_t e = hipSuccess;
   if (m->forced_shape >= 0) {
     best_shape = first + m->forced_shape;
-  } else {
+  } else if (m->autotune) {
+    E.tune_runs++;
     float best_ms = 1e30f;
     for (int sh = first; sh < first + kShapesPerOp && e == hipSuccess; sh++) {
       hipGraphExec_t exec;
@@ -742,12 +762,7 @@ static hipError_t cgm_tune(CGMachine* m, const Geom& g, const double* sm, const 
     }
     if (e != hipSuccess) return e;
   }
-  for (int i = 0; i < 8; i++)
-    if (E.tuned_tag[i] < 0) {
-      E.tuned_tag[i] = best_shape;
-      E.tuned_nrhs[i] = key;
-      break;
-    }
+  E.shapes.put(key, best_shape);
   *best = best_shape;
   return hipSuccess;
 }
@@ -759,7 +774,7 @@ hipError_t cgm_f64_run(void** machine_slot, const Geom& g, const double* sm, con
   hipError_t e = cgm_ensure(m, g, nrhs);
   if (e != hipSuccess) return e;
   int shape = 0;
-  e = cgm_tune(m, g, sm, U, b, nrhs, st, &shape);
+  e = cgm_choose(m, g, sm, U, b, nrhs, st, &shape);
   if (e != hipSuccess) return e;
   hipGraphExec_t exec;
   e = cgm_graph(m, g, nrhs, shape, &exec);
@@ -780,14 +795,15 @@ hipError_t cgm_f64_run(void** machine_slot, const Geom& g, const double* sm, con
 int cgm_shape_of(void* machine, int nrhs, int* spt, int* xcd) {
   CGMachine* m = (CGMachine*)machine;
   if (!m) return -1;
-  for (int i = 0; i < 8; i++)
-    if (m->eng.tuned_tag[i] >= 0 && m->eng.tuned_nrhs[i] == nrhs) {
-      int sh = m->eng.tuned_tag[i];
-      if (spt) *spt = kShapes[sh].spt;
-      if (xcd) *xcd = (kShapes[sh].xcd ? 1 : 0) | (kShapes[sh].one ? 2 : 0);
-      return sh;
-    }
-  return -1;
+  const int sh = m->eng.shapes.peek(cgm_key(nrhs, false));
+  if (sh < 0) return -1;
+  if (spt) *spt = kShapes[sh].spt;
+  if (xcd) *xcd = (kShapes[sh].xcd ? 1 : 0) | (kShapes[sh].one ? 2 : 0);
+  return sh;
 }
+
+void cgm_set_autotune(void** machine_slot, int on) { cgm_get(machine_slot)->autotune = on; }
+
+long cgm_tune_runs(void* machine) { return machine ? ((CGMachine*)machine)->eng.tune_runs : 0; }
 
 }  // namespace gnx

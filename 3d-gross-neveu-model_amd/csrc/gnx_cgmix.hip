@@ -1034,6 +1034,11 @@ struct MixMachine {
   void* t = nullptr;
   CGDevResult* res = nullptr;
   long cap = 0;  // complex elements per vector
+  int autotune = 0;  // 1: choose shapes by timing (GNX_AUTOTUNE=1, gnx_set_autotune)
+  MixMachine() {
+    const char* at = getenv("GNX_AUTOTUNE");
+    if (at && atoi(at) != 0) autotune = 1;
+  }
   void free_vectors() {
     void* bufs[] = {sm, sm32, b, x, u, xs, r, p0, p1, t, res};
     for (void* p : bufs)
@@ -1199,25 +1204,44 @@ static hipError_t mix_begin(MixMachine* m, const Geom& g, const double* sm, cons
   return e;
 }
 
-// Autotuner: two chunks of every shape on the caller's own system, fastest
-// kept per (batch size, storage format); GNX_MIX_SHAPE=<0..15> pins one.
+// Launch shape of a mixed solve.  By default a fixed rule (the same
+// iterates in every process): the XCD slab mapping with the most sites per
+// thread (4, 2, 1) that still gives the batch 512 blocks of at least 8 per
+// system; loads-first kernels for 2-byte storage (FP16 requires them: the
+// per-system scale lives there) and for single systems, the interleaved
+// form for FP32 batches (bandwidth-bound, where the loads-first form loses
+// occupancy) -- the tuner's picks at 64^3.  GNX_MIX_SHAPE=<0..15> pins one;
+// autotune times two chunks of every shape on the caller's own system and
+// keeps the fastest per (batch size, storage format).
 template <typename S>
-static hipError_t mix_tune(MixMachine* m, const Geom& g, const double* sm, const double2* b, double delta,
-                           int nrhs, int tag, hipStream_t st, int* best) {
+static int mix_rule(const Geom& g, int nrhs) {
+  int k = 0;
+  for (int kk = 2; kk >= 1 && k == 0; kk--) {
+    const long nb = (g.V + (long)MIX_BS * (1 << kk) - 1) / ((long)MIX_BS * (1 << kk));
+    if (nb >= NUM_XCD && nb * nrhs >= 512) k = kk;
+  }
+  const bool lf = sizeof(typename S::T) == 4 || S::kScaled || nrhs == 1;
+  return (lf ? 10 : 0) + 2 * k + 1;
+}
+
+template <typename S>
+static hipError_t mix_choose(MixMachine* m, const Geom& g, const double* sm, const double2* b, double delta,
+                             int nrhs, int tag, hipStream_t st, int* best) {
   GraphEngine& E = m->eng;
   const int key = nrhs + 1000 * tag;
-  for (int i = 0; i < 8; i++)
-    if (E.tuned_tag[i] >= 0 && E.tuned_nrhs[i] == key) {
-      *best = E.tuned_tag[i];
-      return hipSuccess;
-    }
-  int best_shape = 0;
+  const int cached = E.shapes.find(key);
+  if (cached >= 0) {
+    *best = cached;
+    return hipSuccess;
+  }
+  int best_shape = mix_rule<S>(g, nrhs);
   hipError_t e = hipSuccess;
   const char* env = getenv("GNX_MIX_SHAPE");
   if (env && atoi(env) >= 0 && atoi(env) < kMixShapes && (nrhs >= 2 || !mix_shapes<S>()[atoi(env)].pair) &&
       (!S::kScaled || mix_shapes<S>()[atoi(env)].early)) {
     best_shape = atoi(env);
-  } else {
+  } else if (m->autotune) {
+    E.tune_runs++;
     float best_ms = 1e30f;
     for (int sh = 0; sh < kMixShapes && e == hipSuccess; sh++) {
       if (mix_shapes<S>()[sh].pair && nrhs < 2) continue;
@@ -1239,12 +1263,7 @@ static hipError_t mix_tune(MixMachine* m, const Geom& g, const double* sm, const
     }
     if (e != hipSuccess) return e;
   }
-  for (int i = 0; i < 8; i++)
-    if (E.tuned_tag[i] < 0) {
-      E.tuned_tag[i] = best_shape;
-      E.tuned_nrhs[i] = key;
-      break;
-    }
+  E.shapes.put(key, best_shape);
   *best = best_shape;
   return hipSuccess;
 }
@@ -1256,7 +1275,7 @@ static hipError_t mix_run_t(MixMachine* m, const Geom& g, const double* sm, cons
   hipError_t e = mix_ensure(m, g, nrhs);
   if (e != hipSuccess) return e;
   int shape = 0;
-  e = mix_tune<S>(m, g, sm, b, delta, nrhs, tag, st, &shape);
+  e = mix_choose<S>(m, g, sm, b, delta, nrhs, tag, st, &shape);
   if (e != hipSuccess) return e;
   hipGraphExec_t exec;
   e = mix_graph<S>(m, g, nrhs, shape, tag, &exec);
@@ -1291,16 +1310,21 @@ int cgmix_shape_of(void* machine, int nrhs, int prec, int* spt, int* xcd, int* p
   MixMachine* m = (MixMachine*)machine;
   if (!m) return -1;
   const int key = nrhs + 1000 * (prec == GNX_PREC_F32 ? 100 : prec == GNX_PREC_F16 ? 102 : 101);
-  for (int i = 0; i < 8; i++)
-    if (m->eng.tuned_tag[i] >= 0 && m->eng.tuned_nrhs[i] == key) {
-      const MixShape& sh = mix_shapes<StoreF32>()[m->eng.tuned_tag[i]];
-      *spt = sh.spt;
-      *xcd = sh.xcd ? 1 : 0;
-      *pair = sh.pair ? 1 : 0;
-      *early = sh.early ? 1 : 0;
-      return m->eng.tuned_tag[i];
-    }
-  return -1;
+  const int t = m->eng.shapes.peek(key);
+  if (t < 0) return -1;
+  const MixShape& sh = mix_shapes<StoreF32>()[t];
+  *spt = sh.spt;
+  *xcd = sh.xcd ? 1 : 0;
+  *pair = sh.pair ? 1 : 0;
+  *early = sh.early ? 1 : 0;
+  return t;
 }
+
+void cgmix_set_autotune(void** machine_slot, int on) {
+  if (!*machine_slot) *machine_slot = new MixMachine();
+  ((MixMachine*)*machine_slot)->autotune = on;
+}
+
+long cgmix_tune_runs(void* machine) { return machine ? ((MixMachine*)machine)->eng.tune_runs : 0; }
 
 }  // namespace gnx

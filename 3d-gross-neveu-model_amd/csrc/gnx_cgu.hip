@@ -430,6 +430,11 @@ gnx_status gnx_multi_create(int n, const int* lspace, const int* ltime, int devi
       set_err("gnx_multi_create: system %d exceeds 2^26 sites", k);
       return GNX_ERR_SIZE;
     }
+    if (!geom_decodable(lspace[k], ltime[k])) {
+      set_err("gnx_multi_create: system %d (%d^2 x %d) is outside the exact range of the in-kernel site decode", k,
+              lspace[k], ltime[k]);
+      return GNX_ERR_SIZE;
+    }
   }
   int ndev = 0;
   HIP_TRY(hipGetDeviceCount(&ndev));

@@ -24,9 +24,36 @@ namespace gnx {
 struct Geom {
   int L, LT, V, half;
   int LLT;            // L*LT   (stride of x0 in lex)
-  unsigned magic_LT;  // ceil(2^32/LT): n/LT = umulhi(n, magic_LT) for n*LT < 2^32
+  unsigned magic_LT;  // ceil(2^32/LT): n/LT = umulhi(n, magic_LT) (exact range: geom_decodable)
   unsigned magic_L;
 };
+
+// Geometry of an L x L x LT lattice (host side).
+inline Geom make_geom(int L, int LT) {
+  Geom g;
+  g.L = L;
+  g.LT = LT;
+  g.V = L * L * LT;
+  g.half = g.V / 2;
+  g.LLT = L * LT;
+  g.magic_LT = (unsigned)((((unsigned long long)1 << 32) + LT - 1) / LT);
+  g.magic_L = (unsigned)((((unsigned long long)1 << 32) + L - 1) / L);
+  return g;
+}
+
+// decode() divides by LT and L with umulhi(n, m), m = ceil(2^32/d).  With
+// e = m d - 2^32 (0 <= e < d), umulhi(n, m) = floor(n/d) exactly while
+// n e < 2^32: n = q d + r gives n m / 2^32 = q + (r + n e / 2^32) / d, and
+// r <= d - 1.  decode() divides lex indices n < V by LT and quotients
+// q < L^2 by L; both bounds are checked here (powers of two have e = 0).
+inline bool geom_decodable(int L, int LT) {
+  const unsigned long long two32 = 1ull << 32;
+  const unsigned long long V = (unsigned long long)L * L * LT;
+  if (V > (1ull << 26)) return false;
+  const unsigned long long eLT = (unsigned long long)make_geom(L, LT).magic_LT * LT - two32;
+  const unsigned long long eL = (unsigned long long)make_geom(L, LT).magic_L * L - two32;
+  return (V - 1) * eLT < two32 && ((unsigned long long)L * L - 1) * eL < two32;
+}
 
 // Site decode: coordinates, parity and lex index of f-site f.
 struct Site {

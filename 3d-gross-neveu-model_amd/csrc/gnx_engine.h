@@ -22,6 +22,54 @@ constexpr int ENG_CHUNK = 32;   // iterations per graph replay
 // launches ENG_CHUNK iterations (every X and Y kernel) into the capture stream
 using ChunkBody = std::function<void(hipStream_t)>;
 
+// Launch shapes chosen per key (batch size with the operator or storage
+// format folded in).  When every slot is taken the least recently used
+// entry is replaced, so a context that sees many keys re-chooses only the
+// keys it evicted, never every key on every call.
+struct ShapeCache {
+  static constexpr int N = 16;
+  int key[N];
+  int shape[N];
+  unsigned long long used[N];
+  unsigned long long clock = 0;
+  ShapeCache() { clear(); }
+  void clear() {
+    for (int i = 0; i < N; i++) {
+      key[i] = -1;
+      shape[i] = -1;
+      used[i] = 0;
+    }
+  }
+  // the shape stored for k (and marks it used), or -1
+  int find(int k) {
+    for (int i = 0; i < N; i++)
+      if (key[i] == k) {
+        used[i] = ++clock;
+        return shape[i];
+      }
+    return -1;
+  }
+  // the shape stored for k without marking it used, or -1
+  int peek(int k) const {
+    for (int i = 0; i < N; i++)
+      if (key[i] == k) return shape[i];
+    return -1;
+  }
+  void put(int k, int s) {
+    int slot = 0;
+    for (int i = 0; i < N; i++) {
+      if (key[i] == k || key[i] < 0) {
+        slot = i;
+        break;
+      }
+      if (used[i] < used[slot]) slot = i;
+    }
+    key[slot] = k;
+    shape[slot] = s;
+    used[slot] = ++clock;
+  }
+};
+
 class GraphEngine {
  public:
   ~GraphEngine();
@@ -43,8 +91,8 @@ class GraphEngine {
   hipError_t time_replays(hipGraphExec_t exec, int chunks, hipStream_t st, float* ms);
 
   int last_iters = 0;
-  int tuned_tag[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
-  int tuned_nrhs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  ShapeCache shapes;   // chosen launch shape per key
+  long tune_runs = 0;  // timing-tuner runs (one per key chosen by timing)
 
  private:
   void release();

@@ -13,7 +13,6 @@ void GraphEngine::invalidate_graphs() {
       (void)hipGraphExecDestroy(g.exec);
       g = Entry();
     }
-  for (int i = 0; i < 8; i++) tuned_tag[i] = -1;
 }
 
 void GraphEngine::release() {
@@ -30,7 +29,6 @@ void GraphEngine::release() {
   capture_ = nullptr;
   cap_ = 0;
   args_bytes_ = state_bytes_ = part_doubles_ = 0;
-  for (int i = 0; i < 8; i++) tuned_tag[i] = -1;
 }
 
 hipError_t GraphEngine::ensure(int nrhs, size_t args_bytes, size_t state_bytes, size_t part_doubles) {

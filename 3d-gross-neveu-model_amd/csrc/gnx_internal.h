@@ -13,7 +13,6 @@
 
 namespace gnx {
 
-Geom make_geom(int L, int LT);
 
 hipError_t launch_sigma_mass(const Geom& g, const double* sigma, double* sm, hipStream_t st);
 hipError_t launch_evolve_sigma(const Geom& g, double* sigma, double* sm, const double* mom, double eps,
@@ -58,6 +57,10 @@ int cgmix_shape_of(void* machine, int nrhs, int prec, int* spt, int* xcd, int* p
 int multi_count(const gnx_multi* m);
 bool multi_system(const gnx_multi* m, int k, int* L, int* LT, int* device);
 int cgm_shape_of(void* machine, int nrhs, int* spt, int* xcd);
+void cgm_set_autotune(void** machine_slot, int on);
+long cgm_tune_runs(void* machine);
+void cgmix_set_autotune(void** machine_slot, int on);
+long cgmix_tune_runs(void* machine);
 
 void set_err(const char* fmt, ...);
 

@@ -28,8 +28,10 @@
 extern "C" {
 #endif
 
-/* 4: gnx_cg_shape reports the one-reduction kernels in bit 1 of xcd_mapped */
-#define GNX_ABI_VERSION 4
+/* 4: gnx_cg_shape reports the one-reduction kernels in bit 1 of xcd_mapped
+ * 5: launch shapes by a fixed rule unless autotuning is on (gnx_set_autotune,
+ *    gnx_tune_runs); geometries outside the exact site decode are refused */
+#define GNX_ABI_VERSION 5
 
 typedef struct gnx_ctx gnx_ctx;
 typedef struct gnx_multi gnx_multi;
@@ -155,13 +157,29 @@ gnx_status gnx_condensate_batch(gnx_ctx *ctx, const double *sm, const double *no
  * precision: 0.05 for FP32 and FP16, 0.2 for BF16. */
 gnx_status gnx_set_reliable_delta(gnx_ctx *ctx, double delta);
 
-/* Launch shape the CG autotuner picked for batches of nrhs systems on this
- * context (sites per thread; xcd_mapped bit 0: XCD-slab block mapping, bit 1:
- * the one-reduction (Chronopoulos-Gear) kernels); GNX_ERR_STATE if no solve
- * of that batch size has run yet.  GNX_CG_SHAPE=<n> in the environment pins
- * the shape instead of tuning: n = 0..5 the two-reduction kernels with 1, 1,
- * 2, 2, 4, 4 sites per thread, n = 6..11 the one-reduction kernels likewise;
- * odd n use the XCD slab mapping. */
+/* Launch shapes.  A solve's iterates depend on its launch shape (the
+ * one-reduction and two-reduction forms follow different recurrences, and
+ * sites per thread set the partition of every global sum), so by default the
+ * shape is a fixed function of the lattice size and batch size: the same
+ * solve gives the same bits in every process, run and restart.  With
+ * autotuning on (gnx_set_autotune(ctx, 1), or GNX_AUTOTUNE=1 in the
+ * environment when the context's first solve runs) each new (batch size,
+ * operator or storage format) is timed over every shape once and the fastest
+ * kept for the context's lifetime: faster where the rule is not the
+ * optimum, but the choice may differ between processes. */
+gnx_status gnx_set_autotune(gnx_ctx *ctx, int on);
+/* how many keys the timing tuner has chosen so far (FP64 and mixed solvers);
+ * each key is timed once per context unless it was evicted from the
+ * 16-entry cache of choices (least recently used first) */
+gnx_status gnx_tune_runs(gnx_ctx *ctx, long *fp64, long *mixed);
+/* Launch shape of the FP64 CG on the live operator (M M^dag) for batches of
+ * nrhs systems on this context (sites per thread; xcd_mapped bit 0:
+ * XCD-slab block mapping, bit 1: the one-reduction (Chronopoulos-Gear)
+ * kernels); GNX_ERR_STATE if no solve of that batch size has run yet.
+ * GNX_CG_SHAPE=<n> in the environment pins the shape: n = 0..5 the
+ * two-reduction kernels with 1, 1, 2, 2, 4, 4 sites per thread, n = 6..11 the
+ * one-reduction kernels likewise; odd n use the XCD slab mapping.  The
+ * link-coupled operator's shape is chosen the same way but not reported. */
 gnx_status gnx_cg_shape(gnx_ctx *ctx, int nrhs, int *sites_per_thread, int *xcd_mapped);
 /* The same for the mixed-precision solver (prec GNX_PREC_F32 or GNX_PREC_BF16);
  * pair = 1: two systems share each block's site arithmetic; loads_first = 1:

@@ -12,16 +12,6 @@
 
 using namespace gnx;
 
-namespace gnx {
-Geom make_geom(int L, int LT) {
-  Geom g;
-  g.L = L; g.LT = LT; g.V = L * L * LT; g.half = g.V / 2; g.LLT = L * LT;
-  g.magic_LT = (unsigned)((((unsigned long long)1 << 32) + LT - 1) / LT);
-  g.magic_L = (unsigned)((((unsigned long long)1 << 32) + L - 1) / L);
-  return g;
-}
-}  // namespace gnx
-
 static int fails = 0;
 #define EXPECT(c, ...) do { if (!(c)) { if (fails < 20) { printf(__VA_ARGS__); printf("\n"); } fails++; } } while (0)
 
@@ -99,9 +89,38 @@ static void check(int L, int LT) {
   orc_geom_free(&og);
 }
 
+// decode() against 64-bit division at every site (no oracle tables: these
+// volumes are too large for them); returns the number of wrong sites.
+static long decode_scan(int L, int LT) {
+  Geom g = make_geom(L, LT);
+  long bad = 0;
+  for (long f = 0; f < (long)g.V; f++) {
+    long P = f >= g.half ? 1 : 0;
+    long n0 = 2 * (f - P * g.half);
+    long q = n0 / LT, x0 = q / L, x1 = q % L, x2 = n0 % LT + ((x0 + x1 + P) & 1);
+    Site s = decode(g, (int)f);
+    if (s.x0 != x0 || s.x1 != x1 || s.x2 != x2 || s.P != P || s.lex != q * LT + x2) bad++;
+  }
+  return bad;
+}
+
 int main() {
   int sizes[][2] = {{4, 4}, {4, 6}, {6, 4}, {6, 10}, {8, 8}, {10, 4}, {12, 12}, {16, 8}, {16, 16}, {24, 24}, {32, 32}};
   for (auto& s : sizes) check(s[0], s[1]);
+  // the exact range of decode(): every accepted shape decodes exactly, and
+  // shapes past the range (accepted by the 2^26-site cap alone) do not
+  int accepted[][2] = {{96, 96}, {256, 1024}, {382, 382}, {388, 200}, {1354, 16}, {2188, 4}, {2048, 16}};
+  for (auto& s : accepted) {
+    EXPECT(geom_decodable(s[0], s[1]), "%d^2 x %d should be accepted", s[0], s[1]);
+    long bad = decode_scan(s[0], s[1]);
+    EXPECT(bad == 0, "%d^2 x %d: %ld sites decode wrong", s[0], s[1], bad);
+  }
+  int rejected[][2] = {{128, 4000}, {256, 1000}, {220, 1364}, {4094, 4}, {2046, 16}};
+  for (auto& s : rejected) {
+    EXPECT(!geom_decodable(s[0], s[1]), "%d^2 x %d should be rejected", s[0], s[1]);
+    EXPECT(decode_scan(s[0], s[1]) > 0, "%d^2 x %d decodes exactly: the range check is too strict here",
+           s[0], s[1]);
+  }
   if (fails) { printf("%d failures\n", fails); return 1; }
   printf("stencil_check: all sizes bit-exact\n");
   return 0;

@@ -23,7 +23,7 @@ def test_library_loads_and_exports():
     assert len(names) >= 20
     for n in names:
         assert hasattr(lib, n), f"{n} missing from libgnx.so"
-    assert lib.gnx_abi_version() == 4
+    assert lib.gnx_abi_version() == 5
 
 
 def test_library_does_not_link_oracle():
@@ -55,3 +55,19 @@ def test_argument_errors_without_a_gpu():
         "gnx_link_hmc_ensemble_trajectory")
     n = ctypes.c_int(0)
     err(lib.gnx_multi_create(0, ctypes.byref(n), ctypes.byref(n), 0, ctypes.byref(vp())), "gnx_multi_create")
+
+
+def test_geometry_outside_exact_decode_is_refused():
+    """Shapes whose lex indices overflow the kernels' 32-bit multiply-high
+    division (accepted by the 2^26-site cap alone) are refused at creation
+    with GNX_ERR_SIZE (4), before any HIP call; tests/host/stencil_check.hip
+    shows on the CPU that these shapes would decode wrong."""
+    lib = ctypes.CDLL(LIB)
+    lib.gnx_last_error.restype = ctypes.c_char_p
+    out = ctypes.c_void_p()
+    for L, LT in ((128, 4000), (256, 1000), (220, 1364), (4094, 4), (2046, 16)):
+        assert lib.gnx_create(L, LT, 0, ctypes.byref(out)) == 4, (L, LT)
+        assert b"exact range" in lib.gnx_last_error()
+        n, t = (ctypes.c_int * 2)(8, L), (ctypes.c_int * 2)(8, LT)
+        assert lib.gnx_multi_create(2, n, t, 0, ctypes.byref(out)) == 4, (L, LT)
+        assert b"system 1" in lib.gnx_last_error()

@@ -31,7 +31,7 @@ def true_residual(lat, sm, b, x):
 @pytest.mark.parametrize("shape", [None, "1", "7"])
 @pytest.mark.parametrize("tol", ["1e-10", "1e-6", "1e-3"])
 def test_golden_L8(torch_dev, golden, P, gnx_mod, tol, shape, monkeypatch):
-    """The tuned shape, and pinned: the two-reduction (1) and the
+    """The default shape, and pinned: the two-reduction (1) and the
     one-reduction (7) kernels, both starts and the prologue early exit."""
     banner(f"cg golden 8^3 tol={tol} shape={shape}")
     if shape is not None:
@@ -82,7 +82,7 @@ def test_vs_oracle(torch_dev, P, gnx_mod, L):
 
 @pytest.mark.parametrize("shape", [None, "9"])
 def test_noconv_and_determinism(torch_dev, P, gnx_mod, shape, monkeypatch):
-    """The tuned shape and, pinned, the one-reduction kernels (shape 9): the
+    """The default shape and, pinned, the one-reduction kernels (shape 9): the
     cap stops after exactly max_iter iterations, repeated solves agree
     bit-for-bit."""
     banner(f"cg max_iter cap and determinism, shape={shape}")

@@ -24,7 +24,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("shape", [None, "7"])
 def test_condensate_batch_golden(golden, P, gnx_mod, shape, monkeypatch):
-    """The tuned CG shapes and, pinned, the one-reduction kernels (shape 7)."""
+    """The default CG shapes and, pinned, the one-reduction kernels (shape 7)."""
     banner(f"condensate batch vs golden 8^3 (trajectory 1 of hmc_e0.1), shape={shape}")
     if shape is not None:
         monkeypatch.setenv("GNX_CG_SHAPE", shape)

@@ -1,5 +1,6 @@
 """GPU sigma-HMC: 20 LFMN2VF trajectories at 8^3 with measurements, against
-the reference's own run (golden) and the oracle.
+the reference's own run (golden) and the oracle: eps 0.1 and 0.2 (the
+README's command, README.md:24) with 2 steps, and eps 0.05 with 4 steps.
 
 Draws come from the host RNG in the reference's order (momenta, chi_0, chi_1,
 Metropolis uniform after the trajectory, then 6V Z2 draws for the condensate).
@@ -24,9 +25,10 @@ def relerr(a, b):
 
 
 @pytest.mark.parametrize("shape", [None, "7"])
-@pytest.mark.parametrize("tag,eps,steps,ntraj", [("hmc_e0.1", 0.1, 2, 20), ("hmc_e0.05s4", 0.05, 4, 5)])
+@pytest.mark.parametrize("tag,eps,steps,ntraj", [("hmc_e0.1", 0.1, 2, 20), ("hmc_e0.2", 0.2, 2, 20),
+                                                   ("hmc_e0.05s4", 0.05, 4, 5)])
 def test_hmc_golden(golden, P, gnx_mod, tag, eps, steps, ntraj, shape, monkeypatch):
-    """The tuned CG shapes and, pinned, the one-reduction kernels (shape 7)
+    """The default CG shapes and, pinned, the one-reduction kernels (shape 7)
     in every solve of the trajectory."""
     banner(f"hmc {tag} shape={shape}")
     if shape is not None:

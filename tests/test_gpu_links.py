@@ -87,7 +87,7 @@ def test_link_forces_vs_oracle(P, gnx_mod):
 @pytest.mark.parametrize("shape", [None, "7"])
 @pytest.mark.parametrize("start", ["null", "imposed"])
 def test_link_cg_vs_oracle(P, gnx_mod, start, shape, monkeypatch):
-    """The tuned shape and, pinned, the one-reduction kernels (shape 7)."""
+    """The default shape and, pinned, the one-reduction kernels (shape 7)."""
     banner(f"link cg 8^3 {start} shape={shape}")
     if shape is not None:
         monkeypatch.setenv("GNX_CG_SHAPE", shape)
