@@ -218,6 +218,30 @@ gnx_status gnx_cg(gnx_ctx* c, const double* sm, const double* b, double* x, doub
   return gnx_cg_batch(c, sm, b, x, 1, tol, max_iter, start, prec, res);
 }
 
+gnx_status gnx_cg_noalloc(gnx_ctx* c, const double* sm, const double* b, double* x, double tol, long max_iter,
+                          gnx_cg_start start, gnx_cg_result* res) {
+  CHECK_ARG(c && sm && b && x && res, "gnx_cg_noalloc: null argument");
+  CHECK_ARG(start == GNX_START_NULL || start == GNX_START_IMPOSED, "gnx_cg_noalloc: bad start mode");
+  CHECK_ARG(tol >= 0.0, "gnx_cg_noalloc: negative tolerance");
+  CHECK_ARG((const void*)b != (const void*)x, "gnx_cg_noalloc: b and x alias");
+  const size_t need = cgm_static_bytes(c->g), cap = cgm_static_capacity();
+  if (need > cap) {
+    set_err("gnx_cg_noalloc: %ld sites need %zu bytes of scratch, the static array holds %zu", c->V, need, cap);
+    return GNX_ERR_SIZE;
+  }
+  if (max_iter <= 0) max_iter = c->V + 1;
+  HIP_TRY(hipSetDevice(c->device));
+  CGDevResult hr;
+  HIP_TRY(cgm_f64_run_static(c->g, c->device, sm, (const double2*)b, (double2*)x, tol, max_iter, (int)start,
+                             c->stream, c->host_out, &hr));
+  fill_result(res, hr);
+  if (!hr.converged) {
+    set_err("gnx_cg_noalloc: not converged within %ld iterations", max_iter);
+    return GNX_ERR_NOCONV;
+  }
+  return GNX_OK;
+}
+
 gnx_status gnx_condensate_batch(gnx_ctx* c, const double* sm, const double* noise, int nsrc, double tol,
                                 long max_iter, gnx_prec prec, double* X, double* per_source, gnx_cg_result* res) {
   CHECK_ARG(c && sm && noise && X && per_source && res, "gnx_condensate_batch: null argument");

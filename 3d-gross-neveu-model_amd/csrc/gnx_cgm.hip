@@ -36,6 +36,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "gnx_cgstate.h"
 #include "gnx_engine.h"
 #include "gnx_internal.h"
@@ -655,8 +657,9 @@ static CGMFix cgm_fix(CGMachine* m, const Geom& g, int nrhs, int shape) {
   return a;
 }
 
-static hipError_t cgm_graph(CGMachine* m, const Geom& g, int nrhs, int shape, hipGraphExec_t* exec) {
-  CGMFix a = cgm_fix(m, g, nrhs, shape);
+// one chunk of ENG_CHUNK iterations (every A and B kernel) on stream cs:
+// captured into a graph by cgm_graph, launched directly by the static solve
+static void cgm_chunk(const CGMFix& a, int nrhs, int shape, hipStream_t cs) {
   dim3 grid(a.nb, nrhs), blk(CGM_BS);
   const CGMState* s0 = a.state;
   const CGMState* s1 = a.state + nrhs;
@@ -668,43 +671,53 @@ static hipError_t cgm_graph(CGMachine* m, const Geom& g, int nrhs, int shape, hi
   double* o1 = a.part + (size_t)nrhs * a.nb;
   const CGMShape& sh = kShapes[shape];
   const bool wh = head_wave(a.nb);
-  auto body = [&](hipStream_t cs) {
-    for (int i = 0; i < ENG_CHUNK; i++) {
-      int odd = i & 1;
-      // A: state[0] -> state[1], reduces slot 1, writes slot 0; B the reverse.
-      // The one-reduction kernels take (a, odd, state in, state out) only.
-      void* argsA[] = {(void*)&a, (void*)&odd, (void*)&s0, (void*)&w1, (void*)&q1, (void*)&o0};
-      void* argsB[] = {(void*)&a, (void*)&odd, (void*)&s1, (void*)&w0, (void*)&q0, (void*)&o1};
-      (void)hipLaunchKernel(wh ? sh.Aw : sh.A, grid, blk, argsA, 0, cs);
-      (void)hipLaunchKernel(wh ? sh.Bw : sh.B, grid, blk, argsB, 0, cs);
-    }
-  };
+  CGMFix ac = a;
+  for (int i = 0; i < ENG_CHUNK; i++) {
+    int odd = i & 1;
+    // A: state[0] -> state[1], reduces slot 1, writes slot 0; B the reverse.
+    // The one-reduction kernels take (a, odd, state in, state out) only.
+    void* argsA[] = {(void*)&ac, (void*)&odd, (void*)&s0, (void*)&w1, (void*)&q1, (void*)&o0};
+    void* argsB[] = {(void*)&ac, (void*)&odd, (void*)&s1, (void*)&w0, (void*)&q0, (void*)&o1};
+    (void)hipLaunchKernel(wh ? sh.Aw : sh.A, grid, blk, argsA, 0, cs);
+    (void)hipLaunchKernel(wh ? sh.Bw : sh.B, grid, blk, argsB, 0, cs);
+  }
+}
+
+static hipError_t cgm_graph(CGMachine* m, const Geom& g, int nrhs, int shape, hipGraphExec_t* exec) {
+  const CGMFix a = cgm_fix(m, g, nrhs, shape);
+  auto body = [&](hipStream_t cs) { cgm_chunk(a, nrhs, shape, cs); };
   return m->eng.graph(shape, nrhs, body, exec);
+}
+
+// the start kernels of one solve into the buffers of a
+static hipError_t cgm_start(const CGMFix& a, const double* sm, const double2* b, const double2* xin, double tol,
+                            long max_iter, int start, int nrhs, int shape, hipStream_t st) {
+  CGMFix ac = a;
+  dim3 grid(a.nb, nrhs), blk(CGM_BS);
+  hipError_t e;
+  if (start == GNX_START_NULL) {
+    void* a1[] = {(void*)&ac, (void*)&sm, (void*)&xin};
+    e = hipLaunchKernel(kShapes[shape].null_p1, grid, blk, a1, 0, st);
+    void* a2[] = {(void*)&ac, (void*)&b};
+    if (e == hipSuccess) e = hipLaunchKernel(kShapes[shape].null_p2, grid, blk, a2, 0, st);
+    void* a3[] = {(void*)&ac, (void*)&b, (void*)&tol, (void*)&max_iter};
+    if (e == hipSuccess) e = hipLaunchKernel(kShapes[shape].null_p3, grid, blk, a3, 0, st);
+  } else {
+    void* a0[] = {(void*)&ac, (void*)&sm, (void*)&b, (void*)&tol, (void*)&max_iter};
+    e = hipLaunchKernel(kShapes[shape].start_imposed, grid, blk, a0, 0, st);
+  }
+  return e;
 }
 
 // run the start kernels of one solve
 static hipError_t cgm_begin(CGMachine* m, const Geom& g, const double* sm, const double2* U, const double2* b,
                             const double2* xin, double tol, long max_iter, int start, int nrhs, int shape,
                             hipStream_t st) {
-  CGMFix a = cgm_fix(m, g, nrhs, shape);
-  dim3 grid(a.nb, nrhs), blk(CGM_BS);
-  hipError_t e;
   if (kShapes[shape].link) {
-    e = hipMemcpyAsync(m->U, U, sizeof(double2) * 3 * (size_t)g.V, hipMemcpyDeviceToDevice, st);
+    hipError_t e = hipMemcpyAsync(m->U, U, sizeof(double2) * 3 * (size_t)g.V, hipMemcpyDeviceToDevice, st);
     if (e != hipSuccess) return e;
   }
-  if (start == GNX_START_NULL) {
-    void* a1[] = {(void*)&a, (void*)&sm, (void*)&xin};
-    e = hipLaunchKernel(kShapes[shape].null_p1, grid, blk, a1, 0, st);
-    void* a2[] = {(void*)&a, (void*)&b};
-    if (e == hipSuccess) e = hipLaunchKernel(kShapes[shape].null_p2, grid, blk, a2, 0, st);
-    void* a3[] = {(void*)&a, (void*)&b, (void*)&tol, (void*)&max_iter};
-    if (e == hipSuccess) e = hipLaunchKernel(kShapes[shape].null_p3, grid, blk, a3, 0, st);
-  } else {
-    void* a0[] = {(void*)&a, (void*)&sm, (void*)&b, (void*)&tol, (void*)&max_iter};
-    e = hipLaunchKernel(kShapes[shape].start_imposed, grid, blk, a0, 0, st);
-  }
-  return e;
+  return cgm_start(cgm_fix(m, g, nrhs, shape), sm, b, xin, tol, max_iter, start, nrhs, shape, st);
 }
 
 // Launch shape of a solve.  By default a fixed rule, so that a solve's
@@ -790,6 +803,110 @@ hipError_t cgm_f64_run(void** machine_slot, const Geom& g, const double* sm, con
     if (e != hipSuccess) return e;
   }
   return hipStreamSynchronize(st);
+}
+
+// ---- solve without allocation ----------------------------------------------------
+// gnx_cg_noalloc runs the kernels of gnx_cg in the same launch shape (the
+// fixed rule) with every buffer of the solver -- sigma mass copy, x, r, the
+// two direction buffers, q, the partial sums, the state and the result -- in
+// one static device array, so the call itself never allocates.  Each device
+// that loads this library holds its own copy of the array until the process
+// ends; the host uses the address hipGetSymbolAddress reports for the
+// current device, never the array's name (a host-side stand-in).  Every call
+// on a device shares that device's array, and it keeps the last call's
+// contents: the start kernels write every word a solve reads before it reads
+// it, so nothing needs clearing, and calls that overlap on one device race.
+constexpr size_t kStaticScratchBytes = (size_t)80 << 20;
+constexpr int kStaticMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+static unsigned char* g_scratch_dev[kStaticMaxDevices];
+static std::mutex g_scratch_mu;
+
+static size_t up256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// byte offsets of the solver's buffers in the array for one system of g
+struct StaticLayout {
+  size_t sm, x, r, p0, p1, t, part, state, res, total;
+};
+static StaticLayout static_layout(const Geom& g, int nb) {
+  StaticLayout l;
+  const size_t vec = up256(sizeof(double2) * (size_t)g.V);
+  l.sm = 0;
+  l.x = l.sm + up256(sizeof(double) * (size_t)g.V);
+  l.r = l.x + vec;
+  l.p0 = l.r + vec;
+  l.p1 = l.p0 + vec;
+  l.t = l.p1 + vec;
+  l.part = l.t + vec;
+  l.state = l.part + up256(sizeof(double) * 2 * (size_t)nb);
+  l.res = l.state + up256(sizeof(CGMState) * 2);
+  l.total = l.res + up256(sizeof(CGDevResult));
+  return l;
+}
+
+size_t cgm_static_bytes(const Geom& g) {
+  return static_layout(g, shape_blocks(g, cgm_rule(g, 1))).total;
+}
+
+size_t cgm_static_capacity() { return kStaticScratchBytes; }
+
+// device address of this device's array (looked up once per device, with
+// that device current)
+static hipError_t static_scratch(int device, unsigned char** out) {
+  if (device < 0 || device >= kStaticMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_dev[device] = (unsigned char*)ptr;
+  }
+  *out = g_scratch_dev[device];
+  return hipSuccess;
+}
+
+hipError_t cgm_f64_run_static(const Geom& g, int device, const double* sm, const double2* b, double2* x, double tol,
+                              long max_iter, int start, hipStream_t st, void* pinned, CGDevResult* host_res) {
+  unsigned char* base = nullptr;
+  hipError_t e = static_scratch(device, &base);
+  if (e != hipSuccess) return e;
+  const int shape = cgm_rule(g, 1);
+  CGMFix a;
+  a.g = g;
+  a.nrhs = 1;
+  a.nb = shape_blocks(g, shape);
+  const StaticLayout l = static_layout(g, a.nb);
+  if (l.total > kStaticScratchBytes) return hipErrorInvalidValue;  // the caller checks the size first
+  a.sm = (double*)(base + l.sm);
+  a.U = nullptr;
+  a.x = (double2*)(base + l.x);
+  a.r = (double2*)(base + l.r);
+  a.p0 = (double2*)(base + l.p0);
+  a.p1 = (double2*)(base + l.p1);
+  a.t = (double2*)(base + l.t);
+  a.part = (double*)(base + l.part);
+  a.state = (CGMState*)(base + l.state);
+  a.res = (CGDevResult*)(base + l.res);
+  e = cgm_start(a, sm, b, x, tol, max_iter, start, 1, shape, st);
+  if (e != hipSuccess) return e;
+  // chunks of ENG_CHUNK iterations launched directly, the state's `active`
+  // and `iters` polled through the caller's pinned buffer after each
+  int* flags = (int*)pinned;
+  for (long done = 0;; done += ENG_CHUNK) {
+    cgm_chunk(a, 1, shape, st);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(flags, a.state, 2 * sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    if (!flags[0]) break;
+    if (done > max_iter + 2 * ENG_CHUNK) return hipErrorUnknown;  // the state machine failed to stop
+  }
+  e = hipMemcpyAsync(x, a.x, sizeof(double2) * (size_t)g.V, hipMemcpyDeviceToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(pinned, a.res, sizeof(CGDevResult), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e == hipSuccess) memcpy(host_res, pinned, sizeof(CGDevResult));
+  return e;
 }
 
 int cgm_shape_of(void* machine, int nrhs, int* spt, int* xcd) {

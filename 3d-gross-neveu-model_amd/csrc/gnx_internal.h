@@ -58,6 +58,13 @@ int multi_count(const gnx_multi* m);
 bool multi_system(const gnx_multi* m, int k, int* L, int* LT, int* device);
 int cgm_shape_of(void* machine, int nrhs, int* spt, int* xcd);
 void cgm_set_autotune(void** machine_slot, int on);
+// the allocation-free FP64 solve (one system): bytes of the static array it
+// needs for g, the array's size, and the solve; pinned: 64 bytes of pinned
+// host memory for the polled flags and the result
+size_t cgm_static_bytes(const Geom& g);
+size_t cgm_static_capacity();
+hipError_t cgm_f64_run_static(const Geom& g, int device, const double* sm, const double2* b, double2* x, double tol,
+                              long max_iter, int start, hipStream_t st, void* pinned, CGDevResult* host_res);
 long cgm_tune_runs(void* machine);
 void cgmix_set_autotune(void** machine_slot, int on);
 long cgmix_tune_runs(void* machine);

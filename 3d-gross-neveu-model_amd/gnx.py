@@ -72,6 +72,7 @@ def lib():
         L.gnx_dirac.argtypes = [vp, C.c_int, vp, vp, vp]
         L.gnx_dirac_batch.argtypes = [vp, C.c_int, vp, vp, vp, C.c_int]
         L.gnx_cg.argtypes = [vp, vp, vp, vp, C.c_double, C.c_long, C.c_int, C.c_int, C.POINTER(CGResult)]
+        L.gnx_cg_noalloc.argtypes = [vp, vp, vp, vp, C.c_double, C.c_long, C.c_int, C.POINTER(CGResult)]
         L.gnx_cg_batch.argtypes = [vp, vp, vp, vp, C.c_int, C.c_double, C.c_long, C.c_int, C.c_int,
                                    C.POINTER(CGResult)]
         L.gnx_set_reliable_delta.argtypes = [vp, C.c_double]
@@ -225,6 +226,16 @@ class Context:
         st = lib().gnx_cg(self.h, _ptr(sm), _ptr(as_real(b)), _ptr(as_real(x)), tol, max_iter, start, prec,
                           C.byref(r))
         if check and st not in (0, 3):
+            _check(st)
+        return st, r
+
+    def cg_noalloc(self, sm, b, x, tol, max_iter=0, start=START_NULL):
+        """gnx_cg_noalloc: the FP64 single-RHS solve with no allocation in
+        the call (buffers in the library's static device array)."""
+        r = CGResult()
+        st = lib().gnx_cg_noalloc(self.h, _ptr(sm), _ptr(as_real(b)), _ptr(as_real(x)), tol, max_iter, start,
+                                  C.byref(r))
+        if st not in (0, 3):
             _check(st)
         return st, r
 

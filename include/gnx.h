@@ -124,6 +124,19 @@ gnx_status gnx_cg_batch(gnx_ctx *ctx, const double *sm, const double *b, double 
                         double tol, long max_iter, gnx_cg_start start, gnx_prec prec,
                         gnx_cg_result *res);
 
+/* gnx_cg with prec GNX_PREC_F64 and one right-hand side, for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  The call allocates nothing: the solver's buffers live in one
+ * static device array of 80 MiB per device that loads the library (held
+ * until the process ends), enough for V <= about 950 000 sites (98^3);
+ * larger lattices return GNX_ERR_SIZE.  The launch shape is the default
+ * rule's, so results equal gnx_cg's without GNX_CG_SHAPE or autotuning.
+ * All calls on one device share the array: calls that overlap on one device
+ * (other threads, other streams) race. */
+gnx_status gnx_cg_noalloc(gnx_ctx *ctx, const double *sm, const double *b, double *x, double tol, long max_iter,
+                          gnx_cg_start start, gnx_cg_result *res);
+
 /* Uneven batches (extension 2: independent chains on lattices of uneven
  * size).  n systems, system k on an lspace[k]^2 x ltime[k] lattice with its
  * own sigma mass, solved by the FP64 CG of gnx_cg (both starts, the same exit
