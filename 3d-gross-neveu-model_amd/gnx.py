@@ -58,6 +58,14 @@ def lib():
         if L.gnx_abi_version() != ABI_VERSION:
             raise RuntimeError(f"{LIB_PATH} implements ABI version {L.gnx_abi_version()}, this binding "
                                f"needs {ABI_VERSION}: rebuild it with `make -C {_HERE}`")
+        L.gnx_rng_create.argtypes = [C.c_uint, C.POINTER(vp)]
+        L.gnx_rng_destroy.argtypes = [vp]
+        L.gnx_rng_casuale.argtypes = [vp]
+        L.gnx_rng_casuale.restype = C.c_double
+        L.gnx_rng_gauss1.argtypes = [vp]
+        L.gnx_rng_gauss1.restype = C.c_double
+        L.gnx_rng_gauss2mv.argtypes = [vp, C.c_double, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.gnx_rng_fill.argtypes = [vp, C.c_int, C.c_long, C.c_double, C.POINTER(C.c_double)]
         L.gnx_set_autotune.argtypes = [vp, C.c_int]
         L.gnx_tune_runs.argtypes = [vp, C.POINTER(C.c_long), C.POINTER(C.c_long)]
         L.gnx_last_error.restype = C.c_char_p
@@ -427,6 +435,63 @@ class HMC:
         calls = C.c_int()
         _check(lib().gnx_hmc_cg_stats(self.h, C.byref(it), C.byref(calls)))
         return it.value, calls.value
+
+
+class Rng:
+    """The reference's host random stream (include/gnx_rng.h): Initrand(seed)
+    then Casuale / Gauss1 / Gauss2MeanVariance (random.c), and bulk draws in
+    the reference's order.  Host memory, numpy arrays."""
+
+    DRAW_CASUALE, DRAW_GAUSS1, DRAW_SIGMA, DRAW_Z2_COMPLEX, DRAW_Z2_REAL = range(5)
+
+    def __init__(self, seed):
+        h = C.c_void_p()
+        _check(lib().gnx_rng_create(seed, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gnx_rng_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _fill(self, kind, n, out_len, var=0.0):
+        import numpy as np
+        out = np.empty(out_len)
+        _check(lib().gnx_rng_fill(self.h, kind, n, var, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def casuale(self, n=None):
+        if n is None:
+            return lib().gnx_rng_casuale(self.h)
+        return self._fill(self.DRAW_CASUALE, n, n)
+
+    def gauss1(self, n=None):
+        if n is None:
+            return lib().gnx_rng_gauss1(self.h)
+        return self._fill(self.DRAW_GAUSS1, n, n)
+
+    def gauss2mv(self, mean, var):
+        a, b = C.c_double(), C.c_double()
+        lib().gnx_rng_gauss2mv(self.h, mean, var, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def sigma(self, V, var):
+        """V values: Gauss2MeanVariance(0, var) pairs into (x, x + V/2) (sigma.c:29-30)."""
+        return self._fill(self.DRAW_SIGMA, V, V, var)
+
+    def z2_complex(self, V):
+        """2V values: per site re then im, each +-1."""
+        return self._fill(self.DRAW_Z2_COMPLEX, V, 2 * V)
+
+    def z2_real(self, V):
+        """V values +-1 (condensate.c:52-60)."""
+        return self._fill(self.DRAW_Z2_REAL, V, V)
 
 
 class Multi:

@@ -1,4 +1,5 @@
-"""The C-ABI library loads and exports every symbol include/gnx.h declares.
+"""The C-ABI library loads and exports every symbol include/gnx.h and
+include/gnx_rng.h declare.
 No GPU calls are made here."""
 import ctypes
 import os
@@ -7,11 +8,11 @@ import subprocess
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "3d-gross-neveu-model_amd", "lib", "libgnx.so")
-HDR = os.path.join(ROOT, "include", "gnx.h")
+HDRS = [os.path.join(ROOT, "include", h) for h in ("gnx.h", "gnx_rng.h")]
 
 
 def declared():
-    txt = open(HDR).read()
+    txt = "".join(open(h).read() for h in HDRS)
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     return sorted(set(re.findall(r"\b(gnx_[a-z0-9_]+)\s*\(", txt)))
 
