@@ -163,6 +163,10 @@ gnx_status gnx_rng_fill(gnx_rng* r, gnx_draw kind, long n, double var, double* o
     case GNX_DRAW_Z2_REAL:
       for (long i = 0; i < n; i++) out[i] = s.z2();
       return GNX_OK;
+    case GNX_DRAW_GAUSS2_PAIRS:
+      if (n % 2) return GNX_ERR_ARG;
+      for (long i = 0; i < n; i += 2) s.gauss2mv(0.0, var, &out[i], &out[i + 1]);
+      return GNX_OK;
   }
   return GNX_ERR_ARG;
 }

@@ -442,7 +442,7 @@ class Rng:
     then Casuale / Gauss1 / Gauss2MeanVariance (random.c), and bulk draws in
     the reference's order.  Host memory, numpy arrays."""
 
-    DRAW_CASUALE, DRAW_GAUSS1, DRAW_SIGMA, DRAW_Z2_COMPLEX, DRAW_Z2_REAL = range(5)
+    DRAW_CASUALE, DRAW_GAUSS1, DRAW_SIGMA, DRAW_Z2_COMPLEX, DRAW_Z2_REAL, DRAW_GAUSS2_PAIRS = range(6)
 
     def __init__(self, seed):
         h = C.c_void_p()
@@ -484,6 +484,10 @@ class Rng:
     def sigma(self, V, var):
         """V values: Gauss2MeanVariance(0, var) pairs into (x, x + V/2) (sigma.c:29-30)."""
         return self._fill(self.DRAW_SIGMA, V, V, var)
+
+    def gauss2_pairs(self, n, var=1.0):
+        """n values: Gauss2MeanVariance(0, var) pairs into (2r, 2r+1) (link momenta)."""
+        return self._fill(self.DRAW_GAUSS2_PAIRS, n, n, var)
 
     def z2_complex(self, V):
         """2V values: per site re then im, each +-1."""

@@ -1,17 +1,22 @@
 """Benchmark of the fermion hot path on one MI355X.
 
-Workload (the single-RHS FP64 drop-in at 64^3): sigma iid N(0, 2/(N_f beta_s))
-with N_f = 4, beta_s = 0.85 on the V dual sites, sigma mass from it, a complex
-Z2 source (re and im each +-1), CG on M M^dag to the absolute residual 1e-10
-(the reference's exit test, conjugategradient.c:87-89).  One step = one CG
-solve with inputs resident in HBM.
+Workload (the single-RHS FP64 drop-in at 64^3): sigma drawn from the
+reference's own random stream (dSFMT-132049 after Initrand(1231)) as
+Gauss2MeanVariance(0, 2/(N_f beta_s)) pairs with N_f = 4, beta_s = 0.85 on
+the V dual sites, sigma mass from it, a complex Z2 source (re and im each
+Casuale() <= 0.5 ? 1 : -1), CG on M M^dag to the absolute residual 1e-10 (the
+reference's exit test, conjugategradient.c:87-89).  One step = one CG solve
+with inputs resident in HBM.
 
 Reported: RHS-solves/s of the FP64 drop-in (the headline `value`), the Dirac
-application rate and GB/s, mixed-precision and batched solve rates, the
-roofline of the dominant kernels (the CG iteration pair) from HIP events on
-their stream, with the HBM traffic per iteration recorded from rocprofv3
-counter runs of the same kernels, and the serial CPU baseline (the oracle, 1 core) on a bounded sample
-of the same workload.
+application rate and GB/s, mixed-precision and batched solve rates, the same
+legs at the configurations' other sizes (32^3, 96^3), the batched condensate
+(48^3 x 64 sources) and the uneven ensembles; the roofline of the dominant
+kernels (the CG iteration pair) from HIP events on their stream, with the
+HBM traffic per iteration recorded from rocprofv3 counter runs of the same
+kernels; the graph-node and grid-barrier probes behind the CG schedule; and
+the serial CPU baseline (the oracle, 1 core: the median of five full solves
+of the same system) with the reference's own O(V^2) set-up as an estimate.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--L 64]
 """
@@ -139,47 +144,151 @@ def summarize_rocprof(kt_dir, fetch_dir, write_dir, L):
     return "\n".join(out), rec
 
 
-def make_inputs(L, seed=1231, beta_s=0.85, nflav=4):
-    rng = np.random.default_rng(seed)
+def make_inputs(L, seed=1231, beta_s=0.85, nflav=4, nsrc=1):
+    """The reference's own random stream (dSFMT-132049 after Initrand(seed),
+    the library's host RNG): sigma by Gauss2MeanVariance(0, 2/(N_f beta_s))
+    pairs into (x, x + V/2) (sigma.c:29-30), then nsrc complex Z2 sources, re
+    and im each Casuale() <= 0.5 ? 1 : -1 (condensate.c:57-58)."""
+    import gnx
+    r = gnx.Rng(seed)
     V = L * L * L
-    sigma = rng.normal(0.0, np.sqrt(2.0 / (nflav * beta_s)), V)
-    b = np.where(rng.random(2 * V) <= 0.5, 1.0, -1.0)
-    return sigma, b
+    sigma = r.sigma(V, 2.0 / (nflav * beta_s))
+    bs = [r.z2_complex(V) for _ in range(nsrc)]
+    return sigma, (bs[0] if nsrc == 1 else np.concatenate(bs))
 
 
-def cpu_baseline(L, sigma, b, iters_full, budget_s=15.0):
-    """Oracle CG (serial, 1 core) on the same fields: time a capped run and
-    scale to a full solve of iters_full iterations."""
+def host_cpu():
+    """Host CPU description for the baseline: model, logical CPUs, CPUs this
+    process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"model": model, "nproc": os.cpu_count(), "usable": usable}
+
+
+def setup_estimate(sizes=(32, 64, 96)):
+    """The reference pays an O(V^2) table set-up before its first solve
+    (coordinate.c:253-267, twice per site from lattice.c:32-33).  Timed
+    here: the reference's own StartLattice at the size its macro.h ships
+    (oracle/_ref/ref_setup_time, built from the reference's sources where
+    they are present), and the oracle's restatement of the same search at
+    three small sizes, which shows the V^2 growth; both extrapolated by V^2."""
+    import statistics
+    import subprocess
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as P
+    out = {"note": "excluded from the CPU baseline's solve rate; extrapolated by V^2"}
+    pts = []
+    for L in (12, 16, 20):
+        lat = P.Lattice(L)
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            lat.tables_by_search()
+            ts.append(time.perf_counter() - t0)
+        pts.append((lat.V, statistics.median(ts)))
+    slope = float(np.polyfit(np.log([v for v, _ in pts]), np.log([t for _, t in pts]), 1)[0])
+    v0, t0s = pts[-1]
+    out["oracle_search"] = {"seconds": {f"{round(v ** (1 / 3))}^3": t for v, t in pts},
+                            "fitted_exponent_in_V": slope,
+                            "extrapolated_seconds": {f"{L}^3": t0s * (L ** 3 / v0) ** 2 for L in sizes}}
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_setup_time")
+    if os.path.exists(exe):
+        try:
+            vol, sec = subprocess.run([exe], capture_output=True, text=True, timeout=60,
+                                      check=True).stdout.split()
+            vol, sec = int(vol), float(sec)
+            out["reference_StartLattice"] = {
+                "volume": vol, "seconds": sec, "kind": "reference",
+                "extrapolated_seconds": {f"{L}^3": sec * (L ** 3 / vol) ** 2 for L in sizes}}
+        except Exception as exc:  # a measurement aid: report, do not fail the bench
+            out["reference_StartLattice"] = {"error": str(exc)}
+    return out
+
+
+def cpu_baseline(L, sigma, b, nsolves=5):
+    """Oracle CG (serial, 1 core, the reference's loops and order) on the
+    same fields: the median of nsolves full solves to the same absolute
+    tolerance, and the median of 100 single Dirac applications."""
+    import statistics
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as P
     lat = P.Lattice(L)
     sm = lat.sigma_mass(sigma)
-    # estimate the per-iteration time with a short run, then size the sample
-    t0 = time.perf_counter()
-    lat.cg(sm, b, None, tol=0.0, max_iter=3, start="imposed")
-    per = (time.perf_counter() - t0) / 3
-    n = int(max(5, min(iters_full, budget_s / max(per, 1e-9))))
-    t0 = time.perf_counter()
-    lat.cg(sm, b, None, tol=0.0, max_iter=n, start="imposed")
-    dt = time.perf_counter() - t0
-    per_iter = dt / n
+    ts, its = [], []
+    for _ in range(nsolves):
+        t0 = time.perf_counter()
+        _, ok, st = lat.cg(sm, b, None, tol=1e-10, start="imposed")
+        ts.append(time.perf_counter() - t0)
+        its.append(st.last_iters)
+    t_solve = statistics.median(ts)
     # the serial Dirac application (M and M^dag alternating, 100 of them)
     v = b
-    t0 = time.perf_counter()
+    ta = []
     for k in range(50):
+        t0 = time.perf_counter()
         w = lat.M(sm, v)
+        t1 = time.perf_counter()
         v = lat.Mdag(sm, w)
-    t_app = (time.perf_counter() - t0) / 100
+        ta += [t1 - t0, time.perf_counter() - t1]
+    t_app = statistics.median(ta)
     return {
-        "value": 1.0 / (per_iter * iters_full),
+        "value": 1.0 / t_solve,
         "unit": "RHS-solves/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{n} CG iterations of the serial oracle at {L}^3 ({dt:.1f} s), "
-                  f"scaled to the {iters_full} iterations of a full solve; per-iteration {per_iter*1e3:.2f} ms",
+        "host": host_cpu(),
+        "sample": f"median of {nsolves} full serial solves of the same {L}^3 system to |r| < 1e-10 "
+                  f"({its[0]} iterations each; {min(ts):.2f}-{max(ts):.2f} s per solve) by the oracle, "
+                  "which keeps the reference's loops and accumulation order (table set-up excluded)",
+        "ms_per_iteration": t_solve / its[0] * 1e3,
         "dirac": {"apps_per_s": 1.0 / t_app, "GB_s": BYTES_DIRAC_F64 * L ** 3 / t_app / 1e9,
-                  "sample": "100 serial applications (M and M^dag alternating) at the same size"},
+                  "sample": "median of 100 serial applications (M and M^dag alternating) at the same size"},
+        "setup_estimate": setup_estimate(),
     }
+
+
+def probes_leg(L=64):
+    """The measurements behind the CG schedule (lib/libgnx_probes.so): the
+    cost of one node of a replayed hipGraph (an empty kernel, and streaming
+    kernels of 40 and 152 B/site at L^3, each reading the previous node's
+    output), and of a grid-wide barrier inside one persistent kernel at
+    several block counts, with and without bytes written between barriers."""
+    import ctypes as C
+    lib = C.CDLL(os.path.join(ROOT, "3d-gross-neveu-model_amd", "lib", "libgnx_probes.so"))
+    V = L ** 3
+    out = {"graph_nodes": {}, "grid_barrier": []}
+    for kind, name, bps in ((0, "empty", 0), (1, "stream_40B", 40), (2, "stream_152B", 152)):
+        us = C.c_double()
+        rc = lib.gnx_probe_graph_nodes(kind, C.c_long(V), 64, 20, C.byref(us))
+        if rc != 0:
+            out["graph_nodes"][name] = {"error": rc}
+            continue
+        d = {"us_per_node": us.value, "nodes": 64}
+        if bps:
+            d["GB_s"] = bps * V / (us.value * 1e-6) / 1e9
+            d["sites"] = V
+        out["graph_nodes"][name] = d
+    for blocks in (256, 512, 1024):
+        for per_block in (0, 32768):
+            us, mx, failed = C.c_double(), C.c_int(), C.c_int()
+            rc = lib.gnx_probe_grid_barrier(blocks, 200, C.c_long(per_block), C.byref(us), C.byref(mx),
+                                            C.byref(failed))
+            out["grid_barrier"].append({"blocks": blocks, "bytes_per_block_between_barriers": per_block,
+                                        "us_per_barrier": us.value if rc == 0 else None,
+                                        "resident_blocks_max": mx.value, "wait_timed_out": bool(failed.value),
+                                        "rc": rc})
+    return out
 
 
 def condensate_leg(ctx_factory, torch, gnx, spec, tol):
@@ -190,9 +299,7 @@ def condensate_leg(ctx_factory, torch, gnx, spec, tol):
     has E|v|^2 = 2, so each source's estimate is Re(v^dag M^-1 v) / (2V)."""
     Lc, n = (int(v) for v in spec.split(":"))
     V = Lc ** 3
-    sigma_h, _ = make_inputs(Lc, seed=4848)
-    rng = np.random.default_rng(4849)
-    noise_h = np.where(rng.random(2 * V * n) <= 0.5, 1.0, -1.0)
+    sigma_h, noise_h = make_inputs(Lc, seed=4848, nsrc=n)
     ctx = ctx_factory(Lc)
     dev = ctx.device
     sm = ctx.field()
@@ -257,13 +364,13 @@ def ensemble_leg(gnx, sizes, ntraj, betas=(0.6, 0.75, 0.85, 1.0), eps=0.02, step
             for bs in betas:
                 V = L ** 3
                 seed = 5000 + 10 * L + int(bs * 100)
-                rng = np.random.default_rng(seed)
+                rng = gnx.Rng(seed)
                 ctx = gnx.Context(L)
                 h = gnx.HMC(ctx, beta_s=bs)
-                h.set_sigma(rng.normal(0.0, np.sqrt(2.0 / (4 * bs)), V))
+                h.set_sigma(rng.sigma(V, 2.0 / (4 * bs)))
                 lh = gnx.LinkHMC(ctx, beta=1.0, pure_gauge=True, linked=False)
                 _, sm = h.get_sigma()
-                lh.set_fields(np.exp(1j * rng.uniform(0.0, 2 * np.pi, 3 * V)), sm)
+                lh.set_fields(np.exp(1j * 2.0 * np.pi * rng.casuale(3 * V)), sm)
                 out.append({"L": L, "beta_s": bs, "V": V, "ctx": ctx, "h": h, "lh": lh, "rng": rng,
                             "dH_sigma": [], "dH_link": [], "acc_sigma": 0, "acc_link": 0, "cg_iterations": 0})
         return out
@@ -273,15 +380,13 @@ def ensemble_leg(gnx, sizes, ntraj, betas=(0.6, 0.75, 0.85, 1.0), eps=0.02, step
     def run(c, n):
         V, rng = c["V"], c["rng"]
         for _ in range(n):
-            st, t = c["h"].trajectory(rng.standard_normal(V), rng.standard_normal(V), rng.standard_normal(V),
-                                      eps, steps, 1e-10, 1e-3)
-            a = bool(t.ok) and rng.random() <= np.exp(-t.dH)
+            st, t = c["h"].trajectory(rng.gauss1(V), rng.gauss1(V), rng.gauss1(V), eps, steps, 1e-10, 1e-3)
+            a = bool(t.ok) and rng.casuale() <= np.exp(-t.dH)
             c["h"].accept(a)
             _, sm = c["h"].get_sigma()
             c["lh"].set_fields(None, sm)
-            st2, tl = c["lh"].trajectory(rng.standard_normal(3 * V), rng.standard_normal(V),
-                                         rng.standard_normal(V), eps, steps, 1e-10)
-            al = bool(tl.ok) and rng.random() <= np.exp(-tl.dH)
+            st2, tl = c["lh"].trajectory(rng.gauss2_pairs(3 * V), rng.gauss1(V), rng.gauss1(V), eps, steps, 1e-10)
+            al = bool(tl.ok) and rng.casuale() <= np.exp(-tl.dH)
             c["lh"].accept(al)
             c["dH_sigma"].append(t.dH)
             c["dH_link"].append(tl.dH)
@@ -316,24 +421,23 @@ def ensemble_leg(gnx, sizes, ntraj, betas=(0.6, 0.75, 0.85, 1.0), eps=0.02, step
 
     def run_lockstep(n):
         for _ in range(n):
-            dr = [(c["rng"].standard_normal(c["V"]), c["rng"].standard_normal(c["V"]),
-                   c["rng"].standard_normal(c["V"])) for c in lock]
+            dr = [(c["rng"].gauss1(c["V"]), c["rng"].gauss1(c["V"]), c["rng"].gauss1(c["V"])) for c in lock]
             _, ts = es.sigma_trajectory([d[0] for d in dr], [d[1] for d in dr], [d[2] for d in dr],
                                         eps, steps, 1e-10, 1e-3)
             for c, t in zip(lock, ts):
-                a = bool(t.ok) and c["rng"].random() <= np.exp(-t.dH)
+                a = bool(t.ok) and c["rng"].casuale() <= np.exp(-t.dH)
                 c["h"].accept(a)
                 _, sm = c["h"].get_sigma()
                 c["lh"].set_fields(None, sm)
                 c["dH_sigma"].append(t.dH)
                 c["acc_sigma"] += int(a)
                 c["cg_iterations"] += t.cg_iterations
-            dl = [(c["rng"].standard_normal(3 * c["V"]), c["rng"].standard_normal(c["V"]),
-                   c["rng"].standard_normal(c["V"])) for c in lock]
+            dl = [(c["rng"].gauss2_pairs(3 * c["V"]), c["rng"].gauss1(c["V"]), c["rng"].gauss1(c["V"]))
+                  for c in lock]
             _, tls = el.link_trajectory([d[0] for d in dl], [d[1] for d in dl], [d[2] for d in dl],
                                         eps, steps, 1e-10)
             for c, tl in zip(lock, tls):
-                al = bool(tl.ok) and c["rng"].random() <= np.exp(-tl.dH)
+                al = bool(tl.ok) and c["rng"].casuale() <= np.exp(-tl.dH)
                 c["lh"].accept(al)
                 c["dH_link"].append(tl.dH)
                 c["acc_link"] += int(al)
@@ -380,10 +484,10 @@ def uneven_leg(gnx, torch, sizes, tol, betas=(0.6, 0.75, 0.85, 1.0)):
         ctx_of[L] = gnx.Context(L)
         for bs in betas:
             V = L ** 3
-            rng = np.random.default_rng(7000 + 10 * L + int(bs * 100))
+            sig_h, b_h = make_inputs(L, seed=7000 + 10 * L + int(bs * 100), beta_s=bs)
             sm = ctx_of[L].field()
-            ctx_of[L].sigma_mass(torch.from_numpy(rng.normal(0.0, np.sqrt(2.0 / (4 * bs)), V)).to(dev), sm)
-            b = torch.from_numpy(np.where(rng.random(2 * V) <= 0.5, 1.0, -1.0)).to(dev)
+            ctx_of[L].sigma_mass(torch.from_numpy(sig_h).to(dev), sm)
+            b = torch.from_numpy(b_h).to(dev)
             systems.append({"L": L, "beta_s": bs, "V": V, "sm": sm, "b": b})
     n = len(systems)
     out = {"workload": f"{n} systems on {sizes}^3 x beta_s {list(betas)}, own sigma each, "
@@ -436,6 +540,142 @@ def uneven_leg(gnx, torch, sizes, tol, betas=(0.6, 0.75, 0.85, 1.0)):
     return out
 
 
+def dirac_graph_rate(gnx, torch, L, sm, b):
+    """Device-side rate of back-to-back single applications: 100 (M then
+    M^dag, each reading the other's output) captured once in a HIP graph on a
+    dedicated stream and replayed, so no host launch sits between them;
+    every replay is checked against the eager result of the same sequence."""
+    V = L ** 3
+    dev = sm.device
+    try:
+        gs = torch.cuda.Stream(dev)
+        gctx = gnx.Context(L, stream=gs)
+        go1, go2 = gctx.vec(), gctx.vec()
+        with torch.cuda.stream(gs):
+            for _ in range(50):
+                gctx.dirac(gnx.OP_M, sm, b, go1)
+                gctx.dirac(gnx.OP_MDAG, sm, go1, go2)
+        gs.synchronize()
+        ref2 = go2.clone()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=gs):
+            for _ in range(50):
+                gctx.dirac(gnx.OP_M, sm, b, go1)
+                gctx.dirac(gnx.OP_MDAG, sm, go1, go2)
+        same = True
+        for _ in range(3):
+            go2.zero_()
+            graph.replay()
+            gs.synchronize()
+            same = same and bool(torch.equal(go2, ref2))
+        g0, g1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        g0.record(gs)
+        for _ in range(10):
+            graph.replay()
+        g1.record(gs)
+        g1.synchronize()
+        t_g = g0.elapsed_time(g1) * 1e-3 / 1000
+        ws = 2 * 16 * V + 8 * V  # in, out, sigma mass
+        res = {"apps_per_s": 1.0 / t_g, "GB_s": BYTES_DIRAC_F64 * V / t_g / 1e9,
+               "frac_hbm": BYTES_DIRAC_F64 * V / t_g / 1e9 / HBM_PEAK_GBS,
+               "replays_match_eager": same, "applications_per_graph": 100,
+               "note": f"working set {ws / 2**20:.1f} MiB: served from the 256 MB Infinity Cache "
+                       "when it fits there, so frac_hbm can exceed 1"}
+        del graph, gctx, go1, go2, ref2
+        return res
+    except Exception as exc:  # the capture is a measurement aid, not the product path
+        return {"error": str(exc)}
+
+
+def size_leg(gnx, torch, L, tol, steps, nbatch, precs):
+    """Configurations 2 and 3 at a lattice size other than the headline's:
+    the same inputs (the reference's stream, seed 1231, nbatch sources), the
+    FP64 single-RHS drop-in (RHS-solves/s over `steps` solves, iterations,
+    us per iteration, the CG iteration's algorithmic bandwidth and true
+    residual), the Dirac application rate, and batched FP64 and mixed
+    solves of nbatch sources with iterations, reliable updates, true
+    residual and error against the FP64 solution."""
+    V = L ** 3
+    sigma_h, B_h = make_inputs(L, nsrc=max(1, nbatch))
+    ctx = gnx.Context(L)
+    dev = ctx.device
+    sm = ctx.field()
+    ctx.sigma_mass(torch.from_numpy(sigma_h).to(dev), sm)
+    B = torch.from_numpy(B_h).to(dev)
+    b = B[:2 * V]
+    x = ctx.vec()
+    out = {"workload": f"{L}^3, sigma and {max(1, nbatch)} complex Z2 sources from the reference's stream "
+                       f"(seed 1231), tol {tol:g}"}
+    ctx.cg(sm, b, x, tol, start=gnx.START_IMPOSED)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(ctx.stream)
+    for _ in range(steps):
+        st, res = ctx.cg(sm, b, x, tol, start=gnx.START_IMPOSED)
+    e1.record(ctx.stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    t_solve = e0.elapsed_time(e1) * 1e-3 / steps
+    tr = ctx.vec()
+    ctx.dirac(gnx.OP_MMDAG, sm, x, tr)
+    torch.cuda.synchronize()
+    f64 = steps / wall
+    out["f64_single"] = {"rhs_solves_per_s": f64, "iterations": res.iterations,
+                         "us_per_iteration": t_solve / res.iterations * 1e6,
+                         "algorithmic_GB_s": BYTES_CG_ITER_F64 * V * res.iterations / t_solve / 1e9,
+                         "frac_hbm": BYTES_CG_ITER_F64 * V * res.iterations / t_solve / 1e9 / HBM_PEAK_GBS,
+                         "true_residual": float(torch.linalg.norm(b - tr).item()),
+                         "shape": ctx.cg_shape(1)}
+    # single Dirac applications, host launched and graph-replayed
+    o = ctx.vec()
+    for _ in range(10):
+        ctx.dirac(gnx.OP_M, sm, b, o)
+    torch.cuda.synchronize()
+    e0.record(ctx.stream)
+    for _ in range(100):
+        ctx.dirac(gnx.OP_M, sm, b, o)
+        ctx.dirac(gnx.OP_MDAG, sm, o, x)
+    e1.record(ctx.stream)
+    e1.synchronize()
+    ta = e0.elapsed_time(e1) * 1e-3 / 200
+    out["dirac"] = {"apps_per_s": 1.0 / ta, "GB_s": BYTES_DIRAC_F64 * V / ta / 1e9}
+    out["dirac_graph"] = dirac_graph_rate(gnx, torch, L, sm, b)
+    if nbatch > 1:
+        X64 = ctx.vec(nbatch)
+        ctx.cg_batch(sm, B, X64, nbatch, tol, start=gnx.START_IMPOSED)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st, res = ctx.cg_batch(sm, B, X64, nbatch, tol, start=gnx.START_IMPOSED)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[f"f64_x{nbatch}"] = {"rhs_solves_per_s": nbatch / dt, "iterations_max": max(r.iterations for r in res),
+                                 "all_converged": all(r.converged for r in res)}
+        X = ctx.vec(nbatch)
+        codes = {"f32": gnx.PREC_F32, "bf16": gnx.PREC_BF16, "f16": gnx.PREC_F16}
+        for pname in precs:
+            ctx.cg_batch(sm, B, X, nbatch, tol, start=gnx.START_IMPOSED, prec=codes[pname])
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st, res = ctx.cg_batch(sm, B, X, nbatch, tol, start=gnx.START_IMPOSED, prec=codes[pname])
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            ctx.dirac(gnx.OP_MMDAG, sm, X[:2 * V], tr)
+            torch.cuda.synchronize()
+            errs = [float((torch.linalg.norm(X[2 * V * k:2 * V * (k + 1)] - X64[2 * V * k:2 * V * (k + 1)])
+                           / torch.linalg.norm(X64[2 * V * k:2 * V * (k + 1)])).item()) for k in range(nbatch)]
+            out[f"{pname}_x{nbatch}"] = {
+                "rhs_solves_per_s": nbatch / dt, "iterations_max": max(r.iterations for r in res),
+                "reliable_updates_max": max(r.reliable_updates for r in res),
+                "all_converged": all(r.converged for r in res),
+                "true_residual_rhs0": float(torch.linalg.norm(b - tr).item()),
+                "max_rel_err_vs_f64": max(errs),
+                "speedup_vs_f64_single": nbatch / dt / f64}
+        del X, X64
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -451,6 +691,10 @@ def main():
     ap.add_argument("--chains", default="16,24,32,48", metavar="L,L,...",
                     help="uneven-ensemble leg: 4 chains per lattice size, sigma and link HMC ('' to skip)")
     ap.add_argument("--chain-trajectories", type=int, default=2)
+    ap.add_argument("--sizes", default="32:32,96:16", metavar="L:N,...",
+                    help="configurations 2 and 3 at other sizes: lattice L^3, FP64 single and N-source "
+                         "batched FP64 and mixed solves ('' to skip)")
+    ap.add_argument("--no-probes", action="store_true", help="skip the graph-node and grid-barrier probes")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also time a plain device copy of MIB MiB (the reference kernel for "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE scaling)")
@@ -500,48 +744,7 @@ def main():
     e1.synchronize()
     t_app = e0.elapsed_time(e1) * 1e-3 / napp
     dirac_gbs = BYTES_DIRAC_F64 * V / t_app / 1e9
-    # device-side rate of back-to-back single applications: 100 captured
-    # once in a HIP graph on a dedicated stream and replayed, so no host
-    # launch sits between them; every replay is checked against the eager
-    # result of the same sequence
-    dirac_graph = None
-    try:
-        gs = torch.cuda.Stream(dev)
-        gctx = gnx.Context(L, stream=gs)
-        go1, go2 = gctx.vec(), gctx.vec()
-        with torch.cuda.stream(gs):
-            for _ in range(50):
-                gctx.dirac(gnx.OP_M, sm, b, go1)
-                gctx.dirac(gnx.OP_MDAG, sm, go1, go2)
-        gs.synchronize()
-        ref2 = go2.clone()
-        graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(graph, stream=gs):
-            for _ in range(50):
-                gctx.dirac(gnx.OP_M, sm, b, go1)
-                gctx.dirac(gnx.OP_MDAG, sm, go1, go2)
-        same = True
-        for _ in range(3):
-            go2.zero_()
-            graph.replay()
-            gs.synchronize()
-            same = same and bool(torch.equal(go2, ref2))
-        g0, g1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        g0.record(gs)
-        for _ in range(10):
-            graph.replay()
-        g1.record(gs)
-        g1.synchronize()
-        t_g = g0.elapsed_time(g1) * 1e-3 / 1000
-        ws = 2 * 16 * V + 8 * V  # in, out, sigma mass
-        dirac_graph = {"apps_per_s": 1.0 / t_g, "GB_s": BYTES_DIRAC_F64 * V / t_g / 1e9,
-                       "frac_hbm": BYTES_DIRAC_F64 * V / t_g / 1e9 / HBM_PEAK_GBS,
-                       "replays_match_eager": same, "applications_per_graph": 100,
-                       "note": f"working set {ws / 2**20:.1f} MiB: served from the 256 MB Infinity Cache "
-                               "when it fits there, so frac_hbm can exceed 1"}
-        del graph, gctx, go1, go2, ref2
-    except Exception as exc:  # the capture is a measurement aid, not the product path
-        dirac_graph = {"error": str(exc)}
+    dirac_graph = dirac_graph_rate(gnx, torch, L, sm, b)
     torch.cuda.synchronize()
     # the normal operators as the drop-in applies them (two launches each):
     # M M^dag (the CG's operator) and M^dag M (offered beside it)
@@ -559,7 +762,7 @@ def main():
         prod[name] = {"apps_per_s": 1.0 / tp, "GB_s": 2 * BYTES_DIRAC_F64 * V / tp / 1e9, "launches": 2}
     # batched: 16 vectors per launch, one sigma mass (2 s_v + s_m / n per site)
     nvec = 16
-    vb = torch.from_numpy(np.where(np.random.default_rng(77).random(2 * V * nvec) <= 0.5, 1.0, -1.0)).to(dev)
+    vb = torch.from_numpy(make_inputs(L, seed=77, nsrc=nvec)[1]).to(dev)
     ob = ctx.vec(nvec)
     wb = ctx.vec(nvec)
     for _ in range(4):
@@ -617,7 +820,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic: sigma iid N(0, 2/(4*0.85)), complex Z2 source, numpy seed 1231",
+        "data": "synthetic: the reference's random stream (dSFMT-132049, Initrand(1231)): sigma by "
+                "Gauss2MeanVariance(0, 2/(4*0.85)) pairs, complex Z2 source by Casuale() <= 0.5",
         "config": {"workload": f"single-RHS FP64 CG on M M^dag to |r|<{args.tol:g} at {L}^3",
                    "L": L, "V": V, "tol": args.tol, "start": "imposed", "cg_iterations": it,
                    "true_residual": true_res,
@@ -655,8 +859,7 @@ def main():
     # ---------------- batched FP64 CG ----------------
     batched = {}
     for nb in [int(v) for v in args.batch.split(",") if v]:
-        rng = np.random.default_rng(7 + nb)
-        B = torch.from_numpy(np.where(rng.random(2 * V * nb) <= 0.5, 1.0, -1.0)).to(dev)
+        B = torch.from_numpy(make_inputs(L, seed=7 + nb, nsrc=nb)[1]).to(dev)
         X = ctx.vec(nb)
         ctx.cg_batch(sm, B, X, nb, args.tol, start=gnx.START_IMPOSED)
         torch.cuda.synchronize()
@@ -680,8 +883,7 @@ def main():
     for pname in [v for v in args.mixed.split(",") if v]:
         pcode = precs[pname]
         for nb in [1] + [int(v) for v in args.batch.split(",") if v]:
-            rng = np.random.default_rng(11 + nb)
-            B = b if nb == 1 else torch.from_numpy(np.where(rng.random(2 * V * nb) <= 0.5, 1.0, -1.0)).to(dev)
+            B = b if nb == 1 else torch.from_numpy(make_inputs(L, seed=11 + nb, nsrc=nb)[1]).to(dev)
             X = ctx.vec(nb)
             ctx.cg_batch(sm, B, X, nb, args.tol, start=gnx.START_IMPOSED, prec=pcode)
             torch.cuda.synchronize()
@@ -720,6 +922,14 @@ def main():
                                                       ntraj=args.chain_trajectories)
         result["uneven_batch"] = uneven_leg(gnx=gnx, torch=torch, sizes=[int(v) for v in args.chains.split(",")],
                                             tol=args.tol)
+    if args.sizes:
+        result["sizes"] = {}
+        for spec in args.sizes.split(","):
+            Ls, n = (int(v) for v in spec.split(":"))
+            result["sizes"][f"{Ls}^3"] = size_leg(gnx, torch, Ls, args.tol, args.steps, n,
+                                                  [v for v in args.mixed.split(",") if v])
+    if not args.no_probes:
+        result["probes"] = probes_leg(L)
     if args.copy_calibration > 0:
         n = args.copy_calibration * (1 << 20) // 8
         src = torch.ones(n, dtype=torch.float64, device=dev)
@@ -736,7 +946,7 @@ def main():
                                       "GB_s": 16 * n / t / 1e9}
         del src, dst
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(L, sigma_h, b_h, it)
+        result["cpu_baseline"] = cpu_baseline(L, sigma_h, b_h)
     print(json.dumps(result), flush=True)
     return 0
 

@@ -40,13 +40,16 @@ void gnx_rng_gauss2mv(gnx_rng *r, double mean, double variance, double *r1, doub
  *                        (x, x + n/2) for x < n/2 (sigma.c:29-30)
  *   GNX_DRAW_Z2_COMPLEX  out[2n]: per site re then im, each Casuale() <= 0.5 ? 1 : -1
  *   GNX_DRAW_Z2_REAL     out[n]: per site Casuale() <= 0.5 ? 1 : -1 (condensate.c:52-60)
+ *   GNX_DRAW_GAUSS2_PAIRS out[n], n even: Gauss2MeanVariance(0, var) pairs into
+ *                        (2r, 2r + 1) (link momenta, moleculardynamics.c:44)
  */
 typedef enum gnx_draw {
     GNX_DRAW_CASUALE = 0,
     GNX_DRAW_GAUSS1 = 1,
     GNX_DRAW_SIGMA = 2,
     GNX_DRAW_Z2_COMPLEX = 3,
-    GNX_DRAW_Z2_REAL = 4
+    GNX_DRAW_Z2_REAL = 4,
+    GNX_DRAW_GAUSS2_PAIRS = 5
 } gnx_draw;
 gnx_status gnx_rng_fill(gnx_rng *r, gnx_draw kind, long n, double var, double *out);
 

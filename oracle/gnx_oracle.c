@@ -90,6 +90,36 @@ void orc_geom_free(orc_geom *g)
     memset(g, 0, sizeof(*g));
 }
 
+/* coordinate.c:253-267: the lex index of f-site fermion, by testing every
+ * lex site (no early exit, as there) */
+static int lex_by_search(const orc_geom *g, int fermion)
+{
+    int stereo = 0;
+    for (int x = 0; x < g->V; x++) {
+        int x2 = x % g->LT, q = x / g->LT, x1 = q % g->L, x0 = q / g->L;
+        if (f_of(g->L, g->LT, g->half, x0, x1, x2) == fermion) stereo = x;
+    }
+    return stereo;
+}
+
+/* coordinate.c:322-374 for every site (lattice.c:32-33): one search per table */
+void orc_setup_tables_by_search(const orc_geom *g, int *fpp, int *fmm)
+{
+    int ext[3] = {g->L, g->L, g->LT};
+    for (int f = 0; f < g->V; f++) {
+        for (int dir = 0; dir < 2; dir++) {
+            int n = lex_by_search(g, f);
+            int c[3] = {n / g->LT / g->L, (n / g->LT) % g->L, n % g->LT};
+            int *dst = dir == 0 ? fpp : fmm;
+            for (int d = 0; d < 3; d++) {
+                int t[3] = {c[0], c[1], c[2]};
+                t[d] = dir == 0 ? (c[d] + 1) % ext[d] : (c[d] + ext[d] - 1) % ext[d];
+                dst[3 * f + d] = f_of(g->L, g->LT, g->half, t[0], t[1], t[2]);
+            }
+        }
+    }
+}
+
 /* ======================================================================= */
 /* dSFMT-132049 (Saito & Matsumoto, "SIMD-oriented Fast Mersenne Twister   */
 /* for double precision"), restated from the published algorithm; the     */

@@ -67,6 +67,12 @@ typedef struct orc_geom {
 
 int  orc_geom_init(orc_geom *g, int L, int LT);
 void orc_geom_free(orc_geom *g);
+/* The reference's own table set-up, by linear search (coordinate.c:253-267
+ * called twice per site through InitializeFermionPP/MMArray, coordinate.c:
+ * 322-374, from StartSite, lattice.c:32-33): O(V^2).  Fills fpp and fmm
+ * [V][3] in f-order; timed by the benchmark's CPU baseline as the set-up the
+ * reference pays before its first solve. */
+void orc_setup_tables_by_search(const orc_geom *g, int *fpp, int *fmm);
 
 /* ---------------- host RNG (random.c, dSFMT-132049) ---------------------- */
 #define ORC_DSFMT_N 1269

@@ -68,6 +68,8 @@ def lib():
         K = C.POINTER(Kit)
         L.orc_geom_init.argtypes = [G, C.c_int, C.c_int]
         L.orc_geom_free.argtypes = [G]
+        ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+        L.orc_setup_tables_by_search.argtypes = [G, ip, ip]
         L.orc_rng_init.argtypes = [R, C.c_uint32]
         L.orc_casuale.argtypes = [R]
         L.orc_casuale.restype = C.c_double
@@ -138,6 +140,13 @@ class Lattice:
     def table(self, name, n):
         p = getattr(self.g, name)
         return np.ctypeslib.as_array(p, shape=(n,)).copy()
+
+    def tables_by_search(self):
+        """The reference's O(V^2) set-up of the neighbour tables (fpp, fmm)."""
+        fpp = np.zeros(3 * self.V, np.int32)
+        fmm = np.zeros(3 * self.V, np.int32)
+        lib().orc_setup_tables_by_search(C.byref(self.g), fpp, fmm)
+        return fpp, fmm
 
     def sigma_mass(self, sigma):
         sm = np.empty(self.V)

@@ -211,3 +211,18 @@ def test_terma_checkpoint_format_golden(golden, P):
         np.testing.assert_array_equal(golden[f"terma_{case}_info"],
                                       [0.02, 3, 5, 7, 5 / 7, 11, 13, 17, 13 / 17, 0.05, 19, 23, 29, 23 / 29, 31])
         assert np.array_equal(golden[f"terma_{case}_U"], golden["link_U"])
+
+
+def test_setup_by_search_matches_reference_tables(golden, P):
+    """The oracle's restatement of the reference's O(V^2) table set-up (timed
+    by the benchmark's CPU baseline) builds the reference's own tables at
+    8^3 and the closed-form tables at non-cubic shapes."""
+    lat = P.Lattice(8)
+    fpp, fmm = lat.tables_by_search()
+    assert np.array_equal(fpp, golden["fcor_fermionpp"])
+    assert np.array_equal(fmm, golden["fcor_fermionmm"])
+    for L, LT in ((4, 6), (6, 4), (6, 10)):
+        lat = P.Lattice(L, LT)
+        fpp, fmm = lat.tables_by_search()
+        assert np.array_equal(fpp, lat.table("fpp", 3 * lat.V))
+        assert np.array_equal(fmm, lat.table("fmm", 3 * lat.V))

@@ -30,6 +30,9 @@ def test_bulk_draws_match_oracle(P, gnx_mod, L, seed):
     assert np.array_equal(r.z2_real(V), o.z2_real(V)[0::2])
     assert np.array_equal(r.gauss1(1000), np.array([o.gauss1() for _ in range(1000)]))
     assert np.array_equal(r.casuale(10), o.casuale(10))
+    pairs = np.array([v for _ in range(3 * V // 2) for v in o.gauss2mv(0.0, 1.0)]) if V <= 4096 else None
+    if pairs is not None:
+        assert np.array_equal(r.gauss2_pairs(3 * V), pairs)
 
 
 def test_bad_arguments(gnx_mod):
