@@ -36,6 +36,17 @@ __global__ void k_geometry_tables(Geom g, int* fpp, int* fmm, double* eta_pp, do
   if (f2lex) f2lex[f] = s.lex;
 }
 
+// the residual drop that triggers a mixed solve's FP64 reliable update:
+// the context's setting, else the default per storage format, measured at
+// 64^3 x 32 sources, tol 1e-10: FP32 needs the same iterations for any
+// delta, so few updates (0.05); BF16 loses direction quality fast and needs
+// frequent true-residual refreshes (0.2: 447 iterations, 19 updates,
+// against 549 at 0.1; the rate peaks at 0.15-0.2); FP16, three mantissa
+// bits more, as FP32
+double gnx::reliable_delta_of(const gnx_ctx* c, int prec) {
+  return c->reliable_delta > 0.0 ? c->reliable_delta : (prec == GNX_PREC_BF16 ? 0.2 : 0.05);
+}
+
 hipError_t gnx::cg_run(gnx_ctx* c, const double* sm, const double2* b, double2* x, double tol, long max_iter,
                       int start, int nrhs, CGDevResult* host_res) {
   return cgm_f64_run(&c->cgm, c->g, sm, nullptr, b, x, tol, max_iter, start, nrhs, c->stream, host_res);
@@ -179,12 +190,7 @@ gnx_status gnx_cg_batch(gnx_ctx* c, const double* sm, const double* b, double* x
   HIP_TRY(hipSetDevice(c->device));
   if (prec == GNX_PREC_F32 || prec == GNX_PREC_BF16 || prec == GNX_PREC_F16) {
     CGDevResult hr[64];
-    // default drop factors measured at 64^3 x 32 sources, tol 1e-10: FP32
-    // needs the same iterations for any delta, so few updates (0.05); BF16
-    // loses direction quality fast and needs frequent true-residual
-    // refreshes (0.2: 447 iterations, 19 updates, against 549 at 0.1; the
-    // rate peaks at 0.15-0.2); FP16, three mantissa bits more, as FP32
-    const double delta = c->reliable_delta > 0.0 ? c->reliable_delta : (prec == GNX_PREC_BF16 ? 0.2 : 0.05);
+    const double delta = reliable_delta_of(c, (int)prec);
     HIP_TRY(cgmix_run(&c->cgmix, c->g, sm, (const double2*)b, (double2*)x, tol, delta, max_iter,
                       (int)start, (int)prec, nrhs, c->stream, hr));
     int all = 1;

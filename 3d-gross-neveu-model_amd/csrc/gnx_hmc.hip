@@ -32,6 +32,7 @@ struct gnx_hmc {
   double2* box_phase = nullptr; // 3*E     exp(i 2pi x_d/L) tables, E = max(L, LT)
   double iterations = 0.0;      // kit->iterations
   int calls = 0;                // kit->CG_call
+  int force_prec = GNX_PREC_F64;  // arithmetic of the MEDIUM-tolerance force solves
 };
 
 namespace {
@@ -97,10 +98,14 @@ __global__ __launch_bounds__(BS) void k_condsig_final(int nparts, const double* 
 inline int nblk(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
 gnx_status hmc_cg(gnx_hmc* h, const double2* b, double2* x, double tol, long max_iter, int start, int nrhs,
-                  bool* ok) {
+                  bool* ok, int prec = GNX_PREC_F64) {
   gnx_ctx* c = h->c;
   CGDevResult r[2];
-  HIP_TRY(cg_run(c, h->sm, b, x, tol, max_iter, start, nrhs, r));
+  if (prec == GNX_PREC_F64)
+    HIP_TRY(cg_run(c, h->sm, b, x, tol, max_iter, start, nrhs, r));
+  else
+    HIP_TRY(cgmix_run(&c->cgmix, c->g, h->sm, b, x, tol, reliable_delta_of(c, prec), max_iter, start, prec, nrhs,
+                      c->stream, r));
   *ok = true;
   for (int k = 0; k < nrhs; k++) {
     if (r[k].converged && !r[k].early_exit) {
@@ -125,9 +130,10 @@ gnx_status hmc_force_post(gnx_hmc* h) {
   return GNX_OK;
 }
 
-// ComputeSigmaForce (moleculardynamics.c:631-647) on the device state
-gnx_status hmc_force(gnx_hmc* h, double tol, long max_iter, bool* ok) {
-  gnx_status s = hmc_cg(h, h->phi, h->X, tol, max_iter, GNX_START_NULL, 2, ok);
+// ComputeSigmaForce (moleculardynamics.c:631-647) on the device state; the
+// solves in prec (FP64 unless the MEDIUM-tolerance solves are mixed)
+gnx_status hmc_force(gnx_hmc* h, double tol, long max_iter, bool* ok, int prec = GNX_PREC_F64) {
+  gnx_status s = hmc_cg(h, h->phi, h->X, tol, max_iter, GNX_START_NULL, 2, ok, prec);
   if (s != GNX_OK || !*ok) return s;
   return hmc_force_post(h);
 }
@@ -324,24 +330,24 @@ gnx_status gnx_hmc_trajectory(gnx_hmc* h, const double* mom_host, const double* 
   double H0 = 0.0, H1 = 0.0;
   if ((s = hmc_energy(h, &H0)) != GNX_OK) return s;
 
-  TRY_CG(hmc_force(h, tol_medium, max_iter, &ok));
+  TRY_CG(hmc_force(h, tol_medium, max_iter, &ok, h->force_prec));
   P_STEP((lam * epsilon));
   Q_STEP((epsilon / 2.0));
-  TRY_CG(hmc_force(h, tol_medium, max_iter, &ok));
+  TRY_CG(hmc_force(h, tol_medium, max_iter, &ok, h->force_prec));
   P_STEP(((1.0 - (2.0 * lam)) * epsilon));
   Q_STEP((epsilon / 2.0));
   for (int st = 1; st < step - 1; st++) {
-    TRY_CG(hmc_force(h, tol_medium, max_iter, &ok));
+    TRY_CG(hmc_force(h, tol_medium, max_iter, &ok, h->force_prec));
     P_STEP((2.0 * lam * epsilon));
     Q_STEP((epsilon / 2.0));
-    TRY_CG(hmc_force(h, tol_medium, max_iter, &ok));
+    TRY_CG(hmc_force(h, tol_medium, max_iter, &ok, h->force_prec));
     P_STEP(((1.0 - (2.0 * lam)) * epsilon));
     Q_STEP((epsilon / 2.0));
   }
-  TRY_CG(hmc_force(h, tol_medium, max_iter, &ok));
+  TRY_CG(hmc_force(h, tol_medium, max_iter, &ok, h->force_prec));
   P_STEP((2.0 * lam * epsilon));
   Q_STEP((epsilon / 2.0));
-  TRY_CG(hmc_force(h, tol_medium, max_iter, &ok));
+  TRY_CG(hmc_force(h, tol_medium, max_iter, &ok, h->force_prec));
   P_STEP(((1.0 - (2.0 * lam)) * epsilon));
   Q_STEP((epsilon / 2.0));
   TRY_CG(hmc_force(h, tol_high, max_iter, &ok));
@@ -357,6 +363,14 @@ gnx_status gnx_hmc_trajectory(gnx_hmc* h, const double* mom_host, const double* 
   res->ok = 1;
   res->cg_solves = h->calls - calls0;
   res->cg_iterations = (long)(h->iterations - it0);
+  return GNX_OK;
+}
+
+gnx_status gnx_hmc_set_force_precision(gnx_hmc* h, gnx_prec prec) {
+  CHECK_ARG(h, "gnx_hmc_set_force_precision: null argument");
+  CHECK_ARG(prec == GNX_PREC_F64 || prec == GNX_PREC_F32 || prec == GNX_PREC_BF16 || prec == GNX_PREC_F16,
+            "gnx_hmc_set_force_precision: bad precision");
+  h->force_prec = (int)prec;
   return GNX_OK;
 }
 

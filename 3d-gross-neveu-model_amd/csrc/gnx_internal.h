@@ -97,6 +97,8 @@ namespace gnx {
 // FP64 CG on the context's engine; host_res[nrhs] filled on return
 hipError_t cg_run(gnx_ctx* c, const double* sm, const double2* b, double2* x, double tol, long max_iter, int start,
                   int nrhs, CGDevResult* host_res);
+// residual drop that triggers a mixed solve's FP64 reliable update
+double reliable_delta_of(const gnx_ctx* c, int prec);
 // the same on the link-coupled operator (links f-order [3V])
 hipError_t cg_run_link(gnx_ctx* c, const double* sm, const double2* U, const double2* b, double2* x, double tol,
                        long max_iter, int start, int nrhs, CGDevResult* host_res);

@@ -21,12 +21,13 @@
 //
 // C ABI (host functions; all return hipError_t as int, 0 on success):
 //   int gnx_probe_graph_nodes(int kind, long sites, int nodes, int reps, double* us_per_node);
-//   int gnx_probe_grid_barrier(int blocks, int iters, long bytes_per_block, double* us_per_barrier,
-//                              int* max_blocks, int* failed);
+//   int gnx_probe_grid_barrier(int mode, int blocks, int iters, long bytes_per_block,
+//                              double* us_per_barrier, int* max_blocks, int* failed);
+//   mode 0: every poll an acquire load; mode 1: relaxed polls, one acquire fence
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-namespace {
+namespace gnx_probe {
 
 constexpr int kBS = 256;
 
@@ -68,24 +69,31 @@ struct BarrierState {
 
 constexpr long kSpinLimit = 1L << 22;
 
+// mode 0 (POLL_ACQUIRE): every poll of the generation word is a
+// device-scope acquire load (each one invalidates the polling XCD's L2);
+// mode 1: relaxed polls and one acquire fence after the wait.  The arrival
+// is a device-scope release either way (the L2 write-back of the block's
+// data).
+template <bool POLL_ACQUIRE>
 __device__ void grid_barrier(BarrierState* bs, unsigned nblocks) {
   __syncthreads();
   if (threadIdx.x == 0) {
     const unsigned g = __hip_atomic_load(&bs->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned arrived =
-        __hip_atomic_fetch_add(&bs->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);  // release our writes
+    const unsigned arrived = __hip_atomic_fetch_add(&bs->count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     if (arrived == nblocks - 1) {
       __hip_atomic_store(&bs->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&bs->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&bs->gen, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     } else {
       long spins = 0;
-      while (__hip_atomic_load(&bs->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+      while (__hip_atomic_load(&bs->gen, POLL_ACQUIRE ? __ATOMIC_ACQUIRE : __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT) == g) {
         if (++spins > kSpinLimit) {
           __hip_atomic_store(&bs->failed, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           break;
         }
         __builtin_amdgcn_s_sleep(1);
       }
+      if (!POLL_ACQUIRE) __atomic_thread_fence(__ATOMIC_ACQUIRE);
     }
   }
   __syncthreads();
@@ -94,12 +102,13 @@ __device__ void grid_barrier(BarrierState* bs, unsigned nblocks) {
 // each block writes its bytes (vector stores over its own slice), then waits
 // at the barrier; iters rounds.  Once a wait has timed out, every block
 // leaves at its next round.
+template <bool POLL_ACQUIRE>
 __global__ __launch_bounds__(kBS) void k_barrier_loop(BarrierState* bs, int iters, double2* slice, long elems_per_block) {
   __shared__ int stop;
   double2* mine = slice + (long)blockIdx.x * elems_per_block;
   for (int it = 0; it < iters; it++) {
     for (long i = threadIdx.x; i < elems_per_block; i += kBS) mine[i] = make_double2((double)it, (double)i);
-    grid_barrier(bs, gridDim.x);
+    grid_barrier<POLL_ACQUIRE>(bs, gridDim.x);
     if (threadIdx.x == 0) stop = __hip_atomic_load(&bs->failed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     if (stop) return;
@@ -113,7 +122,9 @@ int elapsed_us(hipEvent_t a, hipEvent_t b, double* us) {
   return (int)e;
 }
 
-}  // namespace
+}  // namespace gnx_probe
+
+using namespace gnx_probe;
 
 extern "C" {
 
@@ -177,14 +188,16 @@ int gnx_probe_graph_nodes(int kind, long sites, int nodes, int reps, double* us_
   return (int)e;
 }
 
-int gnx_probe_grid_barrier(int blocks, int iters, long bytes_per_block, double* us_per_barrier, int* max_blocks,
-                           int* failed) {
-  if (blocks < 1 || iters < 1 || bytes_per_block < 0 || !us_per_barrier || !max_blocks || !failed)
+int gnx_probe_grid_barrier(int mode, int blocks, int iters, long bytes_per_block, double* us_per_barrier,
+                           int* max_blocks, int* failed) {
+  if (mode < 0 || mode > 1 || blocks < 1 || iters < 1 || bytes_per_block < 0 || !us_per_barrier || !max_blocks ||
+      !failed)
     return (int)hipErrorInvalidValue;
+  const void* kern = mode == 0 ? (const void*)k_barrier_loop<true> : (const void*)k_barrier_loop<false>;
   int dev = 0, cus = 0, per_cu = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_barrier_loop, kBS, 0);
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBS, 0);
   if (e != hipSuccess) return (int)e;
   *max_blocks = cus * per_cu;
   // every block must be resident at once, or the barrier could never fill
@@ -199,25 +212,28 @@ int gnx_probe_grid_barrier(int blocks, int iters, long bytes_per_block, double* 
   if (e == hipSuccess) e = hipMalloc(&slice, sizeof(double2) * (elems > 0 ? elems : 1) * blocks);
   if (e == hipSuccess) e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
-  double us[2] = {0.0, 0.0};
-  int rounds[2] = {1, iters + 1};
-  BarrierState zero = {0u, 0u, 0, 0};
+  double us[3] = {0.0, 0.0, 0.0};
+  int rounds[3] = {1, 1, iters + 1};
+  const BarrierState zero = {0u, 0u, 0, 0};
+  int fail = 0;
   // the time of iters barriers is the difference of two launches (1 and
-  // iters + 1 rounds), which removes the launch and the first round
-  for (int k = 0; k < 2 && e == hipSuccess; k++) {
+  // iters + 1 rounds), which removes the launch and the first round; the
+  // first one-round launch is a warm-up
+  for (int k = 0; k < 3 && e == hipSuccess; k++) {
     e = hipMemcpyAsync(bs, &zero, sizeof zero, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipEventRecord(e0, st);
     void* args[] = {(void*)&bs, (void*)&rounds[k], (void*)&slice, (void*)&elems};
-    if (e == hipSuccess) e = hipLaunchCooperativeKernel((const void*)k_barrier_loop, dim3(blocks), dim3(kBS), args, 0, st);
+    if (e == hipSuccess) e = hipLaunchCooperativeKernel(kern, dim3(blocks), dim3(kBS), args, 0, st);
     if (e == hipSuccess) e = hipEventRecord(e1, st);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     if (e == hipSuccess) e = (hipError_t)elapsed_us(e0, e1, &us[k]);
+    BarrierState got = zero;
+    if (e == hipSuccess) e = hipMemcpy(&got, bs, sizeof got, hipMemcpyDeviceToHost);
+    fail |= got.failed;
   }
-  BarrierState out = zero;
-  if (e == hipSuccess) e = hipMemcpy(&out, bs, sizeof out, hipMemcpyDeviceToHost);
   if (e == hipSuccess) {
-    *failed = out.failed;
-    *us_per_barrier = (us[1] - us[0]) / iters;
+    *failed = fail;
+    *us_per_barrier = (us[2] - us[1]) / iters;
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);

@@ -107,6 +107,7 @@ def lib():
         L.gnx_hmc_trajectory.argtypes = [vp, dp, dp, dp, C.c_double, C.c_int, C.c_double, C.c_double,
                                          C.c_long, C.POINTER(TrajResult)]
         L.gnx_hmc_accept.argtypes = [vp, C.c_int]
+        L.gnx_hmc_set_force_precision.argtypes = [vp, C.c_int]
         L.gnx_hmc_condensate.argtypes = [vp, dp, C.c_double, C.c_long, dp]
         L.gnx_hmc_condensate_sigma.argtypes = [vp, dp]
         L.gnx_hmc_cg_stats.argtypes = [vp, dp, C.POINTER(C.c_int)]
@@ -416,6 +417,10 @@ class HMC:
 
     def accept(self, accepted):
         _check(lib().gnx_hmc_accept(self.h, int(accepted)))
+
+    def set_force_precision(self, prec):
+        """Arithmetic of the MEDIUM-tolerance force solves (PREC_F64 default)."""
+        _check(lib().gnx_hmc_set_force_precision(self.h, prec))
 
     def condensate(self, z2, tol, max_iter=0):
         import numpy as np

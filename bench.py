@@ -79,16 +79,22 @@ def recorded_traffic(L, shape):
     return t.get("bytes_per_iteration", {}).get(shape_tag(shape))
 
 
-def summarize_rocprof(kt_dir, fetch_dir, write_dir, L):
+def summarize_rocprof(kt_dir, fetch_dir, write_dir, L, untraced=None):
     """Summaries of rocprofv3 runs of this bench (no GPU needed): the
     kernel-trace statistics of the CG kernels, and HBM bytes per CG
     iteration from FETCH_SIZE / WRITE_SIZE passes, each scaled by what the
-    counter reads for the plain device copy of --copy-calibration."""
+    counter reads for the plain device copy of --copy-calibration.  With
+    `untraced` (the JSON line of an unprofiled run of the same build), the
+    traced durations are reconciled with the unprofiled clock: the tracer
+    lengthens each dispatch, by an amount the graph-node probes measure
+    (their traced duration against their unprofiled time per node), and the
+    CG kernels' durations less that amount must add up to the unprofiled
+    time per iteration."""
     import csv
     import statistics
 
     def short(n):
-        return n.split("(")[0].replace("void ", "").replace("gnx::", "")
+        return n.split("(")[0].replace("void ", "").replace("gnx::", "").replace("gnx_probe::", "")
 
     out = []
     stats = list(csv.DictReader(open(os.path.join(kt_dir, "run_kernel_stats.csv"))))
@@ -97,6 +103,38 @@ def summarize_rocprof(kt_dir, fetch_dir, write_dir, L):
     for r in stats[:16]:
         out.append(f"| `{short(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']):.0f} | {r['MinNs']} | "
                    f"{r['MaxNs']} | {float(r['Percentage']):.2f} |")
+    avg = {short(r["Name"]): float(r["AverageNs"]) / 1e3 for r in stats}
+    if untraced:
+        u = json.load(open(untraced))
+        nodes = u["probes"]["graph_nodes"]
+        probe_kernel = {"empty": "k_empty", "stream_40B": "k_stream40", "stream_152B": "k_stream152"}
+        out.append("")
+        out.append("Tracer inflation, from the graph-node probes (64 nodes per graph, each node reading the "
+                   "previous node's output; traced average duration against the unprofiled time per node of "
+                   f"the same build's bench line, `{os.path.basename(untraced)}`):")
+        out.append("")
+        out.append("| Probe kernel | Traced (us) | Unprofiled per node (us) | Inflation (us) |")
+        out.append("|---|---:|---:|---:|")
+        infl = {}
+        for name, k in probe_kernel.items():
+            if k in avg and name in nodes:
+                infl[name] = avg[k] - nodes[name]["us_per_node"]
+                out.append(f"| `{k}` | {avg[k]:.2f} | {nodes[name]['us_per_node']:.2f} | {infl[name]:+.2f} |")
+        cg = [k for k in avg if k.startswith(("k_cg1_X", "k_cg1_Y", "k_cgm_A", "k_cgm_B"))]
+        xk = [k for k in cg if k.startswith(("k_cg1_X", "k_cgm_A"))]
+        yk = [k for k in cg if k.startswith(("k_cg1_Y", "k_cgm_B"))]
+        if xk and yk and "stream_40B" in infl and "stream_152B" in infl:
+            x_est = avg[xk[0]] - infl["stream_40B"]
+            y_est = avg[yk[0]] - infl["stream_152B"]
+            per_it = u["roofline"]["us_per_iteration"]
+            frac = BYTES_CG_ITER_F64 * L ** 3 / ((x_est + y_est) * 1e-6) / 1e9 / HBM_PEAK_GBS
+            out.append("")
+            out.append(f"CG iteration: `{xk[0]}` {avg[xk[0]]:.2f} us traced, {x_est:.2f} us less the "
+                       f"40 B/site probe's inflation; `{yk[0]}` {avg[yk[0]]:.2f} us traced, {y_est:.2f} us less "
+                       f"the 152 B/site probe's inflation; sum {x_est + y_est:.2f} us against {per_it:.2f} us per "
+                       f"iteration unprofiled (HIP events over whole solves, graph boundaries and start kernels "
+                       f"included).  From the profile: roofline frac {frac:.3f} (144 B/site); the bench line: "
+                       f"{u['roofline']['frac']:.3f}.")
     counters = {}
     for ctr, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
         per = {}
@@ -258,12 +296,14 @@ def cpu_baseline(L, sigma, b, nsolves=5):
     }
 
 
-def probes_leg(L=64):
+def probes_leg(L=64, barrier=True):
     """The measurements behind the CG schedule (lib/libgnx_probes.so): the
     cost of one node of a replayed hipGraph (an empty kernel, and streaming
     kernels of 40 and 152 B/site at L^3, each reading the previous node's
     output), and of a grid-wide barrier inside one persistent kernel at
-    several block counts, with and without bytes written between barriers."""
+    several block counts, with and without bytes written between barriers
+    (barrier=False skips the barrier's cooperative launches, which the
+    profiler's kernel tracing does not survive)."""
     import ctypes as C
     lib = C.CDLL(os.path.join(ROOT, "3d-gross-neveu-model_amd", "lib", "libgnx_probes.so"))
     V = L ** 3
@@ -279,15 +319,19 @@ def probes_leg(L=64):
             d["GB_s"] = bps * V / (us.value * 1e-6) / 1e9
             d["sites"] = V
         out["graph_nodes"][name] = d
-    for blocks in (256, 512, 1024):
-        for per_block in (0, 32768):
-            us, mx, failed = C.c_double(), C.c_int(), C.c_int()
-            rc = lib.gnx_probe_grid_barrier(blocks, 200, C.c_long(per_block), C.byref(us), C.byref(mx),
-                                            C.byref(failed))
-            out["grid_barrier"].append({"blocks": blocks, "bytes_per_block_between_barriers": per_block,
-                                        "us_per_barrier": us.value if rc == 0 else None,
-                                        "resident_blocks_max": mx.value, "wait_timed_out": bool(failed.value),
-                                        "rc": rc})
+    if not barrier:
+        return out
+    for mode, poll in ((0, "acquire load per poll"), (1, "relaxed polls, one acquire fence")):
+        for blocks in (256, 512, 1024):
+            for per_block in (0, 32768):
+                us, mx, failed = C.c_double(), C.c_int(), C.c_int()
+                rc = lib.gnx_probe_grid_barrier(mode, blocks, 200, C.c_long(per_block), C.byref(us), C.byref(mx),
+                                                C.byref(failed))
+                out["grid_barrier"].append({"polling": poll, "blocks": blocks,
+                                            "bytes_per_block_between_barriers": per_block,
+                                            "us_per_barrier": us.value if rc == 0 else None,
+                                            "resident_blocks_max": mx.value, "wait_timed_out": bool(failed.value),
+                                            "rc": rc})
     return out
 
 
@@ -358,7 +402,7 @@ def ensemble_leg(gnx, sizes, ntraj, betas=(0.6, 0.75, 0.85, 1.0), eps=0.02, step
     lockstep (gnx.Ensemble: each solve point of an integrator one
     uneven-batch launch over all chains)."""
     import concurrent.futures as cf
-    def make_chains():
+    def make_chains(force_prec=None):
         out = []
         for L in sizes:
             for bs in betas:
@@ -368,6 +412,8 @@ def ensemble_leg(gnx, sizes, ntraj, betas=(0.6, 0.75, 0.85, 1.0), eps=0.02, step
                 ctx = gnx.Context(L)
                 h = gnx.HMC(ctx, beta_s=bs)
                 h.set_sigma(rng.sigma(V, 2.0 / (4 * bs)))
+                if force_prec is not None:
+                    h.set_force_precision(force_prec)
                 lh = gnx.LinkHMC(ctx, beta=1.0, pure_gauge=True, linked=False)
                 _, sm = h.get_sigma()
                 lh.set_fields(np.exp(1j * 2.0 * np.pi * rng.casuale(3 * V)), sm)
@@ -412,6 +458,29 @@ def ensemble_leg(gnx, sizes, ntraj, betas=(0.6, 0.75, 0.85, 1.0), eps=0.02, step
            "chains": [{"L": c["L"], "beta_s": c["beta_s"], "dH_sigma": c["dH_sigma"], "dH_link": c["dH_link"],
                        "accepted_sigma": c["acc_sigma"], "accepted_link": c["acc_link"],
                        "cg_iterations": c["cg_iterations"]} for c in chains]}
+
+    # the same chains and draws, thread per chain, with the MEDIUM-tolerance
+    # force solves in mixed precision (FP16 storage, FP64 reliable updates)
+    mixed = make_chains(gnx.PREC_F16)
+    with cf.ThreadPoolExecutor(max_workers=len(mixed)) as ex:
+        list(ex.map(lambda c: run(c, 1), mixed))
+    for c in mixed:
+        c["dH_sigma"].clear(), c["dH_link"].clear()
+        c["acc_sigma"] = c["acc_link"] = c["cg_iterations"] = 0
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=len(mixed)) as ex:
+        list(ex.map(lambda c: run(c, ntraj), mixed))
+    dtm = time.perf_counter() - t0
+    out["thread_per_chain_f16_force_solves"] = {
+        "mode": "as thread per chain, the MEDIUM-tolerance (1e-3) sigma force solves in FP16 storage with "
+                "FP64 reliable updates; energies and the last force in FP64",
+        "trajectories_per_s": len(mixed) * ntraj / dtm, "seconds": dtm, "speedup_vs_fp64": dt / dtm,
+        "max_abs_dH_sigma_diff_vs_fp64": max(abs(a - b) for c, d in zip(chains, mixed)
+                                             for a, b in zip(c["dH_sigma"], d["dH_sigma"])),
+        "same_accept_decisions": all(c["acc_sigma"] == d["acc_sigma"] for c, d in zip(chains, mixed))}
+    for c in mixed:
+        c["lh"].close()
+        c["h"].close()
 
     # the same chains and draws in lockstep: every solve point of both
     # integrators one uneven-batch launch over all chains
@@ -563,16 +632,19 @@ def dirac_graph_rate(gnx, torch, L, sm, b):
                 gctx.dirac(gnx.OP_M, sm, b, go1)
                 gctx.dirac(gnx.OP_MDAG, sm, go1, go2)
         same = True
-        for _ in range(3):
-            go2.zero_()
-            graph.replay()
-            gs.synchronize()
-            same = same and bool(torch.equal(go2, ref2))
-        g0, g1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        g0.record(gs)
-        for _ in range(10):
-            graph.replay()
-        g1.record(gs)
+        # replay() launches on the current stream: every replay and both
+        # timing events go to gs
+        with torch.cuda.stream(gs):
+            for _ in range(3):
+                go2.zero_()
+                graph.replay()
+                gs.synchronize()
+                same = same and bool(torch.equal(go2, ref2))
+            g0, g1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            g0.record(gs)
+            for _ in range(10):
+                graph.replay()
+            g1.record(gs)
         g1.synchronize()
         t_g = g0.elapsed_time(g1) * 1e-3 / 1000
         ws = 2 * 16 * V + 8 * V  # in, out, sigma mass
@@ -694,16 +766,20 @@ def main():
     ap.add_argument("--sizes", default="32:32,96:16", metavar="L:N,...",
                     help="configurations 2 and 3 at other sizes: lattice L^3, FP64 single and N-source "
                          "batched FP64 and mixed solves ('' to skip)")
-    ap.add_argument("--no-probes", action="store_true", help="skip the graph-node and grid-barrier probes")
+    ap.add_argument("--probes", default="all", choices=["all", "nodes", "none"],
+                    help="graph-node and grid-barrier probes (nodes: graph-node probes only)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also time a plain device copy of MIB MiB (the reference kernel for "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE scaling)")
     ap.add_argument("--summarize-rocprof", nargs=3, metavar=("KT_DIR", "FETCH_DIR", "WRITE_DIR"),
                     help="print the summary of rocprofv3 csv runs of this bench and record the "
                          "measured traffic for later runs (no GPU needed)")
+    ap.add_argument("--untraced", metavar="BENCH_JSON",
+                    help="with --summarize-rocprof: an unprofiled bench line of the same build, to reconcile "
+                         "the traced durations with")
     args = ap.parse_args()
     if args.summarize_rocprof:
-        text, rec = summarize_rocprof(*args.summarize_rocprof, L=args.L)
+        text, rec = summarize_rocprof(*args.summarize_rocprof, L=args.L, untraced=args.untraced)
         print(text)
         with open(TRAFFIC_JSON, "w") as f:
             json.dump(rec, f, indent=1)
@@ -928,8 +1004,8 @@ def main():
             Ls, n = (int(v) for v in spec.split(":"))
             result["sizes"][f"{Ls}^3"] = size_leg(gnx, torch, Ls, args.tol, args.steps, n,
                                                   [v for v in args.mixed.split(",") if v])
-    if not args.no_probes:
-        result["probes"] = probes_leg(L)
+    if args.probes != "none":
+        result["probes"] = probes_leg(L, barrier=args.probes == "all")
     if args.copy_calibration > 0:
         n = args.copy_calibration * (1 << 20) // 8
         src = torch.ones(n, dtype=torch.float64, device=dev)

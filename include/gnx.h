@@ -274,6 +274,13 @@ gnx_status gnx_hmc_trajectory(gnx_hmc *h, const double *mom_host, const double *
                               double tol_medium, long max_iter, gnx_traj_result *res);
 /* accepted = 0 restores sigma and its mass (moleculardynamics.c:1407-1408) */
 gnx_status gnx_hmc_accept(gnx_hmc *h, int accepted);
+/* Arithmetic of the trajectory's MEDIUM-tolerance force solves
+ * (moleculardynamics.c:1279,1291,1308,1321,1335,1348): GNX_PREC_F64 (the
+ * default, the reference's) or mixed precision (FP32, BF16 or FP16 storage
+ * with FP64 reliable updates, converged on the FP64 true residual).  The
+ * HIGH-tolerance solves (:1263, :1362) -- the energies and the last force --
+ * stay FP64.  No counterpart in the reference, which is FP64 throughout. */
+gnx_status gnx_hmc_set_force_precision(gnx_hmc *h, gnx_prec prec);
 /* ComputeCondensateDirac (condensate.c:114-244) with the caller's real Z2
  * draws z2_host[6*V] (4 trace vectors, then the two susceptibility vectors);
  * the first solve starts from Y_0 of the last force, as the driver passes

@@ -111,3 +111,47 @@ def test_sigma_force_full(golden, gnx_mod):
     assert np.max(np.abs(host(F) - gF)) < 1e-9 * np.max(np.abs(gF))
     assert relerr(host(X[:2 * V]), golden["force_X0"]) < 1e-6 or \
         np.linalg.norm(host(X[:2 * V]) - golden["force_X0"]) < 1e-8 * np.linalg.norm(golden["force_X0"])
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16", "f16"])
+@pytest.mark.parametrize("tag,eps,steps,ntraj", [("hmc_e0.1", 0.1, 2, 20), ("hmc_e0.2", 0.2, 2, 20)])
+def test_hmc_golden_mixed_force_solves(golden, P, gnx_mod, tag, eps, steps, ntraj, prec):
+    """The MEDIUM-tolerance force solves (1e-3) in mixed precision, the
+    HIGH-tolerance ones (energies, last force) in FP64: the force solves
+    stop anywhere below 1e-3 on the FP64 true residual instead of at the
+    reference's FP64 iterate, so the trajectories move by a part of that
+    tolerance.  Bars: the same accept decisions as the reference's; Delta-H
+    within 1e-6 of the reference's with FP32 storage and within 5e-4 with
+    the 2-byte storage formats (whose iterates leave the FP64 ones sooner,
+    so the 1e-3 stop lands elsewhere; Delta-H itself is 1e-4..1e-2 here);
+    sigma after the run within 1e-6 (FP32) and 1e-4 (2-byte storage)."""
+    banner(f"hmc {tag} with {prec} force solves")
+    pc = {"f32": gnx_mod.PREC_F32, "bf16": gnx_mod.PREC_BF16, "f16": gnx_mod.PREC_F16}[prec]
+    L = 8
+    V = L ** 3
+    rng = P.Random(1231)
+    sigma0 = rng.sigma(V, P.sigma_variance(0.85))
+    h = gnx_mod.HMC(gnx_mod.Context(L), 0.85)
+    h.set_sigma(sigma0)
+    h.set_force_precision(pc)
+    dh, acc = [], []
+    for t in range(ntraj):
+        mom = np.array([rng.gauss1() for _ in range(V)])
+        chi0 = np.array([rng.gauss1() for _ in range(V)])
+        chi1 = np.array([rng.gauss1() for _ in range(V)])
+        st, r = h.trajectory(mom, chi0, chi1, eps, steps, 1e-9, 1e-3)
+        assert st == 0 and r.ok
+        a = rng.casuale() <= np.exp(-r.dH)
+        h.accept(a)
+        dh.append(r.dH)
+        acc.append(int(a))
+        rng.z2_real(6 * V)  # the measurement's draws, not used here
+    err = np.max(np.abs(np.array(dh) - golden[tag + "_dh"]))
+    sN, _ = h.get_sigma()
+    serr = np.max(np.abs(sN - golden[tag + "_sigmaN"]))
+    banner(f"  max |dH - dH_ref| {err:.2e}, max |sigma - sigma_ref| {serr:.2e}")
+    assert np.array_equal(acc, golden[tag + "_acc"])
+    assert err < (1e-6 if prec == "f32" else 5e-4)
+    assert serr < (1e-6 if prec == "f32" else 1e-4)
+    _, calls = h.cg_stats()
+    assert 0 < calls <= ntraj * (4 * steps + 4)  # prologue exits are not counted (conjugategradient.c:22-27)
