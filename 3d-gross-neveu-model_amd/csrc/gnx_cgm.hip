@@ -60,9 +60,14 @@ struct CGMFix {
   double* part;      // [2 slots][nrhs][nb]
   CGMState* state;   // [2][nrhs]
   CGDevResult* res;  // [nrhs]
+  double2* rring;    // deferred form: CGD_R residual buffers [CGD_R][nrhs][V], the first one r
+  size_t rstride;    // elements between two of them
   int nrhs;
   int nb;            // blocks per system of every kernel of this shape
 };
+
+// residual buffer of ring slot j (the deferred form)
+GNX_DEV double2* ring(const CGMFix& a, int j) { return a.rring + (size_t)j * a.rstride; }
 
 GNX_DEV double* slot(const CGMFix& a, int s, int rhs) {
   return a.part + ((size_t)s * a.nrhs + rhs) * a.nb;
@@ -524,12 +529,209 @@ __global__ __launch_bounds__(CGM_BS) void k_cg1_Y(CGMFix a, int odd, const CGMSt
   if (lead) put_state1(out, 1, it + 1, 0, 0, gam, al, be, s);
 }
 
+// ---- one-reduction form with deferred solution updates ------------------------------
+// The same iterates as k_cg1_X / k_cg1_Y, bit for bit, with fewer bytes
+// written per iteration.  The direction p and the solution x enter no other
+// update: each iteration only needs x += alpha_k p_k, p_k = r_k + beta_k
+// p_{k-1} at the site.  So Y records (alpha_k, beta_k) in the state instead,
+// keeps every residual r_k in a ring of CGD_R buffers (Y(k) reads slot k
+// and writes r_{k+1} to slot k+1), and every CGD_M iterations -- and at the
+// exit -- applies the pending updates at the site in iteration order, the
+// same operations as k_cg1_Y performs one iteration at a time.  Per
+// iteration Y writes s and r (and p and x once per CGD_M iterations) instead
+// of s, p, x and r: the kernel boundary's write-back of dirty L2 lines is
+// the larger part of an iteration's memory time here (a 64 B/site write
+// stream costs as much as a 152 B/site read-and-write one).
+
+// x += alpha_j p_j, p_j = r_j + beta_j p_{j-1} (p = r at iteration 0) for the
+// n pending iterations j0 .. j0 + n - 1, at this thread's sites
+template <int SPT, bool XCD>
+GNX_DEVONLY void cgd_flush(const CGMFix& a, size_t off, const SiteMap<SPT, XCD>& map, int j0, int n, const double* hal,
+                           const double* hbe) {
+  const Geom g = a.g;
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    const int f = map.f(g, i);
+    if (f < 0) continue;
+    double2 rj[CGD_M];
+#pragma unroll
+    for (int j = 0; j < CGD_M; j++)
+      if (j < n) rj[j] = *elem(ring(a, (j0 + j) & (CGD_R - 1)) + off, f);
+    double2 p = *elem(a.p0 + off, f), x = *elem(a.x + off, f);
+#pragma unroll
+    for (int j = 0; j < CGD_M; j++) {
+      if (j >= n) break;
+      p = j0 + j == 0 ? rj[j] : cadd(rj[j], rmul(hbe[j], p));
+      x = cadd(x, rmul(hal[j], p));
+    }
+    st_nt(elem(a.p0 + off, f), p);
+    st_nt(elem(a.x + off, f), x);
+  }
+}
+
+template <int SPT, bool XCD, bool LINK, bool WH>
+__global__ __launch_bounds__(CGM_BS) void k_cgd_X(CGMFix a, int rs, const CGMState* __restrict__ sin,
+                                                   CGMState* __restrict__ sout) {
+  __shared__ double lds[2 * CGM_BS / 64];
+  const Geom g = a.g;
+  const int rhs = blockIdx.y;
+  const size_t off = (size_t)rhs * g.V;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const double2* r = ring(a, rs) + off;
+  double2* qo = a.t + off;
+  const SiteMap<SPT, XCD> map(a.nb);
+  const CGMState s = load_state(sin + rhs);
+  end_of_loads();
+  int fs[SPT];
+  double smv[SPT];
+  double2 rv[SPT][7];
+  double2 Uo[LINK ? SPT : 1][3], Um[LINK ? SPT : 1][3];
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    fs[i] = map.f(g, i);
+    const int fc = fs[i] < 0 ? 0 : fs[i];
+    NbrF n = neighbours_f(g, decode(g, fc));
+    int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
+    smv[i] = *elem(a.sm, fc);
+    if (LINK) load_links(a.U, fc, n, Uo[LINK ? i : 0], Um[LINK ? i : 0]);
+#pragma unroll
+    for (int k = 0; k < 7; k++) rv[i][k] = *elem(r, q[k]);
+  }
+  end_of_loads();
+  CGMState* out = sout + rhs;
+  const CGMState* sp = sin + rhs;
+  if (!s.active) {
+    if (lead) put_state_d(out, 0, s.iters, s.converged, s.early, s.rho, s.alpha, s.beta, s.pend, sp->hal, sp->hbe, s);
+    return;
+  }
+  double ag = 0.0, ad = 0.0;
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    if (fs[i] < 0) continue;
+    const Site st = decode(g, fs[i]);
+    const double2 qq = LINK ? link_hops<-1>(g, st, rmul(smv[i], rv[i][0]), &rv[i][1], &rv[i][4], Uo[LINK ? i : 0],
+                                            Um[LINK ? i : 0])
+                            : apply_Mdag_fast_vals(g, st, smv[i], rv[i][0], &rv[i][1], &rv[i][4]);
+    st_nt(elem(qo, fs[i]), qq);
+    ag += cnorm2(rv[i][0]);
+    ad += cnorm2(qq);
+  }
+  const double2 t = block_sum2(ag, ad, lds);
+  if (threadIdx.x == 0) pair_slots(a, rhs)[partial_slot(blockIdx.x, a.nb)] = t;
+  if (lead) put_state_d(out, 1, s.iters, 0, 0, s.rho, s.alpha, s.beta, s.pend, sp->hal, sp->hbe, s);
+}
+
+template <int SPT, bool XCD, bool LINK, bool WH>
+__global__ __launch_bounds__(CGM_BS) void k_cgd_Y(CGMFix a, int rs, const CGMState* __restrict__ sin,
+                                                   CGMState* __restrict__ sout) {
+  __shared__ double lds[2 * CGM_BS / 64];
+  const Geom g = a.g;
+  const int rhs = blockIdx.y;
+  const size_t off = (size_t)rhs * g.V;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const double2* qv = a.t + off;
+  const double2* rk = ring(a, rs) + off;
+  double2* rn = ring(a, (rs + 1) & (CGD_R - 1)) + off;
+  double2* sv = a.p1 + off;
+  const double2* pp = pair_slots(a, rhs);
+  const SiteMap<SPT, XCD> map(a.nb);
+  // loads in the order they are needed: state and partials, then the operands
+  const CGMState s = load_state(sin + rhs);
+  double2 pv2[HEAD_W];
+  head_issue2<WH>(pp, a.nb, pv2);
+  end_of_loads();
+  int fs[SPT];
+  double smv[SPT];
+  double2 tv[SPT][7], rv[SPT], so[SPT];
+  double2 Uo[LINK ? SPT : 1][3], Um[LINK ? SPT : 1][3];
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    fs[i] = map.f(g, i);
+    const int fc = fs[i] < 0 ? 0 : fs[i];
+    NbrF n = neighbours_f(g, decode(g, fc));
+    int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
+    smv[i] = *elem(a.sm, fc);
+    if (LINK) load_links(a.U, fc, n, Uo[LINK ? i : 0], Um[LINK ? i : 0]);
+#pragma unroll
+    for (int k = 0; k < 7; k++) tv[i][k] = *elem(qv, q[k]);
+    rv[i] = *elem(rk, fc);
+    so[i] = *elem(sv, fc);
+  }
+  end_of_loads();
+  const double2 gd = head_finish2<WH>(pp, a.nb, pv2, lds);
+  const double gam = gd.x, del = gd.y;
+  CGMState* out = sout + rhs;
+  const CGMState* sp = sin + rhs;
+  if (!s.active) {
+    if (lead) put_state_d(out, 0, s.iters, s.converged, s.early, s.rho, s.alpha, s.beta, s.pend, sp->hal, sp->hbe, s);
+    return;
+  }
+  const int it = s.iters;
+  double al, be;
+  if (it == 0) {
+    // ||b|| < tol returns before the loop only in the imposed start
+    // (conjugategradient.c:123-128); the null start tested its prologue
+    // residual instead (:22)
+    if (s.start == GNX_START_IMPOSED && sqrt(gam) < s.tol) {
+      if (lead) {
+        put_state_d(out, 0, 0, 1, 1, gam, 0.0, 0.0, 0, sp->hal, sp->hbe, s);
+        CGDevResult res = {1, 1, 0, 0, sqrt(gam), {0, 0, 0}};
+        a.res[rhs] = res;
+      }
+      return;
+    }
+    al = gam / del;
+    be = 0.0;
+  } else {
+    const bool conv = sqrt(gam) < s.tol;
+    if (conv || it >= s.max_iter) {
+      // the pending updates, then the exit
+      cgd_flush(a, off, map, it - s.pend, s.pend, sp->hal, sp->hbe);
+      if (lead) {
+        put_state_d(out, 0, it, conv ? 1 : 0, 0, gam, s.alpha, s.beta, 0, sp->hal, sp->hbe, s);
+        CGDevResult res = {conv ? 1 : 0, 0, it, 0, sqrt(gam), {0, 0, 0}};
+        a.res[rhs] = res;
+      }
+      return;
+    }
+    be = gam / s.rho;
+    al = gam / (del - be * gam / s.alpha);
+  }
+  // the first iteration has no previous direction: s = A r
+  const bool first = it == 0;
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    if (fs[i] < 0) continue;
+    const Site st = decode(g, fs[i]);
+    const double2 m0 = rmul(smv[i], tv[i][0]);
+    const double2 w = LINK ? link_hops<1>(g, st, m0, &tv[i][1], &tv[i][4], Uo[LINK ? i : 0], Um[LINK ? i : 0])
+                           : apply_M_vals(g, st, smv[i], tv[i][0], &tv[i][1], &tv[i][4]);
+    const double2 sn = first ? w : cadd(w, rmul(be, so[i]));
+    st_nt(elem(sv, fs[i]), sn);
+    st_nt(elem(rn, fs[i]), cadd(rv[i], rmul(-al, sn)));
+  }
+  // record this iteration's update; apply the pending ones every CGD_M
+  double hal[CGD_M], hbe[CGD_M];
+#pragma unroll
+  for (int j = 0; j < CGD_M; j++) {
+    hal[j] = j == s.pend ? al : sp->hal[j];
+    hbe[j] = j == s.pend ? be : sp->hbe[j];
+  }
+  int pend = s.pend + 1;
+  if (pend == CGD_M) {
+    cgd_flush(a, off, map, it + 1 - CGD_M, CGD_M, hal, hbe);
+    pend = 0;
+  }
+  if (lead) put_state_d(out, 1, it + 1, 0, 0, gam, al, be, pend, hal, hbe, s);
+}
+
 // ---- shapes the tuner chooses from -----------------------------------------------
 struct CGMShape {
   int spt;
   bool xcd;
   bool link;
-  bool one;        // one-reduction form: A = k_cg1_X, B = k_cg1_Y
+  bool one;        // one-reduction form: A = k_cg1_X, B = k_cg1_Y (or k_cgd_*)
+  bool deferred;   // the one-reduction form with deferred x and p updates (k_cgd_X, k_cgd_Y)
   const void* A;   // block-wide head reduction
   const void* B;
   const void* Aw;  // per-wave head reduction (up to 512 partials)
@@ -541,27 +743,37 @@ struct CGMShape {
 };
 
 #define GNX_SHAPE(S, X, K)                                                                                  \
-  {S, X, K, false, (const void*)k_cgm_A<S, X, K, false>, (const void*)k_cgm_B<S, X, K, false>,               \
+  {S, X, K, false, false, (const void*)k_cgm_A<S, X, K, false>, (const void*)k_cgm_B<S, X, K, false>,               \
    (const void*)k_cgm_A<S, X, K, true>, (const void*)k_cgm_B<S, X, K, true>, (const void*)k_cgm_start_imposed<S, X>, \
    (const void*)k_cgm_null_p1<S, X, K>, (const void*)k_cgm_null_p2<S, X, K>, (const void*)k_cgm_null_p3<S, X>}
 #define GNX_SHAPE1(S, X, K)                                                                                 \
-  {S, X, K, true, (const void*)k_cg1_X<S, X, K, false>, (const void*)k_cg1_Y<S, X, K, false>,                \
+  {S, X, K, true, false, (const void*)k_cg1_X<S, X, K, false>, (const void*)k_cg1_Y<S, X, K, false>,                \
    (const void*)k_cg1_X<S, X, K, true>, (const void*)k_cg1_Y<S, X, K, true>, (const void*)k_cgm_start_imposed<S, X>, \
    (const void*)k_cgm_null_p1<S, X, K>, (const void*)k_cgm_null_p2<S, X, K>, (const void*)k_cgm_null_p3<S, X>}
+#define GNX_SHAPED(S, X, K)                                                                                 \
+  {S, X, K, true, true, (const void*)k_cgd_X<S, X, K, false>, (const void*)k_cgd_Y<S, X, K, false>,          \
+   (const void*)k_cgd_X<S, X, K, true>, (const void*)k_cgd_Y<S, X, K, true>, (const void*)k_cgm_start_imposed<S, X>, \
+   (const void*)k_cgm_null_p1<S, X, K>, (const void*)k_cgm_null_p2<S, X, K>, (const void*)k_cgm_null_p3<S, X>}
 // per operator (the live operator first, then the link-coupled one): 0..5
-// the two-reduction kernels, 6..11 the one-reduction kernels, each with 1, 1,
-// 2, 2, 4, 4 sites per thread, odd indices with the XCD slab mapping
+// the two-reduction kernels, 6..11 the one-reduction kernels, 12..17 the
+// one-reduction kernels with deferred x and p updates, each with 1, 1, 2, 2,
+// 4, 4 sites per thread, odd indices with the XCD slab mapping
 static const CGMShape kShapes[] = {
     GNX_SHAPE(1, false, false),  GNX_SHAPE(1, true, false),  GNX_SHAPE(2, false, false),
     GNX_SHAPE(2, true, false),   GNX_SHAPE(4, false, false), GNX_SHAPE(4, true, false),
     GNX_SHAPE1(1, false, false), GNX_SHAPE1(1, true, false), GNX_SHAPE1(2, false, false),
     GNX_SHAPE1(2, true, false),  GNX_SHAPE1(4, false, false), GNX_SHAPE1(4, true, false),
+    GNX_SHAPED(1, false, false), GNX_SHAPED(1, true, false), GNX_SHAPED(2, false, false),
+    GNX_SHAPED(2, true, false),  GNX_SHAPED(4, false, false), GNX_SHAPED(4, true, false),
     GNX_SHAPE(1, false, true),   GNX_SHAPE(1, true, true),   GNX_SHAPE(2, false, true),
     GNX_SHAPE(2, true, true),    GNX_SHAPE(4, false, true),  GNX_SHAPE(4, true, true),
     GNX_SHAPE1(1, false, true),  GNX_SHAPE1(1, true, true),  GNX_SHAPE1(2, false, true),
-    GNX_SHAPE1(2, true, true),   GNX_SHAPE1(4, false, true), GNX_SHAPE1(4, true, true)};
+    GNX_SHAPE1(2, true, true),   GNX_SHAPE1(4, false, true), GNX_SHAPE1(4, true, true),
+    GNX_SHAPED(1, false, true),  GNX_SHAPED(1, true, true),  GNX_SHAPED(2, false, true),
+    GNX_SHAPED(2, true, true),   GNX_SHAPED(4, false, true), GNX_SHAPED(4, true, true)};
 #undef GNX_SHAPE
 #undef GNX_SHAPE1
+#undef GNX_SHAPED
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 constexpr int kShapesPerOp = kNumShapes / 2;
 
@@ -576,7 +788,7 @@ struct CGMachine {
   double* sm = nullptr;
   double2* U = nullptr;  // [3V] link copy (link-coupled solves)
   double2* x = nullptr;
-  double2* r = nullptr;
+  double2* r = nullptr;  // CGD_R residual buffers of cap elements (the deferred form's ring), r the first
   double2* p0 = nullptr;
   double2* p1 = nullptr;
   double2* t = nullptr;
@@ -626,7 +838,7 @@ static hipError_t cgm_ensure(CGMachine* m, const Geom& g, int nrhs) {
   e = hipMalloc(&m->sm, sizeof(double) * g.V);
   if (e == hipSuccess) e = hipMalloc(&m->U, sizeof(double2) * 3 * (size_t)g.V);
   if (e == hipSuccess) e = hipMalloc(&m->x, sizeof(double2) * need);
-  if (e == hipSuccess) e = hipMalloc(&m->r, sizeof(double2) * need);
+  if (e == hipSuccess) e = hipMalloc(&m->r, sizeof(double2) * need * CGD_R);
   if (e == hipSuccess) e = hipMalloc(&m->p0, sizeof(double2) * need);
   if (e == hipSuccess) e = hipMalloc(&m->p1, sizeof(double2) * need);
   if (e == hipSuccess) e = hipMalloc(&m->t, sizeof(double2) * need);
@@ -652,6 +864,8 @@ static CGMFix cgm_fix(CGMachine* m, const Geom& g, int nrhs, int shape) {
   a.part = m->eng.dev_part();
   a.state = (CGMState*)m->eng.dev_state();
   a.res = m->res;
+  a.rring = m->r;
+  a.rstride = (size_t)m->cap;
   a.nrhs = nrhs;
   a.nb = shape_blocks(g, shape);
   return a;
@@ -672,8 +886,11 @@ static void cgm_chunk(const CGMFix& a, int nrhs, int shape, hipStream_t cs) {
   const CGMShape& sh = kShapes[shape];
   const bool wh = head_wave(a.nb);
   CGMFix ac = a;
+  static_assert(ENG_CHUNK % CGD_R == 0, "the residual ring turns over within a chunk");
   for (int i = 0; i < ENG_CHUNK; i++) {
-    int odd = i & 1;
+    // iteration parity (p buffers of the two-reduction form), or the
+    // residual ring slot of the deferred form
+    int odd = sh.deferred ? (i & (CGD_R - 1)) : (i & 1);
     // A: state[0] -> state[1], reduces slot 1, writes slot 0; B the reverse.
     // The one-reduction kernels take (a, odd, state in, state out) only.
     void* argsA[] = {(void*)&ac, (void*)&odd, (void*)&s0, (void*)&w1, (void*)&q1, (void*)&o0};
@@ -722,19 +939,25 @@ static hipError_t cgm_begin(CGMachine* m, const Geom& g, const double* sm, const
 
 // Launch shape of a solve.  By default a fixed rule, so that a solve's
 // iterates (which depend on the form and on the partial-sum partition) are
-// the same in every process: the one-reduction kernels in the XCD slab
-// mapping with the most sites per thread (4, 2, 1) that still gives the
-// batch 512 blocks of at least 8 per system -- the shape the timing tuner
-// picks at 64^3 single and batched.  GNX_CG_SHAPE pins a shape; autotune
+// the same in every process: the one-reduction kernels (with deferred x and
+// p updates once the batch holds 400 000 sites) in the XCD slab mapping with
+// the most sites per thread (4, 2, 1) that still gives the batch 512 blocks
+// of at least 8 per system (the sites per thread the timing tuner picks at
+// 64^3 single and batched).  GNX_CG_SHAPE pins a shape; autotune
 // (GNX_AUTOTUNE=1 or gnx_set_autotune) times every shape on 2*CHUNK
 // iterations of the caller's own system (imposed start into the solver's
 // buffers; the caller's x is not touched) and keeps the fastest per key.
 static int cgm_rule(const Geom& g, int nrhs) {
+  // the deferred form pays where the iteration is bound by its bytes (64^3
+  // x 2 and larger: -12 % per iteration at x 2, -16 % at x 16, -4 % at 80^3
+  // single), not where it is bound by latency (+9 % at 64^3 single, +2 % at
+  // 48^3)
+  const int base = (long)g.V * nrhs >= 400000 ? 13 : 7;
   for (int k = 2; k >= 1; k--) {
     const long nb = (g.V + (long)CGM_BS * (1 << k) - 1) / ((long)CGM_BS * (1 << k));
-    if (nb >= NUM_XCD && nb * nrhs >= 512) return 7 + 2 * k;
+    if (nb >= NUM_XCD && nb * nrhs >= 512) return base + 2 * k;
   }
-  return 7;
+  return base;
 }
 
 static int cgm_key(int nrhs, bool link) { return nrhs + (link ? 1000 : 0); }
@@ -845,7 +1068,8 @@ static StaticLayout static_layout(const Geom& g, int nb) {
 }
 
 size_t cgm_static_bytes(const Geom& g) {
-  return static_layout(g, shape_blocks(g, cgm_rule(g, 1))).total;
+  const int rule = cgm_rule(g, 1);
+  return static_layout(g, shape_blocks(g, rule >= 12 ? rule - 6 : rule)).total;
 }
 
 size_t cgm_static_capacity() { return kStaticScratchBytes; }
@@ -870,7 +1094,8 @@ hipError_t cgm_f64_run_static(const Geom& g, int device, const double* sm, const
   unsigned char* base = nullptr;
   hipError_t e = static_scratch(device, &base);
   if (e != hipSuccess) return e;
-  const int shape = cgm_rule(g, 1);
+  const int rule = cgm_rule(g, 1);
+  const int shape = rule >= 12 ? rule - 6 : rule;  // the same iterates without the residual ring
   CGMFix a;
   a.g = g;
   a.nrhs = 1;
@@ -887,6 +1112,8 @@ hipError_t cgm_f64_run_static(const Geom& g, int device, const double* sm, const
   a.part = (double*)(base + l.part);
   a.state = (CGMState*)(base + l.state);
   a.res = (CGDevResult*)(base + l.res);
+  a.rring = a.r;
+  a.rstride = 0;
   e = cgm_start(a, sm, b, x, tol, max_iter, start, 1, shape, st);
   if (e != hipSuccess) return e;
   // chunks of ENG_CHUNK iterations launched directly, the state's `active`
@@ -915,7 +1142,7 @@ int cgm_shape_of(void* machine, int nrhs, int* spt, int* xcd) {
   const int sh = m->eng.shapes.peek(cgm_key(nrhs, false));
   if (sh < 0) return -1;
   if (spt) *spt = kShapes[sh].spt;
-  if (xcd) *xcd = (kShapes[sh].xcd ? 1 : 0) | (kShapes[sh].one ? 2 : 0);
+  if (xcd) *xcd = (kShapes[sh].xcd ? 1 : 0) | (kShapes[sh].one ? 2 : 0) | (kShapes[sh].deferred ? 4 : 0);
   return sh;
 }
 

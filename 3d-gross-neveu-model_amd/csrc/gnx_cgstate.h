@@ -6,6 +6,13 @@
 
 namespace gnx {
 
+// Iterations whose solution and direction updates the deferred form of the
+// one-reduction kernels applies together (x += alpha p, p = r + beta p, in
+// iteration order), and the ring of residual buffers that keeps their
+// residuals: CGD_M + 1 buffers, a power of two that divides the graph chunk.
+constexpr int CGD_M = 7;
+constexpr int CGD_R = CGD_M + 1;
+
 // per-system state; `active` and `iters` lead, as the replay engine polls them
 struct CGMState {
   int active;
@@ -22,6 +29,12 @@ struct CGMState {
   // never load it)
   double alpha;
   double beta;
+  // deferred form only: the pending iterations' count, step lengths and
+  // direction coefficients
+  int pend;
+  int pad2;
+  double hal[CGD_M];
+  double hbe[CGD_M];
 };
 
 // field-wise state write (an aggregate copy makes the compiler stage the
@@ -44,6 +57,19 @@ GNX_DEVONLY void put_state1(CGMState* o, int active, int iters, int conv, int ea
   put_state(o, active, iters, conv, early, rho, cfg);
   o->alpha = alpha;
   o->beta = beta;
+}
+
+// the whole state, with the fields of the deferred form
+GNX_DEVONLY void put_state_d(CGMState* o, int active, int iters, int conv, int early, double rho, double alpha,
+                             double beta, int pend, const double* hal, const double* hbe, const CGMState& cfg) {
+  put_state1(o, active, iters, conv, early, rho, alpha, beta, cfg);
+  o->pend = pend;
+  o->pad2 = cfg.pad2;
+#pragma unroll
+  for (int j = 0; j < CGD_M; j++) {
+    o->hal[j] = hal[j];
+    o->hbe[j] = hbe[j];
+  }
 }
 
 }  // namespace gnx

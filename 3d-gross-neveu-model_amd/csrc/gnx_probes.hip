@@ -8,7 +8,10 @@
 //                           bytes of a Dirac application (2 vectors + a
 //                           real field in, one vector out: 40 B/site) and of the
 //                           one-reduction CG's second kernel (152 B/site),
-//                           each node reading the previous node's output.
+//                           each node reading the previous node's output;
+//                           and the 152 B/site node's reads alone (88 B/site,
+//                           one partial sum per block) and writes alone
+//                           (64 B/site).
 //   gnx_probe_grid_barrier  cost of a grid-wide barrier inside one
 //                           persistent kernel (atomic arrival counter and a
 //                           generation word, device-scope release before the
@@ -58,6 +61,37 @@ __global__ __launch_bounds__(kBS) void k_stream152(long n, const double2* __rest
   v2[i] = make_double2(b.x - s * a.x, b.y - s * a.y);
   v3[i] = make_double2(c.x + s * b.x, c.y + s * b.y);
   v4[i] = make_double2(d.x - s * c.x, d.y - s * c.y);
+}
+
+// 88 B/site read, nothing written but one partial sum per block: the
+// reads of k_stream152 alone
+__global__ __launch_bounds__(kBS) void k_read88(long n, const double2* __restrict__ v0, const double* __restrict__ w,
+                                                const double2* __restrict__ v1, const double2* __restrict__ v2,
+                                                const double2* __restrict__ v3, const double2* __restrict__ v4,
+                                                double* __restrict__ part) {
+  __shared__ double lds[kBS / 64];
+  const long i = (long)blockIdx.x * kBS + threadIdx.x;
+  double acc = 0.0;
+  if (i < n) {
+    const double2 q = v0[i], a = v1[i], b = v2[i], c = v3[i], d = v4[i];
+    acc = w[i] * (q.x + a.y) + b.x * c.y + d.x * d.y;
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = lds[0] + lds[1] + lds[2] + lds[3];
+}
+
+// 64 B/site written, nothing read: the writes of k_stream152 alone
+__global__ __launch_bounds__(kBS) void k_write64(long n, double s, double2* __restrict__ v1, double2* __restrict__ v2,
+                                                 double2* __restrict__ v3, double2* __restrict__ v4) {
+  const long i = (long)blockIdx.x * kBS + threadIdx.x;
+  if (i >= n) return;
+  const double t = s + (double)i;
+  v1[i] = make_double2(t, s);
+  v2[i] = make_double2(s, t);
+  v3[i] = make_double2(t, t);
+  v4[i] = make_double2(s, s);
 }
 
 struct BarrierState {
@@ -129,11 +163,12 @@ using namespace gnx_probe;
 extern "C" {
 
 int gnx_probe_graph_nodes(int kind, long sites, int nodes, int reps, double* us_per_node) {
-  if (kind < 0 || kind > 2 || sites < 1 || nodes < 2 || reps < 1 || !us_per_node) return (int)hipErrorInvalidValue;
+  if (kind < 0 || kind > 4 || sites < 1 || nodes < 2 || reps < 1 || !us_per_node) return (int)hipErrorInvalidValue;
   hipError_t e;
   hipStream_t st = nullptr;
   double2* v[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   double* w = nullptr;
+  double* part = nullptr;  // one partial sum per block (k_read88)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -145,6 +180,7 @@ int gnx_probe_graph_nodes(int kind, long sites, int nodes, int reps, double* us_
   }
   if (e == hipSuccess) e = hipMalloc(&w, sizeof(double) * sites);
   if (e == hipSuccess) e = hipMemsetAsync(w, 0, sizeof(double) * sites, st);
+  if (e == hipSuccess) e = hipMalloc(&part, sizeof(double) * ((sites + kBS - 1) / kBS + 1));
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   const unsigned grid = kind == 0 ? 1024u : (unsigned)((sites + kBS - 1) / kBS);
   if (e == hipSuccess) e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
@@ -156,9 +192,15 @@ int gnx_probe_graph_nodes(int kind, long sites, int nodes, int reps, double* us_
         // each node reads the previous node's output
         hipLaunchKernelGGL(k_stream40, dim3(grid), dim3(kBS), 0, st, sites, (const double2*)v[i & 1],
                            (const double2*)v[2], (const double*)w, v[(i + 1) & 1]);
-      } else {
+      } else if (kind == 2) {
         hipLaunchKernelGGL(k_stream152, dim3(grid), dim3(kBS), 0, st, sites, (const double2*)v[i & 1], (const double*)w,
                            v[2], v[3], v[4], v[(i + 1) & 1]);
+      } else if (kind == 3) {
+        hipLaunchKernelGGL(k_read88, dim3(grid), dim3(kBS), 0, st, sites, (const double2*)v[0], (const double*)w,
+                           (const double2*)v[1], (const double2*)v[2], (const double2*)v[3], (const double2*)v[4],
+                           part);
+      } else {
+        hipLaunchKernelGGL(k_write64, dim3(grid), dim3(kBS), 0, st, sites, (double)i, v[1], v[2], v[3], v[4]);
       }
     }
     e = hipStreamEndCapture(st, &graph);
@@ -184,6 +226,7 @@ int gnx_probe_graph_nodes(int kind, long sites, int nodes, int reps, double* us_
   for (int k = 0; k < 5; k++)
     if (v[k]) (void)hipFree(v[k]);
   if (w) (void)hipFree(w);
+  if (part) (void)hipFree(part);
   (void)hipStreamDestroy(st);
   return (int)e;
 }

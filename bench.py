@@ -52,15 +52,18 @@ def kernel_sources_sha():
 
 def shape_tag(shape):
     # kernel template arguments: <sites per thread, XCD slab mapping, link-coupled operator>,
-    # prefixed "cg1" for the one-reduction kernels (k_cg1_X, k_cg1_Y)
-    return (("cg1" if shape.get("one_reduction") else "")
-            + f"<{shape['sites_per_thread']}, {'true' if shape['xcd_slab'] else 'false'}, false>")
+    # prefixed "cg1" for the one-reduction kernels (k_cg1_X, k_cg1_Y), "cgd" for
+    # their deferred-update form (k_cgd_X, k_cgd_Y)
+    pre = "cgd" if shape.get("deferred") else "cg1" if shape.get("one_reduction") else ""
+    return pre + f"<{shape['sites_per_thread']}, {'true' if shape['xcd_slab'] else 'false'}, false>"
 
 
-CG_KERNELS = ("k_cgm_A", "k_cgm_B", "k_cg1_X", "k_cg1_Y")
+CG_KERNELS = ("k_cgm_A", "k_cgm_B", "k_cg1_X", "k_cg1_Y", "k_cgd_X", "k_cgd_Y")
 
 
 def cg_kernel_label(shape):
+    if shape and shape.get("deferred"):
+        return "k_cgd_X + k_cgd_Y (one CG iteration, one-reduction form, deferred x and p updates)"
     if shape and shape.get("one_reduction"):
         return "k_cg1_X + k_cg1_Y (one CG iteration, one-reduction form)"
     return "k_cgm_A + k_cgm_B (one CG iteration)"
@@ -120,9 +123,9 @@ def summarize_rocprof(kt_dir, fetch_dir, write_dir, L, untraced=None):
             if k in avg and name in nodes:
                 infl[name] = avg[k] - nodes[name]["us_per_node"]
                 out.append(f"| `{k}` | {avg[k]:.2f} | {nodes[name]['us_per_node']:.2f} | {infl[name]:+.2f} |")
-        cg = [k for k in avg if k.startswith(("k_cg1_X", "k_cg1_Y", "k_cgm_A", "k_cgm_B"))]
-        xk = [k for k in cg if k.startswith(("k_cg1_X", "k_cgm_A"))]
-        yk = [k for k in cg if k.startswith(("k_cg1_Y", "k_cgm_B"))]
+        cg = [k for k in avg if k.startswith(CG_KERNELS)]
+        xk = [k for k in cg if k.startswith(("k_cg1_X", "k_cgd_X", "k_cgm_A"))]
+        yk = [k for k in cg if k.startswith(("k_cg1_Y", "k_cgd_Y", "k_cgm_B"))]
         if xk and yk and "stream_40B" in infl and "stream_152B" in infl:
             x_est = avg[xk[0]] - infl["stream_40B"]
             y_est = avg[yk[0]] - infl["stream_152B"]
@@ -169,7 +172,7 @@ def summarize_rocprof(kt_dir, fetch_dir, write_dir, L, untraced=None):
         out.append(f"| `{n}` | {len(counters['FETCH_SIZE'][0][n])} | {fk:.0f} | {wk:.0f} | {mb:.2f} |")
         # <sites per thread, XCD slab, link-coupled, per-wave head>: the head
         # form follows from the block count, so the launch shape is the key
-        tag = (("cg1" if n.startswith("k_cg1") else "")
+        tag = (("cgd" if n.startswith("k_cgd") else "cg1" if n.startswith("k_cg1") else "")
                + "<" + ", ".join(n[n.index("<") + 1:n.rindex(">")].split(", ")[:3]) + ">")
         bpi[tag] = bpi.get(tag, 0.0) + mb * 1e6
     alg = BYTES_CG_ITER_F64 * L ** 3
@@ -308,16 +311,19 @@ def probes_leg(L=64, barrier=True):
     lib = C.CDLL(os.path.join(ROOT, "3d-gross-neveu-model_amd", "lib", "libgnx_probes.so"))
     V = L ** 3
     out = {"graph_nodes": {}, "grid_barrier": []}
-    for kind, name, bps in ((0, "empty", 0), (1, "stream_40B", 40), (2, "stream_152B", 152)):
+    legs = [(0, "empty", 0, V), (1, "stream_40B", 40, V), (2, "stream_152B", 152, V), (3, "read_88B", 88, V),
+            (4, "write_64B", 64, V)]
+    legs += [(2, f"stream_152B_{n}^3", 152, n ** 3) for n in (32, 48, 96)]
+    for kind, name, bps, sites in legs:
         us = C.c_double()
-        rc = lib.gnx_probe_graph_nodes(kind, C.c_long(V), 64, 20, C.byref(us))
+        rc = lib.gnx_probe_graph_nodes(kind, C.c_long(sites), 64, 20, C.byref(us))
         if rc != 0:
             out["graph_nodes"][name] = {"error": rc}
             continue
         d = {"us_per_node": us.value, "nodes": 64}
         if bps:
-            d["GB_s"] = bps * V / (us.value * 1e-6) / 1e9
-            d["sites"] = V
+            d["GB_s"] = bps * sites / (us.value * 1e-6) / 1e9
+            d["sites"] = sites
         out["graph_nodes"][name] = d
     if not barrier:
         return out

@@ -30,7 +30,8 @@ extern "C" {
 
 /* 4: gnx_cg_shape reports the one-reduction kernels in bit 1 of xcd_mapped
  * 5: launch shapes by a fixed rule unless autotuning is on (gnx_set_autotune,
- *    gnx_tune_runs); geometries outside the exact site decode are refused */
+ *    gnx_tune_runs); geometries outside the exact site decode are refused;
+ *    bit 2 of gnx_cg_shape's xcd_mapped marks the deferred-update kernels */
 #define GNX_ABI_VERSION 5
 
 typedef struct gnx_ctx gnx_ctx;
@@ -188,11 +189,13 @@ gnx_status gnx_tune_runs(gnx_ctx *ctx, long *fp64, long *mixed);
 /* Launch shape of the FP64 CG on the live operator (M M^dag) for batches of
  * nrhs systems on this context (sites per thread; xcd_mapped bit 0:
  * XCD-slab block mapping, bit 1: the one-reduction (Chronopoulos-Gear)
- * kernels); GNX_ERR_STATE if no solve of that batch size has run yet.
- * GNX_CG_SHAPE=<n> in the environment pins the shape: n = 0..5 the
+ * kernels, bit 2: their form with deferred solution updates, the same
+ * iterates bit for bit); GNX_ERR_STATE if no solve of that batch size has run
+ * yet.  GNX_CG_SHAPE=<n> in the environment pins the shape: n = 0..5 the
  * two-reduction kernels with 1, 1, 2, 2, 4, 4 sites per thread, n = 6..11 the
- * one-reduction kernels likewise; odd n use the XCD slab mapping.  The
- * link-coupled operator's shape is chosen the same way but not reported. */
+ * one-reduction kernels likewise, n = 12..17 the deferred form likewise; odd
+ * n use the XCD slab mapping.  The link-coupled operator's shape is chosen
+ * the same way but not reported. */
 gnx_status gnx_cg_shape(gnx_ctx *ctx, int nrhs, int *sites_per_thread, int *xcd_mapped);
 /* The same for the mixed-precision solver (prec GNX_PREC_F32 or GNX_PREC_BF16);
  * pair = 1: two systems share each block's site arithmetic; loads_first = 1:

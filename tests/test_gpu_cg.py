@@ -105,12 +105,14 @@ def test_noconv_and_determinism(torch_dev, P, gnx_mod, shape, monkeypatch):
     assert np.array_equal(host(x1), host(x2))
 
 
-@pytest.mark.parametrize("shape", list(range(12)))
+@pytest.mark.parametrize("shape", list(range(18)))
 def test_batch_matches_single(torch_dev, P, gnx_mod, shape, monkeypatch):
     """Every launch shape (sites per thread 1/2/4, with and without the XCD
     slab mapping; 0..5 the two-reduction kernels, 6..11 the one-reduction
-    kernels) against the oracle; with the shape fixed, a batch of 3 is
-    bit-identical to 3 single solves (same partial sums, same order)."""
+    kernels, 12..17 their deferred-update form) against the oracle; with the
+    shape fixed, a batch of 3 is bit-identical to 3 single solves (same
+    partial sums, same order), and the deferred form is bit-identical to the
+    one-reduction kernels it defers."""
     banner(f"cg batch of 3 == 3 single solves, shape {shape}")
     monkeypatch.setenv("GNX_CG_SHAPE", str(shape))
     L = 12
@@ -126,7 +128,7 @@ def test_batch_matches_single(torch_dev, P, gnx_mod, shape, monkeypatch):
     st, res = ctx.cg_batch(sm, B, X, 3, 1e-10, start=gnx_mod.START_IMPOSED)
     assert st == 0
     assert ctx.cg_shape(3) == {"sites_per_thread": [1, 1, 2, 2, 4, 4][shape % 6], "xcd_slab": shape % 2 == 1,
-                               "one_reduction": shape >= 6}
+                               "one_reduction": shape >= 6, "deferred": shape >= 12}
     for k in range(3):
         x = ctx.vec()
         s1, r1 = ctx.cg(sm, dev(bs[k]), x, 1e-10, start=gnx_mod.START_IMPOSED)
@@ -135,3 +137,11 @@ def test_batch_matches_single(torch_dev, P, gnx_mod, shape, monkeypatch):
         xr, ok, stc = lat.cg(smh, bs[k], None, tol=1e-10, start="imposed")
         assert abs(res[k].iterations - stc.last_iters) <= 1
         assert rel(host(x), xr) < 1e-7
+    if shape >= 12:
+        # the one-reduction kernels without deferral, in a fresh context
+        monkeypatch.setenv("GNX_CG_SHAPE", str(shape - 6))
+        ctx2 = gnx_mod.Context(L)
+        X2 = ctx2.vec(3)
+        st2, res2 = ctx2.cg_batch(sm, B, X2, 3, 1e-10, start=gnx_mod.START_IMPOSED)
+        assert st2 == 0 and [r.iterations for r in res2] == [r.iterations for r in res]
+        assert np.array_equal(host(X2), host(X))

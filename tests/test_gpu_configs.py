@@ -75,7 +75,7 @@ def oracle_solution(P, L):
 
 
 # ---------------------------------------------------------------- configuration 2
-@pytest.mark.parametrize("shape", [None, "1", "7", "9"])
+@pytest.mark.parametrize("shape", [None, "1", "7", "9", "15"])
 @pytest.mark.parametrize("L", [32, 64])
 def test_config2_cg(P, gnx_mod, L, shape, monkeypatch):
     banner(f"config 2: FP64 CG {L}^3 shape={shape}")

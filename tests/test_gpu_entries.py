@@ -139,7 +139,8 @@ def test_shape_rule_is_default_and_deterministic(P, gnx_mod):
         ctx.cg(sm, b, xm, 1e-10, start=gnx_mod.START_IMPOSED, prec=gnx_mod.PREC_F16)
         torch.cuda.synchronize()
         assert ctx.tune_runs() == {"fp64": 0, "mixed": 0}
-        assert ctx.cg_shape(1) == {"sites_per_thread": 2, "xcd_slab": True, "one_reduction": True}
+        assert ctx.cg_shape(1) == {"sites_per_thread": 2, "xcd_slab": True, "one_reduction": True,
+                                   "deferred": False}
         assert ctx.cg_mixed_shape(1, gnx_mod.PREC_F16) == {"sites_per_thread": 2, "xcd_slab": True, "pair": False,
                                                            "loads_first": True}
         out.append((host(x), host(xm)))
