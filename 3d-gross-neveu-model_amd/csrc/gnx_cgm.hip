@@ -42,6 +42,7 @@
 #include "gnx_engine.h"
 #include "gnx_internal.h"
 #include "gnx_solver.h"
+#include "gnx_tile.h"
 
 namespace gnx {
 
@@ -62,6 +63,8 @@ struct CGMFix {
   CGDevResult* res;  // [nrhs]
   double2* rring;    // deferred form: CGD_R residual buffers [CGD_R][nrhs][V], the first one r
   size_t rstride;    // elements between two of them
+  double2* t1;       // fused form: the second q buffer (q_k in t or t1 by the parity of k)
+  double2* s1;       // fused form: the second s buffer (s_k in p1 or s1)
   int nrhs;
   int nb;            // blocks per system of every kernel of this shape
 };
@@ -545,27 +548,31 @@ __global__ __launch_bounds__(CGM_BS) void k_cg1_Y(CGMFix a, int odd, const CGMSt
 
 // x += alpha_j p_j, p_j = r_j + beta_j p_{j-1} (p = r at iteration 0) for the
 // n pending iterations j0 .. j0 + n - 1, at this thread's sites
+GNX_DEVONLY void cgd_flush_site(const CGMFix& a, size_t off, int f, int j0, int n, const double* hal,
+                                const double* hbe) {
+  double2 rj[CGD_M];
+#pragma unroll
+  for (int j = 0; j < CGD_M; j++)
+    if (j < n) rj[j] = *elem(ring(a, (j0 + j) & (CGD_R - 1)) + off, f);
+  double2 p = *elem(a.p0 + off, f), x = *elem(a.x + off, f);
+#pragma unroll
+  for (int j = 0; j < CGD_M; j++) {
+    if (j >= n) break;
+    p = j0 + j == 0 ? rj[j] : cadd(rj[j], rmul(hbe[j], p));
+    x = cadd(x, rmul(hal[j], p));
+  }
+  st_nt(elem(a.p0 + off, f), p);
+  st_nt(elem(a.x + off, f), x);
+}
+
 template <int SPT, bool XCD>
 GNX_DEVONLY void cgd_flush(const CGMFix& a, size_t off, const SiteMap<SPT, XCD>& map, int j0, int n, const double* hal,
                            const double* hbe) {
-  const Geom g = a.g;
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
-    const int f = map.f(g, i);
+    const int f = map.f(a.g, i);
     if (f < 0) continue;
-    double2 rj[CGD_M];
-#pragma unroll
-    for (int j = 0; j < CGD_M; j++)
-      if (j < n) rj[j] = *elem(ring(a, (j0 + j) & (CGD_R - 1)) + off, f);
-    double2 p = *elem(a.p0 + off, f), x = *elem(a.x + off, f);
-#pragma unroll
-    for (int j = 0; j < CGD_M; j++) {
-      if (j >= n) break;
-      p = j0 + j == 0 ? rj[j] : cadd(rj[j], rmul(hbe[j], p));
-      x = cadd(x, rmul(hal[j], p));
-    }
-    st_nt(elem(a.p0 + off, f), p);
-    st_nt(elem(a.x + off, f), x);
+    cgd_flush_site(a, off, f, j0, n, hal, hbe);
   }
 }
 
@@ -725,6 +732,202 @@ __global__ __launch_bounds__(CGM_BS) void k_cgd_Y(CGMFix a, int rs, const CGMSta
   if (lead) put_state_d(out, 1, it + 1, 0, 0, gam, al, be, pend, hal, hbe, s);
 }
 
+
+// ---- fused form: one kernel per iteration ----------------------------------------
+// Y(k) and X(k+1) of the one-reduction form in one kernel, so that an
+// iteration crosses one kernel boundary instead of two.  X(k+1) needs
+// r_{k+1} at the six neighbours of a site, which other blocks of the same
+// kernel compute; so each block computes s_k and r_{k+1} on its tile and on
+// the tile's one-site halo (a halo site again, with its owner's operations
+// on its owner's operands, so the values are the owner's bit for bit),
+// keeps r_{k+1} of tile and halo in LDS, and applies M^dag from there:
+//
+//   Z(k): gamma_k = |r_k|^2, delta_k = |q_k|^2 from the partials of the
+//         previous kernel; exit test, alpha_k and beta_k as in k_cg1_Y;
+//         on tile and halo: s_k = M q_k + beta_k s_{k-1},
+//                           r_{k+1} = r_k - alpha_k s_k  (into LDS);
+//         on the tile: p and x as in k_cg1_Y, or deferred as in k_cgd_Y;
+//                      q_{k+1} = M^dag r_{k+1} from LDS;
+//                      partials (|r_{k+1}|^2, |q_{k+1}|^2).
+//
+// A T0 x T1 x T2 tile (x0, x1, x2) has 2 (T1 T2 + T0 T2 + T0 T1) halo
+// sites, 448 for 4 x 8 x 16 (0.875 per tile site), each one more M q: seven
+// loads, mostly L2 hits on the neighbouring tiles' lines.  q, s and r are
+// read at halo sites that their owners rewrite in the same kernel, so each
+// is double-buffered by the parity of k (q_k in t / t1, s_k in p1 / s1, r_k
+// in ring slot k mod 8 with deferral, k mod 2 without), and so are the
+// partials.  The tiles divide the lattice (L % T0 = L % T1 = LT % T2 = 0,
+// T2 even: a tile row along x2 starts on an even x2 and holds T2/2 sites of
+// each parity, contiguous in f-order).  Blocks take tiles in XCD slabs along
+// x0.  The prologue runs X(0) (k_cgd_X) and Z(0) before the replayed chunks.
+// the fused form's partial pairs of the parity-j kernel, [2][nrhs][nb]
+GNX_DEV double2* fpart(const CGMFix& a, int j, int rhs) {
+  return (double2*)a.part + ((size_t)j * a.nrhs + rhs) * a.nb;
+}
+
+template <int T0, int T1, int T2, bool LINK, bool WH, bool DEF>
+__global__ __launch_bounds__(CGM_BS) void k_cgf_Z(CGMFix a, int k8, const CGMState* __restrict__ sin,
+                                                   CGMState* __restrict__ sout) {
+  using TL = FTile<T0, T1, T2>;
+  constexpr int R = TL::ROUNDS, SPT = TL::SPT;
+  __shared__ double2 box[TL::NBOX];
+  __shared__ double lds[2 * CGM_BS / 64];
+  const Geom g = a.g;
+  const int rhs = blockIdx.y;
+  const size_t off = (size_t)rhs * g.V;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const int kp = k8 & 1;
+  const double2* qk = (kp ? a.t1 : a.t) + off;
+  double2* qn = (kp ? a.t : a.t1) + off;
+  const double2* sold = (kp ? a.p1 : a.s1) + off;
+  double2* snew = (kp ? a.s1 : a.p1) + off;
+  const double2* rk = ring(a, DEF ? k8 : kp) + off;
+  double2* rn = ring(a, DEF ? (k8 + 1) & (CGD_R - 1) : kp ^ 1) + off;
+  double2* pd = a.p0 + off;
+  double2* xd = a.x + off;
+  const double2* pin = fpart(a, kp, rhs);
+  int o0, o1, o2;
+  ftile_origin<T0, T1, T2>(g, a.nb, (int)blockIdx.x, &o0, &o1, &o2);
+  // loads in the order they are needed: state and partials, then the operands
+  const CGMState s = load_state(sin + rhs);
+  double2 pv2[HEAD_W];
+  head_issue2<WH>(pin, a.nb, pv2);
+  end_of_loads();
+  int fs[R], bx[R];
+  double smv[R];
+  double2 tv[R][7], rv[R], so[R];
+  double2 pv[DEF ? 1 : SPT], xv[DEF ? 1 : SPT];
+  double2 Uo[LINK ? R : 1][3], Um[LINK ? R : 1][3];
+#pragma unroll
+  for (int i = 0; i < R; i++) {
+    // an item past the last one loads the tile's first site and stores nothing
+    const int e = i * CGM_BS + (int)threadIdx.x;
+    const Site st = ftile_site<T0, T1, T2>(g, o0, o1, o2, e < TL::NEXT ? e : 0, &bx[i]);
+    fs[i] = f_of_lex(g, st.lex, st.P);
+    NbrF n = neighbours_f(g, st);
+    int q[7] = {fs[i], n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
+    smv[i] = *elem(a.sm, fs[i]);
+    if (LINK) load_links(a.U, fs[i], n, Uo[LINK ? i : 0], Um[LINK ? i : 0]);
+#pragma unroll
+    for (int k = 0; k < 7; k++) tv[i][k] = *elem(qk, q[k]);
+    rv[i] = *elem(rk, fs[i]);
+    so[i] = *elem(sold, fs[i]);
+    if (!DEF && i < SPT) {
+      pv[DEF ? 0 : i] = *elem(pd, fs[i]);
+      xv[DEF ? 0 : i] = *elem(xd, fs[i]);
+    }
+  }
+  end_of_loads();
+  const double2 gd = head_finish2<WH>(pin, a.nb, pv2, lds);
+  const double gam = gd.x, del = gd.y;
+  CGMState* out = sout + rhs;
+  const CGMState* sp = sin + rhs;
+  auto put = [&](int active, int iters, int conv, int early, double rho, double al, double be, int pend,
+                 const double* hal, const double* hbe) {
+    if constexpr (DEF)
+      put_state_d(out, active, iters, conv, early, rho, al, be, pend, hal, hbe, s);
+    else
+      put_state1(out, active, iters, conv, early, rho, al, be, s);
+  };
+  if (!s.active) {
+    if (lead) put(0, s.iters, s.converged, s.early, s.rho, s.alpha, s.beta, s.pend, sp->hal, sp->hbe);
+    return;
+  }
+  const int it = s.iters;
+  double al, be;
+  if (it == 0) {
+    // ||b|| < tol returns before the loop only in the imposed start
+    // (conjugategradient.c:123-128); the null start tested its prologue
+    // residual instead (:22)
+    if (s.start == GNX_START_IMPOSED && sqrt(gam) < s.tol) {
+      if (lead) {
+        put(0, 0, 1, 1, gam, 0.0, 0.0, 0, sp->hal, sp->hbe);
+        CGDevResult res = {1, 1, 0, 0, sqrt(gam), {0, 0, 0}};
+        a.res[rhs] = res;
+      }
+      return;
+    }
+    al = gam / del;
+    be = 0.0;
+  } else {
+    const bool conv = sqrt(gam) < s.tol;
+    if (conv || it >= s.max_iter) {
+      if (DEF) {
+        // the pending updates, then the exit
+#pragma unroll
+        for (int i = 0; i < SPT; i++) cgd_flush_site(a, off, fs[i], it - s.pend, s.pend, sp->hal, sp->hbe);
+      }
+      if (lead) {
+        put(0, it, conv ? 1 : 0, 0, gam, s.alpha, s.beta, 0, sp->hal, sp->hbe);
+        CGDevResult res = {conv ? 1 : 0, 0, it, 0, sqrt(gam), {0, 0, 0}};
+        a.res[rhs] = res;
+      }
+      return;
+    }
+    be = gam / s.rho;
+    al = gam / (del - be * gam / s.alpha);
+  }
+  // the first iteration has no previous direction: p = r, s = A r
+  const bool first = it == 0;
+  double2 rt[SPT];
+#pragma unroll
+  for (int i = 0; i < R; i++) {
+    const int e = i * CGM_BS + (int)threadIdx.x;
+    if (i >= SPT && e >= TL::NEXT) continue;
+    const Site st = decode(g, fs[i]);
+    const double2 m0 = rmul(smv[i], tv[i][0]);
+    const double2 w = LINK ? link_hops<1>(g, st, m0, &tv[i][1], &tv[i][4], Uo[LINK ? i : 0], Um[LINK ? i : 0])
+                           : apply_M_vals(g, st, smv[i], tv[i][0], &tv[i][1], &tv[i][4]);
+    const double2 sn = first ? w : cadd(w, rmul(be, so[i]));
+    const double2 rr = cadd(rv[i], rmul(-al, sn));
+    box[bx[i]] = rr;
+    if (i < SPT) {
+      rt[i < SPT ? i : 0] = rr;
+      st_nt(elem(snew, fs[i]), sn);
+      st_nt(elem(rn, fs[i]), rr);
+      if (!DEF) {
+        const double2 pn = first ? rv[i] : cadd(rv[i], rmul(be, pv[DEF ? 0 : i]));
+        st_nt(elem(pd, fs[i]), pn);
+        st_nt(elem(xd, fs[i]), cadd(xv[DEF ? 0 : i], rmul(al, pn)));
+      }
+    }
+  }
+  __syncthreads();
+  double ag = 0.0, ad = 0.0;
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    const Site st = decode(g, fs[i]);
+    const int b = bx[i];
+    const double2 vpp[3] = {box[b + TL::S0], box[b + TL::S1], box[b + 1]};
+    const double2 vmm[3] = {box[b - TL::S0], box[b - TL::S1], box[b - 1]};
+    const double2 qq = LINK ? link_hops<-1>(g, st, rmul(smv[i], rt[i]), vpp, vmm, Uo[LINK ? i : 0], Um[LINK ? i : 0])
+                            : apply_Mdag_fast_vals(g, st, smv[i], rt[i], vpp, vmm);
+    st_nt(elem(qn, fs[i]), qq);
+    ag += cnorm2(rt[i]);
+    ad += cnorm2(qq);
+  }
+  const double2 t = block_sum2(ag, ad, lds);
+  if (threadIdx.x == 0) fpart(a, kp ^ 1, rhs)[partial_slot(blockIdx.x, a.nb)] = t;
+  if (DEF) {
+    // record this iteration's update; apply the pending ones every CGD_M
+    double hal[CGD_M], hbe[CGD_M];
+#pragma unroll
+    for (int j = 0; j < CGD_M; j++) {
+      hal[j] = j == s.pend ? al : sp->hal[j];
+      hbe[j] = j == s.pend ? be : sp->hbe[j];
+    }
+    int pend = s.pend + 1;
+    if (pend == CGD_M) {
+#pragma unroll
+      for (int i = 0; i < SPT; i++) cgd_flush_site(a, off, fs[i], it + 1 - CGD_M, CGD_M, hal, hbe);
+      pend = 0;
+    }
+    if (lead) put(1, it + 1, 0, 0, gam, al, be, pend, hal, hbe);
+  } else if (lead) {
+    put(1, it + 1, 0, 0, gam, al, be, 0, nullptr, nullptr);
+  }
+}
+
 // ---- shapes the tuner chooses from -----------------------------------------------
 struct CGMShape {
   int spt;
@@ -740,6 +943,8 @@ struct CGMShape {
   const void* null_p1;
   const void* null_p2;
   const void* null_p3;
+  bool fused;      // one kernel per iteration (k_cgf_Z; A is the prologue X(0), k_cgd_X)
+  int T0, T1, T2;  // its tile
 };
 
 #define GNX_SHAPE(S, X, K)                                                                                  \
@@ -754,10 +959,30 @@ struct CGMShape {
   {S, X, K, true, true, (const void*)k_cgd_X<S, X, K, false>, (const void*)k_cgd_Y<S, X, K, false>,          \
    (const void*)k_cgd_X<S, X, K, true>, (const void*)k_cgd_Y<S, X, K, true>, (const void*)k_cgm_start_imposed<S, X>, \
    (const void*)k_cgm_null_p1<S, X, K>, (const void*)k_cgm_null_p2<S, X, K>, (const void*)k_cgm_null_p3<S, X>}
+#define GNX_SHAPEF(T0, T1, T2, K, D)                                                                      \
+  {(T0) * (T1) * (T2) / CGM_BS,                                                                             \
+   true,                                                                                                    \
+   K,                                                                                                       \
+   true,                                                                                                    \
+   D,                                                                                                       \
+   (const void*)k_cgd_X<(T0) * (T1) * (T2) / CGM_BS, true, K, false>,                                       \
+   (const void*)k_cgf_Z<T0, T1, T2, K, false, D>,                                                           \
+   (const void*)k_cgd_X<(T0) * (T1) * (T2) / CGM_BS, true, K, true>,                                        \
+   (const void*)k_cgf_Z<T0, T1, T2, K, true, D>,                                                            \
+   (const void*)k_cgm_start_imposed<(T0) * (T1) * (T2) / CGM_BS, true>,                                     \
+   (const void*)k_cgm_null_p1<(T0) * (T1) * (T2) / CGM_BS, true, K>,                                        \
+   (const void*)k_cgm_null_p2<(T0) * (T1) * (T2) / CGM_BS, true, K>,                                        \
+   (const void*)k_cgm_null_p3<(T0) * (T1) * (T2) / CGM_BS, true>,                                           \
+   true,                                                                                                    \
+   T0,                                                                                                      \
+   T1,                                                                                                      \
+   T2}
 // per operator (the live operator first, then the link-coupled one): 0..5
 // the two-reduction kernels, 6..11 the one-reduction kernels, 12..17 the
 // one-reduction kernels with deferred x and p updates, each with 1, 1, 2, 2,
-// 4, 4 sites per thread, odd indices with the XCD slab mapping
+// 4, 4 sites per thread, odd indices with the XCD slab mapping; 18..21 the
+// fused kernels (one per iteration) on 4 x 8 x 16 and 8 x 8 x 8 tiles,
+// without and with deferral
 static const CGMShape kShapes[] = {
     GNX_SHAPE(1, false, false),  GNX_SHAPE(1, true, false),  GNX_SHAPE(2, false, false),
     GNX_SHAPE(2, true, false),   GNX_SHAPE(4, false, false), GNX_SHAPE(4, true, false),
@@ -765,21 +990,32 @@ static const CGMShape kShapes[] = {
     GNX_SHAPE1(2, true, false),  GNX_SHAPE1(4, false, false), GNX_SHAPE1(4, true, false),
     GNX_SHAPED(1, false, false), GNX_SHAPED(1, true, false), GNX_SHAPED(2, false, false),
     GNX_SHAPED(2, true, false),  GNX_SHAPED(4, false, false), GNX_SHAPED(4, true, false),
+    GNX_SHAPEF(4, 8, 16, false, false), GNX_SHAPEF(4, 8, 16, false, true),
+    GNX_SHAPEF(8, 8, 8, false, false),  GNX_SHAPEF(8, 8, 8, false, true),
     GNX_SHAPE(1, false, true),   GNX_SHAPE(1, true, true),   GNX_SHAPE(2, false, true),
     GNX_SHAPE(2, true, true),    GNX_SHAPE(4, false, true),  GNX_SHAPE(4, true, true),
     GNX_SHAPE1(1, false, true),  GNX_SHAPE1(1, true, true),  GNX_SHAPE1(2, false, true),
     GNX_SHAPE1(2, true, true),   GNX_SHAPE1(4, false, true), GNX_SHAPE1(4, true, true),
     GNX_SHAPED(1, false, true),  GNX_SHAPED(1, true, true),  GNX_SHAPED(2, false, true),
-    GNX_SHAPED(2, true, true),   GNX_SHAPED(4, false, true), GNX_SHAPED(4, true, true)};
+    GNX_SHAPED(2, true, true),   GNX_SHAPED(4, false, true), GNX_SHAPED(4, true, true),
+    GNX_SHAPEF(4, 8, 16, true, false), GNX_SHAPEF(4, 8, 16, true, true),
+    GNX_SHAPEF(8, 8, 8, true, false),  GNX_SHAPEF(8, 8, 8, true, true)};
 #undef GNX_SHAPE
 #undef GNX_SHAPE1
 #undef GNX_SHAPED
+#undef GNX_SHAPEF
 constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 constexpr int kShapesPerOp = kNumShapes / 2;
 
 static int shape_blocks(const Geom& g, int shape) {
   int per = CGM_BS * kShapes[shape].spt;
   return (g.V + per - 1) / per;
+}
+
+// whether a shape runs on lattice g: a fused shape's tiles must divide it
+static bool shape_fits(const Geom& g, int shape) {
+  const CGMShape& sh = kShapes[shape];
+  return !sh.fused || (g.L % sh.T0 == 0 && g.L % sh.T1 == 0 && g.LT % sh.T2 == 0);
 }
 
 // ---- host side ---------------------------------------------------------------
@@ -792,17 +1028,19 @@ struct CGMachine {
   double2* p0 = nullptr;
   double2* p1 = nullptr;
   double2* t = nullptr;
+  double2* t1 = nullptr;  // the fused form's second q and s buffers
+  double2* s1 = nullptr;
   CGDevResult* res = nullptr;
   long cap = 0;          // complex elements per vector
   int forced_shape = -1; // GNX_CG_SHAPE (index within an operator's shapes)
   int autotune = 0;      // 1: choose shapes by timing (GNX_AUTOTUNE=1, gnx_set_autotune)
   void free_vectors() {
-    void* bufs[] = {sm, U, x, r, p0, p1, t, res};
+    void* bufs[] = {sm, U, x, r, p0, p1, t, t1, s1, res};
     for (void* p : bufs)
       if (p) (void)hipFree(p);
     sm = nullptr;
     U = nullptr;
-    x = r = p0 = p1 = t = nullptr;
+    x = r = p0 = p1 = t = t1 = s1 = nullptr;
     res = nullptr;
     cap = 0;
   }
@@ -829,7 +1067,8 @@ static CGMachine* cgm_get(void** slotp) {
 static hipError_t cgm_ensure(CGMachine* m, const Geom& g, int nrhs) {
   const int nb_max = (g.V + CGM_BS - 1) / CGM_BS;
   const int cap_rhs = nrhs < 2 ? 2 : nrhs;
-  hipError_t e = m->eng.ensure(cap_rhs, 1, sizeof(CGMState), 2 * (size_t)cap_rhs * nb_max);
+  // partials: two slots of nrhs x nb pairs (the fused form's double buffer)
+  hipError_t e = m->eng.ensure(cap_rhs, 1, sizeof(CGMState), 4 * (size_t)cap_rhs * nb_max);
   if (e != hipSuccess) return e;
   const long need = (long)g.V * cap_rhs;
   if (m->cap >= need && m->res) return hipSuccess;
@@ -842,6 +1081,8 @@ static hipError_t cgm_ensure(CGMachine* m, const Geom& g, int nrhs) {
   if (e == hipSuccess) e = hipMalloc(&m->p0, sizeof(double2) * need);
   if (e == hipSuccess) e = hipMalloc(&m->p1, sizeof(double2) * need);
   if (e == hipSuccess) e = hipMalloc(&m->t, sizeof(double2) * need);
+  if (e == hipSuccess) e = hipMalloc(&m->t1, sizeof(double2) * need);
+  if (e == hipSuccess) e = hipMalloc(&m->s1, sizeof(double2) * need);
   if (e == hipSuccess) e = hipMalloc(&m->res, sizeof(CGDevResult) * cap_rhs);
   if (e != hipSuccess) {
     m->free_vectors();
@@ -866,6 +1107,8 @@ static CGMFix cgm_fix(CGMachine* m, const Geom& g, int nrhs, int shape) {
   a.res = m->res;
   a.rring = m->r;
   a.rstride = (size_t)m->cap;
+  a.t1 = m->t1;
+  a.s1 = m->s1;
   a.nrhs = nrhs;
   a.nb = shape_blocks(g, shape);
   return a;
@@ -887,6 +1130,18 @@ static void cgm_chunk(const CGMFix& a, int nrhs, int shape, hipStream_t cs) {
   const bool wh = head_wave(a.nb);
   CGMFix ac = a;
   static_assert(ENG_CHUNK % CGD_R == 0, "the residual ring turns over within a chunk");
+  if (sh.fused) {
+    // Z(k) for k = 1 + i (the prologue ran X(0) and Z(0)): node i reads
+    // state[i & 1] and writes the other half, so the chunk ends in state[0]
+    for (int i = 0; i < ENG_CHUNK; i++) {
+      int k8 = (i + 1) & (CGD_R - 1);
+      const CGMState* sin = (i & 1) ? s1 : s0;
+      CGMState* sout = (i & 1) ? w0 : w1;
+      void* args[] = {(void*)&ac, (void*)&k8, (void*)&sin, (void*)&sout};
+      (void)hipLaunchKernel(wh ? sh.Bw : sh.B, grid, blk, args, 0, cs);
+    }
+    return;
+  }
   for (int i = 0; i < ENG_CHUNK; i++) {
     // iteration parity (p buffers of the two-reduction form), or the
     // residual ring slot of the deferred form
@@ -922,6 +1177,21 @@ static hipError_t cgm_start(const CGMFix& a, const double* sm, const double2* b,
   } else {
     void* a0[] = {(void*)&ac, (void*)&sm, (void*)&b, (void*)&tol, (void*)&max_iter};
     e = hipLaunchKernel(kShapes[shape].start_imposed, grid, blk, a0, 0, st);
+  }
+  const CGMShape& sh = kShapes[shape];
+  if (e == hipSuccess && sh.fused) {
+    // the fused form's prologue: X(0) (state[0] -> state[1], q_0 into t,
+    // partials into slot 0), then Z(0) (state[1] -> state[0])
+    const bool wh = head_wave(a.nb);
+    int k0 = 0;
+    const CGMState* s0 = a.state;
+    const CGMState* s1 = a.state + nrhs;
+    CGMState* w0 = a.state;
+    CGMState* w1 = a.state + nrhs;
+    void* ax[] = {(void*)&ac, (void*)&k0, (void*)&s0, (void*)&w1};
+    e = hipLaunchKernel(wh ? sh.Aw : sh.A, grid, blk, ax, 0, st);
+    void* az[] = {(void*)&ac, (void*)&k0, (void*)&s1, (void*)&w0};
+    if (e == hipSuccess) e = hipLaunchKernel(wh ? sh.Bw : sh.B, grid, blk, az, 0, st);
   }
   return e;
 }
@@ -975,12 +1245,13 @@ static hipError_t cgm_choose(CGMachine* m, const Geom& g, const double* sm, cons
   }
   int best_shape = first + cgm_rule(g, nrhs);
   hipError_t e = hipSuccess;
-  if (m->forced_shape >= 0) {
+  if (m->forced_shape >= 0 && shape_fits(g, first + m->forced_shape)) {
     best_shape = first + m->forced_shape;
   } else if (m->autotune) {
     E.tune_runs++;
     float best_ms = 1e30f;
     for (int sh = first; sh < first + kShapesPerOp && e == hipSuccess; sh++) {
+      if (!shape_fits(g, sh)) continue;
       hipGraphExec_t exec;
       e = cgm_graph(m, g, nrhs, sh, &exec);
       float ms_min = 1e30f;
@@ -1114,6 +1385,7 @@ hipError_t cgm_f64_run_static(const Geom& g, int device, const double* sm, const
   a.res = (CGDevResult*)(base + l.res);
   a.rring = a.r;
   a.rstride = 0;
+  a.t1 = a.s1 = nullptr;  // the static solve runs no fused shape
   e = cgm_start(a, sm, b, x, tol, max_iter, start, 1, shape, st);
   if (e != hipSuccess) return e;
   // chunks of ENG_CHUNK iterations launched directly, the state's `active`
@@ -1142,7 +1414,9 @@ int cgm_shape_of(void* machine, int nrhs, int* spt, int* xcd) {
   const int sh = m->eng.shapes.peek(cgm_key(nrhs, false));
   if (sh < 0) return -1;
   if (spt) *spt = kShapes[sh].spt;
-  if (xcd) *xcd = (kShapes[sh].xcd ? 1 : 0) | (kShapes[sh].one ? 2 : 0) | (kShapes[sh].deferred ? 4 : 0);
+  if (xcd)
+    *xcd = (kShapes[sh].xcd ? 1 : 0) | (kShapes[sh].one ? 2 : 0) | (kShapes[sh].deferred ? 4 : 0) |
+           (kShapes[sh].fused ? 8 : 0);
   return sh;
 }
 

@@ -16,7 +16,7 @@ constexpr int NUM_XCD = 8;
 // next 64 the odd sites at the same positions, so every wave reads one
 // parity's contiguous run (lanes alternating between the two halves, 2 MB
 // apart at 64^3, cost the L1 half its throughput).
-GNX_DEVONLY int xcd_block(int b, int nb) {
+GNX_DEV int xcd_block(int b, int nb) {
   int q = nb / NUM_XCD, rem = nb % NUM_XCD, x = b % NUM_XCD;
   return x * q + (x < rem ? x : rem) + b / NUM_XCD;
 }

@@ -312,7 +312,8 @@ class Context:
         spt, xcd = C.c_int(), C.c_int()
         st = lib().gnx_cg_shape(self.h, nrhs, C.byref(spt), C.byref(xcd))
         return None if st != 0 else {"sites_per_thread": spt.value, "xcd_slab": bool(xcd.value & 1),
-                                     "one_reduction": bool(xcd.value & 2), "deferred": bool(xcd.value & 4)}
+                                     "one_reduction": bool(xcd.value & 2), "deferred": bool(xcd.value & 4),
+                                     "fused": bool(xcd.value & 8)}
 
     def cg_mixed_shape(self, nrhs, prec):
         spt, xcd, pair, early = C.c_int(), C.c_int(), C.c_int(), C.c_int()

@@ -754,6 +754,68 @@ def size_leg(gnx, torch, L, tol, steps, nbatch, precs):
     return out
 
 
+def schedule_leg(gnx, torch, L, tol, batches, reps=3):
+    """The FP64 CG iteration in each kernel schedule, on the headline's
+    inputs: the fixed rule's choice, the two-kernel one-reduction form
+    (pinned shape 9: 2 sites per thread, slab mapping), its deferred-update
+    form (15), and the fused one-kernel-per-iteration form on 4 x 8 x 16
+    tiles with the halo recomputed into LDS (18, and 19 deferred).  Per
+    schedule and batch: us per iteration (HIP events over `reps` solves on
+    the solver's stream), iterations, and the largest relative difference
+    of the solutions from the rule's."""
+    V = L ** 3
+    out = {"workload": f"{L}^3, sigma and complex Z2 sources from the reference's stream (seed 1231), tol {tol:g}"}
+    saved = os.environ.pop("GNX_CG_SHAPE", None)
+    try:
+        for nb in batches:
+            sigma_h, B_h = make_inputs(L, nsrc=nb)
+            row, ref = {}, None
+            for name, pin in (("rule", None), ("two_kernel_9", "9"), ("two_kernel_deferred_15", "15"),
+                              ("fused_18", "18"), ("fused_deferred_19", "19")):
+                if pin is None:
+                    os.environ.pop("GNX_CG_SHAPE", None)
+                else:
+                    os.environ["GNX_CG_SHAPE"] = pin
+                ctx = gnx.Context(L)
+                sm = ctx.field()
+                ctx.sigma_mass(torch.from_numpy(sigma_h).to(ctx.device), sm)
+                B = torch.from_numpy(B_h).to(ctx.device)
+                X = ctx.vec(nb)
+
+                def solve():
+                    if nb == 1:
+                        st, res = ctx.cg(sm, B, X, tol, start=gnx.START_IMPOSED)
+                        return [res]
+                    st, res = ctx.cg_batch(sm, B, X, nb, tol, start=gnx.START_IMPOSED)
+                    return list(res)
+
+                solve()
+                torch.cuda.synchronize()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(ctx.stream)
+                for _ in range(reps):
+                    res = solve()
+                e1.record(ctx.stream)
+                e1.synchronize()
+                t = e0.elapsed_time(e1) * 1e-3 / reps
+                it = max(r.iterations for r in res)
+                xh = X.cpu()
+                if ref is None:
+                    ref = xh
+                diff = max(float((torch.linalg.norm(xh[2 * V * k:2 * V * (k + 1)] - ref[2 * V * k:2 * V * (k + 1)])
+                                  / torch.linalg.norm(ref[2 * V * k:2 * V * (k + 1)])).item()) for k in range(nb))
+                row[name] = {"us_per_iteration": t / it * 1e6, "iterations": it, "rhs_solves_per_s": nb / t,
+                             "max_rel_diff_vs_rule": diff, "shape": ctx.cg_shape(nb)}
+                ctx.close()
+            out[f"x{nb}"] = row
+    finally:
+        if saved is None:
+            os.environ.pop("GNX_CG_SHAPE", None)
+        else:
+            os.environ["GNX_CG_SHAPE"] = saved
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -772,6 +834,8 @@ def main():
     ap.add_argument("--sizes", default="32:32,96:16", metavar="L:N,...",
                     help="configurations 2 and 3 at other sizes: lattice L^3, FP64 single and N-source "
                          "batched FP64 and mixed solves ('' to skip)")
+    ap.add_argument("--schedules", default="1,16", metavar="N,...",
+                    help="batch sizes of the kernel-schedule comparison at the headline size ('' to skip)")
     ap.add_argument("--probes", default="all", choices=["all", "nodes", "none"],
                     help="graph-node and grid-barrier probes (nodes: graph-node probes only)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -1010,6 +1074,8 @@ def main():
             Ls, n = (int(v) for v in spec.split(":"))
             result["sizes"][f"{Ls}^3"] = size_leg(gnx, torch, Ls, args.tol, args.steps, n,
                                                   [v for v in args.mixed.split(",") if v])
+    if args.schedules:
+        result["schedules"] = schedule_leg(gnx, torch, L, args.tol, [int(v) for v in args.schedules.split(",")])
     if args.probes != "none":
         result["probes"] = probes_leg(L, barrier=args.probes == "all")
     if args.copy_calibration > 0:

@@ -190,12 +190,17 @@ gnx_status gnx_tune_runs(gnx_ctx *ctx, long *fp64, long *mixed);
  * nrhs systems on this context (sites per thread; xcd_mapped bit 0:
  * XCD-slab block mapping, bit 1: the one-reduction (Chronopoulos-Gear)
  * kernels, bit 2: their form with deferred solution updates, the same
- * iterates bit for bit); GNX_ERR_STATE if no solve of that batch size has run
- * yet.  GNX_CG_SHAPE=<n> in the environment pins the shape: n = 0..5 the
- * two-reduction kernels with 1, 1, 2, 2, 4, 4 sites per thread, n = 6..11 the
- * one-reduction kernels likewise, n = 12..17 the deferred form likewise; odd
- * n use the XCD slab mapping.  The link-coupled operator's shape is chosen
- * the same way but not reported. */
+ * iterates bit for bit, bit 3: the fused form, one kernel per iteration with
+ * each block's one-site halo recomputed into LDS); GNX_ERR_STATE if no solve
+ * of that batch size has run yet.  GNX_CG_SHAPE=<n> in the environment pins
+ * the shape: n = 0..5 the two-reduction kernels with 1, 1, 2, 2, 4, 4 sites
+ * per thread, n = 6..11 the one-reduction kernels likewise, n = 12..17 the
+ * deferred form likewise; odd n use the XCD slab mapping; n = 18..21 the
+ * fused form on 4 x 8 x 16 and 8 x 8 x 8 tiles, without and with deferral
+ * (where the tiles divide the lattice; elsewhere the fixed rule's shape).
+ * The fixed rule does not choose the fused form: it is slower on the sizes
+ * measured (its halo costs more than the kernel boundary it saves).  The
+ * link-coupled operator's shape is chosen the same way but not reported. */
 gnx_status gnx_cg_shape(gnx_ctx *ctx, int nrhs, int *sites_per_thread, int *xcd_mapped);
 /* The same for the mixed-precision solver (prec GNX_PREC_F32 or GNX_PREC_BF16);
  * pair = 1: two systems share each block's site arithmetic; loads_first = 1:

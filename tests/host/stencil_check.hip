@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../3d-gross-neveu-model_amd/csrc/gnx_stencil.h"
+#include "../../3d-gross-neveu-model_amd/csrc/gnx_tile.h"
 #include "../../oracle/gnx_oracle.h"
 
 using namespace gnx;
@@ -104,7 +105,73 @@ static long decode_scan(int L, int LT) {
   return bad;
 }
 
+// Tiles of the fused CG kernel on an L x L x LT lattice: the blocks' tiles
+// cover every site exactly once; a block's items (tile, then halo) take
+// distinct LDS box slots inside the box; every tile site decodes to the
+// coordinates its item names; and the box slots at +-1 along each axis of a
+// tile site hold exactly the site's six neighbours (f-indices of
+// neighbours_f), so M^dag from LDS reads what M^dag from memory would.
+template <int T0, int T1, int T2>
+static void check_tiles(int L, int LT) {
+  using TL = FTile<T0, T1, T2>;
+  if (L % T0 || L % T1 || LT % T2) { printf("tile does not divide %d^2 x %d\n", L, LT); exit(2); }
+  Geom g = make_geom(L, LT);
+  const int nb = g.V / TL::N;
+  std::vector<int> owner(g.V, 0);
+  std::vector<int> slot_f(TL::NBOX);
+  for (int b = 0; b < nb; b++) {
+    int o0, o1, o2;
+    ftile_origin<T0, T1, T2>(g, nb, b, &o0, &o1, &o2);
+    EXPECT(o0 % T0 == 0 && o1 % T1 == 0 && o2 % T2 == 0 && o0 < L && o1 < L && o2 < LT,
+           "tile %dx%dx%d L=%d LT=%d block %d origin %d %d %d", T0, T1, T2, L, LT, b, o0, o1, o2);
+    std::fill(slot_f.begin(), slot_f.end(), -1);
+    for (int e = 0; e < TL::NEXT; e++) {
+      int bx = -1;
+      const Site st = ftile_site<T0, T1, T2>(g, o0, o1, o2, e, &bx);
+      const bool ok = st.x0 >= 0 && st.x0 < L && st.x1 >= 0 && st.x1 < L && st.x2 >= 0 && st.x2 < LT &&
+                      bx >= 0 && bx < TL::NBOX;
+      EXPECT(ok, "tile %dx%dx%d L=%d block %d item %d out of range", T0, T1, T2, L, b, e);
+      if (!ok) continue;
+      EXPECT(st.P == ((st.x0 + st.x1 + st.x2) & 1) && st.lex == (st.x0 * L + st.x1) * LT + st.x2, "site fields");
+      const int f = f_of_lex(g, st.lex, st.P);
+      EXPECT(slot_f[bx] < 0, "block %d item %d: box slot %d taken twice", b, e, bx);
+      slot_f[bx] = f;
+      if (e < TL::N) {
+        const int a0 = st.x0 - o0, a1 = st.x1 - o1, a2 = st.x2 - o2;
+        EXPECT(a0 >= 0 && a0 < T0 && a1 >= 0 && a1 < T1 && a2 >= 0 && a2 < T2, "item %d outside its tile", e);
+        EXPECT(bx == ((a0 + 1) * TL::B1 + (a1 + 1)) * TL::B2 + (a2 + 1), "tile slot");
+        owner[f]++;
+        const Site d = decode(g, f);
+        EXPECT(d.x0 == st.x0 && d.x1 == st.x1 && d.x2 == st.x2 && d.P == st.P && d.lex == st.lex, "decode");
+      }
+    }
+    for (int e = 0; e < TL::N; e++) {
+      int bx = -1;
+      const Site st = ftile_site<T0, T1, T2>(g, o0, o1, o2, e, &bx);
+      const NbrF n = neighbours_f(g, st);
+      const int sd[3] = {TL::S0, TL::S1, 1};
+      for (int k = 0; k < 3; k++) {
+        EXPECT(slot_f[bx + sd[k]] == n.pp[k], "L=%d block %d item %d: +%d neighbour in LDS %d, expected %d", L, b,
+               e, k, slot_f[bx + sd[k]], n.pp[k]);
+        EXPECT(slot_f[bx - sd[k]] == n.mm[k], "L=%d block %d item %d: -%d neighbour in LDS %d, expected %d", L, b,
+               e, k, slot_f[bx - sd[k]], n.mm[k]);
+      }
+    }
+  }
+  for (int f = 0; f < g.V; f++) EXPECT(owner[f] == 1, "tile %dx%dx%d L=%d: site %d owned %d times", T0, T1, T2, L, f, owner[f]);
+}
+
 int main() {
+  check_tiles<4, 8, 16>(16, 16);
+  check_tiles<4, 8, 16>(8, 16);
+  check_tiles<4, 8, 16>(32, 32);
+  check_tiles<4, 8, 16>(24, 48);
+  check_tiles<4, 8, 16>(64, 64);
+  check_tiles<4, 8, 16>(96, 96);
+  check_tiles<8, 8, 8>(8, 8);
+  check_tiles<8, 8, 8>(16, 24);
+  check_tiles<8, 8, 8>(48, 48);
+  check_tiles<8, 8, 8>(64, 64);
   int sizes[][2] = {{4, 4}, {4, 6}, {6, 4}, {6, 10}, {8, 8}, {10, 4}, {12, 12}, {16, 8}, {16, 16}, {24, 24}, {32, 32}};
   for (auto& s : sizes) check(s[0], s[1]);
   // the exact range of decode(): every accepted shape decodes exactly, and

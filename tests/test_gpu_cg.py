@@ -128,7 +128,7 @@ def test_batch_matches_single(torch_dev, P, gnx_mod, shape, monkeypatch):
     st, res = ctx.cg_batch(sm, B, X, 3, 1e-10, start=gnx_mod.START_IMPOSED)
     assert st == 0
     assert ctx.cg_shape(3) == {"sites_per_thread": [1, 1, 2, 2, 4, 4][shape % 6], "xcd_slab": shape % 2 == 1,
-                               "one_reduction": shape >= 6, "deferred": shape >= 12}
+                               "one_reduction": shape >= 6, "deferred": shape >= 12, "fused": False}
     for k in range(3):
         x = ctx.vec()
         s1, r1 = ctx.cg(sm, dev(bs[k]), x, 1e-10, start=gnx_mod.START_IMPOSED)
@@ -145,3 +145,54 @@ def test_batch_matches_single(torch_dev, P, gnx_mod, shape, monkeypatch):
         st2, res2 = ctx2.cg_batch(sm, B, X2, 3, 1e-10, start=gnx_mod.START_IMPOSED)
         assert st2 == 0 and [r.iterations for r in res2] == [r.iterations for r in res]
         assert np.array_equal(host(X2), host(X))
+
+@pytest.mark.parametrize("shape", [18, 19, 20, 21])
+@pytest.mark.parametrize("L", [16, 32])
+def test_fused_shapes(torch_dev, P, gnx_mod, shape, L, monkeypatch):
+    """The fused kernels (one kernel per iteration; 18/19 on 4 x 8 x 16
+    tiles, 20/21 on 8 x 8 x 8, the odd ones with deferred x and p updates)
+    against the oracle, both starts: iterations within +-1, solutions within
+    1e-7, true residual below 10 tol; a batch of 3 is bit-identical to 3
+    single solves, and the deferred form to the one it defers.  At 16^3 a
+    4 x 8 x 16 tile spans the whole x2 extent and half of x1, so the halo
+    wraps onto the block's own tile."""
+    banner(f"cg fused shape {shape} at {L}^3")
+    monkeypatch.setenv("GNX_CG_SHAPE", str(shape))
+    lat = P.Lattice(L)
+    r = P.Random(40 + L)
+    sigma = r.sigma(lat.V, P.sigma_variance(0.85))
+    smh = lat.sigma_mass(sigma)
+    sm = dev(smh)
+    bs = [r.z2_complex(lat.V) for _ in range(3)]
+    x0 = 0.01 * r.z2_complex(lat.V)
+    ctx = gnx_mod.Context(L)
+    B = dev(np.concatenate(bs))
+    X = ctx.vec(3)
+    st, res = ctx.cg_batch(sm, B, X, 3, 1e-10, start=gnx_mod.START_IMPOSED)
+    assert st == 0
+    assert ctx.cg_shape(3) == {"sites_per_thread": 2, "xcd_slab": True, "one_reduction": True,
+                               "deferred": shape % 2 == 1, "fused": True}
+    for k in range(3):
+        x = ctx.vec()
+        s1, r1 = ctx.cg(sm, dev(bs[k]), x, 1e-10, start=gnx_mod.START_IMPOSED)
+        assert s1 == 0 and r1.iterations == res[k].iterations
+        assert np.array_equal(host(x), host(X[k * 2 * lat.V:(k + 1) * 2 * lat.V]))
+    xr, ok, stc = lat.cg(smh, bs[0], None, tol=1e-10, start="imposed")
+    assert abs(res[0].iterations - stc.last_iters) <= 1
+    assert rel(host(X[:2 * lat.V]), xr) < 1e-7
+    assert true_residual(lat, smh, bs[0], host(X[:2 * lat.V])) < 1e-9
+    # the null start from a nonzero guess
+    xn = dev(x0)
+    sn, rn = ctx.cg(sm, dev(bs[1]), xn, 1e-10, start=gnx_mod.START_NULL)
+    xo, ok, stn = lat.cg(smh, bs[1], x0.copy(), tol=1e-10, start="null")
+    assert sn == 0 and ok and abs(rn.iterations - stn.last_iters) <= 1
+    assert rel(host(xn), xo) < 1e-7
+    if shape % 2 == 1:
+        monkeypatch.setenv("GNX_CG_SHAPE", str(shape - 1))
+        ctx2 = gnx_mod.Context(L)
+        X2 = ctx2.vec(3)
+        st2, res2 = ctx2.cg_batch(sm, B, X2, 3, 1e-10, start=gnx_mod.START_IMPOSED)
+        assert st2 == 0 and [q.iterations for q in res2] == [q.iterations for q in res]
+        assert np.array_equal(host(X2), host(X))
+        ctx2.close()
+    ctx.close()

@@ -84,21 +84,23 @@ def test_link_forces_vs_oracle(P, gnx_mod):
     assert abs(p - po) <= 1e-13 * max(1.0, abs(po))
 
 
-@pytest.mark.parametrize("shape", [None, "7"])
+@pytest.mark.parametrize("L,shape", [(8, None), (8, "7"), (8, "20"), (8, "21"), (16, "18"), (16, "19")])
 @pytest.mark.parametrize("start", ["null", "imposed"])
-def test_link_cg_vs_oracle(P, gnx_mod, start, shape, monkeypatch):
-    """The default shape and, pinned, the one-reduction kernels (shape 7)."""
-    banner(f"link cg 8^3 {start} shape={shape}")
+def test_link_cg_vs_oracle(P, gnx_mod, start, L, shape, monkeypatch):
+    """The default shape and, pinned, the one-reduction kernels (shape 7)
+    and the fused kernels (one per iteration; 20/21 on one 8 x 8 x 8 tile
+    that is the whole 8^3 lattice, 18/19 on 4 x 8 x 16 tiles at 16^3)."""
+    banner(f"link cg {L}^3 {start} shape={shape}")
     if shape is not None:
         monkeypatch.setenv("GNX_CG_SHAPE", shape)
     import torch
-    lat, r, sm, U = setup(P, 8, 8, 33)
+    lat, r, sm, U = setup(P, L, L, 33)
     V = lat.V
     b = r.z2_complex(V)
     x0 = r.z2_complex(V)
     xr, ok, st = lat.link_cg(sm, U, b, x0=x0, tol=1e-10, start=start, max_iter=20000)
     assert ok
-    ctx = gnx_mod.Context(8)
+    ctx = gnx_mod.Context(L)
     uf = torch.zeros(6 * V, dtype=torch.float64, device="cuda")
     ctx.links_from_lex(dev(U.view(np.float64)), uf)
     x = dev(x0)
