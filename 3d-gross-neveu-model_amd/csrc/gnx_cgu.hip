@@ -49,6 +49,7 @@ struct CGUSys {
 struct CGUFix {
   const CGUSys* sys;  // [n]
   double* sm;         // concatenated sigma masses
+  double2* U;         // concatenated links, 3 per site (link-coupled solves), system k at 3 voff_k
   double2* b;         // concatenated right-hand sides
   double2* x;         // concatenated solutions (x_in for the null start)
   double2* r;
@@ -84,6 +85,7 @@ struct CGUPtrs {
   const double* sm;
   const double2* b;
   double2* x;
+  const double2* U;  // links (link-coupled solves), or null
 };
 
 __global__ __launch_bounds__(CGU_BS) void k_cgu_gather(CGUFix a, const CGUPtrs* __restrict__ ptr, int with_x) {
@@ -95,6 +97,10 @@ __global__ __launch_bounds__(CGU_BS) void k_cgu_gather(CGUFix a, const CGUPtrs* 
   a.sm[d.soff + f] = p.sm[f];
   a.b[d.voff + f] = p.b[f];
   if (with_x) a.x[d.voff + f] = p.x[f];
+  if (p.U) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) a.U[3 * d.voff + 3 * f + k] = p.U[3 * f + k];
+  }
 }
 
 __global__ __launch_bounds__(CGU_BS) void k_cgu_scatter(CGUFix a, const CGUPtrs* __restrict__ ptr) {
@@ -129,18 +135,24 @@ __global__ __launch_bounds__(CGU_BS) void k_cgu_start_imposed(CGUFix a, double t
 }
 
 // null start, pass 1 (conjugategradient.c:20): t = M^dag x_in
+template <bool LINK>
 __global__ __launch_bounds__(CGU_BS) void k_cgu_null_p1(CGUFix a) {
   CGUSys d;
   if (!cgu_sys(a, &d)) return;
   const int f = cgu_site(d);
   if (f < 0) return;
   const double2* xv = a.x + d.voff;
-  *elem(a.t + d.voff, f) =
-      apply_Mdag(d.g, decode(d.g, f), *elem(a.sm + d.soff, f), [&](int q) { return *elem(xv, q); });
+  const double smf = *elem(a.sm + d.soff, f);
+  if (LINK)
+    *elem(a.t + d.voff, f) = apply_link(d.g, decode(d.g, f), 1, smf, make_double2(0.0, 0.0), a.U + 3 * d.voff,
+                                        [&](int q) { return *elem(xv, q); });
+  else
+    *elem(a.t + d.voff, f) = apply_Mdag(d.g, decode(d.g, f), smf, [&](int q) { return *elem(xv, q); });
 }
 
 // null start, pass 2 (conjugategradient.c:19-21): r = b - M t; partials |r|^2
 // (slot 0) and |b|^2 (slot 1)
+template <bool LINK>
 __global__ __launch_bounds__(CGU_BS) void k_cgu_null_p2(CGUFix a) {
   __shared__ double lds[CGU_BS / 64];
   CGUSys d;
@@ -150,8 +162,10 @@ __global__ __launch_bounds__(CGU_BS) void k_cgu_null_p2(CGUFix a) {
   if (f >= 0) {
     const double2* t = a.t + d.voff;
     const double2 bb = *elem(a.b + d.voff, f);
+    const double smf = *elem(a.sm + d.soff, f);
     const double2 rr =
-        apply_sub_M(d.g, decode(d.g, f), *elem(a.sm + d.soff, f), bb, [&](int q) { return *elem(t, q); });
+        LINK ? apply_link(d.g, decode(d.g, f), 2, smf, bb, a.U + 3 * d.voff, [&](int q) { return *elem(t, q); })
+             : apply_sub_M(d.g, decode(d.g, f), smf, bb, [&](int q) { return *elem(t, q); });
     *elem(a.r + d.voff, f) = rr;
     ar = cnorm2(rr);
     ab = cnorm2(bb);
@@ -192,7 +206,7 @@ __global__ __launch_bounds__(CGU_BS) void k_cgu_null_p3(CGUFix a, double tol, co
 // ---- iteration kernels (the steps of k_cgm_A / k_cgm_B) -------------------------
 // A: rho from the previous partials, exit test, beta; p_new = r + beta p_old at
 // the site and its neighbours; t = M^dag p_new; partial |t|^2
-template <bool WH>
+template <bool WH, bool LINK>
 __global__ __launch_bounds__(CGU_BS) void k_cgu_A(CGUFix a, int odd, const CGMState* __restrict__ sin,
                                                   CGMState* __restrict__ sout, const double* __restrict__ pin,
                                                   double* __restrict__ pout) {
@@ -219,6 +233,8 @@ __global__ __launch_bounds__(CGU_BS) void k_cgu_A(CGUFix a, int odd, const CGMSt
   const NbrF n = neighbours_f(g, decode(g, fc));
   const int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
   const double smv = *elem(a.sm + d.soff, fc);
+  double2 Uo[3], Um[3];
+  if (LINK) load_links(a.U + 3 * d.voff, fc, n, Uo, Um);
   double2 rv[7], pv[7];
 #pragma unroll
   for (int j = 0; j < 7; j++) {
@@ -260,7 +276,8 @@ __global__ __launch_bounds__(CGU_BS) void k_cgu_A(CGUFix a, int odd, const CGMSt
       pmv[dd] = cadd(rv[4 + dd], rmul(beta, pv[4 + dd]));
     }
     st_cgu(elem(pnew, f), p0v);
-    const double2 tt = apply_Mdag_fast_vals(g, st, smv, p0v, ppv, pmv);
+    const double2 tt = LINK ? link_hops<-1>(g, st, rmul(smv, p0v), ppv, pmv, Uo, Um)
+                            : apply_Mdag_fast_vals(g, st, smv, p0v, ppv, pmv);
     st_cgu(elem(a.t + d.voff, f), tt);
     acc = cnorm2(tt);
   }
@@ -271,7 +288,7 @@ __global__ __launch_bounds__(CGU_BS) void k_cgu_A(CGUFix a, int odd, const CGMSt
 
 // B: gamma from A's partials; alpha = rho / gamma; x += alpha p_new;
 // r -= alpha M t; partial |r|^2
-template <bool WH>
+template <bool WH, bool LINK>
 __global__ __launch_bounds__(CGU_BS) void k_cgu_B(CGUFix a, int odd, const CGMState* __restrict__ sin,
                                                   CGMState* __restrict__ sout, const double* __restrict__ pin,
                                                   double* __restrict__ pout) {
@@ -299,6 +316,8 @@ __global__ __launch_bounds__(CGU_BS) void k_cgu_B(CGUFix a, int odd, const CGMSt
   const NbrF n = neighbours_f(g, decode(g, fc));
   const int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
   const double smv = *elem(a.sm + d.soff, fc);
+  double2 Uo[3], Um[3];
+  if (LINK) load_links(a.U + 3 * d.voff, fc, n, Uo, Um);
   double2 tv[7];
 #pragma unroll
   for (int j = 0; j < 7; j++) tv[j] = *elem(t, q[j]);
@@ -310,7 +329,9 @@ __global__ __launch_bounds__(CGU_BS) void k_cgu_B(CGUFix a, int odd, const CGMSt
   const double al = s.rho / gamma;
   double acc = 0.0;
   if (f >= 0) {
-    const double2 w = apply_M_vals(g, decode(g, f), smv, tv[0], &tv[1], &tv[4]);
+    const Site st = decode(g, f);
+    const double2 w = LINK ? link_hops<1>(g, st, rmul(smv, tv[0]), &tv[1], &tv[4], Uo, Um)
+                           : apply_M_vals(g, st, smv, tv[0], &tv[1], &tv[4]);
     st_cgu(elem(x, f), cadd(xv, rmul(al, pv)));
     const double2 rr = cadd(rv, rmul(-al, w));
     st_cgu(elem(r, f), rr);
@@ -341,6 +362,7 @@ struct gnx_multi {
   gnx::CGUPtrs* hptr = nullptr;  // [n], pinned
   double* sm = nullptr;
   double2 *b = nullptr, *x = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr, *t = nullptr;
+  double2* U = nullptr;  // concatenated links, allocated by the first link-coupled solve
   gnx::CGDevResult* res = nullptr;
   gnx::CGDevResult* hres = nullptr;  // pinned
 };
@@ -360,7 +382,7 @@ bool multi_system(const gnx_multi* m, int k, int* L, int* LT, int* device) {
 namespace gnx {
 
 static void cgu_free(gnx_multi* m) {
-  void* dev[] = {m->dsys, m->dmax_iter, m->dptr, m->sm, m->b, m->x, m->r, m->p0, m->p1, m->t, m->res};
+  void* dev[] = {m->dsys, m->dmax_iter, m->dptr, m->sm, m->b, m->x, m->r, m->p0, m->p1, m->t, m->U, m->res};
   for (void* p : dev)
     if (p) (void)hipFree(p);
   if (m->hmax_iter) (void)hipHostFree(m->hmax_iter);
@@ -372,6 +394,7 @@ static CGUFix cgu_fix(gnx_multi* m) {
   CGUFix a;
   a.sys = m->dsys;
   a.sm = m->sm;
+  a.U = m->U;
   a.b = m->b;
   a.x = m->x;
   a.r = m->r;
@@ -385,8 +408,9 @@ static CGUFix cgu_fix(gnx_multi* m) {
   return a;
 }
 
-// the chunk graph: ENG_CHUNK (A, B) pairs over all systems
-static hipError_t cgu_graph(gnx_multi* m, hipGraphExec_t* exec) {
+// the chunk graph: ENG_CHUNK (A, B) pairs over all systems, on M M^dag or
+// on M_U M_U^dag (link)
+static hipError_t cgu_graph(gnx_multi* m, bool link, hipGraphExec_t* exec) {
   CGUFix a = cgu_fix(m);
   const int n = m->n;
   dim3 grid((m->nbmax + NUM_XCD - 1) / NUM_XCD * NUM_XCD, n), blk(CGU_BS);
@@ -407,11 +431,15 @@ static hipError_t cgu_graph(gnx_multi* m, hipGraphExec_t* exec) {
       // the per-wave head when every system has at most 512 partials (the
       // form a single-lattice solve of each system then also uses)
       const bool wh = head_wave(m->nbmax);
-      (void)hipLaunchKernel(wh ? (const void*)k_cgu_A<true> : (const void*)k_cgu_A<false>, grid, blk, argsA, 0, cs);
-      (void)hipLaunchKernel(wh ? (const void*)k_cgu_B<true> : (const void*)k_cgu_B<false>, grid, blk, argsB, 0, cs);
+      const void* A = link ? (wh ? (const void*)k_cgu_A<true, true> : (const void*)k_cgu_A<false, true>)
+                           : (wh ? (const void*)k_cgu_A<true, false> : (const void*)k_cgu_A<false, false>);
+      const void* B = link ? (wh ? (const void*)k_cgu_B<true, true> : (const void*)k_cgu_B<false, true>)
+                           : (wh ? (const void*)k_cgu_B<true, false> : (const void*)k_cgu_B<false, false>);
+      (void)hipLaunchKernel(A, grid, blk, argsA, 0, cs);
+      (void)hipLaunchKernel(B, grid, blk, argsB, 0, cs);
     }
   };
-  return m->eng.graph(0, n, body, exec);
+  return m->eng.graph(link ? 1 : 0, n, body, exec);
 }
 
 }  // namespace gnx
@@ -498,17 +526,30 @@ gnx_status gnx_multi_set_stream(gnx_multi* m, void* stream) {
   return GNX_OK;
 }
 
-gnx_status gnx_multi_cg(gnx_multi* m, const double* const* sm, const double* const* b, double* const* x, double tol,
-                        long max_iter, gnx_cg_start start, gnx_cg_result* res) {
-  using namespace gnx;
+}  // extern "C"
+
+namespace gnx {
+
+// one uneven-batch solve; U = null: M M^dag, else M_U M_U^dag with system
+// k's links U[k] (3 per site, f-order)
+static gnx_status multi_cg(gnx_multi* m, const double* const* sm, const double* const* Ul, const double* const* b,
+                           double* const* x, double tol, long max_iter, gnx_cg_start start, gnx_cg_result* res) {
   CHECK_ARG(m && sm && b && x, "gnx_multi_cg: null argument");
   CHECK_ARG(tol >= 0.0, "gnx_multi_cg: negative tolerance");
   CHECK_ARG(start == GNX_START_NULL || start == GNX_START_IMPOSED, "gnx_multi_cg: bad start");
   for (int k = 0; k < m->n; k++) {
     CHECK_ARG(sm[k] && b[k] && x[k], "gnx_multi_cg: null vector");
     CHECK_ARG((const void*)b[k] != (const void*)x[k], "gnx_multi_cg: b and x alias");
+    CHECK_ARG(!Ul || Ul[k], "gnx_multi_link_cg: null links");
   }
+  const bool link = Ul != nullptr;
   HIP_TRY(hipSetDevice(m->device));
+  if (link && !m->U) {
+    // the links' buffer; captured graphs point at the old arguments
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    HIP_TRY(hipMalloc(&m->U, sizeof(double2) * 3 * (size_t)m->vtotal));
+    m->eng.invalidate_graphs();
+  }
   hipStream_t st = m->stream;
   // the pinned tables below are rewritten here: no copy of an earlier call
   // (one that returned early on an error) may still be reading them
@@ -520,7 +561,7 @@ gnx_status gnx_multi_cg(gnx_multi* m, const double* const* sm, const double* con
     // the reference's cap V+1 (macro.h:134) of each system when max_iter <= 0
     m->hmax_iter[k] = max_iter > 0 ? max_iter : (long)d.g.V + 1;
     if (m->hmax_iter[k] > cap_max) cap_max = m->hmax_iter[k];
-    m->hptr[k] = CGUPtrs{sm[k], (const double2*)b[k], (double2*)x[k]};
+    m->hptr[k] = CGUPtrs{sm[k], (const double2*)b[k], (double2*)x[k], link ? (const double2*)Ul[k] : nullptr};
   }
   HIP_TRY(hipMemcpyAsync(m->dmax_iter, m->hmax_iter, sizeof(long) * n, hipMemcpyHostToDevice, st));
   HIP_TRY(hipMemcpyAsync(m->dptr, m->hptr, sizeof(CGUPtrs) * n, hipMemcpyHostToDevice, st));
@@ -535,8 +576,10 @@ gnx_status gnx_multi_cg(gnx_multi* m, const double* const* sm, const double* con
   const long* mi = m->dmax_iter;
   if (start == GNX_START_NULL) {
     void* a1[] = {(void*)&a};
-    HIP_TRY(hipLaunchKernel((const void*)k_cgu_null_p1, grid, blk, a1, 0, st));
-    HIP_TRY(hipLaunchKernel((const void*)k_cgu_null_p2, grid, blk, a1, 0, st));
+    HIP_TRY(hipLaunchKernel(link ? (const void*)k_cgu_null_p1<true> : (const void*)k_cgu_null_p1<false>, grid, blk,
+                            a1, 0, st));
+    HIP_TRY(hipLaunchKernel(link ? (const void*)k_cgu_null_p2<true> : (const void*)k_cgu_null_p2<false>, grid, blk,
+                            a1, 0, st));
     void* a3[] = {(void*)&a, (void*)&tol, (void*)&mi};
     HIP_TRY(hipLaunchKernel((const void*)k_cgu_null_p3, grid, blk, a3, 0, st));
   } else {
@@ -544,7 +587,7 @@ gnx_status gnx_multi_cg(gnx_multi* m, const double* const* sm, const double* con
     HIP_TRY(hipLaunchKernel((const void*)k_cgu_start_imposed, grid, blk, a0, 0, st));
   }
   hipGraphExec_t exec;
-  HIP_TRY(cgu_graph(m, &exec));
+  HIP_TRY(cgu_graph(m, link, &exec));
   HIP_TRY(m->eng.run(exec, n, cap_max, st, nullptr));
   {
     const CGUPtrs* dp = m->dptr;
@@ -569,6 +612,21 @@ gnx_status gnx_multi_cg(gnx_multi* m, const double* const* sm, const double* con
     return GNX_ERR_NOCONV;
   }
   return GNX_OK;
+}
+
+}  // namespace gnx
+
+extern "C" {
+
+gnx_status gnx_multi_cg(gnx_multi* m, const double* const* sm, const double* const* b, double* const* x, double tol,
+                        long max_iter, gnx_cg_start start, gnx_cg_result* res) {
+  return gnx::multi_cg(m, sm, nullptr, b, x, tol, max_iter, start, res);
+}
+
+gnx_status gnx_multi_link_cg(gnx_multi* m, const double* const* sm, const double* const* U, const double* const* b,
+                             double* const* x, double tol, long max_iter, gnx_cg_start start, gnx_cg_result* res) {
+  CHECK_ARG(U != nullptr, "gnx_multi_link_cg: null links");
+  return gnx::multi_cg(m, sm, U, b, x, tol, max_iter, start, res);
 }
 
 }  // extern "C"

@@ -127,11 +127,12 @@ namespace gnx {
 // 2k+1 of m).  A chain that is no longer live has its phi and X cleared, so
 // its two systems leave in the prologue; ok[k] and the chain's CG counters
 // follow the single-chain solve (hmc_cg, lh_cg).  H: gnx_hmc or
-// gnx_link_hmc (fields c, sm, phi, X, iterations, calls).
+// gnx_link_hmc (fields c, sm, phi, X, iterations, calls).  links: chain k's
+// links (3 per site) for solves with the link-coupled operator, or null.
 template <typename H>
 inline gnx_status ens_solve(gnx_multi* m, int n, H* const* hs, const std::vector<char>& live, double tol,
-                     long max_iter, int start, std::vector<char>& ok) {
-  std::vector<const double*> sms(2 * n), bs(2 * n);
+                     long max_iter, int start, std::vector<char>& ok, const double2* const* links = nullptr) {
+  std::vector<const double*> sms(2 * n), bs(2 * n), us(2 * n);
   std::vector<double*> xs(2 * n);
   for (int k = 0; k < n; k++) {
     H* h = hs[k];
@@ -147,9 +148,13 @@ inline gnx_status ens_solve(gnx_multi* m, int n, H* const* hs, const std::vector
     bs[2 * k + 1] = (const double*)(h->phi + V);
     xs[2 * k] = (double*)h->X;
     xs[2 * k + 1] = (double*)(h->X + V);
+    if (links) us[2 * k] = us[2 * k + 1] = (const double*)links[k];
   }
   std::vector<gnx_cg_result> r(2 * n);
-  const gnx_status s = gnx_multi_cg(m, sms.data(), bs.data(), xs.data(), tol, max_iter, (gnx_cg_start)start, r.data());
+  const gnx_status s =
+      links ? gnx_multi_link_cg(m, sms.data(), us.data(), bs.data(), xs.data(), tol, max_iter, (gnx_cg_start)start,
+                                r.data())
+            : gnx_multi_cg(m, sms.data(), bs.data(), xs.data(), tol, max_iter, (gnx_cg_start)start, r.data());
   if (s != GNX_OK && s != GNX_ERR_NOCONV) return s;
   for (int k = 0; k < n; k++) {
     ok[k] = 0;

@@ -575,11 +575,13 @@ gnx_status gnx_link_hmc_trajectory(gnx_link_hmc* h, const double* mom_host, cons
   return GNX_OK;
 }
 
-// n chains in lockstep (moleculardynamics.c:419-542 for each), the live
-// operator only: the steps of gnx_link_hmc_trajectory chain by chain, except
-// that every solve point is one uneven-batch solve over all live chains
-// (systems 2k, 2k+1 of m: chain k's two flavours).  A chain whose solve
-// fails leaves as gnx_link_hmc_trajectory would (links restored, ok = 0).
+// n chains in lockstep (moleculardynamics.c:419-542 for each): the steps of
+// gnx_link_hmc_trajectory chain by chain, except that every solve point is
+// one uneven-batch solve over all live chains (systems 2k, 2k+1 of m: chain
+// k's two flavours), with the live operator when the chains were created
+// with linked = 0 and the link-coupled one (each chain's own links) when
+// with linked = 1; all chains of a call alike.  A chain whose solve fails
+// leaves as gnx_link_hmc_trajectory would (links restored, ok = 0).
 gnx_status gnx_link_hmc_ensemble_trajectory(gnx_multi* m, int n, gnx_link_hmc* const* hs,
                                             const double* const* mom_host, const double* const* chi0_host,
                                             const double* const* chi1_host, double eps, int steps, double tol,
@@ -590,8 +592,8 @@ gnx_status gnx_link_hmc_ensemble_trajectory(gnx_multi* m, int n, gnx_link_hmc* c
   CHECK_ARG(multi_count(m) == 2 * n, "gnx_link_hmc_ensemble_trajectory: the solver must hold two systems per chain");
   for (int k = 0; k < n; k++) {
     CHECK_ARG(hs[k] && mom_host[k] && chi0_host[k] && chi1_host[k], "gnx_link_hmc_ensemble_trajectory: null argument");
-    CHECK_ARG(!hs[k]->linked,
-              "gnx_link_hmc_ensemble_trajectory: the shared solve is the live operator (chains created with linked = 0)");
+    CHECK_ARG(hs[k]->linked == hs[0]->linked,
+              "gnx_link_hmc_ensemble_trajectory: the chains share one solve, so all must have the same `linked`");
     for (int f = 0; f < 2; f++) {
       int L = 0, LT = 0, dev = -1;
       multi_system(m, 2 * k + f, &L, &LT, &dev);
@@ -617,14 +619,21 @@ gnx_status gnx_link_hmc_ensemble_trajectory(gnx_multi* m, int n, gnx_link_hmc* c
       if ((s = lh_upload(h, chis[f], (double*)h->Ulex, (size_t)V)) != GNX_OK) return s;
       hipLaunchKernelGGL(k_real_to_complex_l, dim3(nblk(V, BS)), dim3(BS), 0, c->stream, (int)V,
                          (const double*)h->Ulex, h->chi + f * V);
-      HIP_TRY(launch_dirac(c->g, GNX_OP_M, h->sm, (const double*)(h->chi + f * V), (double*)(h->phi + f * V),
-                           c->tmp, c->stream));
+      // phi = M chi (moleculardynamics.c:434-438): the live M, or M_U
+      if (h->linked)
+        HIP_TRY(link_dirac(c, 0, h->sm, h->U, h->chi + f * V, h->phi + f * V));
+      else
+        HIP_TRY(launch_dirac(c->g, GNX_OP_M, h->sm, (const double*)(h->chi + f * V), (double*)(h->phi + f * V),
+                             c->tmp, c->stream));
       HIP_TRY(hipStreamSynchronize(c->stream));
     }
     HIP_TRY(hipMemcpyAsync(h->U_copy, h->U, sizeof(double2) * 3 * V, hipMemcpyDeviceToDevice, c->stream));
   }
+  std::vector<const double2*> chain_links(n);
+  for (int k = 0; k < n; k++) chain_links[k] = hs[k]->U;
   auto solve = [&]() -> gnx_status {
-    gnx_status q = ens_solve(m, n, hs, live, tol, max_iter, GNX_START_NULL, ok);
+    gnx_status q =
+        ens_solve(m, n, hs, live, tol, max_iter, GNX_START_NULL, ok, hs[0]->linked ? chain_links.data() : nullptr);
     if (q != GNX_OK) return q;
     for (int k = 0; k < n; k++)
       if (live[k] && !ok[k]) {

@@ -137,6 +137,8 @@ def lib():
         L.gnx_multi_set_stream.argtypes = [vp, vp]
         L.gnx_multi_cg.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.c_double, C.c_long,
                                    C.c_int, C.POINTER(CGResult)]
+        L.gnx_multi_link_cg.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.c_double,
+                                        C.c_long, C.c_int, C.POINTER(CGResult)]
         L.gnx_hmc_ensemble_trajectory.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(dp), C.POINTER(dp),
                                                   C.POINTER(dp), C.c_double, C.c_int, C.c_double, C.c_double,
                                                   C.c_long, C.POINTER(TrajResult)]
@@ -537,18 +539,26 @@ class Multi:
         except Exception:
             pass
 
-    def cg(self, sms, bs, xs, tol, max_iter=0, start=START_NULL):
-        """sms[k] (V_k), bs[k], xs[k] (2 V_k) device tensors; returns (status, [CGResult])."""
+    def cg(self, sms, bs, xs, tol, max_iter=0, start=START_NULL, links=None):
+        """sms[k] (V_k), bs[k], xs[k] (2 V_k) device tensors; links[k] (6 V_k
+        doubles, f-order, three complex per site) selects the link-coupled
+        operator; returns (status, [CGResult])."""
         vp = C.c_void_p
         n = self.n
         assert len(sms) == len(bs) == len(xs) == n
         for k in range(n):
             assert sms[k].numel() == self.V[k] and as_real(bs[k]).numel() == 2 * self.V[k]
             assert as_real(xs[k]).numel() == 2 * self.V[k]
+            assert links is None or as_real(links[k]).numel() == 6 * self.V[k]
         res = (CGResult * n)()
-        st = lib().gnx_multi_cg(self.h, (vp * n)(*[_ptr(t) for t in sms]),
-                                (vp * n)(*[_ptr(as_real(t)) for t in bs]),
-                                (vp * n)(*[_ptr(as_real(t)) for t in xs]), tol, max_iter, start, res)
+        a_sm = (vp * n)(*[_ptr(t) for t in sms])
+        a_b = (vp * n)(*[_ptr(as_real(t)) for t in bs])
+        a_x = (vp * n)(*[_ptr(as_real(t)) for t in xs])
+        if links is None:
+            st = lib().gnx_multi_cg(self.h, a_sm, a_b, a_x, tol, max_iter, start, res)
+        else:
+            st = lib().gnx_multi_link_cg(self.h, a_sm, (vp * n)(*[_ptr(as_real(t)) for t in links]), a_b, a_x, tol,
+                                         max_iter, start, res)
         if st not in (0, 3):
             _check(st)
         return st, list(res)

@@ -408,7 +408,7 @@ def ensemble_leg(gnx, sizes, ntraj, betas=(0.6, 0.75, 0.85, 1.0), eps=0.02, step
     lockstep (gnx.Ensemble: each solve point of an integrator one
     uneven-batch launch over all chains)."""
     import concurrent.futures as cf
-    def make_chains(force_prec=None):
+    def make_chains(force_prec=None, linked=False):
         out = []
         for L in sizes:
             for bs in betas:
@@ -420,9 +420,14 @@ def ensemble_leg(gnx, sizes, ntraj, betas=(0.6, 0.75, 0.85, 1.0), eps=0.02, step
                 h.set_sigma(rng.sigma(V, 2.0 / (4 * bs)))
                 if force_prec is not None:
                     h.set_force_precision(force_prec)
-                lh = gnx.LinkHMC(ctx, beta=1.0, pure_gauge=True, linked=False)
+                lh = gnx.LinkHMC(ctx, beta=1.0, pure_gauge=True, linked=linked)
                 _, sm = h.get_sigma()
-                lh.set_fields(np.exp(1j * 2.0 * np.pi * rng.casuale(3 * V)), sm)
+                if linked:
+                    # near unity: phases 2 pi (u - 1/2) / 100, u uniform
+                    U = np.exp(1j * 2.0 * np.pi * (rng.casuale(3 * V) - 0.5) / 100.0)
+                else:
+                    U = np.exp(1j * 2.0 * np.pi * rng.casuale(3 * V))
+                lh.set_fields(U, sm)
                 out.append({"L": L, "beta_s": bs, "V": V, "ctx": ctx, "h": h, "lh": lh, "rng": rng,
                             "dH_sigma": [], "dH_link": [], "acc_sigma": 0, "acc_link": 0, "cg_iterations": 0})
         return out
@@ -494,7 +499,7 @@ def ensemble_leg(gnx, sizes, ntraj, betas=(0.6, 0.75, 0.85, 1.0), eps=0.02, step
     es = gnx.Ensemble([c["h"] for c in lock])
     el = gnx.Ensemble([c["lh"] for c in lock])
 
-    def run_lockstep(n):
+    def run_lockstep(n, lock, es, el):
         for _ in range(n):
             dr = [(c["rng"].gauss1(c["V"]), c["rng"].gauss1(c["V"]), c["rng"].gauss1(c["V"])) for c in lock]
             _, ts = es.sigma_trajectory([d[0] for d in dr], [d[1] for d in dr], [d[2] for d in dr],
@@ -518,12 +523,12 @@ def ensemble_leg(gnx, sizes, ntraj, betas=(0.6, 0.75, 0.85, 1.0), eps=0.02, step
                 c["acc_link"] += int(al)
                 c["cg_iterations"] += tl.cg_iterations
 
-    run_lockstep(1)  # warm-up, as above
+    run_lockstep(1, lock, es, el)  # warm-up, as above
     for c in lock:
         c["dH_sigma"].clear(), c["dH_link"].clear()
         c["acc_sigma"] = c["acc_link"] = c["cg_iterations"] = 0
     t0 = time.perf_counter()
-    run_lockstep(ntraj)
+    run_lockstep(ntraj, lock, es, el)
     dtl = time.perf_counter() - t0
     # the two runs tune their own CG launch shapes, so rounding differs; a
     # force solve to 1e-3 can then stop one iteration apart, which moves its
@@ -540,6 +545,30 @@ def ensemble_leg(gnx, sizes, ntraj, betas=(0.6, 0.75, 0.85, 1.0), eps=0.02, step
     es.close()
     el.close()
     for c in chains + lock:
+        c["lh"].close()
+        c["h"].close()
+
+    # the consistent link-coupled theory (linked = 1: solves and force with
+    # the link-coupled operator) from links near unity, in lockstep
+    lk = make_chains(linked=True)
+    es = gnx.Ensemble([c["h"] for c in lk])
+    el = gnx.Ensemble([c["lh"] for c in lk])
+    run_lockstep(1, lk, es, el)
+    for c in lk:
+        c["dH_sigma"].clear(), c["dH_link"].clear()
+        c["acc_sigma"] = c["acc_link"] = c["cg_iterations"] = 0
+    t0 = time.perf_counter()
+    run_lockstep(ntraj, lk, es, el)
+    dtk = time.perf_counter() - t0
+    out["lockstep_linked"] = {
+        "mode": "all chains in lockstep, link trajectories with the link-coupled operator (linked = 1) from "
+                "links near unity (phases uniform in +-pi/100)",
+        "trajectories_per_s": len(lk) * ntraj / dtk, "seconds": dtk,
+        "chains": [{"L": c["L"], "beta_s": c["beta_s"], "dH_link": c["dH_link"], "accepted_link": c["acc_link"],
+                    "accepted_sigma": c["acc_sigma"], "cg_iterations": c["cg_iterations"]} for c in lk]}
+    es.close()
+    el.close()
+    for c in lk:
         c["lh"].close()
         c["h"].close()
     return out

@@ -155,6 +155,12 @@ gnx_status gnx_multi_destroy(gnx_multi *m);
 gnx_status gnx_multi_set_stream(gnx_multi *m, void *stream);
 gnx_status gnx_multi_cg(gnx_multi *m, const double *const *sm, const double *const *b, double *const *x,
                         double tol, long max_iter, gnx_cg_start start, gnx_cg_result *res);
+/* The same on the link-coupled operator M_U M_U^dag (as gnx_link_cg_batch,
+ * copy_dirac.c:46-123), system k with its own links U[k] (device, 3 V_k
+ * complex, f-order, three per site); bit-identical to its gnx_link_cg_batch
+ * solve under the same condition. */
+gnx_status gnx_multi_link_cg(gnx_multi *m, const double *const *sm, const double *const *U, const double *const *b,
+                             double *const *x, double tol, long max_iter, gnx_cg_start start, gnx_cg_result *res);
 
 /* A12, batched.  Stochastic trace term of the condensate for nsrc noise
  * vectors at once (condensate.c:127-145 is this for one vector at a time):
@@ -371,9 +377,10 @@ gnx_status gnx_link_hmc_accept(gnx_link_hmc *h, int accepted);
 gnx_status gnx_link_hmc_energy(gnx_link_hmc *h, double *H);
 gnx_status gnx_link_hmc_cg_stats(gnx_link_hmc *h, double *iterations, int *calls);
 /* n chains' LeapFrog2Link trajectories in lockstep, each solve point one
- * launch per CG kernel over all live chains (as gnx_hmc_ensemble_trajectory;
- * chains created with linked = 0, the live operator).  No counterpart in the
- * reference. */
+ * launch per CG kernel over all live chains (as gnx_hmc_ensemble_trajectory):
+ * the live operator for chains created with linked = 0, the link-coupled one
+ * (gnx_multi_link_cg, each chain's links) for linked = 1; all chains of a
+ * call alike.  No counterpart in the reference. */
 gnx_status gnx_link_hmc_ensemble_trajectory(gnx_multi *m, int n, gnx_link_hmc *const *hs,
                                             const double *const *mom_host, const double *const *chi0_host,
                                             const double *const *chi1_host, double eps, int steps, double tol,

@@ -67,26 +67,32 @@ def test_sigma_ensemble_matches_single_chains(P, gnx_mod, monkeypatch):
     E.close()
 
 
-def test_link_ensemble_matches_single_chains(P, gnx_mod, monkeypatch):
-    banner("link ensemble (4, 8, 6, 10)^3 vs single chains")
+@pytest.mark.parametrize("linked", [False, True])
+def test_link_ensemble_matches_single_chains(P, gnx_mod, linked, monkeypatch):
+    """Both operator semantics: the live operator (linked = 0) and the
+    link-coupled one (linked = 1, each chain's solves on M_U M_U^dag with its
+    own links in the shared uneven-batch launch)."""
+    banner(f"link ensemble (4, 8, 6, 10)^3 linked={linked} vs single chains")
     monkeypatch.setenv("GNX_CG_SHAPE", "1")
     single, ens = [], []
     for i, (L, bs) in enumerate(CHAINS):
         V = L ** 3
-        sm = P.Lattice(L).sigma_mass(_sigma(P, L, bs, 950 + i))
+        # with random links, M_U M_U^dag needs the mass shifted to stay well
+        # conditioned (as in the single-lattice link tests)
+        sm = P.Lattice(L).sigma_mass(_sigma(P, L, bs, 950 + i)) + (0.5 if linked else 0.0)
         U = P.random_links(V, 960 + i)
         for lst in (single, ens):
-            h = gnx_mod.LinkHMC(gnx_mod.Context(L), beta=1.0, pure_gauge=True, linked=False)
+            h = gnx_mod.LinkHMC(gnx_mod.Context(L), beta=1.0, pure_gauge=True, linked=linked)
             h.set_fields(U, sm)
             lst.append(h)
     E = gnx_mod.Ensemble(ens)
     for t in range(2):
         draws = [_draws(300 + 100 * t + i, L ** 3, 3 * L ** 3) for i, (L, _) in enumerate(CHAINS)]
         st, res = E.link_trajectory([d[0] for d in draws], [d[1] for d in draws], [d[2] for d in draws],
-                                    0.02, 4, 1e-10)
+                                    0.02, 4, 1e-10, max_iter=20000)
         assert st == 0
         for i, h in enumerate(single):
-            s1, r1 = h.trajectory(*draws[i], 0.02, 4, 1e-10)
+            s1, r1 = h.trajectory(*draws[i], 0.02, 4, 1e-10, max_iter=20000)
             assert s1 == 0 and r1.ok
             _same(res[i], r1, (t, CHAINS[i]))
             accept = (t + i) % 2 == 1

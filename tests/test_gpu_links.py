@@ -111,15 +111,22 @@ def test_link_cg_vs_oracle(P, gnx_mod, start, L, shape, monkeypatch):
     assert rel(host(x), xr) < 1e-8
 
 
-@pytest.mark.parametrize("linked", [0, 1])
-def test_link_trajectories_vs_oracle(P, gnx_mod, linked):
-    banner(f"LeapFrog2Link 8^3 linked={linked}")
-    lat, r, sm, U = setup(P, 8, 8, 34)
+@pytest.mark.parametrize("L,linked,spread", [(8, 0, None), (8, 1, None), (12, 1, 0.05), (16, 1, 0.05)])
+def test_link_trajectories_vs_oracle(P, gnx_mod, L, linked, spread):
+    """LeapFrog2Link against the oracle's: uniform random links at 8^3
+    (both operator semantics), and the consistent link-coupled theory
+    (linked = 1) at 12^3 and 16^3 from links near unity (phases
+    N(0, spread^2)), where trajectories are accepted; Delta-H within 1e-8
+    relative, identical accept decisions, links within 1e-9."""
+    banner(f"LeapFrog2Link {L}^3 linked={linked} spread={spread}")
+    lat, r, sm, U = setup(P, L, L, 34)
+    if spread is not None:
+        U = np.exp(1j * spread * np.random.default_rng(L).normal(size=3 * lat.V))
     V = lat.V
     eps, steps, tol, beta = 0.02, 10, 1e-10, 1.0
     ref = P.LinkHMC(lat, 777, sm, U, beta=beta, pure_gauge=True, linked=linked)
     draws = P.Random(777)  # the same stream, drawn here for the GPU
-    ctx = gnx_mod.Context(8)
+    ctx = gnx_mod.Context(L)
     h = gnx_mod.LinkHMC(ctx, beta=beta, pure_gauge=True, linked=linked)
     h.set_fields(U, sm)
     for k in range(3):

@@ -101,3 +101,45 @@ def test_multi_iteration_cap(P, gnx_mod, monkeypatch):
         s1, r1, x1 = _single(gnx_mod, lat, sm, b, x0, 1e-10, gnx_mod.START_IMPOSED, max_iter=5)
         assert s1 == 3 and r1.iterations == 5
         assert np.array_equal(host(xs[k]), x1), k
+
+
+@pytest.mark.parametrize("start", ["imposed", "null"])
+def test_multi_link_matches_single_and_oracle(P, gnx_mod, start, monkeypatch):
+    """The link-coupled operator (gnx_multi_link_cg): every system with its
+    own random U(1) links; bit-identical to its gnx_link_cg_batch solve with
+    GNX_CG_SHAPE=1, and against the oracle's link-coupled CG (iterations
+    +-1, solution within 1e-8)."""
+    import torch
+    banner(f"uneven batch, link-coupled {SIZES} {start}")
+    monkeypatch.setenv("GNX_CG_SHAPE", "1")
+    # random U(1) links: the mass shifted by 0.5 keeps M_U M_U^dag well
+    # conditioned (as in the single-lattice link tests)
+    sy = [(lat, sm + 0.5, b, x0) for lat, sm, b, x0 in _systems(P, 760)]
+    links = [P.random_links(lat.V, 770 + k) for k, (lat, _, _, _) in enumerate(sy)]
+    code = gnx_mod.START_IMPOSED if start == "imposed" else gnx_mod.START_NULL
+    ufs = []
+    for (lat, _, _, _), U in zip(sy, links):
+        ctx = gnx_mod.Context(lat.L, lat.LT)
+        uf = torch.zeros(6 * lat.V, dtype=torch.float64, device="cuda")
+        ctx.links_from_lex(dev(U.view(np.float64)), uf)
+        ufs.append((ctx, uf))
+    m = gnx_mod.Multi(SIZES)
+    xs = [dev(x0) for _, _, _, x0 in sy]
+    st, res = m.cg([dev(sm) for _, sm, _, _ in sy], [dev(b) for _, _, b, _ in sy], xs, 1e-10, max_iter=20000,
+                   start=code, links=[uf for _, uf in ufs])
+    assert st == 0 and all(r.converged for r in res)
+    for k, (lat, sm, b, x0) in enumerate(sy):
+        ctx, uf = ufs[k]
+        x1 = dev(x0)
+        s1, r1 = ctx.link_cg_batch(dev(sm), uf, dev(b), x1, 1, 1e-10, max_iter=20000, start=code)
+        assert s1 == 0
+        assert (r1[0].iterations, r1[0].early_exit) == (res[k].iterations, res[k].early_exit), k
+        assert np.array_equal(host(xs[k]), host(x1)), k
+        xr, ok, ost = lat.link_cg(sm, links[k], b, x0=x0.copy(), tol=1e-10, start=start, max_iter=20000)
+        assert ok and abs(res[k].iterations - ost.last_iters) <= 1, (k, res[k].iterations, ost.last_iters)
+        assert rel(host(xs[k]), xr) < 1e-8, k
+    # and the live operator again on the same solver (the other graph)
+    st, res2 = m.cg([dev(sm) for _, sm, _, _ in sy], [dev(b) for _, _, b, _ in sy], [dev(x0) for *_, x0 in sy],
+                    1e-10, max_iter=20000, start=code)
+    assert st == 0 and all(r.converged for r in res2)
+    m.close()
