@@ -28,6 +28,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include "gnx_cgstate.h"
 #include "gnx_engine.h"
 #include "gnx_internal.h"
 #include "gnx_solver.h"
@@ -166,15 +167,60 @@ struct MixFix {
   void* p0;  // direction buffers, slot parity: X reads p[sl], writes p[sl+1]
   void* p1;
   void* t;
-  double* part;     // [2 slots][nrhs][nb]
+  double* part;     // [2 slots][ptotal]
   MixState* state;  // [2][nrhs]
   CGDevResult* res;
   int nrhs;
   int nb;
   int nt;           // non-temporal stores: the working set exceeds the Infinity Cache
+  long ptotal;      // partials per slot (nrhs * nb, or the uneven systems' sum)
+  const CGUSys* sys;  // uneven batches: system k's lattice and offsets
+  const long* caps;   // uneven batches: system k's iteration cap
 };
 
+// Where a block's system lives, a kernel template argument.  MixOne: every
+// system on the lattice of MixFix (a batch sharing one sigma mass; system
+// blockIdx.y at offset blockIdx.y V).  MixUneven: system blockIdx.y on its
+// own lattice, vectors, sigma mass and partials at the offsets of its
+// descriptor (an uneven batch); a block past the system's sites leaves.
+struct MixOne {
+  static constexpr bool kUneven = false;
+  Geom g;
+  size_t off, soff, poff;
+  int nb;
+  GNX_DEVONLY bool init(const struct MixFix& a);
+};
+struct MixUneven {
+  static constexpr bool kUneven = true;
+  Geom g;
+  size_t off, soff, poff;
+  int nb;
+  GNX_DEVONLY bool init(const struct MixFix& a);
+};
+GNX_DEVONLY bool MixOne::init(const MixFix& a) {
+  g = a.g;
+  nb = a.nb;
+  off = (size_t)blockIdx.y * a.g.V;
+  soff = 0;
+  poff = (size_t)blockIdx.y * a.nb;
+  return true;
+}
+GNX_DEVONLY bool MixUneven::init(const MixFix& a) {
+  const CGUSys d = a.sys[blockIdx.y];
+  g = d.g;
+  nb = d.nb;
+  off = (size_t)d.voff;
+  soff = (size_t)d.soff;
+  poff = (size_t)d.poff;
+  return (int)blockIdx.x < d.nb;
+}
+
 GNX_DEV double* mslot(const MixFix& a, int s, int rhs) { return a.part + ((size_t)s * a.nrhs + rhs) * a.nb; }
+// slot s of the block's system
+template <typename SV>
+GNX_DEVONLY double* mslot_v(const MixFix& a, const SV& v, int s) {
+  return a.part + (size_t)s * a.ptotal + v.poff;
+}
 
 GNX_DEVONLY void* mix_p(const MixFix& a, int k) { return (k & 1) ? a.p1 : a.p0; }
 
@@ -184,26 +230,29 @@ GNX_DEVONLY double dnorm2(float2 v) { return (double)v.x * v.x + (double)v.y * v
 // Every kernel of a solve uses the same SPT (sites per thread), so the
 // per-block partial slots have one layout: nb = ceil(V / (SPT * MIX_BS)).
 // Thread sites: f = blockIdx.x * SPT * MIX_BS + i * MIX_BS + threadIdx.x.
-template <int SPT, bool XCD>
-GNX_DEVONLY int mix_site(const MixFix& a, int i) {
-  return SiteMap<SPT, XCD>(a.nb).f(a.g, i);
+template <int SPT, bool XCD, typename SV>
+GNX_DEVONLY int mix_site(const SV& v, int i) {
+  return SiteMap<SPT, XCD>(v.nb).f(v.g, i);
 }
 
 // imposed start: x = 0, r = p0 = (S)b, xs = 0, sm and b copied; partial |b|^2
-template <typename S, int SPT, bool XCD>
+// (an uneven batch passes the solver's own gathered sm and b: nothing to copy)
+template <typename S, int SPT, bool XCD, typename SV = MixOne>
 __global__ __launch_bounds__(MIX_BS) void k_mix_start_imposed(MixFix a, const double* __restrict__ sm_in,
                                                               const double2* __restrict__ b_in, double tol,
                                                               double delta, long max_iter) {
   __shared__ double lds[MIX_BS / 64];
+  SV v;
+  if (!v.init(a)) return;
   const int rhs = blockIdx.y;
-  const size_t off = (size_t)rhs * a.g.V;
+  const size_t off = v.off;
   double acc = 0.0;
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
-    const int f = mix_site<SPT, XCD>(a, i);
+    const int f = mix_site<SPT, XCD>(v, i);
     if (f < 0) break;
     double2 bb = b_in[off + f];
-    a.b[off + f] = bb;
+    if (!SV::kUneven) a.b[off + f] = bb;
     a.x[off + f] = make_double2(0.0, 0.0);
     // a scaled format stores r = s b in the first slot (a reliable update),
     // once ||b|| is known; zeros here keep every loaded value finite
@@ -212,79 +261,86 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_start_imposed(MixFix a, const do
     S::st((typename S::T*)a.p0 + off, f, bs, a.nt);
     S::st((typename S::T*)a.p1 + off, f, make_float2(0.f, 0.f), a.nt);
     a.xs[off + f] = make_float2(0.f, 0.f);
-    if (rhs == 0) {
-      a.sm[f] = sm_in[f];
-      a.sm32[f] = (float)sm_in[f];
+    if (SV::kUneven || rhs == 0) {
+      if (!SV::kUneven) a.sm[f] = sm_in[f];
+      a.sm32[v.soff + f] = (float)sm_in[v.soff + f];
     }
     acc += cnorm2(bb);
   }
   acc = block_sum(acc, lds);
-  if (threadIdx.x == 0) mslot(a, 1, rhs)[partial_slot(blockIdx.x, a.nb)] = acc;
+  if (threadIdx.x == 0) mslot_v(a, v, 1)[partial_slot(blockIdx.x, v.nb)] = acc;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     MixState cfg = {};
     cfg.scale = cfg.scale_next = 1.0;
     cfg.start = GNX_START_IMPOSED;
     cfg.tol = tol;
     cfg.delta = delta;
-    cfg.max_iter = max_iter;
+    cfg.max_iter = SV::kUneven ? a.caps[rhs] : max_iter;
     put_mstate(a.state + rhs, 1, 0, PH_NORMAL, 0, 0, 0, 0.0, 0.0, cfg);
   }
 }
 
 // null start (conjugategradient.c:18-33) pass 1: u = M^dag x_in, x = x_in,
 // b and sm copied
-template <int SPT, bool XCD>
+template <int SPT, bool XCD, typename SV = MixOne>
 __global__ __launch_bounds__(MIX_BS) void k_mix_null_p1(MixFix a, const double* __restrict__ sm_in,
                                                          const double2* __restrict__ b_in,
                                                          const double2* __restrict__ xin) {
-  const size_t off = (size_t)blockIdx.y * a.g.V;
+  SV v;
+  if (!v.init(a)) return;
+  const size_t off = v.off;
   const double2* xv = xin + off;
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
-    const int f = mix_site<SPT, XCD>(a, i);
+    const int f = mix_site<SPT, XCD>(v, i);
     if (f < 0) break;
-    a.u[off + f] = apply_Mdag(a.g, decode(a.g, f), sm_in[f], [&](int q) { return xv[q]; });
-    a.x[off + f] = xv[f];
-    a.b[off + f] = b_in[off + f];
-    if (blockIdx.y == 0) {
-      a.sm[f] = sm_in[f];
-      a.sm32[f] = (float)sm_in[f];
+    a.u[off + f] = apply_Mdag(v.g, decode(v.g, f), sm_in[v.soff + f], [&](int q) { return xv[q]; });
+    if (!SV::kUneven) {
+      a.x[off + f] = xv[f];
+      a.b[off + f] = b_in[off + f];
+    }
+    if (SV::kUneven || blockIdx.y == 0) {
+      if (!SV::kUneven) a.sm[f] = sm_in[f];
+      a.sm32[v.soff + f] = (float)sm_in[v.soff + f];
     }
   }
 }
 
 // pass 2: partials |b - M u|^2 (slot 0) and |b|^2 (slot 1)
-template <int SPT, bool XCD>
+template <int SPT, bool XCD, typename SV = MixOne>
 __global__ __launch_bounds__(MIX_BS) void k_mix_null_p2(MixFix a) {
   __shared__ double lds[MIX_BS / 64];
-  const int rhs = blockIdx.y;
-  const size_t off = (size_t)rhs * a.g.V;
+  SV v;
+  if (!v.init(a)) return;
+  const size_t off = v.off;
   const double2* u = a.u + off;
   double ar = 0.0, ab = 0.0;
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
-    const int f = mix_site<SPT, XCD>(a, i);
+    const int f = mix_site<SPT, XCD>(v, i);
     if (f < 0) break;
     double2 bb = a.b[off + f];
-    double2 rr = apply_sub_M(a.g, decode(a.g, f), a.sm[f], bb, [&](int q) { return u[q]; });
+    double2 rr = apply_sub_M(v.g, decode(v.g, f), a.sm[v.soff + f], bb, [&](int q) { return u[q]; });
     ar += cnorm2(rr);
     ab += cnorm2(bb);
   }
   ar = block_sum(ar, lds);
   ab = block_sum(ab, lds);
   if (threadIdx.x == 0) {
-    mslot(a, 0, rhs)[partial_slot(blockIdx.x, a.nb)] = ar;
-    mslot(a, 1, rhs)[partial_slot(blockIdx.x, a.nb)] = ab;
+    mslot_v(a, v, 0)[partial_slot(blockIdx.x, v.nb)] = ar;
+    mslot_v(a, v, 1)[partial_slot(blockIdx.x, v.nb)] = ab;
   }
 }
 
 // pass 3: an early exit keeps x_in, otherwise as the imposed start (slot 1 kept)
-template <typename S, int SPT, bool XCD>
+template <typename S, int SPT, bool XCD, typename SV = MixOne>
 __global__ __launch_bounds__(MIX_BS) void k_mix_null_p3(MixFix a, double tol, double delta, long max_iter) {
   __shared__ double lds[MIX_BS / 64];
+  SV v;
+  if (!v.init(a)) return;
   const int rhs = blockIdx.y;
-  const size_t off = (size_t)rhs * a.g.V;
-  const double nr = block_sum(partial_share(mslot(a, 0, rhs), a.nb), lds);
+  const size_t off = v.off;
+  const double nr = block_sum(partial_share(mslot_v(a, v, 0), v.nb), lds);
   const bool early = sqrt(nr) < tol;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     MixState cfg = {};
@@ -292,7 +348,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_null_p3(MixFix a, double tol, do
     cfg.start = GNX_START_NULL;
     cfg.tol = tol;
     cfg.delta = delta;
-    cfg.max_iter = max_iter;
+    cfg.max_iter = SV::kUneven ? a.caps[rhs] : max_iter;
     put_mstate(a.state + rhs, early ? 0 : 1, 0, PH_NORMAL, 0, early ? 1 : 0, early ? 1 : 0, 0.0, 0.0, cfg);
     if (early) {
       CGDevResult r = {1, 1, 0, 0, sqrt(nr), {0, 0, 0}};
@@ -302,7 +358,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_null_p3(MixFix a, double tol, do
   if (early) return;
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
-    const int f = mix_site<SPT, XCD>(a, i);
+    const int f = mix_site<SPT, XCD>(v, i);
     if (f < 0) break;
     double2 bb = a.b[off + f];
     a.x[off + f] = make_double2(0.0, 0.0);
@@ -519,15 +575,17 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Y(MixFix a, int odd, const MixSt
 // store (a third and fourth buffer would push a batch of 32 at 64^3 past the
 // 256 MB Infinity Cache); an update's X gathers p_prev at the neighbours and
 // so writes no direction, its Y carries p_old into p[sl+1] after the fold.
-template <typename S, int SPT, bool XCD, bool FLUSH>
+template <typename S, int SPT, bool XCD, bool FLUSH, typename SV = MixOne>
 __global__ __launch_bounds__(MIX_BS) void k_mix_Xe(MixFix a, int sl, const MixState* __restrict__ sin,
                                                   MixState* __restrict__ sout, const double* __restrict__ pin,
                                                   double* __restrict__ pout) {
   __shared__ double lds[MIX_BS / 64], lds2[MIX_BS / 64];  // head and tail reductions
   using T = typename S::T;
-  const Geom g = a.g;
+  SV v;
+  if (!v.init(a)) return;
+  const Geom g = v.g;
   const int rhs = blockIdx.y;
-  const size_t off = (size_t)rhs * g.V;
+  const size_t off = v.off;
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   const T* r = (const T*)a.r + off;
   const T* pold = (const T*)mix_p(a, sl) + off;
@@ -538,18 +596,18 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Xe(MixFix a, int sl, const MixSt
   // reduction runs while the operands are in flight
   const MixState s = load_state(sin + rhs);
   double pl[PART_K];
-  partial_issue(pin + (size_t)rhs * a.nb, a.nb, pl);
+  partial_issue(pin + v.poff, v.nb, pl);
   end_of_loads();
   int fs[SPT];
   float2 rv[SPT][7], pv[SPT][7], xsv[SPT], ppr[SPT];
   float smv[SPT];
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
-    fs[i] = mix_site<SPT, XCD>(a, i);
+    fs[i] = mix_site<SPT, XCD>(v, i);
     const int fc = fs[i] < 0 ? 0 : fs[i];  // past the last site: load site 0, store nothing
     NbrF n = neighbours_f(g, decode(g, fc));
     int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
-    smv[i] = *elem(a.sm32, fc);
+    smv[i] = *elem(a.sm32 + v.soff, fc);
 #pragma unroll
     for (int k = 0; k < 7; k++) {
       rv[i][k] = S::ld(r, q[k]);
@@ -561,7 +619,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Xe(MixFix a, int sl, const MixSt
     }
   }
   end_of_loads();
-  const double rho = block_sum_once(partial_finish(pin + (size_t)rhs * a.nb, a.nb, pl), lds);
+  const double rho = block_sum_once(partial_finish(pin + v.poff, v.nb, pl), lds);
   const double nr = S::kScaled ? sqrt(rho) / s.scale : sqrt(rho);  // the unscaled norm
   MixState* out = sout + rhs;
   if (!s.active) {
@@ -618,7 +676,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Xe(MixFix a, int sl, const MixSt
 #pragma unroll
     for (int i = 0; i < SPT; i++) {
       if (fs[i] < 0) continue;
-      a.u[off + fs[i]] = apply_Mdag(g, decode(g, fs[i]), a.sm[fs[i]], ldx);
+      a.u[off + fs[i]] = apply_Mdag(g, decode(g, fs[i]), a.sm[v.soff + fs[i]], ldx);
     }
     if (lead) {
       put_mstate(out, 1, s.iters, PH_UPDATE, nrel, 0, 0, s.rho, rmax, s);
@@ -649,22 +707,24 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Xe(MixFix a, int sl, const MixSt
     acc += dnorm2(tt);
   }
   acc = block_sum_once(acc, lds2);
-  if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + partial_slot(blockIdx.x, a.nb)] = acc;
+  if (threadIdx.x == 0) pout[v.poff + partial_slot(blockIdx.x, v.nb)] = acc;
   if (lead) {
     put_mstate(out, 1, s.iters, PH_NORMAL, nrel, 0, 0, rho, rmax, s);
     if (FLUSH) put_alphas(out, 0.0, 0.0);
   }
 }
 
-template <typename S, int SPT, bool XCD>
+template <typename S, int SPT, bool XCD, typename SV = MixOne>
 __global__ __launch_bounds__(MIX_BS) void k_mix_Ye(MixFix a, int sl, const MixState* __restrict__ sin,
                                                   MixState* __restrict__ sout, const double* __restrict__ pin,
                                                   double* __restrict__ pout) {
   __shared__ double lds[MIX_BS / 64], lds2[MIX_BS / 64];  // head and tail reductions
   using T = typename S::T;
-  const Geom g = a.g;
+  SV v;
+  if (!v.init(a)) return;
+  const Geom g = v.g;
   const int rhs = blockIdx.y;
-  const size_t off = (size_t)rhs * g.V;
+  const size_t off = v.off;
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   const T* t = (const T*)a.t + off;
   T* r = (T*)a.r + off;
@@ -673,24 +733,24 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Ye(MixFix a, int sl, const MixSt
   // ignores them), so that none waits for the state
   const MixState s = load_state(sin + rhs);
   double pl[PART_K];
-  partial_issue(pin + (size_t)rhs * a.nb, a.nb, pl);
+  partial_issue(pin + v.poff, v.nb, pl);
   end_of_loads();
   int fs[SPT];
   float smv[SPT];
   float2 tv[SPT][7], rv[SPT];
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
-    fs[i] = mix_site<SPT, XCD>(a, i);
+    fs[i] = mix_site<SPT, XCD>(v, i);
     const int fc = fs[i] < 0 ? 0 : fs[i];
     NbrF n = neighbours_f(g, decode(g, fc));
     int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
-    smv[i] = *elem(a.sm32, fc);
+    smv[i] = *elem(a.sm32 + v.soff, fc);
 #pragma unroll
     for (int k = 0; k < 7; k++) tv[i][k] = S::ld(t, q[k]);
     rv[i] = S::ld(r, fc);
   }
   end_of_loads();
-  const double gamma = block_sum_once(partial_finish(pin + (size_t)rhs * a.nb, a.nb, pl), lds);
+  const double gamma = block_sum_once(partial_finish(pin + v.poff, v.nb, pl), lds);
   MixState* out = sout + rhs;
   if (!s.active) {
     if (lead) put_mstate(out, 0, s.iters, s.phase, s.nrel, s.converged, s.early, s.rho, s.rmax, s);
@@ -719,13 +779,13 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Ye(MixFix a, int sl, const MixSt
       S::st(pnext, f, S::kScaled ? make_float2(fr * po.x, fr * po.y) : po, a.nt);
       a.x[off + f] = cadd(a.x[off + f], make_double2((double)c.x, (double)c.y));
       a.xs[off + f] = make_float2(0.f, 0.f);
-      double2 rr = apply_sub_M(g, decode(g, f), a.sm[f], a.b[off + f], [&](int q) { return u[q]; });
+      double2 rr = apply_sub_M(g, decode(g, f), a.sm[v.soff + f], a.b[off + f], [&](int q) { return u[q]; });
       if (S::kScaled) rr = make_double2(sn * rr.x, sn * rr.y);
       S::st(r, f, make_float2((float)rr.x, (float)rr.y), a.nt);
       acc += cnorm2(rr);
     }
     acc = block_sum_once(acc, lds2);
-    if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + partial_slot(blockIdx.x, a.nb)] = acc;
+    if (threadIdx.x == 0) pout[v.poff + partial_slot(blockIdx.x, v.nb)] = acc;
     if (lead) {
       put_mstate(out, 1, s.iters, PH_UPDATED, s.nrel, 0, 0, s.rho * ratio * ratio, s.rmax, s);
       put_alphas(out, 0.0, 0.0);
@@ -744,7 +804,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Ye(MixFix a, int sl, const MixSt
     acc += dnorm2(rn);
   }
   acc = block_sum_once(acc, lds2);
-  if (threadIdx.x == 0) pout[(size_t)rhs * a.nb + partial_slot(blockIdx.x, a.nb)] = acc;
+  if (threadIdx.x == 0) pout[v.poff + partial_slot(blockIdx.x, v.nb)] = acc;
   if (lead) {
     put_mstate(out, 1, s.iters + 1, PH_NORMAL, s.nrel, 0, 0, s.rho, s.rmax, s);
     // pending step length of the unscaled direction: alpha / s
@@ -1149,6 +1209,9 @@ static MixFix mix_fix(MixMachine* m, const Geom& g, int nrhs, int spt, int store
   a.res = m->res;
   a.nrhs = nrhs;
   a.nb = (g.V + spt * MIX_BS - 1) / (spt * MIX_BS);
+  a.ptotal = (long)nrhs * a.nb;
+  a.sys = nullptr;
+  a.caps = nullptr;
   // r, t, two p in the storage format and xs in FP32, per site and system
   const double ws = (double)g.V * nrhs * (4.0 * store_bytes + 8.0);
   a.nt = ws > 240.0 * (1 << 20) ? 1 : 0;
@@ -1326,5 +1389,194 @@ void cgmix_set_autotune(void** machine_slot, int on) {
 }
 
 long cgmix_tune_runs(void* machine) { return machine ? ((MixMachine*)machine)->eng.tune_runs : 0; }
+
+// ---- uneven batches ------------------------------------------------------------------
+// The loads-first kernels over the systems of an uneven solver (gnx_cgu.hip),
+// each on its own lattice: one site per thread in the slab mapping, the
+// partition and order of sums of the single-lattice shape 11
+// (GNX_MIX_SHAPE=11), so a system's solution is bit-identical to its own
+// gnx_cg solve in that shape with the same reliable-update factor.  The
+// systems' sigma masses, right-hand sides and (null start) x_in are gathered
+// into the machine's concatenated buffers by one kernel, the solutions
+// scattered back by another; the descriptors and partial offsets are the
+// FP64 uneven solver's.
+struct MixMulti {
+  GraphEngine eng;
+  double* sm = nullptr;
+  float* sm32 = nullptr;
+  double2 *b = nullptr, *x = nullptr, *u = nullptr;
+  float2* xs = nullptr;
+  void *r = nullptr, *p0 = nullptr, *p1 = nullptr, *t = nullptr;
+  CGDevResult* res = nullptr;
+  long* caps = nullptr;      // [n] iteration cap per system
+  long* hcaps = nullptr;     // pinned
+  CGUPtrs* dptr = nullptr;   // [n] the caller's vectors of the current solve
+  CGUPtrs* hptr = nullptr;   // pinned
+  ~MixMulti() {
+    void* dev[] = {sm, sm32, b, x, u, xs, r, p0, p1, t, res, caps, dptr};
+    for (void* p : dev)
+      if (p) (void)hipFree(p);
+    if (hcaps) (void)hipHostFree(hcaps);
+    if (hptr) (void)hipHostFree(hptr);
+  }
+};
+
+void cgmix_multi_release(void* m) { delete (MixMulti*)m; }
+
+__global__ __launch_bounds__(MIX_BS) void k_mixu_gather(MixFix a, const CGUPtrs* __restrict__ ptr, int with_x) {
+  const CGUSys d = a.sys[blockIdx.y];
+  const int f = (int)blockIdx.x * MIX_BS + (int)threadIdx.x;
+  if (f >= d.g.V) return;
+  const CGUPtrs p = ptr[blockIdx.y];
+  a.sm[d.soff + f] = p.sm[f];
+  a.b[d.voff + f] = p.b[f];
+  if (with_x) a.x[d.voff + f] = p.x[f];
+}
+
+__global__ __launch_bounds__(MIX_BS) void k_mixu_scatter(MixFix a, const CGUPtrs* __restrict__ ptr) {
+  const CGUSys d = a.sys[blockIdx.y];
+  const int f = (int)blockIdx.x * MIX_BS + (int)threadIdx.x;
+  if (f >= d.g.V) return;
+  ptr[blockIdx.y].x[f] = a.x[d.voff + f];
+}
+
+// buffers for the solver's systems (sized once: an uneven solver's systems never change)
+static hipError_t mixu_ensure(MixMulti* mm, const MultiView& mv) {
+  if (mm->res) return hipSuccess;
+  const size_t vt = (size_t)mv.vtotal, n = (size_t)mv.n;
+  hipError_t e = mm->eng.ensure(mv.n, 1, sizeof(MixState), 2 * (size_t)mv.ptotal);
+  if (e == hipSuccess) e = hipMalloc(&mm->sm, sizeof(double) * (size_t)mv.stotal);
+  if (e == hipSuccess) e = hipMalloc(&mm->sm32, sizeof(float) * (size_t)mv.stotal);
+  if (e == hipSuccess) e = hipMalloc(&mm->b, sizeof(double2) * vt);
+  if (e == hipSuccess) e = hipMalloc(&mm->x, sizeof(double2) * vt);
+  if (e == hipSuccess) e = hipMalloc(&mm->u, sizeof(double2) * vt);
+  if (e == hipSuccess) e = hipMalloc(&mm->xs, sizeof(float2) * vt);
+  if (e == hipSuccess) e = hipMalloc(&mm->r, sizeof(float2) * vt);
+  if (e == hipSuccess) e = hipMalloc(&mm->p0, sizeof(float2) * vt);
+  if (e == hipSuccess) e = hipMalloc(&mm->p1, sizeof(float2) * vt);
+  if (e == hipSuccess) e = hipMalloc(&mm->t, sizeof(float2) * vt);
+  if (e == hipSuccess) e = hipMalloc(&mm->caps, sizeof(long) * n);
+  if (e == hipSuccess) e = hipHostMalloc(&mm->hcaps, sizeof(long) * n, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc(&mm->dptr, sizeof(CGUPtrs) * n);
+  if (e == hipSuccess) e = hipHostMalloc(&mm->hptr, sizeof(CGUPtrs) * n, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc(&mm->res, sizeof(CGDevResult) * n);
+  return e;
+}
+
+static MixFix mixu_fix(MixMulti* mm, const MultiView& mv, int store_bytes) {
+  MixFix a;
+  a.g = mv.hsys[0].g;  // unused: every kernel of the uneven form reads its system's descriptor
+  a.sm = mm->sm;
+  a.sm32 = mm->sm32;
+  a.b = mm->b;
+  a.x = mm->x;
+  a.u = mm->u;
+  a.xs = mm->xs;
+  a.r = mm->r;
+  a.p0 = mm->p0;
+  a.p1 = mm->p1;
+  a.t = mm->t;
+  a.part = mm->eng.dev_part();
+  a.state = (MixState*)mm->eng.dev_state();
+  a.res = mm->res;
+  a.nrhs = mv.n;
+  a.nb = mv.nbmax;
+  a.ptotal = mv.ptotal;
+  a.sys = mv.dsys;
+  a.caps = mm->caps;
+  const double ws = (double)mv.vtotal * (4.0 * store_bytes + 8.0);
+  a.nt = ws > 240.0 * (1 << 20) ? 1 : 0;
+  return a;
+}
+
+template <typename S>
+static hipError_t mixu_run_t(MixMulti* mm, const MultiView& mv, const double* const* sm, const double* const* b,
+                             double* const* x, double tol, double delta, long max_iter, int start,
+                             CGDevResult* host_res, int tag) {
+  hipError_t e = mixu_ensure(mm, mv);
+  if (e != hipSuccess) return e;
+  const hipStream_t st = mv.stream;
+  const int n = mv.n;
+  // the pinned tables are rewritten here: no copy of an earlier call may still read them
+  e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return e;
+  long cap_max = 0;
+  for (int k = 0; k < n; k++) {
+    // the reference's cap V+1 (macro.h:134) of each system when max_iter <= 0
+    mm->hcaps[k] = max_iter > 0 ? max_iter : (long)mv.hsys[k].g.V + 1;
+    cap_max = mm->hcaps[k] > cap_max ? mm->hcaps[k] : cap_max;
+    mm->hptr[k] = CGUPtrs{sm[k], (const double2*)b[k], (double2*)x[k], nullptr};
+  }
+  e = hipMemcpyAsync(mm->caps, mm->hcaps, sizeof(long) * n, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(mm->dptr, mm->hptr, sizeof(CGUPtrs) * n, hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return e;
+  MixFix a = mixu_fix(mm, mv, (int)sizeof(typename S::T));
+  const dim3 grid((mv.nbmax + NUM_XCD - 1) / NUM_XCD * NUM_XCD, n), blk(MIX_BS);
+  const CGUPtrs* dp = mm->dptr;
+  int with_x = start == GNX_START_NULL ? 1 : 0;
+  void* ag[] = {(void*)&a, (void*)&dp, (void*)&with_x};
+  e = hipLaunchKernel((const void*)k_mixu_gather, grid, blk, ag, 0, st);
+  if (e != hipSuccess) return e;
+  // start kernels on the gathered buffers (the cap comes from caps)
+  const double* smv = mm->sm;
+  const double2* bv = mm->b;
+  const double2* xv = mm->x;
+  long unused_cap = 0;
+  if (start == GNX_START_NULL) {
+    void* a1[] = {(void*)&a, (void*)&smv, (void*)&bv, (void*)&xv};
+    e = hipLaunchKernel((const void*)k_mix_null_p1<1, true, MixUneven>, grid, blk, a1, 0, st);
+    void* a2[] = {(void*)&a};
+    if (e == hipSuccess) e = hipLaunchKernel((const void*)k_mix_null_p2<1, true, MixUneven>, grid, blk, a2, 0, st);
+    void* a3[] = {(void*)&a, (void*)&tol, (void*)&delta, (void*)&unused_cap};
+    if (e == hipSuccess)
+      e = hipLaunchKernel((const void*)k_mix_null_p3<S, 1, true, MixUneven>, grid, blk, a3, 0, st);
+  } else {
+    void* a0[] = {(void*)&a, (void*)&smv, (void*)&bv, (void*)&tol, (void*)&delta, (void*)&unused_cap};
+    e = hipLaunchKernel((const void*)k_mix_start_imposed<S, 1, true, MixUneven>, grid, blk, a0, 0, st);
+  }
+  if (e != hipSuccess) return e;
+  // the chunk graph: ENG_CHUNK slot pairs over all systems, as mix_graph
+  const MixState* s0 = a.state;
+  const MixState* s1 = a.state + n;
+  MixState* w0 = a.state;
+  MixState* w1 = a.state + n;
+  const double* q0 = a.part;
+  const double* q1 = a.part + a.ptotal;
+  double* o0 = a.part;
+  double* o1 = a.part + a.ptotal;
+  auto body = [&](hipStream_t cs) {
+    for (int i = 0; i < ENG_CHUNK; i++) {
+      int sl = i & 1;
+      void* argsX[] = {(void*)&a, (void*)&sl, (void*)&s0, (void*)&w1, (void*)&q1, (void*)&o0};
+      void* argsY[] = {(void*)&a, (void*)&sl, (void*)&s1, (void*)&w0, (void*)&q0, (void*)&o1};
+      (void)hipLaunchKernel((i & 1) ? (const void*)k_mix_Xe<S, 1, true, false, MixUneven>
+                                    : (const void*)k_mix_Xe<S, 1, true, true, MixUneven>,
+                            grid, blk, argsX, 0, cs);
+      (void)hipLaunchKernel((const void*)k_mix_Ye<S, 1, true, MixUneven>, grid, blk, argsY, 0, cs);
+    }
+  };
+  hipGraphExec_t exec;
+  e = mm->eng.graph(tag, n, body, &exec);
+  if (e != hipSuccess) return e;
+  // at most max_iter normal slot pairs plus one update pair after each
+  e = mm->eng.run(exec, n, 2 * cap_max + 4, st, nullptr);
+  if (e != hipSuccess) return e;
+  void* as[] = {(void*)&a, (void*)&dp};
+  e = hipLaunchKernel((const void*)k_mixu_scatter, grid, blk, as, 0, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(host_res, mm->res, sizeof(CGDevResult) * n, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  return e;
+}
+
+hipError_t cgmix_multi_run(const MultiView& mv, const double* const* sm, const double* const* b, double* const* x,
+                           double tol, double delta, long max_iter, int start, int prec, CGDevResult* host_res) {
+  if (!*mv.mixed) *mv.mixed = new MixMulti();
+  MixMulti* mm = (MixMulti*)*mv.mixed;
+  if (prec == GNX_PREC_F32)
+    return mixu_run_t<StoreF32>(mm, mv, sm, b, x, tol, delta, max_iter, start, host_res, 100);
+  if (prec == GNX_PREC_F16)
+    return mixu_run_t<StoreF16>(mm, mv, sm, b, x, tol, delta, max_iter, start, host_res, 102);
+  return mixu_run_t<StoreBF16>(mm, mv, sm, b, x, tol, delta, max_iter, start, host_res, 101);
+}
 
 }  // namespace gnx

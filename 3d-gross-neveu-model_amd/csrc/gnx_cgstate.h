@@ -37,6 +37,25 @@ struct CGMState {
   double hbe[CGD_M];
 };
 
+// one system of an uneven batch (gnx_cgu.hip; the mixed solver's uneven
+// form in gnx_cgmix.hip): its lattice, block count and offsets into the
+// concatenated buffers
+struct CGUSys {
+  Geom g;
+  int nb;     // its blocks (ceil(V / 256), one site per thread)
+  long voff;  // offset of its vectors, complex elements
+  long soff;  // offset of its sigma mass, doubles
+  long poff;  // offset of its partials within a slot
+};
+
+// the caller's vectors of one system of an uneven solve
+struct CGUPtrs {
+  const double* sm;
+  const double2* b;
+  double2* x;
+  const double2* U;  // links (link-coupled solves), or null
+};
+
 // field-wise state write (an aggregate copy makes the compiler stage the
 // struct through LDS)
 GNX_DEVONLY void put_state(CGMState* o, int active, int iters, int conv, int early, double rho,

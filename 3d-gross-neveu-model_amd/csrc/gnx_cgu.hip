@@ -37,15 +37,6 @@ constexpr int CGU_BS = ENG_BS;
 // 1900 RHS-solves/s, lockstep ensembles 81 against 78 trajectories/s).
 GNX_DEVONLY void st_cgu(double2* p, double2 v) { *p = v; }
 
-// one system of an uneven batch
-struct CGUSys {
-  Geom g;
-  int nb;     // its blocks (ceil(V / CGU_BS))
-  long voff;  // offset of its vectors, complex elements
-  long soff;  // offset of its sigma mass, doubles
-  long poff;  // offset of its partials within a slot
-};
-
 struct CGUFix {
   const CGUSys* sys;  // [n]
   double* sm;         // concatenated sigma masses
@@ -81,13 +72,6 @@ GNX_DEVONLY int cgu_site(const CGUSys& d) {
 // are gathered into the concatenated buffers, and the solution scattered
 // back, by one kernel each instead of one copy per system and vector (three
 // copies a system per solve, each its own dispatch).
-struct CGUPtrs {
-  const double* sm;
-  const double2* b;
-  double2* x;
-  const double2* U;  // links (link-coupled solves), or null
-};
-
 __global__ __launch_bounds__(CGU_BS) void k_cgu_gather(CGUFix a, const CGUPtrs* __restrict__ ptr, int with_x) {
   CGUSys d;
   if (!cgu_sys(a, &d)) return;
@@ -363,6 +347,7 @@ struct gnx_multi {
   double* sm = nullptr;
   double2 *b = nullptr, *x = nullptr, *r = nullptr, *p0 = nullptr, *p1 = nullptr, *t = nullptr;
   double2* U = nullptr;  // concatenated links, allocated by the first link-coupled solve
+  void* mixed = nullptr;  // the mixed-precision solver's buffers (gnx_cgmix.hip), on first use
   gnx::CGDevResult* res = nullptr;
   gnx::CGDevResult* hres = nullptr;  // pinned
 };
@@ -377,6 +362,20 @@ bool multi_system(const gnx_multi* m, int k, int* L, int* LT, int* device) {
   *device = m->device;
   return true;
 }
+bool multi_view(gnx_multi* m, MultiView* v) {
+  if (!m) return false;
+  v->n = m->n;
+  v->device = m->device;
+  v->stream = m->stream;
+  v->dsys = m->dsys;
+  v->hsys = m->sys.data();
+  v->vtotal = m->vtotal;
+  v->stotal = m->stotal;
+  v->ptotal = m->ptotal;
+  v->nbmax = m->nbmax;
+  v->mixed = &m->mixed;
+  return true;
+}
 }  // namespace gnx
 
 namespace gnx {
@@ -388,6 +387,8 @@ static void cgu_free(gnx_multi* m) {
   if (m->hmax_iter) (void)hipHostFree(m->hmax_iter);
   if (m->hptr) (void)hipHostFree(m->hptr);
   if (m->hres) (void)hipHostFree(m->hres);
+  if (m->mixed) cgmix_multi_release(m->mixed);
+  m->mixed = nullptr;
 }
 
 static CGUFix cgu_fix(gnx_multi* m) {
@@ -629,4 +630,46 @@ gnx_status gnx_multi_link_cg(gnx_multi* m, const double* const* sm, const double
   return gnx::multi_cg(m, sm, U, b, x, tol, max_iter, start, res);
 }
 
+gnx_status gnx_multi_cg_prec(gnx_multi* m, const double* const* sm, const double* const* b, double* const* x,
+                             double tol, long max_iter, gnx_cg_start start, gnx_prec prec, gnx_cg_result* res) {
+  // the reliable-update factor of a context that was never set (gnx_set_reliable_delta)
+  return gnx::multi_cg_mixed(m, sm, b, x, tol, max_iter, start, (int)prec, prec == GNX_PREC_BF16 ? 0.2 : 0.05, res);
+}
+
 }  // extern "C"
+
+gnx_status gnx::multi_cg_mixed(gnx_multi* m, const double* const* sm, const double* const* b, double* const* x,
+                               double tol, long max_iter, int start, int prec, double delta, gnx_cg_result* res) {
+  using namespace gnx;
+  CHECK_ARG(prec == GNX_PREC_F64 || prec == GNX_PREC_F32 || prec == GNX_PREC_BF16 || prec == GNX_PREC_F16,
+            "gnx_multi_cg_prec: bad precision");
+  if (prec == GNX_PREC_F64) return multi_cg(m, sm, nullptr, b, x, tol, max_iter, (gnx_cg_start)start, res);
+  CHECK_ARG(delta > 0.0 && delta < 1.0, "gnx_multi_cg_prec: the reliable-update factor must be in (0, 1)");
+  CHECK_ARG(m && sm && b && x, "gnx_multi_cg_prec: null argument");
+  CHECK_ARG(tol >= 0.0, "gnx_multi_cg_prec: negative tolerance");
+  CHECK_ARG(start == GNX_START_NULL || start == GNX_START_IMPOSED, "gnx_multi_cg_prec: bad start");
+  for (int k = 0; k < m->n; k++) {
+    CHECK_ARG(sm[k] && b[k] && x[k], "gnx_multi_cg_prec: null vector");
+    CHECK_ARG((const void*)b[k] != (const void*)x[k], "gnx_multi_cg_prec: b and x alias");
+  }
+  HIP_TRY(hipSetDevice(m->device));
+  MultiView mv;
+  multi_view(m, &mv);
+  HIP_TRY(cgmix_multi_run(mv, sm, b, x, tol, delta, max_iter, start, prec, m->hres));
+  int all = 1;
+  for (int k = 0; k < m->n; k++) {
+    if (res) {
+      res[k].converged = m->hres[k].converged;
+      res[k].early_exit = m->hres[k].early_exit;
+      res[k].iterations = m->hres[k].iterations;
+      res[k].reliable_updates = m->hres[k].reliable_updates;
+      res[k].residual = m->hres[k].residual;
+    }
+    all &= m->hres[k].converged;
+  }
+  if (!all) {
+    set_err("gnx_multi_cg_prec: a system reached its iteration cap");
+    return GNX_ERR_NOCONV;
+  }
+  return GNX_OK;
+}

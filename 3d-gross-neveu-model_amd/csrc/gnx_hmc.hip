@@ -384,7 +384,10 @@ gnx_status gnx_hmc_accept(gnx_hmc* h, int accepted) {
 // n chains in lockstep (moleculardynamics.c:1234-1434 for each): the steps
 // of gnx_hmc_trajectory, chain by chain on each chain's stream, except that
 // every solve point of the integrator is one uneven-batch solve over all
-// live chains (systems 2k, 2k+1 of m: chain k's two flavours).  A chain
+// live chains (systems 2k, 2k+1 of m: chain k's two flavours).  The
+// MEDIUM-tolerance force solves run in the chains' force precision
+// (gnx_hmc_set_force_precision, the same on every chain; mixed precision
+// with the first chain's reliable-update factor), the rest in FP64.  A chain
 // whose solve fails leaves the lockstep as gnx_hmc_trajectory would (sigma
 // restored, ok = 0); the others go on.
 gnx_status gnx_hmc_ensemble_trajectory(gnx_multi* m, int n, gnx_hmc* const* hs, const double* const* mom_host,
@@ -397,6 +400,8 @@ gnx_status gnx_hmc_ensemble_trajectory(gnx_multi* m, int n, gnx_hmc* const* hs, 
   CHECK_ARG(multi_count(m) == 2 * n, "gnx_hmc_ensemble_trajectory: the solver must hold two systems per chain");
   for (int k = 0; k < n; k++) {
     CHECK_ARG(hs[k] && mom_host[k] && chi0_host[k] && chi1_host[k], "gnx_hmc_ensemble_trajectory: null argument");
+    CHECK_ARG(hs[k]->force_prec == hs[0]->force_prec,
+              "gnx_hmc_ensemble_trajectory: the chains share their solves, so all need the same force precision");
     for (int f = 0; f < 2; f++) {
       int L = 0, LT = 0, dev = -1;
       multi_system(m, 2 * k + f, &L, &LT, &dev);
@@ -427,8 +432,10 @@ gnx_status gnx_hmc_ensemble_trajectory(gnx_multi* m, int n, gnx_hmc* const* hs, 
     HIP_TRY(hipMemcpyAsync(h->sigma_copy, h->sigma, sizeof(double) * V, hipMemcpyDeviceToDevice, c->stream));
   }
   // a failed solve: the chain leaves as gnx_hmc_trajectory's TRY_CG does
-  auto solve = [&](double tol, int start) -> gnx_status {
-    gnx_status q = ens_solve(m, n, hs, live, tol, max_iter, start, ok);
+  const int fprec = hs[0]->force_prec;
+  const double fdelta = fprec == GNX_PREC_F64 ? 0.0 : reliable_delta_of(hs[0]->c, fprec);
+  auto solve = [&](double tol, int start, int prec = GNX_PREC_F64) -> gnx_status {
+    gnx_status q = ens_solve(m, n, hs, live, tol, max_iter, start, ok, nullptr, prec, fdelta);
     if (q != GNX_OK) return q;
     for (int k = 0; k < n; k++)
       if (live[k] && !ok[k]) {
@@ -438,8 +445,8 @@ gnx_status gnx_hmc_ensemble_trajectory(gnx_multi* m, int n, gnx_hmc* const* hs, 
       }
     return GNX_OK;
   };
-  auto force = [&](double tol) -> gnx_status {
-    gnx_status q = solve(tol, GNX_START_NULL);
+  auto force = [&](double tol, int prec = GNX_PREC_F64) -> gnx_status {
+    gnx_status q = solve(tol, GNX_START_NULL, prec);
     for (int k = 0; k < n && q == GNX_OK; k++)
       if (live[k]) q = hmc_force_post(hs[k]);
     return q;
@@ -460,19 +467,19 @@ gnx_status gnx_hmc_ensemble_trajectory(gnx_multi* m, int n, gnx_hmc* const* hs, 
   ENS_TRY(solve(tol_high, GNX_START_IMPOSED));
   for (int k = 0; k < n; k++)
     if (live[k]) ENS_TRY(hmc_energy(hs[k], &H0[k]));
-  ENS_TRY(force(tol_medium));
+  ENS_TRY(force(tol_medium, fprec));
   ENS_TRY(pq(lam * epsilon, epsilon / 2.0));
-  ENS_TRY(force(tol_medium));
+  ENS_TRY(force(tol_medium, fprec));
   ENS_TRY(pq((1.0 - (2.0 * lam)) * epsilon, epsilon / 2.0));
   for (int st = 1; st < step - 1; st++) {
-    ENS_TRY(force(tol_medium));
+    ENS_TRY(force(tol_medium, fprec));
     ENS_TRY(pq(2.0 * lam * epsilon, epsilon / 2.0));
-    ENS_TRY(force(tol_medium));
+    ENS_TRY(force(tol_medium, fprec));
     ENS_TRY(pq((1.0 - (2.0 * lam)) * epsilon, epsilon / 2.0));
   }
-  ENS_TRY(force(tol_medium));
+  ENS_TRY(force(tol_medium, fprec));
   ENS_TRY(pq(2.0 * lam * epsilon, epsilon / 2.0));
-  ENS_TRY(force(tol_medium));
+  ENS_TRY(force(tol_medium, fprec));
   ENS_TRY(pq((1.0 - (2.0 * lam)) * epsilon, epsilon / 2.0));
   ENS_TRY(force(tol_high));
   ENS_TRY(pq(lam * epsilon, 0.0));

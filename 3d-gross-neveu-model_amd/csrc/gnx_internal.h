@@ -56,6 +56,30 @@ int cgmix_shape_of(void* machine, int nrhs, int prec, int* spt, int* xcd, int* p
 // uneven-batch solver (gnx_cgu.hip): system count and the lattice of system k
 int multi_count(const gnx_multi* m);
 bool multi_system(const gnx_multi* m, int k, int* L, int* LT, int* device);
+
+// what the mixed-precision form of an uneven solve needs of the solver: its
+// systems' descriptors (struct CGUSys, gnx_cgstate.h) on the device and the
+// host, the concatenated sizes, and the slot of its own buffers
+struct CGUSys;
+struct MultiView {
+  int n = 0, device = 0;
+  hipStream_t stream = nullptr;
+  const CGUSys* dsys = nullptr;
+  const CGUSys* hsys = nullptr;
+  long vtotal = 0, stotal = 0, ptotal = 0;
+  int nbmax = 0;
+  void** mixed = nullptr;
+};
+bool multi_view(gnx_multi* m, MultiView* v);
+// an uneven solve with r, p, t in reduced-precision storage and FP64
+// reliable updates (gnx_cgmix.hip); the caller's sm[k], b[k], x[k]
+hipError_t cgmix_multi_run(const MultiView& mv, const double* const* sm, const double* const* b, double* const* x,
+                           double tol, double delta, long max_iter, int start, int prec, CGDevResult* host_res);
+void cgmix_multi_release(void* machine);
+// gnx_multi_cg_prec with the reliable-update factor given (the ensembles
+// pass their chains' own)
+gnx_status multi_cg_mixed(gnx_multi* m, const double* const* sm, const double* const* b, double* const* x,
+                          double tol, long max_iter, int start, int prec, double delta, gnx_cg_result* res);
 int cgm_shape_of(void* machine, int nrhs, int* spt, int* xcd);
 void cgm_set_autotune(void** machine_slot, int on);
 // the allocation-free FP64 solve (one system): bytes of the static array it
@@ -129,9 +153,12 @@ namespace gnx {
 // follow the single-chain solve (hmc_cg, lh_cg).  H: gnx_hmc or
 // gnx_link_hmc (fields c, sm, phi, X, iterations, calls).  links: chain k's
 // links (3 per site) for solves with the link-coupled operator, or null.
+// prec: the solves' arithmetic (mixed precision with the reliable-update
+// factor delta; FP64 by default).
 template <typename H>
 inline gnx_status ens_solve(gnx_multi* m, int n, H* const* hs, const std::vector<char>& live, double tol,
-                     long max_iter, int start, std::vector<char>& ok, const double2* const* links = nullptr) {
+                     long max_iter, int start, std::vector<char>& ok, const double2* const* links = nullptr,
+                     int prec = GNX_PREC_F64, double delta = 0.0) {
   std::vector<const double*> sms(2 * n), bs(2 * n), us(2 * n);
   std::vector<double*> xs(2 * n);
   for (int k = 0; k < n; k++) {
@@ -154,7 +181,9 @@ inline gnx_status ens_solve(gnx_multi* m, int n, H* const* hs, const std::vector
   const gnx_status s =
       links ? gnx_multi_link_cg(m, sms.data(), us.data(), bs.data(), xs.data(), tol, max_iter, (gnx_cg_start)start,
                                 r.data())
-            : gnx_multi_cg(m, sms.data(), bs.data(), xs.data(), tol, max_iter, (gnx_cg_start)start, r.data());
+      : prec != GNX_PREC_F64
+          ? multi_cg_mixed(m, sms.data(), bs.data(), xs.data(), tol, max_iter, start, prec, delta, r.data())
+          : gnx_multi_cg(m, sms.data(), bs.data(), xs.data(), tol, max_iter, (gnx_cg_start)start, r.data());
   if (s != GNX_OK && s != GNX_ERR_NOCONV) return s;
   for (int k = 0; k < n; k++) {
     ok[k] = 0;

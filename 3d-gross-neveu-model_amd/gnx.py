@@ -139,6 +139,8 @@ def lib():
                                    C.c_int, C.POINTER(CGResult)]
         L.gnx_multi_link_cg.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.c_double,
                                         C.c_long, C.c_int, C.POINTER(CGResult)]
+        L.gnx_multi_cg_prec.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.c_double, C.c_long,
+                                        C.c_int, C.c_int, C.POINTER(CGResult)]
         L.gnx_hmc_ensemble_trajectory.argtypes = [vp, C.c_int, C.POINTER(vp), C.POINTER(dp), C.POINTER(dp),
                                                   C.POINTER(dp), C.c_double, C.c_int, C.c_double, C.c_double,
                                                   C.c_long, C.POINTER(TrajResult)]
@@ -539,10 +541,11 @@ class Multi:
         except Exception:
             pass
 
-    def cg(self, sms, bs, xs, tol, max_iter=0, start=START_NULL, links=None):
+    def cg(self, sms, bs, xs, tol, max_iter=0, start=START_NULL, links=None, prec=PREC_F64):
         """sms[k] (V_k), bs[k], xs[k] (2 V_k) device tensors; links[k] (6 V_k
         doubles, f-order, three complex per site) selects the link-coupled
-        operator; returns (status, [CGResult])."""
+        operator, prec a mixed-precision solve (live operator only); returns
+        (status, [CGResult])."""
         vp = C.c_void_p
         n = self.n
         assert len(sms) == len(bs) == len(xs) == n
@@ -554,9 +557,12 @@ class Multi:
         a_sm = (vp * n)(*[_ptr(t) for t in sms])
         a_b = (vp * n)(*[_ptr(as_real(t)) for t in bs])
         a_x = (vp * n)(*[_ptr(as_real(t)) for t in xs])
-        if links is None:
+        if links is None and prec != PREC_F64:
+            st = lib().gnx_multi_cg_prec(self.h, a_sm, a_b, a_x, tol, max_iter, start, prec, res)
+        elif links is None:
             st = lib().gnx_multi_cg(self.h, a_sm, a_b, a_x, tol, max_iter, start, res)
         else:
+            assert prec == PREC_F64, "the link-coupled uneven solve is FP64" 
             st = lib().gnx_multi_link_cg(self.h, a_sm, (vp * n)(*[_ptr(as_real(t)) for t in links]), a_b, a_x, tol,
                                          max_iter, start, res)
         if st not in (0, 3):

@@ -544,7 +544,30 @@ def ensemble_leg(gnx, sizes, ntraj, betas=(0.6, 0.75, 0.85, 1.0), eps=0.02, step
                                                     for c, d in zip(chains, lock))}
     es.close()
     el.close()
-    for c in chains + lock:
+
+    # the same chains and draws in lockstep, the MEDIUM-tolerance sigma force
+    # solves in mixed precision (FP16 storage, FP64 reliable updates) in the
+    # uneven-batch launch
+    lm = make_chains(gnx.PREC_F16)
+    es = gnx.Ensemble([c["h"] for c in lm])
+    el = gnx.Ensemble([c["lh"] for c in lm])
+    run_lockstep(1, lm, es, el)
+    for c in lm:
+        c["dH_sigma"].clear(), c["dH_link"].clear()
+        c["acc_sigma"] = c["acc_link"] = c["cg_iterations"] = 0
+    t0 = time.perf_counter()
+    run_lockstep(ntraj, lm, es, el)
+    dtm2 = time.perf_counter() - t0
+    out["lockstep_f16_force_solves"] = {
+        "mode": "as lockstep, the MEDIUM-tolerance (1e-3) sigma force solves in FP16 storage with FP64 reliable "
+                "updates (one uneven-batch mixed launch per solve point); energies and the last force in FP64",
+        "trajectories_per_s": len(lm) * ntraj / dtm2, "seconds": dtm2, "speedup_vs_fp64_lockstep": dtl / dtm2,
+        "max_abs_dH_sigma_diff_vs_fp64": max(abs(a - b) for c, d in zip(lock, lm)
+                                             for a, b in zip(c["dH_sigma"], d["dH_sigma"])),
+        "same_accept_decisions": all(c["acc_sigma"] == d["acc_sigma"] for c, d in zip(lock, lm))}
+    es.close()
+    el.close()
+    for c in chains + lock + lm:
         c["lh"].close()
         c["h"].close()
 

@@ -161,6 +161,14 @@ gnx_status gnx_multi_cg(gnx_multi *m, const double *const *sm, const double *con
  * solve under the same condition. */
 gnx_status gnx_multi_link_cg(gnx_multi *m, const double *const *sm, const double *const *U, const double *const *b,
                              double *const *x, double tol, long max_iter, gnx_cg_start start, gnx_cg_result *res);
+/* gnx_multi_cg in another arithmetic: GNX_PREC_F64 is gnx_multi_cg; F32,
+ * BF16 and F16 keep r, p, t in that storage with FP64 reliable updates, as
+ * gnx_cg's mixed precision (the default reliable-update factors), in one
+ * launch per kernel over all systems.  Each system's solution and counters
+ * are bit-identical to its gnx_cg solve with GNX_MIX_SHAPE=11 (loads-first
+ * kernels, one site per thread, XCD slab mapping). */
+gnx_status gnx_multi_cg_prec(gnx_multi *m, const double *const *sm, const double *const *b, double *const *x,
+                             double tol, long max_iter, gnx_cg_start start, gnx_prec prec, gnx_cg_result *res);
 
 /* A12, batched.  Stochastic trace term of the condensate for nsrc noise
  * vectors at once (condensate.c:127-145 is this for one vector at a time):

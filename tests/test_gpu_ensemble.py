@@ -37,15 +37,24 @@ def _same(a, b, what):
                                                       (b.dH, b.H0, b.ok, b.cg_iterations))
 
 
-def test_sigma_ensemble_matches_single_chains(P, gnx_mod, monkeypatch):
-    banner("sigma ensemble (4, 8, 6, 10)^3 vs single chains")
+@pytest.mark.parametrize("fprec", [None, "f32", "f16"])
+def test_sigma_ensemble_matches_single_chains(P, gnx_mod, fprec, monkeypatch):
+    """FP64 solves, and the MEDIUM-tolerance force solves in mixed precision
+    (gnx_hmc_set_force_precision on every chain: the lockstep's uneven mixed
+    solve against each chain's own mixed batch of its two flavours, both in
+    the loads-first one-site-per-thread shape, GNX_MIX_SHAPE=11)."""
+    banner(f"sigma ensemble (4, 8, 6, 10)^3 vs single chains, force solves {fprec or 'f64'}")
     monkeypatch.setenv("GNX_CG_SHAPE", "1")
+    monkeypatch.setenv("GNX_MIX_SHAPE", "11")
+    codes = {"f32": gnx_mod.PREC_F32, "f16": gnx_mod.PREC_F16}
     single, ens = [], []
     for i, (L, bs) in enumerate(CHAINS):
         sig = _sigma(P, L, bs, 900 + i)
         for lst in (single, ens):
             h = gnx_mod.HMC(gnx_mod.Context(L), beta_s=bs)
             h.set_sigma(sig)
+            if fprec is not None:
+                h.set_force_precision(codes[fprec])
             lst.append(h)
     E = gnx_mod.Ensemble(ens)
     for t in range(2):

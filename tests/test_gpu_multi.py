@@ -143,3 +143,37 @@ def test_multi_link_matches_single_and_oracle(P, gnx_mod, start, monkeypatch):
                     1e-10, max_iter=20000, start=code)
     assert st == 0 and all(r.converged for r in res2)
     m.close()
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16", "f16"])
+@pytest.mark.parametrize("start", ["imposed", "null"])
+def test_multi_mixed_matches_single_and_oracle(P, gnx_mod, prec, start, monkeypatch):
+    """Mixed precision over an uneven batch (gnx_multi_cg_prec): every
+    system bit-identical, with the same iterations and reliable updates, to
+    its own gnx_cg solve in that storage with GNX_MIX_SHAPE=11 (loads-first
+    kernels, one site per thread, slab mapping); against the oracle's FP64
+    solve: solution within 1e-7 and the true residual (the oracle's
+    operator) below 2 tol."""
+    banner(f"uneven batch, mixed {prec} {SIZES} {start}")
+    monkeypatch.setenv("GNX_MIX_SHAPE", "11")
+    code = {"f32": gnx_mod.PREC_F32, "bf16": gnx_mod.PREC_BF16, "f16": gnx_mod.PREC_F16}[prec]
+    scode = gnx_mod.START_IMPOSED if start == "imposed" else gnx_mod.START_NULL
+    sy = _systems(P, 780)
+    m = gnx_mod.Multi(SIZES)
+    xs = [dev(x0) for _, _, _, x0 in sy]
+    st, res = m.cg([dev(sm) for _, sm, _, _ in sy], [dev(b) for _, _, b, _ in sy], xs, 1e-10, start=scode,
+                   prec=code)
+    assert st == 0 and all(r.converged for r in res)
+    for k, (lat, sm, b, x0) in enumerate(sy):
+        ctx = gnx_mod.Context(lat.L, lat.LT)
+        x1 = dev(x0)
+        s1, r1 = ctx.cg(dev(sm), dev(b), x1, 1e-10, start=scode, prec=code)
+        assert s1 == 0
+        assert (r1.iterations, r1.reliable_updates, r1.early_exit) == \
+            (res[k].iterations, res[k].reliable_updates, res[k].early_exit), k
+        assert np.array_equal(host(xs[k]), host(x1)), k
+        xr, ok, _ = lat.cg(sm, b, x0.copy(), tol=1e-10, start=start)
+        xk = host(xs[k])
+        assert ok and rel(xk, xr) < 1e-7, k
+        assert np.linalg.norm(b - lat.M(sm, lat.Mdag(sm, xk))) < 2e-10, k
+    m.close()
