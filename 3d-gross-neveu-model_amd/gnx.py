@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libgnx.so")
 OP_M, OP_MDAG, OP_MMDAG, OP_MDAGM, OP_SUB_M, OP_SUB_MDAG = range(6)
 START_NULL, START_IMPOSED = 0, 1
 PREC_F64, PREC_F32, PREC_BF16, PREC_F16 = 0, 1, 2, 3
-ABI_VERSION = 5  # GNX_ABI_VERSION of include/gnx.h this binding follows
+ABI_VERSION = 6  # GNX_ABI_VERSION of include/gnx.h this binding follows
 
 STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_NOCONV", 4: "ERR_SIZE", 5: "ERR_STATE"}
 

@@ -31,8 +31,10 @@ extern "C" {
 /* 4: gnx_cg_shape reports the one-reduction kernels in bit 1 of xcd_mapped
  * 5: launch shapes by a fixed rule unless autotuning is on (gnx_set_autotune,
  *    gnx_tune_runs); geometries outside the exact site decode are refused;
- *    bit 2 of gnx_cg_shape's xcd_mapped marks the deferred-update kernels */
-#define GNX_ABI_VERSION 5
+ *    bit 2 of gnx_cg_shape's xcd_mapped marks the deferred-update kernels
+ * 6: bit 3 of xcd_mapped marks the fused kernels (one per iteration);
+ *    gnx_multi_link_cg, gnx_multi_cg_prec; link-coupled lockstep chains */
+#define GNX_ABI_VERSION 6
 
 typedef struct gnx_ctx gnx_ctx;
 typedef struct gnx_multi gnx_multi;
