@@ -107,22 +107,36 @@ def summarize_rocprof(kt_dir, fetch_dir, write_dir, L, untraced=None):
         out.append(f"| `{short(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']):.0f} | {r['MinNs']} | "
                    f"{r['MaxNs']} | {float(r['Percentage']):.2f} |")
     avg = {short(r["Name"]): float(r["AverageNs"]) / 1e3 for r in stats}
+    # the probes also run at other lattice sizes (the stream_152B_<n>^3
+    # nodes): their averages are taken over the dispatches at L^3 only
+    probe_avg = {}
+    tp = os.path.join(kt_dir, "run_kernel_trace.csv")
+    if os.path.exists(tp):
+        acc = {}
+        for r in csv.DictReader(open(tp)):
+            n = short(r["Kernel_Name"])
+            if n in ("k_empty", "k_stream40", "k_stream152"):
+                want = 1024 * 256 if n == "k_empty" else (L ** 3 + 255) // 256 * 256
+                if int(r["Grid_Size_X"]) == want:
+                    acc.setdefault(n, []).append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+        probe_avg = {n: statistics.mean(v) / 1e3 for n, v in acc.items()}
     if untraced:
         u = json.load(open(untraced))
         nodes = u["probes"]["graph_nodes"]
         probe_kernel = {"empty": "k_empty", "stream_40B": "k_stream40", "stream_152B": "k_stream152"}
         out.append("")
         out.append("Tracer inflation, from the graph-node probes (64 nodes per graph, each node reading the "
-                   "previous node's output; traced average duration against the unprofiled time per node of "
-                   f"the same build's bench line, `{os.path.basename(untraced)}`):")
+                   f"previous node's output; traced average duration of the {L}^3 dispatches against the "
+                   f"unprofiled time per node of the same build's bench line, `{os.path.basename(untraced)}`):")
         out.append("")
         out.append("| Probe kernel | Traced (us) | Unprofiled per node (us) | Inflation (us) |")
         out.append("|---|---:|---:|---:|")
         infl = {}
         for name, k in probe_kernel.items():
-            if k in avg and name in nodes:
-                infl[name] = avg[k] - nodes[name]["us_per_node"]
-                out.append(f"| `{k}` | {avg[k]:.2f} | {nodes[name]['us_per_node']:.2f} | {infl[name]:+.2f} |")
+            t = probe_avg.get(k, avg.get(k))
+            if t is not None and name in nodes:
+                infl[name] = t - nodes[name]["us_per_node"]
+                out.append(f"| `{k}` | {t:.2f} | {nodes[name]['us_per_node']:.2f} | {infl[name]:+.2f} |")
         cg = [k for k in avg if k.startswith(CG_KERNELS)]
         xk = [k for k in cg if k.startswith(("k_cg1_X", "k_cgd_X", "k_cgm_A"))]
         yk = [k for k in cg if k.startswith(("k_cg1_Y", "k_cgd_Y", "k_cgm_B"))]
