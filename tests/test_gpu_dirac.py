@@ -124,3 +124,42 @@ def test_dirac_batch_matches_single(torch_dev, P, gnx_mod, L, LT):
             seg = outh[2 * V * k:2 * V * (k + 1)]
             assert np.array_equal(seg, host(o1)), (op, k)
             assert np.array_equal(seg, ref(vs[k], ts[k])), (op, k)
+
+
+@pytest.mark.parametrize("L,LT", [(256, 1024), (382, 382)])
+def test_ops_at_the_largest_volumes(torch_dev, P, gnx_mod, L, LT):
+    """The largest lattices the library accepts: 256^2 x 1024 is the 2^26-site
+    cap (powers of two), 382^3 (55.7 M sites) the largest cube whose in-kernel
+    decode (umulhi by ceil(2^32/L)) is still exact; there the vectors' 32-bit
+    byte offsets and the decode are closest to their limits.  sigma mass, M
+    and M^dag bit-exact against the oracle; at the cap also three FP64 CG
+    iterations (iteration cap 3, the rule's launch shape) against the
+    oracle's three, within 1e-12 relative."""
+    banner(f"operators at {L}x{L}x{LT}")
+    import torch
+    lat = P.Lattice(L, LT)
+    sigma, b, _ = _fields(P, lat, 77)
+    sm_ref = lat.sigma_mass(sigma)
+    ctx = gnx_mod.Context(L, LT)
+    sm = ctx.field()
+    ctx.sigma_mass(dev(sigma), sm)
+    ctx.synchronize()
+    assert np.array_equal(host(sm), sm_ref)
+    bd = dev(b)
+    out = ctx.vec()
+    ctx.dirac(gnx_mod.OP_M, sm, bd, out)
+    ctx.synchronize()
+    assert np.array_equal(host(out), lat.M(sm_ref, b))
+    ctx.dirac(gnx_mod.OP_MDAG, sm, bd, out)
+    ctx.synchronize()
+    assert np.array_equal(host(out), lat.Mdag(sm_ref, b))
+    if lat.V == 1 << 26:
+        x = ctx.vec()
+        st, res = ctx.cg(sm, bd, x, 1e-10, max_iter=3, start=gnx_mod.START_IMPOSED, check=False)
+        assert st == 3 and res.iterations == 3 and res.converged == 0
+        xo, ok, _ = lat.cg(sm_ref, b, None, tol=1e-10, max_iter=3, start="imposed")
+        xg = host(x)
+        assert np.linalg.norm(xg - xo) <= 1e-12 * np.linalg.norm(xo)
+    del out, bd, sm
+    ctx.close()
+    torch.cuda.empty_cache()
