@@ -1209,15 +1209,28 @@ static hipError_t cgm_begin(CGMachine* m, const Geom& g, const double* sm, const
 
 // Launch shape of a solve.  By default a fixed rule, so that a solve's
 // iterates (which depend on the form and on the partial-sum partition) are
-// the same in every process: the one-reduction kernels (with deferred x and
-// p updates once the batch holds 400 000 sites) in the XCD slab mapping with
-// the most sites per thread (4, 2, 1) that still gives the batch 512 blocks
-// of at least 8 per system (the sites per thread the timing tuner picks at
-// 64^3 single and batched).  GNX_CG_SHAPE pins a shape; autotune
+// the same in every process: on the live operator, the fused form (one
+// kernel per iteration, 4 x 8 x 16 tiles) where the batch's tiles occupy
+// 160 to 256 CUs one each; otherwise the one-reduction kernels (with
+// deferred x and p updates once the batch holds 400 000 sites) in the XCD
+// slab mapping with the most sites per thread (4, 2, 1) that still gives the
+// batch 512 blocks of at least 8 per system (the sites per thread the timing
+// tuner picks at 64^3 single and batched).  GNX_CG_SHAPE pins a shape; autotune
 // (GNX_AUTOTUNE=1 or gnx_set_autotune) times every shape on 2*CHUNK
 // iterations of the caller's own system (imposed start into the solver's
 // buffers; the caller's x is not touched) and keeps the fastest per key.
-static int cgm_rule(const Geom& g, int nrhs) {
+static int cgm_rule(const Geom& g, int nrhs, bool link = false) {
+  // the fused form saves a kernel boundary per iteration but recomputes each
+  // tile's halo: with one tile per CU the halo hides behind the boundary
+  // (48^3 single 7.1 against 8.7 us per iteration; 32^3 x 3 and x 4 7.7 and
+  // 7.9 against 8.5 and 9.1), with fewer tiles it ties (32^3 x 1 and x 2),
+  // with two or more per CU the halo work loses (64^3 single 14.4 against
+  // 11.0, 48^3 x 2 12.4 against 11.3)
+  constexpr int kFused = 18;
+  if (!link && shape_fits(g, kFused)) {
+    const long tiles = (long)shape_blocks(g, kFused) * nrhs;
+    if (tiles >= 160 && tiles <= 256) return kFused;
+  }
   // the deferred form pays where the iteration is bound by its bytes (64^3
   // x 2 and larger: -12 % per iteration at x 2, -16 % at x 16, -4 % at 80^3
   // single), not where it is bound by latency (+9 % at 64^3 single, +2 % at
@@ -1243,7 +1256,7 @@ static hipError_t cgm_choose(CGMachine* m, const Geom& g, const double* sm, cons
     *best = cached;
     return hipSuccess;
   }
-  int best_shape = first + cgm_rule(g, nrhs);
+  int best_shape = first + cgm_rule(g, nrhs, link);
   hipError_t e = hipSuccess;
   if (m->forced_shape >= 0 && shape_fits(g, first + m->forced_shape)) {
     best_shape = first + m->forced_shape;
@@ -1318,29 +1331,43 @@ static std::mutex g_scratch_mu;
 
 static size_t up256(size_t n) { return (n + 255) & ~(size_t)255; }
 
+// The shape of the static solve: the rule's for one system, the deferred
+// form replaced by its equivalent without the residual ring (the same
+// iterates); the fused form (not deferred) keeps two residual slots.
+static int static_shape(const Geom& g) {
+  const int rule = cgm_rule(g, 1);
+  if (kShapes[rule].fused) return rule;
+  return rule >= 12 ? rule - 6 : rule;
+}
+
 // byte offsets of the solver's buffers in the array for one system of g
+// (the fused form's second residual, q and s buffers only for it)
 struct StaticLayout {
-  size_t sm, x, r, p0, p1, t, part, state, res, total;
+  size_t sm, x, r, p0, p1, t, t1, s1, part, state, res, total;
+  size_t vec;
 };
-static StaticLayout static_layout(const Geom& g, int nb) {
+static StaticLayout static_layout(const Geom& g, int nb, bool fused) {
   StaticLayout l;
   const size_t vec = up256(sizeof(double2) * (size_t)g.V);
+  l.vec = vec;
   l.sm = 0;
   l.x = l.sm + up256(sizeof(double) * (size_t)g.V);
-  l.r = l.x + vec;
-  l.p0 = l.r + vec;
+  l.r = l.x + vec;  // and, fused, the second residual slot right after it
+  l.p0 = l.r + (fused ? 2 : 1) * vec;
   l.p1 = l.p0 + vec;
   l.t = l.p1 + vec;
-  l.part = l.t + vec;
-  l.state = l.part + up256(sizeof(double) * 2 * (size_t)nb);
+  l.t1 = l.t + vec;
+  l.s1 = l.t1 + (fused ? vec : 0);
+  l.part = l.s1 + (fused ? vec : 0);
+  l.state = l.part + up256(sizeof(double) * (fused ? 4 : 2) * (size_t)nb);
   l.res = l.state + up256(sizeof(CGMState) * 2);
   l.total = l.res + up256(sizeof(CGDevResult));
   return l;
 }
 
 size_t cgm_static_bytes(const Geom& g) {
-  const int rule = cgm_rule(g, 1);
-  return static_layout(g, shape_blocks(g, rule >= 12 ? rule - 6 : rule)).total;
+  const int sh = static_shape(g);
+  return static_layout(g, shape_blocks(g, sh), kShapes[sh].fused).total;
 }
 
 size_t cgm_static_capacity() { return kStaticScratchBytes; }
@@ -1365,13 +1392,13 @@ hipError_t cgm_f64_run_static(const Geom& g, int device, const double* sm, const
   unsigned char* base = nullptr;
   hipError_t e = static_scratch(device, &base);
   if (e != hipSuccess) return e;
-  const int rule = cgm_rule(g, 1);
-  const int shape = rule >= 12 ? rule - 6 : rule;  // the same iterates without the residual ring
+  const int shape = static_shape(g);
+  const bool fused = kShapes[shape].fused;
   CGMFix a;
   a.g = g;
   a.nrhs = 1;
   a.nb = shape_blocks(g, shape);
-  const StaticLayout l = static_layout(g, a.nb);
+  const StaticLayout l = static_layout(g, a.nb, fused);
   if (l.total > kStaticScratchBytes) return hipErrorInvalidValue;  // the caller checks the size first
   a.sm = (double*)(base + l.sm);
   a.U = nullptr;
@@ -1384,8 +1411,9 @@ hipError_t cgm_f64_run_static(const Geom& g, int device, const double* sm, const
   a.state = (CGMState*)(base + l.state);
   a.res = (CGDevResult*)(base + l.res);
   a.rring = a.r;
-  a.rstride = 0;
-  a.t1 = a.s1 = nullptr;  // the static solve runs no fused shape
+  a.rstride = fused ? l.vec / sizeof(double2) : 0;
+  a.t1 = fused ? (double2*)(base + l.t1) : nullptr;
+  a.s1 = fused ? (double2*)(base + l.s1) : nullptr;
   e = cgm_start(a, sm, b, x, tol, max_iter, start, 1, shape, st);
   if (e != hipSuccess) return e;
   // chunks of ENG_CHUNK iterations launched directly, the state's `active`

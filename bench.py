@@ -831,6 +831,8 @@ def schedule_leg(gnx, torch, L, tol, batches, reps=3):
     of the solutions from the rule's."""
     V = L ** 3
     out = {"workload": f"{L}^3, sigma and complex Z2 sources from the reference's stream (seed 1231), tol {tol:g}"}
+    if L % 16 or L % 8:
+        return out
     saved = os.environ.pop("GNX_CG_SHAPE", None)
     try:
         for nb in batches:
@@ -1142,6 +1144,8 @@ def main():
                                                   [v for v in args.mixed.split(",") if v])
     if args.schedules:
         result["schedules"] = schedule_leg(gnx, torch, L, args.tol, [int(v) for v in args.schedules.split(",")])
+        # 48^3 single: the size where the rule runs the fused form (216 tiles, one per CU)
+        result["schedules_48"] = schedule_leg(gnx, torch, 48, args.tol, [1])
     if args.probes != "none":
         result["probes"] = probes_leg(L, barrier=args.probes == "all")
     if args.copy_calibration > 0:

@@ -214,9 +214,12 @@ gnx_status gnx_tune_runs(gnx_ctx *ctx, long *fp64, long *mixed);
  * deferred form likewise; odd n use the XCD slab mapping; n = 18..21 the
  * fused form on 4 x 8 x 16 and 8 x 8 x 8 tiles, without and with deferral
  * (where the tiles divide the lattice; elsewhere the fixed rule's shape).
- * The fixed rule does not choose the fused form: it is slower on the sizes
- * measured (its halo costs more than the kernel boundary it saves).  The
- * link-coupled operator's shape is chosen the same way but not reported. */
+ * The fixed rule chooses the fused form (n = 18) on the live operator where
+ * the batch's 4 x 8 x 16 tiles number 160 to 256 (one per CU: 48^3 single,
+ * 32^3 x 3 and x 4), where it saves a kernel boundary per iteration; with
+ * more tiles its halo work costs more than the boundary.  The link-coupled
+ * operator's shape is chosen the same way (without the fused form) but not
+ * reported. */
 gnx_status gnx_cg_shape(gnx_ctx *ctx, int nrhs, int *sites_per_thread, int *xcd_mapped);
 /* The same for the mixed-precision solver (prec GNX_PREC_F32 or GNX_PREC_BF16);
  * pair = 1: two systems share each block's site arithmetic; loads_first = 1:

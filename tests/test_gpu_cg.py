@@ -196,3 +196,38 @@ def test_fused_shapes(torch_dev, P, gnx_mod, shape, L, monkeypatch):
         assert np.array_equal(host(X2), host(X))
         ctx2.close()
     ctx.close()
+
+
+def test_rule_picks_the_fused_form_in_its_band(torch_dev, P, gnx_mod, monkeypatch):
+    """Without a pinned shape the fixed rule runs the fused form where the
+    batch's 4 x 8 x 16 tiles number 160 to 256 (48^3 single: 216; 32^3 x 4:
+    256) and the one-reduction pair elsewhere (32^3 single: 64 tiles; 64^3
+    single: 512); the 48^3 default solve against the oracle: solution within
+    1e-7, true residual below 10 tol, iterations within 5 % (this system's
+    residual crosses 1e-10 on a plateau: the reference's recurrences stop at
+    287 iterations, the one-reduction recurrence at 287 or 299 depending on
+    the partial-sum partition -- 2 sites per thread 287, 1 site per thread
+    and the fused tiles 299 -- all at true residuals of 9-10e-11)."""
+    banner("launch rule: the fused band")
+    monkeypatch.delenv("GNX_CG_SHAPE", raising=False)
+    monkeypatch.delenv("GNX_AUTOTUNE", raising=False)
+    for L, n, fused in ((48, 1, True), (32, 4, True), (32, 1, False), (64, 1, False)):
+        lat = P.Lattice(L)
+        r = P.Random(60 + L + n)
+        smh = lat.sigma_mass(r.sigma(lat.V, P.sigma_variance(0.85)))
+        bs = [r.z2_complex(lat.V) for _ in range(n)]
+        ctx = gnx_mod.Context(L)
+        X = ctx.vec(n)
+        if n == 1:
+            st, res = ctx.cg(dev(smh), dev(bs[0]), X, 1e-10, start=gnx_mod.START_IMPOSED)
+            res = [res]
+        else:
+            st, res = ctx.cg_batch(dev(smh), dev(np.concatenate(bs)), X, n, 1e-10, start=gnx_mod.START_IMPOSED)
+        assert st == 0
+        assert ctx.cg_shape(n)["fused"] == fused, (L, n)
+        if L == 48:
+            xr, ok, stc = lat.cg(smh, bs[0], None, tol=1e-10, start="imposed")
+            assert ok and abs(res[0].iterations - stc.last_iters) <= 0.05 * stc.last_iters
+            assert rel(host(X), xr) < 1e-7
+            assert true_residual(lat, smh, bs[0], host(X)) < 1e-9
+        ctx.close()
