@@ -231,3 +231,41 @@ def test_rule_picks_the_fused_form_in_its_band(torch_dev, P, gnx_mod, monkeypatc
             assert rel(host(X), xr) < 1e-7
             assert true_residual(lat, smh, bs[0], host(X)) < 1e-9
         ctx.close()
+
+
+@pytest.mark.parametrize("shape", [18, 19])
+def test_fused_exits(torch_dev, P, gnx_mod, shape, monkeypatch):
+    """The fused kernels' exits: the iteration cap (7: the deferred form
+    applies its pending x and p updates at the cap; the iterate equals the
+    oracle's after 7 iterations within 1e-12 and the non-deferred form's bit
+    for bit), and the imposed start's exit before the loop (b = 0: no
+    iteration, x = 0)."""
+    banner(f"cg fused exits, shape {shape}")
+    monkeypatch.setenv("GNX_CG_SHAPE", str(shape))
+    L = 16
+    lat = P.Lattice(L)
+    r = P.Random(55)
+    smh = lat.sigma_mass(r.sigma(lat.V, P.sigma_variance(0.85)))
+    b = r.z2_complex(lat.V)
+    ctx = gnx_mod.Context(L)
+    x = ctx.vec()
+    st, res = ctx.cg(dev(smh), dev(b), x, 1e-10, max_iter=7, start=gnx_mod.START_IMPOSED, check=False)
+    assert st == 3 and res.iterations == 7 and res.converged == 0
+    assert ctx.cg_shape(1)["fused"]
+    xo, ok, _ = lat.cg(smh, b, None, tol=1e-10, max_iter=7, start="imposed")
+    assert not ok
+    assert np.linalg.norm(host(x) - xo) <= 1e-12 * np.linalg.norm(xo)
+    if shape == 19:
+        monkeypatch.setenv("GNX_CG_SHAPE", "18")
+        ctx2 = gnx_mod.Context(L)
+        x2 = ctx2.vec()
+        ctx2.cg(dev(smh), dev(b), x2, 1e-10, max_iter=7, start=gnx_mod.START_IMPOSED, check=False)
+        assert np.array_equal(host(x2), host(x))
+        ctx2.close()
+    z = ctx.vec()
+    x0 = ctx.vec()
+    x0 += 1.0
+    st, res = ctx.cg(dev(smh), z, x0, 1e-10, start=gnx_mod.START_IMPOSED)
+    assert st == 0 and res.early_exit == 1 and res.iterations == 0
+    assert float(x0.abs().max().item()) == 0.0
+    ctx.close()
