@@ -53,8 +53,9 @@ def kernel_sources_sha():
 def shape_tag(shape):
     # kernel template arguments: <sites per thread, XCD slab mapping, link-coupled operator>,
     # prefixed "cg1" for the one-reduction kernels (k_cg1_X, k_cg1_Y), "cgd" for
-    # their deferred-update form (k_cgd_X, k_cgd_Y)
-    pre = "cgd" if shape.get("deferred") else "cg1" if shape.get("one_reduction") else ""
+    # their deferred-update form (k_cgd_X, k_cgd_Y), "cgf" for the fused form (k_cgf_Z)
+    pre = ("cgf" if shape.get("fused") else "cgd" if shape.get("deferred") else "cg1" if shape.get("one_reduction")
+           else "")
     return pre + f"<{shape['sites_per_thread']}, {'true' if shape['xcd_slab'] else 'false'}, false>"
 
 
