@@ -160,8 +160,10 @@ gnx_status gnx_dirac(gnx_ctx* c, gnx_op op, const double* sm, const double* in, 
 gnx_status gnx_dirac_batch(gnx_ctx* c, gnx_op op, const double* sm, const double* in, double* out, int nvec) {
   CHECK_ARG(c && sm && in && out, "gnx_dirac_batch: null argument");
   CHECK_ARG(nvec >= 1 && nvec <= 64, "gnx_dirac_batch: nvec must be in [1, 64]");
-  CHECK_ARG(op == GNX_OP_M || op == GNX_OP_MDAG || op == GNX_OP_SUB_M || op == GNX_OP_SUB_MDAG,
-            "gnx_dirac_batch: op must be one operator (M, M^dag, r -= M v, r -= M^dag v)");
+  CHECK_ARG(op == GNX_OP_M || op == GNX_OP_MDAG || op == GNX_OP_SUB_M || op == GNX_OP_SUB_MDAG ||
+                ((op == GNX_OP_MMDAG || op == GNX_OP_MDAGM) && normal_fused_fits(c->g)),
+            "gnx_dirac_batch: op must be one operator (M, M^dag, r -= M v, r -= M^dag v), or a normal "
+            "operator on a lattice of 4 x 8 x 16 tiles (L divisible by 8, LT by 16)");
   const char* ib = (const char*)in;
   const char* ob = (const char*)out;
   const size_t bytes = sizeof(double2) * (size_t)c->V * nvec;

@@ -20,6 +20,9 @@ hipError_t launch_evolve_sigma(const Geom& g, double* sigma, double* sm, const d
 hipError_t launch_evolve_momenta(const Geom& g, double* mom, const double* force, double eps, hipStream_t st);
 hipError_t launch_dirac(const Geom& g, int op, const double* sm, const double* in, double* out, double* tmp,
                         hipStream_t st, int nvec = 1);
+// whether launch_dirac applies M M^dag and M^dag M in one launch on g (and
+// accepts batches of them)
+bool normal_fused_fits(const Geom& g);
 hipError_t launch_sigma_force(const Geom& g, const double* sigma, double beta_s, int nflav, const double* X0,
                               const double* Y0, const double* X1, const double* Y1, double* force,
                               hipStream_t st, int accumulate = 0, int sigma_term = 1);

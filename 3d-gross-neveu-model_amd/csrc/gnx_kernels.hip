@@ -5,6 +5,7 @@
 // coalesced double2 loads in f-order, no tables read from memory.
 #include "gnx_internal.h"
 #include "gnx_solver.h"  // elem()
+#include "gnx_tile.h"
 
 namespace gnx {
 
@@ -57,6 +58,72 @@ __global__ __launch_bounds__(256) void k_dirac(Geom g, const double* __restrict_
   else if (OP == GNX_OP_SUB_M) y = apply_sub_M(g, s, smv, *elem(out, f), ld);
   else y = apply_sub_Mdag(g, s, smv, *elem(out, f), ld);
   *elem(out, f) = y;
+}
+
+// ------------------------------------------------------------------ normal operators
+// out = M M^dag in (OP2 = GNX_OP_MMDAG) or M^dag M in (GNX_OP_MDAGM) in one
+// launch, the intermediate vector kept in LDS: each block computes the first
+// operator on its tile and on the tile's one-site halo (gnx_tile.h), into an
+// LDS box, then the second operator on the tile from the box.  Every value is
+// the one the two single-operator kernels compute (the same operations on the
+// same operands: a halo site again by this block), so the result is
+// bit-identical to the two launches through a staged vector, which read and
+// write the intermediate in memory (80 B/site against 40 B/site here, plus a
+// halo of 0.875 first-operator sites per tile site, mostly L2 hits).
+template <int OP2, int T0, int T1, int T2>
+__global__ __launch_bounds__(ENG_BS) void k_normal(Geom g, int nb, const double* __restrict__ sm,
+                                                   const double2* __restrict__ in, double2* __restrict__ out) {
+  using TL = FTile<T0, T1, T2>;
+  constexpr int R = TL::ROUNDS, SPT = TL::SPT;
+  __shared__ double2 box[TL::NBOX];
+  in += (size_t)blockIdx.y * g.V;
+  out += (size_t)blockIdx.y * g.V;
+  int o0, o1, o2;
+  ftile_origin<T0, T1, T2>(g, nb, (int)blockIdx.x, &o0, &o1, &o2);
+  int fs[R], bx[R];
+  double smv[R];
+  double2 v[R][7];
+#pragma unroll
+  for (int i = 0; i < R; i++) {
+    // an item past the last one loads the tile's first site and stores nothing
+    const int e = i * ENG_BS + (int)threadIdx.x;
+    const Site st = ftile_site<T0, T1, T2>(g, o0, o1, o2, e < TL::NEXT ? e : 0, &bx[i]);
+    fs[i] = f_of_lex(g, st.lex, st.P);
+    const NbrF n = neighbours_f(g, st);
+    const int q[7] = {fs[i], n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
+    smv[i] = *elem(sm, fs[i]);
+#pragma unroll
+    for (int k = 0; k < 7; k++) v[i][k] = *elem(in, q[k]);
+  }
+  end_of_loads();
+#pragma unroll
+  for (int i = 0; i < R; i++) {
+    if (i >= SPT && i * ENG_BS + (int)threadIdx.x >= TL::NEXT) continue;
+    const Site st = decode(g, fs[i]);
+    box[bx[i]] = OP2 == GNX_OP_MMDAG ? apply_Mdag_vals(g, st, smv[i], v[i][0], &v[i][1], &v[i][4])
+                                     : apply_M_vals(g, st, smv[i], v[i][0], &v[i][1], &v[i][4]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    const Site st = decode(g, fs[i]);
+    const int b = bx[i];
+    const double2 vpp[3] = {box[b + TL::S0], box[b + TL::S1], box[b + 1]};
+    const double2 vmm[3] = {box[b - TL::S0], box[b - TL::S1], box[b - 1]};
+    *elem(out, fs[i]) = OP2 == GNX_OP_MMDAG ? apply_M_vals(g, st, smv[i], box[b], vpp, vmm)
+                                            : apply_Mdag_vals(g, st, smv[i], box[b], vpp, vmm);
+  }
+}
+
+// whether the one-launch normal operator's 4 x 8 x 16 tiles divide g
+bool normal_fused_fits(const Geom& g) { return g.L % 4 == 0 && g.L % 8 == 0 && g.LT % 16 == 0; }
+
+// One application: the one launch where it is faster than two (host-launched
+// at 16^3-48^3 4.3-4.8 against 8.3 us, 64^3 6.8 against 8.3; at 96^3, where
+// the stencils are bound by bytes, its halo costs more: 17.6 against 15.2).
+// A batch always takes it (the two launches stage a single vector).
+static bool normal_one_launch(const Geom& g, int nvec) {
+  return normal_fused_fits(g) && (nvec > 1 || g.V <= 300000);
 }
 
 // ------------------------------------------------------------------ sigma force
@@ -196,7 +263,16 @@ hipError_t launch_dirac(const Geom& g, int op, const double* sm, const double* i
   const double2* vin = (const double2*)in;
   double2* vout = (double2*)out;
   dim3 grid(nblk(g.V, 256), nvec), blk(256);
-  // the two-operator products stage one vector in tmp
+  if ((op == GNX_OP_MMDAG || op == GNX_OP_MDAGM) && normal_one_launch(g, nvec)) {
+    // one launch, the intermediate in LDS
+    const int nb = g.V / (4 * 8 * 16);
+    if (op == GNX_OP_MMDAG)
+      hipLaunchKernelGGL((k_normal<GNX_OP_MMDAG, 4, 8, 16>), dim3(nb, nvec), blk, 0, st, g, nb, sm, vin, vout);
+    else
+      hipLaunchKernelGGL((k_normal<GNX_OP_MDAGM, 4, 8, 16>), dim3(nb, nvec), blk, 0, st, g, nb, sm, vin, vout);
+    return hipGetLastError();
+  }
+  // otherwise the two-operator products stage one vector in tmp
   if (nvec != 1 && (op == GNX_OP_MMDAG || op == GNX_OP_MDAGM)) return hipErrorInvalidValue;
   switch (op) {
     case GNX_OP_M:

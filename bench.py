@@ -961,8 +961,10 @@ def main():
     dirac_gbs = BYTES_DIRAC_F64 * V / t_app / 1e9
     dirac_graph = dirac_graph_rate(gnx, torch, L, sm, b)
     torch.cuda.synchronize()
-    # the normal operators as the drop-in applies them (two launches each):
-    # M M^dag (the CG's operator) and M^dag M (offered beside it)
+    # the normal operators: M M^dag (the CG's operator) and M^dag M (offered
+    # beside it), one launch with the intermediate in LDS at this size
+    # (40 B/site algorithmic), against the two single-operator launches
+    # through a staged vector (80 B/site)
     prod = {}
     for name, op in (("M_Mdag", gnx.OP_MMDAG), ("Mdag_M", gnx.OP_MDAGM)):
         for _ in range(10):
@@ -974,7 +976,23 @@ def main():
         e1.record(stream)
         e1.synchronize()
         tp = e0.elapsed_time(e1) * 1e-3 / 100
-        prod[name] = {"apps_per_s": 1.0 / tp, "GB_s": 2 * BYTES_DIRAC_F64 * V / tp / 1e9, "launches": 2}
+        prod[name] = {"apps_per_s": 1.0 / tp, "algorithmic_GB_s": BYTES_DIRAC_F64 * V / tp / 1e9,
+                      "launches": 1 if L % 16 == 0 and L ** 3 <= 300000 else 2}
+    tmp = ctx.vec()
+    for _ in range(10):
+        ctx.dirac(gnx.OP_MDAG, sm, b, tmp)
+        ctx.dirac(gnx.OP_M, sm, tmp, out)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(100):
+        ctx.dirac(gnx.OP_MDAG, sm, b, tmp)
+        ctx.dirac(gnx.OP_M, sm, tmp, out)
+    e1.record(stream)
+    e1.synchronize()
+    tp = e0.elapsed_time(e1) * 1e-3 / 100
+    prod["M_Mdag_two_launches"] = {"apps_per_s": 1.0 / tp, "GB_s": 2 * BYTES_DIRAC_F64 * V / tp / 1e9,
+                                   "launches": 2}
+    del tmp
     # batched: 16 vectors per launch, one sigma mass (2 s_v + s_m / n per site)
     nvec = 16
     vb = torch.from_numpy(make_inputs(L, seed=77, nsrc=nvec)[1]).to(dev)

@@ -106,11 +106,17 @@ gnx_status gnx_geometry_tables(gnx_ctx *ctx, int *fermionpp, int *fermionmm, dou
  * the 8 dual sites below x, reference summation order. */
 gnx_status gnx_sigma_mass(gnx_ctx *ctx, const double *sigma, double *sm);
 
-/* A3/A4/A5 and the normal operators.  `in` and `out` must not alias. */
+/* A3/A4/A5 and the normal operators.  `in` and `out` must not alias.  M M^dag
+ * and M^dag M are the compositions of the bit-exact single operators,
+ * applied in one launch with the intermediate in LDS where the lattice is
+ * made of 4 x 8 x 16 tiles (L divisible by 8, LT by 16) and V <= 300 000
+ * (bit-identical to the two launches through a staged vector, used
+ * otherwise). */
 gnx_status gnx_dirac(gnx_ctx *ctx, gnx_op op, const double *sm, const double *in, double *out);
 /* The same operator on nvec vectors (in, out: [nvec][V] complex, not
  * overlapping) with one sigma mass, one launch; op is GNX_OP_M, GNX_OP_MDAG,
- * GNX_OP_SUB_M or GNX_OP_SUB_MDAG.  Each vector's result is bit-identical to
+ * GNX_OP_SUB_M or GNX_OP_SUB_MDAG, or GNX_OP_MMDAG / GNX_OP_MDAGM on a
+ * lattice of 4 x 8 x 16 tiles.  Each vector's result is bit-identical to
  * its own gnx_dirac.  No counterpart in the reference (one vector per call). */
 gnx_status gnx_dirac_batch(gnx_ctx *ctx, gnx_op op, const double *sm, const double *in, double *out, int nvec);
 

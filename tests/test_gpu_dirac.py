@@ -163,3 +163,35 @@ def test_ops_at_the_largest_volumes(torch_dev, P, gnx_mod, L, LT):
     del out, bd, sm
     ctx.close()
     torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("L,LT", [(16, 16), (8, 16), (32, 32)])
+def test_normal_operators_one_launch(torch_dev, P, gnx_mod, L, LT):
+    """M M^dag and M^dag M in one launch (the intermediate in LDS, on
+    lattices of 4 x 8 x 16 tiles; 8^2 x 16 is a single tile along x1 and
+    x2, so its halo wraps onto itself): bit-exact against the oracle's
+    composition, and a batch of 3 (gnx_dirac_batch, which accepts the normal
+    operators on such lattices) bit-identical to single applications."""
+    banner(f"normal operators in one launch {L}x{L}x{LT}")
+    import torch
+    lat = P.Lattice(L, LT)
+    sigma, b, c = _fields(P, lat, 500 + L + LT)
+    sm_ref = lat.sigma_mass(sigma)
+    ctx = gnx_mod.Context(L, LT)
+    sm = ctx.field()
+    ctx.sigma_mass(dev(sigma), sm)
+    vs = [b, c, 0.5 * b - c]
+    for op, ref in ((gnx_mod.OP_MMDAG, lambda v: lat.M(sm_ref, lat.Mdag(sm_ref, v))),
+                    (gnx_mod.OP_MDAGM, lambda v: lat.Mdag(sm_ref, lat.M(sm_ref, v)))):
+        outs = []
+        for v in vs:
+            out = ctx.vec()
+            ctx.dirac(op, sm, dev(v), out)
+            ctx.synchronize()
+            assert np.array_equal(host(out), ref(v)), op
+            outs.append(host(out))
+        O = ctx.vec(3)
+        ctx.dirac_batch(op, sm, dev(np.concatenate(vs)), O, 3)
+        ctx.synchronize()
+        assert np.array_equal(host(O), np.concatenate(outs)), op
+    ctx.close()
