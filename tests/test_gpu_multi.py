@@ -177,3 +177,49 @@ def test_multi_mixed_matches_single_and_oracle(P, gnx_mod, prec, start, monkeypa
         assert ok and rel(xk, xr) < 1e-7, k
         assert np.linalg.norm(b - lat.M(sm, lat.Mdag(sm, xk))) < 2e-10, k
     m.close()
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16"])
+def test_multi_mixed_prologue_exit_and_cap(P, gnx_mod, prec, monkeypatch):
+    """Mixed precision over an uneven batch, the exits: (1) null start with
+    one system handed its own solution -- it leaves in the prologue, x
+    untouched, while the others iterate to their single-solve results bit
+    for bit; (2) the iteration cap (5) -- every system stops unconverged
+    with the iterate of its single mixed solve under the same cap."""
+    banner(f"uneven batch, mixed {prec}: prologue exit and cap")
+    monkeypatch.setenv("GNX_MIX_SHAPE", "11")
+    code = {"f32": gnx_mod.PREC_F32, "f16": gnx_mod.PREC_F16}[prec]
+    sy = _systems(P, 820)
+    m = gnx_mod.Multi(SIZES)
+    sms = [dev(sm) for _, sm, _, _ in sy]
+    bs = [dev(b) for _, _, b, _ in sy]
+    xs = [dev(x0) for _, _, _, x0 in sy]
+    st, _ = m.cg(sms, bs, xs, 1e-10, start=gnx_mod.START_IMPOSED, prec=code)
+    assert st == 0
+    solved = host(xs[1]).copy()
+    xs2 = [dev(x0) for _, _, _, x0 in sy]
+    xs2[1] = dev(solved)
+    st, res = m.cg(sms, bs, xs2, 1e-8, start=gnx_mod.START_NULL, prec=code)
+    assert st == 0
+    assert res[1].early_exit == 1 and res[1].iterations == 0
+    assert np.array_equal(host(xs2[1]), solved)
+    for k, (lat, sm, b, x0) in enumerate(sy):
+        if k == 1:
+            continue
+        ctx = gnx_mod.Context(lat.L, lat.LT)
+        x1 = dev(x0)
+        s1, r1 = ctx.cg(dev(sm), dev(b), x1, 1e-8, start=gnx_mod.START_NULL, prec=code)
+        assert s1 == 0 and res[k].early_exit == 0 and res[k].iterations == r1.iterations, k
+        assert np.array_equal(host(xs2[k]), host(x1)), k
+    xs3 = [dev(x0) for _, _, _, x0 in sy]
+    st, res = m.cg(sms, bs, xs3, 1e-10, max_iter=5, start=gnx_mod.START_IMPOSED, prec=code)
+    assert st == 3
+    for k, (lat, sm, b, x0) in enumerate(sy):
+        assert res[k].converged == 0 and res[k].iterations == 5, k
+        ctx = gnx_mod.Context(lat.L, lat.LT)
+        x1 = dev(x0)
+        s1, r1 = ctx.cg(dev(sm), dev(b), x1, 1e-10, max_iter=5, start=gnx_mod.START_IMPOSED, prec=code,
+                        check=False)
+        assert s1 == 3 and r1.iterations == 5
+        assert np.array_equal(host(xs3[k]), host(x1)), k
+    m.close()
