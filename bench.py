@@ -886,6 +886,7 @@ def schedule_leg(gnx, torch, L, tol, batches, reps=3):
 
 
 def main():
+    t_start = time.perf_counter()
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
@@ -1184,6 +1185,7 @@ def main():
         del src, dst
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(L, sigma_h, b_h)
+    result["bench_seconds"] = time.perf_counter() - t_start  # the whole script, every leg included
     print(json.dumps(result), flush=True)
     return 0
 
