@@ -272,10 +272,37 @@ def setup_estimate(sizes=(32, 64, 96)):
     return out
 
 
-def cpu_baseline(L, sigma, b, nsolves=5):
+def cpu_hmc(P, gpu, ntraj=20, eps=0.1, steps=2, tol_high=1e-10, tol_medium=1e-3, L=8, beta_s=0.85):
+    """The oracle's serial sigma-HMC on configuration 1's workload (its own
+    copy of the reference's stream from Initrand(1231), drawn in the same
+    order), timed, and its Delta-H and accept sequence against the GPU's
+    (hmc_leg)."""
+    h = P.HMC(P.Lattice(L), 1231, beta_s=beta_s)
+    dh, acc = [], []
+    t_traj = 0.0
+    t0 = time.perf_counter()
+    for _ in range(ntraj):
+        ta = time.perf_counter()
+        t = h.trajectory(eps, steps, tol_high, tol_medium)
+        t_traj += time.perf_counter() - ta
+        h.condensate(tol_high)
+        dh.append(t.dH)
+        acc.append(int(t.accepted))
+    dt = time.perf_counter() - t0
+    out = {"trajectories_per_s": ntraj / t_traj, "with_measurement_per_s": ntraj / dt, "cores": 1,
+           "kind": "port", "sample": f"the {ntraj} trajectories of configuration 1 with one measurement each"}
+    if gpu and "dH" in gpu:
+        out["max_abs_dH_diff_vs_gpu"] = max(abs(a - b) for a, b in zip(dh, gpu["dH"]))
+        out["same_accept_decisions_as_gpu"] = acc == gpu["accepted"]
+    return out
+
+
+def cpu_baseline(L, sigma, b, nsolves=5, hmc_gpu=None):
     """Oracle CG (serial, 1 core, the reference's loops and order) on the
     same fields: the median of nsolves full solves to the same absolute
-    tolerance, and the median of 100 single Dirac applications."""
+    tolerance, and the median of 100 single Dirac applications; with
+    hmc_gpu (the GPU's configuration-1 leg), the oracle's HMC on the same
+    workload."""
     import statistics
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as P
@@ -311,6 +338,7 @@ def cpu_baseline(L, sigma, b, nsolves=5):
         "dirac": {"apps_per_s": 1.0 / t_app, "GB_s": BYTES_DIRAC_F64 * L ** 3 / t_app / 1e9,
                   "sample": "median of 100 serial applications (M and M^dag alternating) at the same size"},
         "setup_estimate": setup_estimate(),
+        **({"hmc_8": cpu_hmc(P, hmc_gpu)} if hmc_gpu is not None else {}),
     }
 
 
@@ -821,6 +849,53 @@ def size_leg(gnx, torch, L, tol, steps, nbatch, precs):
     return out
 
 
+def hmc_leg(gnx, ntraj=20, eps=0.1, steps=2, tol_high=1e-10, tol_medium=1e-3, L=8, beta_s=0.85):
+    """Configuration 1: sigma-only HMC at 8^3 -- LFMN2VF trajectories of 2
+    steps of 0.1 (tau 0.2), START_X0_NULL force solves to 1e-3 and the
+    energies' solves to 1e-10, each trajectory followed by its Metropolis
+    test and one measurement (the condensate's 11 outputs) -- with every draw
+    from the reference's stream (Initrand(1231): sigma, then per trajectory
+    momenta, chi_0, chi_1, the Metropolis uniform, 6 V Z2 draws).  Two
+    warm-up trajectories on other draws first capture this context's solver
+    graphs.  Reports trajectories/s with and without the measurement, the
+    Delta-H and accept sequence, and CG iterations per trajectory."""
+    V = L ** 3
+    ctx = gnx.Context(L)
+    h = gnx.HMC(ctx, beta_s=beta_s)
+    var = 2.0 / (4 * beta_s)
+    w = gnx.Rng(4242)
+    h.set_sigma(w.sigma(V, var))
+    for _ in range(2):
+        h.trajectory(w.gauss1(V), w.gauss1(V), w.gauss1(V), eps, steps, tol_high, tol_medium)
+        h.accept(True)
+        h.condensate(w.z2_real(6 * V), tol_high)
+    rng = gnx.Rng(1231)
+    h.set_sigma(rng.sigma(V, var))
+    dh, acc, its, cond = [], [], [], []
+    t_traj = 0.0
+    t0 = time.perf_counter()
+    for _ in range(ntraj):
+        ta = time.perf_counter()
+        st, r = h.trajectory(rng.gauss1(V), rng.gauss1(V), rng.gauss1(V), eps, steps, tol_high, tol_medium)
+        a = bool(r.ok) and rng.casuale() <= np.exp(-r.dH)
+        h.accept(a)
+        t_traj += time.perf_counter() - ta
+        cond.append(float(h.condensate(rng.z2_real(6 * V), tol_high)[0]))
+        dh.append(r.dH)
+        acc.append(int(a))
+        its.append(r.cg_iterations)
+    dt = time.perf_counter() - t0
+    sig, _ = h.get_sigma()
+    h.close()
+    ctx.close()
+    return {"workload": f"{L}^3 sigma-HMC (LFMN2VF), {ntraj} trajectories of {steps} steps of {eps}, tol "
+                        f"{tol_high:g} / {tol_medium:g}, beta_s {beta_s}, one measurement per trajectory, "
+                        f"draws from Initrand(1231)",
+            "trajectories_per_s": ntraj / t_traj, "with_measurement_per_s": ntraj / dt,
+            "acceptance": sum(acc) / ntraj, "dH": dh, "accepted": acc, "cg_iterations_per_trajectory": its,
+            "condensate_tr1": cond, "sigma_mean_final": float(np.mean(sig))}
+
+
 def schedule_leg(gnx, torch, L, tol, batches, reps=3):
     """The FP64 CG iteration in each kernel schedule, on the headline's
     inputs: the fixed rule's choice, the two-kernel one-reduction form
@@ -904,6 +979,8 @@ def main():
     ap.add_argument("--sizes", default="32:32,96:16", metavar="L:N,...",
                     help="configurations 2 and 3 at other sizes: lattice L^3, FP64 single and N-source "
                          "batched FP64 and mixed solves ('' to skip)")
+    ap.add_argument("--no-hmc", dest="hmc", action="store_false",
+                    help="skip configuration 1 (sigma-HMC at 8^3, 20 trajectories with measurements)")
     ap.add_argument("--schedules", default="1,16", metavar="N,...",
                     help="batch sizes of the kernel-schedule comparison at the headline size ('' to skip)")
     ap.add_argument("--probes", default="all", choices=["all", "nodes", "none"],
@@ -1162,6 +1239,8 @@ def main():
             Ls, n = (int(v) for v in spec.split(":"))
             result["sizes"][f"{Ls}^3"] = size_leg(gnx, torch, Ls, args.tol, args.steps, n,
                                                   [v for v in args.mixed.split(",") if v])
+    if args.hmc:
+        result["hmc_8"] = hmc_leg(gnx)
     if args.schedules:
         result["schedules"] = schedule_leg(gnx, torch, L, args.tol, [int(v) for v in args.schedules.split(",")])
         # 48^3 single: the size where the rule runs the fused form (216 tiles, one per CU)
@@ -1184,7 +1263,7 @@ def main():
                                       "GB_s": 16 * n / t / 1e9}
         del src, dst
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(L, sigma_h, b_h)
+        result["cpu_baseline"] = cpu_baseline(L, sigma_h, b_h, hmc_gpu=result.get("hmc_8"))
     result["bench_seconds"] = time.perf_counter() - t_start  # the whole script, every leg included
     print(json.dumps(result), flush=True)
     return 0
