@@ -47,8 +47,30 @@ double gnx::reliable_delta_of(const gnx_ctx* c, int prec) {
   return c->reliable_delta > 0.0 ? c->reliable_delta : (prec == GNX_PREC_BF16 ? 0.2 : 0.05);
 }
 
+// Whether the FP64 solves of this context run in one launch per solve (the
+// small-lattice kernel, up to 12^3 sites and 100 000 iterations) rather than
+// as replayed graphs: under the fixed launch rule only -- a shape pinned by
+// GNX_CG_SHAPE, or timing autotune, selects among the graph forms.
+static bool cg_small(gnx_ctx* c, long max_iter, int nrhs) {
+  if (c->cg_pinned < 0) {
+    const char* at = getenv("GNX_AUTOTUNE");
+    c->cg_pinned = (getenv("GNX_CG_SHAPE") || (at && atoi(at) != 0)) ? 1 : 0;
+  }
+  return !c->cg_pinned && !c->autotune && nrhs <= 64 && cgs_fits(c->g, max_iter);
+}
+
 hipError_t gnx::cg_run(gnx_ctx* c, const double* sm, const double2* b, double2* x, double tol, long max_iter,
                       int start, int nrhs, CGDevResult* host_res) {
+  if (cg_small(c, max_iter, nrhs)) {
+    hipError_t e = hipSuccess;
+    if (!c->cgs_res) e = hipMalloc(&c->cgs_res, sizeof(CGDevResult) * 64);
+    if (e == hipSuccess) e = cgs_run(c->g, sm, b, x, tol, max_iter, start, nrhs, c->cgs_res, c->stream);
+    if (e == hipSuccess && host_res)
+      e = hipMemcpyAsync(host_res, c->cgs_res, sizeof(CGDevResult) * nrhs, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) c->cgs_seen |= 1ull << (nrhs - 1);
+    return e;
+  }
   return cgm_f64_run(&c->cgm, c->g, sm, nullptr, b, x, tol, max_iter, start, nrhs, c->stream, host_res);
 }
 
@@ -110,6 +132,7 @@ gnx_status gnx_destroy(gnx_ctx* c) {
   if (c->host_out) (void)hipHostFree(c->host_out);
   if (c->bat_part) (void)hipFree(c->bat_part);
   if (c->bat_out) (void)hipFree(c->bat_out);
+  if (c->cgs_res) (void)hipFree(c->cgs_res);
   if (c->cgm) cgm_release(c->cgm);
   if (c->cgmix) cgmix_release(c->cgmix);
   delete c;
@@ -240,6 +263,22 @@ gnx_status gnx_cg_noalloc(gnx_ctx* c, const double* sm, const double* b, double*
   if (max_iter <= 0) max_iter = c->V + 1;
   HIP_TRY(hipSetDevice(c->device));
   CGDevResult hr;
+  if (cg_small(c, max_iter, 1)) {
+    // gnx_cg's one-launch solve; its only buffer besides the caller's is the
+    // result, in the context's 16 reduction scalars (128 bytes)
+    static_assert(sizeof(CGDevResult) <= 16 * sizeof(double), "result block exceeds red_out");
+    CGDevResult* dres = (CGDevResult*)c->red_out;
+    HIP_TRY(cgs_run(c->g, sm, (const double2*)b, (double2*)x, tol, max_iter, (int)start, 1, dres, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->host_out, dres, sizeof(CGDevResult), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    memcpy(&hr, c->host_out, sizeof(CGDevResult));
+    fill_result(res, hr);
+    if (!hr.converged) {
+      set_err("gnx_cg_noalloc: not converged within %ld iterations", max_iter);
+      return GNX_ERR_NOCONV;
+    }
+    return GNX_OK;
+  }
   HIP_TRY(cgm_f64_run_static(c->g, c->device, sm, (const double2*)b, (double2*)x, tol, max_iter, (int)start,
                              c->stream, c->host_out, &hr));
   fill_result(res, hr);
@@ -285,6 +324,11 @@ gnx_status gnx_set_reliable_delta(gnx_ctx* c, double delta) {
 
 gnx_status gnx_cg_shape(gnx_ctx* c, int nrhs, int* spt, int* xcd) {
   CHECK_ARG(c, "gnx_cg_shape: null context");
+  if (nrhs >= 1 && nrhs <= 64 && (c->cgs_seen >> (nrhs - 1) & 1) && cg_small(c, 0, nrhs)) {
+    if (spt) *spt = cgs_sites_per_thread(c->g);
+    if (xcd) *xcd = 16;
+    return GNX_OK;
+  }
   if (cgm_shape_of(c->cgm, nrhs, spt, xcd) < 0) {
     set_err("gnx_cg_shape: no shape chosen for nrhs=%d yet", nrhs);
     return GNX_ERR_STATE;
@@ -294,6 +338,7 @@ gnx_status gnx_cg_shape(gnx_ctx* c, int nrhs, int* spt, int* xcd) {
 
 gnx_status gnx_set_autotune(gnx_ctx* c, int on) {
   CHECK_ARG(c, "gnx_set_autotune: null context");
+  c->autotune = on ? 1 : 0;
   cgm_set_autotune(&c->cgm, on ? 1 : 0);
   cgmix_set_autotune(&c->cgmix, on ? 1 : 0);
   return GNX_OK;

@@ -179,6 +179,33 @@ GNX_DEVONLY T wave_sum_t(T v) {
 GNX_DEVONLY double wave_sum(double v) { return wave_sum_t(v); }
 GNX_DEVONLY float wave_sumf(float v) { return wave_sum_t(v); }
 
+// The nw wave sums in lds added in wave order.  For the block sizes the
+// kernels use (4, 8 or 16 waves) the count is a constant, so every read
+// issues before the first add; a loop over a run-time count reads, waits
+// and adds one value per trip (3 to 15 LDS round trips in a row).
+template <int NW>
+GNX_DEVONLY double sum_waves_n(const double* lds) {
+  double w[NW];
+#pragma unroll
+  for (int k = 0; k < NW; k++) w[k] = lds[k];
+  double t = w[0];
+#pragma unroll
+  for (int k = 1; k < NW; k++) t += w[k];
+  return t;
+}
+GNX_DEVONLY double sum_waves(const double* lds, int nw) {
+  switch (nw) {
+    case 4: return sum_waves_n<4>(lds);
+    case 8: return sum_waves_n<8>(lds);
+    case 16: return sum_waves_n<16>(lds);
+    default: {
+      double t = lds[0];
+      for (int w = 1; w < nw; w++) t += lds[w];
+      return t;
+    }
+  }
+}
+
 // Deterministic block sum; every thread gets the same bits: the wave sums
 // added in wave order.  lds must hold blockDim.x/64 doubles.  Two
 // __syncthreads(): the first keeps a previous call's readers ahead of this
@@ -189,9 +216,7 @@ GNX_DEVONLY double block_sum(double v, double* lds) {
   __syncthreads();
   if ((threadIdx.x & 63) == 0) lds[wid] = v;
   __syncthreads();
-  double t = lds[0];
-  for (int w = 1; w < nw; w++) t += lds[w];
-  return t;
+  return sum_waves(lds, nw);
 }
 
 // block_sum for an lds array that the kernel uses only once: no leading
@@ -201,9 +226,7 @@ GNX_DEVONLY double block_sum_once(double v, double* lds) {
   v = wave_sum(v);
   if ((threadIdx.x & 63) == 0) lds[wid] = v;
   __syncthreads();
-  double t = lds[0];
-  for (int w = 1; w < nw; w++) t += lds[w];
-  return t;
+  return sum_waves(lds, nw);
 }
 
 // Two deterministic block sums at once (each bit-identical to block_sum of
@@ -218,12 +241,7 @@ GNX_DEVONLY double2 block_sum2(double a, double b, double* lds) {
     lds[nw + wid] = b;
   }
   __syncthreads();
-  double ta = lds[0], tb = lds[nw];
-  for (int w = 1; w < nw; w++) {
-    ta += lds[w];
-    tb += lds[nw + w];
-  }
-  return make_double2(ta, tb);
+  return make_double2(sum_waves(lds, nw), sum_waves(lds + nw, nw));
 }
 
 }  // namespace gnx

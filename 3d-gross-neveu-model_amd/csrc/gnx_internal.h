@@ -93,6 +93,13 @@ size_t cgm_static_capacity();
 hipError_t cgm_f64_run_static(const Geom& g, int device, const double* sm, const double2* b, double2* x, double tol,
                               long max_iter, int start, hipStream_t st, void* pinned, CGDevResult* host_res);
 long cgm_tune_runs(void* machine);
+// FP64 CG of small lattices in one launch per solve, one workgroup per
+// system (gnx_cgs.hip): whether g fits (sites and iteration cap), the solve
+// (results to dres[nrhs] on the device) and its sites per thread
+bool cgs_fits(const Geom& g, long max_iter);
+hipError_t cgs_run(const Geom& g, const double* sm, const double2* b, double2* x, double tol, long max_iter,
+                   int start, int nrhs, CGDevResult* dres, hipStream_t st);
+int cgs_sites_per_thread(const Geom& g);
 void cgmix_set_autotune(void** machine_slot, int on);
 long cgmix_tune_runs(void* machine);
 
@@ -117,6 +124,10 @@ struct gnx_ctx {
   double* bat_part = nullptr;   // per-vector partials of the batched trace terms
   size_t bat_cap = 0;           // doubles in bat_part
   double* bat_out = nullptr;    // 64 device scalars
+  int cg_pinned = -1;           // GNX_CG_SHAPE or GNX_AUTOTUNE set at the first solve (-1: no solve yet)
+  int autotune = 0;             // gnx_set_autotune
+  gnx::CGDevResult* cgs_res = nullptr;  // 64 device results of the one-launch small-lattice solves
+  unsigned long long cgs_seen = 0;      // bit n-1: a batch of n systems took the one-launch solve
 };
 
 

@@ -960,6 +960,70 @@ def schedule_leg(gnx, torch, L, tol, batches, reps=3):
     return out
 
 
+def small_leg(gnx, torch, sizes, tol, batches=(1, 2), reps=20):
+    """Small lattices (configuration 1's 8^3 and its neighbours): the FP64
+    solve in one launch per solve (the fixed rule's choice up to 12^3: one
+    workgroup per system iterating to the exit) against the graph-replayed
+    forms pinned by shape: 1, the two-reduction kernels (the same
+    recurrences), and 7, the one-reduction kernels (the rule's choice at
+    these sizes before the one-launch solve).  Per size, batch and form: us
+    per iteration and per solve (HIP events over `reps` solves on the
+    solver's stream), iterations, and the largest relative difference of the
+    solutions from the one-launch ones."""
+    out = {"workload": "L^3, sigma and complex Z2 sources from the reference's stream (seed 1231), "
+                       f"imposed start, tol {tol:g}"}
+    saved = os.environ.pop("GNX_CG_SHAPE", None)
+    try:
+        for L in sizes:
+            V = L ** 3
+            for nb in batches:
+                sigma_h, B_h = make_inputs(L, nsrc=nb)
+                row, ref = {}, None
+                for name, pin in (("one_launch", None), ("graph_two_reduction_1", "1"),
+                                  ("graph_one_reduction_7", "7")):
+                    if pin is None:
+                        os.environ.pop("GNX_CG_SHAPE", None)
+                    else:
+                        os.environ["GNX_CG_SHAPE"] = pin
+                    ctx = gnx.Context(L)
+                    sm = ctx.field()
+                    ctx.sigma_mass(torch.from_numpy(sigma_h).to(ctx.device), sm)
+                    B = torch.from_numpy(B_h).to(ctx.device)
+                    X = ctx.vec(nb)
+
+                    def solve():
+                        st, res = ctx.cg_batch(sm, B, X, nb, tol, start=gnx.START_IMPOSED)
+                        assert st == 0
+                        return list(res)
+
+                    solve()
+                    torch.cuda.synchronize()
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record(ctx.stream)
+                    for _ in range(reps):
+                        res = solve()
+                    e1.record(ctx.stream)
+                    e1.synchronize()
+                    t = e0.elapsed_time(e1) * 1e-3 / reps
+                    it = max(r.iterations for r in res)
+                    xh = X.cpu()
+                    if ref is None:
+                        ref = xh
+                    diff = max(float((torch.linalg.norm(xh[2 * V * k:2 * V * (k + 1)] - ref[2 * V * k:2 * V * (k + 1)])
+                                      / torch.linalg.norm(ref[2 * V * k:2 * V * (k + 1)])).item())
+                               for k in range(nb))
+                    row[name] = {"us_per_iteration": t / it * 1e6, "us_per_solve": t * 1e6, "iterations": it,
+                                 "max_rel_diff_vs_one_launch": diff, "shape": ctx.cg_shape(nb)}
+                    ctx.close()
+                out[f"{L}^3 x{nb}"] = row
+    finally:
+        if saved is None:
+            os.environ.pop("GNX_CG_SHAPE", None)
+        else:
+            os.environ["GNX_CG_SHAPE"] = saved
+    return out
+
+
 def main():
     t_start = time.perf_counter()
     ap = argparse.ArgumentParser()
@@ -979,6 +1043,8 @@ def main():
     ap.add_argument("--sizes", default="32:32,96:16", metavar="L:N,...",
                     help="configurations 2 and 3 at other sizes: lattice L^3, FP64 single and N-source "
                          "batched FP64 and mixed solves ('' to skip)")
+    ap.add_argument("--small", default="4,8,12", metavar="L,...",
+                    help="small lattices: the one-launch FP64 solve against the graph-replayed forms ('' to skip)")
     ap.add_argument("--no-hmc", dest="hmc", action="store_false",
                     help="skip configuration 1 (sigma-HMC at 8^3, 20 trajectories with measurements)")
     ap.add_argument("--schedules", default="1,16", metavar="N,...",
@@ -1241,6 +1307,8 @@ def main():
                                                   [v for v in args.mixed.split(",") if v])
     if args.hmc:
         result["hmc_8"] = hmc_leg(gnx)
+    if args.small:
+        result["small"] = small_leg(gnx, torch, [int(v) for v in args.small.split(",")], args.tol)
     if args.schedules:
         result["schedules"] = schedule_leg(gnx, torch, L, args.tol, [int(v) for v in args.schedules.split(",")])
         # 48^3 single: the size where the rule runs the fused form (216 tiles, one per CU)

@@ -33,8 +33,9 @@ extern "C" {
  *    gnx_tune_runs); geometries outside the exact site decode are refused;
  *    bit 2 of gnx_cg_shape's xcd_mapped marks the deferred-update kernels
  * 6: bit 3 of xcd_mapped marks the fused kernels (one per iteration);
- *    gnx_multi_link_cg, gnx_multi_cg_prec; link-coupled lockstep chains */
-#define GNX_ABI_VERSION 6
+ *    gnx_multi_link_cg, gnx_multi_cg_prec; link-coupled lockstep chains
+ * 7: bit 4 of xcd_mapped marks the one-launch solve of small lattices */
+#define GNX_ABI_VERSION 7
 
 typedef struct gnx_ctx gnx_ctx;
 typedef struct gnx_multi gnx_multi;
@@ -140,8 +141,10 @@ gnx_status gnx_cg_batch(gnx_ctx *ctx, const double *sm, const double *b, double 
  * static device array of 80 MiB per device that loads the library (held
  * until the process ends), enough for V <= about 950 000 sites (98^3);
  * larger lattices return GNX_ERR_SIZE.  The launch shape is the default
- * rule's, so results equal gnx_cg's without GNX_CG_SHAPE or autotuning.
- * All calls on one device share the array: calls that overlap on one device
+ * rule's, so results equal gnx_cg's without GNX_CG_SHAPE or autotuning (on
+ * small lattices the one-launch solve, which needs no buffer of its own: its
+ * result block sits in the context's reduction scalars, allocated by
+ * gnx_create).  All calls on one device share the array: calls that overlap on one device
  * (other threads, other streams) race. */
 gnx_status gnx_cg_noalloc(gnx_ctx *ctx, const double *sm, const double *b, double *x, double tol, long max_iter,
                           gnx_cg_start start, gnx_cg_result *res);
@@ -225,7 +228,13 @@ gnx_status gnx_tune_runs(gnx_ctx *ctx, long *fp64, long *mixed);
  * 32^3 x 3 and x 4), where it saves a kernel boundary per iteration; with
  * more tiles its halo work costs more than the boundary.  The link-coupled
  * operator's shape is chosen the same way (without the fused form) but not
- * reported. */
+ * reported.  On lattices of at most 1728 sites (12^3), with an iteration cap
+ * of at most 100 000, the fixed rule runs each solve as ONE launch instead:
+ * one 512-thread workgroup per system iterates to the exit with the
+ * direction and M^dag of it in LDS (the two-reduction recurrences, every
+ * global sum a workgroup sum); it reports bit 4 (16) and its sites per
+ * thread (1, 2 or 4).  GNX_CG_SHAPE or autotuning turns it off (they choose
+ * among the graph forms). */
 gnx_status gnx_cg_shape(gnx_ctx *ctx, int nrhs, int *sites_per_thread, int *xcd_mapped);
 /* The same for the mixed-precision solver (prec GNX_PREC_F32 or GNX_PREC_BF16);
  * pair = 1: two systems share each block's site arithmetic; loads_first = 1:

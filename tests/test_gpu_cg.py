@@ -128,7 +128,8 @@ def test_batch_matches_single(torch_dev, P, gnx_mod, shape, monkeypatch):
     st, res = ctx.cg_batch(sm, B, X, 3, 1e-10, start=gnx_mod.START_IMPOSED)
     assert st == 0
     assert ctx.cg_shape(3) == {"sites_per_thread": [1, 1, 2, 2, 4, 4][shape % 6], "xcd_slab": shape % 2 == 1,
-                               "one_reduction": shape >= 6, "deferred": shape >= 12, "fused": False}
+                               "one_reduction": shape >= 6, "deferred": shape >= 12, "fused": False,
+                               "one_launch": False}
     for k in range(3):
         x = ctx.vec()
         s1, r1 = ctx.cg(sm, dev(bs[k]), x, 1e-10, start=gnx_mod.START_IMPOSED)
@@ -171,7 +172,7 @@ def test_fused_shapes(torch_dev, P, gnx_mod, shape, L, monkeypatch):
     st, res = ctx.cg_batch(sm, B, X, 3, 1e-10, start=gnx_mod.START_IMPOSED)
     assert st == 0
     assert ctx.cg_shape(3) == {"sites_per_thread": 2, "xcd_slab": True, "one_reduction": True,
-                               "deferred": shape % 2 == 1, "fused": True}
+                               "deferred": shape % 2 == 1, "fused": True, "one_launch": False}
     for k in range(3):
         x = ctx.vec()
         s1, r1 = ctx.cg(sm, dev(bs[k]), x, 1e-10, start=gnx_mod.START_IMPOSED)
@@ -269,3 +270,75 @@ def test_fused_exits(torch_dev, P, gnx_mod, shape, monkeypatch):
     assert st == 0 and res.early_exit == 1 and res.iterations == 0
     assert float(x0.abs().max().item()) == 0.0
     ctx.close()
+
+
+@pytest.mark.parametrize("L,LT", [(4, 4), (6, 10), (8, 8), (10, 10), (8, 16), (12, 12)])
+def test_small_lattice_one_launch(torch_dev, P, gnx_mod, L, LT, monkeypatch):
+    """Lattices of at most 1728 sites under the fixed rule: each solve is one
+    launch, one workgroup per system (1, 2 or 4 sites per thread by volume).
+    Against the oracle, both starts: iterations within +-1, solutions within
+    1e-7, true residual below 10 tol; a batch of 3 is bit-identical to 3
+    single solves; the null start's prologue exit leaves x untouched, the
+    imposed start exits before the loop at b = 0 (x = 0, no iteration), and
+    the iteration cap (7) stops with the oracle's 7th iterate within 1e-12.
+    A pinned graph shape (GNX_CG_SHAPE) turns the one-launch solve off."""
+    banner(f"cg one-launch small lattice {L}^2 x {LT}")
+    monkeypatch.delenv("GNX_CG_SHAPE", raising=False)
+    monkeypatch.delenv("GNX_AUTOTUNE", raising=False)
+    lat = P.Lattice(L, LT)
+    r = P.Random(300 + L + LT)
+    smh = lat.sigma_mass(r.sigma(lat.V, P.sigma_variance(0.85)))
+    sm = dev(smh)
+    bs = [r.z2_complex(lat.V) for _ in range(3)]
+    x0 = 0.01 * r.z2_complex(lat.V)
+    ctx = gnx_mod.Context(L, LT)
+    B = dev(np.concatenate(bs))
+    X = ctx.vec(3)
+    st, res = ctx.cg_batch(sm, B, X, 3, 1e-10, start=gnx_mod.START_IMPOSED)
+    assert st == 0
+    spt = 1 if lat.V <= 512 else 2 if lat.V <= 1024 else 4
+    assert ctx.cg_shape(3) == {"sites_per_thread": spt, "xcd_slab": False, "one_reduction": False,
+                               "deferred": False, "fused": False, "one_launch": True}
+    for k in range(3):
+        x = ctx.vec()
+        s1, r1 = ctx.cg(sm, dev(bs[k]), x, 1e-10, start=gnx_mod.START_IMPOSED)
+        assert s1 == 0 and r1.iterations == res[k].iterations and r1.residual == res[k].residual
+        assert np.array_equal(host(x), host(X[k * 2 * lat.V:(k + 1) * 2 * lat.V]))
+        xr, ok, stc = lat.cg(smh, bs[k], None, tol=1e-10, start="imposed")
+        assert ok and abs(res[k].iterations - stc.last_iters) <= 1, (k, res[k].iterations, stc.last_iters)
+        assert rel(host(x), xr) < 1e-7
+        assert true_residual(lat, smh, bs[k], host(x)) < 1e-9
+    # the null start from a nonzero guess, then its prologue exit
+    xn = dev(x0)
+    sn, rn = ctx.cg(sm, dev(bs[1]), xn, 1e-10, start=gnx_mod.START_NULL)
+    xo, ok, stn = lat.cg(smh, bs[1], x0.copy(), tol=1e-10, start="null")
+    assert sn == 0 and rn.early_exit == 0 and ok and abs(rn.iterations - stn.last_iters) <= 1
+    assert rel(host(xn), xo) < 1e-7
+    xc = host(xn).copy()
+    sp, rp = ctx.cg(sm, dev(bs[1]), xn, 1e-10, start=gnx_mod.START_NULL)
+    assert sp == 0 and rp.early_exit == 1 and rp.iterations == 0 and rp.converged == 1
+    assert np.array_equal(host(xn), xc)
+    # the iteration cap
+    xcap = ctx.vec()
+    sc, rc = ctx.cg(sm, dev(bs[2]), xcap, 1e-10, max_iter=7, start=gnx_mod.START_IMPOSED, check=False)
+    assert sc == 3 and rc.iterations == 7 and rc.converged == 0
+    xo7, ok7, _ = lat.cg(smh, bs[2], None, tol=1e-10, max_iter=7, start="imposed")
+    assert not ok7
+    assert np.linalg.norm(host(xcap) - xo7) <= 1e-12 * np.linalg.norm(xo7)
+    # b = 0: the imposed start exits before the loop with x = 0
+    z = ctx.vec()
+    xz = ctx.vec()
+    xz += 1.0
+    sz, rz = ctx.cg(sm, z, xz, 1e-10, start=gnx_mod.START_IMPOSED)
+    assert sz == 0 and rz.early_exit == 1 and rz.iterations == 0
+    assert float(xz.abs().max().item()) == 0.0
+    ctx.close()
+    # a pinned shape runs the graph form, to the same tolerance
+    monkeypatch.setenv("GNX_CG_SHAPE", "1")
+    ctx2 = gnx_mod.Context(L, LT)
+    x2 = ctx2.vec()
+    s2, r2 = ctx2.cg(sm, dev(bs[0]), x2, 1e-10, start=gnx_mod.START_IMPOSED)
+    assert s2 == 0 and not ctx2.cg_shape(1)["one_launch"]
+    assert abs(r2.iterations - res[0].iterations) <= 1
+    assert rel(host(x2), host(X[:2 * lat.V])) < 1e-7
+    ctx2.close()

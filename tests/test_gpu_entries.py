@@ -140,7 +140,7 @@ def test_shape_rule_is_default_and_deterministic(P, gnx_mod):
         torch.cuda.synchronize()
         assert ctx.tune_runs() == {"fp64": 0, "mixed": 0}
         assert ctx.cg_shape(1) == {"sites_per_thread": 2, "xcd_slab": True, "one_reduction": True,
-                                   "deferred": False, "fused": False}
+                                   "deferred": False, "fused": False, "one_launch": False}
         assert ctx.cg_mixed_shape(1, gnx_mod.PREC_F16) == {"sites_per_thread": 2, "xcd_slab": True, "pair": False,
                                                            "loads_first": True}
         out.append((host(x), host(xm)))

@@ -34,8 +34,8 @@ def test_noalloc_matches_gnx_cg(P, gnx_mod, monkeypatch):
     import torch
     # each call on a smaller lattice than the one before; 48^3 runs the fused
     # form (one kernel per iteration, the rule's choice there), 32^3 and
-    # 16^3 the one-reduction pair
-    for L, seed in ((48, 70), (32, 71), (16, 72)):
+    # 16^3 the one-reduction pair, 8^3 the one-launch solve
+    for L, seed in ((48, 70), (32, 71), (16, 72), (8, 74)):
         for start in (gnx_mod.START_IMPOSED, gnx_mod.START_NULL):
             banner(f"cg_noalloc {L}^3 start={start} vs gnx_cg")
             lat, sm, b, x0 = _inputs(P, L, seed)
