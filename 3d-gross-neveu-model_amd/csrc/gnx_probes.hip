@@ -21,12 +21,23 @@
 //                           spins past the limit raises a flag, and every
 //                           block leaves the loop at the next barrier once
 //                           the flag is up, so the kernel always ends.
+//   gnx_probe_halo          the communication of a persistent CG iteration
+//   gnx_probe_halo_tag      with no grid barrier: neighbour-face exchanges
+//                           through per-block flags (or round-tagged data
+//                           words) and an all-block gather of partial sums,
+//                           one block per 16 x 8 x 8 tile, every wait
+//                           bounded, every received value checked.
 //
 // C ABI (host functions; all return hipError_t as int, 0 on success):
 //   int gnx_probe_graph_nodes(int kind, long sites, int nodes, int reps, double* us_per_node);
 //   int gnx_probe_grid_barrier(int mode, int blocks, int iters, long bytes_per_block,
 //                              double* us_per_barrier, int* max_blocks, int* failed);
 //   mode 0: every poll an acquire load; mode 1: relaxed polls, one acquire fence
+//   int gnx_probe_halo(int mode, int nt0, int nt1, int nt2, int iters, long lds_pad,
+//                      double* us_per_round, int* failed, int* stale, int* per_cu);
+//   int gnx_probe_halo_tag(int red, int nt0, int nt1, int nt2, int iters, long lds_pad,
+//                          double* us_per_round, int* failed, int* stale, int* per_cu);
+//   (lds_pad: dynamic LDS per block, which sets how many blocks share a CU)
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -147,6 +158,314 @@ __global__ __launch_bounds__(kBS) void k_barrier_loop(BarrierState* bs, int iter
     __syncthreads();
     if (stop) return;
   }
+}
+
+// ---- halo-exchange probe: the communication of a persistent CG iteration --
+// One block per tile of a periodic nt0 x nt1 x nt2 grid of 16 x 8 x 8-site
+// tiles (1024 threads, one per site).  Per round k, as a persistent CG on
+// M M^dag would do it: publish the tile's six faces of r (640 complex
+// values, two 8-byte device-scope stores each) and raise rflag = k; wait
+// for the six neighbours' rflag >= k and read their faces; publish the q
+// faces and a partial pair and raise qflag = k; wait for the neighbours'
+// qflag, read their q faces, then wait for every block's qflag and read all
+// partials.  Every value carries (k, block, index), and the reader counts
+// values that are not the ones of round k (stale).  Double-buffered by the
+// parity of k.  Flags are only written by their block and only grow.
+//   mode bit 0: the writer's ordering before a flag is a device-scope
+//               release fence (L2 write-back and wait) instead of a plain
+//               wait for its stores (s_waitcnt vmcnt(0));
+//   mode bit 1: the reader issues a device-scope acquire fence after a
+//               wait (L2 invalidate) before it reads the faces.
+// Every wait is bounded as in the barrier probe.
+constexpr int kHB = 1024;
+constexpr int kFace = 640;  // face sites of a 16 x 8 x 8 tile: 2 (64 + 128 + 128)
+constexpr long kHaloSpin = 1L << 20;
+
+struct HaloArgs {
+  unsigned* rflag;           // [blocks]
+  unsigned* qflag;           // [blocks]
+  unsigned long long* rface;  // [2][blocks][kFace][2] bit patterns of (re, im)
+  unsigned long long* qface;
+  unsigned long long* part;   // [2][blocks][2]
+  int* failed;
+  int* stale;
+  int nt0, nt1, nt2;
+};
+
+// face of halo slot i: 0/1 = x0 low/high (64 sites), 2/3 = x1 (128), 4/5 = x2 (128)
+__device__ __forceinline__ int face_of(int i) { return i < 128 ? (i >> 6) : 2 + (i - 128) / 128; }
+__device__ __forceinline__ int face_off(int f) { return f < 2 ? 64 * f : 128 + 128 * (f - 2); }
+
+__device__ __forceinline__ void st_dev(unsigned long long* p, double v) {
+  __hip_atomic_store(p, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double ld_dev(const unsigned long long* p) {
+  return __longlong_as_double((long long)__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+template <bool WB>
+__device__ __forceinline__ void publish_order() {
+  if (WB)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  else
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+// bounded wait for *f >= k; false once this or another block gave up
+__device__ bool wait_ge(const unsigned* f, unsigned k, int* failed) {
+  long spins = 0;
+  while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < k) {
+    if (++spins > kHaloSpin) {
+      __hip_atomic_store(failed, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    if ((spins & 255) == 0 && __hip_atomic_load(failed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return true;
+}
+
+__device__ __forceinline__ double tagv(int k, int b, int i, int kind) {
+  return (double)k * 1e7 + (double)b * 2000.0 + (double)i + (kind ? 0.5 : 0.0);
+}
+
+template <bool WB, bool INV>
+__global__ __launch_bounds__(kHB) void k_halo_loop(HaloArgs a, int iters) {
+  __shared__ double2 halo[2 * kFace];
+  __shared__ int stop;
+  __shared__ int bad;
+  const int b = blockIdx.x, tid = threadIdx.x, nblk = gridDim.x;
+  const int c0 = b / (a.nt1 * a.nt2), c1 = (b / a.nt2) % a.nt1, c2 = b % a.nt2;
+  auto id = [&](int x, int y, int z) {
+    x = (x + a.nt0) % a.nt0;
+    y = (y + a.nt1) % a.nt1;
+    z = (z + a.nt2) % a.nt2;
+    return (x * a.nt1 + y) * a.nt2 + z;
+  };
+  // neighbour across face f: the block whose face f^1 touches this tile
+  const int nb[6] = {id(c0 - 1, c1, c2), id(c0 + 1, c1, c2), id(c0, c1 - 1, c2),
+                     id(c0, c1 + 1, c2), id(c0, c1, c2 - 1), id(c0, c1, c2 + 1)};
+  if (tid == 0) bad = 0;
+  int mybad = 0;
+  for (int k = 1; k <= iters; k++) {
+    const int s = k & 1;
+    for (int i = tid; i < kFace; i += kHB) {
+      unsigned long long* d = a.rface + (((size_t)s * nblk + b) * kFace + i) * 2;
+      st_dev(d, tagv(k, b, i, 0));
+      st_dev(d + 1, -tagv(k, b, i, 0));
+    }
+    publish_order<WB>();
+    __syncthreads();
+    if (tid == 0) {
+      stop = 0;
+      __hip_atomic_store(a.rflag + b, (unsigned)k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    if (tid < 6 && !wait_ge(a.rflag + nb[tid], (unsigned)k, a.failed)) stop = 1;
+    __syncthreads();
+    if (stop) break;
+    if (INV) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    for (int i = tid; i < kFace; i += kHB) {
+      const int f = face_of(i), src = nb[f], j = face_off(f ^ 1) + (i - face_off(f));
+      const unsigned long long* p = a.rface + (((size_t)s * nblk + src) * kFace + j) * 2;
+      const double re = ld_dev(p), im = ld_dev(p + 1);
+      if (re != tagv(k, src, j, 0) || im != -tagv(k, src, j, 0)) mybad++;
+      halo[i] = make_double2(re, im);
+    }
+    // q faces and the partial pair, behind one flag
+    for (int i = tid; i < kFace; i += kHB) {
+      unsigned long long* d = a.qface + (((size_t)s * nblk + b) * kFace + i) * 2;
+      st_dev(d, tagv(k, b, i, 1) + halo[i].x * 0.0);
+      st_dev(d + 1, -tagv(k, b, i, 1));
+    }
+    if (tid == 0) {
+      st_dev(a.part + ((size_t)s * nblk + b) * 2, tagv(k, b, 0, 1));
+      st_dev(a.part + ((size_t)s * nblk + b) * 2 + 1, tagv(k, b, 1, 1));
+    }
+    publish_order<WB>();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(a.qflag + b, (unsigned)k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid < 6 && !wait_ge(a.qflag + nb[tid], (unsigned)k, a.failed)) stop = 1;
+    __syncthreads();
+    if (stop) break;
+    if (INV) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    for (int i = tid; i < kFace; i += kHB) {
+      const int f = face_of(i), src = nb[f], j = face_off(f ^ 1) + (i - face_off(f));
+      const unsigned long long* p = a.qface + (((size_t)s * nblk + src) * kFace + j) * 2;
+      const double re = ld_dev(p), im = ld_dev(p + 1);
+      if (re != tagv(k, src, j, 1) || im != -tagv(k, src, j, 1)) mybad++;
+      halo[kFace + i] = make_double2(re, im);
+    }
+    if (tid < nblk && !wait_ge(a.qflag + tid, (unsigned)k, a.failed)) stop = 1;
+    __syncthreads();
+    if (stop) break;
+    if (INV) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (tid < nblk) {
+      const unsigned long long* p = a.part + ((size_t)s * nblk + tid) * 2;
+      if (ld_dev(p) != tagv(k, tid, 0, 1) || ld_dev(p + 1) != tagv(k, tid, 1, 1)) mybad++;
+    }
+    __syncthreads();
+  }
+  if (mybad) atomicAdd(&bad, mybad);
+  __syncthreads();
+  if (tid == 0 && bad) atomicAdd(a.stale, bad);
+}
+
+// ---- tagged-word exchange: no flags and no fences --------------------------
+// The same rounds as k_halo_loop, with every double sent as two 64-bit words
+// (tag << 32 | high half, tag << 32 | low half), tag = round.  A 64-bit
+// device-scope store is single-copy atomic, so a reader that polls a word
+// until it carries the round's tag has that word's half of the value: no
+// flag, no wait for store completion and no separate data load.  The
+// partial pairs: RED 0 none; RED 1 every block polls all blocks' pairs; RED
+// 2 two levels, the first block of each group of 16 polls its group's
+// pairs and publishes a group pair, and every block polls the group pairs.
+constexpr int kGroup = 16;
+
+__device__ __forceinline__ void st_tag(unsigned long long* p, unsigned tag, double v) {
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+  const unsigned long long t = (unsigned long long)tag << 32;
+  __hip_atomic_store(p, t | (bits >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p + 1, t | (bits & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// bounded poll of n tagged doubles at p (2n words) until all carry tag
+template <int N>
+__device__ bool ld_tag(const unsigned long long* p, unsigned tag, double* v, int* failed) {
+  unsigned long long w[2 * N];
+  long spins = 0;
+  while (true) {
+#pragma unroll
+    for (int j = 0; j < 2 * N; j++) w[j] = __hip_atomic_load(p + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < 2 * N; j++) ok = ok && (unsigned)(w[j] >> 32) == tag;
+    if (ok) break;
+    if (++spins > kHaloSpin) {
+      __hip_atomic_store(failed, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    if ((spins & 255) == 0 && __hip_atomic_load(failed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+    __builtin_amdgcn_s_sleep(1);
+  }
+#pragma unroll
+  for (int j = 0; j < N; j++) v[j] = __hiloint2double((int)(unsigned)w[2 * j], (int)(unsigned)w[2 * j + 1]);
+  return true;
+}
+
+struct TagArgs {
+  unsigned long long* rface;  // [2][blocks][kFace][4]
+  unsigned long long* qface;
+  unsigned long long* part;   // [2][blocks][4]
+  unsigned long long* gpart;  // [2][groups][4]
+  int* failed;
+  int* stale;
+  int nt0, nt1, nt2;
+};
+
+template <int RED>
+__global__ __launch_bounds__(kHB) void k_halo_tag(TagArgs a, int iters) {
+  __shared__ double2 halo[2 * kFace];
+  __shared__ int stop;
+  __shared__ int bad;
+  const int b = blockIdx.x, tid = threadIdx.x, nblk = gridDim.x;
+  const int ngroups = (nblk + kGroup - 1) / kGroup;
+  const int c0 = b / (a.nt1 * a.nt2), c1 = (b / a.nt2) % a.nt1, c2 = b % a.nt2;
+  auto id = [&](int x, int y, int z) {
+    x = (x + a.nt0) % a.nt0;
+    y = (y + a.nt1) % a.nt1;
+    z = (z + a.nt2) % a.nt2;
+    return (x * a.nt1 + y) * a.nt2 + z;
+  };
+  const int nb[6] = {id(c0 - 1, c1, c2), id(c0 + 1, c1, c2), id(c0, c1 - 1, c2),
+                     id(c0, c1 + 1, c2), id(c0, c1, c2 - 1), id(c0, c1, c2 + 1)};
+  if (tid == 0) {
+    bad = 0;
+    stop = 0;
+  }
+  __syncthreads();
+  int mybad = 0;
+  for (int k = 1; k <= iters; k++) {
+    const int s = k & 1;
+    const unsigned tag = (unsigned)k;
+    for (int i = tid; i < kFace; i += kHB) {
+      unsigned long long* d = a.rface + (((size_t)s * nblk + b) * kFace + i) * 4;
+      st_tag(d, tag, tagv(k, b, i, 0));
+      st_tag(d + 2, tag, -tagv(k, b, i, 0));
+    }
+    for (int i = tid; i < kFace; i += kHB) {
+      const int f = face_of(i), src = nb[f], j = face_off(f ^ 1) + (i - face_off(f));
+      double v[2];
+      if (!ld_tag<2>(a.rface + (((size_t)s * nblk + src) * kFace + j) * 4, tag, v, a.failed)) {
+        stop = 1;
+        break;
+      }
+      if (v[0] != tagv(k, src, j, 0) || v[1] != -tagv(k, src, j, 0)) mybad++;
+      halo[i] = make_double2(v[0], v[1]);
+    }
+    __syncthreads();
+    if (stop) break;
+    for (int i = tid; i < kFace; i += kHB) {
+      unsigned long long* d = a.qface + (((size_t)s * nblk + b) * kFace + i) * 4;
+      st_tag(d, tag, tagv(k, b, i, 1) + halo[i].x * 0.0);
+      st_tag(d + 2, tag, -tagv(k, b, i, 1));
+    }
+    if (RED && tid == 0) {
+      st_tag(a.part + ((size_t)s * nblk + b) * 4, tag, tagv(k, b, 0, 1));
+      st_tag(a.part + ((size_t)s * nblk + b) * 4 + 2, tag, tagv(k, b, 1, 1));
+    }
+    for (int i = tid; i < kFace; i += kHB) {
+      const int f = face_of(i), src = nb[f], j = face_off(f ^ 1) + (i - face_off(f));
+      double v[2];
+      if (!ld_tag<2>(a.qface + (((size_t)s * nblk + src) * kFace + j) * 4, tag, v, a.failed)) {
+        stop = 1;
+        break;
+      }
+      if (v[0] != tagv(k, src, j, 1) || v[1] != -tagv(k, src, j, 1)) mybad++;
+      halo[kFace + i] = make_double2(v[0], v[1]);
+    }
+    if (RED == 1 && tid < nblk) {
+      double v[2];
+      if (!ld_tag<2>(a.part + ((size_t)s * nblk + tid) * 4, tag, v, a.failed))
+        stop = 1;
+      else if (v[0] != tagv(k, tid, 0, 1) || v[1] != tagv(k, tid, 1, 1))
+        mybad++;
+    }
+    if (RED == 2) {
+      const int g = b / kGroup;
+      if (b % kGroup == 0) {
+        // group leader: its members' pairs, then the group pair
+        __shared__ double gs[2];
+        if (tid == 0) gs[0] = gs[1] = 0.0;
+        __syncthreads();
+        const int m = g * kGroup + tid;
+        if (tid < kGroup && m < nblk) {
+          double v[2];
+          if (!ld_tag<2>(a.part + ((size_t)s * nblk + m) * 4, tag, v, a.failed))
+            stop = 1;
+          else if (v[0] != tagv(k, m, 0, 1) || v[1] != tagv(k, m, 1, 1))
+            mybad++;
+        }
+        __syncthreads();
+        if (tid == 0 && !stop) {
+          st_tag(a.gpart + ((size_t)s * ngroups + g) * 4, tag, tagv(k, g, 0, 0));
+          st_tag(a.gpart + ((size_t)s * ngroups + g) * 4 + 2, tag, tagv(k, g, 1, 0));
+        }
+      }
+      if (tid < ngroups) {
+        double v[2];
+        if (!ld_tag<2>(a.gpart + ((size_t)s * ngroups + tid) * 4, tag, v, a.failed))
+          stop = 1;
+        else if (v[0] != tagv(k, tid, 0, 0) || v[1] != tagv(k, tid, 1, 0))
+          mybad++;
+      }
+    }
+    __syncthreads();
+    if (stop) break;
+  }
+  if (mybad) atomicAdd(&bad, mybad);
+  __syncthreads();
+  if (tid == 0 && bad) atomicAdd(a.stale, bad);
 }
 
 int elapsed_us(hipEvent_t a, hipEvent_t b, double* us) {
@@ -282,6 +601,154 @@ int gnx_probe_grid_barrier(int mode, int blocks, int iters, long bytes_per_block
   if (e1) (void)hipEventDestroy(e1);
   if (bs) (void)hipFree(bs);
   if (slice) (void)hipFree(slice);
+  if (st) (void)hipStreamDestroy(st);
+  return (int)e;
+}
+
+int gnx_probe_halo(int mode, int nt0, int nt1, int nt2, int iters, long lds_pad, double* us_per_round, int* failed,
+                   int* stale, int* per_cu) {
+  if (mode < 0 || mode > 3 || nt0 < 1 || nt1 < 1 || nt2 < 1 || iters < 1 || lds_pad < 0 || !us_per_round ||
+      !failed || !stale || !per_cu)
+    return (int)hipErrorInvalidValue;
+  const int blocks = nt0 * nt1 * nt2;
+  const void* kerns[4] = {(const void*)k_halo_loop<false, false>, (const void*)k_halo_loop<true, false>,
+                          (const void*)k_halo_loop<false, true>, (const void*)k_halo_loop<true, true>};
+  const void* kern = kerns[mode];
+  int dev = 0, cus = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess && lds_pad > 0)
+    e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_pad);
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, kern, kHB, (size_t)lds_pad);
+  if (e != hipSuccess) return (int)e;
+  // every block must be resident at once (one per CU), or the waits could
+  // never end; the partial wait has one thread per block
+  if (blocks > cus * *per_cu || blocks > kHB) return (int)hipErrorInvalidConfiguration;
+  HaloArgs a = {};
+  a.nt0 = nt0;
+  a.nt1 = nt1;
+  a.nt2 = nt2;
+  const size_t face_words = (size_t)2 * blocks * kFace * 2;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&a.rflag, sizeof(unsigned) * blocks);
+  if (e == hipSuccess) e = hipMalloc(&a.qflag, sizeof(unsigned) * blocks);
+  if (e == hipSuccess) e = hipMalloc(&a.rface, sizeof(unsigned long long) * face_words);
+  if (e == hipSuccess) e = hipMalloc(&a.qface, sizeof(unsigned long long) * face_words);
+  if (e == hipSuccess) e = hipMalloc(&a.part, sizeof(unsigned long long) * 4 * blocks);
+  if (e == hipSuccess) e = hipMalloc(&a.failed, sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&a.stale, sizeof(int));
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  double us[3] = {0.0, 0.0, 0.0};
+  int rounds[3] = {1, 1, iters + 1};
+  int fail = 0, nstale = 0;
+  for (int k = 0; k < 3 && e == hipSuccess; k++) {
+    e = hipMemsetAsync(a.rflag, 0, sizeof(unsigned) * blocks, st);
+    if (e == hipSuccess) e = hipMemsetAsync(a.qflag, 0, sizeof(unsigned) * blocks, st);
+    if (e == hipSuccess) e = hipMemsetAsync(a.failed, 0, sizeof(int), st);
+    if (e == hipSuccess) e = hipMemsetAsync(a.stale, 0, sizeof(int), st);
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    void* args[] = {(void*)&a, (void*)&rounds[k]};
+    if (e == hipSuccess) e = hipLaunchKernel(kern, dim3(blocks), dim3(kHB), args, (size_t)lds_pad, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = (hipError_t)elapsed_us(e0, e1, &us[k]);
+    int f = 0, sv = 0;
+    if (e == hipSuccess) e = hipMemcpy(&f, a.failed, sizeof f, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(&sv, a.stale, sizeof sv, hipMemcpyDeviceToHost);
+    fail |= f;
+    nstale += sv;
+  }
+  if (e == hipSuccess) {
+    *failed = fail;
+    *stale = nstale;
+    *us_per_round = (us[2] - us[1]) / iters;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(a.rflag);
+  (void)hipFree(a.qflag);
+  (void)hipFree(a.rface);
+  (void)hipFree(a.qface);
+  (void)hipFree(a.part);
+  (void)hipFree(a.failed);
+  (void)hipFree(a.stale);
+  if (st) (void)hipStreamDestroy(st);
+  return (int)e;
+}
+
+int gnx_probe_halo_tag(int red, int nt0, int nt1, int nt2, int iters, long lds_pad, double* us_per_round,
+                       int* failed, int* stale, int* per_cu) {
+  if (red < 0 || red > 2 || nt0 < 1 || nt1 < 1 || nt2 < 1 || iters < 1 || lds_pad < 0 || !us_per_round ||
+      !failed || !stale || !per_cu)
+    return (int)hipErrorInvalidValue;
+  const int blocks = nt0 * nt1 * nt2;
+  const int ngroups = (blocks + kGroup - 1) / kGroup;
+  const void* kerns[3] = {(const void*)k_halo_tag<0>, (const void*)k_halo_tag<1>, (const void*)k_halo_tag<2>};
+  const void* kern = kerns[red];
+  int dev = 0, cus = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess && lds_pad > 0)
+    e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_pad);
+  if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, kern, kHB, (size_t)lds_pad);
+  if (e != hipSuccess) return (int)e;
+  if (blocks > cus * *per_cu || blocks > kHB || ngroups > kHB) return (int)hipErrorInvalidConfiguration;
+  TagArgs a = {};
+  a.nt0 = nt0;
+  a.nt1 = nt1;
+  a.nt2 = nt2;
+  const size_t face_words = (size_t)2 * blocks * kFace * 4;
+  const size_t part_words = (size_t)2 * blocks * 4, gpart_words = (size_t)2 * ngroups * 4;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&a.rface, sizeof(unsigned long long) * face_words);
+  if (e == hipSuccess) e = hipMalloc(&a.qface, sizeof(unsigned long long) * face_words);
+  if (e == hipSuccess) e = hipMalloc(&a.part, sizeof(unsigned long long) * part_words);
+  if (e == hipSuccess) e = hipMalloc(&a.gpart, sizeof(unsigned long long) * gpart_words);
+  if (e == hipSuccess) e = hipMalloc(&a.failed, sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&a.stale, sizeof(int));
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  double us[3] = {0.0, 0.0, 0.0};
+  int rounds[3] = {1, 1, iters + 1};
+  int fail = 0, nstale = 0;
+  for (int k = 0; k < 3 && e == hipSuccess; k++) {
+    // tag 0 matches no round
+    e = hipMemsetAsync(a.rface, 0, sizeof(unsigned long long) * face_words, st);
+    if (e == hipSuccess) e = hipMemsetAsync(a.qface, 0, sizeof(unsigned long long) * face_words, st);
+    if (e == hipSuccess) e = hipMemsetAsync(a.part, 0, sizeof(unsigned long long) * part_words, st);
+    if (e == hipSuccess) e = hipMemsetAsync(a.gpart, 0, sizeof(unsigned long long) * gpart_words, st);
+    if (e == hipSuccess) e = hipMemsetAsync(a.failed, 0, sizeof(int), st);
+    if (e == hipSuccess) e = hipMemsetAsync(a.stale, 0, sizeof(int), st);
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    void* args[] = {(void*)&a, (void*)&rounds[k]};
+    if (e == hipSuccess) e = hipLaunchKernel(kern, dim3(blocks), dim3(kHB), args, (size_t)lds_pad, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = (hipError_t)elapsed_us(e0, e1, &us[k]);
+    int f = 0, sv = 0;
+    if (e == hipSuccess) e = hipMemcpy(&f, a.failed, sizeof f, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(&sv, a.stale, sizeof sv, hipMemcpyDeviceToHost);
+    fail |= f;
+    nstale += sv;
+  }
+  if (e == hipSuccess) {
+    *failed = fail;
+    *stale = nstale;
+    *us_per_round = (us[2] - us[1]) / iters;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(a.rface);
+  (void)hipFree(a.qface);
+  (void)hipFree(a.part);
+  (void)hipFree(a.gpart);
+  (void)hipFree(a.failed);
+  (void)hipFree(a.stale);
   if (st) (void)hipStreamDestroy(st);
   return (int)e;
 }

@@ -368,6 +368,26 @@ def probes_leg(L=64, barrier=True):
             d["GB_s"] = bps * sites / (us.value * 1e-6) / 1e9
             d["sites"] = sites
         out["graph_nodes"][name] = d
+    # the communication of a persistent CG iteration without a grid barrier:
+    # one 1024-thread block per 16x8x8 tile (tiles x 1024 sites), two
+    # neighbour-face exchanges and one all-block partial gather per round
+    out["halo_exchange"] = []
+    forms = [("flags", lib.gnx_probe_halo, 0, "device-scope stores, wait for them, neighbour flags"),
+             ("flags+release", lib.gnx_probe_halo, 1, "as flags, device-scope release fence before a flag"),
+             ("flags+acquire", lib.gnx_probe_halo, 2, "as flags, device-scope acquire fence after a wait"),
+             ("tagged", lib.gnx_probe_halo_tag, 0, "round-tagged 64-bit words, no flags, no partials"),
+             ("tagged+gather", lib.gnx_probe_halo_tag, 1, "tagged words, every block polls every partial"),
+             ("tagged+tree", lib.gnx_probe_halo_tag, 2, "tagged words, partials through 16-block group leaders")]
+    for tiles in ((4, 8, 8), (2, 4, 4)):
+        for name, fn, mode, what in forms:
+            us, failed, stale, per_cu = C.c_double(), C.c_int(), C.c_int(), C.c_int()
+            rc = fn(mode, tiles[0], tiles[1], tiles[2], 200, C.c_long(98304), C.byref(us), C.byref(failed),
+                    C.byref(stale), C.byref(per_cu))
+            out["halo_exchange"].append({"form": name, "what": what, "blocks": tiles[0] * tiles[1] * tiles[2],
+                                         "sites": tiles[0] * tiles[1] * tiles[2] * 1024,
+                                         "us_per_round": us.value if rc == 0 else None,
+                                         "wait_timed_out": bool(failed.value), "stale_values": stale.value,
+                                         "blocks_per_cu": per_cu.value, "rc": rc})
     if not barrier:
         return out
     for mode, poll in ((0, "acquire load per poll"), (1, "relaxed polls, one acquire fence")):
