@@ -5,7 +5,10 @@
 // Why kernels and not a persistent kernel: on gfx950 a grid-wide barrier
 // inside a kernel costs 10-80 us (cooperative grid sync or an atomic counter,
 // growing with the block count), a kernel node inside a hipGraph about
-// 1.6 us.  Standard CG needs two global reductions per iteration, so each
+// 1.7 us, and even the neighbour-face and partial-sum traffic of one
+// persistent iteration alone, with per-tile flags and no barrier or L2
+// fence, costs 8.3 us at 64^3 (the halo-exchange probes of gnx_probes.hip).
+// Standard CG needs two global reductions per iteration, so each
 // iteration is two kernels whose boundaries are the barriers:
 //
 //   A(k): rho_k from the partials of the previous B (or of the start kernel);
@@ -219,6 +222,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_A(CGMFix a, int odd, const CGMSt
   head_issue<WH>(pin + (size_t)rhs * a.nb, a.nb, pl);
   end_of_loads();
   int fs[SPT];
+  SiteCode kc[SPT];
   double smv[SPT];
   double2 rv[SPT][7], pv[SPT][7];
   double2 Uo[LINK ? SPT : 1][3], Um[LINK ? SPT : 1][3];
@@ -228,7 +232,9 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_A(CGMFix a, int odd, const CGMSt
     // a thread past the last site loads site 0 and stores nothing: the loads
     // stay one branch-free block, so none of them waits for another
     const int fc = fs[i] < 0 ? 0 : fs[i];
-    NbrF n = neighbours_f(g, decode(g, fc));
+    const SiteGeo sgeo = site_geo(g, fc);
+    const NbrF n = sgeo.n;
+    kc[i] = sgeo.k;
     int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
     smv[i] = *elem(a.sm, fc);
     if (LINK) load_links(a.U, fc, n, Uo[LINK ? i : 0], Um[LINK ? i : 0]);
@@ -274,7 +280,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_A(CGMFix a, int odd, const CGMSt
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
     if (fs[i] < 0) continue;
-    const Site st = decode(g, fs[i]);
+    const SiteCode st = kc[i];
     // p_new at the site and its neighbours, from the preloaded r and p_old
     const double2 p0v = cadd(rv[i][0], rmul(beta, pv[i][0]));
     double2 ppv[3], pmv[3];
@@ -284,8 +290,8 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_A(CGMFix a, int odd, const CGMSt
       pmv[d] = cadd(rv[i][4 + d], rmul(beta, pv[i][4 + d]));
     }
     st_nt(elem(pnew, fs[i]), p0v);
-    double2 tt = LINK ? link_hops<-1>(g, st, rmul(smv[i], p0v), ppv, pmv, Uo[LINK ? i : 0], Um[LINK ? i : 0])
-                      : apply_Mdag_fast_vals(g, st, smv[i], p0v, ppv, pmv);
+    double2 tt = LINK ? link_hops<-1>(st, rmul(smv[i], p0v), ppv, pmv, Uo[LINK ? i : 0], Um[LINK ? i : 0])
+                      : apply_Mdag_fast_vals(st, smv[i], p0v, ppv, pmv);
     st_nt(elem(a.t + off, fs[i]), tt);
     acc += cnorm2(tt);
   }
@@ -314,6 +320,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_B(CGMFix a, int odd, const CGMSt
   head_issue<WH>(pin + (size_t)rhs * a.nb, a.nb, pl);
   end_of_loads();
   int fs[SPT];
+  SiteCode kc[SPT];
   double smv[SPT];
   double2 tv[SPT][7], pv[SPT], xv[SPT], rv[SPT];
   double2 Uo[LINK ? SPT : 1][3], Um[LINK ? SPT : 1][3];
@@ -323,7 +330,9 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_B(CGMFix a, int odd, const CGMSt
     // a thread past the last site loads site 0 and stores nothing: the loads
     // stay one branch-free block, so none of them waits for another
     const int fc = fs[i] < 0 ? 0 : fs[i];
-    NbrF n = neighbours_f(g, decode(g, fc));
+    const SiteGeo sgeo = site_geo(g, fc);
+    const NbrF n = sgeo.n;
+    kc[i] = sgeo.k;
     int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
     smv[i] = *elem(a.sm, fc);
     if (LINK) load_links(a.U, fc, n, Uo[LINK ? i : 0], Um[LINK ? i : 0]);
@@ -346,10 +355,10 @@ __global__ __launch_bounds__(CGM_BS) void k_cgm_B(CGMFix a, int odd, const CGMSt
   for (int i = 0; i < SPT; i++) {
     if (fs[i] < 0) continue;
     // M t from the preloaded operands, the order of apply_M / apply_link
-    const Site st = decode(g, fs[i]);
+    const SiteCode st = kc[i];
     const double2 m0 = rmul(smv[i], tv[i][0]);
-    const double2 w = LINK ? link_hops<1>(g, st, m0, &tv[i][1], &tv[i][4], Uo[LINK ? i : 0], Um[LINK ? i : 0])
-                           : apply_M_vals(g, st, smv[i], tv[i][0], &tv[i][1], &tv[i][4]);
+    const double2 w = LINK ? link_hops<1>(st, m0, &tv[i][1], &tv[i][4], Uo[LINK ? i : 0], Um[LINK ? i : 0])
+                           : apply_M_vals(st, smv[i], tv[i][0], &tv[i][1], &tv[i][4]);
     st_nt(elem(x, fs[i]), cadd(xv[i], rmul(al, pv[i])));
     double2 rr = cadd(rv[i], rmul(-al, w));
     st_nt(elem(r, fs[i]), rr);
@@ -398,6 +407,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cg1_X(CGMFix a, int odd, const CGMSt
   const CGMState s = load_state(sin + rhs);
   end_of_loads();
   int fs[SPT];
+  SiteCode kc[SPT];
   double smv[SPT];
   double2 rv[SPT][7];
   double2 Uo[LINK ? SPT : 1][3], Um[LINK ? SPT : 1][3];
@@ -406,7 +416,9 @@ __global__ __launch_bounds__(CGM_BS) void k_cg1_X(CGMFix a, int odd, const CGMSt
     fs[i] = map.f(g, i);
     // a thread past the last site loads site 0 and stores nothing
     const int fc = fs[i] < 0 ? 0 : fs[i];
-    NbrF n = neighbours_f(g, decode(g, fc));
+    const SiteGeo sgeo = site_geo(g, fc);
+    const NbrF n = sgeo.n;
+    kc[i] = sgeo.k;
     int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
     smv[i] = *elem(a.sm, fc);
     if (LINK) load_links(a.U, fc, n, Uo[LINK ? i : 0], Um[LINK ? i : 0]);
@@ -423,10 +435,10 @@ __global__ __launch_bounds__(CGM_BS) void k_cg1_X(CGMFix a, int odd, const CGMSt
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
     if (fs[i] < 0) continue;
-    const Site st = decode(g, fs[i]);
-    const double2 qq = LINK ? link_hops<-1>(g, st, rmul(smv[i], rv[i][0]), &rv[i][1], &rv[i][4], Uo[LINK ? i : 0],
+    const SiteCode st = kc[i];
+    const double2 qq = LINK ? link_hops<-1>(st, rmul(smv[i], rv[i][0]), &rv[i][1], &rv[i][4], Uo[LINK ? i : 0],
                                             Um[LINK ? i : 0])
-                            : apply_Mdag_fast_vals(g, st, smv[i], rv[i][0], &rv[i][1], &rv[i][4]);
+                            : apply_Mdag_fast_vals(st, smv[i], rv[i][0], &rv[i][1], &rv[i][4]);
     st_nt(elem(qo, fs[i]), qq);
     ag += cnorm2(rv[i][0]);
     ad += cnorm2(qq);
@@ -458,6 +470,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cg1_Y(CGMFix a, int odd, const CGMSt
   head_issue2<WH>(pp, a.nb, pv2);
   end_of_loads();
   int fs[SPT];
+  SiteCode kc[SPT];
   double smv[SPT];
   double2 tv[SPT][7], rv[SPT], xv[SPT], pv[SPT], so[SPT];
   double2 Uo[LINK ? SPT : 1][3], Um[LINK ? SPT : 1][3];
@@ -465,7 +478,9 @@ __global__ __launch_bounds__(CGM_BS) void k_cg1_Y(CGMFix a, int odd, const CGMSt
   for (int i = 0; i < SPT; i++) {
     fs[i] = map.f(g, i);
     const int fc = fs[i] < 0 ? 0 : fs[i];
-    NbrF n = neighbours_f(g, decode(g, fc));
+    const SiteGeo sgeo = site_geo(g, fc);
+    const NbrF n = sgeo.n;
+    kc[i] = sgeo.k;
     int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
     smv[i] = *elem(a.sm, fc);
     if (LINK) load_links(a.U, fc, n, Uo[LINK ? i : 0], Um[LINK ? i : 0]);
@@ -518,10 +533,10 @@ __global__ __launch_bounds__(CGM_BS) void k_cg1_Y(CGMFix a, int odd, const CGMSt
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
     if (fs[i] < 0) continue;
-    const Site st = decode(g, fs[i]);
+    const SiteCode st = kc[i];
     const double2 m0 = rmul(smv[i], tv[i][0]);
-    const double2 w = LINK ? link_hops<1>(g, st, m0, &tv[i][1], &tv[i][4], Uo[LINK ? i : 0], Um[LINK ? i : 0])
-                           : apply_M_vals(g, st, smv[i], tv[i][0], &tv[i][1], &tv[i][4]);
+    const double2 w = LINK ? link_hops<1>(st, m0, &tv[i][1], &tv[i][4], Uo[LINK ? i : 0], Um[LINK ? i : 0])
+                           : apply_M_vals(st, smv[i], tv[i][0], &tv[i][1], &tv[i][4]);
     const double2 sn = first ? w : cadd(w, rmul(be, so[i]));
     const double2 pn = first ? rv[i] : cadd(rv[i], rmul(be, pv[i]));
     st_nt(elem(sv, fs[i]), sn);
@@ -590,6 +605,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgd_X(CGMFix a, int rs, const CGMSta
   const CGMState s = load_state(sin + rhs);
   end_of_loads();
   int fs[SPT];
+  SiteCode kc[SPT];
   double smv[SPT];
   double2 rv[SPT][7];
   double2 Uo[LINK ? SPT : 1][3], Um[LINK ? SPT : 1][3];
@@ -597,7 +613,9 @@ __global__ __launch_bounds__(CGM_BS) void k_cgd_X(CGMFix a, int rs, const CGMSta
   for (int i = 0; i < SPT; i++) {
     fs[i] = map.f(g, i);
     const int fc = fs[i] < 0 ? 0 : fs[i];
-    NbrF n = neighbours_f(g, decode(g, fc));
+    const SiteGeo sgeo = site_geo(g, fc);
+    const NbrF n = sgeo.n;
+    kc[i] = sgeo.k;
     int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
     smv[i] = *elem(a.sm, fc);
     if (LINK) load_links(a.U, fc, n, Uo[LINK ? i : 0], Um[LINK ? i : 0]);
@@ -615,10 +633,10 @@ __global__ __launch_bounds__(CGM_BS) void k_cgd_X(CGMFix a, int rs, const CGMSta
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
     if (fs[i] < 0) continue;
-    const Site st = decode(g, fs[i]);
-    const double2 qq = LINK ? link_hops<-1>(g, st, rmul(smv[i], rv[i][0]), &rv[i][1], &rv[i][4], Uo[LINK ? i : 0],
+    const SiteCode st = kc[i];
+    const double2 qq = LINK ? link_hops<-1>(st, rmul(smv[i], rv[i][0]), &rv[i][1], &rv[i][4], Uo[LINK ? i : 0],
                                             Um[LINK ? i : 0])
-                            : apply_Mdag_fast_vals(g, st, smv[i], rv[i][0], &rv[i][1], &rv[i][4]);
+                            : apply_Mdag_fast_vals(st, smv[i], rv[i][0], &rv[i][1], &rv[i][4]);
     st_nt(elem(qo, fs[i]), qq);
     ag += cnorm2(rv[i][0]);
     ad += cnorm2(qq);
@@ -648,6 +666,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgd_Y(CGMFix a, int rs, const CGMSta
   head_issue2<WH>(pp, a.nb, pv2);
   end_of_loads();
   int fs[SPT];
+  SiteCode kc[SPT];
   double smv[SPT];
   double2 tv[SPT][7], rv[SPT], so[SPT];
   double2 Uo[LINK ? SPT : 1][3], Um[LINK ? SPT : 1][3];
@@ -655,7 +674,9 @@ __global__ __launch_bounds__(CGM_BS) void k_cgd_Y(CGMFix a, int rs, const CGMSta
   for (int i = 0; i < SPT; i++) {
     fs[i] = map.f(g, i);
     const int fc = fs[i] < 0 ? 0 : fs[i];
-    NbrF n = neighbours_f(g, decode(g, fc));
+    const SiteGeo sgeo = site_geo(g, fc);
+    const NbrF n = sgeo.n;
+    kc[i] = sgeo.k;
     int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
     smv[i] = *elem(a.sm, fc);
     if (LINK) load_links(a.U, fc, n, Uo[LINK ? i : 0], Um[LINK ? i : 0]);
@@ -709,10 +730,10 @@ __global__ __launch_bounds__(CGM_BS) void k_cgd_Y(CGMFix a, int rs, const CGMSta
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
     if (fs[i] < 0) continue;
-    const Site st = decode(g, fs[i]);
+    const SiteCode st = kc[i];
     const double2 m0 = rmul(smv[i], tv[i][0]);
-    const double2 w = LINK ? link_hops<1>(g, st, m0, &tv[i][1], &tv[i][4], Uo[LINK ? i : 0], Um[LINK ? i : 0])
-                           : apply_M_vals(g, st, smv[i], tv[i][0], &tv[i][1], &tv[i][4]);
+    const double2 w = LINK ? link_hops<1>(st, m0, &tv[i][1], &tv[i][4], Uo[LINK ? i : 0], Um[LINK ? i : 0])
+                           : apply_M_vals(st, smv[i], tv[i][0], &tv[i][1], &tv[i][4]);
     const double2 sn = first ? w : cadd(w, rmul(be, so[i]));
     st_nt(elem(sv, fs[i]), sn);
     st_nt(elem(rn, fs[i]), cadd(rv[i], rmul(-al, sn)));
@@ -794,6 +815,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgf_Z(CGMFix a, int k8, const CGMSta
   head_issue2<WH>(pin, a.nb, pv2);
   end_of_loads();
   int fs[R], bx[R];
+  SiteCode kc[R];
   double smv[R];
   double2 tv[R][7], rv[R], so[R];
   double2 pv[DEF ? 1 : SPT], xv[DEF ? 1 : SPT];
@@ -804,6 +826,7 @@ __global__ __launch_bounds__(CGM_BS) void k_cgf_Z(CGMFix a, int k8, const CGMSta
     const int e = i * CGM_BS + (int)threadIdx.x;
     const Site st = ftile_site<T0, T1, T2>(g, o0, o1, o2, e < TL::NEXT ? e : 0, &bx[i]);
     fs[i] = f_of_lex(g, st.lex, st.P);
+    kc[i] = site_code(g, st);
     NbrF n = neighbours_f(g, st);
     int q[7] = {fs[i], n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
     smv[i] = *elem(a.sm, fs[i]);
@@ -874,10 +897,10 @@ __global__ __launch_bounds__(CGM_BS) void k_cgf_Z(CGMFix a, int k8, const CGMSta
   for (int i = 0; i < R; i++) {
     const int e = i * CGM_BS + (int)threadIdx.x;
     if (i >= SPT && e >= TL::NEXT) continue;
-    const Site st = decode(g, fs[i]);
+    const SiteCode st = kc[i];
     const double2 m0 = rmul(smv[i], tv[i][0]);
-    const double2 w = LINK ? link_hops<1>(g, st, m0, &tv[i][1], &tv[i][4], Uo[LINK ? i : 0], Um[LINK ? i : 0])
-                           : apply_M_vals(g, st, smv[i], tv[i][0], &tv[i][1], &tv[i][4]);
+    const double2 w = LINK ? link_hops<1>(st, m0, &tv[i][1], &tv[i][4], Uo[LINK ? i : 0], Um[LINK ? i : 0])
+                           : apply_M_vals(st, smv[i], tv[i][0], &tv[i][1], &tv[i][4]);
     const double2 sn = first ? w : cadd(w, rmul(be, so[i]));
     const double2 rr = cadd(rv[i], rmul(-al, sn));
     box[bx[i]] = rr;
@@ -896,12 +919,12 @@ __global__ __launch_bounds__(CGM_BS) void k_cgf_Z(CGMFix a, int k8, const CGMSta
   double ag = 0.0, ad = 0.0;
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
-    const Site st = decode(g, fs[i]);
+    const SiteCode st = kc[i];
     const int b = bx[i];
     const double2 vpp[3] = {box[b + TL::S0], box[b + TL::S1], box[b + 1]};
     const double2 vmm[3] = {box[b - TL::S0], box[b - TL::S1], box[b - 1]};
-    const double2 qq = LINK ? link_hops<-1>(g, st, rmul(smv[i], rt[i]), vpp, vmm, Uo[LINK ? i : 0], Um[LINK ? i : 0])
-                            : apply_Mdag_fast_vals(g, st, smv[i], rt[i], vpp, vmm);
+    const double2 qq = LINK ? link_hops<-1>(st, rmul(smv[i], rt[i]), vpp, vmm, Uo[LINK ? i : 0], Um[LINK ? i : 0])
+                            : apply_Mdag_fast_vals(st, smv[i], rt[i], vpp, vmm);
     st_nt(elem(qn, fs[i]), qq);
     ag += cnorm2(rt[i]);
     ad += cnorm2(qq);

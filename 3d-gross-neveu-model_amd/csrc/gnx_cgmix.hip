@@ -389,13 +389,16 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_X(MixFix a, int odd, const MixSt
   const MixState s = sin[rhs];
   const double share = partial_share(pin + (size_t)rhs * a.nb, a.nb);
   int fs[SPT];
+  SiteCode kc[SPT];
   float2 rv[SPT][7], pv[SPT][7];
   float smv[SPT];
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
     fs[i] = mix_site<SPT, XCD>(a, i);
     if (fs[i] < 0) continue;
-    NbrF n = neighbours_f(g, decode(g, fs[i]));
+    const SiteGeo sgeo = site_geo(g, fs[i]);
+    const NbrF n = sgeo.n;
+    kc[i] = sgeo.k;
     int q[7] = {fs[i], n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
     smv[i] = a.sm32[fs[i]];
 #pragma unroll
@@ -470,7 +473,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_X(MixFix a, int odd, const MixSt
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
     if (fs[i] < 0) continue;
-    const Site st = decode(g, fs[i]);
+    const SiteCode st = kc[i];
     float2 p0v = cfma(fb, pv[i][0], rv[i][0]);
     float2 ppv[3], pmv[3];
 #pragma unroll
@@ -479,7 +482,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_X(MixFix a, int odd, const MixSt
       pmv[d] = cfma(fb, pv[i][4 + d], rv[i][4 + d]);
     }
     S::st(pnew, fs[i], p0v, a.nt);
-    float2 tt = apply_Mdag_fast_vals(g, st, smv[i], p0v, ppv, pmv);
+    float2 tt = apply_Mdag_fast_vals(st, smv[i], p0v, ppv, pmv);
     S::st((T*)a.t + off, fs[i], tt, a.nt);
     acc += dnorm2(tt);
   }
@@ -599,13 +602,16 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Xe(MixFix a, int sl, const MixSt
   partial_issue(pin + v.poff, v.nb, pl);
   end_of_loads();
   int fs[SPT];
+  SiteCode kc[SPT];
   float2 rv[SPT][7], pv[SPT][7], xsv[SPT], ppr[SPT];
   float smv[SPT];
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
     fs[i] = mix_site<SPT, XCD>(v, i);
     const int fc = fs[i] < 0 ? 0 : fs[i];  // past the last site: load site 0, store nothing
-    NbrF n = neighbours_f(g, decode(g, fc));
+    const SiteGeo sgeo = site_geo(g, fc);
+    const NbrF n = sgeo.n;
+    kc[i] = sgeo.k;
     int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
     smv[i] = *elem(a.sm32 + v.soff, fc);
 #pragma unroll
@@ -693,7 +699,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Xe(MixFix a, int sl, const MixSt
   for (int i = 0; i < SPT; i++) {
     if (fs[i] < 0) continue;
     if (FLUSH) st_mix(elem(a.xs + off, fs[i]), cfma(fa, pv[i][0], cfma(fp, ppr[i], xsv[i])), a.nt);
-    const Site st = decode(g, fs[i]);
+    const SiteCode st = kc[i];
     float2 p0v = cfma(fb, pv[i][0], rv[i][0]);
     float2 ppv[3], pmv[3];
 #pragma unroll
@@ -702,7 +708,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Xe(MixFix a, int sl, const MixSt
       pmv[d] = cfma(fb, pv[i][4 + d], rv[i][4 + d]);
     }
     S::st(pnew, fs[i], p0v, a.nt);
-    float2 tt = apply_Mdag_fast_vals(g, st, smv[i], p0v, ppv, pmv);
+    float2 tt = apply_Mdag_fast_vals(st, smv[i], p0v, ppv, pmv);
     S::st((T*)a.t + off, fs[i], tt, a.nt);
     acc += dnorm2(tt);
   }
@@ -736,13 +742,16 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Ye(MixFix a, int sl, const MixSt
   partial_issue(pin + v.poff, v.nb, pl);
   end_of_loads();
   int fs[SPT];
+  SiteCode kc[SPT];
   float smv[SPT];
   float2 tv[SPT][7], rv[SPT];
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
     fs[i] = mix_site<SPT, XCD>(v, i);
     const int fc = fs[i] < 0 ? 0 : fs[i];
-    NbrF n = neighbours_f(g, decode(g, fc));
+    const SiteGeo sgeo = site_geo(g, fc);
+    const NbrF n = sgeo.n;
+    kc[i] = sgeo.k;
     int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
     smv[i] = *elem(a.sm32 + v.soff, fc);
 #pragma unroll
@@ -798,7 +807,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Ye(MixFix a, int sl, const MixSt
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
     if (fs[i] < 0) continue;
-    const float2 w = apply_M_vals(g, decode(g, fs[i]), smv[i], tv[i][0], &tv[i][1], &tv[i][4]);
+    const float2 w = apply_M_vals(kc[i], smv[i], tv[i][0], &tv[i][1], &tv[i][4]);
     float2 rn = cfma(-al, w, rv[i]);
     S::st(r, fs[i], rn, a.nt);
     acc += dnorm2(rn);
@@ -890,13 +899,16 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_X2(MixFix a, int odd, const MixS
     pnew[k] = (T*)(odd ? a.p0 : a.p1) + off;
   }
   int fs[SPT];
+  SiteCode kc[SPT];
   float smv[SPT];
   float2 rv[2][SPT][7], pv[2][SPT][7];
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
     fs[i] = mix_site<SPT, XCD>(a, i);
     if (fs[i] < 0) continue;
-    NbrF n = neighbours_f(g, decode(g, fs[i]));
+    const SiteGeo sgeo = site_geo(g, fs[i]);
+    const NbrF n = sgeo.n;
+    kc[i] = sgeo.k;
     int q[7] = {fs[i], n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
     smv[i] = a.sm32[fs[i]];
 #pragma unroll
@@ -945,7 +957,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_X2(MixFix a, int odd, const MixS
 #pragma unroll
       for (int i = 0; i < SPT; i++) {
         if (fs[i] < 0) continue;
-        const Site st = decode(g, fs[i]);
+        const SiteCode st = kc[i];
         float2 p0v = cfma(fb, pv[k][i][0], rv[k][i][0]);
         float2 ppv[3], pmv[3];
 #pragma unroll
@@ -954,7 +966,7 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_X2(MixFix a, int odd, const MixS
           pmv[d] = cfma(fb, pv[k][i][4 + d], rv[k][i][4 + d]);
         }
         S::st(pnew[k], fs[i], p0v, a.nt);
-        float2 tt = apply_Mdag_fast_vals(g, st, smv[i], p0v, ppv, pmv);
+        float2 tt = apply_Mdag_fast_vals(st, smv[i], p0v, ppv, pmv);
         S::st((T*)a.t + off, fs[i], tt, a.nt);
         acc[k] += dnorm2(tt);
       }

@@ -60,6 +60,7 @@ __global__ __launch_bounds__(CGS_BS) void k_cgs(Geom g, const double* __restrict
   // coordinates and neighbours, sigma mass, residual, solution, direction
   int fs[SPT];
   Site st[SPT];
+  SiteCode kc[SPT];
   NbrF nb[SPT];
   double smv[SPT];
   double2 rv[SPT], xv[SPT], pv[SPT];
@@ -69,6 +70,7 @@ __global__ __launch_bounds__(CGS_BS) void k_cgs(Geom g, const double* __restrict
     fs[i] = f < g.V ? f : -1;
     const int fc = f < g.V ? f : 0;
     st[i] = decode(g, fc);
+    kc[i] = site_code(g, st[i]);
     nb[i] = neighbours_f(g, st[i]);
     smv[i] = sm[fc];
     rv[i] = b[fc];
@@ -145,7 +147,7 @@ __global__ __launch_bounds__(CGS_BS) void k_cgs(Geom g, const double* __restrict
       if (fs[i] < 0) continue;
       const double2 vpp[3] = {P[nb[i].pp[0]], P[nb[i].pp[1]], P[nb[i].pp[2]]};
       const double2 vmm[3] = {P[nb[i].mm[0]], P[nb[i].mm[1]], P[nb[i].mm[2]]};
-      const double2 tt = apply_Mdag_fast_vals(g, st[i], smv[i], pv[i], vpp, vmm);
+      const double2 tt = apply_Mdag_fast_vals(kc[i], smv[i], pv[i], vpp, vmm);
       T[fs[i]] = tt;
       acc += cnorm2(tt);
     }
@@ -159,7 +161,7 @@ __global__ __launch_bounds__(CGS_BS) void k_cgs(Geom g, const double* __restrict
       if (fs[i] < 0) continue;
       const double2 tpp[3] = {T[nb[i].pp[0]], T[nb[i].pp[1]], T[nb[i].pp[2]]};
       const double2 tmm[3] = {T[nb[i].mm[0]], T[nb[i].mm[1]], T[nb[i].mm[2]]};
-      const double2 w = apply_M_vals(g, st[i], smv[i], T[fs[i]], tpp, tmm);
+      const double2 w = apply_M_vals(kc[i], smv[i], T[fs[i]], tpp, tmm);
       xv[i] = cadd(xv[i], rmul(al, pv[i]));
       rv[i] = cadd(rv[i], rmul(-al, w));
       acc += cnorm2(rv[i]);

@@ -214,7 +214,8 @@ __global__ __launch_bounds__(CGU_BS) void k_cgu_A(CGUFix a, int odd, const CGMSt
   end_of_loads();
   const int f = cgu_site(d);
   const int fc = f < 0 ? 0 : f;  // past the last site: load site 0, store nothing
-  const NbrF n = neighbours_f(g, decode(g, fc));
+  const SiteGeo sgeo = site_geo(g, fc);
+  const NbrF n = sgeo.n;
   const int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
   const double smv = *elem(a.sm + d.soff, fc);
   double2 Uo[3], Um[3];
@@ -251,7 +252,7 @@ __global__ __launch_bounds__(CGU_BS) void k_cgu_A(CGUFix a, int odd, const CGMSt
   }
   double acc = 0.0;
   if (f >= 0) {
-    const Site st = decode(g, f);
+    const SiteCode st = sgeo.k;
     const double2 p0v = cadd(rv[0], rmul(beta, pv[0]));
     double2 ppv[3], pmv[3];
 #pragma unroll
@@ -260,8 +261,8 @@ __global__ __launch_bounds__(CGU_BS) void k_cgu_A(CGUFix a, int odd, const CGMSt
       pmv[dd] = cadd(rv[4 + dd], rmul(beta, pv[4 + dd]));
     }
     st_cgu(elem(pnew, f), p0v);
-    const double2 tt = LINK ? link_hops<-1>(g, st, rmul(smv, p0v), ppv, pmv, Uo, Um)
-                            : apply_Mdag_fast_vals(g, st, smv, p0v, ppv, pmv);
+    const double2 tt = LINK ? link_hops<-1>(st, rmul(smv, p0v), ppv, pmv, Uo, Um)
+                            : apply_Mdag_fast_vals(st, smv, p0v, ppv, pmv);
     st_cgu(elem(a.t + d.voff, f), tt);
     acc = cnorm2(tt);
   }
@@ -297,7 +298,8 @@ __global__ __launch_bounds__(CGU_BS) void k_cgu_B(CGUFix a, int odd, const CGMSt
   end_of_loads();
   const int f = cgu_site(d);
   const int fc = f < 0 ? 0 : f;
-  const NbrF n = neighbours_f(g, decode(g, fc));
+  const SiteGeo sgeo = site_geo(g, fc);
+  const NbrF n = sgeo.n;
   const int q[7] = {fc, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
   const double smv = *elem(a.sm + d.soff, fc);
   double2 Uo[3], Um[3];
@@ -313,9 +315,9 @@ __global__ __launch_bounds__(CGU_BS) void k_cgu_B(CGUFix a, int odd, const CGMSt
   const double al = s.rho / gamma;
   double acc = 0.0;
   if (f >= 0) {
-    const Site st = decode(g, f);
-    const double2 w = LINK ? link_hops<1>(g, st, rmul(smv, tv[0]), &tv[1], &tv[4], Uo, Um)
-                           : apply_M_vals(g, st, smv, tv[0], &tv[1], &tv[4]);
+    const SiteCode st = sgeo.k;
+    const double2 w = LINK ? link_hops<1>(st, rmul(smv, tv[0]), &tv[1], &tv[4], Uo, Um)
+                           : apply_M_vals(st, smv, tv[0], &tv[1], &tv[4]);
     st_cgu(elem(x, f), cadd(xv, rmul(al, pv)));
     const double2 rr = cadd(rv, rmul(-al, w));
     st_cgu(elem(r, f), rr);

@@ -113,20 +113,61 @@ GNX_DEV NbrF neighbours_f(const Geom& g, const Site& s) {
   return r;
 }
 
-// Half-phases 0.5*eta (sign folded): hp_pp[d] = 0.5*eta_pp[x,d], hp_mm[d] = 0.5*eta_mm[x,d]
-struct HalfEta {
-  double pp[3], mm[3];
+// The parities and boundary flags the operator needs at a site, in one word,
+// so that a kernel decodes a site once, in its load phase, and keeps one
+// register for its arithmetic:
+//   bit 0: x0 odd          bit 1: x0 + x1 odd
+//   bit 2: x2 == LT - 1    bit 3: x2 == 0
+//   bit 4: x0 == L - 1     bit 5: x0 == 0
+//   bit 6: x1 == L - 1     bit 7: x1 == 0
+struct SiteCode {
+  unsigned c;
+};
+enum : unsigned {
+  SC_X0ODD = 1u, SC_X01ODD = 2u, SC_X2TOP = 4u, SC_X2BOT = 8u,
+  SC_X0TOP = 16u, SC_X0BOT = 32u, SC_X1TOP = 64u, SC_X1BOT = 128u
 };
 
-GNX_DEV HalfEta half_eta(const Geom& g, const Site& s) {
-  HalfEta e;
-  double e1 = (s.x0 & 1) ? -0.5 : 0.5;
-  double e2 = ((s.x0 + s.x1) & 1) ? -0.5 : 0.5;
-  e.pp[0] = 0.5; e.mm[0] = 0.5;
-  e.pp[1] = e1;  e.mm[1] = e1;
-  e.pp[2] = (s.x2 == g.LT - 1) ? -e2 : e2;
-  e.mm[2] = (s.x2 == 0) ? -e2 : e2;
+GNX_DEV SiteCode site_code(const Geom& g, const Site& s) {
+  unsigned c = (unsigned)(s.x0 & 1) | ((unsigned)((s.x0 + s.x1) & 1) << 1);
+  c |= (s.x2 == g.LT - 1 ? SC_X2TOP : 0u) | (s.x2 == 0 ? SC_X2BOT : 0u);
+  c |= (s.x0 == g.L - 1 ? SC_X0TOP : 0u) | (s.x0 == 0 ? SC_X0BOT : 0u);
+  c |= (s.x1 == g.L - 1 ? SC_X1TOP : 0u) | (s.x1 == 0 ? SC_X1BOT : 0u);
+  SiteCode k = {c};
+  return k;
+}
+
+// Half-phases 0.5*eta (sign folded): hp_pp[d] = 0.5*eta_pp[x,d], hp_mm[d] = 0.5*eta_mm[x,d]
+template <typename R>
+struct HalfEtaT {
+  R pp[3], mm[3];
+};
+using HalfEta = HalfEtaT<double>;
+
+template <typename R>
+GNX_DEV HalfEtaT<R> half_eta_t(SiteCode k) {
+  const R h = (R)0.5;
+  const R e1 = (k.c & SC_X0ODD) ? -h : h;
+  const R e2 = (k.c & SC_X01ODD) ? -h : h;
+  HalfEtaT<R> e;
+  e.pp[0] = h;  e.mm[0] = h;
+  e.pp[1] = e1; e.mm[1] = e1;
+  e.pp[2] = (k.c & SC_X2TOP) ? -e2 : e2;
+  e.mm[2] = (k.c & SC_X2BOT) ? -e2 : e2;
   return e;
+}
+GNX_DEV HalfEta half_eta(SiteCode k) { return half_eta_t<double>(k); }
+GNX_DEV HalfEta half_eta(const Geom& g, const Site& s) { return half_eta(site_code(g, s)); }
+
+// neighbours and code of f-site f from one decode (a kernel's load phase)
+struct SiteGeo {
+  NbrF n;
+  SiteCode k;
+};
+GNX_DEV SiteGeo site_geo(const Geom& g, int f) {
+  const Site s = decode(g, f);
+  SiteGeo r = {neighbours_f(g, s), site_code(g, s)};
+  return r;
 }
 
 // ---------------------------------------------------------------- complex

@@ -81,6 +81,7 @@ __global__ __launch_bounds__(ENG_BS) void k_normal(Geom g, int nb, const double*
   int o0, o1, o2;
   ftile_origin<T0, T1, T2>(g, nb, (int)blockIdx.x, &o0, &o1, &o2);
   int fs[R], bx[R];
+  SiteCode kc[R];
   double smv[R];
   double2 v[R][7];
 #pragma unroll
@@ -89,6 +90,7 @@ __global__ __launch_bounds__(ENG_BS) void k_normal(Geom g, int nb, const double*
     const int e = i * ENG_BS + (int)threadIdx.x;
     const Site st = ftile_site<T0, T1, T2>(g, o0, o1, o2, e < TL::NEXT ? e : 0, &bx[i]);
     fs[i] = f_of_lex(g, st.lex, st.P);
+    kc[i] = site_code(g, st);
     const NbrF n = neighbours_f(g, st);
     const int q[7] = {fs[i], n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
     smv[i] = *elem(sm, fs[i]);
@@ -99,19 +101,19 @@ __global__ __launch_bounds__(ENG_BS) void k_normal(Geom g, int nb, const double*
 #pragma unroll
   for (int i = 0; i < R; i++) {
     if (i >= SPT && i * ENG_BS + (int)threadIdx.x >= TL::NEXT) continue;
-    const Site st = decode(g, fs[i]);
-    box[bx[i]] = OP2 == GNX_OP_MMDAG ? apply_Mdag_vals(g, st, smv[i], v[i][0], &v[i][1], &v[i][4])
-                                     : apply_M_vals(g, st, smv[i], v[i][0], &v[i][1], &v[i][4]);
+    const SiteCode st = kc[i];
+    box[bx[i]] = OP2 == GNX_OP_MMDAG ? apply_Mdag_vals(st, smv[i], v[i][0], &v[i][1], &v[i][4])
+                                     : apply_M_vals(st, smv[i], v[i][0], &v[i][1], &v[i][4]);
   }
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < SPT; i++) {
-    const Site st = decode(g, fs[i]);
+    const SiteCode st = kc[i];
     const int b = bx[i];
     const double2 vpp[3] = {box[b + TL::S0], box[b + TL::S1], box[b + 1]};
     const double2 vmm[3] = {box[b - TL::S0], box[b - TL::S1], box[b - 1]};
-    *elem(out, fs[i]) = OP2 == GNX_OP_MMDAG ? apply_M_vals(g, st, smv[i], box[b], vpp, vmm)
-                                            : apply_Mdag_vals(g, st, smv[i], box[b], vpp, vmm);
+    *elem(out, fs[i]) = OP2 == GNX_OP_MMDAG ? apply_M_vals(st, smv[i], box[b], vpp, vmm)
+                                            : apply_Mdag_vals(st, smv[i], box[b], vpp, vmm);
   }
 }
 

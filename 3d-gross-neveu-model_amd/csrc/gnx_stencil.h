@@ -12,8 +12,13 @@
 //
 // Each hop term 0.5*eta*v is exact (a power of two times +-v), so only the
 // order of the additions matters; the mass product sm*v is rounded first
-// (the library is compiled with -ffp-contract=off).  Absent slots add -0.0,
-// the exact additive identity.
+// (the library is compiled with -ffp-contract=off).  In the gather forms a
+// hop term enters by one explicit fused multiply-add (hop()), which rounds
+// the exact product's sum once, as the separate product and sum do; the
+// reference-order M^+ keeps its selected terms as separate products.
+// Absent slots add -0.0, the exact additive identity.  The operators take
+// the site's SiteCode (gnx_common.h), which a kernel forms in its load
+// phase; the Geom/Site overloads form it on the spot.
 #pragma once
 
 #include <utility>
@@ -42,60 +47,85 @@ GNX_DEV float2 rmul(float s, float2 a) { return make_float2(s * a.x, s * a.y); }
 GNX_DEV float2 rmul(double s, float2 a) { return rmul((float)s, a); }
 GNX_DEV float cnorm2(float2 a) { return a.x * a.x + a.y * a.y; }
 
-template <typename LoadF, typename T2 = decltype(std::declval<LoadF>()(0))>
-GNX_DEV T2 apply_M(const Geom& g, const Site& s, typename CT<T2>::R sm, LoadF load) {
-  NbrF n = neighbours_f(g, s);
-  HalfEta e = half_eta(g, s);
-  int f = f_of_lex(g, s.lex, s.P);
-  T2 acc = rmul(sm, load(f));
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    acc = cadd(acc, rmul(e.pp[d], load(n.pp[d])));
-    acc = cadd(acc, rmul(-e.mm[d], load(n.mm[d])));
-  }
-  return acc;
-}
+// acc + c v for a hop coefficient c = +-0.5.  The product is exact (a power
+// of two times v, for every v above the subnormal range), so one fused
+// multiply-add rounds exactly like the separate product and sum of the
+// reference, in one instruction per component.
+GNX_DEV double2 hop(double c, double2 v, double2 acc) { return make_double2(fma(c, v.x, acc.x), fma(c, v.y, acc.y)); }
+GNX_DEV float2 hop(float c, float2 v, float2 acc) { return make_float2(fmaf(c, v.x, acc.x), fmaf(c, v.y, acc.y)); }
 
-// M v from operand values already in registers, the order of apply_M
+// M v from operand values already in registers, the order of apply_M:
+// v0 = v[site], vpp[d] = v[pp(z,d)], vmm[d] = v[mm(z,d)]
 template <typename T2>
-GNX_DEV T2 apply_M_vals(const Geom& g, const Site& s, typename CT<T2>::R sm, T2 v0, const T2 vpp[3],
-                        const T2 vmm[3]) {
-  const HalfEta e = half_eta(g, s);
+GNX_DEV T2 apply_M_vals(SiteCode k, typename CT<T2>::R sm, T2 v0, const T2 vpp[3], const T2 vmm[3]) {
+  using R = typename CT<T2>::R;
+  const HalfEtaT<R> e = half_eta_t<R>(k);
   T2 acc = rmul(sm, v0);
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    acc = cadd(acc, rmul(e.pp[d], vpp[d]));
-    acc = cadd(acc, rmul(-e.mm[d], vmm[d]));
+    acc = hop(e.pp[d], vpp[d], acc);
+    acc = hop(-e.mm[d], vmm[d], acc);
   }
   return acc;
 }
+template <typename T2>
+GNX_DEV T2 apply_M_vals(const Geom& g, const Site& s, typename CT<T2>::R sm, T2 v0, const T2 vpp[3],
+                        const T2 vmm[3]) {
+  return apply_M_vals(site_code(g, s), sm, v0, vpp, vmm);
+}
+
+// loads in the order the sum consumes them (the site first), so the
+// arithmetic starts on the first value back
+template <typename LoadF, typename T2 = decltype(std::declval<LoadF>()(0))>
+GNX_DEV T2 apply_M(const Geom& g, const Site& s, typename CT<T2>::R sm, LoadF load) {
+  NbrF n = neighbours_f(g, s);
+  const T2 v0 = load(f_of_lex(g, s.lex, s.P));
+  T2 vpp[3], vmm[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    vpp[d] = load(n.pp[d]);
+    vmm[d] = load(n.mm[d]);
+  }
+  return apply_M_vals(site_code(g, s), sm, v0, vpp, vmm);
+}
 
 // r - M v, reference order (r -= sm*v, then -= pp, += mm per direction)
+template <typename T2>
+GNX_DEV T2 apply_sub_M_vals(SiteCode k, typename CT<T2>::R sm, T2 r, T2 v0, const T2 vpp[3], const T2 vmm[3]) {
+  using R = typename CT<T2>::R;
+  const HalfEtaT<R> e = half_eta_t<R>(k);
+  T2 acc = csub(r, rmul(sm, v0));
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    acc = hop(-e.pp[d], vpp[d], acc);
+    acc = hop(e.mm[d], vmm[d], acc);
+  }
+  return acc;
+}
 template <typename LoadF, typename T2 = decltype(std::declval<LoadF>()(0))>
 GNX_DEV T2 apply_sub_M(const Geom& g, const Site& s, typename CT<T2>::R sm, T2 r, LoadF load) {
   NbrF n = neighbours_f(g, s);
-  HalfEta e = half_eta(g, s);
-  int f = f_of_lex(g, s.lex, s.P);
-  T2 acc = csub(r, rmul(sm, load(f)));
+  const T2 v0 = load(f_of_lex(g, s.lex, s.P));
+  T2 vpp[3], vmm[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    acc = cadd(acc, rmul(-e.pp[d], load(n.pp[d])));
-    acc = cadd(acc, rmul(e.mm[d], load(n.mm[d])));
+    vpp[d] = load(n.pp[d]);
+    vmm[d] = load(n.mm[d]);
   }
-  return acc;
+  return apply_sub_M_vals(site_code(g, s), sm, r, v0, vpp, vmm);
 }
 
 // Ordered accumulation of the six M^+ (or -M^+) hop terms onto acc.
 // tpp[d]: term from source pp(z,d); tmm[d]: term from source mm(z,d).
 template <typename T2>
-GNX_DEV T2 accumulate_adj_order(const Geom& g, const Site& s, T2 acc, const T2 tpp[3], const T2 tmm[3]) {
+GNX_DEV T2 accumulate_adj_order(SiteCode k, T2 acc, const T2 tpp[3], const T2 tmm[3]) {
   const T2 z = CT<T2>::nzero();
-  const int c[3] = {s.x0, s.x1, s.x2};
-  const int E[3] = {g.L, g.L, g.LT};
+  const unsigned topb[3] = {SC_X0TOP, SC_X1TOP, SC_X2TOP};
+  const unsigned botb[3] = {SC_X0BOT, SC_X1BOT, SC_X2BOT};
   // below-slots: d = 0, 1, 2
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    bool top = (c[d] == E[d] - 1), bot = (c[d] == 0);
+    const bool top = (k.c & topb[d]) != 0, bot = (k.c & botb[d]) != 0;
     T2 s1 = top ? tpp[d] : (bot ? z : tmm[d]);
     T2 s2 = top ? tmm[d] : z;
     acc = cadd(acc, s1);
@@ -104,8 +134,8 @@ GNX_DEV T2 accumulate_adj_order(const Geom& g, const Site& s, T2 acc, const T2 t
   // above-slots: d = 2, 1, 0
 #pragma unroll
   for (int dd = 0; dd < 3; dd++) {
-    int d = 2 - dd;
-    bool top = (c[d] == E[d] - 1), bot = (c[d] == 0);
+    const int d = 2 - dd;
+    const bool top = (k.c & topb[d]) != 0, bot = (k.c & botb[d]) != 0;
     T2 s3 = top ? z : tpp[d];
     T2 s4 = bot ? tmm[d] : z;
     acc = cadd(acc, s3);
@@ -113,85 +143,71 @@ GNX_DEV T2 accumulate_adj_order(const Geom& g, const Site& s, T2 acc, const T2 t
   }
   return acc;
 }
+template <typename T2>
+GNX_DEV T2 accumulate_adj_order(const Geom& g, const Site& s, T2 acc, const T2 tpp[3], const T2 tmm[3]) {
+  return accumulate_adj_order(site_code(g, s), acc, tpp, tmm);
+}
 
 // M^+ v at site s.  Source mm(z,d) contributes +0.5*eta_pp[mm(z,d),d]*v, source
 // pp(z,d) contributes -0.5*eta_mm[pp(z,d),d]*v.  eta_d depends only on
 // coordinates below d, so eta at the source equals eta at z except for the
 // antiperiodic time flips: eta_pp[mm(z,2),2] flips when z2 == 0 and
 // eta_mm[pp(z,2),2] flips when z2 == LT-1.
-GNX_DEV void adj_terms(const Geom& g, const Site& s, double hp[3], double hm[3]) {
-  double e1 = (s.x0 & 1) ? -0.5 : 0.5;
-  double e2 = ((s.x0 + s.x1) & 1) ? -0.5 : 0.5;
+GNX_DEV void adj_terms(SiteCode k, double hp[3], double hm[3]) {
+  const double e1 = (k.c & SC_X0ODD) ? -0.5 : 0.5;
+  const double e2 = (k.c & SC_X01ODD) ? -0.5 : 0.5;
   hp[0] = 0.5;  hm[0] = -0.5;
   hp[1] = e1;   hm[1] = -e1;
-  hp[2] = (s.x2 == 0) ? -e2 : e2;           // coefficient of v[mm(z,2)]
-  hm[2] = (s.x2 == g.LT - 1) ? e2 : -e2;    // coefficient of v[pp(z,2)]
+  hp[2] = (k.c & SC_X2BOT) ? -e2 : e2;   // coefficient of v[mm(z,2)]
+  hm[2] = (k.c & SC_X2TOP) ? e2 : -e2;   // coefficient of v[pp(z,2)]
 }
+GNX_DEV void adj_terms(const Geom& g, const Site& s, double hp[3], double hm[3]) { adj_terms(site_code(g, s), hp, hm); }
 
 // M^+ from operand values already in registers:
 // v0 = v[site], vpp[d] = v[pp(z,d)], vmm[d] = v[mm(z,d)]
 template <typename T2>
-GNX_DEV T2 apply_Mdag_vals(const Geom& g, const Site& s, typename CT<T2>::R sm, T2 v0, const T2 vpp[3],
-                           const T2 vmm[3]) {
+GNX_DEV T2 apply_Mdag_vals(SiteCode k, typename CT<T2>::R sm, T2 v0, const T2 vpp[3], const T2 vmm[3]) {
   double cm[3], cp[3];
-  adj_terms(g, s, cm, cp);
+  adj_terms(k, cm, cp);
   T2 tpp[3], tmm[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     tmm[d] = rmul(cm[d], vmm[d]);
     tpp[d] = rmul(cp[d], vpp[d]);
   }
-  return accumulate_adj_order(g, s, rmul(sm, v0), tpp, tmm);
+  return accumulate_adj_order(k, rmul(sm, v0), tpp, tmm);
+}
+template <typename T2>
+GNX_DEV T2 apply_Mdag_vals(const Geom& g, const Site& s, typename CT<T2>::R sm, T2 v0, const T2 vpp[3],
+                           const T2 vmm[3]) {
+  return apply_Mdag_vals(site_code(g, s), sm, v0, vpp, vmm);
 }
 
 // M^+ in the plain gather form, M^+ = sm - D with D real antisymmetric:
 //   y = sm v - sum_d [0.5 eta_pp(x,d) v(x+d) - 0.5 eta_mm(x,d) v(x-d)]
-// For the reduced-precision inner solvers, which keep no reference order
-// (the exact-order form above costs about 30 selects per site).
+// For the solvers, which keep no reference order (the exact-order form
+// above costs about 30 selects per site).
 template <typename T2>
-GNX_DEV T2 apply_Mdag_fast_vals(const Geom& g, const Site& s, typename CT<T2>::R sm, T2 v0, const T2 vpp[3],
-                                const T2 vmm[3]) {
-  const HalfEta e = half_eta(g, s);
+GNX_DEV T2 apply_Mdag_fast_vals(SiteCode k, typename CT<T2>::R sm, T2 v0, const T2 vpp[3], const T2 vmm[3]) {
+  using R = typename CT<T2>::R;
+  const HalfEtaT<R> e = half_eta_t<R>(k);
   T2 acc = rmul(sm, v0);
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    acc = csub(acc, rmul(e.pp[d], vpp[d]));
-    acc = cadd(acc, rmul(e.mm[d], vmm[d]));
+    acc = hop(-e.pp[d], vpp[d], acc);
+    acc = hop(e.mm[d], vmm[d], acc);
   }
   return acc;
 }
+template <typename T2>
+GNX_DEV T2 apply_Mdag_fast_vals(const Geom& g, const Site& s, typename CT<T2>::R sm, T2 v0, const T2 vpp[3],
+                                const T2 vmm[3]) {
+  return apply_Mdag_fast_vals(site_code(g, s), sm, v0, vpp, vmm);
+}
 
-// FP32 forms for the reduced-precision inner solvers.  Each hop term
-// +-0.5 v is exact, so a fused multiply-add rounds like the separate product
-// and sum, in one instruction; the FP64 forms above keep product and sum
-// apart (they are built without contraction, for the bit-exact operators).
+// fused multiply-add of a complex vector by a real, for the reduced-precision
+// inner solvers
 GNX_DEV float2 cfma(float a, float2 x, float2 y) { return make_float2(fmaf(a, x.x, y.x), fmaf(a, x.y, y.y)); }
-
-template <>
-GNX_DEV float2 apply_M_vals<float2>(const Geom& g, const Site& s, float sm, float2 v0, const float2 vpp[3],
-                                    const float2 vmm[3]) {
-  const HalfEta e = half_eta(g, s);
-  float2 acc = rmul(sm, v0);
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    acc = cfma((float)e.pp[d], vpp[d], acc);
-    acc = cfma(-(float)e.mm[d], vmm[d], acc);
-  }
-  return acc;
-}
-
-template <>
-GNX_DEV float2 apply_Mdag_fast_vals<float2>(const Geom& g, const Site& s, float sm, float2 v0, const float2 vpp[3],
-                                            const float2 vmm[3]) {
-  const HalfEta e = half_eta(g, s);
-  float2 acc = rmul(sm, v0);
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    acc = cfma(-(float)e.pp[d], vpp[d], acc);
-    acc = cfma((float)e.mm[d], vmm[d], acc);
-  }
-  return acc;
-}
 
 template <typename LoadF, typename T2 = decltype(std::declval<LoadF>()(0))>
 GNX_DEV T2 apply_Mdag(const Geom& g, const Site& s, typename CT<T2>::R sm, LoadF load) {
@@ -202,7 +218,7 @@ GNX_DEV T2 apply_Mdag(const Geom& g, const Site& s, typename CT<T2>::R sm, LoadF
     vmm[d] = load(n.mm[d]);
     vpp[d] = load(n.pp[d]);
   }
-  return apply_Mdag_vals(g, s, sm, load(f_of_lex(g, s.lex, s.P)), vpp, vmm);
+  return apply_Mdag_vals(site_code(g, s), sm, load(f_of_lex(g, s.lex, s.P)), vpp, vmm);
 }
 
 // r - M^+ v (dirac.c:253-287): r -= sm*v, then the scatter terms negated, in
@@ -210,8 +226,9 @@ GNX_DEV T2 apply_Mdag(const Geom& g, const Site& s, typename CT<T2>::R sm, LoadF
 template <typename LoadF, typename T2 = decltype(std::declval<LoadF>()(0))>
 GNX_DEV T2 apply_sub_Mdag(const Geom& g, const Site& s, typename CT<T2>::R sm, T2 r, LoadF load) {
   NbrF n = neighbours_f(g, s);
+  const SiteCode k = site_code(g, s);
   double cm[3], cp[3];
-  adj_terms(g, s, cm, cp);
+  adj_terms(k, cm, cp);
   int f = f_of_lex(g, s.lex, s.P);
   T2 tpp[3], tmm[3];
 #pragma unroll
@@ -219,7 +236,7 @@ GNX_DEV T2 apply_sub_Mdag(const Geom& g, const Site& s, typename CT<T2>::R sm, T
     tmm[d] = rmul(-cm[d], load(n.mm[d]));
     tpp[d] = rmul(-cp[d], load(n.pp[d]));
   }
-  return accumulate_adj_order(g, s, csub(r, rmul(sm, load(f))), tpp, tmm);
+  return accumulate_adj_order(k, csub(r, rmul(sm, load(f))), tpp, tmm);
 }
 
 // ------------------------------------------------------------------ link-coupled operator
@@ -236,9 +253,9 @@ GNX_DEV double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 // hop terms of site s onto acc: SIGN = +1 for M_U, -1 for M_U^+ and r -= M_U v
 // vpp[d] = v(x+d), vmm[d] = v(x-d), Uo[d] = U_d(x), Um[d] = U_d(x-d)
 template <int SIGN>
-GNX_DEV double2 link_hops(const Geom& g, const Site& s, double2 acc, const double2 vpp[3], const double2 vmm[3],
-                          const double2 Uo[3], const double2 Um[3]) {
-  HalfEta e = half_eta(g, s);
+GNX_DEV double2 link_hops(SiteCode k, double2 acc, const double2 vpp[3], const double2 vmm[3], const double2 Uo[3],
+                          const double2 Um[3]) {
+  const HalfEta e = half_eta(k);
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     const double2 tp = cmul(rmul(e.pp[d], Uo[d]), vpp[d]);
@@ -252,6 +269,12 @@ GNX_DEV double2 link_hops(const Geom& g, const Site& s, double2 acc, const doubl
     }
   }
   return acc;
+}
+
+template <int SIGN>
+GNX_DEV double2 link_hops(const Geom& g, const Site& s, double2 acc, const double2 vpp[3], const double2 vmm[3],
+                          const double2 Uo[3], const double2 Um[3]) {
+  return link_hops<SIGN>(site_code(g, s), acc, vpp, vmm, Uo, Um);
 }
 
 // links of site s: own three and the three of its mm neighbours
