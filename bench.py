@@ -347,9 +347,11 @@ def probes_leg(L=64, barrier=True):
     cost of one node of a replayed hipGraph (an empty kernel, and streaming
     kernels of 40 and 152 B/site at L^3, each reading the previous node's
     output), and of a grid-wide barrier inside one persistent kernel at
-    several block counts, with and without bytes written between barriers
-    (barrier=False skips the barrier's cooperative launches, which the
-    profiler's kernel tracing does not survive)."""
+    several block counts, with and without bytes written between barriers,
+    and of a persistent iteration's halo exchange without a barrier
+    (barrier=False skips both: the barrier's cooperative launches, which the
+    profiler's kernel tracing does not survive, and the halo exchange, whose
+    bounded spins a traced run stretches to milliseconds)."""
     import ctypes as C
     lib = C.CDLL(os.path.join(ROOT, "3d-gross-neveu-model_amd", "lib", "libgnx_probes.so"))
     V = L ** 3
@@ -368,6 +370,8 @@ def probes_leg(L=64, barrier=True):
             d["GB_s"] = bps * sites / (us.value * 1e-6) / 1e9
             d["sites"] = sites
         out["graph_nodes"][name] = d
+    if not barrier:
+        return out
     # the communication of a persistent CG iteration without a grid barrier:
     # one 1024-thread block per 16x8x8 tile (tiles x 1024 sites), two
     # neighbour-face exchanges and one all-block partial gather per round
@@ -388,8 +392,6 @@ def probes_leg(L=64, barrier=True):
                                          "us_per_round": us.value if rc == 0 else None,
                                          "wait_timed_out": bool(failed.value), "stale_values": stale.value,
                                          "blocks_per_cu": per_cu.value, "rc": rc})
-    if not barrier:
-        return out
     for mode, poll in ((0, "acquire load per poll"), (1, "relaxed polls, one acquire fence")):
         for blocks in (256, 512, 1024):
             for per_block in (0, 32768):
@@ -1070,7 +1072,7 @@ def main():
     ap.add_argument("--schedules", default="1,16", metavar="N,...",
                     help="batch sizes of the kernel-schedule comparison at the headline size ('' to skip)")
     ap.add_argument("--probes", default="all", choices=["all", "nodes", "none"],
-                    help="graph-node and grid-barrier probes (nodes: graph-node probes only)")
+                    help="graph-node, grid-barrier and halo-exchange probes (nodes: graph-node probes only)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also time a plain device copy of MIB MiB (the reference kernel for "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE scaling)")
