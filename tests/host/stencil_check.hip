@@ -1,6 +1,7 @@
 // CPU check of the device-side geometry and stencil code (the same inline
 // functions the kernels use, instantiated for the host) against the oracle.
-// Bit-exact: tables, sigma mass, M, M^dag, r-=Mv, r-=M^dag v, sigma force.
+// Bit-exact: tables, sigma mass, M, M^dag, r-=Mv, r-=M^dag v, sigma force;
+// the site code and the gather forms built on it (FP64 and FP32).
 // Build and run by tests/test_host_stencil.py; exit status 0 on success.
 #include <cstdio>
 #include <cstdlib>
@@ -72,6 +73,58 @@ static void check(int L, int LT) {
     double2 r = make_double2(c[2 * f], c[2 * f + 1]);
     double2 y = apply_sub_Mdag(g, decode(g, f), smo[f], r, ld);
     EXPECT(same(y.x, ref[2 * f]) && same(y.y, ref[2 * f + 1]), "subMdag f=%d", f);
+  }
+  // the site code from the load phase: its bits are the coordinates'
+  // parities and boundary flags, and the gather forms that take it (hop
+  // terms by fused multiply-add) equal the separate product-and-sum formula
+  // with the oracle's eta tables, bit for bit, in FP64 and in FP32
+  // (Gaussian operands, so that every sum rounds)
+  const double2* cv = (const double2*)c.data();
+  std::vector<float> c32(2 * V);
+  for (int i = 0; i < 2 * V; i++) c32[i] = (float)c[i];
+  const float2* bf = (const float2*)c32.data();
+  for (int f = 0; f < V; f++) {
+    const Site s = decode(g, f);
+    const SiteGeo sg = site_geo(g, f);
+    const NbrF n = neighbours_f(g, s);
+    unsigned want = (unsigned)(s.x0 & 1) | ((unsigned)((s.x0 + s.x1) & 1) << 1);
+    want |= (s.x2 == LT - 1 ? 4u : 0u) | (s.x2 == 0 ? 8u : 0u) | (s.x0 == L - 1 ? 16u : 0u) |
+            (s.x0 == 0 ? 32u : 0u) | (s.x1 == L - 1 ? 64u : 0u) | (s.x1 == 0 ? 128u : 0u);
+    EXPECT(sg.k.c == want, "site code f=%d: %u vs %u", f, sg.k.c, want);
+    for (int d = 0; d < 3; d++)
+      EXPECT(sg.n.pp[d] == n.pp[d] && sg.n.mm[d] == n.mm[d], "site_geo neighbours f=%d d=%d", f, d);
+    double2 vpp[3], vmm[3];
+    float2 wpp[3], wmm[3];
+    for (int d = 0; d < 3; d++) {
+      vpp[d] = cv[n.pp[d]];
+      vmm[d] = cv[n.mm[d]];
+      wpp[d] = bf[n.pp[d]];
+      wmm[d] = bf[n.mm[d]];
+    }
+    double ex = smo[f] * cv[f].x, ey = smo[f] * cv[f].y;  // M^dag, gather form, separate products
+    double mx = ex, my = ey;                               // M, the reference's order
+    float fx = (float)smo[f] * bf[f].x, fy = (float)smo[f] * bf[f].y;
+    float gx = fx, gy = fy;
+    for (int d = 0; d < 3; d++) {
+      const double hp = 0.5 * og.eta_pp[3 * f + d], hm = 0.5 * og.eta_mm[3 * f + d];
+      ex = ex - hp * vpp[d].x; ey = ey - hp * vpp[d].y;
+      ex = ex + hm * vmm[d].x; ey = ey + hm * vmm[d].y;
+      mx = mx + hp * vpp[d].x; my = my + hp * vpp[d].y;
+      mx = mx + (-hm) * vmm[d].x; my = my + (-hm) * vmm[d].y;
+      const float hpf = (float)hp, hmf = (float)hm;
+      fx = fx - hpf * wpp[d].x; fy = fy - hpf * wpp[d].y;
+      fx = fx + hmf * wmm[d].x; fy = fy + hmf * wmm[d].y;
+      gx = gx + hpf * wpp[d].x; gy = gy + hpf * wpp[d].y;
+      gx = gx + (-hmf) * wmm[d].x; gy = gy + (-hmf) * wmm[d].y;
+    }
+    const double2 yd = apply_Mdag_fast_vals(sg.k, smo[f], cv[f], vpp, vmm);
+    const double2 ym = apply_M_vals(sg.k, smo[f], cv[f], vpp, vmm);
+    const float2 zd = apply_Mdag_fast_vals(sg.k, (float)smo[f], bf[f], wpp, wmm);
+    const float2 zm = apply_M_vals(sg.k, (float)smo[f], bf[f], wpp, wmm);
+    EXPECT(same(yd.x, ex) && same(yd.y, ey), "gather Mdag (FP64) f=%d", f);
+    EXPECT(same(ym.x, mx) && same(ym.y, my), "M from the code (FP64) f=%d", f);
+    EXPECT(zd.x == fx && zd.y == fy, "gather Mdag (FP32) f=%d", f);
+    EXPECT(zm.x == gx && zm.y == gy, "M (FP32) f=%d", f);
   }
   // force from arbitrary X, Y
   std::vector<double> X0(2 * V), Y0(2 * V), X1(2 * V), Y1(2 * V), F(V), Fo(V);
