@@ -30,6 +30,7 @@
 //
 // C ABI (host functions; all return hipError_t as int, 0 on success):
 //   int gnx_probe_graph_nodes(int kind, long sites, int nodes, int reps, double* us_per_node);
+//   int gnx_probe_empty_nodes(int blocks, int threads, int nodes, int reps, double* us_per_node);
 //   int gnx_probe_grid_barrier(int mode, int blocks, int iters, long bytes_per_block,
 //                              double* us_per_barrier, int* max_blocks, int* failed);
 //   mode 0: every poll an acquire load; mode 1: relaxed polls, one acquire fence
@@ -47,6 +48,11 @@ constexpr int kBS = 256;
 
 __global__ __launch_bounds__(kBS) void k_empty(int* sink) {
   if (sink && threadIdx.x == 0 && blockIdx.x == 0x7fffffff) *sink = 1;  // never true: keeps the kernel
+}
+
+// the same for any block size up to 1024
+__global__ __launch_bounds__(1024) void k_empty_any(int* sink) {
+  if (sink && threadIdx.x == 0 && blockIdx.x == 0x7fffffff) *sink = 1;
 }
 
 // 40 B/site: out = a + w*b (two vectors and a field in, one vector out)
@@ -546,6 +552,47 @@ int gnx_probe_graph_nodes(int kind, long sites, int nodes, int reps, double* us_
     if (v[k]) (void)hipFree(v[k]);
   if (w) (void)hipFree(w);
   if (part) (void)hipFree(part);
+  (void)hipStreamDestroy(st);
+  return (int)e;
+}
+
+// empty kernel nodes of a given grid and block size (the boundary's cost as
+// a function of the workgroups it dispatches)
+int gnx_probe_empty_nodes(int blocks, int threads, int nodes, int reps, double* us_per_node) {
+  if (blocks < 1 || (threads != 64 && threads != 128 && threads != 256 && threads != 512 && threads != 1024) ||
+      nodes < 2 || reps < 1 || !us_per_node)
+    return (int)hipErrorInvalidValue;
+  hipError_t e;
+  hipStream_t st = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double best = 1e30;
+  if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess) return (int)e;
+  e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    for (int i = 0; i < nodes; i++)
+      hipLaunchKernelGGL(k_empty_any, dim3(blocks), dim3(threads), 0, st, (int*)nullptr);
+    e = hipStreamEndCapture(st, &graph);
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipGraphLaunch(exec, st);  // warm-up
+  for (int r = 0; r < reps && e == hipSuccess; r++) {
+    e = hipEventRecord(e0, st);
+    if (e == hipSuccess) e = hipGraphLaunch(exec, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    double us = 0.0;
+    if (e == hipSuccess) e = (hipError_t)elapsed_us(e0, e1, &us);
+    if (e == hipSuccess && us < best) best = us;
+  }
+  if (e == hipSuccess) *us_per_node = best / nodes;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
   (void)hipStreamDestroy(st);
   return (int)e;
 }

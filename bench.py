@@ -370,6 +370,14 @@ def probes_leg(L=64, barrier=True):
             d["GB_s"] = bps * sites / (us.value * 1e-6) / 1e9
             d["sites"] = sites
         out["graph_nodes"][name] = d
+    # the empty node against the workgroups it dispatches
+    out["empty_node_by_grid"] = []
+    for threads in (256, 1024):
+        for blocks in (1, 256, 1024, 2048, 4096):
+            us = C.c_double()
+            rc = lib.gnx_probe_empty_nodes(blocks, threads, 64, 20, C.byref(us))
+            out["empty_node_by_grid"].append({"threads_per_block": threads, "blocks": blocks,
+                                              "us_per_node": us.value if rc == 0 else None, "rc": rc})
     if not barrier:
         return out
     # the communication of a persistent CG iteration without a grid barrier:
