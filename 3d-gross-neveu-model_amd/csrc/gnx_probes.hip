@@ -31,6 +31,8 @@
 // C ABI (host functions; all return hipError_t as int, 0 on success):
 //   int gnx_probe_graph_nodes(int kind, long sites, int nodes, int reps, double* us_per_node);
 //   int gnx_probe_empty_nodes(int blocks, int threads, int nodes, int reps, double* us_per_node);
+//   int gnx_probe_width_nodes(int width, long bytes, int nodes, int reps, double* us_per_node);
+//   int gnx_probe_gather_nodes(int width, long elems, int nodes, int reps, double* us_per_node);
 //   int gnx_probe_grid_barrier(int mode, int blocks, int iters, long bytes_per_block,
 //                              double* us_per_barrier, int* max_blocks, int* failed);
 //   mode 0: every poll an acquire load; mode 1: relaxed polls, one acquire fence
@@ -109,6 +111,67 @@ __global__ __launch_bounds__(kBS) void k_write64(long n, double s, double2* __re
   v2[i] = make_double2(s, t);
   v3[i] = make_double2(t, t);
   v4[i] = make_double2(s, s);
+}
+
+// The same bytes through lanes of W bytes: three arrays read and one
+// written, each of `bytes` bytes, element type T (4, 8 or 16 bytes) -- the
+// load/store instruction count per byte scales as 1/W.
+template <typename T>
+__global__ __launch_bounds__(kBS) void k_width(long n, const T* __restrict__ a, const T* __restrict__ b,
+                                              const T* __restrict__ c, T* __restrict__ out) {
+  const long i = (long)blockIdx.x * kBS + threadIdx.x;
+  if (i >= n) return;
+  T x = a[i], y = b[i], z = c[i];
+  unsigned* px = (unsigned*)&x;
+  const unsigned* py = (const unsigned*)&y;
+  const unsigned* pz = (const unsigned*)&z;
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(T) / 4); k++) px[k] ^= py[k] + pz[k];
+  out[i] = x;
+}
+
+// A seven-point gather over two 4-byte-element arrays laid out like one
+// parity of a 64^3 FP16 vector (rows of 32 elements, planes of 2048), one
+// element per lane (W = 4) or four consecutive elements per lane loaded as
+// 16 bytes (W = 16; the +-1 neighbours are then the same 16 bytes shifted
+// by one element, read as two aligned 16-byte loads) -- the same bytes and
+// the same neighbour pattern, 1/4 of the load instructions.  One 4-byte
+// output element per site.
+template <int W>
+__global__ __launch_bounds__(kBS) void k_gather7(long n, const unsigned* __restrict__ a, const unsigned* __restrict__ b,
+                                                unsigned* __restrict__ out) {
+  constexpr long ROW = 32, PLANE = 2048;
+  const long t = (long)blockIdx.x * kBS + threadIdx.x;
+  if (W == 4) {
+    const long i = t;
+    if (i >= n) return;
+    auto at = [&](const unsigned* v, long j) { return v[(j + n) % n]; };
+    unsigned s = 0;
+    const long o[7] = {0, 1, -1, ROW, -ROW, PLANE, -PLANE};
+#pragma unroll
+    for (int k = 0; k < 7; k++) s += at(a, i + o[k]) ^ at(b, i + o[k]);
+    out[i] = s;
+  } else {
+    const long i = 4 * t;  // four consecutive elements
+    if (i >= n) return;
+    const uint4* a4 = (const uint4*)a;
+    const uint4* b4 = (const uint4*)b;
+    const long q = t, nq = n / 4;
+    auto ld = [&](const uint4* v, long j) { return v[(j + nq) % nq]; };
+    uint4 acc = make_uint4(0, 0, 0, 0);
+    auto add = [&](uint4 x, uint4 y) {
+      acc.x += x.x ^ y.x; acc.y += x.y ^ y.y; acc.z += x.z ^ y.z; acc.w += x.w ^ y.w;
+    };
+    const uint4 a0 = ld(a4, q), b0 = ld(b4, q), ap = ld(a4, q + 1), bp = ld(b4, q + 1), am = ld(a4, q - 1),
+                bm = ld(b4, q - 1);
+    add(a0, b0);
+    add(make_uint4(a0.y, a0.z, a0.w, ap.x), make_uint4(b0.y, b0.z, b0.w, bp.x));  // +1
+    add(make_uint4(am.w, a0.x, a0.y, a0.z), make_uint4(bm.w, b0.x, b0.y, b0.z));  // -1
+    const long o[4] = {ROW / 4, -ROW / 4, PLANE / 4, -PLANE / 4};
+#pragma unroll
+    for (int k = 0; k < 4; k++) add(ld(a4, q + o[k]), ld(b4, q + o[k]));
+    ((uint4*)out)[q] = acc;
+  }
 }
 
 struct BarrierState {
@@ -552,6 +615,123 @@ int gnx_probe_graph_nodes(int kind, long sites, int nodes, int reps, double* us_
     if (v[k]) (void)hipFree(v[k]);
   if (w) (void)hipFree(w);
   if (part) (void)hipFree(part);
+  (void)hipStreamDestroy(st);
+  return (int)e;
+}
+
+// nodes of k_width: 3 x bytes read and 1 x bytes written per node, through
+// W-byte lanes (W = 4, 8 or 16), each node reading the previous node's output
+int gnx_probe_width_nodes(int width, long bytes, int nodes, int reps, double* us_per_node) {
+  if ((width != 4 && width != 8 && width != 16) || bytes < 16 || nodes < 2 || reps < 1 || !us_per_node)
+    return (int)hipErrorInvalidValue;
+  hipError_t e;
+  hipStream_t st = nullptr;
+  char* v[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double best = 1e30;
+  const long n = bytes / width;
+  if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess) return (int)e;
+  for (int k = 0; k < 4 && e == hipSuccess; k++) {
+    e = hipMalloc(&v[k], (size_t)n * width);
+    if (e == hipSuccess) e = hipMemsetAsync(v[k], 0, (size_t)n * width, st);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  const unsigned grid = (unsigned)((n + kBS - 1) / kBS);
+  if (e == hipSuccess) e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    for (int i = 0; i < nodes; i++) {
+      char* in0 = v[i & 1];
+      char* o = v[(i + 1) & 1];
+      if (width == 4)
+        hipLaunchKernelGGL(k_width<unsigned>, dim3(grid), dim3(kBS), 0, st, n, (const unsigned*)in0,
+                           (const unsigned*)v[2], (const unsigned*)v[3], (unsigned*)o);
+      else if (width == 8)
+        hipLaunchKernelGGL(k_width<uint2>, dim3(grid), dim3(kBS), 0, st, n, (const uint2*)in0, (const uint2*)v[2],
+                           (const uint2*)v[3], (uint2*)o);
+      else
+        hipLaunchKernelGGL(k_width<uint4>, dim3(grid), dim3(kBS), 0, st, n, (const uint4*)in0, (const uint4*)v[2],
+                           (const uint4*)v[3], (uint4*)o);
+    }
+    e = hipStreamEndCapture(st, &graph);
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipGraphLaunch(exec, st);  // warm-up
+  for (int r = 0; r < reps && e == hipSuccess; r++) {
+    e = hipEventRecord(e0, st);
+    if (e == hipSuccess) e = hipGraphLaunch(exec, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    double us = 0.0;
+    if (e == hipSuccess) e = (hipError_t)elapsed_us(e0, e1, &us);
+    if (e == hipSuccess && us < best) best = us;
+  }
+  if (e == hipSuccess) *us_per_node = best / nodes;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  for (int k = 0; k < 4; k++)
+    if (v[k]) (void)hipFree(v[k]);
+  (void)hipStreamDestroy(st);
+  return (int)e;
+}
+
+// nodes of k_gather7 over arrays of `elems` 4-byte elements (a multiple of
+// 2048), each node reading the previous node's output
+int gnx_probe_gather_nodes(int width, long elems, int nodes, int reps, double* us_per_node) {
+  if ((width != 4 && width != 16) || elems < 4096 || elems % 2048 || nodes < 2 || reps < 1 || !us_per_node)
+    return (int)hipErrorInvalidValue;
+  hipError_t e;
+  hipStream_t st = nullptr;
+  unsigned* v[3] = {nullptr, nullptr, nullptr};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double best = 1e30;
+  if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess) return (int)e;
+  for (int k = 0; k < 3 && e == hipSuccess; k++) {
+    e = hipMalloc(&v[k], sizeof(unsigned) * elems);
+    if (e == hipSuccess) e = hipMemsetAsync(v[k], 0, sizeof(unsigned) * elems, st);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  const long lanes = width == 4 ? elems : elems / 4;
+  const unsigned grid = (unsigned)((lanes + kBS - 1) / kBS);
+  if (e == hipSuccess) e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    for (int i = 0; i < nodes; i++) {
+      const unsigned* in0 = v[i & 1];
+      unsigned* o = v[(i + 1) & 1];
+      if (width == 4)
+        hipLaunchKernelGGL(k_gather7<4>, dim3(grid), dim3(kBS), 0, st, elems, in0, (const unsigned*)v[2], o);
+      else
+        hipLaunchKernelGGL(k_gather7<16>, dim3(grid), dim3(kBS), 0, st, elems, in0, (const unsigned*)v[2], o);
+    }
+    e = hipStreamEndCapture(st, &graph);
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) e = hipGraphLaunch(exec, st);  // warm-up
+  for (int r = 0; r < reps && e == hipSuccess; r++) {
+    e = hipEventRecord(e0, st);
+    if (e == hipSuccess) e = hipGraphLaunch(exec, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    double us = 0.0;
+    if (e == hipSuccess) e = (hipError_t)elapsed_us(e0, e1, &us);
+    if (e == hipSuccess && us < best) best = us;
+  }
+  if (e == hipSuccess) *us_per_node = best / nodes;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  for (int k = 0; k < 3; k++)
+    if (v[k]) (void)hipFree(v[k]);
   (void)hipStreamDestroy(st);
   return (int)e;
 }

@@ -370,6 +370,25 @@ def probes_leg(L=64, barrier=True):
             d["GB_s"] = bps * sites / (us.value * 1e-6) / 1e9
             d["sites"] = sites
         out["graph_nodes"][name] = d
+    # the same bytes through 4-, 8- and 16-byte lanes: a stream (3 arrays in,
+    # 1 out, 33.5 MB each) and a seven-point gather of two arrays laid out
+    # like one parity of 32 FP16 vectors at 64^3 (8.4 M 4-byte elements)
+    out["lane_width"] = {"stream": [], "gather": []}
+    for w in (4, 8, 16):
+        us = C.c_double()
+        nbytes = 33554432
+        rc = lib.gnx_probe_width_nodes(w, C.c_long(nbytes), 64, 10, C.byref(us))
+        out["lane_width"]["stream"].append({"bytes_per_lane": w, "bytes_per_array": nbytes,
+                                            "us_per_node": us.value if rc == 0 else None,
+                                            "GB_s": 4 * nbytes / (us.value * 1e3) if rc == 0 else None, "rc": rc})
+    for w in (4, 16):
+        us = C.c_double()
+        elems = 8388608
+        rc = lib.gnx_probe_gather_nodes(w, C.c_long(elems), 64, 10, C.byref(us))
+        out["lane_width"]["gather"].append({"bytes_per_lane": w, "elements": elems,
+                                            "us_per_node": us.value if rc == 0 else None,
+                                            "unique_GB_s": 12 * elems / (us.value * 1e3) if rc == 0 else None,
+                                            "rc": rc})
     # the empty node against the workgroups it dispatches
     out["empty_node_by_grid"] = []
     for threads in (256, 1024):
