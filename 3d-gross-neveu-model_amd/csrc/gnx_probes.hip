@@ -174,6 +174,38 @@ __global__ __launch_bounds__(kBS) void k_gather7(long n, const unsigned* __restr
   }
 }
 
+// The same gather with each block's tile staged in LDS first: a tile of 2
+// planes x 16 rows x 32 elements (1024 sites, 4 per thread) and its halo, a
+// box of 4 planes x 18 rows x 32 (2304 elements per array), loaded with
+// whole 128-byte rows, then the seven neighbours of every site read from
+// the box.  Flat index arithmetic wraps like k_gather7's.
+__global__ __launch_bounds__(kBS) void k_gather7_lds(long n, const unsigned* __restrict__ a,
+                                                    const unsigned* __restrict__ b, unsigned* __restrict__ out) {
+  constexpr int ROW = 32, PLANE = 2048, BR = 18, BP = 4, SLICE = BR * ROW;  // 576
+  __shared__ unsigned box[2][BP * SLICE];
+  const long tile0 = (long)(blockIdx.x / 4) * 2 * PLANE + (long)(blockIdx.x % 4) * 16 * ROW;
+  // box element (dp, dr, c) <-> flat tile0 + (dp - 1) PLANE + (dr - 1) ROW + c
+  for (int e = threadIdx.x; e < BP * SLICE; e += kBS) {
+    const int dp = e / SLICE, rest = e % SLICE;
+    const long j = ((tile0 + (long)(dp - 1) * PLANE + (long)(rest - ROW)) % n + n) % n;
+    box[0][e] = a[j];
+    box[1][e] = b[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int st = k * kBS + (int)threadIdx.x;  // tile site: planes 1-2, rows 1-16 of the box
+    const int dp = 1 + st / 512, dr = 1 + (st % 512) / 32, c = st % 32;
+    const int e = dp * SLICE + dr * ROW + c;
+    const int o[7] = {0, 1, -1, ROW, -ROW, SLICE, -SLICE};
+    unsigned sum = 0;
+#pragma unroll
+    for (int q = 0; q < 7; q++) sum += box[0][e + o[q]] ^ box[1][e + o[q]];
+    const long i = tile0 + (long)(dp - 1) * PLANE + (long)(dr - 1) * ROW + c;
+    if (i < n) out[i] = sum;
+  }
+}
+
 struct BarrierState {
   unsigned count;
   unsigned gen;
@@ -680,10 +712,12 @@ int gnx_probe_width_nodes(int width, long bytes, int nodes, int reps, double* us
   return (int)e;
 }
 
-// nodes of k_gather7 over arrays of `elems` 4-byte elements (a multiple of
-// 2048), each node reading the previous node's output
+// nodes of k_gather7 (width 4 or 16) or k_gather7_lds (width 0) over arrays
+// of `elems` 4-byte elements (a multiple of 8192), each node reading the
+// previous node's output
 int gnx_probe_gather_nodes(int width, long elems, int nodes, int reps, double* us_per_node) {
-  if ((width != 4 && width != 16) || elems < 4096 || elems % 2048 || nodes < 2 || reps < 1 || !us_per_node)
+  if ((width != 0 && width != 4 && width != 16) || elems < 4096 || elems % 8192 || nodes < 2 || reps < 1 ||
+      !us_per_node)
     return (int)hipErrorInvalidValue;
   hipError_t e;
   hipStream_t st = nullptr;
@@ -699,13 +733,16 @@ int gnx_probe_gather_nodes(int width, long elems, int nodes, int reps, double* u
   }
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   const long lanes = width == 4 ? elems : elems / 4;
-  const unsigned grid = (unsigned)((lanes + kBS - 1) / kBS);
+  // width 0: the LDS-staged form, one block per 1024-site tile
+  const unsigned grid = width == 0 ? (unsigned)(elems / 1024) : (unsigned)((lanes + kBS - 1) / kBS);
   if (e == hipSuccess) e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
     for (int i = 0; i < nodes; i++) {
       const unsigned* in0 = v[i & 1];
       unsigned* o = v[(i + 1) & 1];
-      if (width == 4)
+      if (width == 0)
+        hipLaunchKernelGGL(k_gather7_lds, dim3(grid), dim3(kBS), 0, st, elems, in0, (const unsigned*)v[2], o);
+      else if (width == 4)
         hipLaunchKernelGGL(k_gather7<4>, dim3(grid), dim3(kBS), 0, st, elems, in0, (const unsigned*)v[2], o);
       else
         hipLaunchKernelGGL(k_gather7<16>, dim3(grid), dim3(kBS), 0, st, elems, in0, (const unsigned*)v[2], o);

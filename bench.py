@@ -381,11 +381,11 @@ def probes_leg(L=64, barrier=True):
         out["lane_width"]["stream"].append({"bytes_per_lane": w, "bytes_per_array": nbytes,
                                             "us_per_node": us.value if rc == 0 else None,
                                             "GB_s": 4 * nbytes / (us.value * 1e3) if rc == 0 else None, "rc": rc})
-    for w in (4, 16):
+    for w in (4, 16, 0):  # 0: each block's tile and halo staged in LDS first
         us = C.c_double()
         elems = 8388608
         rc = lib.gnx_probe_gather_nodes(w, C.c_long(elems), 64, 10, C.byref(us))
-        out["lane_width"]["gather"].append({"bytes_per_lane": w, "elements": elems,
+        out["lane_width"]["gather"].append({"bytes_per_lane": w if w else 4, "lds_staged": w == 0, "elements": elems,
                                             "us_per_node": us.value if rc == 0 else None,
                                             "unique_GB_s": 12 * elems / (us.value * 1e3) if rc == 0 else None,
                                             "rc": rc})
