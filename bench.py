@@ -1253,6 +1253,8 @@ def main():
                    "true_residual": true_res,
                    "relative_residual": true_res / float(torch.linalg.norm(b).item()),
                    "cg_shape": ctx.cg_shape(1)},
+        # CG RHS-iterations/s: right-hand sides times CG iterations per second
+        "rhs_iterations_per_s": solves_per_s * it,
         "dirac": {"apps_per_s": 1.0 / t_app, "GB_s": dirac_gbs, "frac_hbm": dirac_gbs / HBM_PEAK_GBS,
                   "bytes_per_site": BYTES_DIRAC_F64},
         "dirac_graph": dirac_graph,
@@ -1296,7 +1298,8 @@ def main():
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         its = [r.iterations for r in res]
-        batched[str(nb)] = {"rhs_solves_per_s": nb * reps / dt, "ms_per_batch": dt / reps * 1e3,
+        batched[str(nb)] = {"rhs_solves_per_s": nb * reps / dt, "rhs_iterations_per_s": sum(its) * reps / dt,
+                            "ms_per_batch": dt / reps * 1e3,
                             "iterations_max": max(its), "iterations_min": min(its),
                             "all_converged": all(r.converged for r in res), "shape": ctx.cg_shape(nb),
                             "algorithmic_GB_s": BYTES_CG_ITER_F64 * V * sum(its) / (dt / reps) / 1e9}
@@ -1326,7 +1329,9 @@ def main():
             ctx.dirac(gnx.OP_MMDAG, sm, X[:2 * V], tr)
             torch.cuda.synchronize()
             mixed[f"{pname}_x{nb}"] = {
-                "rhs_solves_per_s": nb * reps / dt, "ms_per_batch": dt / reps * 1e3,
+                "rhs_solves_per_s": nb * reps / dt,
+                "rhs_iterations_per_s": sum(r.iterations for r in res) * reps / dt,
+                "ms_per_batch": dt / reps * 1e3,
                 "iterations_max": max(r.iterations for r in res),
                 "reliable_updates_max": max(r.reliable_updates for r in res),
                 "all_converged": all(r.converged for r in res),
