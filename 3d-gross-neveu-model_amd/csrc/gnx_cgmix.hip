@@ -31,6 +31,7 @@
 #include "gnx_cgstate.h"
 #include "gnx_engine.h"
 #include "gnx_internal.h"
+#include "gnx_group.h"
 #include "gnx_solver.h"
 
 namespace gnx {
@@ -71,14 +72,13 @@ struct StoreBF16 {
   using T = unsigned int;
   static constexpr bool kScaled = false;
   GNX_DEVONLY static float up(unsigned int h) { return __uint_as_float(h << 16); }
-  GNX_DEVONLY static float2 ld(const T* p, int i) {
-    unsigned int v = *elem(p, i);
-    return make_float2(up(v & 0xFFFFu), up(v >> 16));
-  }
-  GNX_DEVONLY static void st(T* p, int i, float2 v, int nt) {
+  GNX_DEVONLY static float2 cvt(T v) { return make_float2(up(v & 0xFFFFu), up(v >> 16)); }
+  GNX_DEVONLY static T pack(float2 v) {
     const gnx_f32x2 w = {v.x, v.y};
-    st_mix(elem(p, i), __builtin_bit_cast(unsigned int, __builtin_convertvector(w, gnx_bf16x2)), nt);
+    return __builtin_bit_cast(unsigned int, __builtin_convertvector(w, gnx_bf16x2));
   }
+  GNX_DEVONLY static float2 ld(const T* p, int i) { return cvt(*elem(p, i)); }
+  GNX_DEVONLY static void st(T* p, int i, float2 v, int nt) { st_mix(elem(p, i), pack(v), nt); }
 };
 
 // IEEE half pair (re in the low half), round-to-nearest-even on store (one
@@ -91,14 +91,16 @@ typedef _Float16 gnx_f16x2 __attribute__((ext_vector_type(2)));
 struct StoreF16 {
   using T = unsigned int;
   static constexpr bool kScaled = true;
-  GNX_DEVONLY static float2 ld(const T* p, int i) {
-    const gnx_f32x2 w = __builtin_convertvector(__builtin_bit_cast(gnx_f16x2, *elem(p, i)), gnx_f32x2);
+  GNX_DEVONLY static float2 cvt(T v) {
+    const gnx_f32x2 w = __builtin_convertvector(__builtin_bit_cast(gnx_f16x2, v), gnx_f32x2);
     return make_float2(w.x, w.y);
   }
-  GNX_DEVONLY static void st(T* p, int i, float2 v, int nt) {
+  GNX_DEVONLY static T pack(float2 v) {
     const gnx_f32x2 w = {v.x, v.y};
-    st_mix(elem(p, i), __builtin_bit_cast(unsigned int, __builtin_convertvector(w, gnx_f16x2)), nt);
+    return __builtin_bit_cast(unsigned int, __builtin_convertvector(w, gnx_f16x2));
   }
+  GNX_DEVONLY static float2 ld(const T* p, int i) { return cvt(*elem(p, i)); }
+  GNX_DEVONLY static void st(T* p, int i, float2 v, int nt) { st_mix(elem(p, i), pack(v), nt); }
 };
 
 constexpr int MIX_BS = ENG_BS;
@@ -821,6 +823,274 @@ __global__ __launch_bounds__(MIX_BS) void k_mix_Ye(MixFix a, int sl, const MixSt
   }
 }
 
+// ---- loads-first kernels on groups of four consecutive sites ------------------------
+// For the 4-byte storage formats (BF16, FP16) on lattices whose time extent
+// is a multiple of 8: a thread owns one group of four consecutive f-sites
+// (gnx_group.h), so its operands of one vector are seven aligned 16-byte
+// loads for four sites, against twenty-eight 4-byte loads in k_mix_Xe /
+// k_mix_Ye.  Each site's arithmetic is theirs (the same helpers on the same
+// operands, the same state logic and reliable updates); the partition of
+// the global sums differs (a thread's four sites are consecutive), so the
+// iterates are this shape's own.  The blocks and partial layout are those of
+// SiteMap<4, true>, so the start kernels of that shape are shared.
+// Measured against the loads-first shapes at 64^3 and 96^3 (BF16, FP16; 1,
+// 16 and 32 systems) it is never more than 2 % faster than the rule's
+// shape and up to 6 % slower on batches: the gathers stay bound by their
+// cache lines, not by the number of load instructions.  Reached by pin or
+// by the tuner, not by the rule.
+GNX_DEVONLY gnx_u4 ld_u4(const unsigned int* p, int i) { return *(const gnx_u4*)elem(p, i); }
+GNX_DEVONLY void st_u4(unsigned int* p, int i, gnx_u4 v, int nt) {
+  if (nt)
+    __builtin_nontemporal_store(v, (gnx_u4*)elem(p, i));
+  else
+    *(gnx_u4*)elem(p, i) = v;
+}
+typedef float gnx_f4v __attribute__((ext_vector_type(4)));
+
+template <typename S, bool FLUSH>
+__global__ __launch_bounds__(MIX_BS) void k_mix_Xq(MixFix a, int sl, const MixState* __restrict__ sin,
+                                                  MixState* __restrict__ sout, const double* __restrict__ pin,
+                                                  double* __restrict__ pout) {
+  __shared__ double lds[MIX_BS / 64], lds2[MIX_BS / 64];  // head and tail reductions
+  using T = typename S::T;
+  static_assert(sizeof(T) == 4, "four sites per 16-byte load");
+  MixOne v;
+  v.init(a);
+  const Geom g = v.g;
+  const int rhs = blockIdx.y;
+  const size_t off = v.off;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const T* r = (const T*)a.r + off;
+  const T* pold = (const T*)mix_p(a, sl) + off;
+  const T* pprev = (const T*)mix_p(a, sl + 1) + off;  // = pnew: own sites only, read before the store
+  T* pnew = (T*)mix_p(a, sl + 1) + off;
+  const MixState s = load_state(sin + rhs);
+  double pl[PART_K];
+  partial_issue(pin + v.poff, v.nb, pl);
+  end_of_loads();
+  const int f0 = group_first(g, v.nb, blockIdx.x, threadIdx.x);
+  const int fc = f0 < 0 ? 0 : f0;  // past the last group: load group 0, store nothing
+  const GroupGeo q = group_geo(g, fc);
+  const int gi[7] = {q.own, q.pp0, q.mm0, q.pp1, q.mm1, q.g2, q.s2};
+  const gnx_f4v sm4 = *(const gnx_f4v*)elem(a.sm32 + v.soff, fc);
+  gnx_u4 rw[7], pw[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) {
+    rw[k] = ld_u4(r, gi[k]);
+    pw[k] = ld_u4(pold, gi[k]);
+  }
+  gnx_f4v xs01 = {0.f, 0.f, 0.f, 0.f}, xs23 = {0.f, 0.f, 0.f, 0.f};
+  gnx_u4 ppw = {0u, 0u, 0u, 0u};
+  if (FLUSH) {
+    xs01 = *(const gnx_f4v*)elem(a.xs + off, fc);
+    xs23 = *(const gnx_f4v*)elem(a.xs + off, fc + 2);
+    ppw = ld_u4(pprev, fc);
+  }
+  end_of_loads();
+  const double rho = block_sum_once(partial_finish(pin + v.poff, v.nb, pl), lds);
+  const double nr = S::kScaled ? sqrt(rho) / s.scale : sqrt(rho);  // the unscaled norm
+  MixState* out = sout + rhs;
+  if (!s.active) {
+    if (lead) put_mstate(out, 0, s.iters, s.phase, s.nrel, s.converged, s.early, s.rho, s.rmax, s);
+    return;
+  }
+  double beta = 0.0;
+  double rmax = s.rmax;
+  int nrel = s.nrel;
+  bool update = false;
+  if (s.phase == PH_UPDATED) {
+    nrel = s.nrel + 1;
+    if (nr < s.tol || s.iters >= s.max_iter) {
+      const int conv = nr < s.tol ? 1 : 0;
+      if (lead) {
+        put_mstate(out, 0, s.iters, PH_NORMAL, nrel, conv, 0, rho, nr, s);
+        CGDevResult res = {conv, 0, s.iters, nrel, nr, {0, 0, 0}};
+        a.res[rhs] = res;
+      }
+      return;
+    }
+    rmax = nr;
+    beta = (S::kScaled && s.iters == 0 && s.nrel == 0) ? 0.0 : rho / s.rho;
+  } else if (s.iters == 0) {
+    if (s.start == GNX_START_IMPOSED && nr < s.tol) {  // conjugategradient.c:123-128
+      if (lead) {
+        put_mstate(out, 0, 0, PH_NORMAL, 0, 1, 1, rho, nr, s);
+        CGDevResult res = {1, 1, 0, 0, nr, {0, 0, 0}};
+        a.res[rhs] = res;
+      }
+      return;
+    }
+    rmax = nr;
+    update = S::kScaled;
+  } else {
+    update = (nr < s.delta * s.rmax) || (nr < s.tol) || (s.iters >= s.max_iter);
+    beta = rho / s.rho;
+  }
+  const float fa = (float)s.alpha_a, fp = (float)s.alpha_b;
+  if (update) {
+    // reliable update, part 1, per site as in k_mix_Xe
+    const double2* x = a.x + off;
+    const float2* xs = a.xs + off;
+    auto ldx = [&](int qq) {
+      float2 c = cfma(fa, S::ld(pold, qq), cfma(fp, S::ld(pprev, qq), xs[qq]));
+      return cadd(x[qq], make_double2((double)c.x, (double)c.y));
+    };
+    if (f0 >= 0) {
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        a.u[off + f0 + j] = apply_Mdag(g, decode(g, f0 + j), a.sm[v.soff + f0 + j], ldx);
+    }
+    if (lead) {
+      put_mstate(out, 1, s.iters, PH_UPDATE, nrel, 0, 0, s.rho, rmax, s);
+      if (S::kScaled) out->scale_next = nr > 0.0 ? sqrt((double)g.V) / nr : s.scale;
+    }
+    return;
+  }
+  // normal step: (even slots) xs += alpha_b p_prev + alpha_a p_old;
+  // p = r + beta p, t = M^dag p
+  const float fb = (float)beta;
+  double acc = 0.0;
+  if (f0 >= 0) {
+    T ro[4][7], po[4][7];
+    group_operands<T>(rw, q.even, ro);
+    group_operands<T>(pw, q.even, po);
+    const T ppr[4] = {ppw.x, ppw.y, ppw.z, ppw.w};
+    const float2 xsv[4] = {make_float2(xs01.x, xs01.y), make_float2(xs01.z, xs01.w), make_float2(xs23.x, xs23.y),
+                           make_float2(xs23.z, xs23.w)};
+    const float smv[4] = {sm4.x, sm4.y, sm4.z, sm4.w};
+    float2 xn[4];
+    T pn[4], tn[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (FLUSH) xn[j] = cfma(fa, S::cvt(po[j][0]), cfma(fp, S::cvt(ppr[j]), xsv[j]));
+      float2 pv[7];
+#pragma unroll
+      for (int k = 0; k < 7; k++) pv[k] = cfma(fb, S::cvt(po[j][k]), S::cvt(ro[j][k]));
+      pn[j] = S::pack(pv[0]);
+      const float2 tt = apply_Mdag_fast_vals(q.k[j], smv[j], pv[0], &pv[1], &pv[4]);
+      tn[j] = S::pack(tt);
+      acc += dnorm2(tt);
+    }
+    if (FLUSH) {
+      const gnx_f4v w01 = {xn[0].x, xn[0].y, xn[1].x, xn[1].y}, w23 = {xn[2].x, xn[2].y, xn[3].x, xn[3].y};
+      if (a.nt) {
+        __builtin_nontemporal_store(w01, (gnx_f4v*)elem(a.xs + off, f0));
+        __builtin_nontemporal_store(w23, (gnx_f4v*)elem(a.xs + off, f0 + 2));
+      } else {
+        *(gnx_f4v*)elem(a.xs + off, f0) = w01;
+        *(gnx_f4v*)elem(a.xs + off, f0 + 2) = w23;
+      }
+    }
+    const gnx_u4 pn4 = {pn[0], pn[1], pn[2], pn[3]}, tn4 = {tn[0], tn[1], tn[2], tn[3]};
+    st_u4(pnew, f0, pn4, a.nt);
+    st_u4((T*)a.t + off, f0, tn4, a.nt);
+  }
+  acc = block_sum_once(acc, lds2);
+  if (threadIdx.x == 0) pout[v.poff + partial_slot(blockIdx.x, v.nb)] = acc;
+  if (lead) {
+    put_mstate(out, 1, s.iters, PH_NORMAL, nrel, 0, 0, rho, rmax, s);
+    if (FLUSH) put_alphas(out, 0.0, 0.0);
+  }
+}
+
+template <typename S>
+__global__ __launch_bounds__(MIX_BS) void k_mix_Yq(MixFix a, int sl, const MixState* __restrict__ sin,
+                                                  MixState* __restrict__ sout, const double* __restrict__ pin,
+                                                  double* __restrict__ pout) {
+  __shared__ double lds[MIX_BS / 64], lds2[MIX_BS / 64];  // head and tail reductions
+  using T = typename S::T;
+  static_assert(sizeof(T) == 4, "four sites per 16-byte load");
+  MixOne v;
+  v.init(a);
+  const Geom g = v.g;
+  const int rhs = blockIdx.y;
+  const size_t off = v.off;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const T* t = (const T*)a.t + off;
+  T* r = (T*)a.r + off;
+  const MixState s = load_state(sin + rhs);
+  double pl[PART_K];
+  partial_issue(pin + v.poff, v.nb, pl);
+  end_of_loads();
+  const int f0 = group_first(g, v.nb, blockIdx.x, threadIdx.x);
+  const int fc = f0 < 0 ? 0 : f0;
+  const GroupGeo q = group_geo(g, fc);
+  const int gi[7] = {q.own, q.pp0, q.mm0, q.pp1, q.mm1, q.g2, q.s2};
+  const gnx_f4v sm4 = *(const gnx_f4v*)elem(a.sm32 + v.soff, fc);
+  gnx_u4 tw[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) tw[k] = ld_u4(t, gi[k]);
+  const gnx_u4 rw = ld_u4(r, fc);
+  end_of_loads();
+  const double gamma = block_sum_once(partial_finish(pin + v.poff, v.nb, pl), lds);
+  MixState* out = sout + rhs;
+  if (!s.active) {
+    if (lead) put_mstate(out, 0, s.iters, s.phase, s.nrel, s.converged, s.early, s.rho, s.rmax, s);
+    return;
+  }
+  double acc = 0.0;
+  if (s.phase == PH_UPDATE) {
+    // reliable update, part 2, per site as in k_mix_Ye
+    const double2* u = a.u + off;
+    const T* pold = (const T*)mix_p(a, sl) + off;
+    T* pnext = (T*)mix_p(a, sl + 1) + off;
+    const float fa = (float)s.alpha_a, fp = (float)s.alpha_b;
+    const double sn = S::kScaled ? s.scale_next : 1.0;
+    const double ratio = S::kScaled ? s.scale_next / s.scale : 1.0;
+    const float fr = (float)ratio;
+    if (f0 >= 0) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int f = f0 + j;
+        const float2 po = S::ld(pold, f);
+        const float2 c = cfma(fa, po, cfma(fp, S::ld(pnext, f), a.xs[off + f]));
+        S::st(pnext, f, S::kScaled ? make_float2(fr * po.x, fr * po.y) : po, a.nt);
+        a.x[off + f] = cadd(a.x[off + f], make_double2((double)c.x, (double)c.y));
+        a.xs[off + f] = make_float2(0.f, 0.f);
+        double2 rr = apply_sub_M(g, decode(g, f), a.sm[v.soff + f], a.b[off + f], [&](int qq) { return u[qq]; });
+        if (S::kScaled) rr = make_double2(sn * rr.x, sn * rr.y);
+        S::st(r, f, make_float2((float)rr.x, (float)rr.y), a.nt);
+        acc += cnorm2(rr);
+      }
+    }
+    acc = block_sum_once(acc, lds2);
+    if (threadIdx.x == 0) pout[v.poff + partial_slot(blockIdx.x, v.nb)] = acc;
+    if (lead) {
+      put_mstate(out, 1, s.iters, PH_UPDATED, s.nrel, 0, 0, s.rho * ratio * ratio, s.rmax, s);
+      put_alphas(out, 0.0, 0.0);
+      if (S::kScaled) out->scale = sn;
+    }
+    return;
+  }
+  // normal step: r -= alpha M t; alpha p is left pending for the X kernel
+  const float al = (float)(s.rho / gamma);
+  if (f0 >= 0) {
+    T to[4][7];
+    group_operands<T>(tw, q.even, to);
+    const T rv[4] = {rw.x, rw.y, rw.z, rw.w};
+    const float smv[4] = {sm4.x, sm4.y, sm4.z, sm4.w};
+    T rn[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      float2 tv[7];
+#pragma unroll
+      for (int k = 0; k < 7; k++) tv[k] = S::cvt(to[j][k]);
+      const float2 w = apply_M_vals(q.k[j], smv[j], tv[0], &tv[1], &tv[4]);
+      const float2 rr = cfma(-al, w, S::cvt(rv[j]));
+      rn[j] = S::pack(rr);
+      acc += dnorm2(rr);
+    }
+    const gnx_u4 rn4 = {rn[0], rn[1], rn[2], rn[3]};
+    st_u4(r, f0, rn4, a.nt);
+  }
+  acc = block_sum_once(acc, lds2);
+  if (threadIdx.x == 0) pout[v.poff + partial_slot(blockIdx.x, v.nb)] = acc;
+  if (lead) {
+    put_mstate(out, 1, s.iters + 1, PH_NORMAL, s.nrel, 0, 0, s.rho, s.rmax, s);
+    put_alphas(out, S::kScaled ? (double)al / s.scale : (double)al, s.alpha_a);
+  }
+}
+
 // ---- paired iteration kernels -------------------------------------------------------
 // Two systems per block (blockIdx.y = pair r0 = 2 y, r0 + 1): the site decode,
 // neighbours, phases and addresses are shared by the pair; each system keeps
@@ -1169,6 +1439,7 @@ struct MixShape {
   const void* null_p1;
   const void* null_p2;
   const void* null_p3;
+  bool quad = false;  // four consecutive sites per thread (k_mix_Xq / k_mix_Yq)
 };
 
 #define GNX_MIX_SHAPE(S, N, X)                                                                     \
@@ -1186,8 +1457,25 @@ struct MixShape {
    (const void*)k_mix_Ye<S, N, X>,                                                                \
    (const void*)k_mix_start_imposed<S, N, X>, (const void*)k_mix_null_p1<N, X>,                   \
    (const void*)k_mix_null_p2<N, X>, (const void*)k_mix_null_p3<S, N, X>}
+// the group kernels exist for the 4-byte storage formats only
+template <typename S, bool FLUSH>
+static const void* mix_xq() {
+  if constexpr (sizeof(typename S::T) == 4) return (const void*)k_mix_Xq<S, FLUSH>;
+  return nullptr;
+}
+template <typename S>
+static const void* mix_yq() {
+  if constexpr (sizeof(typename S::T) == 4) return (const void*)k_mix_Yq<S>;
+  return nullptr;
+}
+#define GNX_MIX_QUAD(S)                                                                            \
+  {4, true, false, true, mix_xq<S, false>(), mix_xq<S, true>(), mix_yq<S>(),                      \
+   (const void*)k_mix_start_imposed<S, 4, true>, (const void*)k_mix_null_p1<4, true>,             \
+   (const void*)k_mix_null_p2<4, true>, (const void*)k_mix_null_p3<S, 4, true>, true}
 // shapes 0..5: one system per block; 6..9: two (batches of two or more);
-// 10..15: one system per block, loads-first kernels
+// 10..15: one system per block, loads-first kernels; 16: loads-first on
+// groups of four consecutive sites (4-byte storage, LT a multiple of 8; the
+// start kernels and partial layout of shape 15)
 template <typename S>
 static const MixShape* mix_shapes() {
   static const MixShape t[] = {GNX_MIX_SHAPE(S, 1, false), GNX_MIX_SHAPE(S, 1, true), GNX_MIX_SHAPE(S, 2, false),
@@ -1195,13 +1483,24 @@ static const MixShape* mix_shapes() {
                                GNX_MIX_PAIR(S, 1, false),  GNX_MIX_PAIR(S, 1, true),   GNX_MIX_PAIR(S, 2, false),
                                GNX_MIX_PAIR(S, 2, true),   GNX_MIX_EARLY(S, 1, false), GNX_MIX_EARLY(S, 1, true),
                                GNX_MIX_EARLY(S, 2, false), GNX_MIX_EARLY(S, 2, true),  GNX_MIX_EARLY(S, 4, false),
-                               GNX_MIX_EARLY(S, 4, true)};
+                               GNX_MIX_EARLY(S, 4, true),  GNX_MIX_QUAD(S)};
   return t;
 }
 #undef GNX_MIX_SHAPE
 #undef GNX_MIX_PAIR
 #undef GNX_MIX_EARLY
-constexpr int kMixShapes = 16;
+#undef GNX_MIX_QUAD
+constexpr int kMixShapes = 17;
+
+// whether a shape runs this storage format, lattice and batch
+template <typename S>
+static bool mix_fits(const Geom& g, int nrhs, int shape) {
+  const MixShape& sh = mix_shapes<S>()[shape];
+  if (sh.pair && nrhs < 2) return false;
+  if (S::kScaled && !sh.early) return false;  // the scale lives in the loads-first form
+  if (sh.quad && (sh.X == nullptr || g.LT % 8 != 0)) return false;
+  return true;
+}
 
 static MixFix mix_fix(MixMachine* m, const Geom& g, int nrhs, int spt, int store_bytes) {
   MixFix a;
@@ -1285,9 +1584,10 @@ static hipError_t mix_begin(MixMachine* m, const Geom& g, const double* sm, cons
 // system; loads-first kernels for 2-byte storage (FP16 requires them: the
 // per-system scale lives there) and for single systems, the interleaved
 // form for FP32 batches (bandwidth-bound, where the loads-first form loses
-// occupancy) -- the tuner's picks at 64^3.  GNX_MIX_SHAPE=<0..15> pins one;
-// autotune times two chunks of every shape on the caller's own system and
-// keeps the fastest per (batch size, storage format).
+// occupancy) -- the tuner's picks at 64^3.  GNX_MIX_SHAPE=<0..16> pins one
+// (ignored where it does not fit); autotune times two chunks of every shape
+// that fits on the caller's own system and keeps the fastest per (batch
+// size, storage format).
 template <typename S>
 static int mix_rule(const Geom& g, int nrhs) {
   int k = 0;
@@ -1312,15 +1612,13 @@ static hipError_t mix_choose(MixMachine* m, const Geom& g, const double* sm, con
   int best_shape = mix_rule<S>(g, nrhs);
   hipError_t e = hipSuccess;
   const char* env = getenv("GNX_MIX_SHAPE");
-  if (env && atoi(env) >= 0 && atoi(env) < kMixShapes && (nrhs >= 2 || !mix_shapes<S>()[atoi(env)].pair) &&
-      (!S::kScaled || mix_shapes<S>()[atoi(env)].early)) {
+  if (env && atoi(env) >= 0 && atoi(env) < kMixShapes && mix_fits<S>(g, nrhs, atoi(env))) {
     best_shape = atoi(env);
   } else if (m->autotune) {
     E.tune_runs++;
     float best_ms = 1e30f;
     for (int sh = 0; sh < kMixShapes && e == hipSuccess; sh++) {
-      if (mix_shapes<S>()[sh].pair && nrhs < 2) continue;
-      if (S::kScaled && !mix_shapes<S>()[sh].early) continue;  // the scale lives in the loads-first form
+      if (!mix_fits<S>(g, nrhs, sh)) continue;
       hipGraphExec_t exec;
       e = mix_graph<S>(m, g, nrhs, sh, tag, &exec);
       float ms_min = 1e30f;
@@ -1389,7 +1687,7 @@ int cgmix_shape_of(void* machine, int nrhs, int prec, int* spt, int* xcd, int* p
   if (t < 0) return -1;
   const MixShape& sh = mix_shapes<StoreF32>()[t];
   *spt = sh.spt;
-  *xcd = sh.xcd ? 1 : 0;
+  *xcd = (sh.xcd ? 1 : 0) | (sh.quad ? 2 : 0);
   *pair = sh.pair ? 1 : 0;
   *early = sh.early ? 1 : 0;
   return t;

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libgnx.so")
 OP_M, OP_MDAG, OP_MMDAG, OP_MDAGM, OP_SUB_M, OP_SUB_MDAG = range(6)
 START_NULL, START_IMPOSED = 0, 1
 PREC_F64, PREC_F32, PREC_BF16, PREC_F16 = 0, 1, 2, 3
-ABI_VERSION = 7  # GNX_ABI_VERSION of include/gnx.h this binding follows
+ABI_VERSION = 8  # GNX_ABI_VERSION of include/gnx.h this binding follows
 
 STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_NOCONV", 4: "ERR_SIZE", 5: "ERR_STATE"}
 
@@ -323,8 +323,9 @@ class Context:
         spt, xcd, pair, early = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         st = lib().gnx_cg_mixed_shape(self.h, nrhs, prec, C.byref(spt), C.byref(xcd), C.byref(pair),
                                       C.byref(early))
-        return None if st != 0 else {"sites_per_thread": spt.value, "xcd_slab": bool(xcd.value),
-                                     "pair": bool(pair.value), "loads_first": bool(early.value)}
+        return None if st != 0 else {"sites_per_thread": spt.value, "xcd_slab": bool(xcd.value & 1),
+                                     "pair": bool(pair.value), "loads_first": bool(early.value),
+                                     "consecutive": bool(xcd.value & 2)}
 
     # -- A9/A10 -----------------------------------------------------------
     def sigma_force(self, sigma, sm, beta_s, tol, phi, X, Y, force, max_iter=0, nflav=4):

@@ -35,7 +35,7 @@ extern "C" {
  * 6: bit 3 of xcd_mapped marks the fused kernels (one per iteration);
  *    gnx_multi_link_cg, gnx_multi_cg_prec; link-coupled lockstep chains
  * 7: bit 4 of xcd_mapped marks the one-launch solve of small lattices */
-#define GNX_ABI_VERSION 7
+#define GNX_ABI_VERSION 8
 
 typedef struct gnx_ctx gnx_ctx;
 typedef struct gnx_multi gnx_multi;
@@ -242,7 +242,10 @@ gnx_status gnx_cg_shape(gnx_ctx *ctx, int nrhs, int *sites_per_thread, int *xcd_
  * less latency).  GNX_MIX_SHAPE=<n> pins one: n = 0..5 one system per block
  * with 1, 1, 2, 2, 4, 4 sites per thread, n = 6..9 two systems per block with
  * 1, 1, 2, 2, n = 10..15 as 0..5 with loads-first kernels; odd n use the XCD
- * slab mapping. */
+ * slab mapping; n = 16 loads-first kernels on groups of four consecutive
+ * sites (16-byte loads; BF16 and FP16 storage with LT a multiple of 8 only, a
+ * pin that does not fit is ignored), reported as xcd_mapped = 3 (bit 1: the
+ * groups). */
 gnx_status gnx_cg_mixed_shape(gnx_ctx *ctx, int nrhs, gnx_prec prec, int *sites_per_thread, int *xcd_mapped,
                               int *pair, int *loads_first);
 

@@ -1,7 +1,8 @@
 // CPU check of the device-side geometry and stencil code (the same inline
 // functions the kernels use, instantiated for the host) against the oracle.
 // Bit-exact: tables, sigma mass, M, M^dag, r-=Mv, r-=M^dag v, sigma force;
-// the site code and the gather forms built on it (FP64 and FP32).
+// the site code and the gather forms built on it (FP64 and FP32); the
+// four-site group map and operand assembly.
 // Build and run by tests/test_host_stencil.py; exit status 0 on success.
 #include <cstdio>
 #include <cstdlib>
@@ -9,6 +10,7 @@
 #include <vector>
 
 #include "../../3d-gross-neveu-model_amd/csrc/gnx_stencil.h"
+#include "../../3d-gross-neveu-model_amd/csrc/gnx_group.h"
 #include "../../3d-gross-neveu-model_amd/csrc/gnx_tile.h"
 #include "../../oracle/gnx_oracle.h"
 
@@ -214,7 +216,49 @@ static void check_tiles(int L, int LT) {
   for (int f = 0; f < g.V; f++) EXPECT(owner[f] == 1, "tile %dx%dx%d L=%d: site %d owned %d times", T0, T1, T2, L, f, owner[f]);
 }
 
+// Four-site groups (LT a multiple of 8): the thread-to-group map covers every
+// group once with aligned, in-range first sites, and each group's seven
+// 16-byte loads and operand assembly give every site the operand indices and
+// site code of its own decode.
+static void check_groups(int L, int LT) {
+  const Geom g = make_geom(L, LT);
+  const int nb = (g.V + 4 * ENG_BS - 1) / (4 * ENG_BS);
+  std::vector<int> seen(g.V / 4, 0);
+  for (int b = 0; b < nb; b++)
+    for (int t = 0; t < ENG_BS; t++) {
+      const int f0 = group_first(g, nb, b, t);
+      if (f0 < 0) continue;
+      EXPECT(f0 % 4 == 0 && f0 + 3 < g.V, "groups %dx%d: block %d thread %d first site %d", L, LT, b, t, f0);
+      if (f0 % 4 == 0 && f0 >= 0 && f0 + 3 < g.V) seen[f0 / 4]++;
+    }
+  for (int h = 0; h < g.V / 4; h++) EXPECT(seen[h] == 1, "groups %dx%d: group %d owned %d times", L, LT, h, seen[h]);
+  for (int f0 = 0; f0 < g.V; f0 += 4) {
+    const GroupGeo q = group_geo(g, f0);
+    const int gi[7] = {q.own, q.pp0, q.mm0, q.pp1, q.mm1, q.g2, q.s2};
+    gnx_u4 w[7];
+    for (int k = 0; k < 7; k++) {
+      EXPECT(gi[k] % 4 == 0 && gi[k] >= 0 && gi[k] + 3 < g.V, "groups %dx%d: group %d load %d at %d", L, LT, f0, k,
+             gi[k]);
+      for (int e = 0; e < 4; e++) w[k][e] = (unsigned)(gi[k] + e);
+    }
+    unsigned o[4][7];
+    group_operands<unsigned>(w, q.even, o);
+    for (int j = 0; j < 4; j++) {
+      const Site s = decode(g, f0 + j);
+      const NbrF n = neighbours_f(g, s);
+      const int want[7] = {f0 + j, n.pp[0], n.pp[1], n.pp[2], n.mm[0], n.mm[1], n.mm[2]};
+      for (int k = 0; k < 7; k++)
+        EXPECT((int)o[j][k] == want[k], "groups %dx%d: site %d operand %d is %u, expected %d", L, LT, f0 + j, k,
+               o[j][k], want[k]);
+      EXPECT(q.k[j].c == site_code(g, s).c, "groups %dx%d: site %d code %u, expected %u", L, LT, f0 + j, q.k[j].c,
+             site_code(g, s).c);
+    }
+  }
+}
+
 int main() {
+  int gsizes[][2] = {{4, 8}, {8, 8}, {12, 8}, {6, 24}, {16, 16}, {16, 32}, {10, 40}, {64, 64}, {96, 96}};
+  for (auto& s : gsizes) check_groups(s[0], s[1]);
   check_tiles<4, 8, 16>(16, 16);
   check_tiles<4, 8, 16>(8, 16);
   check_tiles<4, 8, 16>(32, 32);

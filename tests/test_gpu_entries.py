@@ -142,7 +142,7 @@ def test_shape_rule_is_default_and_deterministic(P, gnx_mod):
         assert ctx.cg_shape(1) == {"sites_per_thread": 2, "xcd_slab": True, "one_reduction": True,
                                    "deferred": False, "fused": False, "one_launch": False}
         assert ctx.cg_mixed_shape(1, gnx_mod.PREC_F16) == {"sites_per_thread": 2, "xcd_slab": True, "pair": False,
-                                                           "loads_first": True}
+                                                           "loads_first": True, "consecutive": False}
         out.append((host(x), host(xm)))
         ctx.close()
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])

@@ -50,21 +50,28 @@ def test_mixed_vs_oracle(P, gnx_mod, prec, L):
     banner(f"  iterations fp64={st.last_iters} {prec}={res.iterations} updates={res.reliable_updates}")
 
 
-@pytest.mark.parametrize("shape", list(range(16)))
+SPT_OF_SHAPE = [1, 1, 2, 2, 4, 4, 1, 1, 2, 2, 1, 1, 2, 2, 4, 4, 4]
+
+
+@pytest.mark.parametrize("shape", list(range(17)))
 @pytest.mark.parametrize("prec", PRECS)
 def test_mixed_batch_and_null_start(P, gnx_mod, prec, shape, monkeypatch):
     """Every launch shape: 0..5 one system per block (1/2/4 sites per thread,
     with and without the XCD slab mapping), 6..9 two systems per block,
     10..15 as 0..5 with the loads-first kernels (which also defer the
-    correction update to every other slot).  A batch of 3 (the last pair half
+    correction update to every other slot), 16 the loads-first kernels on
+    groups of four consecutive sites (4-byte storage, LT a multiple of 8: run
+    at 16^3, two groups per parity row).  A batch of 3 (the last pair half
     empty) is bit-identical to 3 single solves with the same steps: a paired
     batch to the unpaired kernels of its site mapping, the others to their own
     shape; each solution meets the FP64 bar."""
     if prec == "f16" and shape < 10:
         pytest.skip("FP16 storage runs the loads-first shapes only (they carry the scale)")
+    if prec == "f32" and shape == 16:
+        pytest.skip("the four-site group kernels exist for the 4-byte storage formats only")
     banner(f"mixed {prec} shape {shape}: batch of 3 and null-start early exit")
     import torch
-    L = 12
+    L = 16 if shape == 16 else 12
     lat = P.Lattice(L)
     r = P.Random(41)
     sigma = r.sigma(lat.V, P.sigma_variance(0.85))
@@ -79,8 +86,8 @@ def test_mixed_batch_and_null_start(P, gnx_mod, prec, shape, monkeypatch):
     assert st == 0
     pair = 6 <= shape < 10
     early = shape >= 10
-    assert ctx.cg_mixed_shape(3, p) == {"sites_per_thread": [1, 1, 2, 2, 4, 4, 1, 1, 2, 2][shape % 10],
-                                        "xcd_slab": shape % 2 == 1, "pair": pair, "loads_first": early}
+    assert ctx.cg_mixed_shape(3, p) == {"sites_per_thread": SPT_OF_SHAPE[shape], "xcd_slab": shape % 2 == 1 or shape == 16,
+                                        "pair": pair, "loads_first": early, "consecutive": shape == 16}
     # single solves: a paired batch against the unpaired kernels of the same
     # site mapping, the other shapes against themselves
     monkeypatch.setenv("GNX_MIX_SHAPE", str(shape - 6 if pair else shape))
@@ -126,7 +133,9 @@ def test_mixed_iteration_cap_folds_pending(P, gnx_mod, prec, cap, monkeypatch):
     b = r.z2_complex(lat.V)
     p = _prec(gnx_mod, prec)
     xs = {}
-    for shape in (0, 10):
+    # 16: the four-site groups, one per parity row at LT = 8 (the x2 hops of
+    # both row ends inside one group)
+    for shape in (0, 10) if prec == "f32" else (0, 10, 16):
         monkeypatch.setenv("GNX_MIX_SHAPE", str(shape))
         c = gnx_mod.Context(L)
         x = c.vec()
@@ -139,7 +148,68 @@ def test_mixed_iteration_cap_folds_pending(P, gnx_mod, prec, cap, monkeypatch):
         tr = true_res(lat, smh, b, xh)
         assert abs(tr - res.residual) <= 1e-9 * np.linalg.norm(b), (tr, res.residual)
         xs[shape] = xh
-    assert rel(xs[10], xs[0]) < (2e-2 if prec == "f16" else 1e-4), rel(xs[10], xs[0])
+        assert c.cg_mixed_shape(1, p if shape else pp)["consecutive"] == (shape == 16)
+    for shape in xs:
+        assert rel(xs[shape], xs[0]) < (2e-2 if prec == "f16" else 1e-4), (shape, rel(xs[shape], xs[0]))
+
+
+@pytest.mark.parametrize("prec", ["bf16", "f16"])
+@pytest.mark.parametrize("dims", [(12, 8), (6, 24), (16, 32)])
+def test_mixed_four_site_groups(P, gnx_mod, prec, dims):
+    """The four-site group kernels on L^2 x LT lattices: 12^2 x 8 (V / 2 = 576,
+    the last block partly past the last group, one group per parity row),
+    6^2 x 24 (three groups per row) and 16^2 x 32 (x0, x1 hops across row
+    blocks of 16 groups).  Two systems: each meets the FP64 bar against the
+    oracle's operator, and the solution matches the oracle's FP64 solve."""
+    L, LT = dims
+    banner(f"mixed {prec} four-site groups {L}^2x{LT}")
+    import os
+    import torch
+    lat = P.Lattice(L, LT)
+    r = P.Random(500 + L + LT)
+    smh = lat.sigma_mass(r.sigma(lat.V, P.sigma_variance(0.85)))
+    bs = [r.z2_complex(lat.V) for _ in range(2)]
+    ctx = gnx_mod.Context(L, LT)
+    p = _prec(gnx_mod, prec)
+    os.environ["GNX_MIX_SHAPE"] = "16"
+    try:
+        X = ctx.vec(2)
+        st, res = ctx.cg_batch(dev(smh), dev(np.concatenate(bs)), X, 2, 1e-10, start=gnx_mod.START_IMPOSED,
+                               prec=p)
+        torch.cuda.synchronize()
+    finally:
+        del os.environ["GNX_MIX_SHAPE"]
+    assert st == 0 and ctx.cg_mixed_shape(2, p)["consecutive"]
+    for k in range(2):
+        assert res[k].converged and res[k].reliable_updates >= 1
+        xk = host(X[k * 2 * lat.V:(k + 1) * 2 * lat.V])
+        assert true_res(lat, smh, bs[k], xk) < 1e-10
+        xr, ok, _ = lat.cg(smh, bs[k], None, tol=1e-10, start="imposed")
+        assert rel(xk, xr) < 1e-6
+
+
+def test_mixed_four_site_groups_not_pinned_where_unfit(P, gnx_mod):
+    """A pinned four-site shape that does not fit (FP32 storage, or LT not a
+    multiple of 8) is ignored: the rule's shape runs and the solve converges."""
+    banner("mixed: four-site shape ignored where it does not fit")
+    import os
+    L = 12
+    lat = P.Lattice(L)
+    r = P.Random(9)
+    smh = lat.sigma_mass(r.sigma(lat.V, P.sigma_variance(0.85)))
+    b = r.z2_complex(lat.V)
+    os.environ["GNX_MIX_SHAPE"] = "16"
+    try:
+        for prec in (gnx_mod.PREC_F32, gnx_mod.PREC_F16):
+            ctx = gnx_mod.Context(L)
+            x = ctx.vec()
+            st, res = ctx.cg(dev(smh), dev(b), x, 1e-10, start=gnx_mod.START_IMPOSED, prec=prec)
+            assert st == 0 and res.converged
+            assert not ctx.cg_mixed_shape(1, prec)["consecutive"]
+            assert true_res(lat, smh, b, host(x)) < 1e-10
+            ctx.close()
+    finally:
+        del os.environ["GNX_MIX_SHAPE"]
 
 
 @pytest.mark.parametrize("mag", [1e-8, 1e6])
