@@ -344,6 +344,16 @@ gnx_status gnx_set_autotune(gnx_ctx* c, int on) {
   return GNX_OK;
 }
 
+gnx_status gnx_replay_stats(gnx_ctx* c, long* slots, long* polls) {
+  CHECK_ARG(c && slots && polls, "gnx_replay_stats: null argument");
+  long s1, p1, s2, p2;
+  cgm_replay_stats(c->cgm, &s1, &p1);
+  cgmix_replay_stats(c->cgmix, &s2, &p2);
+  *slots = s1 + s2;
+  *polls = p1 + p2;
+  return GNX_OK;
+}
+
 gnx_status gnx_tune_runs(gnx_ctx* c, long* fp64, long* mixed) {
   CHECK_ARG(c && fp64 && mixed, "gnx_tune_runs: null argument");
   *fp64 = cgm_tune_runs(c->cgm);

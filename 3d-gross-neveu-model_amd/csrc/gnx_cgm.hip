@@ -1137,9 +1137,10 @@ static CGMFix cgm_fix(CGMachine* m, const Geom& g, int nrhs, int shape) {
   return a;
 }
 
-// one chunk of ENG_CHUNK iterations (every A and B kernel) on stream cs:
-// captured into a graph by cgm_graph, launched directly by the static solve
-static void cgm_chunk(const CGMFix& a, int nrhs, int shape, hipStream_t cs) {
+// `iters` iterations (every A and B kernel) on stream cs, ENG_CHUNK or
+// ENG_TAIL: captured into graphs by cgm_graph, launched directly by the
+// static solve
+static void cgm_chunk(const CGMFix& a, int nrhs, int shape, hipStream_t cs, int iters) {
   dim3 grid(a.nb, nrhs), blk(CGM_BS);
   const CGMState* s0 = a.state;
   const CGMState* s1 = a.state + nrhs;
@@ -1152,11 +1153,11 @@ static void cgm_chunk(const CGMFix& a, int nrhs, int shape, hipStream_t cs) {
   const CGMShape& sh = kShapes[shape];
   const bool wh = head_wave(a.nb);
   CGMFix ac = a;
-  static_assert(ENG_CHUNK % CGD_R == 0, "the residual ring turns over within a chunk");
+  static_assert(ENG_CHUNK % CGD_R == 0 && ENG_TAIL % CGD_R == 0, "the residual ring turns over within a replay");
   if (sh.fused) {
     // Z(k) for k = 1 + i (the prologue ran X(0) and Z(0)): node i reads
     // state[i & 1] and writes the other half, so the chunk ends in state[0]
-    for (int i = 0; i < ENG_CHUNK; i++) {
+    for (int i = 0; i < iters; i++) {
       int k8 = (i + 1) & (CGD_R - 1);
       const CGMState* sin = (i & 1) ? s1 : s0;
       CGMState* sout = (i & 1) ? w0 : w1;
@@ -1165,7 +1166,7 @@ static void cgm_chunk(const CGMFix& a, int nrhs, int shape, hipStream_t cs) {
     }
     return;
   }
-  for (int i = 0; i < ENG_CHUNK; i++) {
+  for (int i = 0; i < iters; i++) {
     // iteration parity (p buffers of the two-reduction form), or the
     // residual ring slot of the deferred form
     int odd = sh.deferred ? (i & (CGD_R - 1)) : (i & 1);
@@ -1178,9 +1179,9 @@ static void cgm_chunk(const CGMFix& a, int nrhs, int shape, hipStream_t cs) {
   }
 }
 
-static hipError_t cgm_graph(CGMachine* m, const Geom& g, int nrhs, int shape, hipGraphExec_t* exec) {
+static hipError_t cgm_graph(CGMachine* m, const Geom& g, int nrhs, int shape, EngGraph* exec) {
   const CGMFix a = cgm_fix(m, g, nrhs, shape);
-  auto body = [&](hipStream_t cs) { cgm_chunk(a, nrhs, shape, cs); };
+  auto body = [&](hipStream_t cs, int iters) { cgm_chunk(a, nrhs, shape, cs, iters); };
   return m->eng.graph(shape, nrhs, body, exec);
 }
 
@@ -1288,7 +1289,7 @@ static hipError_t cgm_choose(CGMachine* m, const Geom& g, const double* sm, cons
     float best_ms = 1e30f;
     for (int sh = first; sh < first + kShapesPerOp && e == hipSuccess; sh++) {
       if (!shape_fits(g, sh)) continue;
-      hipGraphExec_t exec;
+      EngGraph exec;
       e = cgm_graph(m, g, nrhs, sh, &exec);
       float ms_min = 1e30f;
       for (int rep = 0; rep < 3 && e == hipSuccess; rep++) {
@@ -1319,12 +1320,12 @@ hipError_t cgm_f64_run(void** machine_slot, const Geom& g, const double* sm, con
   int shape = 0;
   e = cgm_choose(m, g, sm, U, b, nrhs, st, &shape);
   if (e != hipSuccess) return e;
-  hipGraphExec_t exec;
+  EngGraph exec;
   e = cgm_graph(m, g, nrhs, shape, &exec);
   if (e != hipSuccess) return e;
   e = cgm_begin(m, g, sm, U, b, x, tol, max_iter, start, nrhs, shape, st);
   if (e != hipSuccess) return e;
-  e = m->eng.run(exec, nrhs, max_iter, st, nullptr);
+  e = m->eng.run(exec, nrhs, max_iter, st, nullptr, eng_key(shape, nrhs, tol, start), nullptr);
   if (e != hipSuccess) return e;
   e = hipMemcpyAsync(x, m->x, sizeof(double2) * (size_t)g.V * nrhs, hipMemcpyDeviceToDevice, st);
   if (e != hipSuccess) return e;
@@ -1443,7 +1444,7 @@ hipError_t cgm_f64_run_static(const Geom& g, int device, const double* sm, const
   // and `iters` polled through the caller's pinned buffer after each
   int* flags = (int*)pinned;
   for (long done = 0;; done += ENG_CHUNK) {
-    cgm_chunk(a, 1, shape, st);
+    cgm_chunk(a, 1, shape, st, ENG_CHUNK);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     e = hipMemcpyAsync(flags, a.state, 2 * sizeof(int), hipMemcpyDeviceToHost, st);
@@ -1474,5 +1475,9 @@ int cgm_shape_of(void* machine, int nrhs, int* spt, int* xcd) {
 void cgm_set_autotune(void** machine_slot, int on) { cgm_get(machine_slot)->autotune = on; }
 
 long cgm_tune_runs(void* machine) { return machine ? ((CGMachine*)machine)->eng.tune_runs : 0; }
+void cgm_replay_stats(void* machine, long* slots, long* polls) {
+  *slots = machine ? ((CGMachine*)machine)->eng.launched_slots : 0;
+  *polls = machine ? ((CGMachine*)machine)->eng.polls : 0;
+}
 
 }  // namespace gnx

@@ -226,6 +226,14 @@ GNX_DEVONLY double* mslot_v(const MixFix& a, const SV& v, int s) {
 
 GNX_DEVONLY void* mix_p(const MixFix& a, int k) { return (k & 1) ? a.p1 : a.p0; }
 
+// Slots a finished system used (its polled state): one pair per normal step
+// (iters), one per reliable update (nrel, the first update of a scaled
+// format's start included), and the pair of the X kernel that stops it.
+static int mix_slots_of(const char* state) {
+  const MixState* s = (const MixState*)state;
+  return s->iters + s->nrel + 2;
+}
+
 GNX_DEVONLY double dnorm2(float2 v) { return (double)v.x * v.x + (double)v.y * v.y; }
 
 // ---- start kernels ------------------------------------------------------------
@@ -1530,7 +1538,7 @@ static MixFix mix_fix(MixMachine* m, const Geom& g, int nrhs, int spt, int store
 }
 
 template <typename S>
-static hipError_t mix_graph(MixMachine* m, const Geom& g, int nrhs, int shape, int tag, hipGraphExec_t* exec) {
+static hipError_t mix_graph(MixMachine* m, const Geom& g, int nrhs, int shape, int tag, EngGraph* exec) {
   const MixShape& sh = mix_shapes<S>()[shape];
   MixFix a = mix_fix(m, g, nrhs, sh.spt, (int)sizeof(typename S::T));
   dim3 grid(a.nb, sh.pair ? (nrhs + 1) / 2 : nrhs), blk(MIX_BS);
@@ -1542,9 +1550,9 @@ static hipError_t mix_graph(MixMachine* m, const Geom& g, int nrhs, int shape, i
   const double* q1 = mslot(a, 1, 0);
   double* o0 = mslot(a, 0, 0);
   double* o1 = mslot(a, 1, 0);
-  auto body = [&](hipStream_t cs) {
-    static_assert(ENG_CHUNK % 2 == 0, "the direction buffers alternate by slot parity");
-    for (int i = 0; i < ENG_CHUNK; i++) {
+  auto body = [&](hipStream_t cs, int iters) {
+    static_assert(ENG_CHUNK % 2 == 0 && ENG_TAIL % 2 == 0, "the direction buffers alternate by slot parity");
+    for (int i = 0; i < iters; i++) {
       int sl = i & 1;  // slot parity
       // X: state[0] -> state[1], reduces slot 1, writes slot 0; Y the reverse
       void* argsX[] = {(void*)&a, (void*)&sl, (void*)&s0, (void*)&w1, (void*)&q1, (void*)&o0};
@@ -1619,7 +1627,7 @@ static hipError_t mix_choose(MixMachine* m, const Geom& g, const double* sm, con
     float best_ms = 1e30f;
     for (int sh = 0; sh < kMixShapes && e == hipSuccess; sh++) {
       if (!mix_fits<S>(g, nrhs, sh)) continue;
-      hipGraphExec_t exec;
+      EngGraph exec;
       e = mix_graph<S>(m, g, nrhs, sh, tag, &exec);
       float ms_min = 1e30f;
       for (int rep = 0; rep < 3 && e == hipSuccess; rep++) {
@@ -1650,13 +1658,14 @@ static hipError_t mix_run_t(MixMachine* m, const Geom& g, const double* sm, cons
   int shape = 0;
   e = mix_choose<S>(m, g, sm, b, delta, nrhs, tag, st, &shape);
   if (e != hipSuccess) return e;
-  hipGraphExec_t exec;
+  EngGraph exec;
   e = mix_graph<S>(m, g, nrhs, shape, tag, &exec);
   if (e != hipSuccess) return e;
   e = mix_begin<S>(m, g, sm, b, x, tol, delta, max_iter, start, nrhs, shape, st);
   if (e != hipSuccess) return e;
   // at most max_iter normal slot pairs plus one update pair after each
-  e = m->eng.run(exec, nrhs, 2 * max_iter + 4, st, nullptr);
+  e = m->eng.run(exec, nrhs, 2 * max_iter + 4, st, nullptr, eng_key(tag + 10 * shape, nrhs, tol, start),
+                 mix_slots_of);
   if (e != hipSuccess) return e;
   e = hipMemcpyAsync(x, m->x, sizeof(double2) * (size_t)g.V * nrhs, hipMemcpyDeviceToDevice, st);
   if (e != hipSuccess) return e;
@@ -1699,6 +1708,10 @@ void cgmix_set_autotune(void** machine_slot, int on) {
 }
 
 long cgmix_tune_runs(void* machine) { return machine ? ((MixMachine*)machine)->eng.tune_runs : 0; }
+void cgmix_replay_stats(void* machine, long* slots, long* polls) {
+  *slots = machine ? ((MixMachine*)machine)->eng.launched_slots : 0;
+  *polls = machine ? ((MixMachine*)machine)->eng.polls : 0;
+}
 
 // ---- uneven batches ------------------------------------------------------------------
 // The loads-first kernels over the systems of an uneven solver (gnx_cgu.hip),
@@ -1845,7 +1858,7 @@ static hipError_t mixu_run_t(MixMulti* mm, const MultiView& mv, const double* co
     e = hipLaunchKernel((const void*)k_mix_start_imposed<S, 1, true, MixUneven>, grid, blk, a0, 0, st);
   }
   if (e != hipSuccess) return e;
-  // the chunk graph: ENG_CHUNK slot pairs over all systems, as mix_graph
+  // the chunk and tail graphs: slot pairs over all systems, as mix_graph
   const MixState* s0 = a.state;
   const MixState* s1 = a.state + n;
   MixState* w0 = a.state;
@@ -1854,8 +1867,8 @@ static hipError_t mixu_run_t(MixMulti* mm, const MultiView& mv, const double* co
   const double* q1 = a.part + a.ptotal;
   double* o0 = a.part;
   double* o1 = a.part + a.ptotal;
-  auto body = [&](hipStream_t cs) {
-    for (int i = 0; i < ENG_CHUNK; i++) {
+  auto body = [&](hipStream_t cs, int iters) {
+    for (int i = 0; i < iters; i++) {
       int sl = i & 1;
       void* argsX[] = {(void*)&a, (void*)&sl, (void*)&s0, (void*)&w1, (void*)&q1, (void*)&o0};
       void* argsY[] = {(void*)&a, (void*)&sl, (void*)&s1, (void*)&w0, (void*)&q0, (void*)&o1};
@@ -1865,11 +1878,11 @@ static hipError_t mixu_run_t(MixMulti* mm, const MultiView& mv, const double* co
       (void)hipLaunchKernel((const void*)k_mix_Ye<S, 1, true, MixUneven>, grid, blk, argsY, 0, cs);
     }
   };
-  hipGraphExec_t exec;
+  EngGraph exec;
   e = mm->eng.graph(tag, n, body, &exec);
   if (e != hipSuccess) return e;
   // at most max_iter normal slot pairs plus one update pair after each
-  e = mm->eng.run(exec, n, 2 * cap_max + 4, st, nullptr);
+  e = mm->eng.run(exec, n, 2 * cap_max + 4, st, nullptr, eng_key(tag, n, tol, start), mix_slots_of);
   if (e != hipSuccess) return e;
   void* as[] = {(void*)&a, (void*)&dp};
   e = hipLaunchKernel((const void*)k_mixu_scatter, grid, blk, as, 0, st);

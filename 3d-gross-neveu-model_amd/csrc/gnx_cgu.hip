@@ -411,9 +411,9 @@ static CGUFix cgu_fix(gnx_multi* m) {
   return a;
 }
 
-// the chunk graph: ENG_CHUNK (A, B) pairs over all systems, on M M^dag or
+// the chunk and tail graphs: (A, B) pairs over all systems, on M M^dag or
 // on M_U M_U^dag (link)
-static hipError_t cgu_graph(gnx_multi* m, bool link, hipGraphExec_t* exec) {
+static hipError_t cgu_graph(gnx_multi* m, bool link, EngGraph* exec) {
   CGUFix a = cgu_fix(m);
   const int n = m->n;
   dim3 grid((m->nbmax + NUM_XCD - 1) / NUM_XCD * NUM_XCD, n), blk(CGU_BS);
@@ -425,8 +425,8 @@ static hipError_t cgu_graph(gnx_multi* m, bool link, hipGraphExec_t* exec) {
   const double* q1 = a.part + a.ptotal;
   double* o0 = a.part;
   double* o1 = a.part + a.ptotal;
-  auto body = [&](hipStream_t cs) {
-    for (int i = 0; i < ENG_CHUNK; i++) {
+  auto body = [&](hipStream_t cs, int iters) {
+    for (int i = 0; i < iters; i++) {
       int odd = i & 1;
       // A: state[0] -> state[1], reduces slot 1, writes slot 0; B the reverse
       void* argsA[] = {(void*)&a, (void*)&odd, (void*)&s0, (void*)&w1, (void*)&q1, (void*)&o0};
@@ -589,9 +589,9 @@ static gnx_status multi_cg(gnx_multi* m, const double* const* sm, const double* 
     void* a0[] = {(void*)&a, (void*)&tol, (void*)&mi};
     HIP_TRY(hipLaunchKernel((const void*)k_cgu_start_imposed, grid, blk, a0, 0, st));
   }
-  hipGraphExec_t exec;
+  EngGraph exec;
   HIP_TRY(cgu_graph(m, link, &exec));
-  HIP_TRY(m->eng.run(exec, n, cap_max, st, nullptr));
+  HIP_TRY(m->eng.run(exec, n, cap_max, st, nullptr, eng_key(link ? 1 : 0, n, tol, start), nullptr));
   {
     const CGUPtrs* dp = m->dptr;
     void* as[] = {(void*)&a, (void*)&dp};

@@ -6,8 +6,12 @@
 // struct and keep per-system state in a device array state[2][nrhs]; the
 // first two ints of each state entry are `active` and `iters`.  The engine
 // captures CHUNK (X, Y) pairs into one hipGraph per (kernel pair, batch
-// size), replays it until no system is active, and polls only the `active`
-// flags between replays.
+// size), and 1, 2 and 3 TAILs of pairs into three more, replays them until
+// no system is active, and polls only the state between replays.  A
+// replayed slot of a finished system still costs its two graph nodes, and
+// every replay a graph launch, so each run first launches about as many
+// slots as the last run of the same kind needed, as whole chunks and at most
+// one tail graph, and polls once.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -18,9 +22,31 @@ namespace gnx {
 
 constexpr int ENG_BS = 256;     // threads per block of every solver kernel
 constexpr int ENG_CHUNK = 32;   // iterations per graph replay
+constexpr int ENG_TAIL = 8;     // tail replays are 1, 2 or 3 of these (even, and a multiple of the deferred form's ring)
+constexpr int ENG_TAILS = 3;
+static_assert(ENG_CHUNK == (ENG_TAILS + 1) * ENG_TAIL, "a chunk is one tail more than the longest tail graph");
 
-// launches ENG_CHUNK iterations (every X and Y kernel) into the capture stream
-using ChunkBody = std::function<void(hipStream_t)>;
+// launches `iters` iterations (every X and Y kernel) into the capture stream;
+// iters is ENG_CHUNK or a multiple of ENG_TAIL, so every replay starts at
+// slot parity 0
+using ChunkBody = std::function<void(hipStream_t, int iters)>;
+
+// the captured graphs of a (kernel pair, batch size): a chunk, and tails of
+// ENG_TAIL, 2 ENG_TAIL and 3 ENG_TAIL iterations
+struct EngGraph {
+  hipGraphExec_t chunk = nullptr;
+  hipGraphExec_t tail[ENG_TAILS] = {nullptr, nullptr, nullptr};
+};
+
+// Slots (X, Y pairs) a finished system used, from its polled state entry, at
+// least: the engine launches this many for the next run of the same kind.
+// The default, for states whose every slot but the last counts an
+// iteration, is iters + 2.
+using SlotsOf = int (*)(const char* state);
+
+// what identifies runs of one kind (expected to take similar slot counts):
+// the graph, the batch size, the tolerance and the start
+unsigned long long eng_key(int tag, int nrhs, double tol, int start);
 
 // Launch shapes chosen per key (batch size with the operator or storage
 // format folded in).  When every slot is taken the least recently used
@@ -81,16 +107,19 @@ class GraphEngine {
   double* dev_part() { return part_; }
   // wait for the stream, then copy the staged host arguments to the device
   hipError_t stage_args(hipStream_t st);
-  // graph of one chunk for (tag, nrhs), captured from `body` on first use
-  hipError_t graph(int tag, int nrhs, const ChunkBody& body, hipGraphExec_t* exec);
+  // graphs of one chunk and the tails for (tag, nrhs), captured from
+  // `body` on first use
+  hipError_t graph(int tag, int nrhs, const ChunkBody& body, EngGraph* g);
   // drop every captured graph (their kernel arguments point at old buffers)
   void invalidate_graphs();
-  // replay until every system is inactive; returns the largest `iters`
-  hipError_t run(hipGraphExec_t exec, int nrhs, long max_slots, hipStream_t st, int* max_iters);
-  // time `chunks` replays (after the caller staged a start), in ms
-  hipError_t time_replays(hipGraphExec_t exec, int chunks, hipStream_t st, float* ms);
-
-  int last_iters = 0;
+  // replay until every system is inactive; returns the largest `iters`.
+  // key: eng_key() of the run; slots_of: null for the default
+  hipError_t run(const EngGraph& g, int nrhs, long max_slots, hipStream_t st, int* max_iters,
+                 unsigned long long key, SlotsOf slots_of);
+  // time `chunks` chunk replays (after the caller staged a start), in ms
+  hipError_t time_replays(const EngGraph& g, int chunks, hipStream_t st, float* ms);
+  // slots launched and polls made by run() since construction
+  long launched_slots = 0, polls = 0;
   ShapeCache shapes;   // chosen launch shape per key
   long tune_runs = 0;  // timing-tuner runs (one per key chosen by timing)
 
@@ -99,7 +128,13 @@ class GraphEngine {
   struct Entry {
     int tag = -1;
     int nrhs = 0;
-    hipGraphExec_t exec = nullptr;
+    EngGraph g;
+  };
+  // slots needed by the last two runs of a kind, least recently used replaced
+  struct Pred {
+    unsigned long long key = 0;
+    int last = -1, prev = -1;
+    unsigned long long used = 0;
   };
   void* dargs_ = nullptr;
   void* hargs_ = nullptr;
@@ -111,6 +146,10 @@ class GraphEngine {
   hipStream_t capture_ = nullptr;
   Entry graphs_[12];
   int next_evict_ = 0;
+  Pred pred_[16];
+  unsigned long long pred_clock_ = 0;
+  int last_slots_ = 0;  // the last run's, for a kind not seen before
+  Pred* pred_find(unsigned long long key);
 };
 
 }  // namespace gnx

@@ -9,10 +9,35 @@ GraphEngine::~GraphEngine() { release(); }
 
 void GraphEngine::invalidate_graphs() {
   for (auto& g : graphs_)
-    if (g.exec) {
-      (void)hipGraphExecDestroy(g.exec);
+    if (g.g.chunk) {
+      (void)hipGraphExecDestroy(g.g.chunk);
+      for (hipGraphExec_t t : g.g.tail)
+        if (t) (void)hipGraphExecDestroy(t);
       g = Entry();
     }
+}
+
+unsigned long long eng_key(int tag, int nrhs, double tol, int start) {
+  unsigned long long t;
+  memcpy(&t, &tol, sizeof t);
+  unsigned long long h = 1469598103934665603ull;  // FNV-1a over the four fields
+  const unsigned long long w[4] = {(unsigned long long)(unsigned)tag, (unsigned long long)(unsigned)nrhs, t,
+                                   (unsigned long long)(unsigned)start};
+  for (unsigned long long v : w)
+    for (int b = 0; b < 8; b++) {
+      h ^= (v >> (8 * b)) & 0xFFu;
+      h *= 1099511628211ull;
+    }
+  return h;
+}
+
+GraphEngine::Pred* GraphEngine::pred_find(unsigned long long key) {
+  for (auto& p : pred_)
+    if (p.last >= 0 && p.key == key) {
+      p.used = ++pred_clock_;
+      return &p;
+    }
+  return nullptr;
 }
 
 void GraphEngine::release() {
@@ -61,75 +86,132 @@ hipError_t GraphEngine::stage_args(hipStream_t st) {
   return hipMemcpyAsync(dargs_, hargs_, args_bytes_, hipMemcpyHostToDevice, st);
 }
 
-hipError_t GraphEngine::graph(int tag, int nrhs, const ChunkBody& body, hipGraphExec_t* exec) {
+// capture `iters` iterations of `body` into an executable graph
+static hipError_t capture_graph(hipStream_t cs, const ChunkBody& body, int iters, hipGraphExec_t* exec) {
+  hipGraph_t graph;
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  body(cs, iters);
+  e = hipStreamEndCapture(cs, &graph);
+  if (e != hipSuccess) return e;
+  e = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) *exec = nullptr;
+  return e;
+}
+
+hipError_t GraphEngine::graph(int tag, int nrhs, const ChunkBody& body, EngGraph* out) {
   for (auto& g : graphs_)
-    if (g.exec && g.tag == tag && g.nrhs == nrhs) {
-      *exec = g.exec;
+    if (g.g.chunk && g.tag == tag && g.nrhs == nrhs) {
+      *out = g.g;
       return hipSuccess;
     }
   Entry* slot = nullptr;
   for (auto& g : graphs_)
-    if (!g.exec) {
+    if (!g.g.chunk) {
       slot = &g;
       break;
     }
   if (!slot) {
     slot = &graphs_[next_evict_];
     next_evict_ = (next_evict_ + 1) % (int)(sizeof(graphs_) / sizeof(graphs_[0]));
-    (void)hipGraphExecDestroy(slot->exec);
+    (void)hipGraphExecDestroy(slot->g.chunk);
+    for (hipGraphExec_t t : slot->g.tail)
+      if (t) (void)hipGraphExecDestroy(t);
     *slot = Entry();
   }
-  hipGraph_t graph;
-  hipError_t e = hipStreamBeginCapture(capture_, hipStreamCaptureModeThreadLocal);
-  if (e != hipSuccess) return e;
-  body(capture_);
-  e = hipStreamEndCapture(capture_, &graph);
-  if (e != hipSuccess) return e;
-  e = hipGraphInstantiate(&slot->exec, graph, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(graph);
+  EngGraph g;
+  hipError_t e = capture_graph(capture_, body, ENG_CHUNK, &g.chunk);
+  for (int k = 0; k < ENG_TAILS && e == hipSuccess; k++)
+    e = capture_graph(capture_, body, (k + 1) * ENG_TAIL, &g.tail[k]);
   if (e != hipSuccess) {
-    slot->exec = nullptr;
+    if (g.chunk) (void)hipGraphExecDestroy(g.chunk);
+    for (hipGraphExec_t t : g.tail)
+      if (t) (void)hipGraphExecDestroy(t);
     return e;
   }
   slot->tag = tag;
   slot->nrhs = nrhs;
-  *exec = slot->exec;
+  slot->g = g;
+  *out = g;
   return hipSuccess;
 }
 
-hipError_t GraphEngine::run(hipGraphExec_t exec, int nrhs, long max_slots, hipStream_t st, int* max_iters) {
+hipError_t GraphEngine::run(const EngGraph& g, int nrhs, long max_slots, hipStream_t st, int* max_iters,
+                            unsigned long long key, SlotsOf slots_of) {
+  // first burst: the slots the last run of this kind needed, plus the
+  // change between its last two runs (a kind whose count moves gets that
+  // much headroom, at most half a chunk), as whole chunks and one tail of
+  // 1..3 ENG_TAIL (a remainder past that takes a whole chunk)
+  Pred* p = pred_find(key);
+  const int need = p ? p->last : last_slots_;
+  int margin = (p && p->prev >= 0) ? (p->last > p->prev ? p->last - p->prev : p->prev - p->last) : 0;
+  if (margin > ENG_CHUNK / 2) margin = ENG_CHUNK / 2;
+  const long want = need > 0 ? (long)need + margin : ENG_CHUNK;
+  int full = (int)(want / ENG_CHUNK);
+  int tails = (int)((want % ENG_CHUNK + ENG_TAIL - 1) / ENG_TAIL);  // 0 .. ENG_TAILS + 1
+  if (tails > ENG_TAILS) {
+    full++;
+    tails = 0;
+  }
   long done = 0;
-  int burst = (last_iters + ENG_CHUNK - 1) / ENG_CHUNK;
-  if (burst < 1) burst = 1;
+  int misses = 0;
   hipError_t e;
   for (;;) {
-    for (int i = 0; i < burst; i++) {
-      e = hipGraphLaunch(exec, st);
+    for (int i = 0; i < full; i++) {
+      e = hipGraphLaunch(g.chunk, st);
       if (e != hipSuccess) return e;
     }
-    done += (long)burst * ENG_CHUNK;
+    if (tails > 0) {
+      e = hipGraphLaunch(g.tail[tails - 1], st);
+      if (e != hipSuccess) return e;
+    }
+    const long n = (long)full * ENG_CHUNK + (long)tails * ENG_TAIL;
+    done += n;
+    launched_slots += n;
+    polls++;
     e = hipMemcpyAsync(hstate_, state_, state_bytes_ * nrhs, hipMemcpyDeviceToHost, st);
     if (e != hipSuccess) return e;
     e = hipStreamSynchronize(st);
     if (e != hipSuccess) return e;
     bool any = false;
-    int mx = 0;
+    int mx = 0, slots = 0;
     for (int k = 0; k < nrhs; k++) {
-      const int* s = (const int*)(hstate_ + state_bytes_ * k);
+      const char* sk = hstate_ + state_bytes_ * k;
+      const int* s = (const int*)sk;
       any |= s[0] != 0;
       mx = s[1] > mx ? s[1] : mx;
+      const int u = slots_of ? slots_of(sk) : s[1] + 2;
+      slots = u > slots ? u : slots;
     }
     if (!any) {
-      last_iters = mx;
+      if (!p) {  // a new kind: take the least recently used entry
+        p = &pred_[0];
+        for (auto& q : pred_)
+          if (q.used < p->used) p = &q;
+        *p = Pred();
+        p->key = key;
+        p->used = ++pred_clock_;
+      }
+      p->prev = p->last;
+      p->last = slots;
+      last_slots_ = slots;
       if (max_iters) *max_iters = mx;
       return hipSuccess;
     }
     if (done > max_slots + 2 * ENG_CHUNK) return hipErrorUnknown;  // the state machine failed to stop
-    burst = 1;
+    // short of the need: a tail of two first (a near miss), then whole chunks
+    if (misses++ == 0) {
+      full = 0;
+      tails = 2;
+    } else {
+      full = 1;
+      tails = 0;
+    }
   }
 }
 
-hipError_t GraphEngine::time_replays(hipGraphExec_t exec, int chunks, hipStream_t st, float* ms) {
+hipError_t GraphEngine::time_replays(const EngGraph& g, int chunks, hipStream_t st, float* ms) {
   hipEvent_t e0, e1;
   hipError_t e = hipEventCreate(&e0);
   if (e != hipSuccess) return e;
@@ -139,7 +221,7 @@ hipError_t GraphEngine::time_replays(hipGraphExec_t exec, int chunks, hipStream_
     return e;
   }
   (void)hipEventRecord(e0, st);
-  for (int k = 0; k < chunks && e == hipSuccess; k++) e = hipGraphLaunch(exec, st);
+  for (int k = 0; k < chunks && e == hipSuccess; k++) e = hipGraphLaunch(g.chunk, st);
   (void)hipEventRecord(e1, st);
   if (e == hipSuccess) e = hipEventSynchronize(e1);
   if (e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);

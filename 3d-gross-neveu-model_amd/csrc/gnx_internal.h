@@ -93,6 +93,7 @@ size_t cgm_static_capacity();
 hipError_t cgm_f64_run_static(const Geom& g, int device, const double* sm, const double2* b, double2* x, double tol,
                               long max_iter, int start, hipStream_t st, void* pinned, CGDevResult* host_res);
 long cgm_tune_runs(void* machine);
+void cgm_replay_stats(void* machine, long* slots, long* polls);
 // FP64 CG of small lattices in one launch per solve, one workgroup per
 // system (gnx_cgs.hip): whether g fits (sites and iteration cap), the solve
 // (results to dres[nrhs] on the device) and its sites per thread
@@ -102,6 +103,7 @@ hipError_t cgs_run(const Geom& g, const double* sm, const double2* b, double2* x
 int cgs_sites_per_thread(const Geom& g);
 void cgmix_set_autotune(void** machine_slot, int on);
 long cgmix_tune_runs(void* machine);
+void cgmix_replay_stats(void* machine, long* slots, long* polls);
 
 void set_err(const char* fmt, ...);
 

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libgnx.so")
 OP_M, OP_MDAG, OP_MMDAG, OP_MDAGM, OP_SUB_M, OP_SUB_MDAG = range(6)
 START_NULL, START_IMPOSED = 0, 1
 PREC_F64, PREC_F32, PREC_BF16, PREC_F16 = 0, 1, 2, 3
-ABI_VERSION = 8  # GNX_ABI_VERSION of include/gnx.h this binding follows
+ABI_VERSION = 9  # GNX_ABI_VERSION of include/gnx.h this binding follows
 
 STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_NOCONV", 4: "ERR_SIZE", 5: "ERR_STATE"}
 
@@ -68,6 +68,7 @@ def lib():
         L.gnx_rng_fill.argtypes = [vp, C.c_int, C.c_long, C.c_double, C.POINTER(C.c_double)]
         L.gnx_set_autotune.argtypes = [vp, C.c_int]
         L.gnx_tune_runs.argtypes = [vp, C.POINTER(C.c_long), C.POINTER(C.c_long)]
+        L.gnx_replay_stats.argtypes = [vp, C.POINTER(C.c_long), C.POINTER(C.c_long)]
         L.gnx_last_error.restype = C.c_char_p
         L.gnx_create.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
         L.gnx_destroy.argtypes = [vp]
@@ -311,6 +312,11 @@ class Context:
         a, b = C.c_long(), C.c_long()
         _check(lib().gnx_tune_runs(self.h, C.byref(a), C.byref(b)))
         return {"fp64": a.value, "mixed": b.value}
+
+    def replay_stats(self):
+        a, b = C.c_long(), C.c_long()
+        _check(lib().gnx_replay_stats(self.h, C.byref(a), C.byref(b)))
+        return {"slots": a.value, "polls": b.value}
 
     def cg_shape(self, nrhs=1):
         spt, xcd = C.c_int(), C.c_int()

@@ -550,14 +550,20 @@ def ensemble_leg(gnx, sizes, ntraj, betas=(0.6, 0.75, 0.85, 1.0), eps=0.02, step
     for c in chains:
         c["dH_sigma"].clear(), c["dH_link"].clear()
         c["acc_sigma"] = c["acc_link"] = c["cg_iterations"] = 0
+    def replay_totals():
+        st = [c["ctx"].replay_stats() for c in chains]
+        return sum(r["slots"] for r in st), sum(r["polls"] for r in st)
+    r0 = replay_totals()
     t0 = time.perf_counter()
     with cf.ThreadPoolExecutor(max_workers=len(chains)) as ex:
         list(ex.map(lambda c: run(c, ntraj), chains))
     dt = time.perf_counter() - t0
+    r1 = replay_totals()
     out = {"workload": f"{len(chains)} chains on {sizes}^3 x beta_s {list(betas)}, links on (beta 1), "
                        f"{ntraj} sigma + link trajectories each, {steps} steps of {eps}",
            "mode": "one context, stream and host thread per chain",
            "trajectories_per_s": len(chains) * ntraj / dt, "seconds": dt,
+           "replay": {"slots_launched": r1[0] - r0[0], "polls": r1[1] - r0[1]},
            "chains": [{"L": c["L"], "beta_s": c["beta_s"], "dH_sigma": c["dH_sigma"], "dH_link": c["dH_link"],
                        "accepted_sigma": c["acc_sigma"], "accepted_link": c["acc_link"],
                        "cg_iterations": c["cg_iterations"]} for c in chains]}

@@ -35,7 +35,7 @@ extern "C" {
  * 6: bit 3 of xcd_mapped marks the fused kernels (one per iteration);
  *    gnx_multi_link_cg, gnx_multi_cg_prec; link-coupled lockstep chains
  * 7: bit 4 of xcd_mapped marks the one-launch solve of small lattices */
-#define GNX_ABI_VERSION 8
+#define GNX_ABI_VERSION 9
 
 typedef struct gnx_ctx gnx_ctx;
 typedef struct gnx_multi gnx_multi;
@@ -211,6 +211,14 @@ gnx_status gnx_set_autotune(gnx_ctx *ctx, int on);
  * each key is timed once per context unless it was evicted from the
  * 16-entry cache of choices (least recently used first) */
 gnx_status gnx_tune_runs(gnx_ctx *ctx, long *fp64, long *mixed);
+/* Iteration slots the context's graph-replayed solves (FP64 and mixed) have
+ * launched and the times they polled their state, since it was created.
+ * A replay runs a graph of 32 iterations or a tail of 8, 16 or 24, and a
+ * slot after a system stopped costs its kernels' launch; each solve first
+ * launches about as many slots as the last solve of the same kind (graph,
+ * batch size, tolerance, start) needed, as whole graphs and one tail, and
+ * polls once. */
+gnx_status gnx_replay_stats(gnx_ctx *ctx, long *slots, long *polls);
 /* Launch shape of the FP64 CG on the live operator (M M^dag) for batches of
  * nrhs systems on this context (sites per thread; xcd_mapped bit 0:
  * XCD-slab block mapping, bit 1: the one-reduction (Chronopoulos-Gear)

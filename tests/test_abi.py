@@ -24,7 +24,7 @@ def test_library_loads_and_exports():
     assert len(names) >= 20
     for n in names:
         assert hasattr(lib, n), f"{n} missing from libgnx.so"
-    assert lib.gnx_abi_version() == 8
+    assert lib.gnx_abi_version() == 9
 
 
 def test_library_does_not_link_oracle():
