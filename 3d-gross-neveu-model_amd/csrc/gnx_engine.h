@@ -130,10 +130,14 @@ class GraphEngine {
     int nrhs = 0;
     EngGraph g;
   };
-  // slots needed by the last two runs of a kind, least recently used replaced
+  // Slots needed by the last run of a kind, and by its last two runs that
+  // iterated (a solve that stops in its prologue, e.g. a null start already
+  // below the tolerance, needs 2: its count says nothing about the next
+  // one); least recently used replaced.
   struct Pred {
     unsigned long long key = 0;
-    int last = -1, prev = -1;
+    int last = -1;
+    int it_last = -1, it_prev = -1;
     unsigned long long used = 0;
   };
   void* dargs_ = nullptr;

@@ -139,21 +139,28 @@ hipError_t GraphEngine::graph(int tag, int nrhs, const ChunkBody& body, EngGraph
 
 hipError_t GraphEngine::run(const EngGraph& g, int nrhs, long max_slots, hipStream_t st, int* max_iters,
                             unsigned long long key, SlotsOf slots_of) {
-  // first burst: the slots the last run of this kind needed, plus the
-  // change between its last two runs (a kind whose count moves gets that
-  // much headroom, at most half a chunk), as whole chunks and one tail of
-  // 1..3 ENG_TAIL (a remainder past that takes a whole chunk)
+  // first burst: the slots the last run of this kind needed, plus (when
+  // that run iterated) the change between its last two iterating runs (a
+  // kind whose count moves gets that much headroom, at most half a chunk),
+  // as whole chunks and one tail of 1..3 ENG_TAIL (a remainder past that
+  // takes a whole chunk)
   Pred* p = pred_find(key);
   const int need = p ? p->last : last_slots_;
-  int margin = (p && p->prev >= 0) ? (p->last > p->prev ? p->last - p->prev : p->prev - p->last) : 0;
+  int margin = 0;
+  if (p && p->last == p->it_last && p->it_prev >= 0)
+    margin = p->it_last > p->it_prev ? p->it_last - p->it_prev : p->it_prev - p->it_last;
   if (margin > ENG_CHUNK / 2) margin = ENG_CHUNK / 2;
   const long want = need > 0 ? (long)need + margin : ENG_CHUNK;
-  int full = (int)(want / ENG_CHUNK);
-  int tails = (int)((want % ENG_CHUNK + ENG_TAIL - 1) / ENG_TAIL);  // 0 .. ENG_TAILS + 1
-  if (tails > ENG_TAILS) {
-    full++;
-    tails = 0;
-  }
+  int full = 0, tails = 0;
+  auto split = [&](long w) {  // w slots as whole chunks and at most one tail
+    full = (int)(w / ENG_CHUNK);
+    tails = (int)((w % ENG_CHUNK + ENG_TAIL - 1) / ENG_TAIL);  // 0 .. ENG_TAILS + 1
+    if (tails > ENG_TAILS) {
+      full++;
+      tails = 0;
+    }
+  };
+  split(want);
   long done = 0;
   int misses = 0;
   hipError_t e;
@@ -193,20 +200,28 @@ hipError_t GraphEngine::run(const EngGraph& g, int nrhs, long max_slots, hipStre
         p->key = key;
         p->used = ++pred_clock_;
       }
-      p->prev = p->last;
       p->last = slots;
+      if (mx > 0) {
+        p->it_prev = p->it_last;
+        p->it_last = slots;
+      }
       last_slots_ = slots;
       if (max_iters) *max_iters = mx;
       return hipSuccess;
     }
     if (done > max_slots + 2 * ENG_CHUNK) return hipErrorUnknown;  // the state machine failed to stop
-    // short of the need: a tail of two first (a near miss), then whole chunks
-    if (misses++ == 0) {
-      full = 0;
-      tails = 2;
-    } else {
+    // Short of the need.  First what the kind's last iterating run used
+    // beyond what is launched (after a run that stopped in its prologue),
+    // else a tail of two (a near miss); then whole chunks.
+    const long rest = (misses++ == 0 && p && p->it_last > 0) ? p->it_last - done : 0;
+    if (misses > 1) {
       full = 1;
       tails = 0;
+    } else if (rest >= 2 * ENG_TAIL) {
+      split(rest);
+    } else {
+      full = 0;
+      tails = 2;
     }
   }
 }

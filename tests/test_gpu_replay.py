@@ -142,3 +142,41 @@ def test_mixed_repeat_one_poll(P, gnx_mod, prec, monkeypatch):
     assert np.array_equal(xs[0], xs[1])
     assert true_res(lat, smh, b, xs[1]) < 1e-10
     ctx.close()
+
+
+def test_prologue_exit_does_not_spoil_prediction(P, gnx_mod, monkeypatch):
+    """The HMC's force solves alternate within one kind (same tolerance,
+    null start) between solves that stop in their prologue (the incoming x
+    already meets the tolerance) and solves that iterate.  After a prologue
+    exit the next iterating solve launches one short burst, misses, and
+    then launches what the kind's last iterating solve used: two polls,
+    within one tail of its need."""
+    banner("replay prediction after a prologue exit")
+    import torch
+    monkeypatch.delenv("GNX_CG_SHAPE", raising=False)
+    L = 16
+    lat, smh, b = _system(P, L, 65)
+    ctx = gnx_mod.Context(L)
+    sm = dev(smh)
+    tol = 1e-8
+
+    def from_zero():
+        x = ctx.vec()
+        return ctx.cg(sm, dev(b), x, tol, start=gnx_mod.START_NULL), x
+
+    (st, res), x_conv = from_zero()
+    from_zero()
+    for rnd in range(2):
+        xc = x_conv.clone()
+        (st0, r0), slots0, polls0 = _solve_counted(
+            ctx, lambda: ctx.cg(sm, dev(b), xc, tol, start=gnx_mod.START_NULL))
+        torch.cuda.synchronize()
+        assert st0 == 0 and r0.early_exit == 1 and r0.iterations == 0 and polls0 == 1
+        ((st1, r1), _), slots1, polls1 = _solve_counted(ctx, from_zero)
+        torch.cuda.synchronize()
+        banner(f"  round {rnd}: prologue exit {slots0} slots; then {r1.iterations} iterations, "
+               f"{slots1} slots, {polls1} polls")
+        assert st1 == 0 and r1.converged and r1.iterations == res.iterations
+        assert polls1 == 2, polls1
+        assert r1.iterations + 1 <= slots1 <= r1.iterations + 2 + 7, (slots1, r1.iterations)
+    ctx.close()
