@@ -130,14 +130,14 @@ class GraphEngine {
     int nrhs = 0;
     EngGraph g;
   };
-  // Slots needed by the last run of a kind, and by its last two runs that
-  // iterated (a solve that stops in its prologue, e.g. a null start already
-  // below the tolerance, needs 2: its count says nothing about the next
-  // one); least recently used replaced.
+  // what the runs of a kind needed (gnx_replay_plan.h), least recently
+  // used replaced
   struct Pred {
     unsigned long long key = 0;
-    int last = -1;
-    int it_last = -1, it_prev = -1;
+    bool valid = false;
+    int last = -1, it_last = -1, it_prev = -1;
+    unsigned exits = 0;
+    int runs = 0;
     unsigned long long used = 0;
   };
   void* dargs_ = nullptr;

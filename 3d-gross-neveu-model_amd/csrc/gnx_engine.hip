@@ -2,6 +2,7 @@
 #include <string.h>
 
 #include "gnx_engine.h"
+#include "gnx_replay_plan.h"
 
 namespace gnx {
 
@@ -33,7 +34,7 @@ unsigned long long eng_key(int tag, int nrhs, double tol, int start) {
 
 GraphEngine::Pred* GraphEngine::pred_find(unsigned long long key) {
   for (auto& p : pred_)
-    if (p.last >= 0 && p.key == key) {
+    if (p.valid && p.key == key) {
       p.used = ++pred_clock_;
       return &p;
     }
@@ -139,43 +140,30 @@ hipError_t GraphEngine::graph(int tag, int nrhs, const ChunkBody& body, EngGraph
 
 hipError_t GraphEngine::run(const EngGraph& g, int nrhs, long max_slots, hipStream_t st, int* max_iters,
                             unsigned long long key, SlotsOf slots_of) {
-  // first burst: the slots the last run of this kind needed, plus (when
-  // that run iterated) the change between its last two iterating runs (a
-  // kind whose count moves gets that much headroom, at most half a chunk),
-  // as whole chunks and one tail of 1..3 ENG_TAIL (a remainder past that
-  // takes a whole chunk)
   Pred* p = pred_find(key);
-  const int need = p ? p->last : last_slots_;
-  int margin = 0;
-  if (p && p->last == p->it_last && p->it_prev >= 0)
-    margin = p->it_last > p->it_prev ? p->it_last - p->it_prev : p->it_prev - p->it_last;
-  if (margin > ENG_CHUNK / 2) margin = ENG_CHUNK / 2;
-  const long want = need > 0 ? (long)need + margin : ENG_CHUNK;
-  int full = 0, tails = 0;
-  auto split = [&](long w) {  // w slots as whole chunks and at most one tail
-    full = (int)(w / ENG_CHUNK);
-    tails = (int)((w % ENG_CHUNK + ENG_TAIL - 1) / ENG_TAIL);  // 0 .. ENG_TAILS + 1
-    if (tails > ENG_TAILS) {
-      full++;
-      tails = 0;
-    }
-  };
-  split(want);
+  ReplayKind kind;
+  if (p) {
+    kind.last = p->last;
+    kind.it_last = p->it_last;
+    kind.it_prev = p->it_prev;
+    kind.exits = p->exits;
+    kind.runs = p->runs;
+  }
+  ReplayBurst b = replay_first(p ? &kind : nullptr, last_slots_);
   long done = 0;
   int misses = 0;
   hipError_t e;
   for (;;) {
-    for (int i = 0; i < full; i++) {
+    for (int i = 0; i < b.full; i++) {
       e = hipGraphLaunch(g.chunk, st);
       if (e != hipSuccess) return e;
     }
-    if (tails > 0) {
-      e = hipGraphLaunch(g.tail[tails - 1], st);
+    if (b.tails > 0) {
+      e = hipGraphLaunch(g.tail[b.tails - 1], st);
       if (e != hipSuccess) return e;
     }
-    const long n = (long)full * ENG_CHUNK + (long)tails * ENG_TAIL;
-    done += n;
-    launched_slots += n;
+    done += b.slots();
+    launched_slots += b.slots();
     polls++;
     e = hipMemcpyAsync(hstate_, state_, state_bytes_ * nrhs, hipMemcpyDeviceToHost, st);
     if (e != hipSuccess) return e;
@@ -198,31 +186,21 @@ hipError_t GraphEngine::run(const EngGraph& g, int nrhs, long max_slots, hipStre
           if (q.used < p->used) p = &q;
         *p = Pred();
         p->key = key;
+        p->valid = true;
         p->used = ++pred_clock_;
       }
-      p->last = slots;
-      if (mx > 0) {
-        p->it_prev = p->it_last;
-        p->it_last = slots;
-      }
+      replay_record(&kind, slots, mx);
+      p->last = kind.last;
+      p->it_last = kind.it_last;
+      p->it_prev = kind.it_prev;
+      p->exits = kind.exits;
+      p->runs = kind.runs;
       last_slots_ = slots;
       if (max_iters) *max_iters = mx;
       return hipSuccess;
     }
     if (done > max_slots + 2 * ENG_CHUNK) return hipErrorUnknown;  // the state machine failed to stop
-    // Short of the need.  First what the kind's last iterating run used
-    // beyond what is launched (after a run that stopped in its prologue),
-    // else a tail of two (a near miss); then whole chunks.
-    const long rest = (misses++ == 0 && p && p->it_last > 0) ? p->it_last - done : 0;
-    if (misses > 1) {
-      full = 1;
-      tails = 0;
-    } else if (rest >= 2 * ENG_TAIL) {
-      split(rest);
-    } else {
-      full = 0;
-      tails = 2;
-    }
+    b = replay_after_miss(p ? &kind : nullptr, done, misses++);
   }
 }
 
