@@ -249,3 +249,40 @@ def test_mixed_f16_scaled_storage(P, gnx_mod, mag):
     assert s1 == 0 and r1.converged and true_res(lat, smh, b1, host(x1)) < 1e-10
     # the iteration count does not depend on the magnitude
     assert abs(res[0].iterations - r1.iterations) <= 3, (res[0].iterations, r1.iterations)
+
+
+def test_mixed_four_site_groups_streaming_stores(P, gnx_mod, monkeypatch):
+    """40 FP16 systems at 64^3: the working set (r, t, two p, xs: 24 B per
+    site and system, 252 MB) passes the Infinity Cache, so the kernels store
+    non-temporally.  The four-site groups (shape 16) against the site-strided
+    loads-first kernels (shape 15) on the same batch: every system converges,
+    the two solutions agree to the solver tolerance, and two systems meet
+    the FP64 bar against the oracle's operator."""
+    banner("mixed f16 x40 at 64^3: four-site groups with streaming stores")
+    import torch
+    L, n = 64, 40
+    lat = P.Lattice(L)
+    r = P.Random(640)
+    smh = lat.sigma_mass(r.sigma(lat.V, P.sigma_variance(0.85)))
+    bs = [r.z2_complex(lat.V) for _ in range(n)]
+    B = dev(np.concatenate(bs))
+    sm = dev(smh)
+    xs, its = {}, {}
+    for shape in (15, 16):
+        monkeypatch.setenv("GNX_MIX_SHAPE", str(shape))
+        ctx = gnx_mod.Context(L)
+        X = ctx.vec(n)
+        st, res = ctx.cg_batch(sm, B, X, n, 1e-10, start=gnx_mod.START_IMPOSED, prec=gnx_mod.PREC_F16)
+        torch.cuda.synchronize()
+        assert st == 0 and all(q.converged for q in res)
+        assert ctx.cg_mixed_shape(n, gnx_mod.PREC_F16)["consecutive"] == (shape == 16)
+        xs[shape] = X.clone()
+        its[shape] = [q.iterations for q in res]
+        ctx.close()
+    banner(f"  iterations shape 15 {min(its[15])}..{max(its[15])}, shape 16 {min(its[16])}..{max(its[16])}")
+    assert max(abs(a - b) for a, b in zip(its[15], its[16])) <= 10
+    d = float(torch.linalg.norm(xs[16] - xs[15]) / torch.linalg.norm(xs[15]))
+    assert d < 1e-6, d
+    for k in (0, n - 1):
+        xk = host(xs[16][k * 2 * lat.V:(k + 1) * 2 * lat.V])
+        assert true_res(lat, smh, bs[k], xk) < 1e-10
