@@ -1325,15 +1325,13 @@ hipError_t cgm_f64_run(void** machine_slot, const Geom& g, const double* sm, con
   if (e != hipSuccess) return e;
   e = cgm_begin(m, g, sm, U, b, x, tol, max_iter, start, nrhs, shape, st);
   if (e != hipSuccess) return e;
-  e = m->eng.run(exec, nrhs, max_iter, st, nullptr, eng_key(shape, nrhs, tol, start), nullptr);
-  if (e != hipSuccess) return e;
-  e = hipMemcpyAsync(x, m->x, sizeof(double2) * (size_t)g.V * nrhs, hipMemcpyDeviceToDevice, st);
-  if (e != hipSuccess) return e;
-  if (host_res) {
-    e = hipMemcpyAsync(host_res, m->res, sizeof(CGDevResult) * nrhs, hipMemcpyDeviceToHost, st);
-    if (e != hipSuccess) return e;
-  }
-  return hipStreamSynchronize(st);
+  const FinishFn out = [&](hipStream_t s) {
+    hipError_t f = hipMemcpyAsync(x, m->x, sizeof(double2) * (size_t)g.V * nrhs, hipMemcpyDeviceToDevice, s);
+    if (f == hipSuccess && host_res)
+      f = hipMemcpyAsync(host_res, m->res, sizeof(CGDevResult) * nrhs, hipMemcpyDeviceToHost, s);
+    return f;
+  };
+  return m->eng.run(exec, nrhs, max_iter, st, nullptr, eng_key(shape, nrhs, tol, start), nullptr, &out);
 }
 
 // ---- solve without allocation ----------------------------------------------------

@@ -1664,16 +1664,14 @@ static hipError_t mix_run_t(MixMachine* m, const Geom& g, const double* sm, cons
   e = mix_begin<S>(m, g, sm, b, x, tol, delta, max_iter, start, nrhs, shape, st);
   if (e != hipSuccess) return e;
   // at most max_iter normal slot pairs plus one update pair after each
-  e = m->eng.run(exec, nrhs, 2 * max_iter + 4, st, nullptr, eng_key(tag + 10 * shape, nrhs, tol, start),
-                 mix_slots_of);
-  if (e != hipSuccess) return e;
-  e = hipMemcpyAsync(x, m->x, sizeof(double2) * (size_t)g.V * nrhs, hipMemcpyDeviceToDevice, st);
-  if (e != hipSuccess) return e;
-  if (host_res) {
-    e = hipMemcpyAsync(host_res, m->res, sizeof(CGDevResult) * nrhs, hipMemcpyDeviceToHost, st);
-    if (e != hipSuccess) return e;
-  }
-  return hipStreamSynchronize(st);
+  const FinishFn out = [&](hipStream_t s) {
+    hipError_t f = hipMemcpyAsync(x, m->x, sizeof(double2) * (size_t)g.V * nrhs, hipMemcpyDeviceToDevice, s);
+    if (f == hipSuccess && host_res)
+      f = hipMemcpyAsync(host_res, m->res, sizeof(CGDevResult) * nrhs, hipMemcpyDeviceToHost, s);
+    return f;
+  };
+  return m->eng.run(exec, nrhs, 2 * max_iter + 4, st, nullptr, eng_key(tag + 10 * shape, nrhs, tol, start),
+                    mix_slots_of, &out);
 }
 
 hipError_t cgmix_run(void** machine_slot, const Geom& g, const double* sm, const double2* b, double2* x, double tol,
@@ -1882,13 +1880,15 @@ static hipError_t mixu_run_t(MixMulti* mm, const MultiView& mv, const double* co
   e = mm->eng.graph(tag, n, body, &exec);
   if (e != hipSuccess) return e;
   // at most max_iter normal slot pairs plus one update pair after each
-  e = mm->eng.run(exec, n, 2 * cap_max + 4, st, nullptr, eng_key(tag, n, tol, start), mix_slots_of);
-  if (e != hipSuccess) return e;
-  void* as[] = {(void*)&a, (void*)&dp};
-  e = hipLaunchKernel((const void*)k_mixu_scatter, grid, blk, as, 0, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(host_res, mm->res, sizeof(CGDevResult) * n, hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
-  return e;
+  // the solutions scattered to the callers' vectors and the results copied
+  // after each burst (the gathered x_in were read by the start kernels)
+  const FinishFn out = [&](hipStream_t s) {
+    void* as[] = {(void*)&a, (void*)&dp};
+    hipError_t f = hipLaunchKernel((const void*)k_mixu_scatter, grid, blk, as, 0, s);
+    if (f == hipSuccess) f = hipMemcpyAsync(host_res, mm->res, sizeof(CGDevResult) * n, hipMemcpyDeviceToHost, s);
+    return f;
+  };
+  return mm->eng.run(exec, n, 2 * cap_max + 4, st, nullptr, eng_key(tag, n, tol, start), mix_slots_of, &out);
 }
 
 hipError_t cgmix_multi_run(const MultiView& mv, const double* const* sm, const double* const* b, double* const* x,

@@ -591,14 +591,19 @@ static gnx_status multi_cg(gnx_multi* m, const double* const* sm, const double* 
   }
   EngGraph exec;
   HIP_TRY(cgu_graph(m, link, &exec));
-  HIP_TRY(m->eng.run(exec, n, cap_max, st, nullptr, eng_key(link ? 1 : 0, n, tol, start), nullptr));
   {
+    // the solutions scattered to the callers' vectors and the results
+    // copied after each burst (the gathered x_in were read by the start
+    // kernels)
     const CGUPtrs* dp = m->dptr;
-    void* as[] = {(void*)&a, (void*)&dp};
-    HIP_TRY(hipLaunchKernel((const void*)k_cgu_scatter, grid, blk, as, 0, st));
+    const FinishFn out = [&](hipStream_t s) {
+      void* as[] = {(void*)&a, (void*)&dp};
+      hipError_t f = hipLaunchKernel((const void*)k_cgu_scatter, grid, blk, as, 0, s);
+      if (f == hipSuccess) f = hipMemcpyAsync(m->hres, m->res, sizeof(CGDevResult) * n, hipMemcpyDeviceToHost, s);
+      return f;
+    };
+    HIP_TRY(m->eng.run(exec, n, cap_max, st, nullptr, eng_key(link ? 1 : 0, n, tol, start), nullptr, &out));
   }
-  HIP_TRY(hipMemcpyAsync(m->hres, m->res, sizeof(CGDevResult) * n, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
   int all = 1;
   for (int k = 0; k < n; k++) {
     if (res) {

@@ -48,6 +48,14 @@ using SlotsOf = int (*)(const char* state);
 // the graph, the batch size, the tolerance and the start
 unsigned long long eng_key(int tag, int nrhs, double tol, int start);
 
+// work a solve enqueues once its systems have stopped (copies of the
+// solutions and results out of the solver's buffers): the engine enqueues it
+// after every burst, ahead of the poll's wait, so that a run that ends at a
+// poll needs no second wait; a copy made after a burst that fell short is
+// made again after the next one (the solvers read only their own buffers
+// while they run)
+using FinishFn = std::function<hipError_t(hipStream_t)>;
+
 // Launch shapes chosen per key (batch size with the operator or storage
 // format folded in).  When every slot is taken the least recently used
 // entry is replaced, so a context that sees many keys re-chooses only the
@@ -113,9 +121,10 @@ class GraphEngine {
   // drop every captured graph (their kernel arguments point at old buffers)
   void invalidate_graphs();
   // replay until every system is inactive; returns the largest `iters`.
-  // key: eng_key() of the run; slots_of: null for the default
+  // key: eng_key() of the run; slots_of: null for the default; finish:
+  // null, or the solve's closing work, complete on a successful return
   hipError_t run(const EngGraph& g, int nrhs, long max_slots, hipStream_t st, int* max_iters,
-                 unsigned long long key, SlotsOf slots_of);
+                 unsigned long long key, SlotsOf slots_of, const FinishFn* finish = nullptr);
   // time `chunks` chunk replays (after the caller staged a start), in ms
   hipError_t time_replays(const EngGraph& g, int chunks, hipStream_t st, float* ms);
   // slots launched and polls made by run() since construction

@@ -139,7 +139,7 @@ hipError_t GraphEngine::graph(int tag, int nrhs, const ChunkBody& body, EngGraph
 }
 
 hipError_t GraphEngine::run(const EngGraph& g, int nrhs, long max_slots, hipStream_t st, int* max_iters,
-                            unsigned long long key, SlotsOf slots_of) {
+                            unsigned long long key, SlotsOf slots_of, const FinishFn* finish) {
   Pred* p = pred_find(key);
   ReplayKind kind;
   if (p) {
@@ -167,6 +167,10 @@ hipError_t GraphEngine::run(const EngGraph& g, int nrhs, long max_slots, hipStre
     polls++;
     e = hipMemcpyAsync(hstate_, state_, state_bytes_ * nrhs, hipMemcpyDeviceToHost, st);
     if (e != hipSuccess) return e;
+    if (finish) {
+      e = (*finish)(st);
+      if (e != hipSuccess) return e;
+    }
     e = hipStreamSynchronize(st);
     if (e != hipSuccess) return e;
     bool any = false;
